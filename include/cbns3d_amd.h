@@ -1,0 +1,165 @@
+/*
+ * cbns3d_amd — C ABI of the MI355X-native right-hand-side and time-step path
+ * of the cbns3d_cart_mpi compressible Navier-Stokes solver.
+ *
+ * Plain C types only (no HIP or torch types).  Every call returns 0 on
+ * success and a negative cbns_status on failure; nothing aborts inside the
+ * library.  One context is bound to one device and one HIP stream; calls on
+ * a context are stream-ordered and must come from one host thread at a time.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   SimulationParams ............ include/simulation_params.h:15-53
+ *   read_input derivations ...... src/read_input.cpp:78-90
+ *   Block3d ctor / gen_mesh ..... src/Block3d.cpp:11-33, 208-235
+ *   initial_condition +
+ *     calc_conservative ......... src/Block3d.cpp:237-300
+ *   read_bin / output_bin ....... src/Block3d.cpp:318-357, 964-987
+ *   boundary_condition .......... src/boundary_condition.cpp:4-231
+ *   calc_primitive .............. src/Block3d.cpp:359-385
+ *   init_Q_p .................... src/Block3d.cpp:387-394
+ *   is_finished ................. src/Block3d.cpp:502-513
+ *   calc_dt + MPI_Allreduce MIN . src/Block3d.cpp:515-553, src/cbns3d.cpp:186-194
+ *   rec_riemann_x/y/z ........... src/Block3d.cpp:555-748
+ *   calc_viscous_flux_contribution src/Block3d.cpp:750-918
+ *   update_rk3 .................. src/Block3d.cpp:920-962
+ *   time loop body .............. src/cbns3d.cpp:176-240
+ */
+#ifndef CBNS3D_AMD_H
+#define CBNS3D_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CBNS_NEQ 5 /* conserved variables (include/constants.h:13) */
+#define CBNS_NG 4  /* ghost layers (include/constants.h:15-17) */
+#define CBNS_NGV 2 /* viscous halo (include/constants.h:16) */
+
+typedef enum cbns_status {
+  CBNS_OK = 0,
+  CBNS_ERR_ARG = -1,      /* invalid argument (null pointer, bad index, bad size) */
+  CBNS_ERR_HIP = -2,      /* a HIP runtime call failed */
+  CBNS_ERR_NOMEM = -3,    /* device allocation failed */
+  CBNS_ERR_NODEV = -4,    /* no usable gfx950 device */
+  CBNS_ERR_STATE = -5,    /* call out of order (e.g. stepping before a state is set) */
+  CBNS_ERR_IO = -6,       /* file could not be opened, read or written */
+  CBNS_ERR_FORMAT = -7,   /* checkpoint dimensions do not match (src/Block3d.cpp:334-338) */
+  CBNS_ERR_NONFINITE = -8 /* a non-finite value met the time-step reduction */
+} cbns_status;
+
+/* SimulationParams (include/simulation_params.h:15-53) as plain data, with the
+ * checkpoint name left to the host facade.  The first group is what input.txt
+ * holds (src/read_input.cpp:24-72); the second is derived by cbns_params_derive. */
+typedef struct cbns_params {
+  double CFL, t_cur, t_end, Pr, gamma, Mach, Re, T_inf, L;
+  uint32_t IM, JM, KM, nstep_max, checkpoint_freq, num_output;
+  int32_t is_restart;
+  double Mach2, gM2, Re_inv, p_inf, gam1, gam1_inv, Pr_inv, C_T_inf, C_dt_v;
+} cbns_params;
+
+/* Arithmetic of the inviscid face flux. */
+typedef enum cbns_precision {
+  CBNS_FP64 = 0,      /* the plain path: every operation in FP64, as the reference */
+  CBNS_FP32_FLUX = 1  /* WENO, characteristic projections and LF flux in FP32 on
+                         deviations from a reference state; state, Roe averages,
+                         viscous terms, RK3 and reductions in FP64 */
+} cbns_precision;
+
+typedef struct cbns_ctx cbns_ctx;
+
+/* Which state array a call addresses: the reference's Q or Q_p. */
+#define CBNS_Q 0
+#define CBNS_QP 1
+
+/* src/read_input.cpp:78-90: Mach2, Re_inv, Pr_inv, gam1, gM2, gam1_inv, p_inf,
+ * C_T_inf, C_dt_v from the input values. */
+int cbns_params_derive(cbns_params* p);
+
+/* Number of visible devices that this library can run on (gfx950). */
+int cbns_device_count(int* n);
+
+/* Create a context on the current device: the periodic IMxJMxKM domain split
+ * into dims[0] x dims[1] x dims[2] on-device blocks exactly as the reference
+ * splits it over MPI ranks (src/cbns3d.cpp:102-126: N/dims cells per block, the
+ * last block takes the remainder).  Blocks are numbered like the reference's
+ * Cartesian ranks (row-major, z fastest).  Every block needs >= 4 cells per
+ * axis (the reference's halo depth).  No state is set yet. */
+int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx** out);
+int cbns_destroy(cbns_ctx* ctx);
+
+int cbns_num_blocks(const cbns_ctx* ctx, int* nb);
+/* Interior size and global origin of block b. */
+int cbns_block_info(const cbns_ctx* ctx, int b, int n[3], int origin[3]);
+/* Device memory held by the context, in bytes. */
+int cbns_memory_bytes(const cbns_ctx* ctx, uint64_t* bytes);
+
+/* Taylor-Green start (src/cbns3d.cpp:136-153 with is_restart = 1):
+ * initial_condition + calc_conservative, boundary_condition, calc_primitive,
+ * init_Q_p.  mu := 1.0 on the interior (src/Block3d.cpp:265).  The sin/cos
+ * factors are evaluated on the host with the C library, as the reference does. */
+int cbns_init_tgv(cbns_ctx* ctx);
+
+/* Set Q from a host array holding the whole periodic interior, AoS
+ * [k][j][i][eq] (IMxJMxKMx5 doubles), then boundary_condition, calc_primitive
+ * and init_Q_p; t_cur := t_cur.  mu on the interior := mu0 (0.0 reproduces the
+ * reference's restart path, where mu is never written before the first dt). */
+int cbns_set_state(cbns_ctx* ctx, const double* Q_aos, double t_cur, double mu0);
+/* Copy the interior of Q (which = CBNS_Q) or Q_p (CBNS_QP) to a host AoS array. */
+int cbns_get_state(cbns_ctx* ctx, int which, double* Q_aos);
+
+/* Per-block ghosted AoS arrays in the checkpoint layout (src/Block3d.cpp:964-987:
+ * [k][j][i][eq] over (KM+8)(JM+8)(IM+8)), for the Block3d facade. */
+int cbns_get_block_state(cbns_ctx* ctx, int b, int which, double* Q_aos_ghosted);
+int cbns_set_block_state(cbns_ctx* ctx, int b, int which, const double* Q_aos_ghosted);
+
+/* Primitive field f (0 rho, 1 u, 2 v, 3 w, 4 p, 5 T, 6 mu) of block b, ghosted
+ * SoA [k][j][i] ((KM+8)(JM+8)(IM+8) doubles); for mu only the interior is kept
+ * on the device and ghosts read as 0. */
+int cbns_get_block_primitive(cbns_ctx* ctx, int b, int f, double* out_ghosted);
+
+/* ---- method-level entry points (the reference's stage order) ----------- */
+int cbns_boundary_condition(cbns_ctx* ctx, int which);  /* src/boundary_condition.cpp:4-231 */
+int cbns_calc_primitive(cbns_ctx* ctx, int which);      /* src/Block3d.cpp:359-385 */
+int cbns_init_Q_p(cbns_ctx* ctx);                       /* src/Block3d.cpp:387-394 */
+/* Global CFL/s_max over every block = the reference's local calc_dt followed by
+ * MPI_Allreduce(MIN) (src/Block3d.cpp:515-553, src/cbns3d.cpp:186-189). */
+int cbns_calc_dt(cbns_ctx* ctx, double* dt);
+int cbns_rec_riemann(cbns_ctx* ctx, int which, int dir); /* src/Block3d.cpp:555-748 */
+int cbns_calc_viscous_flux_contribution(cbns_ctx* ctx); /* src/Block3d.cpp:870-918 */
+int cbns_update_rk3(cbns_ctx* ctx, double dt, int stage); /* src/Block3d.cpp:920-962 */
+
+/* ---- fused time stepping ----------------------------------------------- */
+/* One step of the driver loop (src/cbns3d.cpp:178-240): returns 1 in *finished
+ * (and does nothing) when t_cur + 1e-5 > t_end, else computes dt, clips it to
+ * t_end - t_cur, advances t_cur and runs the three stages.  *dt_out may be NULL. */
+int cbns_step(cbns_ctx* ctx, double* dt_out, int* finished);
+/* nsteps steps; dts (may be NULL) receives each step's dt.  *steps_done = the
+ * steps taken before is_finished stopped the loop. */
+int cbns_run(cbns_ctx* ctx, int nsteps, double* dts, int* steps_done);
+
+int cbns_get_time(const cbns_ctx* ctx, double* t_cur);
+int cbns_set_time(cbns_ctx* ctx, double t_cur);
+
+/* Interior means of 0.5 rho |u|^2 and 0.5 rho |omega|^2 (omega by the
+ * reference's 4th-order central stencil, src/calc_gradient.cpp:29-64) from the
+ * current primitives of Q.  No counterpart in the reference. */
+int cbns_diagnostics(cbns_ctx* ctx, double* ke, double* enstrophy);
+
+/* Synchronise the context's stream. */
+int cbns_sync(cbns_ctx* ctx);
+
+/* Evaluate the device face flux of one face along axis dir on host-given data:
+ * cells[6][5] conserved values of cells i-3..i+2 (AoS), prim[2][5] = rho, u, v,
+ * w, p of cells i-1 and i.  Same arithmetic as cbns_rec_riemann; for parity
+ * tests of the per-face path (src/Block3d.cpp:586-612).  nfaces faces at once. */
+int cbns_face_flux_eval(int precision, int dir, double gamma, int nfaces,
+                        const double* cells, const double* prim, double* flux);
+
+const char* cbns_strerror(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
