@@ -1,0 +1,150 @@
+"""Benchmark of the plain FP64 RHS + TVD-RK3 time-step path on one MI355X.
+
+Workload: Taylor-Green vortex, 256^3 periodic cells, FP64, one on-device block
+(the reference's TGV setup: gamma 1.4, Mach 0.1, Re 1600, Pr 0.71,
+T_inf 255.556, CFL 0.5).  A step is one full TVD-RK3 step (dt reduction, three
+RHS evaluations, three ghost fills, three primitive passes) over the whole
+grid, with the state resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--dims 1,1,1]
+
+Prints one JSON line.  value = Mcell-steps/s = N^3 * K / wall / 1e6 over the
+K timed steps.  The dominant kernel (inviscid face flux, one launch per axis
+per stage) is timed live with HIP events on the solver's stream during the
+timed steps; its roofline is FP64 arithmetic (algorithmic 1570 flop per face:
+calc_eigenvectors 161 + 2 x rec_weno5 642 + lf_flux 125, counted literally
+from the reference source).  The CPU baseline is the repository's CPU
+restatement of the reference (oracle/, OpenMP) on the host's cores, timed on a
+bounded sample of steps of the same 256^3 workload.
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_FACE = 1570          # literal count of the reference's per-face arithmetic
+FLOP_PER_CELL_STEP = 15145    # literal count per cell-step, all kernels
+PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector (and matrix) peak
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--dims", default="1,1,1")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-max-steps", type=int, default=3)
+    ap.add_argument("--cpu-budget-s", type=float, default=25.0)
+    return ap.parse_args()
+
+
+def read_traffic():
+    """HBM bytes per face-flux launch from the committed rocprofv3 PMC summary."""
+    fn = os.path.join(ROOT, "profiles", "face_flux_tgv256_pmc.md")
+    if not os.path.exists(fn):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(fn).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(n, max_steps, budget_s):
+    """The oracle (CPU restatement) on this host's cores, a bounded sample."""
+    from oracle import oracle as O
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    O.set_threads(cores)
+    b = O.Block(O.make_params(n))
+    b.init_tgv()
+    b.step()  # warm-up (page-in of ~10 GB of arrays)
+    t0 = time.perf_counter()
+    k = 0
+    while k < max_steps:
+        b.step()
+        k += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    del b
+    return {"value": n ** 3 * k / el / 1e6, "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{k} TGV {n}^3 FP64 steps after 1 warm-up step, oracle/ C restatement with OpenMP"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the path shards by "
+              "on-device blocks but multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+    from cbns3d_cart_mpi_amd import capi as C
+
+    dims = tuple(int(x) for x in a.dims.split(","))
+    N = a.n
+    p = C.make_params(N, nstep_max=a.steps + a.warmup, t_end=20.0)
+    s = C.Solver(p, dims)
+    s.init_tgv()
+    s.run(a.warmup)
+    s.sync()
+    s.set_profiling(True)
+    t0 = time.perf_counter()
+    dts = s.run(a.steps)
+    el = time.perf_counter() - t0
+    kt = s.kernel_times()
+    s.set_profiling(False)
+    assert len(dts) == a.steps
+    ke, en = s.diagnostics()
+    mem = s.memory_bytes
+    s.close()
+
+    cells = N ** 3
+    value = cells * a.steps / el / 1e6
+    ms_step = el / a.steps * 1e3
+    face_ms = sum(kt[k][0] for k in ("face_x", "face_y", "face_z"))
+    face_n = sum(kt[k][1] for k in ("face_x", "face_y", "face_z"))
+    avg_face_ms = face_ms / max(face_n, 1)
+    faces_per_launch = (N + 1) * N * N  # single block; per-axis launch covers every block
+    achieved = FLOP_PER_FACE * faces_per_launch / (avg_face_ms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": read_traffic(),
+            "kernel": "k_face_flux (x, y, z), FP64 VALU; the FP64 peak is the same for VALU and matrix cores",
+            "avg_launch_ms": round(avg_face_ms, 4)}
+    line = {
+        "metric": "Mcell-steps/s TGV 256^3 FP64 (plain path)",
+        "value": round(value, 2),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Taylor-Green initial condition, no dataset)",
+        "config": {"workload": f"Taylor-Green vortex {N}^3 periodic, FP64 plain path, blocks {a.dims}",
+                   "grid": [N, N, N], "blocks": list(dims), "Re": 1600.0, "Mach": 0.1, "CFL": 0.5},
+        "roofline": roof,
+        "algorithmic_tflops_all_kernels": round(FLOP_PER_CELL_STEP * value * 1e6 / 1e12, 3),
+        "kernels_avg_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items()},
+        "kernel_share": {k: round(v[0] / (el * 1e3), 4) for k, v in kt.items()},
+        "device_bytes": mem,
+        "ke_enstrophy_end": [ke, en],
+    }
+    if not a.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(N, a.cpu_max_steps, a.cpu_budget_s)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,278 @@
+"""ctypes binding of libcbns3d_amd.so (the C ABI declared in include/cbns3d_amd.h).
+
+The native library is required: importing this module does not load it, but
+every call does, and a missing library raises instead of falling back to
+anything else.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcbns3d_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "cbns3d_amd.h")
+
+CBNS_FP64 = 0
+CBNS_FP32_FLUX = 1
+CBNS_Q = 0
+CBNS_QP = 1
+
+_D = ctypes.c_double
+_PD = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.c_int
+_PI = ctypes.POINTER(ctypes.c_int)
+_VP = ctypes.c_void_p
+
+
+class CbnsError(RuntimeError):
+    def __init__(self, status, what):
+        self.status = status
+        super().__init__(f"{what}: {strerror(status)} ({status})")
+
+
+class Params(ctypes.Structure):
+    """cbns_params: SimulationParams (include/simulation_params.h:15-53) as POD."""
+
+    _fields_ = [(n, _D) for n in ("CFL", "t_cur", "t_end", "Pr", "gamma", "Mach", "Re", "T_inf", "L")] + [
+        (n, ctypes.c_uint32) for n in ("IM", "JM", "KM", "nstep_max", "checkpoint_freq", "num_output")
+    ] + [("is_restart", ctypes.c_int32)] + [
+        (n, _D) for n in ("Mach2", "gM2", "Re_inv", "p_inf", "gam1", "gam1_inv", "Pr_inv", "C_T_inf", "C_dt_v")
+    ]
+
+
+_SIGS = {
+    "cbns_params_derive": (_I, [ctypes.POINTER(Params)]),
+    "cbns_device_count": (_I, [_PI]),
+    "cbns_create": (_I, [ctypes.POINTER(Params), _PI, _I, ctypes.POINTER(_VP)]),
+    "cbns_destroy": (_I, [_VP]),
+    "cbns_num_blocks": (_I, [_VP, _PI]),
+    "cbns_block_info": (_I, [_VP, _I, _PI, _PI]),
+    "cbns_memory_bytes": (_I, [_VP, ctypes.POINTER(ctypes.c_uint64)]),
+    "cbns_init_tgv": (_I, [_VP]),
+    "cbns_set_state": (_I, [_VP, _PD, _D, _D]),
+    "cbns_get_state": (_I, [_VP, _I, _PD]),
+    "cbns_get_block_state": (_I, [_VP, _I, _I, _PD]),
+    "cbns_set_block_state": (_I, [_VP, _I, _I, _PD]),
+    "cbns_get_block_primitive": (_I, [_VP, _I, _I, _PD]),
+    "cbns_boundary_condition": (_I, [_VP, _I]),
+    "cbns_calc_primitive": (_I, [_VP, _I]),
+    "cbns_init_Q_p": (_I, [_VP]),
+    "cbns_calc_dt": (_I, [_VP, _PD]),
+    "cbns_rec_riemann": (_I, [_VP, _I, _I]),
+    "cbns_calc_viscous_flux_contribution": (_I, [_VP]),
+    "cbns_update_rk3": (_I, [_VP, _D, _I]),
+    "cbns_step": (_I, [_VP, _PD, _PI]),
+    "cbns_run": (_I, [_VP, _I, _PD, _PI]),
+    "cbns_get_time": (_I, [_VP, _PD]),
+    "cbns_set_time": (_I, [_VP, _D]),
+    "cbns_diagnostics": (_I, [_VP, _PD, _PD]),
+    "cbns_sync": (_I, [_VP]),
+    "cbns_face_flux_eval": (_I, [_I, _I, _D, _I, _PD, _PD, _PD]),
+    "cbns_set_profiling": (_I, [_VP, _I]),
+    "cbns_kernel_times": (_I, [_VP, _PD, ctypes.POINTER(ctypes.c_int64)]),
+    "cbns_strerror": (ctypes.c_char_p, [_I]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"native library missing: {LIB_PATH}; build it with "
+                               "`python -m cbns3d_cart_mpi_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def header_functions():
+    """Names of every function the public header declares."""
+    src = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(cbns_\w+)\s*\(", src, re.M)))
+
+
+def strerror(status):
+    return lib().cbns_strerror(status).decode()
+
+
+def check(status, what):
+    if status != 0:
+        raise CbnsError(status, what)
+
+
+def _p(a):
+    return a.ctypes.data_as(_PD)
+
+
+def make_params(N, Mach=0.1, Re=1600.0, Pr=0.71, gamma=1.4, T_inf=255.556, CFL=0.5, t_cur=0.0, t_end=20.0,
+                nstep_max=100, L=1.0):
+    """Values of python/taylor_green.ini with the grid and step count set."""
+    if isinstance(N, int):
+        N = (N, N, N)
+    p = Params(CFL=CFL, t_cur=t_cur, t_end=t_end, Pr=Pr, gamma=gamma, Mach=Mach, Re=Re, T_inf=T_inf, L=L,
+               IM=N[0], JM=N[1], KM=N[2], nstep_max=nstep_max, checkpoint_freq=nstep_max, num_output=10,
+               is_restart=1)
+    check(lib().cbns_params_derive(ctypes.byref(p)), "cbns_params_derive")
+    return p
+
+
+def device_count():
+    n = _I(0)
+    check(lib().cbns_device_count(ctypes.byref(n)), "cbns_device_count")
+    return n.value
+
+
+class Solver:
+    """One context: the periodic domain split into on-device blocks."""
+
+    def __init__(self, params, dims=(1, 1, 1), precision=CBNS_FP64):
+        self.params = params
+        self.N = (params.IM, params.JM, params.KM)
+        d = (_I * 3)(*dims)
+        h = _VP()
+        check(lib().cbns_create(ctypes.byref(params), d, precision, ctypes.byref(h)), "cbns_create")
+        self._h = h
+        self.dims = tuple(dims)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cbns_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # info
+    @property
+    def num_blocks(self):
+        n = _I()
+        check(lib().cbns_num_blocks(self._h, ctypes.byref(n)), "cbns_num_blocks")
+        return n.value
+
+    def block_info(self, b):
+        n, o = (_I * 3)(), (_I * 3)()
+        check(lib().cbns_block_info(self._h, b, n, o), "cbns_block_info")
+        return tuple(n), tuple(o)
+
+    @property
+    def memory_bytes(self):
+        v = ctypes.c_uint64()
+        check(lib().cbns_memory_bytes(self._h, ctypes.byref(v)), "cbns_memory_bytes")
+        return v.value
+
+    # setup
+    def init_tgv(self):
+        check(lib().cbns_init_tgv(self._h), "cbns_init_tgv")
+
+    def set_state(self, Q, t_cur, mu0=0.0):
+        a = np.ascontiguousarray(Q, dtype=np.float64)
+        assert a.size == 5 * self.N[0] * self.N[1] * self.N[2]
+        check(lib().cbns_set_state(self._h, _p(a), t_cur, mu0), "cbns_set_state")
+
+    def get_state(self, which=CBNS_Q):
+        out = np.empty((self.N[2], self.N[1], self.N[0], 5))
+        check(lib().cbns_get_state(self._h, which, _p(out)), "cbns_get_state")
+        return out
+
+    def get_block_state(self, b, which=CBNS_Q):
+        n, _ = self.block_info(b)
+        out = np.empty((n[2] + 8, n[1] + 8, n[0] + 8, 5))
+        check(lib().cbns_get_block_state(self._h, b, which, _p(out)), "cbns_get_block_state")
+        return out
+
+    def set_block_state(self, b, Qg, which=CBNS_Q):
+        a = np.ascontiguousarray(Qg, dtype=np.float64)
+        check(lib().cbns_set_block_state(self._h, b, which, _p(a)), "cbns_set_block_state")
+
+    def get_block_primitive(self, b, f):
+        n, _ = self.block_info(b)
+        out = np.empty((n[2] + 8, n[1] + 8, n[0] + 8))
+        check(lib().cbns_get_block_primitive(self._h, b, f, _p(out)), "cbns_get_block_primitive")
+        return out
+
+    # method level
+    def boundary_condition(self, which=CBNS_Q):
+        check(lib().cbns_boundary_condition(self._h, which), "cbns_boundary_condition")
+
+    def calc_primitive(self, which=CBNS_Q):
+        check(lib().cbns_calc_primitive(self._h, which), "cbns_calc_primitive")
+
+    def init_Q_p(self):
+        check(lib().cbns_init_Q_p(self._h), "cbns_init_Q_p")
+
+    def calc_dt(self):
+        dt = _D()
+        check(lib().cbns_calc_dt(self._h, ctypes.byref(dt)), "cbns_calc_dt")
+        return dt.value
+
+    def rec_riemann(self, which, d):
+        check(lib().cbns_rec_riemann(self._h, which, d), "cbns_rec_riemann")
+
+    def calc_viscous_flux_contribution(self):
+        check(lib().cbns_calc_viscous_flux_contribution(self._h), "cbns_calc_viscous_flux_contribution")
+
+    def update_rk3(self, dt, stage):
+        check(lib().cbns_update_rk3(self._h, dt, stage), "cbns_update_rk3")
+
+    # fused
+    def step(self):
+        dt, fin = _D(), _I()
+        check(lib().cbns_step(self._h, ctypes.byref(dt), ctypes.byref(fin)), "cbns_step")
+        return None if fin.value else dt.value
+
+    def run(self, nsteps):
+        dts = np.zeros(max(nsteps, 1))
+        done = _I()
+        check(lib().cbns_run(self._h, nsteps, _p(dts), ctypes.byref(done)), "cbns_run")
+        return dts[:done.value]
+
+    @property
+    def t_cur(self):
+        t = _D()
+        check(lib().cbns_get_time(self._h, ctypes.byref(t)), "cbns_get_time")
+        return t.value
+
+    def diagnostics(self):
+        ke, en = _D(), _D()
+        check(lib().cbns_diagnostics(self._h, ctypes.byref(ke), ctypes.byref(en)), "cbns_diagnostics")
+        return ke.value, en.value
+
+    def sync(self):
+        check(lib().cbns_sync(self._h), "cbns_sync")
+
+    KERNEL_CLASSES = ("face_x", "face_y", "face_z", "viscous", "update", "ghost_fill", "primitive", "dt")
+
+    def set_profiling(self, on):
+        check(lib().cbns_set_profiling(self._h, int(bool(on))), "cbns_set_profiling")
+
+    def kernel_times(self):
+        """{class: (total_ms, launches)} of launches timed since profiling was turned on."""
+        ms = (_D * 8)()
+        n = (ctypes.c_int64 * 8)()
+        check(lib().cbns_kernel_times(self._h, ms, n), "cbns_kernel_times")
+        return {k: (ms[i], n[i]) for i, k in enumerate(self.KERNEL_CLASSES)}
+
+
+def face_flux_eval(direction, gamma, cells, prim, precision=CBNS_FP64):
+    """Device face flux on explicit inputs: cells (n, 6, 5), prim (n, 2, 5)."""
+    cells = np.ascontiguousarray(cells, dtype=np.float64)
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    n = cells.shape[0]
+    out = np.empty((n, 5))
+    check(lib().cbns_face_flux_eval(precision, direction, gamma, n, _p(cells), _p(prim), _p(out)),
+          "cbns_face_flux_eval")
+    return out

@@ -1,0 +1,582 @@
+// C ABI of the MI355X-native cbns3d right-hand-side and time-step path.
+// Owns all device memory of a context; host arrays are caller-owned and
+// cross the boundary only through the explicit set/get calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/cbns3d_amd.h"
+#include "cbns_kernels.h"
+
+using namespace cbns;
+
+struct cbns_ctx {
+  cbns_params par;
+  int precision;
+  int device;
+  Domain dom;
+  Phys ph;
+  std::vector<BlockDesc> hb;
+  BlockDesc* db = nullptr;
+  hipStream_t stream = nullptr;
+  unsigned long long* d_smax = nullptr;
+  int* d_flag = nullptr;
+  double* d_diag = nullptr;  // partials then the two sums
+  int diag_parts = 0;
+  std::vector<void*> allocs;
+  uint64_t bytes = 0;
+  bool have_state = false;
+  int nonfinite = 0;
+  // live kernel timing
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  double k_ms[CBNS_KCLASSES] = {0};
+  int64_t k_n[CBNS_KCLASSES] = {0};
+};
+
+static hipEvent_t take_event(cbns_ctx* c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Add the durations of completed bracketed launches (called after a stream sync).
+static void harvest(cbns_ctx* c) {
+  for (auto& p : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+      c->k_ms[p.first] += ms;
+      c->k_n[p.first] += 1;
+    }
+    c->ev_pool.push_back(p.second.first);
+    c->ev_pool.push_back(p.second.second);
+  }
+  c->pending.clear();
+}
+
+// Launch `body` bracketed by events of class k when timing is on.
+template <class F>
+static hipError_t timed(cbns_ctx* c, int k, F body) {
+  if (!c->prof) return body();
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) return hipErrorOutOfMemory;
+  hipError_t e = hipEventRecord(a, c->stream);
+  if (e == hipSuccess) e = body();
+  if (e == hipSuccess) e = hipEventRecord(b, c->stream);
+  c->pending.push_back({k, {a, b}});
+  return e;
+}
+
+#define CK(call)                                   \
+  do {                                             \
+    hipError_t e_ = (call);                        \
+    if (e_ != hipSuccess) return CBNS_ERR_HIP;     \
+  } while (0)
+
+static int dev_alloc(cbns_ctx* c, void** p, size_t bytes) {
+  if (hipMalloc(p, bytes) != hipSuccess) return CBNS_ERR_NOMEM;
+  c->allocs.push_back(*p);
+  c->bytes += bytes;
+  return CBNS_OK;
+}
+
+extern "C" {
+
+const char* cbns_strerror(int s) {
+  switch (s) {
+    case CBNS_OK: return "ok";
+    case CBNS_ERR_ARG: return "invalid argument";
+    case CBNS_ERR_HIP: return "HIP runtime error";
+    case CBNS_ERR_NOMEM: return "device allocation failed";
+    case CBNS_ERR_NODEV: return "no gfx950 device";
+    case CBNS_ERR_STATE: return "call out of order";
+    case CBNS_ERR_IO: return "file I/O failed";
+    case CBNS_ERR_FORMAT: return "checkpoint dimensions do not match";
+    case CBNS_ERR_NONFINITE: return "non-finite value in the time-step reduction";
+    default: return "unknown status";
+  }
+}
+
+// src/read_input.cpp:78-90
+int cbns_params_derive(cbns_params* p) {
+  if (!p) return CBNS_ERR_ARG;
+  p->Mach2 = p->Mach * p->Mach;
+  p->Re_inv = 1.0 / p->Re;
+  p->Pr_inv = 1.0 / p->Pr;
+  p->gam1 = p->gamma - 1.0;
+  p->gM2 = p->gamma * p->Mach2;
+  p->gam1_inv = 1.0 / p->gam1;
+  p->p_inf = 1.0 / p->gM2;
+  p->C_T_inf = 110.5 / p->T_inf;
+  const double a = 4.0 / 3.0, b = p->gamma * p->Pr_inv;
+  p->C_dt_v = ((a < b) ? b : a) * p->Re_inv;
+  return CBNS_OK;
+}
+
+int cbns_device_count(int* n) {
+  if (!n) return CBNS_ERR_ARG;
+  *n = 0;
+  int cnt = 0;
+  if (hipGetDeviceCount(&cnt) != hipSuccess) return CBNS_OK;
+  for (int d = 0; d < cnt; d++) {
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, d) == hipSuccess && std::strncmp(pr.gcnArchName, "gfx950", 6) == 0) (*n)++;
+  }
+  return CBNS_OK;
+}
+
+int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx** out) {
+  if (!p || !dims || !out) return CBNS_ERR_ARG;
+  *out = nullptr;
+  if (precision != CBNS_FP64 && precision != CBNS_FP32_FLUX) return CBNS_ERR_ARG;
+  const int N[3] = {(int)p->IM, (int)p->JM, (int)p->KM};
+  for (int a = 0; a < 3; a++) {
+    if (dims[a] < 1 || N[a] < 1) return CBNS_ERR_ARG;
+    const int base = N[a] / dims[a];
+    const int last = N[a] - (dims[a] - 1) * base;
+    if (base < NG || last < NG) return CBNS_ERR_ARG;  // halo must fit in one neighbour
+  }
+  const int nb = dims[0] * dims[1] * dims[2];
+  if (nb > MAXB) return CBNS_ERR_ARG;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return CBNS_ERR_NODEV;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return CBNS_ERR_NODEV;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return CBNS_ERR_NODEV;
+
+  cbns_ctx* c = new (std::nothrow) cbns_ctx();
+  if (!c) return CBNS_ERR_NOMEM;
+  c->par = *p;
+  c->precision = precision;
+  c->device = dev;
+  for (int a = 0; a < 3; a++) {
+    c->dom.N[a] = N[a];
+    c->dom.dims[a] = dims[a];
+    c->dom.base[a] = N[a] / dims[a];
+  }
+  c->dom.nb = nb;
+  // static caches of the reference (src/Block3d.cpp:208-219, 563-564, 764-767)
+  const double two_pi = 2.0 * 3.14159265358979323846;
+  c->ph.gamma = p->gamma;
+  c->ph.gam1 = p->gam1;
+  c->ph.gM2 = p->gM2;
+  c->ph.CFL = p->CFL;
+  c->ph.C_T_inf = p->C_T_inf;
+  c->ph.Re_inv = p->Re_inv;
+  c->ph.kappa = -p->Pr_inv * p->gam1_inv / p->Mach2;
+  c->ph.C_dt_v = p->C_dt_v;
+  c->ph.dx = two_pi / p->IM;
+  c->ph.dy = two_pi / p->JM;
+  c->ph.dz = two_pi / p->KM;
+
+  int st = CBNS_OK;
+  c->hb.resize(nb);
+  for (int bx = 0; bx < dims[0] && st == CBNS_OK; bx++)
+    for (int by = 0; by < dims[1] && st == CBNS_OK; by++)
+      for (int bz = 0; bz < dims[2] && st == CBNS_OK; bz++) {
+        const int b = (bx * dims[1] + by) * dims[2] + bz;
+        BlockDesc& B = c->hb[b];
+        std::memset(&B, 0, sizeof(B));
+        const int bc[3] = {bx, by, bz};
+        for (int a = 0; a < 3; a++) {
+          const int base = N[a] / dims[a];
+          B.n[a] = (bc[a] == dims[a] - 1) ? N[a] - (dims[a] - 1) * base : base;
+          B.g[a] = B.n[a] + 2 * NG;
+          B.o[a] = bc[a] * base;
+        }
+        B.sg = (long long)B.g[0] * B.g[1] * B.g[2];
+        B.sv = (long long)(B.n[0] + 2 * NGV) * (B.n[1] + 2 * NGV) * (B.n[2] + 2 * NGV);
+        B.sn = (long long)B.n[0] * B.n[1] * B.n[2];
+        for (int d = 0; d < 3; d++)
+          B.sf[d] = (long long)(B.n[0] + (d == 0)) * (B.n[1] + (d == 1)) * (B.n[2] + (d == 2));
+        void* q;
+        if ((st = dev_alloc(c, &q, sizeof(double) * NEQ * B.sg))) break;
+        B.Q[0] = (double*)q;
+        if ((st = dev_alloc(c, &q, sizeof(double) * NEQ * B.sg))) break;
+        B.Q[1] = (double*)q;
+        for (int d = 0; d < 3 && !st; d++) {
+          if ((st = dev_alloc(c, &q, sizeof(double) * NEQ * B.sf[d]))) break;
+          B.F[d] = (double*)q;
+        }
+        if (st) break;
+        if ((st = dev_alloc(c, &q, sizeof(double) * 4 * B.sn))) break;
+        B.dfv = (double*)q;
+        if ((st = dev_alloc(c, &q, sizeof(double) * 6 * B.sg))) break;
+        B.prim = (double*)q;
+        if ((st = dev_alloc(c, &q, sizeof(double) * B.sn))) break;
+        B.mu = (double*)q;
+        if ((st = dev_alloc(c, &q, sizeof(double) * 9 * B.sv))) break;
+        B.visc = (double*)q;
+      }
+  void* q = nullptr;
+  if (!st) st = dev_alloc(c, &q, sizeof(BlockDesc) * nb);
+  c->db = (BlockDesc*)q;
+  if (!st) st = dev_alloc(c, &q, 64);
+  c->d_smax = (unsigned long long*)q;
+  if (!st) st = dev_alloc(c, &q, 64);
+  c->d_flag = (int*)q;
+  c->diag_parts = 256 * nb;
+  if (!st) st = dev_alloc(c, &q, sizeof(double) * (2 * c->diag_parts + 2));
+  c->d_diag = (double*)q;
+  if (!st && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
+  if (!st && hipMemcpy(c->db, c->hb.data(), sizeof(BlockDesc) * nb, hipMemcpyHostToDevice) != hipSuccess)
+    st = CBNS_ERR_HIP;
+  if (st) {
+    cbns_destroy(c);
+    return st;
+  }
+  *out = c;
+  return CBNS_OK;
+}
+
+int cbns_destroy(cbns_ctx* c) {
+  if (!c) return CBNS_ERR_ARG;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  harvest(c);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CBNS_OK;
+}
+
+int cbns_num_blocks(const cbns_ctx* c, int* nb) {
+  if (!c || !nb) return CBNS_ERR_ARG;
+  *nb = c->dom.nb;
+  return CBNS_OK;
+}
+
+int cbns_block_info(const cbns_ctx* c, int b, int n[3], int origin[3]) {
+  if (!c || b < 0 || b >= c->dom.nb) return CBNS_ERR_ARG;
+  for (int a = 0; a < 3; a++) {
+    if (n) n[a] = c->hb[b].n[a];
+    if (origin) origin[a] = c->hb[b].o[a];
+  }
+  return CBNS_OK;
+}
+
+int cbns_memory_bytes(const cbns_ctx* c, uint64_t* bytes) {
+  if (!c || !bytes) return CBNS_ERR_ARG;
+  *bytes = c->bytes;
+  return CBNS_OK;
+}
+
+int cbns_sync(cbns_ctx* c) {
+  if (!c) return CBNS_ERR_ARG;
+  CK(hipStreamSynchronize(c->stream));
+  return CBNS_OK;
+}
+
+// ---- method-level -------------------------------------------------------
+
+int cbns_boundary_condition(cbns_ctx* c, int which) {
+  if (!c || (which != CBNS_Q && which != CBNS_QP)) return CBNS_ERR_ARG;
+  CK(launch_ghost_fill(c->db, c->hb.data(), c->dom, which, c->stream));
+  return CBNS_OK;
+}
+
+int cbns_calc_primitive(cbns_ctx* c, int which) {
+  if (!c || (which != CBNS_Q && which != CBNS_QP)) return CBNS_ERR_ARG;
+  CK(launch_primitive(c->db, c->hb.data(), c->dom.nb, which, c->ph, c->stream));
+  return CBNS_OK;
+}
+
+int cbns_init_Q_p(cbns_ctx* c) {
+  if (!c) return CBNS_ERR_ARG;
+  CK(launch_copy_state(c->db, c->hb.data(), c->dom.nb, CBNS_Q, CBNS_QP, c->stream));
+  return CBNS_OK;
+}
+
+int cbns_calc_dt(cbns_ctx* c, double* dt) {
+  if (!c || !dt) return CBNS_ERR_ARG;
+  if (!c->have_state) return CBNS_ERR_STATE;
+  CK(hipMemsetAsync(c->d_smax, 0, sizeof(unsigned long long), c->stream));
+  CK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+  CK(timed(c, 7, [&] { return launch_dt_reduce(c->db, c->hb.data(), c->dom.nb, c->ph, c->d_smax, c->d_flag, c->stream); }));
+  unsigned long long bits = 0;
+  int flag = 0;
+  CK(hipMemcpyAsync(&bits, c->d_smax, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+  CK(hipMemcpyAsync(&flag, c->d_flag, sizeof(flag), hipMemcpyDeviceToHost, c->stream));
+  CK(hipStreamSynchronize(c->stream));
+  harvest(c);
+  double s_max;
+  std::memcpy(&s_max, &bits, sizeof(s_max));
+  c->nonfinite = flag;
+  *dt = c->par.CFL / s_max;  // src/Block3d.cpp:551
+  return CBNS_OK;
+}
+
+int cbns_rec_riemann(cbns_ctx* c, int which, int dir) {
+  if (!c || (which != CBNS_Q && which != CBNS_QP) || dir < 0 || dir > 2) return CBNS_ERR_ARG;
+  CK(launch_face_flux(c->db, c->hb.data(), c->dom.nb, dir, which, c->ph, c->stream));
+  return CBNS_OK;
+}
+
+int cbns_calc_viscous_flux_contribution(cbns_ctx* c) {
+  if (!c) return CBNS_ERR_ARG;
+  CK(launch_viscous(c->db, c->hb.data(), c->dom.nb, c->ph, c->stream));
+  return CBNS_OK;
+}
+
+int cbns_update_rk3(cbns_ctx* c, double dt, int stage) {
+  if (!c) return CBNS_ERR_ARG;
+  if (stage < 1 || stage > 3) return CBNS_OK;  // the reference silently does nothing
+  CK(launch_update(c->db, c->hb.data(), c->dom.nb, dt, stage, c->ph, c->stream));
+  return CBNS_OK;
+}
+
+// ---- setup ----------------------------------------------------------------
+
+static int finish_setup(cbns_ctx* c) {
+  // src/cbns3d.cpp:149-153
+  int st;
+  if ((st = cbns_boundary_condition(c, CBNS_Q))) return st;
+  if ((st = cbns_calc_primitive(c, CBNS_Q))) return st;
+  if ((st = cbns_init_Q_p(c))) return st;
+  CK(hipStreamSynchronize(c->stream));
+  c->have_state = true;
+  return CBNS_OK;
+}
+
+int cbns_init_tgv(cbns_ctx* c) {
+  if (!c) return CBNS_ERR_ARG;
+  const int* N = c->dom.N;
+  const int ts = std::max(N[0], std::max(N[1], N[2]));
+  std::vector<double> tab(9 * (size_t)ts, 0.0);
+  const double tmp = 2.0 * 3.14159265358979323846;
+  const double h[3] = {c->ph.dx, c->ph.dy, c->ph.dz};
+  for (int a = 0; a < 3; a++)
+    for (int i = 0; i < N[a]; i++) {
+      // gen_mesh edge (src/Block3d.cpp:223-233) and cell centre (src/Block3d.cpp:243-251)
+      const double edge = (unsigned)i * h[a] - tmp / 2.0;
+      const double xc = edge + 0.5 * h[a];
+      tab[(3 * a + 0) * ts + i] = std::sin(xc);
+      tab[(3 * a + 1) * ts + i] = std::cos(xc);
+      tab[(3 * a + 2) * ts + i] = std::cos(2.0 * xc);
+    }
+  double* dtab = nullptr;
+  CK(hipMalloc(&dtab, sizeof(double) * tab.size()));
+  int st = CBNS_OK;
+  if (hipMemcpy(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      launch_tgv(c->db, c->hb.data(), c->dom.nb, dtab, ts, c->ph, c->par.p_inf, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    st = CBNS_ERR_HIP;
+  (void)hipFree(dtab);
+  if (st) return st;
+  return finish_setup(c);
+}
+
+int cbns_set_state(cbns_ctx* c, const double* Q_aos, double t_cur, double mu0) {
+  if (!c || !Q_aos) return CBNS_ERR_ARG;
+  const size_t n = (size_t)NEQ * c->dom.N[0] * c->dom.N[1] * c->dom.N[2];
+  double* d = nullptr;
+  CK(hipMalloc(&d, n * sizeof(double)));
+  int st = CBNS_OK;
+  if (hipMemcpy(d, Q_aos, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      launch_scatter_aos(c->db, c->hb.data(), c->dom, CBNS_Q, d, c->stream) != hipSuccess ||
+      launch_fill_mu(c->db, c->hb.data(), c->dom.nb, mu0, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    st = CBNS_ERR_HIP;
+  (void)hipFree(d);
+  if (st) return st;
+  c->par.t_cur = t_cur;
+  return finish_setup(c);
+}
+
+int cbns_get_state(cbns_ctx* c, int which, double* Q_aos) {
+  if (!c || !Q_aos || (which != CBNS_Q && which != CBNS_QP)) return CBNS_ERR_ARG;
+  const size_t n = (size_t)NEQ * c->dom.N[0] * c->dom.N[1] * c->dom.N[2];
+  double* d = nullptr;
+  CK(hipMalloc(&d, n * sizeof(double)));
+  int st = CBNS_OK;
+  if (launch_gather_aos(c->db, c->hb.data(), c->dom, which, d, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess ||
+      hipMemcpy(Q_aos, d, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    st = CBNS_ERR_HIP;
+  (void)hipFree(d);
+  return st;
+}
+
+int cbns_get_block_state(cbns_ctx* c, int b, int which, double* out) {
+  if (!c || !out || b < 0 || b >= c->dom.nb || (which != CBNS_Q && which != CBNS_QP)) return CBNS_ERR_ARG;
+  const BlockDesc& B = c->hb[b];
+  std::vector<double> soa((size_t)NEQ * B.sg);
+  CK(hipStreamSynchronize(c->stream));
+  CK(hipMemcpy(soa.data(), B.Q[which], soa.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (long long t = 0; t < B.sg; t++)
+    for (int e = 0; e < NEQ; e++) out[NEQ * t + e] = soa[e * B.sg + t];
+  return CBNS_OK;
+}
+
+int cbns_set_block_state(cbns_ctx* c, int b, int which, const double* in) {
+  if (!c || !in || b < 0 || b >= c->dom.nb || (which != CBNS_Q && which != CBNS_QP)) return CBNS_ERR_ARG;
+  const BlockDesc& B = c->hb[b];
+  std::vector<double> soa((size_t)NEQ * B.sg);
+  for (long long t = 0; t < B.sg; t++)
+    for (int e = 0; e < NEQ; e++) soa[e * B.sg + t] = in[NEQ * t + e];
+  CK(hipStreamSynchronize(c->stream));
+  CK(hipMemcpy(B.Q[which], soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->have_state = true;
+  return CBNS_OK;
+}
+
+int cbns_get_block_primitive(cbns_ctx* c, int b, int f, double* out) {
+  if (!c || !out || b < 0 || b >= c->dom.nb || f < 0 || f > 6) return CBNS_ERR_ARG;
+  const BlockDesc& B = c->hb[b];
+  CK(hipStreamSynchronize(c->stream));
+  if (f < 6) {
+    CK(hipMemcpy(out, B.prim + f * B.sg, B.sg * sizeof(double), hipMemcpyDeviceToHost));
+    return CBNS_OK;
+  }
+  std::vector<double> mu(B.sn);
+  CK(hipMemcpy(mu.data(), B.mu, B.sn * sizeof(double), hipMemcpyDeviceToHost));
+  std::fill(out, out + B.sg, 0.0);
+  for (int k = 0; k < B.n[2]; k++)
+    for (int j = 0; j < B.n[1]; j++)
+      for (int i = 0; i < B.n[0]; i++)
+        out[(i + NG) + (long long)B.g[0] * ((j + NG) + (long long)B.g[1] * (k + NG))] =
+            mu[i + (long long)B.n[0] * (j + (long long)B.n[1] * k)];
+  return CBNS_OK;
+}
+
+// ---- fused stepping -------------------------------------------------------
+
+int cbns_get_time(const cbns_ctx* c, double* t) {
+  if (!c || !t) return CBNS_ERR_ARG;
+  *t = c->par.t_cur;
+  return CBNS_OK;
+}
+
+int cbns_set_time(cbns_ctx* c, double t) {
+  if (!c) return CBNS_ERR_ARG;
+  c->par.t_cur = t;
+  return CBNS_OK;
+}
+
+static int run_stages(cbns_ctx* c, double dt) {
+  // src/cbns3d.cpp:203-240
+  for (int stage = 1; stage <= 3; stage++) {
+    const int in = stage == 1 ? CBNS_Q : CBNS_QP;
+    const int out = stage == 3 ? CBNS_Q : CBNS_QP;
+    const BlockDesc* hb = c->hb.data();
+    for (int d = 0; d < 3; d++)
+      CK(timed(c, d, [&] { return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->stream); }));
+    CK(timed(c, 3, [&] { return launch_viscous(c->db, hb, c->dom.nb, c->ph, c->stream); }));
+    CK(timed(c, 4, [&] { return launch_update(c->db, hb, c->dom.nb, dt, stage, c->ph, c->stream); }));
+    CK(timed(c, 5, [&] { return launch_ghost_fill(c->db, hb, c->dom, out, c->stream); }));
+    CK(timed(c, 6, [&] { return launch_primitive(c->db, hb, c->dom.nb, out, c->ph, c->stream); }));
+  }
+  return CBNS_OK;
+}
+
+int cbns_step(cbns_ctx* c, double* dt_out, int* finished) {
+  if (!c) return CBNS_ERR_ARG;
+  if (!c->have_state) return CBNS_ERR_STATE;
+  if (finished) *finished = 0;
+  // is_finished (src/Block3d.cpp:502-513)
+  if (c->par.t_cur + 1.0e-5 > c->par.t_end) {
+    if (finished) *finished = 1;
+    return CBNS_OK;
+  }
+  double dt;
+  int st = cbns_calc_dt(c, &dt);
+  if (st) return st;
+  if (c->nonfinite) return CBNS_ERR_NONFINITE;
+  // src/cbns3d.cpp:191-194
+  if (c->par.t_end - c->par.t_cur < dt) dt = c->par.t_end - c->par.t_cur;
+  c->par.t_cur += dt;
+  if (dt_out) *dt_out = dt;
+  return run_stages(c, dt);
+}
+
+int cbns_run(cbns_ctx* c, int nsteps, double* dts, int* steps_done) {
+  if (!c || nsteps < 0) return CBNS_ERR_ARG;
+  int n = 0;
+  for (; n < nsteps; n++) {
+    int fin = 0;
+    double dt = 0.0;
+    int st = cbns_step(c, &dt, &fin);
+    if (st) {
+      if (steps_done) *steps_done = n;
+      return st;
+    }
+    if (fin) break;
+    if (dts) dts[n] = dt;
+  }
+  if (steps_done) *steps_done = n;
+  CK(hipStreamSynchronize(c->stream));
+  harvest(c);
+  return CBNS_OK;
+}
+
+int cbns_set_profiling(cbns_ctx* c, int on) {
+  if (!c) return CBNS_ERR_ARG;
+  CK(hipStreamSynchronize(c->stream));
+  harvest(c);
+  c->prof = on != 0;
+  for (int k = 0; k < CBNS_KCLASSES; k++) {
+    c->k_ms[k] = 0.0;
+    c->k_n[k] = 0;
+  }
+  return CBNS_OK;
+}
+
+int cbns_kernel_times(cbns_ctx* c, double ms[CBNS_KCLASSES], int64_t n[CBNS_KCLASSES]) {
+  if (!c || !ms || !n) return CBNS_ERR_ARG;
+  CK(hipStreamSynchronize(c->stream));
+  harvest(c);
+  for (int k = 0; k < CBNS_KCLASSES; k++) {
+    ms[k] = c->k_ms[k];
+    n[k] = c->k_n[k];
+  }
+  return CBNS_OK;
+}
+
+int cbns_diagnostics(cbns_ctx* c, double* ke, double* ens) {
+  if (!c || !ke || !ens) return CBNS_ERR_ARG;
+  CK(launch_diagnostics(c->db, c->hb.data(), c->dom.nb, c->ph, c->d_diag, c->diag_parts,
+                        c->d_diag + 2 * c->diag_parts, c->stream));
+  double out[2];
+  CK(hipMemcpyAsync(out, c->d_diag + 2 * c->diag_parts, sizeof(out), hipMemcpyDeviceToHost, c->stream));
+  CK(hipStreamSynchronize(c->stream));
+  const double n = (double)c->dom.N[0] * c->dom.N[1] * c->dom.N[2];
+  *ke = out[0] / n;
+  *ens = out[1] / n;
+  return CBNS_OK;
+}
+
+int cbns_face_flux_eval(int precision, int dir, double gamma, int nf, const double* cells, const double* prim,
+                        double* flux) {
+  if (!cells || !prim || !flux || nf < 0 || dir < 0 || dir > 2) return CBNS_ERR_ARG;
+  if (precision != CBNS_FP64) return CBNS_ERR_ARG;
+  if (nf == 0) return CBNS_OK;
+  double *dc = nullptr, *dp = nullptr, *df = nullptr;
+  int st = CBNS_OK;
+  if (hipMalloc(&dc, sizeof(double) * 30 * (size_t)nf) != hipSuccess ||
+      hipMalloc(&dp, sizeof(double) * 10 * (size_t)nf) != hipSuccess ||
+      hipMalloc(&df, sizeof(double) * 5 * (size_t)nf) != hipSuccess)
+    st = CBNS_ERR_NOMEM;
+  if (!st && (hipMemcpy(dc, cells, sizeof(double) * 30 * (size_t)nf, hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(dp, prim, sizeof(double) * 10 * (size_t)nf, hipMemcpyHostToDevice) != hipSuccess ||
+              launch_face_eval(dir, gamma, nf, dc, dp, df, nullptr) != hipSuccess ||
+              hipDeviceSynchronize() != hipSuccess ||
+              hipMemcpy(flux, df, sizeof(double) * 5 * (size_t)nf, hipMemcpyDeviceToHost) != hipSuccess))
+    st = CBNS_ERR_HIP;
+  if (dc) (void)hipFree(dc);
+  if (dp) (void)hipFree(dp);
+  if (df) (void)hipFree(df);
+  return st;
+}
+
+}  // extern "C"

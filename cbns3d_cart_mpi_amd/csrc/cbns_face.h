@@ -1,0 +1,218 @@
+// Per-face inviscid arithmetic of the plain FP64 path, for gfx950.
+//
+// One face = Roe frame of the two adjacent cells (src/eigenvectors_roe.cpp:32-171),
+// characteristic WENO5 reconstruction of the left and right states from the
+// six cells around the face (src/weno.cpp:25-152), and the local
+// Lax-Friedrichs flux (src/lf_flux.cpp:32-99), as rec_riemann_x/y/z drive them
+// (src/Block3d.cpp:586-612).
+//
+// Exactness.  The library is compiled with -ffp-contract=off, so every
+// expression below rounds exactly as the reference's (x86-64 SSE2, no FMA)
+// does, provided the operation order is kept.  It is, with these exact
+// simplifications for axis-aligned normals:
+//   * n, l, m are compile-time unit vectors; the reference's normalisation
+//     divides by sqrt(1) = 1, which is exact;
+//   * a product with a frame component of 1 is the operand itself; a product
+//     with -1 is its negation (both exact);
+//   * a term whose frame component is 0 adds a signed zero, which leaves every
+//     non-zero sum unchanged; only the sign of an exactly-zero result can
+//     differ, and no such zero is ever divided by, so all values agree;
+//   * the six cells of a face are projected once each with the face's L and
+//     shared by the left and right stencils — the reference projects the same
+//     cells with the same L in the same order, once per stencil;
+//   * the characteristic fields are streamed m = 0..4, accumulating R's column
+//     m into both reconstructed states, which adds R[li][m] * w_m in the same
+//     order m = 0..4 as the reference's back-projection loop;
+//   * the LF wave speed |ws . n| reduces to the normal component, the only
+//     non-zero term.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace cbns {
+
+// src/weno.cpp:25-56
+__device__ __forceinline__ double weno5(double Um2, double Um1, double U0, double Up1, double Up2) {
+  const double eps = 1.0e-6;
+  double diff1 = Um2 - 2.0 * Um1 + U0;
+  double diff2 = Um2 - 4.0 * Um1 + 3.0 * U0;
+  double diff3 = Um1 - 2.0 * U0 + Up1;
+  double diff4 = Um1 - Up1;
+  double diff5 = U0 - 2.0 * Up1 + Up2;
+  double diff6 = 3.0 * U0 - 4.0 * Up1 + Up2;
+  double IS1 = (13.0 / 12.0) * diff1 * diff1 + 0.25 * diff2 * diff2 + eps;
+  double IS2 = (13.0 / 12.0) * diff3 * diff3 + 0.25 * diff4 * diff4 + eps;
+  double IS3 = (13.0 / 12.0) * diff5 * diff5 + 0.25 * diff6 * diff6 + eps;
+  double w1 = 0.1 / (IS1 * IS1);
+  double w2 = 0.6 / (IS2 * IS2);
+  double w3 = 0.3 / (IS3 * IS3);
+  double sum = w1 + w2 + w3;
+  double f1 = (2.0 * Um2 - 7.0 * Um1 + 11.0 * U0) / 6.0;
+  double f2 = (-Um1 + 5.0 * U0 + 2.0 * Up1) / 6.0;
+  double f3 = (2.0 * U0 + 5.0 * Up1 - Up2) / 6.0;
+  return (w1 * f1 + w2 * f2 + w3 * f3) / sum;
+}
+
+// Reconstructed conserved state -> primitive (src/weno.cpp:146-150).
+__device__ __forceinline__ void cons_to_prim(const double (&a)[5], double gam1, double (&s)[5]) {
+  double rho = a[0];
+  double u = a[1] / rho;
+  double v = a[2] / rho;
+  double w = a[3] / rho;
+  s[0] = rho;
+  s[1] = u;
+  s[2] = v;
+  s[3] = w;
+  s[4] = gam1 * (a[4] - 0.5 * rho * (u * u + v * v + w * w));
+}
+
+// Face flux along axis D (0 = x, 1 = y, 2 = z).
+//   q[c][e]  conserved values of cells i-3 .. i+2 along D (c = 0..5)
+//   pl, pr   rho, u, v, w, p of cells i-1 and i (calc_primitive of the same Q)
+template <int D>
+__device__ __forceinline__ void face_flux(const double (&q)[6][5], const double (&pl)[5],
+                                          const double (&pr)[5], double gamma, double gam1,
+                                          double (&flux)[5]) {
+  // ---- Roe average (src/eigenvectors_roe.cpp:87-106)
+  double c_L = sqrt(gamma * pl[4] / pl[0]);
+  double H_L = c_L * c_L / gam1 + 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+  double c_R = sqrt(gamma * pr[4] / pr[0]);
+  double H_R = c_R * c_R / gam1 + 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+  double ratio = sqrt(pr[0] / pl[0]);
+  double uu = (pl[1] + ratio * pr[1]) / (1.0 + ratio);
+  double vv = (pl[2] + ratio * pr[2]) / (1.0 + ratio);
+  double ww = (pl[3] + ratio * pr[3]) / (1.0 + ratio);
+  double H = (H_L + ratio * H_R) / (1.0 + ratio);
+  double q2 = uu * uu + vv * vv + ww * ww;
+  double c = sqrt(gam1 * (H - 0.5 * q2));
+  // frame (n, l, m) per axis (src/eigenvectors_roe.cpp:48-85):
+  //   x: n=(1,0,0) l=(-0,0,1) m=(0,-1,0)   qn=uu ql=ww  qm=-vv
+  //   y: n=(0,1,0) l=(0,0,-1) m=(-1,0,0)   qn=vv ql=-ww qm=-uu
+  //   z: n=(0,0,1) l=(0,1,-0) m=(-1,0,0)   qn=ww ql=vv  qm=-uu
+  const double qn = D == 0 ? uu : (D == 1 ? vv : ww);
+
+  // ---- L rows (src/eigenvectors_roe.cpp:138-169)
+  double K = gam1 / (2.0 * c * c);
+  double one_2c = 0.5 / c;
+  double L00 = 0.5 * K * q2 + one_2c * qn;
+  double L10 = 1.0 - K * q2;
+  double L20 = 0.5 * K * q2 - one_2c * qn;
+  double Ku = K * uu, Kv = K * vv, Kw = K * ww;
+  double L01 = D == 0 ? -(Ku + one_2c) : -Ku;
+  double L02 = D == 1 ? -(Kv + one_2c) : -Kv;
+  double L03 = D == 2 ? -(Kw + one_2c) : -Kw;
+  double L21 = D == 0 ? -(Ku - one_2c) : -Ku;
+  double L22 = D == 1 ? -(Kv - one_2c) : -Kv;
+  double L23 = D == 2 ? -(Kw - one_2c) : -Kw;
+  double L11 = 2.0 * K * uu, L12 = 2.0 * K * vv, L13 = 2.0 * K * ww, L14 = -2.0 * K;
+
+  // ---- R entries that are not 0 or +-1 (src/eigenvectors_roe.cpp:108-136)
+  double Rm0[3], Rp2[3];  // velocity rows of columns 0 and 2
+  Rm0[0] = D == 0 ? uu - c : uu;
+  Rm0[1] = D == 1 ? vv - c : vv;
+  Rm0[2] = D == 2 ? ww - c : ww;
+  Rp2[0] = D == 0 ? uu + c : uu;
+  Rp2[1] = D == 1 ? vv + c : vv;
+  Rp2[2] = D == 2 ? ww + c : ww;
+  double R40 = H - qn * c;
+  double R41 = 0.5 * q2;
+  double R42 = H + qn * c;
+
+  double aL[5], aR[5];  // reconstructed conserved states (back-projection accumulators)
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    double fc[6];
+#pragma unroll
+    for (int cc = 0; cc < 6; cc++) {
+      const double f0 = q[cc][0], f1 = q[cc][1], f2 = q[cc][2], f3 = q[cc][3], f4 = q[cc][4];
+      if (m == 0) {
+        fc[cc] = L00 * f0 + L01 * f1 + L02 * f2 + L03 * f3 + K * f4;
+      } else if (m == 1) {
+        fc[cc] = L10 * f0 + L11 * f1 + L12 * f2 + L13 * f3 + L14 * f4;
+      } else if (m == 2) {
+        fc[cc] = L20 * f0 + L21 * f1 + L22 * f2 + L23 * f3 + K * f4;
+      } else if (m == 3) {  // row (-ql, l)
+        fc[cc] = D == 0 ? f3 - ww * f0 : (D == 1 ? ww * f0 - f3 : f2 - vv * f0);
+      } else {  // row (-qm, m)
+        fc[cc] = D == 0 ? vv * f0 - f2 : uu * f0 - f1;
+      }
+    }
+    // left state from cells i-3..i+1, right state from cells i+2..i-2
+    const double wl = weno5(fc[0], fc[1], fc[2], fc[3], fc[4]);
+    const double wr = weno5(fc[5], fc[4], fc[3], fc[2], fc[1]);
+    if (m == 0) {
+      aL[0] = wl;              aR[0] = wr;
+      aL[1] = Rm0[0] * wl;     aR[1] = Rm0[0] * wr;
+      aL[2] = Rm0[1] * wl;     aR[2] = Rm0[1] * wr;
+      aL[3] = Rm0[2] * wl;     aR[3] = Rm0[2] * wr;
+      aL[4] = R40 * wl;        aR[4] = R40 * wr;
+    } else if (m == 1) {
+      aL[0] = aL[0] + wl;      aR[0] = aR[0] + wr;
+      aL[1] = aL[1] + uu * wl; aR[1] = aR[1] + uu * wr;
+      aL[2] = aL[2] + vv * wl; aR[2] = aR[2] + vv * wr;
+      aL[3] = aL[3] + ww * wl; aR[3] = aR[3] + ww * wr;
+      aL[4] = aL[4] + R41 * wl; aR[4] = aR[4] + R41 * wr;
+    } else if (m == 2) {
+      aL[0] = aL[0] + wl;          aR[0] = aR[0] + wr;
+      aL[1] = aL[1] + Rp2[0] * wl; aR[1] = aR[1] + Rp2[0] * wr;
+      aL[2] = aL[2] + Rp2[1] * wl; aR[2] = aR[2] + Rp2[1] * wr;
+      aL[3] = aL[3] + Rp2[2] * wl; aR[3] = aR[3] + Rp2[2] * wr;
+      aL[4] = aL[4] + R42 * wl;    aR[4] = aR[4] + R42 * wr;
+    } else if (m == 3) {  // column (0, l, ql)
+      if (D == 0) {
+        aL[3] = aL[3] + wl;      aR[3] = aR[3] + wr;
+        aL[4] = aL[4] + ww * wl; aR[4] = aR[4] + ww * wr;
+      } else if (D == 1) {
+        aL[3] = aL[3] - wl;      aR[3] = aR[3] - wr;
+        aL[4] = aL[4] - ww * wl; aR[4] = aR[4] - ww * wr;
+      } else {
+        aL[2] = aL[2] + wl;      aR[2] = aR[2] + wr;
+        aL[4] = aL[4] + vv * wl; aR[4] = aR[4] + vv * wr;
+      }
+    } else {  // column (0, m, qm)
+      if (D == 0) {
+        aL[2] = aL[2] - wl;      aR[2] = aR[2] - wr;
+        aL[4] = aL[4] - vv * wl; aR[4] = aR[4] - vv * wr;
+      } else {
+        aL[1] = aL[1] - wl;      aR[1] = aR[1] - wr;
+        aL[4] = aL[4] - uu * wl; aR[4] = aR[4] - uu * wr;
+      }
+    }
+  }
+
+  double sL[5], sR[5];
+  cons_to_prim(aL, gam1, sL);
+  cons_to_prim(aR, gam1, sR);
+
+  // ---- local Lax-Friedrichs flux (src/lf_flux.cpp:32-99)
+  double cR = sqrt(gamma * sR[4] / sR[0]);
+  double HR = cR * cR / gam1 + 0.5 * (sR[1] * sR[1] + sR[2] * sR[2] + sR[3] * sR[3]);
+  double rER = sR[4] / gam1 + 0.5 * sR[0] * (sR[1] * sR[1] + sR[2] * sR[2] + sR[3] * sR[3]);
+  double cL = sqrt(gamma * sL[4] / sL[0]);
+  double HL = cL * cL / gam1 + 0.5 * (sL[1] * sL[1] + sL[2] * sL[2] + sL[3] * sL[3]);
+  double rEL = sL[4] / gam1 + 0.5 * sL[0] * (sL[1] * sL[1] + sL[2] * sL[2] + sL[3] * sL[3]);
+  const double qR = sR[1 + D], qL = sL[1 + D];
+  double ws = fabs(qR) + cR;
+  double wsL = fabs(qL) + cL;
+  if (wsL > ws) ws = wsL;
+
+  double diss[5], fL[5], fR[5];
+  diss[0] = ws * (sR[0] - sL[0]);
+  diss[1] = ws * (sR[0] * sR[1] - sL[0] * sL[1]);
+  diss[2] = ws * (sR[0] * sR[2] - sL[0] * sL[2]);
+  diss[3] = ws * (sR[0] * sR[3] - sL[0] * sL[3]);
+  diss[4] = ws * (rER - rEL);
+  const double mL = sL[0] * qL, mR = sR[0] * qR;
+  fL[0] = mL;
+  fR[0] = mR;
+#pragma unroll
+  for (int a = 1; a <= 3; a++) {
+    fL[a] = (a == 1 + D) ? mL * sL[a] + sL[4] : mL * sL[a];
+    fR[a] = (a == 1 + D) ? mR * sR[a] + sR[4] : mR * sR[a];
+  }
+  fL[4] = mL * HL;
+  fR[4] = mR * HR;
+#pragma unroll
+  for (int e = 0; e < 5; e++) flux[e] = 0.5 * (fL[e] + fR[e] - diss[e]);
+}
+
+}  // namespace cbns

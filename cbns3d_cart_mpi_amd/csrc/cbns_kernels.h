@@ -1,0 +1,83 @@
+// Device-side data layout and kernel launchers shared by the kernels and the
+// C ABI.  All state lives in HBM in SoA form (one array per conserved or
+// primitive field, x fastest) so that every stencil load of a wavefront is
+// one contiguous run along x.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cbns {
+
+constexpr int NEQ = 5;
+constexpr int NG = 4;   // ghost layers
+constexpr int NGV = 2;  // viscous halo
+constexpr int MAXB = 512;
+
+// Physical constants of one solve (src/read_input.cpp:78-90 and the static
+// caches of src/Block3d.cpp:563-564, 764-767).
+struct Phys {
+  double gamma, gam1, gM2, CFL, C_T_inf, Re_inv, kappa, C_dt_v;
+  double dx, dy, dz;
+};
+
+// One on-device block: the reference's Block3d arrays, SoA.
+struct BlockDesc {
+  double* Q[2];   // Q, Q_p: [5][g2][g1][g0], ghosted
+  double* F[3];   // face fluxes Ep, Fp, Gp: [5][n2+dz][n1+dy][n0+dx]
+  double* dfv;    // viscous divergence [4][n2][n1][n0]
+  double* prim;   // rho, u, v, w, p, T: [6][g2][g1][g0]
+  double* mu;     // Sutherland viscosity on the interior [n2][n1][n0]
+  double* visc;   // txx tyy tzz txy txz tyz ex ey ez on the (n+4)^3 box
+  int n[3];       // interior cells
+  int g[3];       // ghosted extent n + 8
+  int o[3];       // global origin
+  int pad;
+  long long sg;   // ghosted field stride
+  long long sv;   // viscous-box field stride
+  long long sf[3];// face-array field strides
+  long long sn;   // interior field stride
+};
+
+// Partition of the periodic domain (src/cbns3d.cpp:102-126).
+struct Domain {
+  int N[3];     // global cells
+  int dims[3];  // blocks per axis
+  int base[3];  // N / dims
+  int nb;
+};
+
+struct Launch {
+  hipStream_t stream;
+  const BlockDesc* dblocks;  // device copy
+  int nb;
+  int maxcount;              // per-kernel max work per block (set by launcher)
+};
+
+// kernel launchers (cbns_kernels.hip)
+hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
+                            const Phys& ph, hipStream_t s);
+hipError_t launch_primitive(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
+                            hipStream_t s);
+hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
+                             hipStream_t s);
+hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, hipStream_t s);
+hipError_t launch_update(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
+                         const Phys& ph, hipStream_t s);
+hipError_t launch_dt_reduce(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph,
+                            unsigned long long* d_smax, int* d_flag, hipStream_t s);
+hipError_t launch_copy_state(const BlockDesc* db, const BlockDesc* hb, int nb, int from, int to, hipStream_t s);
+hipError_t launch_tgv(const BlockDesc* db, const BlockDesc* hb, int nb, const double* tab, int tabstride,
+                      const Phys& ph, double p_inf, hipStream_t s);
+// interior AoS (global) <-> block SoA
+hipError_t launch_scatter_aos(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
+                              const double* aos, hipStream_t s);
+hipError_t launch_gather_aos(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
+                             double* aos, hipStream_t s);
+hipError_t launch_fill_mu(const BlockDesc* db, const BlockDesc* hb, int nb, double mu0, hipStream_t s);
+hipError_t launch_diagnostics(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, double* d_part,
+                              int nparts, double* d_out, hipStream_t s);
+hipError_t launch_face_eval(int dir, double gamma, int nfaces, const double* cells, const double* prim,
+                            double* flux, hipStream_t s);
+
+}  // namespace cbns

@@ -1,0 +1,587 @@
+// HIP kernels of the plain FP64 right-hand-side and time-step path (gfx950).
+//
+// Work mapping: grid.y selects the on-device block, grid.x * 256 threads walk
+// a flat index over that block's work items with x fastest, so each wavefront
+// touches 64 consecutive cells of one x-row (coalesced SoA loads).  Stencil
+// neighbours along y and z are the same rows shifted by g0 or g0*g1, served
+// from L1/L2.
+#include "cbns_face.h"
+#include "cbns_kernels.h"
+
+namespace cbns {
+
+constexpr int TPB = 256;
+
+static inline unsigned nblocks_for(long long count) { return (unsigned)((count + TPB - 1) / TPB); }
+
+__device__ __forceinline__ int gidx(const BlockDesc& B, int i, int j, int k) {
+  return (i + NG) + B.g[0] * ((j + NG) + B.g[1] * (k + NG));
+}
+
+// ---------------------------------------------------------------------------
+// Inviscid face fluxes: rec_riemann_x/y/z (src/Block3d.cpp:555-748).
+// One thread per face; face (i,j,k) along D sits between cells i-1 and i.
+template <int D>
+__global__ __launch_bounds__(TPB) void k_face_flux(const BlockDesc* __restrict__ blocks, int which,
+                                                   double gamma, double gam1) {
+  const BlockDesc& B = blocks[blockIdx.y];
+  const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= ni * nj * nk) return;
+  const int i = t % ni;
+  const int r = t / ni;
+  const int j = r % nj;
+  const int k = r / nj;
+  const int c0 = gidx(B, i, j, k);
+  const int sD = D == 0 ? 1 : (D == 1 ? B.g[0] : B.g[0] * B.g[1]);
+  const double* __restrict__ Q = B.Q[which];
+  const long long sg = B.sg;
+  double q[6][5];
+#pragma unroll
+  for (int cc = 0; cc < 6; cc++)
+#pragma unroll
+    for (int e = 0; e < 5; e++) q[cc][e] = Q[e * sg + c0 + (cc - 3) * sD];
+  const double* __restrict__ P = B.prim;
+  double pl[5], pr[5];
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    pl[f] = P[f * sg + c0 - sD];
+    pr[f] = P[f * sg + c0];
+  }
+  double fl[5];
+  face_flux<D>(q, pl, pr, gamma, gam1, fl);
+  double* __restrict__ F = B.F[D];
+  const long long sf = B.sf[D];
+#pragma unroll
+  for (int e = 0; e < 5; e++) F[e * sf + t] = fl[e];
+}
+
+hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
+                            const Phys& ph, hipStream_t s) {
+  long long mx = 0;
+  for (int b = 0; b < nb; b++) {
+    long long c = (long long)(hb[b].n[0] + (dir == 0)) * (hb[b].n[1] + (dir == 1)) * (hb[b].n[2] + (dir == 2));
+    mx = c > mx ? c : mx;
+  }
+  dim3 grid(nblocks_for(mx), nb);
+  if (dir == 0) hipLaunchKernelGGL(k_face_flux<0>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1);
+  else if (dir == 1) hipLaunchKernelGGL(k_face_flux<1>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1);
+  else hipLaunchKernelGGL(k_face_flux<2>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// calc_primitive over the whole ghosted box (src/Block3d.cpp:359-385).
+__global__ __launch_bounds__(TPB) void k_primitive(const BlockDesc* __restrict__ blocks, int which, double gam1,
+                                                   double gM2) {
+  const BlockDesc& B = blocks[blockIdx.y];
+  const long long sg = B.sg;
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= sg) return;
+  const double* __restrict__ Q = B.Q[which];
+  double* __restrict__ P = B.prim;
+  double rr = Q[t];
+  double uu = Q[sg + t] / rr;
+  double vv = Q[2 * sg + t] / rr;
+  double ww = Q[3 * sg + t] / rr;
+  double pp = gam1 * (Q[4 * sg + t] - 0.5 * rr * (uu * uu + vv * vv + ww * ww));
+  P[t] = rr;
+  P[sg + t] = uu;
+  P[2 * sg + t] = vv;
+  P[3 * sg + t] = ww;
+  P[4 * sg + t] = pp;
+  P[5 * sg + t] = gM2 * pp / rr;
+}
+
+hipError_t launch_primitive(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
+                            hipStream_t s) {
+  long long mx = 0;
+  for (int b = 0; b < nb; b++) mx = hb[b].sg > mx ? hb[b].sg : mx;
+  hipLaunchKernelGGL(k_primitive, dim3(nblocks_for(mx), nb), dim3(TPB), 0, s, db, which, ph.gam1, ph.gM2);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Ghost fill: boundary_condition (src/boundary_condition.cpp:4-231) without
+// MPI.  The reference's three directional exchange phases leave every ghost
+// cell equal to the interior cell of its periodic image; here each ghost cell
+// gathers that value directly from the block that owns it.
+__global__ __launch_bounds__(TPB) void k_ghost_fill(const BlockDesc* __restrict__ blocks, Domain dom, int which) {
+  const BlockDesc& B = blocks[blockIdx.y];
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= B.sg) return;
+  const int i = t % B.g[0] - NG;
+  const int r = t / B.g[0];
+  const int j = r % B.g[1] - NG;
+  const int k = r / B.g[1] - NG;
+  if (i >= 0 && i < B.n[0] && j >= 0 && j < B.n[1] && k >= 0 && k < B.n[2]) return;
+  int gl[3] = {B.o[0] + i, B.o[1] + j, B.o[2] + k};
+  int bc[3], lc[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    int x = gl[a] % dom.N[a];
+    x = x < 0 ? x + dom.N[a] : x;
+    int bb = x / dom.base[a];
+    bb = bb > dom.dims[a] - 1 ? dom.dims[a] - 1 : bb;
+    bc[a] = bb;
+    lc[a] = x - bb * dom.base[a];
+  }
+  const BlockDesc& S = blocks[(bc[0] * dom.dims[1] + bc[1]) * dom.dims[2] + bc[2]];
+  const int src = gidx(S, lc[0], lc[1], lc[2]);
+  const double* __restrict__ Qs = S.Q[which];
+  double* __restrict__ Qd = B.Q[which];
+#pragma unroll
+  for (int e = 0; e < NEQ; e++) Qd[e * B.sg + t] = Qs[e * S.sg + src];
+}
+
+hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
+                             hipStream_t s) {
+  long long mx = 0;
+  for (int b = 0; b < dom.nb; b++) mx = hb[b].sg > mx ? hb[b].sg : mx;
+  hipLaunchKernelGGL(k_ghost_fill, dim3(nblocks_for(mx), dom.nb), dim3(TPB), 0, s, db, dom, which);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Viscous terms.  k_visc_flux: calc_gradient (src/calc_gradient.cpp:17-66),
+// calc_viscous_terms (src/Block3d.cpp:750-810) and calc_viscous_flux
+// (src/Block3d.cpp:812-868) fused per point of the extended box [-2, n+2)^3,
+// keeping the nine distinct flux components.  mu is kept on the interior for
+// calc_dt (the reference stores it everywhere; only the interior is read).
+__device__ __forceinline__ double grad4(const double* __restrict__ f, int c, int s, double h) {
+  double d = (1.0 / 12.0) * f[c - 2 * s] - 2.0 / 3.0 * f[c - s] + (2.0 / 3.0) * f[c + s] - 1.0 / 12.0 * f[c + 2 * s];
+  return d / h;
+}
+
+__global__ __launch_bounds__(TPB) void k_visc_flux(const BlockDesc* __restrict__ blocks, Phys ph) {
+  const BlockDesc& B = blocks[blockIdx.y];
+  const int v0 = B.n[0] + 2 * NGV, v1 = B.n[1] + 2 * NGV, v2 = B.n[2] + 2 * NGV;
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= v0 * v1 * v2) return;
+  const int i = t % v0 - NGV;
+  const int r = t / v0;
+  const int j = r % v1 - NGV;
+  const int k = r / v1 - NGV;
+  const int c = gidx(B, i, j, k);
+  const int sy = B.g[0], sz = B.g[0] * B.g[1];
+  const long long sg = B.sg;
+  const double* __restrict__ u = B.prim + sg;
+  const double* __restrict__ v = B.prim + 2 * sg;
+  const double* __restrict__ w = B.prim + 3 * sg;
+  const double* __restrict__ T = B.prim + 5 * sg;
+  const double u_x = grad4(u, c, 1, ph.dx), u_y = grad4(u, c, sy, ph.dy), u_z = grad4(u, c, sz, ph.dz);
+  const double v_x = grad4(v, c, 1, ph.dx), v_y = grad4(v, c, sy, ph.dy), v_z = grad4(v, c, sz, ph.dz);
+  const double w_x = grad4(w, c, 1, ph.dx), w_y = grad4(w, c, sy, ph.dy), w_z = grad4(w, c, sz, ph.dz);
+  const double T_x = grad4(T, c, 1, ph.dx), T_y = grad4(T, c, sy, ph.dy), T_z = grad4(T, c, sz, ph.dz);
+
+  const double T_l = T[c];
+  double mu_l = (1.0 + ph.C_T_inf) / (T_l + ph.C_T_inf) * sqrt(T_l * T_l * T_l);
+  if (i >= 0 && i < B.n[0] && j >= 0 && j < B.n[1] && k >= 0 && k < B.n[2])
+    B.mu[i + B.n[0] * (j + B.n[1] * k)] = mu_l;
+  mu_l *= ph.Re_inv;
+  const double txx = (2.0 / 3.0) * mu_l * (2.0 * u_x - v_y - w_z);
+  const double tyy = (2.0 / 3.0) * mu_l * (2.0 * v_y - u_x - w_z);
+  const double tzz = (2.0 / 3.0) * mu_l * (2.0 * w_z - u_x - v_y);
+  const double txy = mu_l * (u_y + v_x);
+  const double txz = mu_l * (u_z + w_x);
+  const double tyz = mu_l * (v_z + w_y);
+  mu_l *= ph.kappa;
+  const double qx = mu_l * T_x, qy = mu_l * T_y, qz = mu_l * T_z;
+  const double uu = u[c], vv = v[c], ww = w[c];
+  const double ex = uu * txx + vv * txy + ww * txz - qx;
+  const double ey = uu * txy + vv * tyy + ww * tyz - qy;
+  const double ez = uu * txz + vv * tyz + ww * tzz - qz;
+  double* __restrict__ V = B.visc;
+  const long long sv = B.sv;
+  V[t] = txx;
+  V[sv + t] = tyy;
+  V[2 * sv + t] = tzz;
+  V[3 * sv + t] = txy;
+  V[4 * sv + t] = txz;
+  V[5 * sv + t] = tyz;
+  V[6 * sv + t] = ex;
+  V[7 * sv + t] = ey;
+  V[8 * sv + t] = ez;
+}
+
+// Divergence part of calc_viscous_flux_contribution (src/Block3d.cpp:879-916).
+__device__ __forceinline__ double cdiff4(const double* __restrict__ f, int c, int s) {
+  const double c0 = 1.0 / 12.0, c1 = -2.0 / 3.0;
+  return c0 * f[c - 2 * s] + c1 * f[c - s] - c1 * f[c + s] - c0 * f[c + 2 * s];
+}
+
+__global__ __launch_bounds__(TPB) void k_visc_div(const BlockDesc* __restrict__ blocks, Phys ph) {
+  const BlockDesc& B = blocks[blockIdx.y];
+  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= n0 * n1 * n2) return;
+  const int i = t % n0;
+  const int r = t / n0;
+  const int j = r % n1;
+  const int k = r / n1;
+  const int v0 = n0 + 2 * NGV, v1 = n1 + 2 * NGV;
+  const int c = (i + NGV) + v0 * ((j + NGV) + v1 * (k + NGV));
+  const int sy = v0, sz = v0 * v1;
+  const double* __restrict__ V = B.visc;
+  const long long sv = B.sv;
+  // Ev = (txx, txy, txz, ex), Fv = (txy, tyy, tyz, ey), Gv = (txz, tyz, tzz, ez)
+  const int Ef[4] = {0, 3, 4, 6}, Ff[4] = {3, 1, 5, 7}, Gf[4] = {4, 5, 2, 8};
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    double df = cdiff4(V + Ef[e] * sv, c, 1) / ph.dx;
+    df += cdiff4(V + Ff[e] * sv, c, sy) / ph.dy;
+    df += cdiff4(V + Gf[e] * sv, c, sz) / ph.dz;
+    B.dfv[e * B.sn + t] = df;
+  }
+}
+
+hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, hipStream_t s) {
+  long long mv = 0, mn = 0;
+  for (int b = 0; b < nb; b++) {
+    long long cv = (long long)(hb[b].n[0] + 4) * (hb[b].n[1] + 4) * (hb[b].n[2] + 4);
+    mv = cv > mv ? cv : mv;
+    mn = hb[b].sn > mn ? hb[b].sn : mn;
+  }
+  hipLaunchKernelGGL(k_visc_flux, dim3(nblocks_for(mv), nb), dim3(TPB), 0, s, db, ph);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_visc_div, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, ph);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// update_rk3 (src/Block3d.cpp:920-962) on the interior.
+__global__ __launch_bounds__(TPB) void k_update(const BlockDesc* __restrict__ blocks, double dt, int stage,
+                                                Phys ph) {
+  const BlockDesc& B = blocks[blockIdx.y];
+  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= n0 * n1 * n2) return;
+  const int i = t % n0;
+  const int r = t / n0;
+  const int j = r % n1;
+  const int k = r / n1;
+  const int c = gidx(B, i, j, k);
+  const int iE = i + (n0 + 1) * (j + n1 * k);
+  const int iF = i + n0 * (j + (n1 + 1) * k);
+  const int iG = t;
+  double* __restrict__ Q = B.Q[0];
+  double* __restrict__ Qp = B.Q[1];
+  const long long sg = B.sg;
+#pragma unroll
+  for (int e = 0; e < NEQ; e++) {
+    const double* __restrict__ Ep = B.F[0] + e * B.sf[0];
+    const double* __restrict__ Fp = B.F[1] + e * B.sf[1];
+    const double* __restrict__ Gp = B.F[2] + e * B.sf[2];
+    double df = (Ep[iE + 1] - Ep[iE]) / ph.dx;
+    df += (Fp[iF + n0] - Fp[iF]) / ph.dy;
+    df += (Gp[iG + n0 * n1] - Gp[iG]) / ph.dz;
+    if (e > 0) df -= B.dfv[(e - 1) * B.sn + t];
+    const long long q = e * sg + c;
+    if (1 == stage) {
+      Qp[q] = Q[q] - dt * df;
+    } else if (2 == stage) {
+      Qp[q] = 0.75 * Q[q] + 0.25 * (Qp[q] - dt * df);
+    } else if (3 == stage) {
+      Q[q] = (Q[q] + 2.0 * (Qp[q] - dt * df)) / 3.0;
+    }
+  }
+}
+
+hipError_t launch_update(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
+                         const Phys& ph, hipStream_t s) {
+  long long mn = 0;
+  for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
+  hipLaunchKernelGGL(k_update, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, dt, stage, ph);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// calc_dt spectral radius (src/Block3d.cpp:515-553), max over every block.
+// The max is exact and order-independent, so one global max followed by one
+// division equals the reference's per-rank CFL/s_max and MIN-allreduce.
+// Non-negative doubles order like their bit patterns, so the cross-workgroup
+// max is an integer atomic max.  A NaN never wins (the reference's ternary
+// skips it) but raises the non-finite flag, as does +inf.
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(TPB) void k_dt(const BlockDesc* __restrict__ blocks, Phys ph,
+                                            unsigned long long* smax, int* flag) {
+  const BlockDesc& B = blocks[blockIdx.y];
+  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  const int total = n0 * n1 * n2;
+  const long long sg = B.sg;
+  double s_max = 0.0;
+  int bad = 0;
+  for (int t = blockIdx.x * TPB + threadIdx.x; t < total; t += gridDim.x * TPB) {
+    const int i = t % n0;
+    const int r = t / n0;
+    const int j = r % n1;
+    const int k = r / n1;
+    const int c = gidx(B, i, j, k);
+    const double rr = B.prim[c], uu = B.prim[sg + c], vv = B.prim[2 * sg + c], ww = B.prim[3 * sg + c];
+    const double pp = B.prim[4 * sg + c];
+    const double cs = sqrt(ph.gamma * pp / rr);
+    const double factor_v = 2.0 * B.mu[t] * ph.C_dt_v * sqrt(uu * uu + vv * vv + ww * ww) / cs / rr;
+    double t1 = fabs(uu) + cs / ph.dx;
+    t1 += factor_v / ph.dx / ph.dx;
+    double t2 = fabs(vv) + cs / ph.dy;
+    t2 += factor_v / ph.dy / ph.dy;
+    double t3 = fabs(ww) + cs / ph.dz;
+    t3 += factor_v / ph.dz / ph.dz;
+    const double s = t1 + t2 + t3;
+    if (!(s < 1.0e308)) bad = 1;  // NaN or inf
+    s_max = (s_max < s) ? s : s_max;
+  }
+  s_max = wave_max(s_max);
+  __shared__ double red[TPB / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = s_max;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int a = 1; a < TPB / 64; a++) m = red[a] > m ? red[a] : m;
+    atomicMax(smax, (unsigned long long)__double_as_longlong(m));
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
+hipError_t launch_dt_reduce(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph,
+                            unsigned long long* d_smax, int* d_flag, hipStream_t s) {
+  long long mn = 0;
+  for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
+  unsigned gx = nblocks_for(mn);
+  gx = gx > 1024 ? 1024 : gx;
+  hipLaunchKernelGGL(k_dt, dim3(gx, nb), dim3(TPB), 0, s, db, ph, d_smax, d_flag);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// init_Q_p and other whole-array copies between Q and Q_p (src/Block3d.cpp:387-394).
+__global__ __launch_bounds__(TPB) void k_copy_state(const BlockDesc* __restrict__ blocks, int from, int to) {
+  const BlockDesc& B = blocks[blockIdx.y];
+  const long long n = NEQ * B.sg;
+  const double* __restrict__ a = B.Q[from];
+  double* __restrict__ b = B.Q[to];
+  for (long long t = blockIdx.x * (long long)TPB + threadIdx.x; t < n; t += (long long)gridDim.x * TPB) b[t] = a[t];
+}
+
+hipError_t launch_copy_state(const BlockDesc* db, const BlockDesc* hb, int nb, int from, int to, hipStream_t s) {
+  long long mx = 0;
+  for (int b = 0; b < nb; b++) mx = NEQ * hb[b].sg > mx ? NEQ * hb[b].sg : mx;
+  unsigned gx = nblocks_for(mx);
+  gx = gx > 4096 ? 4096 : gx;
+  hipLaunchKernelGGL(k_copy_state, dim3(gx, nb), dim3(TPB), 0, s, db, from, to);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Taylor-Green start: initial_condition + calc_conservative
+// (src/Block3d.cpp:237-300) from per-axis tables of sin(x_c), cos(x_c),
+// cos(2 x_c) evaluated on the host with the C library.
+__global__ __launch_bounds__(TPB) void k_tgv(const BlockDesc* __restrict__ blocks, const double* __restrict__ tab,
+                                             int ts, double gM2, double gam1, double p_inf) {
+  const BlockDesc& B = blocks[blockIdx.y];
+  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= n0 * n1 * n2) return;
+  const int i = t % n0;
+  const int r = t / n0;
+  const int j = r % n1;
+  const int k = r / n1;
+  const int gi = B.o[0] + i, gj = B.o[1] + j, gk = B.o[2] + k;
+  // tab[(3*a + f) * ts + index], f: 0 sin, 1 cos, 2 cos(2x)
+  const double sx = tab[0 * ts + gi], cx = tab[1 * ts + gi], c2x = tab[2 * ts + gi];
+  const double sy = tab[3 * ts + gj], cy = tab[4 * ts + gj], c2y = tab[5 * ts + gj];
+  const double cz = tab[7 * ts + gk], c2z = tab[8 * ts + gk];
+  const double uu = sx * cy * cz;
+  const double vv = -cx * sy * cz;
+  const double ww = 0.0;
+  const double pp = p_inf + (c2x + c2y) * (c2z + 2.0) / 16.0;
+  const double rr = gM2 * pp;
+  const int c = gidx(B, i, j, k);
+  double* __restrict__ Q = B.Q[0];
+  const long long sg = B.sg;
+  Q[c] = rr;
+  Q[sg + c] = rr * uu;
+  Q[2 * sg + c] = rr * vv;
+  Q[3 * sg + c] = rr * ww;
+  Q[4 * sg + c] = pp / gam1 + 0.5 * rr * (uu * uu + vv * vv + ww * ww);
+  B.mu[t] = 1.0;
+}
+
+hipError_t launch_tgv(const BlockDesc* db, const BlockDesc* hb, int nb, const double* tab, int ts,
+                      const Phys& ph, double p_inf, hipStream_t s) {
+  long long mn = 0;
+  for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
+  hipLaunchKernelGGL(k_tgv, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, tab, ts, ph.gM2, ph.gam1, p_inf);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Global interior AoS [k][j][i][eq] <-> per-block SoA.
+__global__ __launch_bounds__(TPB) void k_aos(const BlockDesc* __restrict__ blocks, Domain dom, int which,
+                                             double* __restrict__ aos, int to_blocks) {
+  const BlockDesc& B = blocks[blockIdx.y];
+  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= n0 * n1 * n2) return;
+  const int i = t % n0;
+  const int r = t / n0;
+  const int j = r % n1;
+  const int k = r / n1;
+  const long long gidx_ = (long long)NEQ *
+      ((long long)(B.o[0] + i) + (long long)dom.N[0] * ((long long)(B.o[1] + j) + (long long)dom.N[1] * (B.o[2] + k)));
+  const int c = gidx(B, i, j, k);
+  double* __restrict__ Q = B.Q[which];
+#pragma unroll
+  for (int e = 0; e < NEQ; e++) {
+    if (to_blocks) Q[e * B.sg + c] = aos[gidx_ + e];
+    else aos[gidx_ + e] = Q[e * B.sg + c];
+  }
+}
+
+hipError_t launch_scatter_aos(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
+                              const double* aos, hipStream_t s) {
+  long long mn = 0;
+  for (int b = 0; b < dom.nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
+  hipLaunchKernelGGL(k_aos, dim3(nblocks_for(mn), dom.nb), dim3(TPB), 0, s, db, dom, which, (double*)aos, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_aos(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which, double* aos,
+                             hipStream_t s) {
+  long long mn = 0;
+  for (int b = 0; b < dom.nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
+  hipLaunchKernelGGL(k_aos, dim3(nblocks_for(mn), dom.nb), dim3(TPB), 0, s, db, dom, which, aos, 0);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(TPB) void k_fill_mu(const BlockDesc* __restrict__ blocks, double mu0) {
+  const BlockDesc& B = blocks[blockIdx.y];
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t < B.sn) B.mu[t] = mu0;
+}
+
+hipError_t launch_fill_mu(const BlockDesc* db, const BlockDesc* hb, int nb, double mu0, hipStream_t s) {
+  long long mn = 0;
+  for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
+  hipLaunchKernelGGL(k_fill_mu, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, mu0);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics: per-workgroup partial sums of 0.5 rho |u|^2 and 0.5 rho |w|^2
+// (vorticity by the 4th-order central stencil), then one fixed-order sum.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(TPB) void k_diag(const BlockDesc* __restrict__ blocks, Phys ph, double* part) {
+  const BlockDesc& B = blocks[blockIdx.y];
+  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  const int total = n0 * n1 * n2;
+  const long long sg = B.sg;
+  const int sy = B.g[0], sz = B.g[0] * B.g[1];
+  const double* u = B.prim + sg;
+  const double* v = B.prim + 2 * sg;
+  const double* w = B.prim + 3 * sg;
+  double ke = 0.0, en = 0.0;
+  for (int t = blockIdx.x * TPB + threadIdx.x; t < total; t += gridDim.x * TPB) {
+    const int i = t % n0;
+    const int r = t / n0;
+    const int j = r % n1;
+    const int k = r / n1;
+    const int c = gidx(B, i, j, k);
+    const double rr = B.prim[c], uu = u[c], vv = v[c], ww = w[c];
+    ke += 0.5 * rr * (uu * uu + vv * vv + ww * ww);
+    const double ox = grad4(w, c, sy, ph.dy) - grad4(v, c, sz, ph.dz);
+    const double oy = grad4(u, c, sz, ph.dz) - grad4(w, c, 1, ph.dx);
+    const double oz = grad4(v, c, 1, ph.dx) - grad4(u, c, sy, ph.dy);
+    en += 0.5 * rr * (ox * ox + oy * oy + oz * oz);
+  }
+  ke = wave_sum(ke);
+  en = wave_sum(en);
+  __shared__ double red[2][TPB / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wv] = ke;
+    red[1][wv] = en;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int x = 0; x < TPB / 64; x++) {
+      a += red[0][x];
+      b += red[1][x];
+    }
+    const int p = blockIdx.y * gridDim.x + blockIdx.x;
+    part[2 * p] = a;
+    part[2 * p + 1] = b;
+  }
+}
+
+__global__ void k_diag_final(const double* part, int nparts, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int p = 0; p < nparts; p++) {
+      a += part[2 * p];
+      b += part[2 * p + 1];
+    }
+    out[0] = a;
+    out[1] = b;
+  }
+}
+
+hipError_t launch_diagnostics(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, double* d_part,
+                              int nparts, double* d_out, hipStream_t s) {
+  (void)hb;
+  const int gx = nparts / nb;
+  hipLaunchKernelGGL(k_diag, dim3(gx, nb), dim3(TPB), 0, s, db, ph, d_part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s, d_part, gx * nb, d_out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Face flux on explicit inputs (parity tests of the per-face arithmetic).
+template <int D>
+__global__ __launch_bounds__(TPB) void k_face_eval(double gamma, int nf, const double* __restrict__ cells,
+                                                   const double* __restrict__ prim, double* __restrict__ flux) {
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= nf) return;
+  double q[6][5], pl[5], pr[5], fl[5];
+#pragma unroll
+  for (int c = 0; c < 6; c++)
+#pragma unroll
+    for (int e = 0; e < 5; e++) q[c][e] = cells[(long long)t * 30 + 5 * c + e];
+#pragma unroll
+  for (int e = 0; e < 5; e++) {
+    pl[e] = prim[(long long)t * 10 + e];
+    pr[e] = prim[(long long)t * 10 + 5 + e];
+  }
+  face_flux<D>(q, pl, pr, gamma, gamma - 1.0, fl);
+#pragma unroll
+  for (int e = 0; e < 5; e++) flux[(long long)t * 5 + e] = fl[e];
+}
+
+hipError_t launch_face_eval(int dir, double gamma, int nf, const double* cells, const double* prim, double* flux,
+                            hipStream_t s) {
+  dim3 grid(nblocks_for(nf));
+  if (dir == 0) hipLaunchKernelGGL(k_face_eval<0>, grid, dim3(TPB), 0, s, gamma, nf, cells, prim, flux);
+  else if (dir == 1) hipLaunchKernelGGL(k_face_eval<1>, grid, dim3(TPB), 0, s, gamma, nf, cells, prim, flux);
+  else hipLaunchKernelGGL(k_face_eval<2>, grid, dim3(TPB), 0, s, gamma, nf, cells, prim, flux);
+  return hipGetLastError();
+}
+
+}  // namespace cbns

@@ -150,6 +150,14 @@ int cbns_diagnostics(cbns_ctx* ctx, double* ke, double* enstrophy);
 /* Synchronise the context's stream. */
 int cbns_sync(cbns_ctx* ctx);
 
+/* Live kernel timing: while on, every launch made by cbns_step / cbns_run is
+ * bracketed by HIP events on the context's stream and its duration is added
+ * to its class.  Turning it on clears the totals.  No counterpart in the
+ * reference (which times only the whole loop, src/cbns3d.cpp:174,272-286). */
+#define CBNS_KCLASSES 8 /* face x, face y, face z, viscous, update, ghost fill, primitive, dt */
+int cbns_set_profiling(cbns_ctx* ctx, int on);
+int cbns_kernel_times(cbns_ctx* ctx, double ms_total[CBNS_KCLASSES], int64_t launches[CBNS_KCLASSES]);
+
 /* Evaluate the device face flux of one face along axis dir on host-given data:
  * cells[6][5] conserved values of cells i-3..i+2 (AoS), prim[2][5] = rho, u, v,
  * w, p of cells i-1 and i.  Same arithmetic as cbns_rec_riemann; for parity

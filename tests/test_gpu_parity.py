@@ -1,0 +1,214 @@
+"""Parity of the HIP plain FP64 path with the reference (golden fixtures) and
+with the CPU oracle, through the C ABI.  GPU only.
+
+Bar: exact equality of every value (IEEE ==, so -0.0 == +0.0) against the
+reference's own outputs; size-independent properties at full size."""
+import hashlib
+
+import numpy as np
+import pytest
+
+from cbns3d_cart_mpi_amd import capi as C
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+NG = 4
+
+
+def solver(N, dims=(1, 1, 1), **kw):
+    return C.Solver(C.make_params(N, **kw), dims)
+
+
+# ------------------------------------------------------------- per face
+def _random_faces(rng, n, gam1=0.4):
+    """Six-cell stencils of physical states: smooth, rough and discontinuous."""
+    rho = np.exp(rng.uniform(-0.7, 0.7, (n, 6)))
+    vel = rng.standard_normal((n, 6, 3)) * rng.choice([0.01, 0.3, 1.0], size=(n, 1, 1))
+    p = np.exp(rng.uniform(0.0, 4.3, (n, 6)))
+    smooth = rng.random(n) < 0.5
+    rho[smooth] = rho[smooth, :1] * (1 + 0.01 * np.arange(6))
+    p[smooth] = p[smooth, :1] * (1 + 0.02 * np.sin(np.arange(6)))
+    jump = rng.random(n) < 0.2
+    rho[jump, 3:] *= 4.0
+    cells = np.empty((n, 6, 5))
+    cells[..., 0] = rho
+    cells[..., 1:4] = rho[..., None] * vel
+    cells[..., 4] = p / gam1 + 0.5 * rho * (vel ** 2).sum(-1)
+    # calc_primitive of cells i-1, i (src/Block3d.cpp:369-373), same operation order
+    prim = np.empty((n, 2, 5))
+    for s, c in ((0, 2), (1, 3)):
+        rr = cells[:, c, 0]
+        uu, vv, ww = cells[:, c, 1] / rr, cells[:, c, 2] / rr, cells[:, c, 3] / rr
+        prim[:, s] = np.column_stack([rr, uu, vv, ww, gam1 * (cells[:, c, 4] - 0.5 * rr * (uu * uu + vv * vv + ww * ww))])
+    return cells, prim
+
+
+def _oracle_face(d, gamma, cells, prim):
+    n = np.zeros(3)
+    n[d] = 1.0
+    R, L = O.calc_eigenvectors(gamma, prim[0], prim[1], n)
+    sL = O.rec_weno5(cells[0:5], R, L, gamma - 1.0)
+    sR = O.rec_weno5(cells[[5, 4, 3, 2, 1]], R, L, gamma - 1.0)
+    return O.lf_flux(gamma, sR, sL, n)
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_face_flux_matches_oracle_exactly(d):
+    rng = np.random.default_rng(100 + d)
+    cells, prim = _random_faces(rng, 4000)
+    dev = C.face_flux_eval(d, 1.4, cells, prim)
+    ref = np.array([_oracle_face(d, 1.4, cells[f], prim[f]) for f in range(cells.shape[0])])
+    assert np.array_equal(dev, ref)
+
+
+# ---------------------------------------------------------- whole solver
+@pytest.mark.parametrize("n", [1, 2, 10])
+def test_tgv16_matches_reference(golden, n):
+    g = golden("ref_tgv16.npz")
+    with solver(16) as s:
+        s.init_tgv()
+        s.run(n)
+        assert s.t_cur == float(g["t_n%d" % n])
+        assert np.array_equal(s.get_state(), g["Q_n%d" % n])
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (3, 2, 1), (2, 1, 3)])
+def test_uneven_grid_matches_reference(golden, dims):
+    g = golden("ref_uneven.npz")
+    N = tuple(int(x) for x in g["N"])
+    with solver(N, dims) as s:
+        s.init_tgv()
+        s.run(int(g["nsteps"]))
+        assert s.t_cur == float(g["t"])
+        assert np.array_equal(s.get_state(), g["Q"])
+
+
+def test_restart_path_matches_reference(golden):
+    g = golden("ref_restart16.npz")
+    with solver(16) as s:
+        s.set_state(g["Q_in"], float(g["t_in"]), mu0=0.0)
+        s.run(int(g["nsteps"]))
+        assert s.t_cur == float(g["t_out"])
+        assert np.array_equal(s.get_state(), g["Q_out"])
+
+
+def _digest_ok(Q, g, n):
+    Qn = np.ascontiguousarray(Q + 0.0)
+    assert np.array_equal(Q.reshape(-1, 5)[g["sample_idx_n%d" % n]], g["sample_val_n%d" % n])
+    assert hashlib.sha256(Qn.tobytes()).digest() == bytes(g["sha256_n%d" % n])
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 2, 2), (4, 2, 2)])
+def test_tgv64_one_and_hundred_steps_match_reference(golden, dims):
+    g = golden("ref_tgv64.npz")
+    with solver(64, dims) as s:
+        s.init_tgv()
+        s.run(1)
+        assert s.t_cur == float(g["t_n1"])
+        _digest_ok(s.get_state(), g, 1)
+        dts = s.run(99)
+        assert s.t_cur == float(g["t_n100"])
+        _digest_ok(s.get_state(), g, 100)
+        # the driver prints dt with 6 significant digits
+        assert np.allclose(dts, g["dt_printed"][1:], rtol=5e-6, atol=0)
+
+
+def test_method_level_sequence_equals_fused_step():
+    with solver(24) as a, solver(24) as b:
+        a.init_tgv()
+        b.init_tgv()
+        a.run(2)
+        for _ in range(2):  # src/cbns3d.cpp:186-240 call by call
+            dt = b.calc_dt()
+            for stage in (1, 2, 3):
+                src = C.CBNS_Q if stage == 1 else C.CBNS_QP
+                for d in range(3):
+                    b.rec_riemann(src, d)
+                b.calc_viscous_flux_contribution()
+                b.update_rk3(dt, stage)
+                dst = C.CBNS_Q if stage == 3 else C.CBNS_QP
+                b.boundary_condition(dst)
+                b.calc_primitive(dst)
+        assert np.array_equal(a.get_state(), b.get_state())
+
+
+def test_dt_and_primitives_match_oracle():
+    N = (20, 16, 12)
+    ob = O.Block(O.make_params(N))
+    ob.init_tgv()
+    ob.run(3)
+    with solver(N) as s:
+        s.init_tgv()
+        s.run(3)
+        assert s.calc_dt() == ob.calc_dt()
+        for f in range(6):
+            assert np.array_equal(s.get_block_primitive(0, f), ob.prim(f)), f
+        mu = s.get_block_primitive(0, 6)[NG:-NG, NG:-NG, NG:-NG]
+        assert np.array_equal(mu, ob.prim(6)[NG:-NG, NG:-NG, NG:-NG])
+
+
+def test_ghost_fill_is_periodic_image_across_blocks():
+    N = (20, 16, 12)
+    with solver(N, (2, 2, 3)) as s:
+        s.init_tgv()
+        s.run(1)
+        Qfull = s.get_state()
+        for b in range(s.num_blocks):
+            n, o = s.block_info(b)
+            Qg = s.get_block_state(b)
+            kk = (np.arange(o[2] - NG, o[2] + n[2] + NG)) % N[2]
+            jj = (np.arange(o[1] - NG, o[1] + n[1] + NG)) % N[1]
+            ii = (np.arange(o[0] - NG, o[0] + n[0] + NG)) % N[0]
+            assert np.array_equal(Qg, Qfull[np.ix_(kk, jj, ii)])
+
+
+def test_diagnostics_match_oracle():
+    ob = O.Block(O.make_params(32))
+    ob.init_tgv()
+    ob.run(2)
+    with solver(32, (2, 1, 2)) as s:
+        s.init_tgv()
+        s.run(2)
+        ke, en = s.diagnostics()
+    ke0, en0 = ob.diagnostics()
+    assert abs(ke - ke0) <= 1e-12 * abs(ke0)
+    assert abs(en - en0) <= 1e-12 * abs(en0)
+
+
+def test_block_state_round_trip_and_checkpoint_layout():
+    with solver(16, (2, 1, 1)) as s:
+        s.init_tgv()
+        s.run(1)
+        Q0 = s.get_block_state(0)
+        Q1 = s.get_block_state(1)
+        s.set_block_state(0, Q1 * 0 + Q0)
+        assert np.array_equal(s.get_block_state(0), Q0)
+
+
+def test_bad_arguments_raise():
+    p = C.make_params(16)
+    with pytest.raises(C.CbnsError):
+        C.Solver(p, (5, 1, 1))  # blocks of 3 cells: the halo does not fit
+    with solver(16) as s:
+        with pytest.raises(C.CbnsError):
+            s.rec_riemann(0, 3)
+
+
+# ------------------------------------------------ full size (properties)
+def test_tgv256_properties():
+    """256^3: conservation, the TGV z-symmetry, determinism and decomposition
+    invariance hold at full size."""
+    with solver(256) as a:
+        a.init_tgv()
+        s0 = a.get_state().reshape(-1, 5).sum(axis=0)
+        a.run(2)
+        Qa = a.get_state()
+    s1 = Qa.reshape(-1, 5).sum(axis=0)
+    scale = np.abs(Qa).reshape(-1, 5).max(axis=0) * 256 ** 3
+    assert np.all(np.abs(s1 - s0) <= 1e-13 * scale)
+    w = Qa[..., 3]
+    assert np.allclose(w, -w[::-1], atol=1e-13)
+    with solver(256, (2, 2, 2)) as b:
+        b.init_tgv()
+        b.run(2)
+        assert np.array_equal(b.get_state(), Qa)
