@@ -4,6 +4,7 @@ with the CPU oracle, through the C ABI.  GPU only.
 Bar: exact equality of every value (IEEE ==, so -0.0 == +0.0) against the
 reference's own outputs; size-independent properties at full size."""
 import hashlib
+import math
 
 import numpy as np
 import pytest
@@ -58,7 +59,12 @@ def test_face_flux_matches_oracle_exactly(d):
     cells, prim = _random_faces(rng, 4000)
     dev = C.face_flux_eval(d, 1.4, cells, prim)
     ref = np.array([_oracle_face(d, 1.4, cells[f], prim[f]) for f in range(cells.shape[0])])
-    assert np.array_equal(dev, ref)
+    # harsh random stencils can reconstruct a negative pressure: those faces are
+    # NaN in the reference arithmetic too, and must be NaN at the same places
+    nan_ref = np.isnan(ref)
+    assert 0 < nan_ref.any(axis=1).sum() < cells.shape[0] // 10
+    bad = ~((dev == ref) | (nan_ref & np.isnan(dev)))
+    assert not bad.any(), "%d faces differ, e.g. %s" % (bad.any(axis=1).sum(), np.argwhere(bad)[:3].tolist())
 
 
 # ---------------------------------------------------------- whole solver
@@ -166,10 +172,13 @@ def test_diagnostics_match_oracle():
     ob = O.Block(O.make_params(32))
     ob.init_tgv()
     ob.run(2)
-    with solver(32, (2, 1, 2)) as s:
+    with solver(32, (2, 1, 2)) as s, solver(32) as s1:
         s.init_tgv()
         s.run(2)
         ke, en = s.diagnostics()
+        s1.init_tgv()
+        s1.run(2)
+        assert s1.diagnostics() == (ke, en)  # fixed global summation order
     ke0, en0 = ob.diagnostics()
     assert abs(ke - ke0) <= 1e-12 * abs(ke0)
     assert abs(en - en0) <= 1e-12 * abs(en0)
@@ -198,12 +207,15 @@ def test_bad_arguments_raise():
 def test_tgv256_properties():
     """256^3: conservation, the TGV z-symmetry, determinism and decomposition
     invariance hold at full size."""
+    def sums(Q):  # exactly rounded sums, so only the solver's rounding shows
+        return np.array([math.fsum(Q[..., e].ravel()) for e in range(5)])
+
     with solver(256) as a:
         a.init_tgv()
-        s0 = a.get_state().reshape(-1, 5).sum(axis=0)
+        s0 = sums(a.get_state())
         a.run(2)
         Qa = a.get_state()
-    s1 = Qa.reshape(-1, 5).sum(axis=0)
+    s1 = sums(Qa)
     scale = np.abs(Qa).reshape(-1, 5).max(axis=0) * 256 ** 3
     assert np.all(np.abs(s1 - s0) <= 1e-13 * scale)
     w = Qa[..., 3]

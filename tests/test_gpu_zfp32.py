@@ -1,0 +1,55 @@
+"""FP32-flux form against the FP64 oracle.  GPU only.
+
+Tolerance (stated by the target): relative L2 error <= 1e-5 per conserved
+field versus the FP64 oracle on the same inputs; the relative L-inf error is
+asserted <= 1e-4 and both are printed per field."""
+import numpy as np
+import pytest
+
+from cbns3d_cart_mpi_amd import capi as C
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+FIELDS = ("rho", "rho_u", "rho_v", "rho_w", "E")
+
+
+def rel_errors(Q, R):
+    out = {}
+    for e, name in enumerate(FIELDS):
+        d = Q[..., e] - R[..., e]
+        ref = R[..., e]
+        # momentum fields of TGV are O(1) while rho, E carry large backgrounds:
+        # normalise by the field's own RMS (its L2 norm over the grid)
+        l2 = np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(ref * ref))
+        li = np.abs(d).max() / np.abs(ref).max()
+        out[name] = (l2, li)
+    return out
+
+
+@pytest.mark.parametrize("N,steps,dims", [(32, 50, (1, 1, 1)), (48, 30, (2, 1, 1))])
+def test_fp32_flux_tgv_error_vs_fp64_oracle(N, steps, dims):
+    b = O.Block(O.make_params(N))
+    b.init_tgv()
+    b.run(steps)
+    with C.Solver(C.make_params(N), dims, precision=C.CBNS_FP32_FLUX) as s:
+        s.init_tgv()
+        s.run(steps)
+        Q = s.get_state()
+        t = s.t_cur
+    # dt is a max over the (slightly different) state, so t drifts at the state error level
+    assert abs(t - b.t_cur) <= 1e-7 * b.t_cur
+    errs = rel_errors(Q, b.Q_interior(0))
+    for name, (l2, li) in errs.items():
+        print(f"FP32-flux TGV {N}^3 {steps} steps {name}: rel L2 {l2:.3e} rel Linf {li:.3e}")
+        assert l2 <= 1e-5, name
+        assert li <= 1e-4, name
+
+
+def test_fp32_flux_is_deterministic_and_decomposition_invariant():
+    with C.Solver(C.make_params(32), (1, 1, 1), precision=C.CBNS_FP32_FLUX) as a, \
+            C.Solver(C.make_params(32), (2, 2, 1), precision=C.CBNS_FP32_FLUX) as b:
+        a.init_tgv()
+        b.init_tgv()
+        a.run(5)
+        b.run(5)
+        assert np.array_equal(a.get_state(), b.get_state())
