@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/cbns3d_amd.h"
+#include "cbns_face.h"
 #include "cbns_kernels.h"
 
 using namespace cbns;
@@ -32,6 +33,7 @@ struct cbns_ctx {
   uint64_t bytes = 0;
   bool have_state = false;
   int nonfinite = 0;
+  RefState ref{1.0, 0.0, 0.0};  // FP32-flux reference state
   // live kernel timing
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -226,7 +228,7 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
   c->d_smax = (unsigned long long*)q;
   if (!st) st = dev_alloc(c, &q, 64);
   c->d_flag = (int*)q;
-  c->diag_parts = 256 * nb;
+  c->diag_parts = GSUM_WG;
   if (!st) st = dev_alloc(c, &q, sizeof(double) * (2 * c->diag_parts + 2));
   c->d_diag = (double*)q;
   if (!st && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
@@ -319,7 +321,7 @@ int cbns_calc_dt(cbns_ctx* c, double* dt) {
 
 int cbns_rec_riemann(cbns_ctx* c, int which, int dir) {
   if (!c || (which != CBNS_Q && which != CBNS_QP) || dir < 0 || dir > 2) return CBNS_ERR_ARG;
-  CK(launch_face_flux(c->db, c->hb.data(), c->dom.nb, dir, which, c->ph, c->stream));
+  CK(launch_face_flux(c->db, c->hb.data(), c->dom.nb, dir, which, c->ph, c->precision, c->ref, c->stream));
   return CBNS_OK;
 }
 
@@ -332,7 +334,7 @@ int cbns_calc_viscous_flux_contribution(cbns_ctx* c) {
 int cbns_update_rk3(cbns_ctx* c, double dt, int stage) {
   if (!c) return CBNS_ERR_ARG;
   if (stage < 1 || stage > 3) return CBNS_OK;  // the reference silently does nothing
-  CK(launch_update(c->db, c->hb.data(), c->dom.nb, dt, stage, c->ph, c->stream));
+  CK(launch_update(c->db, c->hb.data(), c->dom.nb, dt, stage, c->ph, c->precision, c->stream));
   return CBNS_OK;
 }
 
@@ -346,6 +348,18 @@ static int finish_setup(cbns_ctx* c) {
   if ((st = cbns_init_Q_p(c))) return st;
   CK(hipStreamSynchronize(c->stream));
   c->have_state = true;
+  if (c->precision == CBNS_FP32_FLUX) {
+    // reference state of the deviation form: interior means of rho and E
+    double* out = c->d_diag + 2 * c->diag_parts;
+    CK(launch_global_sums(c->db, c->dom, c->ph, 0, c->d_diag, out, c->stream));
+    double sums[2];
+    CK(hipMemcpyAsync(sums, out, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    const double n = (double)c->dom.N[0] * c->dom.N[1] * c->dom.N[2];
+    c->ref.rho = sums[0] / n;
+    c->ref.E = sums[1] / n;
+    c->ref.p = c->ph.gam1 * c->ref.E;
+  }
   return CBNS_OK;
 }
 
@@ -471,9 +485,9 @@ static int run_stages(cbns_ctx* c, double dt) {
     const int out = stage == 3 ? CBNS_Q : CBNS_QP;
     const BlockDesc* hb = c->hb.data();
     for (int d = 0; d < 3; d++)
-      CK(timed(c, d, [&] { return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->stream); }));
+      CK(timed(c, d, [&] { return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->precision, c->ref, c->stream); }));
     CK(timed(c, 3, [&] { return launch_viscous(c->db, hb, c->dom.nb, c->ph, c->stream); }));
-    CK(timed(c, 4, [&] { return launch_update(c->db, hb, c->dom.nb, dt, stage, c->ph, c->stream); }));
+    CK(timed(c, 4, [&] { return launch_update(c->db, hb, c->dom.nb, dt, stage, c->ph, c->precision, c->stream); }));
     CK(timed(c, 5, [&] { return launch_ghost_fill(c->db, hb, c->dom, out, c->stream); }));
     CK(timed(c, 6, [&] { return launch_primitive(c->db, hb, c->dom.nb, out, c->ph, c->stream); }));
   }
@@ -545,8 +559,7 @@ int cbns_kernel_times(cbns_ctx* c, double ms[CBNS_KCLASSES], int64_t n[CBNS_KCLA
 
 int cbns_diagnostics(cbns_ctx* c, double* ke, double* ens) {
   if (!c || !ke || !ens) return CBNS_ERR_ARG;
-  CK(launch_diagnostics(c->db, c->hb.data(), c->dom.nb, c->ph, c->d_diag, c->diag_parts,
-                        c->d_diag + 2 * c->diag_parts, c->stream));
+  CK(launch_global_sums(c->db, c->dom, c->ph, 1, c->d_diag, c->d_diag + 2 * c->diag_parts, c->stream));
   double out[2];
   CK(hipMemcpyAsync(out, c->d_diag + 2 * c->diag_parts, sizeof(out), hipMemcpyDeviceToHost, c->stream));
   CK(hipStreamSynchronize(c->stream));

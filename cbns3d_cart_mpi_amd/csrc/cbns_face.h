@@ -215,4 +215,173 @@ __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double 
   for (int e = 0; e < 5; e++) flux[e] = 0.5 * (fL[e] + fR[e] - diss[e]);
 }
 
+
+// ---------------------------------------------------------------------------
+// FP32-flux form.  The characteristic projections, WENO5 (smoothness
+// indicators, weights, candidates) and the LF flux run in FP32 on deviations
+// from a constant reference state Ug = (rho_g, 0, 0, 0, E_g) of the solve:
+//   * WENO5 is translation-equivariant and the projection is linear, so
+//     reconstructing U - Ug and adding Ug back is the same reconstruction;
+//   * the flux is returned minus F(Ug) = (0, p_g n, 0), a constant that
+//     cancels exactly in the flux difference of the update.
+// Large backgrounds (p ~ 1/(gamma M^2), E) therefore never meet FP32
+// rounding directly.  Roe averages are taken in FP32 except at
+// near-vacuum or strong-gradient faces, where they are taken in FP64.
+struct RefState {
+  double rho, E, p;  // rho_g, E_g, p_g = (gamma - 1) E_g
+};
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+__device__ __forceinline__ float weno5f(float Um2, float Um1, float U0, float Up1, float Up2) {
+  const float eps = 1.0e-6f;
+  float diff1 = Um2 - 2.0f * Um1 + U0;
+  float diff2 = Um2 - 4.0f * Um1 + 3.0f * U0;
+  float diff3 = Um1 - 2.0f * U0 + Up1;
+  float diff4 = Um1 - Up1;
+  float diff5 = U0 - 2.0f * Up1 + Up2;
+  float diff6 = 3.0f * U0 - 4.0f * Up1 + Up2;
+  float IS1 = (13.0f / 12.0f) * diff1 * diff1 + 0.25f * diff2 * diff2 + eps;
+  float IS2 = (13.0f / 12.0f) * diff3 * diff3 + 0.25f * diff4 * diff4 + eps;
+  float IS3 = (13.0f / 12.0f) * diff5 * diff5 + 0.25f * diff6 * diff6 + eps;
+  float w1 = 0.1f * frcp(IS1 * IS1);
+  float w2 = 0.6f * frcp(IS2 * IS2);
+  float w3 = 0.3f * frcp(IS3 * IS3);
+  float f1 = 2.0f * Um2 - 7.0f * Um1 + 11.0f * U0;
+  float f2 = -Um1 + 5.0f * U0 + 2.0f * Up1;
+  float f3 = 2.0f * U0 + 5.0f * Up1 - Up2;
+  return (w1 * f1 + w2 * f2 + w3 * f3) * frcp(6.0f * (w1 + w2 + w3));
+}
+
+template <int D>
+__device__ __forceinline__ void face_flux_f32(const double (&q)[6][5], const double (&pl)[5],
+                                              const double (&pr)[5], double gamma, double gam1,
+                                              const RefState& g, float (&flux)[5]) {
+  const float gf = (float)gamma, g1 = (float)gam1;
+  // ---- Roe average
+  float uu, vv, ww, H, c;
+  const double rr = pr[0] / pl[0];
+  const bool wide = !(rr > 0.5 && rr < 2.0) || pl[0] < 1.0e-3 * g.rho || pr[0] < 1.0e-3 * g.rho;
+  if (wide) {
+    double c_L = sqrt(gamma * pl[4] / pl[0]);
+    double H_L = c_L * c_L / gam1 + 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+    double c_R = sqrt(gamma * pr[4] / pr[0]);
+    double H_R = c_R * c_R / gam1 + 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+    double ratio = sqrt(rr);
+    double inv = 1.0 / (1.0 + ratio);
+    double u_ = (pl[1] + ratio * pr[1]) * inv, v_ = (pl[2] + ratio * pr[2]) * inv;
+    double w_ = (pl[3] + ratio * pr[3]) * inv, H_ = (H_L + ratio * H_R) * inv;
+    uu = (float)u_;
+    vv = (float)v_;
+    ww = (float)w_;
+    H = (float)H_;
+    c = (float)sqrt(gam1 * (H_ - 0.5 * (u_ * u_ + v_ * v_ + w_ * w_)));
+  } else {
+    const float rl = (float)pl[0], rrf = (float)pr[0];
+    const float ul = (float)pl[1], vl = (float)pl[2], wl = (float)pl[3], plf = (float)pl[4];
+    const float ur = (float)pr[1], vr = (float)pr[2], wr = (float)pr[3], prf = (float)pr[4];
+    const float HL = gf / g1 * plf * frcp(rl) + 0.5f * (ul * ul + vl * vl + wl * wl);
+    const float HR = gf / g1 * prf * frcp(rrf) + 0.5f * (ur * ur + vr * vr + wr * wr);
+    const float ratio = fsqrt((float)rr);
+    const float inv = frcp(1.0f + ratio);
+    uu = (ul + ratio * ur) * inv;
+    vv = (vl + ratio * vr) * inv;
+    ww = (wl + ratio * wr) * inv;
+    H = (HL + ratio * HR) * inv;
+    c = fsqrt(g1 * (H - 0.5f * (uu * uu + vv * vv + ww * ww)));
+  }
+  const float q2 = uu * uu + vv * vv + ww * ww;
+  const float qn = D == 0 ? uu : (D == 1 ? vv : ww);
+  const float ic = frcp(c);
+  const float K = 0.5f * g1 * ic * ic;
+  const float one_2c = 0.5f * ic;
+  const float L00 = 0.5f * K * q2 + one_2c * qn, L10 = 1.0f - K * q2, L20 = 0.5f * K * q2 - one_2c * qn;
+  const float Ku = K * uu, Kv = K * vv, Kw = K * ww;
+  const float L01 = D == 0 ? -(Ku + one_2c) : -Ku, L21 = D == 0 ? -(Ku - one_2c) : -Ku;
+  const float L02 = D == 1 ? -(Kv + one_2c) : -Kv, L22 = D == 1 ? -(Kv - one_2c) : -Kv;
+  const float L03 = D == 2 ? -(Kw + one_2c) : -Kw, L23 = D == 2 ? -(Kw - one_2c) : -Kw;
+  const float L11 = 2.0f * Ku, L12 = 2.0f * Kv, L13 = 2.0f * Kw, L14 = -2.0f * K;
+  const float Rm0[3] = {D == 0 ? uu - c : uu, D == 1 ? vv - c : vv, D == 2 ? ww - c : ww};
+  const float Rp2[3] = {D == 0 ? uu + c : uu, D == 1 ? vv + c : vv, D == 2 ? ww + c : ww};
+  const float R40 = H - qn * c, R41 = 0.5f * q2, R42 = H + qn * c;
+
+  // deviations from the reference state (FP64 difference, then FP32)
+  float d[6][5];
+#pragma unroll
+  for (int cc = 0; cc < 6; cc++) {
+    d[cc][0] = (float)(q[cc][0] - g.rho);
+    d[cc][1] = (float)q[cc][1];
+    d[cc][2] = (float)q[cc][2];
+    d[cc][3] = (float)q[cc][3];
+    d[cc][4] = (float)(q[cc][4] - g.E);
+  }
+  float aL[5], aR[5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    float fc[6];
+#pragma unroll
+    for (int cc = 0; cc < 6; cc++) {
+      const float f0 = d[cc][0], f1 = d[cc][1], f2 = d[cc][2], f3 = d[cc][3], f4 = d[cc][4];
+      if (m == 0) fc[cc] = L00 * f0 + L01 * f1 + L02 * f2 + L03 * f3 + K * f4;
+      else if (m == 1) fc[cc] = L10 * f0 + L11 * f1 + L12 * f2 + L13 * f3 + L14 * f4;
+      else if (m == 2) fc[cc] = L20 * f0 + L21 * f1 + L22 * f2 + L23 * f3 + K * f4;
+      else if (m == 3) fc[cc] = D == 0 ? f3 - ww * f0 : (D == 1 ? ww * f0 - f3 : f2 - vv * f0);
+      else fc[cc] = D == 0 ? vv * f0 - f2 : uu * f0 - f1;
+    }
+    const float wl = weno5f(fc[0], fc[1], fc[2], fc[3], fc[4]);
+    const float wr = weno5f(fc[5], fc[4], fc[3], fc[2], fc[1]);
+    if (m == 0) {
+      aL[0] = wl; aR[0] = wr;
+      aL[1] = Rm0[0] * wl; aR[1] = Rm0[0] * wr;
+      aL[2] = Rm0[1] * wl; aR[2] = Rm0[1] * wr;
+      aL[3] = Rm0[2] * wl; aR[3] = Rm0[2] * wr;
+      aL[4] = R40 * wl; aR[4] = R40 * wr;
+    } else if (m == 1) {
+      aL[0] += wl; aR[0] += wr;
+      aL[1] += uu * wl; aR[1] += uu * wr;
+      aL[2] += vv * wl; aR[2] += vv * wr;
+      aL[3] += ww * wl; aR[3] += ww * wr;
+      aL[4] += R41 * wl; aR[4] += R41 * wr;
+    } else if (m == 2) {
+      aL[0] += wl; aR[0] += wr;
+      aL[1] += Rp2[0] * wl; aR[1] += Rp2[0] * wr;
+      aL[2] += Rp2[1] * wl; aR[2] += Rp2[1] * wr;
+      aL[3] += Rp2[2] * wl; aR[3] += Rp2[2] * wr;
+      aL[4] += R42 * wl; aR[4] += R42 * wr;
+    } else if (m == 3) {
+      if (D == 0) { aL[3] += wl; aR[3] += wr; aL[4] += ww * wl; aR[4] += ww * wr; }
+      else if (D == 1) { aL[3] -= wl; aR[3] -= wr; aL[4] -= ww * wl; aR[4] -= ww * wr; }
+      else { aL[2] += wl; aR[2] += wr; aL[4] += vv * wl; aR[4] += vv * wr; }
+    } else {
+      if (D == 0) { aL[2] -= wl; aR[2] -= wr; aL[4] -= vv * wl; aR[4] -= vv * wr; }
+      else { aL[1] -= wl; aR[1] -= wr; aL[4] -= uu * wl; aR[4] -= uu * wr; }
+    }
+  }
+  // ---- LF flux minus F(Ug), from the reconstructed deviations
+  const float rho_g = (float)g.rho, p_g = (float)g.p, Hg = (float)(g.E + g.p);
+  float F[2][5], un[2], cs[2];
+  const float* a[2] = {aL, aR};
+#pragma unroll
+  for (int s = 0; s < 2; s++) {
+    const float rho = rho_g + a[s][0];
+    const float ir = frcp(rho);
+    const float u = a[s][1] * ir, v = a[s][2] * ir, w = a[s][3] * ir;
+    const float dp = g1 * (a[s][4] - 0.5f * (a[s][1] * u + a[s][2] * v + a[s][3] * w));
+    const float vn = D == 0 ? u : (D == 1 ? v : w);
+    cs[s] = fsqrt(gf * (p_g + dp) * ir);
+    un[s] = vn;
+    F[s][0] = a[s][1 + D];
+    F[s][1] = a[s][1] * vn + (D == 0 ? dp : 0.0f);
+    F[s][2] = a[s][2] * vn + (D == 1 ? dp : 0.0f);
+    F[s][3] = a[s][3] * vn + (D == 2 ? dp : 0.0f);
+    F[s][4] = (Hg + a[s][4] + dp) * vn;
+  }
+  float ws = fabsf(un[1]) + cs[1];
+  const float wsL = fabsf(un[0]) + cs[0];
+  if (wsL > ws) ws = wsL;
+#pragma unroll
+  for (int e = 0; e < 5; e++) flux[e] = 0.5f * (F[0][e] + F[1][e] - ws * (aR[e] - aL[e]));
+}
+
 }  // namespace cbns

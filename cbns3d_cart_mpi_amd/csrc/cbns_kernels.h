@@ -55,15 +55,16 @@ struct Launch {
 };
 
 // kernel launchers (cbns_kernels.hip)
+struct RefState;
 hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
-                            const Phys& ph, hipStream_t s);
+                            const Phys& ph, int precision, const RefState& g, hipStream_t s);
 hipError_t launch_primitive(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
                             hipStream_t s);
 hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
                              hipStream_t s);
 hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, hipStream_t s);
 hipError_t launch_update(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
-                         const Phys& ph, hipStream_t s);
+                         const Phys& ph, int precision, hipStream_t s);
 hipError_t launch_dt_reduce(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph,
                             unsigned long long* d_smax, int* d_flag, hipStream_t s);
 hipError_t launch_copy_state(const BlockDesc* db, const BlockDesc* hb, int nb, int from, int to, hipStream_t s);
@@ -75,8 +76,9 @@ hipError_t launch_scatter_aos(const BlockDesc* db, const BlockDesc* hb, const Do
 hipError_t launch_gather_aos(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
                              double* aos, hipStream_t s);
 hipError_t launch_fill_mu(const BlockDesc* db, const BlockDesc* hb, int nb, double mu0, hipStream_t s);
-hipError_t launch_diagnostics(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, double* d_part,
-                              int nparts, double* d_out, hipStream_t s);
+constexpr int GSUM_WG = 512;  // workgroups of the fixed-order global sums
+hipError_t launch_global_sums(const BlockDesc* db, const Domain& dom, const Phys& ph, int what, double* d_part,
+                              double* d_out, hipStream_t s);
 hipError_t launch_face_eval(int dir, double gamma, int nfaces, const double* cells, const double* prim,
                             double* flux, hipStream_t s);
 

@@ -56,17 +56,59 @@ __global__ __launch_bounds__(TPB) void k_face_flux(const BlockDesc* __restrict__
   for (int e = 0; e < 5; e++) F[e * sf + t] = fl[e];
 }
 
+// FP32-flux form: same stencil and mapping, FP32 face arrays holding F - F(Ug).
+template <int D>
+__global__ __launch_bounds__(TPB) void k_face_flux_f32(const BlockDesc* __restrict__ blocks, int which,
+                                                       double gamma, double gam1, RefState g) {
+  const BlockDesc& B = blocks[blockIdx.y];
+  const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= ni * nj * nk) return;
+  const int i = t % ni;
+  const int r = t / ni;
+  const int j = r % nj;
+  const int k = r / nj;
+  const int c0 = gidx(B, i, j, k);
+  const int sD = D == 0 ? 1 : (D == 1 ? B.g[0] : B.g[0] * B.g[1]);
+  const double* __restrict__ Q = B.Q[which];
+  const long long sg = B.sg;
+  double q[6][5];
+#pragma unroll
+  for (int cc = 0; cc < 6; cc++)
+#pragma unroll
+    for (int e = 0; e < 5; e++) q[cc][e] = Q[e * sg + c0 + (cc - 3) * sD];
+  const double* __restrict__ P = B.prim;
+  double pl[5], pr[5];
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    pl[f] = P[f * sg + c0 - sD];
+    pr[f] = P[f * sg + c0];
+  }
+  float fl[5];
+  face_flux_f32<D>(q, pl, pr, gamma, gam1, g, fl);
+  float* __restrict__ F = (float*)B.F[D];
+  const long long sf = B.sf[D];
+#pragma unroll
+  for (int e = 0; e < 5; e++) F[e * sf + t] = fl[e];
+}
+
 hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
-                            const Phys& ph, hipStream_t s) {
+                            const Phys& ph, int precision, const RefState& g, hipStream_t s) {
   long long mx = 0;
   for (int b = 0; b < nb; b++) {
     long long c = (long long)(hb[b].n[0] + (dir == 0)) * (hb[b].n[1] + (dir == 1)) * (hb[b].n[2] + (dir == 2));
     mx = c > mx ? c : mx;
   }
   dim3 grid(nblocks_for(mx), nb);
-  if (dir == 0) hipLaunchKernelGGL(k_face_flux<0>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1);
-  else if (dir == 1) hipLaunchKernelGGL(k_face_flux<1>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1);
-  else hipLaunchKernelGGL(k_face_flux<2>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1);
+  if (precision == 0) {
+    if (dir == 0) hipLaunchKernelGGL(k_face_flux<0>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1);
+    else if (dir == 1) hipLaunchKernelGGL(k_face_flux<1>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1);
+    else hipLaunchKernelGGL(k_face_flux<2>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1);
+  } else {
+    if (dir == 0) hipLaunchKernelGGL(k_face_flux_f32<0>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1, g);
+    else if (dir == 1) hipLaunchKernelGGL(k_face_flux_f32<1>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1, g);
+    else hipLaunchKernelGGL(k_face_flux_f32<2>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1, g);
+  }
   return hipGetLastError();
 }
 
@@ -251,6 +293,7 @@ hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, cons
 
 // ---------------------------------------------------------------------------
 // update_rk3 (src/Block3d.cpp:920-962) on the interior.
+template <typename FT>
 __global__ __launch_bounds__(TPB) void k_update(const BlockDesc* __restrict__ blocks, double dt, int stage,
                                                 Phys ph) {
   const BlockDesc& B = blocks[blockIdx.y];
@@ -270,12 +313,12 @@ __global__ __launch_bounds__(TPB) void k_update(const BlockDesc* __restrict__ bl
   const long long sg = B.sg;
 #pragma unroll
   for (int e = 0; e < NEQ; e++) {
-    const double* __restrict__ Ep = B.F[0] + e * B.sf[0];
-    const double* __restrict__ Fp = B.F[1] + e * B.sf[1];
-    const double* __restrict__ Gp = B.F[2] + e * B.sf[2];
-    double df = (Ep[iE + 1] - Ep[iE]) / ph.dx;
-    df += (Fp[iF + n0] - Fp[iF]) / ph.dy;
-    df += (Gp[iG + n0 * n1] - Gp[iG]) / ph.dz;
+    const FT* __restrict__ Ep = (const FT*)B.F[0] + e * B.sf[0];
+    const FT* __restrict__ Fp = (const FT*)B.F[1] + e * B.sf[1];
+    const FT* __restrict__ Gp = (const FT*)B.F[2] + e * B.sf[2];
+    double df = ((double)Ep[iE + 1] - (double)Ep[iE]) / ph.dx;
+    df += ((double)Fp[iF + n0] - (double)Fp[iF]) / ph.dy;
+    df += ((double)Gp[iG + n0 * n1] - (double)Gp[iG]) / ph.dz;
     if (e > 0) df -= B.dfv[(e - 1) * B.sn + t];
     const long long q = e * sg + c;
     if (1 == stage) {
@@ -289,12 +332,17 @@ __global__ __launch_bounds__(TPB) void k_update(const BlockDesc* __restrict__ bl
 }
 
 hipError_t launch_update(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
-                         const Phys& ph, hipStream_t s) {
+                         const Phys& ph, int precision, hipStream_t s) {
   long long mn = 0;
   for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
-  hipLaunchKernelGGL(k_update, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, dt, stage, ph);
+  if (precision == 0)
+    hipLaunchKernelGGL(k_update<double>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, dt, stage, ph);
+  else
+    hipLaunchKernelGGL(k_update<float>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, dt, stage, ph);
   return hipGetLastError();
 }
+
+
 
 // ---------------------------------------------------------------------------
 // calc_dt spectral radius (src/Block3d.cpp:515-553), max over every block.
@@ -478,44 +526,65 @@ hipError_t launch_fill_mu(const BlockDesc* db, const BlockDesc* hb, int nb, doub
 }
 
 // ---------------------------------------------------------------------------
-// Diagnostics: per-workgroup partial sums of 0.5 rho |u|^2 and 0.5 rho |w|^2
-// (vorticity by the 4th-order central stencil), then one fixed-order sum.
+// Global sums over the periodic interior in a fixed global cell order, so the
+// result is bitwise independent of the block decomposition: thread t of a
+// fixed grid of GSUM_WG workgroups visits global cells t, t + GSUM_WG*TPB, ...
+// (each read from the block that owns it), then fixed-order wave, workgroup
+// and final sums.
+//   what = 0: (sum rho, sum E)           -- FP32-flux reference state
+//   what = 1: (sum 0.5 rho |u|^2, sum 0.5 rho |omega|^2) with omega by the
+//             4th-order central stencil   -- diagnostics
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
 
-__global__ __launch_bounds__(TPB) void k_diag(const BlockDesc* __restrict__ blocks, Phys ph, double* part) {
-  const BlockDesc& B = blocks[blockIdx.y];
-  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
-  const int total = n0 * n1 * n2;
-  const long long sg = B.sg;
-  const int sy = B.g[0], sz = B.g[0] * B.g[1];
-  const double* u = B.prim + sg;
-  const double* v = B.prim + 2 * sg;
-  const double* w = B.prim + 3 * sg;
-  double ke = 0.0, en = 0.0;
-  for (int t = blockIdx.x * TPB + threadIdx.x; t < total; t += gridDim.x * TPB) {
-    const int i = t % n0;
-    const int r = t / n0;
-    const int j = r % n1;
-    const int k = r / n1;
-    const int c = gidx(B, i, j, k);
-    const double rr = B.prim[c], uu = u[c], vv = v[c], ww = w[c];
-    ke += 0.5 * rr * (uu * uu + vv * vv + ww * ww);
-    const double ox = grad4(w, c, sy, ph.dy) - grad4(v, c, sz, ph.dz);
-    const double oy = grad4(u, c, sz, ph.dz) - grad4(w, c, 1, ph.dx);
-    const double oz = grad4(v, c, 1, ph.dx) - grad4(u, c, sy, ph.dy);
-    en += 0.5 * rr * (ox * ox + oy * oy + oz * oz);
+template <int WHAT>
+__global__ __launch_bounds__(TPB) void k_global_sums(const BlockDesc* __restrict__ blocks, Domain dom, Phys ph,
+                                                     double* part) {
+  const long long total = (long long)dom.N[0] * dom.N[1] * dom.N[2];
+  double sa = 0.0, sb = 0.0;
+  for (long long t = blockIdx.x * (long long)TPB + threadIdx.x; t < total; t += (long long)gridDim.x * TPB) {
+    const int gi = (int)(t % dom.N[0]);
+    const long long r = t / dom.N[0];
+    const int gj = (int)(r % dom.N[1]);
+    const int gk = (int)(r / dom.N[1]);
+    const int g3[3] = {gi, gj, gk};
+    int bc[3], lc[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      int bb = g3[a] / dom.base[a];
+      bb = bb > dom.dims[a] - 1 ? dom.dims[a] - 1 : bb;
+      bc[a] = bb;
+      lc[a] = g3[a] - bb * dom.base[a];
+    }
+    const BlockDesc& B = blocks[(bc[0] * dom.dims[1] + bc[1]) * dom.dims[2] + bc[2]];
+    const int c = gidx(B, lc[0], lc[1], lc[2]);
+    if (WHAT == 0) {
+      sa += B.Q[0][c];
+      sb += B.Q[0][4 * B.sg + c];
+    } else {
+      const long long sg = B.sg;
+      const int sy = B.g[0], sz = B.g[0] * B.g[1];
+      const double* u = B.prim + sg;
+      const double* v = B.prim + 2 * sg;
+      const double* w = B.prim + 3 * sg;
+      const double rr = B.prim[c], uu = u[c], vv = v[c], ww = w[c];
+      sa += 0.5 * rr * (uu * uu + vv * vv + ww * ww);
+      const double ox = grad4(w, c, sy, ph.dy) - grad4(v, c, sz, ph.dz);
+      const double oy = grad4(u, c, sz, ph.dz) - grad4(w, c, 1, ph.dx);
+      const double oz = grad4(v, c, 1, ph.dx) - grad4(u, c, sy, ph.dy);
+      sb += 0.5 * rr * (ox * ox + oy * oy + oz * oz);
+    }
   }
-  ke = wave_sum(ke);
-  en = wave_sum(en);
+  sa = wave_sum(sa);
+  sb = wave_sum(sb);
   __shared__ double red[2][TPB / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) {
-    red[0][wv] = ke;
-    red[1][wv] = en;
+    red[0][wv] = sa;
+    red[1][wv] = sb;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -524,13 +593,12 @@ __global__ __launch_bounds__(TPB) void k_diag(const BlockDesc* __restrict__ bloc
       a += red[0][x];
       b += red[1][x];
     }
-    const int p = blockIdx.y * gridDim.x + blockIdx.x;
-    part[2 * p] = a;
-    part[2 * p + 1] = b;
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
   }
 }
 
-__global__ void k_diag_final(const double* part, int nparts, double* out) {
+__global__ void k_sums_final(const double* part, int nparts, double* out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     double a = 0.0, b = 0.0;
     for (int p = 0; p < nparts; p++) {
@@ -542,14 +610,13 @@ __global__ void k_diag_final(const double* part, int nparts, double* out) {
   }
 }
 
-hipError_t launch_diagnostics(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, double* d_part,
-                              int nparts, double* d_out, hipStream_t s) {
-  (void)hb;
-  const int gx = nparts / nb;
-  hipLaunchKernelGGL(k_diag, dim3(gx, nb), dim3(TPB), 0, s, db, ph, d_part);
+hipError_t launch_global_sums(const BlockDesc* db, const Domain& dom, const Phys& ph, int what, double* d_part,
+                              double* d_out, hipStream_t s) {
+  if (what == 0) hipLaunchKernelGGL(k_global_sums<0>, dim3(GSUM_WG), dim3(TPB), 0, s, db, dom, ph, d_part);
+  else hipLaunchKernelGGL(k_global_sums<1>, dim3(GSUM_WG), dim3(TPB), 0, s, db, dom, ph, d_part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s, d_part, gx * nb, d_out);
+  hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(64), 0, s, d_part, GSUM_WG, d_out);
   return hipGetLastError();
 }
 
