@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--dims", default="1,1,1")
+    ap.add_argument("--band", default="auto", help="rows per work band, or 'auto' to tune")
+    ap.add_argument("--overlap", default="auto", choices=["auto", "on", "off"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-max-steps", type=int, default=3)
     ap.add_argument("--cpu-budget-s", type=float, default=25.0)
@@ -95,6 +97,17 @@ def main():
     s = C.Solver(p, dims)
     s.init_tgv()
     s.run(a.warmup)
+    tuned = None
+    if a.band == "auto" or a.overlap == "auto":
+        from cbns3d_cart_mpi_amd import autotune
+        bands = autotune.BANDS if a.band == "auto" else (int(a.band),)
+        ovs = (True, False) if a.overlap == "auto" else (a.overlap == "on",)
+        best, tms = autotune.tune(s, steps=2, bands=bands, overlaps=ovs)
+        tuned = {"band": best[0], "overlap": best[1],
+                 "ms_per_step": {f"{b}/{'on' if o else 'off'}": round(v * 1e3, 3) for (b, o), v in tms.items()}}
+    else:
+        s.set_band(int(a.band))
+        s.set_overlap(a.overlap == "on")
     s.sync()
     s.set_profiling(True)
     t0 = time.perf_counter()
@@ -139,6 +152,7 @@ def main():
         "kernels_avg_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items()},
         "kernel_share": {k: round(v[0] / (el * 1e3), 4) for k, v in kt.items()},
         "device_bytes": mem,
+        "autotune": tuned,
         "ke_enstrophy_end": [ke, en],
     }
     if not a.no_cpu_baseline:

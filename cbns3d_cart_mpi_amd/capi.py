@@ -71,6 +71,8 @@ _SIGS = {
     "cbns_sync": (_I, [_VP]),
     "cbns_face_flux_eval": (_I, [_I, _I, _D, _I, _PD, _PD, _PD]),
     "cbns_set_profiling": (_I, [_VP, _I]),
+    "cbns_set_overlap": (_I, [_VP, _I]),
+    "cbns_set_band": (_I, [_VP, _I]),
     "cbns_kernel_times": (_I, [_VP, _PD, ctypes.POINTER(ctypes.c_int64)]),
     "cbns_strerror": (ctypes.c_char_p, [_I]),
 }
@@ -255,6 +257,12 @@ class Solver:
         check(lib().cbns_sync(self._h), "cbns_sync")
 
     KERNEL_CLASSES = ("face_x", "face_y", "face_z", "viscous", "update", "ghost_fill", "primitive", "dt")
+
+    def set_band(self, rows):
+        check(lib().cbns_set_band(self._h, int(rows)), "cbns_set_band")
+
+    def set_overlap(self, on):
+        check(lib().cbns_set_overlap(self._h, int(bool(on))), "cbns_set_overlap")
 
     def set_profiling(self, on):
         check(lib().cbns_set_profiling(self._h, int(bool(on))), "cbns_set_profiling")

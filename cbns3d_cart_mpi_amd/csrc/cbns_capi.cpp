@@ -25,6 +25,11 @@ struct cbns_ctx {
   std::vector<BlockDesc> hb;
   BlockDesc* db = nullptr;
   hipStream_t stream = nullptr;
+  // second stream: the viscous kernels of a stage run beside its face kernels
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool overlap = true;
+  int jband = 16;  // rows per band of the stencil kernels' work order
   unsigned long long* d_smax = nullptr;
   int* d_flag = nullptr;
   double* d_diag = nullptr;  // partials then the two sums
@@ -67,17 +72,21 @@ static void harvest(cbns_ctx* c) {
   c->pending.clear();
 }
 
-// Launch `body` bracketed by events of class k when timing is on.
+// Launch `body` on stream s bracketed by events of class k when timing is on.
 template <class F>
-static hipError_t timed(cbns_ctx* c, int k, F body) {
+static hipError_t timed_on(cbns_ctx* c, hipStream_t s, int k, F body) {
   if (!c->prof) return body();
   hipEvent_t a = take_event(c), b = take_event(c);
   if (!a || !b) return hipErrorOutOfMemory;
-  hipError_t e = hipEventRecord(a, c->stream);
+  hipError_t e = hipEventRecord(a, s);
   if (e == hipSuccess) e = body();
-  if (e == hipSuccess) e = hipEventRecord(b, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(b, s);
   c->pending.push_back({k, {a, b}});
   return e;
+}
+template <class F>
+static hipError_t timed(cbns_ctx* c, int k, F body) {
+  return timed_on(c, c->stream, k, body);
 }
 
 #define CK(call)                                   \
@@ -232,6 +241,9 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
   if (!st) st = dev_alloc(c, &q, sizeof(double) * (2 * c->diag_parts + 2));
   c->d_diag = (double*)q;
   if (!st && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
+  if (!st && hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
+  if (!st && hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) st = CBNS_ERR_HIP;
+  if (!st && hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && hipMemcpy(c->db, c->hb.data(), sizeof(BlockDesc) * nb, hipMemcpyHostToDevice) != hipSuccess)
     st = CBNS_ERR_HIP;
   if (st) {
@@ -247,7 +259,11 @@ int cbns_destroy(cbns_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   harvest(c);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->side) (void)hipStreamSynchronize(c->side);
   for (void* p : c->allocs) (void)hipFree(p);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CBNS_OK;
@@ -321,20 +337,20 @@ int cbns_calc_dt(cbns_ctx* c, double* dt) {
 
 int cbns_rec_riemann(cbns_ctx* c, int which, int dir) {
   if (!c || (which != CBNS_Q && which != CBNS_QP) || dir < 0 || dir > 2) return CBNS_ERR_ARG;
-  CK(launch_face_flux(c->db, c->hb.data(), c->dom.nb, dir, which, c->ph, c->precision, c->ref, c->stream));
+  CK(launch_face_flux(c->db, c->hb.data(), c->dom.nb, dir, which, c->ph, c->precision, c->ref, c->jband, c->stream));
   return CBNS_OK;
 }
 
 int cbns_calc_viscous_flux_contribution(cbns_ctx* c) {
   if (!c) return CBNS_ERR_ARG;
-  CK(launch_viscous(c->db, c->hb.data(), c->dom.nb, c->ph, c->stream));
+  CK(launch_viscous(c->db, c->hb.data(), c->dom.nb, c->ph, c->jband, c->stream));
   return CBNS_OK;
 }
 
 int cbns_update_rk3(cbns_ctx* c, double dt, int stage) {
   if (!c) return CBNS_ERR_ARG;
   if (stage < 1 || stage > 3) return CBNS_OK;  // the reference silently does nothing
-  CK(launch_update(c->db, c->hb.data(), c->dom.nb, dt, stage, c->ph, c->precision, c->stream));
+  CK(launch_update(c->db, c->hb.data(), c->dom.nb, dt, stage, c->ph, c->precision, c->jband, c->stream));
   return CBNS_OK;
 }
 
@@ -484,10 +500,19 @@ static int run_stages(cbns_ctx* c, double dt) {
     const int in = stage == 1 ? CBNS_Q : CBNS_QP;
     const int out = stage == 3 ? CBNS_Q : CBNS_QP;
     const BlockDesc* hb = c->hb.data();
+    if (c->overlap) {
+      // the viscous terms need only the stage input's primitives: run them on
+      // the side stream while the (compute-bound) face fluxes run here
+      CK(hipEventRecord(c->ev_fork, c->stream));
+      CK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+      CK(timed_on(c, c->side, 3, [&] { return launch_viscous(c->db, hb, c->dom.nb, c->ph, c->jband, c->side); }));
+      CK(hipEventRecord(c->ev_join, c->side));
+    }
     for (int d = 0; d < 3; d++)
-      CK(timed(c, d, [&] { return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->precision, c->ref, c->stream); }));
-    CK(timed(c, 3, [&] { return launch_viscous(c->db, hb, c->dom.nb, c->ph, c->stream); }));
-    CK(timed(c, 4, [&] { return launch_update(c->db, hb, c->dom.nb, dt, stage, c->ph, c->precision, c->stream); }));
+      CK(timed(c, d, [&] { return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->precision, c->ref, c->jband, c->stream); }));
+    if (c->overlap) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    else CK(timed(c, 3, [&] { return launch_viscous(c->db, hb, c->dom.nb, c->ph, c->jband, c->stream); }));
+    CK(timed(c, 4, [&] { return launch_update(c->db, hb, c->dom.nb, dt, stage, c->ph, c->precision, c->jband, c->stream); }));
     CK(timed(c, 5, [&] { return launch_ghost_fill(c->db, hb, c->dom, out, c->stream); }));
     CK(timed(c, 6, [&] { return launch_primitive(c->db, hb, c->dom.nb, out, c->ph, c->stream); }));
   }
@@ -531,6 +556,19 @@ int cbns_run(cbns_ctx* c, int nsteps, double* dts, int* steps_done) {
   if (steps_done) *steps_done = n;
   CK(hipStreamSynchronize(c->stream));
   harvest(c);
+  return CBNS_OK;
+}
+
+int cbns_set_band(cbns_ctx* c, int rows) {
+  if (!c || rows < 1) return CBNS_ERR_ARG;
+  c->jband = rows;
+  return CBNS_OK;
+}
+
+int cbns_set_overlap(cbns_ctx* c, int on) {
+  if (!c) return CBNS_ERR_ARG;
+  CK(hipStreamSynchronize(c->stream));
+  c->overlap = on != 0;
   return CBNS_OK;
 }
 

@@ -30,6 +30,20 @@
 
 namespace cbns {
 
+// x / 6.0, correctly rounded, in three operations.  y = RN(1/6) has relative
+// error 2^-54, so q0 = RN(x y) is within one ulp of x/6; the remainder
+// r = x - 6 q0 is then exact in one FMA, and RN(q0 + r y) is the correctly
+// rounded quotient (Markstein's correction theorem).  x/6 is never a
+// rounding tie (that would need a 55-bit x).  Differs from IEEE division only
+// in the sign of a zero result and for infinite x, neither of which reaches
+// it here.
+__device__ __forceinline__ double div6(double x) {
+  const double y = 1.0 / 6.0;
+  const double q0 = x * y;
+  const double r = __builtin_fma(-q0, 6.0, x);
+  return __builtin_fma(r, y, q0);
+}
+
 // src/weno.cpp:25-56
 __device__ __forceinline__ double weno5(double Um2, double Um1, double U0, double Up1, double Up2) {
   const double eps = 1.0e-6;
@@ -46,9 +60,9 @@ __device__ __forceinline__ double weno5(double Um2, double Um1, double U0, doubl
   double w2 = 0.6 / (IS2 * IS2);
   double w3 = 0.3 / (IS3 * IS3);
   double sum = w1 + w2 + w3;
-  double f1 = (2.0 * Um2 - 7.0 * Um1 + 11.0 * U0) / 6.0;
-  double f2 = (-Um1 + 5.0 * U0 + 2.0 * Up1) / 6.0;
-  double f3 = (2.0 * U0 + 5.0 * Up1 - Up2) / 6.0;
+  double f1 = div6(2.0 * Um2 - 7.0 * Um1 + 11.0 * U0);
+  double f2 = div6(-Um1 + 5.0 * U0 + 2.0 * Up1);
+  double f3 = div6(2.0 * U0 + 5.0 * Up1 - Up2);
   return (w1 * f1 + w2 * f2 + w3 * f3) / sum;
 }
 

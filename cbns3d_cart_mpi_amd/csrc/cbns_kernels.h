@@ -57,14 +57,14 @@ struct Launch {
 // kernel launchers (cbns_kernels.hip)
 struct RefState;
 hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
-                            const Phys& ph, int precision, const RefState& g, hipStream_t s);
+                            const Phys& ph, int precision, const RefState& g, int jb, hipStream_t s);
 hipError_t launch_primitive(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
                             hipStream_t s);
 hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
                              hipStream_t s);
-hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, hipStream_t s);
+hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int jb, hipStream_t s);
 hipError_t launch_update(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
-                         const Phys& ph, int precision, hipStream_t s);
+                         const Phys& ph, int precision, int jb, hipStream_t s);
 hipError_t launch_dt_reduce(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph,
                             unsigned long long* d_smax, int* d_flag, hipStream_t s);
 hipError_t launch_copy_state(const BlockDesc* db, const BlockDesc* hb, int nb, int from, int to, hipStream_t s);

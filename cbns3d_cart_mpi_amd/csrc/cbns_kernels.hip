@@ -18,20 +18,61 @@ __device__ __forceinline__ int gidx(const BlockDesc& B, int i, int j, int k) {
   return (i + NG) + B.g[0] * ((j + NG) + B.g[1] * (k + NG));
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Stencil kernels want neighbouring rows on the same L2, so the linear
+// workgroup id is remapped so that XCD r runs one contiguous range of logical
+// tiles (a bijection on [0, total) for any total).
+constexpr int NXCD = 8;
+struct Tile {
+  int x, y;  // logical (blockIdx.x, blockIdx.y)
+};
+__device__ __forceinline__ Tile xcd_tile() {
+  const int gx = gridDim.x;
+  const int total = gx * gridDim.y;
+  const int lin = blockIdx.y * gx + blockIdx.x;
+  const int base = total / NXCD, rem = total % NXCD;
+  const int r = lin % NXCD, q = lin / NXCD;
+  const int logical = r * base + (r < rem ? r : rem) + q;
+  return {logical % gx, logical / gx};
+}
+
+// Work index t -> (i, j, k) of an ni x nj x nk box in band order
+// (j-band of jb rows, k, j within the band, i): a contiguous run of t covers
+// a band of rows over successive planes, so with xcd_tile each XCD keeps the
+// y-neighbour rows (same band) and the z-neighbour rows (same rows, nearby
+// planes) of its stencils in its own L2.
+__device__ __forceinline__ void band_decode(int t, int ni, int nj, int nk, int jb, int& i, int& j, int& k) {
+  i = t % ni;
+  const int R = t / ni;
+  const int full = nj / jb;
+  const int Rfull = full * nk * jb;
+  if (R < Rfull) {
+    const int b = R / (nk * jb);
+    const int rem = R - b * nk * jb;
+    k = rem / jb;
+    j = b * jb + (rem - k * jb);
+  } else {
+    const int last = nj - full * jb;
+    const int rem = R - Rfull;
+    k = rem / last;
+    j = full * jb + (rem - k * last);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Inviscid face fluxes: rec_riemann_x/y/z (src/Block3d.cpp:555-748).
 // One thread per face; face (i,j,k) along D sits between cells i-1 and i.
 template <int D>
-__global__ __launch_bounds__(TPB) void k_face_flux(const BlockDesc* __restrict__ blocks, int which,
+__global__ __launch_bounds__(TPB) void k_face_flux(const BlockDesc* __restrict__ blocks, int which, int jb,
                                                    double gamma, double gam1) {
-  const BlockDesc& B = blocks[blockIdx.y];
+  const Tile tl = xcd_tile();
+  const BlockDesc& B = blocks[tl.y];
   const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
-  const int t = blockIdx.x * TPB + threadIdx.x;
+  const int t = tl.x * TPB + threadIdx.x;
   if (t >= ni * nj * nk) return;
-  const int i = t % ni;
-  const int r = t / ni;
-  const int j = r % nj;
-  const int k = r / nj;
+  int i, j, k;
+  band_decode(t, ni, nj, nk, jb, i, j, k);
+  const int fi = i + ni * (j + nj * k);
   const int c0 = gidx(B, i, j, k);
   const int sD = D == 0 ? 1 : (D == 1 ? B.g[0] : B.g[0] * B.g[1]);
   const double* __restrict__ Q = B.Q[which];
@@ -53,21 +94,21 @@ __global__ __launch_bounds__(TPB) void k_face_flux(const BlockDesc* __restrict__
   double* __restrict__ F = B.F[D];
   const long long sf = B.sf[D];
 #pragma unroll
-  for (int e = 0; e < 5; e++) F[e * sf + t] = fl[e];
+  for (int e = 0; e < 5; e++) F[e * sf + fi] = fl[e];
 }
 
 // FP32-flux form: same stencil and mapping, FP32 face arrays holding F - F(Ug).
 template <int D>
-__global__ __launch_bounds__(TPB) void k_face_flux_f32(const BlockDesc* __restrict__ blocks, int which,
+__global__ __launch_bounds__(TPB) void k_face_flux_f32(const BlockDesc* __restrict__ blocks, int which, int jb,
                                                        double gamma, double gam1, RefState g) {
-  const BlockDesc& B = blocks[blockIdx.y];
+  const Tile tl = xcd_tile();
+  const BlockDesc& B = blocks[tl.y];
   const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
-  const int t = blockIdx.x * TPB + threadIdx.x;
+  const int t = tl.x * TPB + threadIdx.x;
   if (t >= ni * nj * nk) return;
-  const int i = t % ni;
-  const int r = t / ni;
-  const int j = r % nj;
-  const int k = r / nj;
+  int i, j, k;
+  band_decode(t, ni, nj, nk, jb, i, j, k);
+  const int fi = i + ni * (j + nj * k);
   const int c0 = gidx(B, i, j, k);
   const int sD = D == 0 ? 1 : (D == 1 ? B.g[0] : B.g[0] * B.g[1]);
   const double* __restrict__ Q = B.Q[which];
@@ -89,11 +130,11 @@ __global__ __launch_bounds__(TPB) void k_face_flux_f32(const BlockDesc* __restri
   float* __restrict__ F = (float*)B.F[D];
   const long long sf = B.sf[D];
 #pragma unroll
-  for (int e = 0; e < 5; e++) F[e * sf + t] = fl[e];
+  for (int e = 0; e < 5; e++) F[e * sf + fi] = fl[e];
 }
 
 hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
-                            const Phys& ph, int precision, const RefState& g, hipStream_t s) {
+                            const Phys& ph, int precision, const RefState& g, int jb, hipStream_t s) {
   long long mx = 0;
   for (int b = 0; b < nb; b++) {
     long long c = (long long)(hb[b].n[0] + (dir == 0)) * (hb[b].n[1] + (dir == 1)) * (hb[b].n[2] + (dir == 2));
@@ -101,13 +142,13 @@ hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, in
   }
   dim3 grid(nblocks_for(mx), nb);
   if (precision == 0) {
-    if (dir == 0) hipLaunchKernelGGL(k_face_flux<0>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1);
-    else if (dir == 1) hipLaunchKernelGGL(k_face_flux<1>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1);
-    else hipLaunchKernelGGL(k_face_flux<2>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1);
+    if (dir == 0) hipLaunchKernelGGL(k_face_flux<0>, grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1);
+    else if (dir == 1) hipLaunchKernelGGL(k_face_flux<1>, grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1);
+    else hipLaunchKernelGGL(k_face_flux<2>, grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1);
   } else {
-    if (dir == 0) hipLaunchKernelGGL(k_face_flux_f32<0>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1, g);
-    else if (dir == 1) hipLaunchKernelGGL(k_face_flux_f32<1>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1, g);
-    else hipLaunchKernelGGL(k_face_flux_f32<2>, grid, dim3(TPB), 0, s, db, which, ph.gamma, ph.gam1, g);
+    if (dir == 0) hipLaunchKernelGGL(k_face_flux_f32<0>, grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1, g);
+    else if (dir == 1) hipLaunchKernelGGL(k_face_flux_f32<1>, grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1, g);
+    else hipLaunchKernelGGL(k_face_flux_f32<2>, grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1, g);
   }
   return hipGetLastError();
 }
@@ -116,9 +157,10 @@ hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, in
 // calc_primitive over the whole ghosted box (src/Block3d.cpp:359-385).
 __global__ __launch_bounds__(TPB) void k_primitive(const BlockDesc* __restrict__ blocks, int which, double gam1,
                                                    double gM2) {
-  const BlockDesc& B = blocks[blockIdx.y];
+  const Tile tl = xcd_tile();
+  const BlockDesc& B = blocks[tl.y];
   const long long sg = B.sg;
-  const int t = blockIdx.x * TPB + threadIdx.x;
+  const int t = tl.x * TPB + threadIdx.x;
   if (t >= sg) return;
   const double* __restrict__ Q = B.Q[which];
   double* __restrict__ P = B.prim;
@@ -149,8 +191,9 @@ hipError_t launch_primitive(const BlockDesc* db, const BlockDesc* hb, int nb, in
 // cell equal to the interior cell of its periodic image; here each ghost cell
 // gathers that value directly from the block that owns it.
 __global__ __launch_bounds__(TPB) void k_ghost_fill(const BlockDesc* __restrict__ blocks, Domain dom, int which) {
-  const BlockDesc& B = blocks[blockIdx.y];
-  const int t = blockIdx.x * TPB + threadIdx.x;
+  const Tile tl = xcd_tile();
+  const BlockDesc& B = blocks[tl.y];
+  const int t = tl.x * TPB + threadIdx.x;
   if (t >= B.sg) return;
   const int i = t % B.g[0] - NG;
   const int r = t / B.g[0];
@@ -195,15 +238,18 @@ __device__ __forceinline__ double grad4(const double* __restrict__ f, int c, int
   return d / h;
 }
 
-__global__ __launch_bounds__(TPB) void k_visc_flux(const BlockDesc* __restrict__ blocks, Phys ph) {
-  const BlockDesc& B = blocks[blockIdx.y];
+__global__ __launch_bounds__(TPB) void k_visc_flux(const BlockDesc* __restrict__ blocks, Phys ph, int jb) {
+  const Tile tl = xcd_tile();
+  const BlockDesc& B = blocks[tl.y];
   const int v0 = B.n[0] + 2 * NGV, v1 = B.n[1] + 2 * NGV, v2 = B.n[2] + 2 * NGV;
-  const int t = blockIdx.x * TPB + threadIdx.x;
+  const int t = tl.x * TPB + threadIdx.x;
   if (t >= v0 * v1 * v2) return;
-  const int i = t % v0 - NGV;
-  const int r = t / v0;
-  const int j = r % v1 - NGV;
-  const int k = r / v1 - NGV;
+  int i, j, k;
+  band_decode(t, v0, v1, v2, jb, i, j, k);
+  const int vt = i + v0 * (j + v1 * k);
+  i -= NGV;
+  j -= NGV;
+  k -= NGV;
   const int c = gidx(B, i, j, k);
   const int sy = B.g[0], sz = B.g[0] * B.g[1];
   const long long sg = B.sg;
@@ -235,15 +281,15 @@ __global__ __launch_bounds__(TPB) void k_visc_flux(const BlockDesc* __restrict__
   const double ez = uu * txz + vv * tyz + ww * tzz - qz;
   double* __restrict__ V = B.visc;
   const long long sv = B.sv;
-  V[t] = txx;
-  V[sv + t] = tyy;
-  V[2 * sv + t] = tzz;
-  V[3 * sv + t] = txy;
-  V[4 * sv + t] = txz;
-  V[5 * sv + t] = tyz;
-  V[6 * sv + t] = ex;
-  V[7 * sv + t] = ey;
-  V[8 * sv + t] = ez;
+  V[vt] = txx;
+  V[sv + vt] = tyy;
+  V[2 * sv + vt] = tzz;
+  V[3 * sv + vt] = txy;
+  V[4 * sv + vt] = txz;
+  V[5 * sv + vt] = tyz;
+  V[6 * sv + vt] = ex;
+  V[7 * sv + vt] = ey;
+  V[8 * sv + vt] = ez;
 }
 
 // Divergence part of calc_viscous_flux_contribution (src/Block3d.cpp:879-916).
@@ -252,15 +298,15 @@ __device__ __forceinline__ double cdiff4(const double* __restrict__ f, int c, in
   return c0 * f[c - 2 * s] + c1 * f[c - s] - c1 * f[c + s] - c0 * f[c + 2 * s];
 }
 
-__global__ __launch_bounds__(TPB) void k_visc_div(const BlockDesc* __restrict__ blocks, Phys ph) {
-  const BlockDesc& B = blocks[blockIdx.y];
+__global__ __launch_bounds__(TPB) void k_visc_div(const BlockDesc* __restrict__ blocks, Phys ph, int jb) {
+  const Tile tl = xcd_tile();
+  const BlockDesc& B = blocks[tl.y];
   const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
-  const int t = blockIdx.x * TPB + threadIdx.x;
+  const int t = tl.x * TPB + threadIdx.x;
   if (t >= n0 * n1 * n2) return;
-  const int i = t % n0;
-  const int r = t / n0;
-  const int j = r % n1;
-  const int k = r / n1;
+  int i, j, k;
+  band_decode(t, n0, n1, n2, jb, i, j, k);
+  const int ct = i + n0 * (j + n1 * k);
   const int v0 = n0 + 2 * NGV, v1 = n1 + 2 * NGV;
   const int c = (i + NGV) + v0 * ((j + NGV) + v1 * (k + NGV));
   const int sy = v0, sz = v0 * v1;
@@ -273,21 +319,21 @@ __global__ __launch_bounds__(TPB) void k_visc_div(const BlockDesc* __restrict__ 
     double df = cdiff4(V + Ef[e] * sv, c, 1) / ph.dx;
     df += cdiff4(V + Ff[e] * sv, c, sy) / ph.dy;
     df += cdiff4(V + Gf[e] * sv, c, sz) / ph.dz;
-    B.dfv[e * B.sn + t] = df;
+    B.dfv[e * B.sn + ct] = df;
   }
 }
 
-hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, hipStream_t s) {
+hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int jb, hipStream_t s) {
   long long mv = 0, mn = 0;
   for (int b = 0; b < nb; b++) {
     long long cv = (long long)(hb[b].n[0] + 4) * (hb[b].n[1] + 4) * (hb[b].n[2] + 4);
     mv = cv > mv ? cv : mv;
     mn = hb[b].sn > mn ? hb[b].sn : mn;
   }
-  hipLaunchKernelGGL(k_visc_flux, dim3(nblocks_for(mv), nb), dim3(TPB), 0, s, db, ph);
+  hipLaunchKernelGGL(k_visc_flux, dim3(nblocks_for(mv), nb), dim3(TPB), 0, s, db, ph, jb);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_visc_div, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, ph);
+  hipLaunchKernelGGL(k_visc_div, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, ph, jb);
   return hipGetLastError();
 }
 
@@ -295,19 +341,19 @@ hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, cons
 // update_rk3 (src/Block3d.cpp:920-962) on the interior.
 template <typename FT>
 __global__ __launch_bounds__(TPB) void k_update(const BlockDesc* __restrict__ blocks, double dt, int stage,
-                                                Phys ph) {
-  const BlockDesc& B = blocks[blockIdx.y];
+                                                Phys ph, int jb) {
+  const Tile tl = xcd_tile();
+  const BlockDesc& B = blocks[tl.y];
   const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
-  const int t = blockIdx.x * TPB + threadIdx.x;
+  const int t = tl.x * TPB + threadIdx.x;
   if (t >= n0 * n1 * n2) return;
-  const int i = t % n0;
-  const int r = t / n0;
-  const int j = r % n1;
-  const int k = r / n1;
+  int i, j, k;
+  band_decode(t, n0, n1, n2, jb, i, j, k);
+  const int ct = i + n0 * (j + n1 * k);
   const int c = gidx(B, i, j, k);
   const int iE = i + (n0 + 1) * (j + n1 * k);
   const int iF = i + n0 * (j + (n1 + 1) * k);
-  const int iG = t;
+  const int iG = ct;
   double* __restrict__ Q = B.Q[0];
   double* __restrict__ Qp = B.Q[1];
   const long long sg = B.sg;
@@ -319,7 +365,7 @@ __global__ __launch_bounds__(TPB) void k_update(const BlockDesc* __restrict__ bl
     double df = ((double)Ep[iE + 1] - (double)Ep[iE]) / ph.dx;
     df += ((double)Fp[iF + n0] - (double)Fp[iF]) / ph.dy;
     df += ((double)Gp[iG + n0 * n1] - (double)Gp[iG]) / ph.dz;
-    if (e > 0) df -= B.dfv[(e - 1) * B.sn + t];
+    if (e > 0) df -= B.dfv[(e - 1) * B.sn + ct];
     const long long q = e * sg + c;
     if (1 == stage) {
       Qp[q] = Q[q] - dt * df;
@@ -332,13 +378,13 @@ __global__ __launch_bounds__(TPB) void k_update(const BlockDesc* __restrict__ bl
 }
 
 hipError_t launch_update(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
-                         const Phys& ph, int precision, hipStream_t s) {
+                         const Phys& ph, int precision, int jb, hipStream_t s) {
   long long mn = 0;
   for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
   if (precision == 0)
-    hipLaunchKernelGGL(k_update<double>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, dt, stage, ph);
+    hipLaunchKernelGGL(k_update<double>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, dt, stage, ph, jb);
   else
-    hipLaunchKernelGGL(k_update<float>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, dt, stage, ph);
+    hipLaunchKernelGGL(k_update<float>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, dt, stage, ph, jb);
   return hipGetLastError();
 }
 

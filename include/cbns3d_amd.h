@@ -150,6 +150,14 @@ int cbns_diagnostics(cbns_ctx* ctx, double* ke, double* enstrophy);
 /* Synchronise the context's stream. */
 int cbns_sync(cbns_ctx* ctx);
 
+/* Run the viscous kernels of each stage on a second stream beside the face
+ * fluxes (default on).  Results are identical either way. */
+int cbns_set_overlap(cbns_ctx* ctx, int on);
+/* Rows per band of the stencil kernels' work order (default 16): each XCD
+ * walks bands of this many y-rows through all z-planes.  A tuning knob only;
+ * results are identical for every value. */
+int cbns_set_band(cbns_ctx* ctx, int rows);
+
 /* Live kernel timing: while on, every launch made by cbns_step / cbns_run is
  * bracketed by HIP events on the context's stream and its duration is added
  * to its class.  Turning it on clears the totals.  No counterpart in the

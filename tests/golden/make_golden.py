@@ -14,6 +14,8 @@ Fixtures
   ref_tgv64.npz      TGV 64^3 digests after 1 and 100 steps (-np 8, 2 2 2)
   ref_restart16.npz  checkpoint-injected state (is_restart = 0) under
                      MALLOC_PERTURB_=255, 5 steps, run twice
+  ref_vtk8.npz       TGV 8^3, 2 steps: the reference's VTK files (output000,
+                     output001) and checkpoint, as bytes
   ref_point.npz      the reference's compiled weno5 / rec_weno5 /
                      calc_eigenvectors / lf_flux on seeded inputs
 """
@@ -259,7 +261,20 @@ def gen_point():
     print("ref_point done")
 
 
+def gen_vtk8():
+    N = (8, 8, 8)
+    d, out = run_ref(N, 2)
+    res = {}
+    for nm in ("output000/output000.vtr", "output000/output.vtm", "output001/output000.vtr",
+               "output001/output.vtm", "checkpoint000.dat"):
+        res[nm.replace("/", "__").replace(".", "_")] = np.frombuffer(open(os.path.join(d, nm), "rb").read(), np.uint8)
+    res["stdout"] = np.frombuffer(out.encode(), np.uint8)
+    shutil.rmtree(d)
+    np.savez_compressed(os.path.join(HERE, "ref_vtk8.npz"), **res)
+    print("ref_vtk8 done")
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["point", "tgv16", "uneven", "restart16", "tgv64"]
+    which = sys.argv[1:] or ["point", "tgv16", "uneven", "restart16", "vtk8", "tgv64"]
     for w in which:
         globals()["gen_" + w]()

@@ -224,3 +224,24 @@ def test_tgv256_properties():
         b.init_tgv()
         b.run(2)
         assert np.array_equal(b.get_state(), Qa)
+
+
+def test_side_stream_overlap_changes_nothing():
+    with solver(32, (2, 1, 1)) as a, solver(32, (2, 1, 1)) as b:
+        a.set_overlap(False)
+        a.init_tgv()
+        b.init_tgv()
+        a.run(3)
+        b.run(3)
+        assert np.array_equal(a.get_state(), b.get_state())
+
+
+@pytest.mark.parametrize("band", [1, 5, 16, 64])
+def test_band_order_changes_nothing(band):
+    with solver((26, 20, 16), (2, 1, 1)) as a, solver((26, 20, 16), (2, 1, 1)) as b:
+        b.set_band(band)
+        a.init_tgv()
+        b.init_tgv()
+        a.run(2)
+        b.run(2)
+        assert np.array_equal(a.get_state(), b.get_state())
