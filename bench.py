@@ -103,8 +103,9 @@ def main():
         bands = autotune.BANDS if a.band == "auto" else (int(a.band),)
         ovs = (True, False) if a.overlap == "auto" else (a.overlap == "on",)
         best, tms = autotune.tune(s, steps=2, bands=bands, overlaps=ovs)
-        tuned = {"band": best[0], "overlap": best[1],
-                 "ms_per_step": {f"{b}/{'on' if o else 'off'}": round(v * 1e3, 3) for (b, o), v in tms.items()}}
+        tuned = {"band": best[0], "overlap": best[1], "face_waves_per_simd": best[2],
+                 "ms_per_step": {f"{b}/{'on' if o else 'off'}/{w}": round(v * 1e3, 3)
+                                 for (b, o, w), v in tms.items()}}
     else:
         s.set_band(int(a.band))
         s.set_overlap(a.overlap == "on")

@@ -73,6 +73,7 @@ _SIGS = {
     "cbns_set_profiling": (_I, [_VP, _I]),
     "cbns_set_overlap": (_I, [_VP, _I]),
     "cbns_set_band": (_I, [_VP, _I]),
+    "cbns_set_face_occupancy": (_I, [_VP, _I]),
     "cbns_kernel_times": (_I, [_VP, _PD, ctypes.POINTER(ctypes.c_int64)]),
     "cbns_strerror": (ctypes.c_char_p, [_I]),
 }
@@ -260,6 +261,9 @@ class Solver:
 
     def set_band(self, rows):
         check(lib().cbns_set_band(self._h, int(rows)), "cbns_set_band")
+
+    def set_face_occupancy(self, waves):
+        check(lib().cbns_set_face_occupancy(self._h, int(waves)), "cbns_set_face_occupancy")
 
     def set_overlap(self, on):
         check(lib().cbns_set_overlap(self._h, int(bool(on))), "cbns_set_overlap")

@@ -30,6 +30,7 @@ struct cbns_ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool overlap = true;
   int jband = 16;  // rows per band of the stencil kernels' work order
+  int face_occ = 2;  // register-allocation variant of the FP64 face kernel
   unsigned long long* d_smax = nullptr;
   int* d_flag = nullptr;
   double* d_diag = nullptr;  // partials then the two sums
@@ -94,6 +95,16 @@ static hipError_t timed(cbns_ctx* c, int k, F body) {
     hipError_t e_ = (call);                        \
     if (e_ != hipSuccess) return CBNS_ERR_HIP;     \
   } while (0)
+
+// y = RN(1/h); the three-operation quotient x*y corrected by one FMA
+// remainder is correctly rounded for every x when |y h - 1| <= 2^-54
+// (Markstein's condition: q0 = RN(x y) is then within one ulp of x/h).
+// fma(y, h, -1) is exact here (|y h - 1| < 2^-52 needs < 53 bits).
+void cbns::reciprocal_check(double h, double* rh, int* fast) {
+  const double y = 1.0 / h;
+  *rh = y;
+  *fast = (std::isfinite(y) && std::fabs(std::fma(y, h, -1.0)) <= 0x1p-54) ? 1 : 0;
+}
 
 static int dev_alloc(cbns_ctx* c, void** p, size_t bytes) {
   if (hipMalloc(p, bytes) != hipSuccess) return CBNS_ERR_NOMEM;
@@ -190,6 +201,10 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
   c->ph.dx = two_pi / p->IM;
   c->ph.dy = two_pi / p->JM;
   c->ph.dz = two_pi / p->KM;
+  reciprocal_check(c->ph.dx, &c->ph.rdx, &c->ph.fdx);
+  reciprocal_check(c->ph.dy, &c->ph.rdy, &c->ph.fdy);
+  reciprocal_check(c->ph.dz, &c->ph.rdz, &c->ph.fdz);
+  reciprocal_check(c->ph.gam1, &c->ph.rgam1, &c->ph.fgam1);
 
   int st = CBNS_OK;
   c->hb.resize(nb);
@@ -337,7 +352,7 @@ int cbns_calc_dt(cbns_ctx* c, double* dt) {
 
 int cbns_rec_riemann(cbns_ctx* c, int which, int dir) {
   if (!c || (which != CBNS_Q && which != CBNS_QP) || dir < 0 || dir > 2) return CBNS_ERR_ARG;
-  CK(launch_face_flux(c->db, c->hb.data(), c->dom.nb, dir, which, c->ph, c->precision, c->ref, c->jband, c->stream));
+  CK(launch_face_flux(c->db, c->hb.data(), c->dom.nb, dir, which, c->ph, c->precision, c->ref, c->jband, c->face_occ, c->stream));
   return CBNS_OK;
 }
 
@@ -509,7 +524,7 @@ static int run_stages(cbns_ctx* c, double dt) {
       CK(hipEventRecord(c->ev_join, c->side));
     }
     for (int d = 0; d < 3; d++)
-      CK(timed(c, d, [&] { return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->precision, c->ref, c->jband, c->stream); }));
+      CK(timed(c, d, [&] { return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->precision, c->ref, c->jband, c->face_occ, c->stream); }));
     if (c->overlap) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     else CK(timed(c, 3, [&] { return launch_viscous(c->db, hb, c->dom.nb, c->ph, c->jband, c->stream); }));
     CK(timed(c, 4, [&] { return launch_update(c->db, hb, c->dom.nb, dt, stage, c->ph, c->precision, c->jband, c->stream); }));
@@ -565,6 +580,12 @@ int cbns_set_band(cbns_ctx* c, int rows) {
   return CBNS_OK;
 }
 
+int cbns_set_face_occupancy(cbns_ctx* c, int waves) {
+  if (!c || (waves != 2 && waves != 3)) return CBNS_ERR_ARG;
+  c->face_occ = waves;
+  return CBNS_OK;
+}
+
 int cbns_set_overlap(cbns_ctx* c, int on) {
   if (!c) return CBNS_ERR_ARG;
   CK(hipStreamSynchronize(c->stream));
@@ -610,6 +631,10 @@ int cbns_diagnostics(cbns_ctx* c, double* ke, double* ens) {
 int cbns_face_flux_eval(int precision, int dir, double gamma, int nf, const double* cells, const double* prim,
                         double* flux) {
   if (!cells || !prim || !flux || nf < 0 || dir < 0 || dir > 2) return CBNS_ERR_ARG;
+  Phys ph{};
+  ph.gamma = gamma;
+  ph.gam1 = gamma - 1.0;
+  reciprocal_check(ph.gam1, &ph.rgam1, &ph.fgam1);
   if (precision != CBNS_FP64) return CBNS_ERR_ARG;
   if (nf == 0) return CBNS_OK;
   double *dc = nullptr, *dp = nullptr, *df = nullptr;
@@ -620,7 +645,7 @@ int cbns_face_flux_eval(int precision, int dir, double gamma, int nf, const doub
     st = CBNS_ERR_NOMEM;
   if (!st && (hipMemcpy(dc, cells, sizeof(double) * 30 * (size_t)nf, hipMemcpyHostToDevice) != hipSuccess ||
               hipMemcpy(dp, prim, sizeof(double) * 10 * (size_t)nf, hipMemcpyHostToDevice) != hipSuccess ||
-              launch_face_eval(dir, gamma, nf, dc, dp, df, nullptr) != hipSuccess ||
+              launch_face_eval(dir, ph, nf, dc, dp, df, nullptr) != hipSuccess ||
               hipDeviceSynchronize() != hipSuccess ||
               hipMemcpy(flux, df, sizeof(double) * 5 * (size_t)nf, hipMemcpyDeviceToHost) != hipSuccess))
     st = CBNS_ERR_HIP;

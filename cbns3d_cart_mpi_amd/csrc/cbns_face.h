@@ -44,6 +44,19 @@ __device__ __forceinline__ double div6(double x) {
   return __builtin_fma(r, y, q0);
 }
 
+// x / h for a run-time constant h with y = RN(1/h): the same correction as
+// div6, valid (and then correctly rounded) when y is within 2^-54 relative of
+// 1/h, which the host establishes exactly before setting `fast`.  Otherwise
+// the IEEE division.  `fast` is uniform across the launch.
+__device__ __forceinline__ double divc(double x, double h, double y, int fast) {
+  if (fast) {
+    const double q0 = x * y;
+    const double r = __builtin_fma(-q0, h, x);
+    return __builtin_fma(r, y, q0);
+  }
+  return x / h;
+}
+
 // src/weno.cpp:25-56
 __device__ __forceinline__ double weno5(double Um2, double Um1, double U0, double Up1, double Up2) {
   const double eps = 1.0e-6;
@@ -85,12 +98,12 @@ __device__ __forceinline__ void cons_to_prim(const double (&a)[5], double gam1, 
 template <int D>
 __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double (&pl)[5],
                                           const double (&pr)[5], double gamma, double gam1,
-                                          double (&flux)[5]) {
+                                          double rg1, int fg1, double (&flux)[5]) {
   // ---- Roe average (src/eigenvectors_roe.cpp:87-106)
   double c_L = sqrt(gamma * pl[4] / pl[0]);
-  double H_L = c_L * c_L / gam1 + 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+  double H_L = divc(c_L * c_L, gam1, rg1, fg1) + 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
   double c_R = sqrt(gamma * pr[4] / pr[0]);
-  double H_R = c_R * c_R / gam1 + 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+  double H_R = divc(c_R * c_R, gam1, rg1, fg1) + 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
   double ratio = sqrt(pr[0] / pl[0]);
   double uu = (pl[1] + ratio * pr[1]) / (1.0 + ratio);
   double vv = (pl[2] + ratio * pr[2]) / (1.0 + ratio);
@@ -199,11 +212,11 @@ __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double 
 
   // ---- local Lax-Friedrichs flux (src/lf_flux.cpp:32-99)
   double cR = sqrt(gamma * sR[4] / sR[0]);
-  double HR = cR * cR / gam1 + 0.5 * (sR[1] * sR[1] + sR[2] * sR[2] + sR[3] * sR[3]);
-  double rER = sR[4] / gam1 + 0.5 * sR[0] * (sR[1] * sR[1] + sR[2] * sR[2] + sR[3] * sR[3]);
+  double HR = divc(cR * cR, gam1, rg1, fg1) + 0.5 * (sR[1] * sR[1] + sR[2] * sR[2] + sR[3] * sR[3]);
+  double rER = divc(sR[4], gam1, rg1, fg1) + 0.5 * sR[0] * (sR[1] * sR[1] + sR[2] * sR[2] + sR[3] * sR[3]);
   double cL = sqrt(gamma * sL[4] / sL[0]);
-  double HL = cL * cL / gam1 + 0.5 * (sL[1] * sL[1] + sL[2] * sL[2] + sL[3] * sL[3]);
-  double rEL = sL[4] / gam1 + 0.5 * sL[0] * (sL[1] * sL[1] + sL[2] * sL[2] + sL[3] * sL[3]);
+  double HL = divc(cL * cL, gam1, rg1, fg1) + 0.5 * (sL[1] * sL[1] + sL[2] * sL[2] + sL[3] * sL[3]);
+  double rEL = divc(sL[4], gam1, rg1, fg1) + 0.5 * sL[0] * (sL[1] * sL[1] + sL[2] * sL[2] + sL[3] * sL[3]);
   const double qR = sR[1 + D], qL = sL[1 + D];
   double ws = fabs(qR) + cR;
   double wsL = fabs(qL) + cL;

@@ -19,6 +19,11 @@ constexpr int MAXB = 512;
 struct Phys {
   double gamma, gam1, gM2, CFL, C_T_inf, Re_inv, kappa, C_dt_v;
   double dx, dy, dz;
+  // Correctly rounded division by a run-time constant h in three operations
+  // is valid when RN(1/h) is within 2^-54 relative of 1/h (checked exactly on
+  // the host, cbns_create); rh = RN(1/h), and the flag says whether it holds.
+  double rdx, rdy, rdz, rgam1;
+  int fdx, fdy, fdz, fgam1;
 };
 
 // One on-device block: the reference's Block3d arrays, SoA.
@@ -57,7 +62,7 @@ struct Launch {
 // kernel launchers (cbns_kernels.hip)
 struct RefState;
 hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
-                            const Phys& ph, int precision, const RefState& g, int jb, hipStream_t s);
+                            const Phys& ph, int precision, const RefState& g, int jb, int occ, hipStream_t s);
 hipError_t launch_primitive(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
                             hipStream_t s);
 hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
@@ -79,7 +84,9 @@ hipError_t launch_fill_mu(const BlockDesc* db, const BlockDesc* hb, int nb, doub
 constexpr int GSUM_WG = 512;  // workgroups of the fixed-order global sums
 hipError_t launch_global_sums(const BlockDesc* db, const Domain& dom, const Phys& ph, int what, double* d_part,
                               double* d_out, hipStream_t s);
-hipError_t launch_face_eval(int dir, double gamma, int nfaces, const double* cells, const double* prim,
+hipError_t launch_face_eval(int dir, const Phys& ph, int nfaces, const double* cells, const double* prim,
                             double* flux, hipStream_t s);
+// RN(1/h) and whether three-operation division by h is exact (see Phys).
+void reciprocal_check(double h, double* rh, int* fast);
 
 }  // namespace cbns

@@ -62,9 +62,12 @@ __device__ __forceinline__ void band_decode(int t, int ni, int nj, int nk, int j
 // ---------------------------------------------------------------------------
 // Inviscid face fluxes: rec_riemann_x/y/z (src/Block3d.cpp:555-748).
 // One thread per face; face (i,j,k) along D sits between cells i-1 and i.
-template <int D>
-__global__ __launch_bounds__(TPB) void k_face_flux(const BlockDesc* __restrict__ blocks, int which, int jb,
-                                                   double gamma, double gam1) {
+// OCC = minimum workgroups per CU the register allocation must allow: 2 keeps
+// every value in registers (194 VGPRs), 3 trades a few spills for a third
+// wave per SIMD.  Both are compiled; the launch picks one (autotuned).
+template <int D, int OCC>
+__global__ __launch_bounds__(TPB, OCC) void k_face_flux(const BlockDesc* __restrict__ blocks, int which, int jb,
+                                                        double gamma, double gam1, double rg1, int fg1) {
   const Tile tl = xcd_tile();
   const BlockDesc& B = blocks[tl.y];
   const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(TPB) void k_face_flux(const BlockDesc* __restrict__
     pr[f] = P[f * sg + c0];
   }
   double fl[5];
-  face_flux<D>(q, pl, pr, gamma, gam1, fl);
+  face_flux<D>(q, pl, pr, gamma, gam1, rg1, fg1, fl);
   double* __restrict__ F = B.F[D];
   const long long sf = B.sf[D];
 #pragma unroll
@@ -134,7 +137,7 @@ __global__ __launch_bounds__(TPB) void k_face_flux_f32(const BlockDesc* __restri
 }
 
 hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
-                            const Phys& ph, int precision, const RefState& g, int jb, hipStream_t s) {
+                            const Phys& ph, int precision, const RefState& g, int jb, int occ, hipStream_t s) {
   long long mx = 0;
   for (int b = 0; b < nb; b++) {
     long long c = (long long)(hb[b].n[0] + (dir == 0)) * (hb[b].n[1] + (dir == 1)) * (hb[b].n[2] + (dir == 2));
@@ -142,9 +145,19 @@ hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, in
   }
   dim3 grid(nblocks_for(mx), nb);
   if (precision == 0) {
-    if (dir == 0) hipLaunchKernelGGL(k_face_flux<0>, grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1);
-    else if (dir == 1) hipLaunchKernelGGL(k_face_flux<1>, grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1);
-    else hipLaunchKernelGGL(k_face_flux<2>, grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1);
+#define CBNS_FACE(DIR, OCC) \
+  hipLaunchKernelGGL((k_face_flux<DIR, OCC>), grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1, ph.rgam1, \
+                     ph.fgam1)
+    if (occ >= 3) {
+      if (dir == 0) CBNS_FACE(0, 3);
+      else if (dir == 1) CBNS_FACE(1, 3);
+      else CBNS_FACE(2, 3);
+    } else {
+      if (dir == 0) CBNS_FACE(0, 2);
+      else if (dir == 1) CBNS_FACE(1, 2);
+      else CBNS_FACE(2, 2);
+    }
+#undef CBNS_FACE
   } else {
     if (dir == 0) hipLaunchKernelGGL(k_face_flux_f32<0>, grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1, g);
     else if (dir == 1) hipLaunchKernelGGL(k_face_flux_f32<1>, grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1, g);
@@ -233,9 +246,9 @@ hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Dom
 // (src/Block3d.cpp:812-868) fused per point of the extended box [-2, n+2)^3,
 // keeping the nine distinct flux components.  mu is kept on the interior for
 // calc_dt (the reference stores it everywhere; only the interior is read).
-__device__ __forceinline__ double grad4(const double* __restrict__ f, int c, int s, double h) {
+__device__ __forceinline__ double grad4(const double* __restrict__ f, int c, int s, double h, double rh, int fh) {
   double d = (1.0 / 12.0) * f[c - 2 * s] - 2.0 / 3.0 * f[c - s] + (2.0 / 3.0) * f[c + s] - 1.0 / 12.0 * f[c + 2 * s];
-  return d / h;
+  return divc(d, h, rh, fh);
 }
 
 __global__ __launch_bounds__(TPB) void k_visc_flux(const BlockDesc* __restrict__ blocks, Phys ph, int jb) {
@@ -257,10 +270,10 @@ __global__ __launch_bounds__(TPB) void k_visc_flux(const BlockDesc* __restrict__
   const double* __restrict__ v = B.prim + 2 * sg;
   const double* __restrict__ w = B.prim + 3 * sg;
   const double* __restrict__ T = B.prim + 5 * sg;
-  const double u_x = grad4(u, c, 1, ph.dx), u_y = grad4(u, c, sy, ph.dy), u_z = grad4(u, c, sz, ph.dz);
-  const double v_x = grad4(v, c, 1, ph.dx), v_y = grad4(v, c, sy, ph.dy), v_z = grad4(v, c, sz, ph.dz);
-  const double w_x = grad4(w, c, 1, ph.dx), w_y = grad4(w, c, sy, ph.dy), w_z = grad4(w, c, sz, ph.dz);
-  const double T_x = grad4(T, c, 1, ph.dx), T_y = grad4(T, c, sy, ph.dy), T_z = grad4(T, c, sz, ph.dz);
+  const double u_x = grad4(u, c, 1, ph.dx, ph.rdx, ph.fdx), u_y = grad4(u, c, sy, ph.dy, ph.rdy, ph.fdy), u_z = grad4(u, c, sz, ph.dz, ph.rdz, ph.fdz);
+  const double v_x = grad4(v, c, 1, ph.dx, ph.rdx, ph.fdx), v_y = grad4(v, c, sy, ph.dy, ph.rdy, ph.fdy), v_z = grad4(v, c, sz, ph.dz, ph.rdz, ph.fdz);
+  const double w_x = grad4(w, c, 1, ph.dx, ph.rdx, ph.fdx), w_y = grad4(w, c, sy, ph.dy, ph.rdy, ph.fdy), w_z = grad4(w, c, sz, ph.dz, ph.rdz, ph.fdz);
+  const double T_x = grad4(T, c, 1, ph.dx, ph.rdx, ph.fdx), T_y = grad4(T, c, sy, ph.dy, ph.rdy, ph.fdy), T_z = grad4(T, c, sz, ph.dz, ph.rdz, ph.fdz);
 
   const double T_l = T[c];
   double mu_l = (1.0 + ph.C_T_inf) / (T_l + ph.C_T_inf) * sqrt(T_l * T_l * T_l);
@@ -316,9 +329,9 @@ __global__ __launch_bounds__(TPB) void k_visc_div(const BlockDesc* __restrict__ 
   const int Ef[4] = {0, 3, 4, 6}, Ff[4] = {3, 1, 5, 7}, Gf[4] = {4, 5, 2, 8};
 #pragma unroll
   for (int e = 0; e < 4; e++) {
-    double df = cdiff4(V + Ef[e] * sv, c, 1) / ph.dx;
-    df += cdiff4(V + Ff[e] * sv, c, sy) / ph.dy;
-    df += cdiff4(V + Gf[e] * sv, c, sz) / ph.dz;
+    double df = divc(cdiff4(V + Ef[e] * sv, c, 1), ph.dx, ph.rdx, ph.fdx);
+    df += divc(cdiff4(V + Ff[e] * sv, c, sy), ph.dy, ph.rdy, ph.fdy);
+    df += divc(cdiff4(V + Gf[e] * sv, c, sz), ph.dz, ph.rdz, ph.fdz);
     B.dfv[e * B.sn + ct] = df;
   }
 }
@@ -362,9 +375,9 @@ __global__ __launch_bounds__(TPB) void k_update(const BlockDesc* __restrict__ bl
     const FT* __restrict__ Ep = (const FT*)B.F[0] + e * B.sf[0];
     const FT* __restrict__ Fp = (const FT*)B.F[1] + e * B.sf[1];
     const FT* __restrict__ Gp = (const FT*)B.F[2] + e * B.sf[2];
-    double df = ((double)Ep[iE + 1] - (double)Ep[iE]) / ph.dx;
-    df += ((double)Fp[iF + n0] - (double)Fp[iF]) / ph.dy;
-    df += ((double)Gp[iG + n0 * n1] - (double)Gp[iG]) / ph.dz;
+    double df = divc((double)Ep[iE + 1] - (double)Ep[iE], ph.dx, ph.rdx, ph.fdx);
+    df += divc((double)Fp[iF + n0] - (double)Fp[iF], ph.dy, ph.rdy, ph.fdy);
+    df += divc((double)Gp[iG + n0 * n1] - (double)Gp[iG], ph.dz, ph.rdz, ph.fdz);
     if (e > 0) df -= B.dfv[(e - 1) * B.sn + ct];
     const long long q = e * sg + c;
     if (1 == stage) {
@@ -424,12 +437,12 @@ __global__ __launch_bounds__(TPB) void k_dt(const BlockDesc* __restrict__ blocks
     const double pp = B.prim[4 * sg + c];
     const double cs = sqrt(ph.gamma * pp / rr);
     const double factor_v = 2.0 * B.mu[t] * ph.C_dt_v * sqrt(uu * uu + vv * vv + ww * ww) / cs / rr;
-    double t1 = fabs(uu) + cs / ph.dx;
-    t1 += factor_v / ph.dx / ph.dx;
-    double t2 = fabs(vv) + cs / ph.dy;
-    t2 += factor_v / ph.dy / ph.dy;
-    double t3 = fabs(ww) + cs / ph.dz;
-    t3 += factor_v / ph.dz / ph.dz;
+    double t1 = fabs(uu) + divc(cs, ph.dx, ph.rdx, ph.fdx);
+    t1 += divc(divc(factor_v, ph.dx, ph.rdx, ph.fdx), ph.dx, ph.rdx, ph.fdx);
+    double t2 = fabs(vv) + divc(cs, ph.dy, ph.rdy, ph.fdy);
+    t2 += divc(divc(factor_v, ph.dy, ph.rdy, ph.fdy), ph.dy, ph.rdy, ph.fdy);
+    double t3 = fabs(ww) + divc(cs, ph.dz, ph.rdz, ph.fdz);
+    t3 += divc(divc(factor_v, ph.dz, ph.rdz, ph.fdz), ph.dz, ph.rdz, ph.fdz);
     const double s = t1 + t2 + t3;
     if (!(s < 1.0e308)) bad = 1;  // NaN or inf
     s_max = (s_max < s) ? s : s_max;
@@ -618,9 +631,9 @@ __global__ __launch_bounds__(TPB) void k_global_sums(const BlockDesc* __restrict
       const double* w = B.prim + 3 * sg;
       const double rr = B.prim[c], uu = u[c], vv = v[c], ww = w[c];
       sa += 0.5 * rr * (uu * uu + vv * vv + ww * ww);
-      const double ox = grad4(w, c, sy, ph.dy) - grad4(v, c, sz, ph.dz);
-      const double oy = grad4(u, c, sz, ph.dz) - grad4(w, c, 1, ph.dx);
-      const double oz = grad4(v, c, 1, ph.dx) - grad4(u, c, sy, ph.dy);
+      const double ox = grad4(w, c, sy, ph.dy, ph.rdy, ph.fdy) - grad4(v, c, sz, ph.dz, ph.rdz, ph.fdz);
+      const double oy = grad4(u, c, sz, ph.dz, ph.rdz, ph.fdz) - grad4(w, c, 1, ph.dx, ph.rdx, ph.fdx);
+      const double oz = grad4(v, c, 1, ph.dx, ph.rdx, ph.fdx) - grad4(u, c, sy, ph.dy, ph.rdy, ph.fdy);
       sb += 0.5 * rr * (ox * ox + oy * oy + oz * oz);
     }
   }
@@ -669,7 +682,8 @@ hipError_t launch_global_sums(const BlockDesc* db, const Domain& dom, const Phys
 // ---------------------------------------------------------------------------
 // Face flux on explicit inputs (parity tests of the per-face arithmetic).
 template <int D>
-__global__ __launch_bounds__(TPB) void k_face_eval(double gamma, int nf, const double* __restrict__ cells,
+__global__ __launch_bounds__(TPB) void k_face_eval(double gamma, double gam1, double rg1, int fg1, int nf,
+                                                   const double* __restrict__ cells,
                                                    const double* __restrict__ prim, double* __restrict__ flux) {
   const int t = blockIdx.x * TPB + threadIdx.x;
   if (t >= nf) return;
@@ -683,17 +697,23 @@ __global__ __launch_bounds__(TPB) void k_face_eval(double gamma, int nf, const d
     pl[e] = prim[(long long)t * 10 + e];
     pr[e] = prim[(long long)t * 10 + 5 + e];
   }
-  face_flux<D>(q, pl, pr, gamma, gamma - 1.0, fl);
+  face_flux<D>(q, pl, pr, gamma, gam1, rg1, fg1, fl);
 #pragma unroll
   for (int e = 0; e < 5; e++) flux[(long long)t * 5 + e] = fl[e];
 }
 
-hipError_t launch_face_eval(int dir, double gamma, int nf, const double* cells, const double* prim, double* flux,
+hipError_t launch_face_eval(int dir, const Phys& ph, int nf, const double* cells, const double* prim, double* flux,
                             hipStream_t s) {
   dim3 grid(nblocks_for(nf));
-  if (dir == 0) hipLaunchKernelGGL(k_face_eval<0>, grid, dim3(TPB), 0, s, gamma, nf, cells, prim, flux);
-  else if (dir == 1) hipLaunchKernelGGL(k_face_eval<1>, grid, dim3(TPB), 0, s, gamma, nf, cells, prim, flux);
-  else hipLaunchKernelGGL(k_face_eval<2>, grid, dim3(TPB), 0, s, gamma, nf, cells, prim, flux);
+  if (dir == 0)
+    hipLaunchKernelGGL(k_face_eval<0>, grid, dim3(TPB), 0, s, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, nf, cells, prim,
+                       flux);
+  else if (dir == 1)
+    hipLaunchKernelGGL(k_face_eval<1>, grid, dim3(TPB), 0, s, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, nf, cells, prim,
+                       flux);
+  else
+    hipLaunchKernelGGL(k_face_eval<2>, grid, dim3(TPB), 0, s, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, nf, cells, prim,
+                       flux);
   return hipGetLastError();
 }
 

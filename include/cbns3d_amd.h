@@ -157,6 +157,9 @@ int cbns_set_overlap(cbns_ctx* ctx, int on);
  * walks bands of this many y-rows through all z-planes.  A tuning knob only;
  * results are identical for every value. */
 int cbns_set_band(cbns_ctx* ctx, int rows);
+/* Register-allocation variant of the FP64 face kernel: 2 or 3 waves per SIMD
+ * (default 2).  A tuning knob only; results are identical. */
+int cbns_set_face_occupancy(cbns_ctx* ctx, int waves);
 
 /* Live kernel timing: while on, every launch made by cbns_step / cbns_run is
  * bracketed by HIP events on the context's stream and its duration is added

@@ -245,3 +245,13 @@ def test_band_order_changes_nothing(band):
         a.run(2)
         b.run(2)
         assert np.array_equal(a.get_state(), b.get_state())
+
+
+def test_face_occupancy_variant_changes_nothing():
+    with solver(24, (1, 2, 1)) as a, solver(24, (1, 2, 1)) as b:
+        b.set_face_occupancy(3)
+        a.init_tgv()
+        b.init_tgv()
+        a.run(2)
+        b.run(2)
+        assert np.array_equal(a.get_state(), b.get_state())
