@@ -510,26 +510,31 @@ int cbns_set_time(cbns_ctx* c, double t) {
 }
 
 static int run_stages(cbns_ctx* c, double dt) {
-  // src/cbns3d.cpp:203-240
+  // src/cbns3d.cpp:203-240, as five launches per stage: the three face-flux
+  // sweeps, the viscous fluxes (beside them on the side stream when overlap
+  // is on), the fused divergence + RK3 update + primitive recovery, and the
+  // ghost gather of the state with its primitives.
   for (int stage = 1; stage <= 3; stage++) {
     const int in = stage == 1 ? CBNS_Q : CBNS_QP;
     const int out = stage == 3 ? CBNS_Q : CBNS_QP;
     const BlockDesc* hb = c->hb.data();
     if (c->overlap) {
-      // the viscous terms need only the stage input's primitives: run them on
-      // the side stream while the (compute-bound) face fluxes run here
       CK(hipEventRecord(c->ev_fork, c->stream));
       CK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-      CK(timed_on(c, c->side, 3, [&] { return launch_viscous(c->db, hb, c->dom.nb, c->ph, c->jband, c->side); }));
+      CK(timed_on(c, c->side, 3, [&] { return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->jband, c->side); }));
       CK(hipEventRecord(c->ev_join, c->side));
     }
     for (int d = 0; d < 3; d++)
-      CK(timed(c, d, [&] { return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->precision, c->ref, c->jband, c->face_occ, c->stream); }));
+      CK(timed(c, d, [&] {
+        return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->precision, c->ref, c->jband, c->face_occ,
+                                c->stream);
+      }));
     if (c->overlap) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
-    else CK(timed(c, 3, [&] { return launch_viscous(c->db, hb, c->dom.nb, c->ph, c->jband, c->stream); }));
-    CK(timed(c, 4, [&] { return launch_update(c->db, hb, c->dom.nb, dt, stage, c->ph, c->precision, c->jband, c->stream); }));
-    CK(timed(c, 5, [&] { return launch_ghost_fill(c->db, hb, c->dom, out, c->stream); }));
-    CK(timed(c, 6, [&] { return launch_primitive(c->db, hb, c->dom.nb, out, c->ph, c->stream); }));
+    else CK(timed(c, 3, [&] { return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->jband, c->stream); }));
+    CK(timed(c, 4, [&] {
+      return launch_update_fused(c->db, hb, c->dom.nb, dt, stage, c->ph, c->precision, c->jband, c->stream);
+    }));
+    CK(timed(c, 5, [&] { return launch_ghost_fill_prim(c->db, hb, c->dom, out, c->stream); }));
   }
   return CBNS_OK;
 }

@@ -336,6 +336,16 @@ __global__ __launch_bounds__(TPB) void k_visc_div(const BlockDesc* __restrict__ 
   }
 }
 
+hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int jb, hipStream_t s) {
+  long long mv = 0;
+  for (int b = 0; b < nb; b++) {
+    long long cv = (long long)(hb[b].n[0] + 4) * (hb[b].n[1] + 4) * (hb[b].n[2] + 4);
+    mv = cv > mv ? cv : mv;
+  }
+  hipLaunchKernelGGL(k_visc_flux, dim3(nblocks_for(mv), nb), dim3(TPB), 0, s, db, ph, jb);
+  return hipGetLastError();
+}
+
 hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int jb, hipStream_t s) {
   long long mv = 0, mn = 0;
   for (int b = 0; b < nb; b++) {
@@ -402,6 +412,134 @@ hipError_t launch_update(const BlockDesc* db, const BlockDesc* hb, int nb, doubl
 }
 
 
+
+// ---------------------------------------------------------------------------
+// Fused stage tail of the time-step path: the viscous divergence
+// (src/Block3d.cpp:879-916), the RK3 update (src/Block3d.cpp:920-962) and the
+// primitive recovery of the updated interior cell (src/Block3d.cpp:359-385),
+// one pass per interior cell.  Each value is formed by the same expressions
+// as the separate kernels, so the results are identical; the viscous
+// divergence and the primitives never round-trip through HBM.
+template <typename FT>
+__global__ __launch_bounds__(TPB) void k_update_fused(const BlockDesc* __restrict__ blocks, double dt, int stage,
+                                                      Phys ph, int jb) {
+  const Tile tl = xcd_tile();
+  const BlockDesc& B = blocks[tl.y];
+  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  const int t = tl.x * TPB + threadIdx.x;
+  if (t >= n0 * n1 * n2) return;
+  int i, j, k;
+  band_decode(t, n0, n1, n2, jb, i, j, k);
+  const int ct = i + n0 * (j + n1 * k);
+  const int c = gidx(B, i, j, k);
+  // viscous divergence, dfv[e] for e = 0..3 (Ev, Fv, Gv components as k_visc_div)
+  const int v0 = n0 + 2 * NGV, v1 = n1 + 2 * NGV;
+  const int cv = (i + NGV) + v0 * ((j + NGV) + v1 * (k + NGV));
+  const double* __restrict__ V = B.visc;
+  const long long sv = B.sv;
+  const int Ef[4] = {0, 3, 4, 6}, Ff[4] = {3, 1, 5, 7}, Gf[4] = {4, 5, 2, 8};
+  double dfv[4];
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    double d = divc(cdiff4(V + Ef[e] * sv, cv, 1), ph.dx, ph.rdx, ph.fdx);
+    d += divc(cdiff4(V + Ff[e] * sv, cv, v0), ph.dy, ph.rdy, ph.fdy);
+    d += divc(cdiff4(V + Gf[e] * sv, cv, v0 * v1), ph.dz, ph.rdz, ph.fdz);
+    dfv[e] = d;
+  }
+  const int iE = i + (n0 + 1) * (j + n1 * k);
+  const int iF = i + n0 * (j + (n1 + 1) * k);
+  const int iG = ct;
+  double* __restrict__ Q = B.Q[0];
+  double* __restrict__ Qp = B.Q[1];
+  const long long sg = B.sg;
+  double qn[NEQ];
+#pragma unroll
+  for (int e = 0; e < NEQ; e++) {
+    const FT* __restrict__ Ep = (const FT*)B.F[0] + e * B.sf[0];
+    const FT* __restrict__ Fp = (const FT*)B.F[1] + e * B.sf[1];
+    const FT* __restrict__ Gp = (const FT*)B.F[2] + e * B.sf[2];
+    double df = divc((double)Ep[iE + 1] - (double)Ep[iE], ph.dx, ph.rdx, ph.fdx);
+    df += divc((double)Fp[iF + n0] - (double)Fp[iF], ph.dy, ph.rdy, ph.fdy);
+    df += divc((double)Gp[iG + n0 * n1] - (double)Gp[iG], ph.dz, ph.rdz, ph.fdz);
+    if (e > 0) df -= dfv[e - 1];
+    const long long q = e * sg + c;
+    if (1 == stage) {
+      qn[e] = Q[q] - dt * df;
+      Qp[q] = qn[e];
+    } else if (2 == stage) {
+      qn[e] = 0.75 * Q[q] + 0.25 * (Qp[q] - dt * df);
+      Qp[q] = qn[e];
+    } else {
+      qn[e] = (Q[q] + 2.0 * (Qp[q] - dt * df)) / 3.0;
+      Q[q] = qn[e];
+    }
+  }
+  // calc_primitive of the updated cell
+  double* __restrict__ P = B.prim;
+  const double rr = qn[0];
+  const double uu = qn[1] / rr;
+  const double vv = qn[2] / rr;
+  const double ww = qn[3] / rr;
+  const double pp = ph.gam1 * (qn[4] - 0.5 * rr * (uu * uu + vv * vv + ww * ww));
+  P[c] = rr;
+  P[sg + c] = uu;
+  P[2 * sg + c] = vv;
+  P[3 * sg + c] = ww;
+  P[4 * sg + c] = pp;
+  P[5 * sg + c] = ph.gM2 * pp / rr;
+}
+
+hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
+                               const Phys& ph, int precision, int jb, hipStream_t s) {
+  long long mn = 0;
+  for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
+  if (stage < 1 || stage > 3) return hipSuccess;
+  if (precision == 0)
+    hipLaunchKernelGGL(k_update_fused<double>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, dt, stage, ph, jb);
+  else
+    hipLaunchKernelGGL(k_update_fused<float>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, dt, stage, ph, jb);
+  return hipGetLastError();
+}
+
+// Ghost fill of the state and of its primitives together: the primitives of
+// a ghost cell are those of its periodic image (calc_primitive is pointwise).
+__global__ __launch_bounds__(TPB) void k_ghost_fill_prim(const BlockDesc* __restrict__ blocks, Domain dom, int which) {
+  const BlockDesc& B = blocks[blockIdx.y];
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= B.sg) return;
+  const int i = t % B.g[0] - NG;
+  const int r = t / B.g[0];
+  const int j = r % B.g[1] - NG;
+  const int k = r / B.g[1] - NG;
+  if (i >= 0 && i < B.n[0] && j >= 0 && j < B.n[1] && k >= 0 && k < B.n[2]) return;
+  int gl[3] = {B.o[0] + i, B.o[1] + j, B.o[2] + k};
+  int bc[3], lc[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    int x = gl[a] % dom.N[a];
+    x = x < 0 ? x + dom.N[a] : x;
+    int bb = x / dom.base[a];
+    bb = bb > dom.dims[a] - 1 ? dom.dims[a] - 1 : bb;
+    bc[a] = bb;
+    lc[a] = x - bb * dom.base[a];
+  }
+  const BlockDesc& S = blocks[(bc[0] * dom.dims[1] + bc[1]) * dom.dims[2] + bc[2]];
+  const int src = gidx(S, lc[0], lc[1], lc[2]);
+  const double* __restrict__ Qs = S.Q[which];
+  double* __restrict__ Qd = B.Q[which];
+#pragma unroll
+  for (int e = 0; e < NEQ; e++) Qd[e * B.sg + t] = Qs[e * S.sg + src];
+#pragma unroll
+  for (int f = 0; f < 6; f++) B.prim[f * B.sg + t] = S.prim[f * S.sg + src];
+}
+
+hipError_t launch_ghost_fill_prim(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
+                                  hipStream_t s) {
+  long long mx = 0;
+  for (int b = 0; b < dom.nb; b++) mx = hb[b].sg > mx ? hb[b].sg : mx;
+  hipLaunchKernelGGL(k_ghost_fill_prim, dim3(nblocks_for(mx), dom.nb), dim3(TPB), 0, s, db, dom, which);
+  return hipGetLastError();
+}
 
 // ---------------------------------------------------------------------------
 // calc_dt spectral radius (src/Block3d.cpp:515-553), max over every block.
