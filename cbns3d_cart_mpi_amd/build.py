@@ -21,7 +21,7 @@ HIP_FLAGS = [
     "-fPIC", "-Wall", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-SOURCES = ["cbns_kernels.hip", "cbns_capi.cpp"]
+SOURCES = ["cbns_kernels.hip", "cbns_capi.cpp", "cbns_batch.hip", "cbns_batch_capi.cpp"]
 
 
 def run(cmd):

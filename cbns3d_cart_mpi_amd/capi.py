@@ -75,6 +75,17 @@ _SIGS = {
     "cbns_set_band": (_I, [_VP, _I]),
     "cbns_set_face_occupancy": (_I, [_VP, _I]),
     "cbns_kernel_times": (_I, [_VP, _PD, ctypes.POINTER(ctypes.c_int64)]),
+    "cbns_batch_create": (_I, [ctypes.c_uint64, _I, ctypes.POINTER(_VP)]),
+    "cbns_batch_destroy": (_I, [_VP]),
+    "cbns_batch_bytes_needed": (_I, [ctypes.POINTER(Params), _PI, ctypes.POINTER(ctypes.c_uint64)]),
+    "cbns_batch_memory": (_I, [_VP, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    "cbns_batch_add": (_I, [_VP, ctypes.POINTER(Params), _PI, _I, _PD, _D, _D, _I, _PI]),
+    "cbns_batch_remove": (_I, [_VP, _I]),
+    "cbns_batch_run": (_I, [_VP, _I]),
+    "cbns_batch_sync": (_I, [_VP]),
+    "cbns_batch_status": (_I, [_VP, _I, _PI, _PI, _PD, _PI]),
+    "cbns_batch_poll": (_I, [_VP, _PI, _I, _PI, _PI]),
+    "cbns_batch_get_state": (_I, [_VP, _I, _PD]),
     "cbns_strerror": (ctypes.c_char_p, [_I]),
 }
 
@@ -288,3 +299,84 @@ def face_flux_eval(direction, gamma, cells, prim, precision=CBNS_FP64):
     check(lib().cbns_face_flux_eval(precision, direction, gamma, n, _p(cells), _p(prim), _p(out)),
           "cbns_face_flux_eval")
     return out
+
+
+def batch_bytes_needed(params, dims=(1, 1, 1)):
+    v = ctypes.c_uint64()
+    check(lib().cbns_batch_bytes_needed(ctypes.byref(params), (_I * 3)(*dims), ctypes.byref(v)),
+          "cbns_batch_bytes_needed")
+    return v.value
+
+
+class Batch:
+    """Many independent solves advanced together (persistent kernels over a
+    packed block table, one hipGraph per RK3 step)."""
+
+    def __init__(self, budget_bytes, capacity=256):
+        h = _VP()
+        check(lib().cbns_batch_create(int(budget_bytes), int(capacity), ctypes.byref(h)), "cbns_batch_create")
+        self._h = h
+        self._N = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cbns_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, params, dims=(1, 1, 1), precision=CBNS_FP64, Q=None, t_cur=0.0, mu0=0.0, steps=1 << 30):
+        """Admit a solve; returns its id.  Raises CbnsError(-3) when the budget
+        or capacity is full."""
+        qp = None
+        if Q is not None:
+            Q = np.ascontiguousarray(Q, dtype=np.float64)
+            assert Q.size == 5 * params.IM * params.JM * params.KM
+            qp = _p(Q)
+        i = _I()
+        check(lib().cbns_batch_add(self._h, ctypes.byref(params), (_I * 3)(*dims), precision, qp, t_cur, mu0,
+                                   int(steps), ctypes.byref(i)), "cbns_batch_add")
+        self._N[i.value] = (params.IM, params.JM, params.KM)
+        return i.value
+
+    def remove(self, i):
+        check(lib().cbns_batch_remove(self._h, i), "cbns_batch_remove")
+        self._N.pop(i, None)
+
+    def run(self, nsteps):
+        check(lib().cbns_batch_run(self._h, int(nsteps)), "cbns_batch_run")
+
+    def sync(self):
+        check(lib().cbns_batch_sync(self._h), "cbns_batch_sync")
+
+    def status(self, i):
+        a, n, nf = _I(), _I(), _I()
+        t = _D()
+        check(lib().cbns_batch_status(self._h, i, ctypes.byref(a), ctypes.byref(n), ctypes.byref(t),
+                                      ctypes.byref(nf)), "cbns_batch_status")
+        return {"active": bool(a.value), "steps_done": n.value, "t_cur": t.value, "nonfinite": bool(nf.value)}
+
+    def poll(self, max_ids=4096):
+        ids = (_I * max_ids)()
+        nd, nl = _I(), _I()
+        check(lib().cbns_batch_poll(self._h, ids, max_ids, ctypes.byref(nd), ctypes.byref(nl)), "cbns_batch_poll")
+        return [ids[k] for k in range(nd.value)], nl.value
+
+    def get_state(self, i):
+        N = self._N[i]
+        out = np.empty((N[2], N[1], N[0], 5))
+        check(lib().cbns_batch_get_state(self._h, i, _p(out)), "cbns_batch_get_state")
+        return out
+
+    @property
+    def memory(self):
+        u, b = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().cbns_batch_memory(self._h, ctypes.byref(u), ctypes.byref(b)), "cbns_batch_memory")
+        return u.value, b.value

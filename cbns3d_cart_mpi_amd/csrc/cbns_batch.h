@@ -1,0 +1,52 @@
+// Batched form: many independent solves (instances) of uneven sizes advanced
+// together.  Each stage kernel is one persistent launch over a packed table of
+// work tiles drawn from every live instance's blocks; dt, t and the active
+// flag of every instance live on the device, so a whole RK3 step is a fixed
+// launch sequence that is captured once as a hipGraph and replayed.
+#pragma once
+#include "cbns_face.h"
+#include "cbns_kernels.h"
+
+namespace cbns {
+
+struct InstDesc {
+  Phys ph;
+  RefState ref;
+  Domain dom;
+  int block0, nblk;
+  int precision;
+  int active;        // 1 while stepping; 0 when idle, finished or failed
+  int nonfinite;     // set by the dt reduction, ends the solve
+  int steps_done, steps_target;
+  double t_cur, t_end, dt;
+  unsigned long long smax;  // dt reduction target, reset after each use
+};
+
+// One work tile: 256 consecutive work items of one block.
+struct TileRef {
+  int block, tile;
+};
+
+// Work classes of a step, each with its own packed tile table.
+enum BatchWork {
+  BW_FACE0 = 0, BW_FACE1, BW_FACE2,  // per precision: see BatchTables
+  BW_VISC, BW_UPDATE, BW_GHOST, BW_DT, BW_COUNT
+};
+
+struct TileTable {
+  const TileRef* tiles;
+  int n;
+};
+
+constexpr int BATCH_WG = 1024;  // persistent workgroups per launch
+
+hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int dir, int which, int jb,
+                             int precision, hipStream_t s);
+hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, hipStream_t s);
+hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int stage, int jb,
+                               int precision, hipStream_t s);
+hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, hipStream_t s);
+hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, hipStream_t s);
+hipError_t batch_launch_finalize(InstDesc* insts, int ninst, hipStream_t s);
+
+}  // namespace cbns

@@ -1,0 +1,196 @@
+// Persistent batch kernels: each launch walks a packed tile table that mixes
+// the blocks of many solves.  Workgroup w takes one contiguous chunk of the
+// table, and chunks are assigned so that the workgroups of one XCD (w mod 8
+// under round-robin dispatch) own one contiguous range of it, keeping each
+// solve's stencil neighbours in that XCD's L2.  The per-item arithmetic is the
+// shared bodies', so every solve's values equal a single-solve run's.
+#include "cbns_batch.h"
+#include "cbns_bodies.h"
+
+namespace cbns {
+
+__device__ __forceinline__ void tile_chunk(int n, int& first, int& last) {
+  const int G = gridDim.x;
+  const int base = G / NXCD, rem = G % NXCD;
+  const int r = blockIdx.x % NXCD, q = blockIdx.x / NXCD;
+  const int L = r * base + (r < rem ? r : rem) + q;
+  first = (int)((long long)n * L / G);
+  last = (int)((long long)n * (L + 1) / G);
+}
+
+template <int D, int P>
+__global__ __launch_bounds__(TPB, 2) void kb_face(const BlockDesc* __restrict__ blocks,
+                                                  const InstDesc* __restrict__ insts, TileTable tt, int which,
+                                                  int jb) {
+  int first, last;
+  tile_chunk(tt.n, first, last);
+  for (int x = first; x < last; x++) {
+    const TileRef tr = tt.tiles[x];
+    const BlockDesc& B = blocks[tr.block];
+    const InstDesc& I = insts[B.inst];
+    if (!I.active) continue;
+    const int t = tr.tile * TPB + threadIdx.x;
+    if (t >= face_count(B, D)) continue;
+    if (P == 0) face_body<D>(B, t, which, jb, I.ph);
+    else face_body_f32<D>(B, t, which, jb, I.ph, I.ref);
+  }
+}
+
+__global__ __launch_bounds__(TPB) void kb_visc(const BlockDesc* __restrict__ blocks,
+                                               const InstDesc* __restrict__ insts, TileTable tt, int jb) {
+  int first, last;
+  tile_chunk(tt.n, first, last);
+  for (int x = first; x < last; x++) {
+    const TileRef tr = tt.tiles[x];
+    const BlockDesc& B = blocks[tr.block];
+    const InstDesc& I = insts[B.inst];
+    if (!I.active) continue;
+    const int t = tr.tile * TPB + threadIdx.x;
+    if (t >= visc_count(B)) continue;
+    visc_flux_body(B, t, I.ph, jb);
+  }
+}
+
+template <typename FT>
+__global__ __launch_bounds__(TPB) void kb_update(const BlockDesc* __restrict__ blocks,
+                                                 const InstDesc* __restrict__ insts, TileTable tt, int stage, int jb) {
+  int first, last;
+  tile_chunk(tt.n, first, last);
+  for (int x = first; x < last; x++) {
+    const TileRef tr = tt.tiles[x];
+    const BlockDesc& B = blocks[tr.block];
+    const InstDesc& I = insts[B.inst];
+    if (!I.active) continue;
+    const int t = tr.tile * TPB + threadIdx.x;
+    if (t >= B.n[0] * B.n[1] * B.n[2]) continue;
+    update_body<FT>(B, t, I.dt, stage, I.ph, jb);
+  }
+}
+
+__global__ __launch_bounds__(TPB) void kb_ghost(const BlockDesc* __restrict__ blocks,
+                                                const InstDesc* __restrict__ insts, TileTable tt, int which) {
+  int first, last;
+  tile_chunk(tt.n, first, last);
+  for (int x = first; x < last; x++) {
+    const TileRef tr = tt.tiles[x];
+    const BlockDesc& B = blocks[tr.block];
+    const InstDesc& I = insts[B.inst];
+    if (!I.active) continue;
+    const int t = tr.tile * TPB + threadIdx.x;
+    if (t >= B.sg) continue;
+    ghost_body(B, blocks, I.dom, I.block0, t, which, true);
+  }
+}
+
+// Per-solve max of the spectral radius: a workgroup keeps a running max
+// for the solve it is on and flushes it (integer atomic max on the bits of a
+// non-negative double) when its chunk moves to another solve or ends.
+__global__ __launch_bounds__(TPB) void kb_dt(const BlockDesc* __restrict__ blocks, InstDesc* insts, TileTable tt) {
+  int first, last;
+  tile_chunk(tt.n, first, last);
+  const int lane = threadIdx.x & 63;
+  int cur = -1;
+  double acc = 0.0;
+  int bad = 0;
+  for (int x = first; x < last; x++) {
+    const TileRef tr = tt.tiles[x];
+    const BlockDesc& B = blocks[tr.block];
+    const int inst = B.inst;
+    if (!insts[inst].active) continue;
+    if (inst != cur) {
+      if (cur >= 0) {
+        const double m = wave_max(acc);
+        if (lane == 0) atomicMax(&insts[cur].smax, (unsigned long long)__double_as_longlong(m));
+        if (bad) atomicOr(&insts[cur].nonfinite, 1);
+      }
+      cur = inst;
+      acc = 0.0;
+      bad = 0;
+    }
+    const int t = tr.tile * TPB + threadIdx.x;
+    if (t < B.n[0] * B.n[1] * B.n[2]) {
+      const double s = dt_cell(B, t, insts[inst].ph, &bad);
+      acc = (acc < s) ? s : acc;  // src/Block3d.cpp:546: a NaN never wins
+    }
+  }
+  if (cur >= 0) {
+    const double m = wave_max(acc);
+    if (lane == 0) atomicMax(&insts[cur].smax, (unsigned long long)__double_as_longlong(m));
+    if (bad) atomicOr(&insts[cur].nonfinite, 1);
+  }
+}
+
+// The driver's per-step bookkeeping on the device (src/cbns3d.cpp:178-194):
+// is_finished, dt = CFL / s_max, the clip to t_end, t_cur += dt; a solve also
+// stops when it has taken its requested steps or met a non-finite value.
+__global__ void kb_finalize(InstDesc* insts, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  InstDesc& I = insts[i];
+  if (!I.active) return;
+  const unsigned long long bits = I.smax;
+  I.smax = 0ull;
+  if (I.nonfinite || I.steps_done >= I.steps_target || I.t_cur + 1.0e-5 > I.t_end) {
+    I.active = 0;
+    I.dt = 0.0;
+    return;
+  }
+  const double s_max = __longlong_as_double((long long)bits);
+  double dt = I.ph.CFL / s_max;
+  if (I.t_end - I.t_cur < dt) dt = I.t_end - I.t_cur;
+  I.t_cur += dt;
+  I.dt = dt;
+  I.steps_done += 1;
+}
+
+hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int dir, int which, int jb,
+                             int precision, hipStream_t s) {
+  if (tt.n == 0) return hipSuccess;
+  const dim3 g(BATCH_WG), b(TPB);
+  if (precision == 0) {
+    if (dir == 0) hipLaunchKernelGGL((kb_face<0, 0>), g, b, 0, s, blocks, insts, tt, which, jb);
+    else if (dir == 1) hipLaunchKernelGGL((kb_face<1, 0>), g, b, 0, s, blocks, insts, tt, which, jb);
+    else hipLaunchKernelGGL((kb_face<2, 0>), g, b, 0, s, blocks, insts, tt, which, jb);
+  } else {
+    if (dir == 0) hipLaunchKernelGGL((kb_face<0, 1>), g, b, 0, s, blocks, insts, tt, which, jb);
+    else if (dir == 1) hipLaunchKernelGGL((kb_face<1, 1>), g, b, 0, s, blocks, insts, tt, which, jb);
+    else hipLaunchKernelGGL((kb_face<2, 1>), g, b, 0, s, blocks, insts, tt, which, jb);
+  }
+  return hipGetLastError();
+}
+
+hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, hipStream_t s) {
+  if (tt.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_visc, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt, jb);
+  return hipGetLastError();
+}
+
+hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int stage, int jb,
+                               int precision, hipStream_t s) {
+  if (tt.n == 0) return hipSuccess;
+  if (precision == 0)
+    hipLaunchKernelGGL(kb_update<double>, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt, stage, jb);
+  else
+    hipLaunchKernelGGL(kb_update<float>, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt, stage, jb);
+  return hipGetLastError();
+}
+
+hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, hipStream_t s) {
+  if (tt.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_ghost, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt, which);
+  return hipGetLastError();
+}
+
+hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, hipStream_t s) {
+  if (tt.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_dt, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt);
+  return hipGetLastError();
+}
+
+hipError_t batch_launch_finalize(InstDesc* insts, int ninst, hipStream_t s) {
+  if (ninst == 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_finalize, dim3((ninst + 63) / 64), dim3(64), 0, s, insts, ninst);
+  return hipGetLastError();
+}
+
+}  // namespace cbns

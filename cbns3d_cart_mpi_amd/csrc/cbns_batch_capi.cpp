@@ -1,0 +1,489 @@
+// C ABI of the batched form: many independent solves of uneven sizes in one
+// set of persistent launches per RK stage, one hipGraph per RK3 step.
+//
+// Composition changes (add / remove) rebuild the packed block and tile tables
+// and re-capture the step graph; between changes, cbns_batch_run replays the
+// graph with no host synchronisation.  Every solve's arithmetic is the
+// single-solve path's (shared per-item bodies, per-solve constants, an
+// order-independent dt maximum), so a solve's result does not depend on what
+// it is batched with.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/cbns3d_amd.h"
+#include "cbns_batch.h"
+#include "cbns_kernels.h"
+
+using namespace cbns;
+
+namespace {
+
+struct Inst {
+  bool live = false;
+  cbns_params par{};
+  int dims[3] = {1, 1, 1};
+  int precision = 0;
+  Domain dom{};
+  std::vector<BlockDesc> blocks;  // block0-relative (instance-local) descriptors
+  BlockDesc* dblocks = nullptr;   // instance-local device copy (setup, gather)
+  std::vector<void*> allocs;
+  uint64_t bytes = 0;
+  int block0 = 0;
+};
+
+enum { T_FACE64 = 0, T_FACE32 = 3, T_VISC = 6, T_UPD64, T_UPD32, T_GHOST, T_DT, T_COUNT };
+
+}  // namespace
+
+struct cbns_batch {
+  uint64_t budget = 0, used = 0;
+  int cap = 0;
+  std::vector<Inst> inst;
+  InstDesc* d_insts = nullptr;
+  BlockDesc* d_blocks = nullptr;
+  int d_blocks_cap = 0;
+  TileRef* d_tiles[T_COUNT] = {nullptr};
+  int tiles_cap[T_COUNT] = {0};
+  TileTable tt[T_COUNT] = {};
+  double* d_part = nullptr;  // global-sum partials (FP32 reference states)
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool dirty = true;
+  int jb = 32;
+  int ninst_hw = 0;  // highest live id + 1
+};
+
+#define CK(call)                                 \
+  do {                                           \
+    hipError_t e_ = (call);                      \
+    if (e_ != hipSuccess) return CBNS_ERR_HIP;   \
+  } while (0)
+
+static Phys make_phys(const cbns_params* p) {
+  Phys ph{};
+  const double two_pi = 2.0 * 3.14159265358979323846;
+  ph.gamma = p->gamma;
+  ph.gam1 = p->gam1;
+  ph.gM2 = p->gM2;
+  ph.CFL = p->CFL;
+  ph.C_T_inf = p->C_T_inf;
+  ph.Re_inv = p->Re_inv;
+  ph.kappa = -p->Pr_inv * p->gam1_inv / p->Mach2;
+  ph.C_dt_v = p->C_dt_v;
+  ph.dx = two_pi / p->IM;
+  ph.dy = two_pi / p->JM;
+  ph.dz = two_pi / p->KM;
+  reciprocal_check(ph.dx, &ph.rdx, &ph.fdx);
+  reciprocal_check(ph.dy, &ph.rdy, &ph.fdy);
+  reciprocal_check(ph.dz, &ph.rdz, &ph.fdz);
+  reciprocal_check(ph.gam1, &ph.rgam1, &ph.fgam1);
+  return ph;
+}
+
+// Bytes of device memory one solve needs (its blocks' arrays).
+static uint64_t solve_bytes(const cbns_params* p, const int dims[3]) {
+  const int N[3] = {(int)p->IM, (int)p->JM, (int)p->KM};
+  uint64_t total = 0;
+  for (int bx = 0; bx < dims[0]; bx++)
+    for (int by = 0; by < dims[1]; by++)
+      for (int bz = 0; bz < dims[2]; bz++) {
+        const int bc[3] = {bx, by, bz};
+        uint64_t n[3];
+        for (int a = 0; a < 3; a++) {
+          const int base = N[a] / dims[a];
+          n[a] = (bc[a] == dims[a] - 1) ? N[a] - (dims[a] - 1) * base : base;
+        }
+        const uint64_t g = (n[0] + 8) * (n[1] + 8) * (n[2] + 8);
+        const uint64_t v = (n[0] + 4) * (n[1] + 4) * (n[2] + 4);
+        const uint64_t in = n[0] * n[1] * n[2];
+        const uint64_t f = (n[0] + 1) * n[1] * n[2] + n[0] * (n[1] + 1) * n[2] + n[0] * n[1] * (n[2] + 1);
+        total += 8 * (2 * NEQ * g + NEQ * f + 6 * g + in + 9 * v);
+      }
+  return total + sizeof(BlockDesc) * dims[0] * dims[1] * dims[2];
+}
+
+static void free_inst(Inst& I) {
+  for (void* p : I.allocs) (void)hipFree(p);
+  I.allocs.clear();
+  I.blocks.clear();
+  I.dblocks = nullptr;
+  I.live = false;
+}
+
+static int grow(void** p, int* cap, int need, size_t elem) {
+  if (need <= *cap) return CBNS_OK;
+  int nc = std::max(need, *cap * 2 + 16);
+  void* q = nullptr;
+  if (hipMalloc(&q, (size_t)nc * elem) != hipSuccess) return CBNS_ERR_NOMEM;
+  if (*p) (void)hipFree(*p);
+  *p = q;
+  *cap = nc;
+  return CBNS_OK;
+}
+
+// Rebuild the packed block table, the per-class tile tables and the step graph.
+static int rebuild(cbns_batch* b) {
+  CK(hipStreamSynchronize(b->stream));
+  std::vector<BlockDesc> all;
+  std::vector<TileRef> tab[T_COUNT];
+  int hw = 0;
+  for (int id = 0; id < b->cap; id++) {
+    Inst& I = b->inst[id];
+    if (!I.live) continue;
+    hw = id + 1;
+    I.block0 = (int)all.size();
+    for (size_t k = 0; k < I.blocks.size(); k++) {
+      BlockDesc B = I.blocks[k];
+      B.inst = id;
+      const int bi = (int)all.size();
+      all.push_back(B);
+      auto add = [&](int cls, long long count) {
+        const int nt = (int)((count + 255) / 256);
+        for (int t = 0; t < nt; t++) tab[cls].push_back({bi, t});
+      };
+      for (int d = 0; d < 3; d++) add((I.precision ? T_FACE32 : T_FACE64) + d, B.sf[d]);
+      add(T_VISC, B.sv);
+      add(I.precision ? T_UPD32 : T_UPD64, B.sn);
+      add(T_GHOST, B.sg);
+      add(T_DT, B.sn);
+    }
+    // the device descriptor's block0 (other fields hold live state)
+    CK(hipMemcpy((char*)(b->d_insts + id) + offsetof(InstDesc, block0), &I.block0, sizeof(int),
+                 hipMemcpyHostToDevice));
+  }
+  b->ninst_hw = hw;
+  int st;
+  if ((st = grow((void**)&b->d_blocks, &b->d_blocks_cap, (int)all.size() + 1, sizeof(BlockDesc)))) return st;
+  if (!all.empty())
+    CK(hipMemcpy(b->d_blocks, all.data(), all.size() * sizeof(BlockDesc), hipMemcpyHostToDevice));
+  for (int c = 0; c < T_COUNT; c++) {
+    if ((st = grow((void**)&b->d_tiles[c], &b->tiles_cap[c], (int)tab[c].size() + 1, sizeof(TileRef)))) return st;
+    if (!tab[c].empty())
+      CK(hipMemcpy(b->d_tiles[c], tab[c].data(), tab[c].size() * sizeof(TileRef), hipMemcpyHostToDevice));
+    b->tt[c] = {b->d_tiles[c], (int)tab[c].size()};
+  }
+  // capture one RK3 step (src/cbns3d.cpp:178-240) as a graph
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  if (hw == 0) {
+    b->dirty = false;
+    return CBNS_OK;
+  }
+  hipStream_t s = b->stream;
+  CK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipError_t e = batch_launch_dt(b->d_blocks, b->d_insts, b->tt[T_DT], s);
+  if (e == hipSuccess) e = batch_launch_finalize(b->d_insts, hw, s);
+  for (int stage = 1; stage <= 3 && e == hipSuccess; stage++) {
+    const int in = stage == 1 ? CBNS_Q : CBNS_QP;
+    const int out = stage == 3 ? CBNS_Q : CBNS_QP;
+    e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC], b->jb, s);
+    for (int d = 0; d < 3 && e == hipSuccess; d++) {
+      e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE64 + d], d, in, b->jb, 0, s);
+      if (e == hipSuccess) e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE32 + d], d, in, b->jb, 1, s);
+    }
+    if (e == hipSuccess) e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD64], stage, b->jb, 0, s);
+    if (e == hipSuccess) e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD32], stage, b->jb, 1, s);
+    if (e == hipSuccess) e = batch_launch_ghost(b->d_blocks, b->d_insts, b->tt[T_GHOST], out, s);
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(s, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return CBNS_ERR_HIP;
+  }
+  b->graph = g;
+  CK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  b->dirty = false;
+  return CBNS_OK;
+}
+
+extern "C" {
+
+int cbns_batch_create(uint64_t hbm_budget_bytes, int capacity, cbns_batch** out) {
+  if (!out || capacity < 1) return CBNS_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  cbns_device_count(&n);
+  if (n == 0) return CBNS_ERR_NODEV;
+  cbns_batch* b = new (std::nothrow) cbns_batch();
+  if (!b) return CBNS_ERR_NOMEM;
+  b->budget = hbm_budget_bytes;
+  b->cap = capacity;
+  b->inst.resize(capacity);
+  int st = CBNS_OK;
+  if (hipMalloc(&b->d_insts, sizeof(InstDesc) * capacity) != hipSuccess ||
+      hipMalloc(&b->d_part, sizeof(double) * (2 * GSUM_WG + 2)) != hipSuccess)
+    st = CBNS_ERR_NOMEM;
+  if (!st && hipMemset(b->d_insts, 0, sizeof(InstDesc) * capacity) != hipSuccess) st = CBNS_ERR_HIP;
+  if (!st && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
+  if (st) {
+    cbns_batch_destroy(b);
+    return st;
+  }
+  *out = b;
+  return CBNS_OK;
+}
+
+int cbns_batch_destroy(cbns_batch* b) {
+  if (!b) return CBNS_ERR_ARG;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (Inst& I : b->inst) free_inst(I);
+  for (int c = 0; c < T_COUNT; c++)
+    if (b->d_tiles[c]) (void)hipFree(b->d_tiles[c]);
+  if (b->d_blocks) (void)hipFree(b->d_blocks);
+  if (b->d_insts) (void)hipFree(b->d_insts);
+  if (b->d_part) (void)hipFree(b->d_part);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return CBNS_OK;
+}
+
+int cbns_batch_bytes_needed(const cbns_params* p, const int dims[3], uint64_t* bytes) {
+  if (!p || !dims || !bytes) return CBNS_ERR_ARG;
+  *bytes = solve_bytes(p, dims);
+  return CBNS_OK;
+}
+
+int cbns_batch_memory(const cbns_batch* b, uint64_t* used, uint64_t* budget) {
+  if (!b) return CBNS_ERR_ARG;
+  if (used) *used = b->used;
+  if (budget) *budget = b->budget;
+  return CBNS_OK;
+}
+
+int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int precision, const double* Q_aos,
+                   double t_cur, double mu0, int steps, int* id_out) {
+  if (!b || !p || !dims || !id_out || steps < 0) return CBNS_ERR_ARG;
+  if (precision != CBNS_FP64 && precision != CBNS_FP32_FLUX) return CBNS_ERR_ARG;
+  const int N[3] = {(int)p->IM, (int)p->JM, (int)p->KM};
+  for (int a = 0; a < 3; a++) {
+    if (dims[a] < 1 || N[a] < 1) return CBNS_ERR_ARG;
+    const int base = N[a] / dims[a];
+    if (base < NG || N[a] - (dims[a] - 1) * base < NG) return CBNS_ERR_ARG;
+  }
+  const uint64_t need = solve_bytes(p, dims);
+  if (b->used + need > b->budget) return CBNS_ERR_NOMEM;
+  int id = -1;
+  for (int k = 0; k < b->cap; k++)
+    if (!b->inst[k].live) {
+      id = k;
+      break;
+    }
+  if (id < 0) return CBNS_ERR_NOMEM;
+  CK(hipStreamSynchronize(b->stream));
+  Inst& I = b->inst[id];
+  I = Inst();
+  I.par = *p;
+  I.precision = precision;
+  std::memcpy(I.dims, dims, sizeof(I.dims));
+  for (int a = 0; a < 3; a++) {
+    I.dom.N[a] = N[a];
+    I.dom.dims[a] = dims[a];
+    I.dom.base[a] = N[a] / dims[a];
+  }
+  I.dom.nb = dims[0] * dims[1] * dims[2];
+  auto alloc = [&](size_t bytes) -> double* {
+    void* q = nullptr;
+    if (hipMalloc(&q, bytes) != hipSuccess) return nullptr;
+    I.allocs.push_back(q);
+    return (double*)q;
+  };
+  bool ok = true;
+  for (int bx = 0; bx < dims[0] && ok; bx++)
+    for (int by = 0; by < dims[1] && ok; by++)
+      for (int bz = 0; bz < dims[2] && ok; bz++) {
+        BlockDesc B;
+        std::memset(&B, 0, sizeof(B));
+        const int bc[3] = {bx, by, bz};
+        for (int a = 0; a < 3; a++) {
+          const int base = N[a] / dims[a];
+          B.n[a] = (bc[a] == dims[a] - 1) ? N[a] - (dims[a] - 1) * base : base;
+          B.g[a] = B.n[a] + 2 * NG;
+          B.o[a] = bc[a] * base;
+        }
+        B.inst = id;
+        B.sg = (long long)B.g[0] * B.g[1] * B.g[2];
+        B.sv = (long long)(B.n[0] + 2 * NGV) * (B.n[1] + 2 * NGV) * (B.n[2] + 2 * NGV);
+        B.sn = (long long)B.n[0] * B.n[1] * B.n[2];
+        for (int d = 0; d < 3; d++)
+          B.sf[d] = (long long)(B.n[0] + (d == 0)) * (B.n[1] + (d == 1)) * (B.n[2] + (d == 2));
+        ok = (B.Q[0] = alloc(8 * NEQ * B.sg)) && (B.Q[1] = alloc(8 * NEQ * B.sg)) &&
+             (B.F[0] = alloc(8 * NEQ * B.sf[0])) && (B.F[1] = alloc(8 * NEQ * B.sf[1])) &&
+             (B.F[2] = alloc(8 * NEQ * B.sf[2])) && (B.prim = alloc(8 * 6 * B.sg)) && (B.mu = alloc(8 * B.sn)) &&
+             (B.visc = alloc(8 * 9 * B.sv));
+        I.blocks.push_back(B);
+      }
+  if (ok) ok = (I.dblocks = (BlockDesc*)alloc(sizeof(BlockDesc) * I.blocks.size())) != nullptr;
+  if (!ok) {
+    free_inst(I);
+    return CBNS_ERR_NOMEM;
+  }
+  I.bytes = need;
+  const Phys ph = make_phys(p);
+  int st = CBNS_OK;
+  hipStream_t s = b->stream;
+  const int nb = (int)I.blocks.size();
+  if (hipMemcpy(I.dblocks, I.blocks.data(), sizeof(BlockDesc) * nb, hipMemcpyHostToDevice) != hipSuccess)
+    st = CBNS_ERR_HIP;
+  // initial state: Taylor-Green (Q_aos == NULL) or the given interior state
+  if (!st && !Q_aos) {
+    const int ts = std::max(N[0], std::max(N[1], N[2]));
+    std::vector<double> tab(9 * (size_t)ts, 0.0);
+    const double tmp = 2.0 * 3.14159265358979323846;
+    const double h[3] = {ph.dx, ph.dy, ph.dz};
+    for (int a = 0; a < 3; a++)
+      for (int i = 0; i < N[a]; i++) {
+        const double xc = ((unsigned)i * h[a] - tmp / 2.0) + 0.5 * h[a];
+        tab[(3 * a + 0) * ts + i] = std::sin(xc);
+        tab[(3 * a + 1) * ts + i] = std::cos(xc);
+        tab[(3 * a + 2) * ts + i] = std::cos(2.0 * xc);
+      }
+    double* dtab = nullptr;
+    if (hipMalloc(&dtab, sizeof(double) * tab.size()) != hipSuccess) st = CBNS_ERR_NOMEM;
+    if (!st && (hipMemcpy(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                launch_tgv(I.dblocks, I.blocks.data(), nb, dtab, ts, ph, p->p_inf, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess))
+      st = CBNS_ERR_HIP;
+    if (dtab) (void)hipFree(dtab);
+    t_cur = p->t_cur;
+  } else if (!st) {
+    const size_t n = (size_t)NEQ * N[0] * N[1] * N[2];
+    double* d = nullptr;
+    if (hipMalloc(&d, n * sizeof(double)) != hipSuccess) st = CBNS_ERR_NOMEM;
+    if (!st && (hipMemcpy(d, Q_aos, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+                launch_scatter_aos(I.dblocks, I.blocks.data(), I.dom, CBNS_Q, d, s) != hipSuccess ||
+                launch_fill_mu(I.dblocks, I.blocks.data(), nb, mu0, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess))
+      st = CBNS_ERR_HIP;
+    if (d) (void)hipFree(d);
+  }
+  // boundary_condition, calc_primitive, init_Q_p (src/cbns3d.cpp:149-153)
+  if (!st && (launch_ghost_fill(I.dblocks, I.blocks.data(), I.dom, CBNS_Q, s) != hipSuccess ||
+              launch_primitive(I.dblocks, I.blocks.data(), nb, CBNS_Q, ph, s) != hipSuccess ||
+              launch_copy_state(I.dblocks, I.blocks.data(), nb, CBNS_Q, CBNS_QP, s) != hipSuccess))
+    st = CBNS_ERR_HIP;
+  InstDesc D;
+  std::memset(&D, 0, sizeof(D));
+  D.ph = ph;
+  D.ref = RefState{1.0, 0.0, 0.0};
+  if (!st && precision == CBNS_FP32_FLUX) {
+    double sums[2];
+    if (launch_global_sums(I.dblocks, I.dom, ph, 0, b->d_part, b->d_part + 2 * GSUM_WG, s) != hipSuccess ||
+        hipMemcpyAsync(sums, b->d_part + 2 * GSUM_WG, sizeof(sums), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      st = CBNS_ERR_HIP;
+    const double n = (double)N[0] * N[1] * N[2];
+    D.ref.rho = sums[0] / n;
+    D.ref.E = sums[1] / n;
+    D.ref.p = ph.gam1 * D.ref.E;
+  }
+  if (st) {
+    free_inst(I);
+    return st;
+  }
+  D.dom = I.dom;
+  D.nblk = nb;
+  D.precision = precision;
+  D.active = 1;
+  D.steps_target = steps;
+  D.t_cur = t_cur;
+  D.t_end = p->t_end;
+  if (hipMemcpy(b->d_insts + id, &D, sizeof(D), hipMemcpyHostToDevice) != hipSuccess) {
+    free_inst(I);
+    return CBNS_ERR_HIP;
+  }
+  I.live = true;
+  b->used += need;
+  b->dirty = true;
+  *id_out = id;
+  return CBNS_OK;
+}
+
+int cbns_batch_remove(cbns_batch* b, int id) {
+  if (!b || id < 0 || id >= b->cap || !b->inst[id].live) return CBNS_ERR_ARG;
+  CK(hipStreamSynchronize(b->stream));
+  int zero = 0;
+  CK(hipMemcpy((char*)(b->d_insts + id) + offsetof(InstDesc, active), &zero, sizeof(int), hipMemcpyHostToDevice));
+  b->used -= b->inst[id].bytes;
+  free_inst(b->inst[id]);
+  b->dirty = true;
+  return CBNS_OK;
+}
+
+int cbns_batch_run(cbns_batch* b, int nsteps) {
+  if (!b || nsteps < 0) return CBNS_ERR_ARG;
+  if (b->dirty) {
+    int st = rebuild(b);
+    if (st) return st;
+  }
+  if (!b->exec) return CBNS_OK;
+  for (int n = 0; n < nsteps; n++) CK(hipGraphLaunch(b->exec, b->stream));
+  return CBNS_OK;
+}
+
+int cbns_batch_sync(cbns_batch* b) {
+  if (!b) return CBNS_ERR_ARG;
+  CK(hipStreamSynchronize(b->stream));
+  return CBNS_OK;
+}
+
+int cbns_batch_status(cbns_batch* b, int id, int* active, int* steps_done, double* t_cur, int* nonfinite) {
+  if (!b || id < 0 || id >= b->cap || !b->inst[id].live) return CBNS_ERR_ARG;
+  InstDesc D;
+  CK(hipStreamSynchronize(b->stream));
+  CK(hipMemcpy(&D, b->d_insts + id, sizeof(D), hipMemcpyDeviceToHost));
+  if (active) *active = D.active;
+  if (steps_done) *steps_done = D.steps_done;
+  if (t_cur) *t_cur = D.t_cur;
+  if (nonfinite) *nonfinite = D.nonfinite;
+  return CBNS_OK;
+}
+
+int cbns_batch_poll(cbns_batch* b, int* ids, int max_ids, int* n_done, int* n_live) {
+  if (!b || (!ids && max_ids > 0)) return CBNS_ERR_ARG;
+  CK(hipStreamSynchronize(b->stream));
+  std::vector<InstDesc> D(b->cap);
+  CK(hipMemcpy(D.data(), b->d_insts, sizeof(InstDesc) * b->cap, hipMemcpyDeviceToHost));
+  int nd = 0, nl = 0;
+  for (int id = 0; id < b->cap; id++) {
+    if (!b->inst[id].live) continue;
+    nl++;
+    const bool done = !D[id].active;
+    if (done) {
+      if (nd < max_ids) ids[nd] = id;
+      nd++;
+    }
+  }
+  if (n_done) *n_done = std::min(nd, max_ids);
+  if (n_live) *n_live = nl;
+  return CBNS_OK;
+}
+
+int cbns_batch_get_state(cbns_batch* b, int id, double* Q_aos) {
+  if (!b || !Q_aos || id < 0 || id >= b->cap || !b->inst[id].live) return CBNS_ERR_ARG;
+  Inst& I = b->inst[id];
+  const size_t n = (size_t)NEQ * I.dom.N[0] * I.dom.N[1] * I.dom.N[2];
+  CK(hipStreamSynchronize(b->stream));
+  double* d = nullptr;
+  CK(hipMalloc(&d, n * sizeof(double)));
+  int st = CBNS_OK;
+  if (launch_gather_aos(I.dblocks, I.blocks.data(), I.dom, CBNS_Q, d, b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess ||
+      hipMemcpy(Q_aos, d, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    st = CBNS_ERR_HIP;
+  (void)hipFree(d);
+  return st;
+}
+
+}  // extern "C"

@@ -37,7 +37,7 @@ struct BlockDesc {
   int n[3];       // interior cells
   int g[3];       // ghosted extent n + 8
   int o[3];       // global origin
-  int pad;
+  int inst;       // owning solve (batched form); 0 for a single solve
   long long sg;   // ghosted field stride
   long long sv;   // viscous-box field stride
   long long sf[3];// face-array field strides

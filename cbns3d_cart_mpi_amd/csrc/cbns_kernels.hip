@@ -5,24 +5,17 @@
 // touches 64 consecutive cells of one x-row (coalesced SoA loads).  Stencil
 // neighbours along y and z are the same rows shifted by g0 or g0*g1, served
 // from L1/L2.
-#include "cbns_face.h"
-#include "cbns_kernels.h"
+#include "cbns_bodies.h"
 
 namespace cbns {
 
-constexpr int TPB = 256;
-
 static inline unsigned nblocks_for(long long count) { return (unsigned)((count + TPB - 1) / TPB); }
 
-__device__ __forceinline__ int gidx(const BlockDesc& B, int i, int j, int k) {
-  return (i + NG) + B.g[0] * ((j + NG) + B.g[1] * (k + NG));
-}
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2.  Stencil kernels want neighbouring rows on the same L2, so the linear
 // workgroup id is remapped so that XCD r runs one contiguous range of logical
 // tiles (a bijection on [0, total) for any total).
-constexpr int NXCD = 8;
 struct Tile {
   int x, y;  // logical (blockIdx.x, blockIdx.y)
 };
@@ -36,28 +29,6 @@ __device__ __forceinline__ Tile xcd_tile() {
   return {logical % gx, logical / gx};
 }
 
-// Work index t -> (i, j, k) of an ni x nj x nk box in band order
-// (j-band of jb rows, k, j within the band, i): a contiguous run of t covers
-// a band of rows over successive planes, so with xcd_tile each XCD keeps the
-// y-neighbour rows (same band) and the z-neighbour rows (same rows, nearby
-// planes) of its stencils in its own L2.
-__device__ __forceinline__ void band_decode(int t, int ni, int nj, int nk, int jb, int& i, int& j, int& k) {
-  i = t % ni;
-  const int R = t / ni;
-  const int full = nj / jb;
-  const int Rfull = full * nk * jb;
-  if (R < Rfull) {
-    const int b = R / (nk * jb);
-    const int rem = R - b * nk * jb;
-    k = rem / jb;
-    j = b * jb + (rem - k * jb);
-  } else {
-    const int last = nj - full * jb;
-    const int rem = R - Rfull;
-    k = rem / last;
-    j = full * jb + (rem - k * last);
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Inviscid face fluxes: rec_riemann_x/y/z (src/Block3d.cpp:555-748).
@@ -67,73 +38,23 @@ __device__ __forceinline__ void band_decode(int t, int ni, int nj, int nk, int j
 // wave per SIMD.  Both are compiled; the launch picks one (autotuned).
 template <int D, int OCC>
 __global__ __launch_bounds__(TPB, OCC) void k_face_flux(const BlockDesc* __restrict__ blocks, int which, int jb,
-                                                        double gamma, double gam1, double rg1, int fg1) {
+                                                        Phys ph) {
   const Tile tl = xcd_tile();
   const BlockDesc& B = blocks[tl.y];
-  const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
   const int t = tl.x * TPB + threadIdx.x;
-  if (t >= ni * nj * nk) return;
-  int i, j, k;
-  band_decode(t, ni, nj, nk, jb, i, j, k);
-  const int fi = i + ni * (j + nj * k);
-  const int c0 = gidx(B, i, j, k);
-  const int sD = D == 0 ? 1 : (D == 1 ? B.g[0] : B.g[0] * B.g[1]);
-  const double* __restrict__ Q = B.Q[which];
-  const long long sg = B.sg;
-  double q[6][5];
-#pragma unroll
-  for (int cc = 0; cc < 6; cc++)
-#pragma unroll
-    for (int e = 0; e < 5; e++) q[cc][e] = Q[e * sg + c0 + (cc - 3) * sD];
-  const double* __restrict__ P = B.prim;
-  double pl[5], pr[5];
-#pragma unroll
-  for (int f = 0; f < 5; f++) {
-    pl[f] = P[f * sg + c0 - sD];
-    pr[f] = P[f * sg + c0];
-  }
-  double fl[5];
-  face_flux<D>(q, pl, pr, gamma, gam1, rg1, fg1, fl);
-  double* __restrict__ F = B.F[D];
-  const long long sf = B.sf[D];
-#pragma unroll
-  for (int e = 0; e < 5; e++) F[e * sf + fi] = fl[e];
+  if (t >= face_count(B, D)) return;
+  face_body<D>(B, t, which, jb, ph);
 }
 
 // FP32-flux form: same stencil and mapping, FP32 face arrays holding F - F(Ug).
 template <int D>
 __global__ __launch_bounds__(TPB) void k_face_flux_f32(const BlockDesc* __restrict__ blocks, int which, int jb,
-                                                       double gamma, double gam1, RefState g) {
+                                                       Phys ph, RefState g) {
   const Tile tl = xcd_tile();
   const BlockDesc& B = blocks[tl.y];
-  const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
   const int t = tl.x * TPB + threadIdx.x;
-  if (t >= ni * nj * nk) return;
-  int i, j, k;
-  band_decode(t, ni, nj, nk, jb, i, j, k);
-  const int fi = i + ni * (j + nj * k);
-  const int c0 = gidx(B, i, j, k);
-  const int sD = D == 0 ? 1 : (D == 1 ? B.g[0] : B.g[0] * B.g[1]);
-  const double* __restrict__ Q = B.Q[which];
-  const long long sg = B.sg;
-  double q[6][5];
-#pragma unroll
-  for (int cc = 0; cc < 6; cc++)
-#pragma unroll
-    for (int e = 0; e < 5; e++) q[cc][e] = Q[e * sg + c0 + (cc - 3) * sD];
-  const double* __restrict__ P = B.prim;
-  double pl[5], pr[5];
-#pragma unroll
-  for (int f = 0; f < 5; f++) {
-    pl[f] = P[f * sg + c0 - sD];
-    pr[f] = P[f * sg + c0];
-  }
-  float fl[5];
-  face_flux_f32<D>(q, pl, pr, gamma, gam1, g, fl);
-  float* __restrict__ F = (float*)B.F[D];
-  const long long sf = B.sf[D];
-#pragma unroll
-  for (int e = 0; e < 5; e++) F[e * sf + fi] = fl[e];
+  if (t >= face_count(B, D)) return;
+  face_body_f32<D>(B, t, which, jb, ph, g);
 }
 
 hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
@@ -146,8 +67,7 @@ hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, in
   dim3 grid(nblocks_for(mx), nb);
   if (precision == 0) {
 #define CBNS_FACE(DIR, OCC) \
-  hipLaunchKernelGGL((k_face_flux<DIR, OCC>), grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1, ph.rgam1, \
-                     ph.fgam1)
+  hipLaunchKernelGGL((k_face_flux<DIR, OCC>), grid, dim3(TPB), 0, s, db, which, jb, ph)
     if (occ >= 3) {
       if (dir == 0) CBNS_FACE(0, 3);
       else if (dir == 1) CBNS_FACE(1, 3);
@@ -159,9 +79,9 @@ hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, in
     }
 #undef CBNS_FACE
   } else {
-    if (dir == 0) hipLaunchKernelGGL(k_face_flux_f32<0>, grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1, g);
-    else if (dir == 1) hipLaunchKernelGGL(k_face_flux_f32<1>, grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1, g);
-    else hipLaunchKernelGGL(k_face_flux_f32<2>, grid, dim3(TPB), 0, s, db, which, jb, ph.gamma, ph.gam1, g);
+    if (dir == 0) hipLaunchKernelGGL(k_face_flux_f32<0>, grid, dim3(TPB), 0, s, db, which, jb, ph, g);
+    else if (dir == 1) hipLaunchKernelGGL(k_face_flux_f32<1>, grid, dim3(TPB), 0, s, db, which, jb, ph, g);
+    else hipLaunchKernelGGL(k_face_flux_f32<2>, grid, dim3(TPB), 0, s, db, which, jb, ph, g);
   }
   return hipGetLastError();
 }
@@ -204,32 +124,10 @@ hipError_t launch_primitive(const BlockDesc* db, const BlockDesc* hb, int nb, in
 // cell equal to the interior cell of its periodic image; here each ghost cell
 // gathers that value directly from the block that owns it.
 __global__ __launch_bounds__(TPB) void k_ghost_fill(const BlockDesc* __restrict__ blocks, Domain dom, int which) {
-  const Tile tl = xcd_tile();
-  const BlockDesc& B = blocks[tl.y];
-  const int t = tl.x * TPB + threadIdx.x;
+  const BlockDesc& B = blocks[blockIdx.y];
+  const int t = blockIdx.x * TPB + threadIdx.x;
   if (t >= B.sg) return;
-  const int i = t % B.g[0] - NG;
-  const int r = t / B.g[0];
-  const int j = r % B.g[1] - NG;
-  const int k = r / B.g[1] - NG;
-  if (i >= 0 && i < B.n[0] && j >= 0 && j < B.n[1] && k >= 0 && k < B.n[2]) return;
-  int gl[3] = {B.o[0] + i, B.o[1] + j, B.o[2] + k};
-  int bc[3], lc[3];
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    int x = gl[a] % dom.N[a];
-    x = x < 0 ? x + dom.N[a] : x;
-    int bb = x / dom.base[a];
-    bb = bb > dom.dims[a] - 1 ? dom.dims[a] - 1 : bb;
-    bc[a] = bb;
-    lc[a] = x - bb * dom.base[a];
-  }
-  const BlockDesc& S = blocks[(bc[0] * dom.dims[1] + bc[1]) * dom.dims[2] + bc[2]];
-  const int src = gidx(S, lc[0], lc[1], lc[2]);
-  const double* __restrict__ Qs = S.Q[which];
-  double* __restrict__ Qd = B.Q[which];
-#pragma unroll
-  for (int e = 0; e < NEQ; e++) Qd[e * B.sg + t] = Qs[e * S.sg + src];
+  ghost_body(B, blocks, dom, 0, t, which, false);
 }
 
 hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
@@ -246,70 +144,16 @@ hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Dom
 // (src/Block3d.cpp:812-868) fused per point of the extended box [-2, n+2)^3,
 // keeping the nine distinct flux components.  mu is kept on the interior for
 // calc_dt (the reference stores it everywhere; only the interior is read).
-__device__ __forceinline__ double grad4(const double* __restrict__ f, int c, int s, double h, double rh, int fh) {
-  double d = (1.0 / 12.0) * f[c - 2 * s] - 2.0 / 3.0 * f[c - s] + (2.0 / 3.0) * f[c + s] - 1.0 / 12.0 * f[c + 2 * s];
-  return divc(d, h, rh, fh);
-}
 
 __global__ __launch_bounds__(TPB) void k_visc_flux(const BlockDesc* __restrict__ blocks, Phys ph, int jb) {
   const Tile tl = xcd_tile();
   const BlockDesc& B = blocks[tl.y];
-  const int v0 = B.n[0] + 2 * NGV, v1 = B.n[1] + 2 * NGV, v2 = B.n[2] + 2 * NGV;
   const int t = tl.x * TPB + threadIdx.x;
-  if (t >= v0 * v1 * v2) return;
-  int i, j, k;
-  band_decode(t, v0, v1, v2, jb, i, j, k);
-  const int vt = i + v0 * (j + v1 * k);
-  i -= NGV;
-  j -= NGV;
-  k -= NGV;
-  const int c = gidx(B, i, j, k);
-  const int sy = B.g[0], sz = B.g[0] * B.g[1];
-  const long long sg = B.sg;
-  const double* __restrict__ u = B.prim + sg;
-  const double* __restrict__ v = B.prim + 2 * sg;
-  const double* __restrict__ w = B.prim + 3 * sg;
-  const double* __restrict__ T = B.prim + 5 * sg;
-  const double u_x = grad4(u, c, 1, ph.dx, ph.rdx, ph.fdx), u_y = grad4(u, c, sy, ph.dy, ph.rdy, ph.fdy), u_z = grad4(u, c, sz, ph.dz, ph.rdz, ph.fdz);
-  const double v_x = grad4(v, c, 1, ph.dx, ph.rdx, ph.fdx), v_y = grad4(v, c, sy, ph.dy, ph.rdy, ph.fdy), v_z = grad4(v, c, sz, ph.dz, ph.rdz, ph.fdz);
-  const double w_x = grad4(w, c, 1, ph.dx, ph.rdx, ph.fdx), w_y = grad4(w, c, sy, ph.dy, ph.rdy, ph.fdy), w_z = grad4(w, c, sz, ph.dz, ph.rdz, ph.fdz);
-  const double T_x = grad4(T, c, 1, ph.dx, ph.rdx, ph.fdx), T_y = grad4(T, c, sy, ph.dy, ph.rdy, ph.fdy), T_z = grad4(T, c, sz, ph.dz, ph.rdz, ph.fdz);
-
-  const double T_l = T[c];
-  double mu_l = (1.0 + ph.C_T_inf) / (T_l + ph.C_T_inf) * sqrt(T_l * T_l * T_l);
-  if (i >= 0 && i < B.n[0] && j >= 0 && j < B.n[1] && k >= 0 && k < B.n[2])
-    B.mu[i + B.n[0] * (j + B.n[1] * k)] = mu_l;
-  mu_l *= ph.Re_inv;
-  const double txx = (2.0 / 3.0) * mu_l * (2.0 * u_x - v_y - w_z);
-  const double tyy = (2.0 / 3.0) * mu_l * (2.0 * v_y - u_x - w_z);
-  const double tzz = (2.0 / 3.0) * mu_l * (2.0 * w_z - u_x - v_y);
-  const double txy = mu_l * (u_y + v_x);
-  const double txz = mu_l * (u_z + w_x);
-  const double tyz = mu_l * (v_z + w_y);
-  mu_l *= ph.kappa;
-  const double qx = mu_l * T_x, qy = mu_l * T_y, qz = mu_l * T_z;
-  const double uu = u[c], vv = v[c], ww = w[c];
-  const double ex = uu * txx + vv * txy + ww * txz - qx;
-  const double ey = uu * txy + vv * tyy + ww * tyz - qy;
-  const double ez = uu * txz + vv * tyz + ww * tzz - qz;
-  double* __restrict__ V = B.visc;
-  const long long sv = B.sv;
-  V[vt] = txx;
-  V[sv + vt] = tyy;
-  V[2 * sv + vt] = tzz;
-  V[3 * sv + vt] = txy;
-  V[4 * sv + vt] = txz;
-  V[5 * sv + vt] = tyz;
-  V[6 * sv + vt] = ex;
-  V[7 * sv + vt] = ey;
-  V[8 * sv + vt] = ez;
+  if (t >= visc_count(B)) return;
+  visc_flux_body(B, t, ph, jb);
 }
 
 // Divergence part of calc_viscous_flux_contribution (src/Block3d.cpp:879-916).
-__device__ __forceinline__ double cdiff4(const double* __restrict__ f, int c, int s) {
-  const double c0 = 1.0 / 12.0, c1 = -2.0 / 3.0;
-  return c0 * f[c - 2 * s] + c1 * f[c - s] - c1 * f[c + s] - c0 * f[c + 2 * s];
-}
 
 __global__ __launch_bounds__(TPB) void k_visc_div(const BlockDesc* __restrict__ blocks, Phys ph, int jb) {
   const Tile tl = xcd_tile();
@@ -425,68 +269,9 @@ __global__ __launch_bounds__(TPB) void k_update_fused(const BlockDesc* __restric
                                                       Phys ph, int jb) {
   const Tile tl = xcd_tile();
   const BlockDesc& B = blocks[tl.y];
-  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
   const int t = tl.x * TPB + threadIdx.x;
-  if (t >= n0 * n1 * n2) return;
-  int i, j, k;
-  band_decode(t, n0, n1, n2, jb, i, j, k);
-  const int ct = i + n0 * (j + n1 * k);
-  const int c = gidx(B, i, j, k);
-  // viscous divergence, dfv[e] for e = 0..3 (Ev, Fv, Gv components as k_visc_div)
-  const int v0 = n0 + 2 * NGV, v1 = n1 + 2 * NGV;
-  const int cv = (i + NGV) + v0 * ((j + NGV) + v1 * (k + NGV));
-  const double* __restrict__ V = B.visc;
-  const long long sv = B.sv;
-  const int Ef[4] = {0, 3, 4, 6}, Ff[4] = {3, 1, 5, 7}, Gf[4] = {4, 5, 2, 8};
-  double dfv[4];
-#pragma unroll
-  for (int e = 0; e < 4; e++) {
-    double d = divc(cdiff4(V + Ef[e] * sv, cv, 1), ph.dx, ph.rdx, ph.fdx);
-    d += divc(cdiff4(V + Ff[e] * sv, cv, v0), ph.dy, ph.rdy, ph.fdy);
-    d += divc(cdiff4(V + Gf[e] * sv, cv, v0 * v1), ph.dz, ph.rdz, ph.fdz);
-    dfv[e] = d;
-  }
-  const int iE = i + (n0 + 1) * (j + n1 * k);
-  const int iF = i + n0 * (j + (n1 + 1) * k);
-  const int iG = ct;
-  double* __restrict__ Q = B.Q[0];
-  double* __restrict__ Qp = B.Q[1];
-  const long long sg = B.sg;
-  double qn[NEQ];
-#pragma unroll
-  for (int e = 0; e < NEQ; e++) {
-    const FT* __restrict__ Ep = (const FT*)B.F[0] + e * B.sf[0];
-    const FT* __restrict__ Fp = (const FT*)B.F[1] + e * B.sf[1];
-    const FT* __restrict__ Gp = (const FT*)B.F[2] + e * B.sf[2];
-    double df = divc((double)Ep[iE + 1] - (double)Ep[iE], ph.dx, ph.rdx, ph.fdx);
-    df += divc((double)Fp[iF + n0] - (double)Fp[iF], ph.dy, ph.rdy, ph.fdy);
-    df += divc((double)Gp[iG + n0 * n1] - (double)Gp[iG], ph.dz, ph.rdz, ph.fdz);
-    if (e > 0) df -= dfv[e - 1];
-    const long long q = e * sg + c;
-    if (1 == stage) {
-      qn[e] = Q[q] - dt * df;
-      Qp[q] = qn[e];
-    } else if (2 == stage) {
-      qn[e] = 0.75 * Q[q] + 0.25 * (Qp[q] - dt * df);
-      Qp[q] = qn[e];
-    } else {
-      qn[e] = (Q[q] + 2.0 * (Qp[q] - dt * df)) / 3.0;
-      Q[q] = qn[e];
-    }
-  }
-  // calc_primitive of the updated cell
-  double* __restrict__ P = B.prim;
-  const double rr = qn[0];
-  const double uu = qn[1] / rr;
-  const double vv = qn[2] / rr;
-  const double ww = qn[3] / rr;
-  const double pp = ph.gam1 * (qn[4] - 0.5 * rr * (uu * uu + vv * vv + ww * ww));
-  P[c] = rr;
-  P[sg + c] = uu;
-  P[2 * sg + c] = vv;
-  P[3 * sg + c] = ww;
-  P[4 * sg + c] = pp;
-  P[5 * sg + c] = ph.gM2 * pp / rr;
+  if (t >= B.n[0] * B.n[1] * B.n[2]) return;
+  update_body<FT>(B, t, dt, stage, ph, jb);
 }
 
 hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
@@ -507,30 +292,7 @@ __global__ __launch_bounds__(TPB) void k_ghost_fill_prim(const BlockDesc* __rest
   const BlockDesc& B = blocks[blockIdx.y];
   const int t = blockIdx.x * TPB + threadIdx.x;
   if (t >= B.sg) return;
-  const int i = t % B.g[0] - NG;
-  const int r = t / B.g[0];
-  const int j = r % B.g[1] - NG;
-  const int k = r / B.g[1] - NG;
-  if (i >= 0 && i < B.n[0] && j >= 0 && j < B.n[1] && k >= 0 && k < B.n[2]) return;
-  int gl[3] = {B.o[0] + i, B.o[1] + j, B.o[2] + k};
-  int bc[3], lc[3];
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    int x = gl[a] % dom.N[a];
-    x = x < 0 ? x + dom.N[a] : x;
-    int bb = x / dom.base[a];
-    bb = bb > dom.dims[a] - 1 ? dom.dims[a] - 1 : bb;
-    bc[a] = bb;
-    lc[a] = x - bb * dom.base[a];
-  }
-  const BlockDesc& S = blocks[(bc[0] * dom.dims[1] + bc[1]) * dom.dims[2] + bc[2]];
-  const int src = gidx(S, lc[0], lc[1], lc[2]);
-  const double* __restrict__ Qs = S.Q[which];
-  double* __restrict__ Qd = B.Q[which];
-#pragma unroll
-  for (int e = 0; e < NEQ; e++) Qd[e * B.sg + t] = Qs[e * S.sg + src];
-#pragma unroll
-  for (int f = 0; f < 6; f++) B.prim[f * B.sg + t] = S.prim[f * S.sg + src];
+  ghost_body(B, blocks, dom, 0, t, which, true);
 }
 
 hipError_t launch_ghost_fill_prim(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
@@ -548,42 +310,16 @@ hipError_t launch_ghost_fill_prim(const BlockDesc* db, const BlockDesc* hb, cons
 // Non-negative doubles order like their bit patterns, so the cross-workgroup
 // max is an integer atomic max.  A NaN never wins (the reference's ternary
 // skips it) but raises the non-finite flag, as does +inf.
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    double o = __shfl_xor(v, off, 64);
-    v = o > v ? o : v;
-  }
-  return v;
-}
 
 __global__ __launch_bounds__(TPB) void k_dt(const BlockDesc* __restrict__ blocks, Phys ph,
                                             unsigned long long* smax, int* flag) {
   const BlockDesc& B = blocks[blockIdx.y];
-  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
-  const int total = n0 * n1 * n2;
-  const long long sg = B.sg;
+  const int total = B.n[0] * B.n[1] * B.n[2];
   double s_max = 0.0;
   int bad = 0;
   for (int t = blockIdx.x * TPB + threadIdx.x; t < total; t += gridDim.x * TPB) {
-    const int i = t % n0;
-    const int r = t / n0;
-    const int j = r % n1;
-    const int k = r / n1;
-    const int c = gidx(B, i, j, k);
-    const double rr = B.prim[c], uu = B.prim[sg + c], vv = B.prim[2 * sg + c], ww = B.prim[3 * sg + c];
-    const double pp = B.prim[4 * sg + c];
-    const double cs = sqrt(ph.gamma * pp / rr);
-    const double factor_v = 2.0 * B.mu[t] * ph.C_dt_v * sqrt(uu * uu + vv * vv + ww * ww) / cs / rr;
-    double t1 = fabs(uu) + divc(cs, ph.dx, ph.rdx, ph.fdx);
-    t1 += divc(divc(factor_v, ph.dx, ph.rdx, ph.fdx), ph.dx, ph.rdx, ph.fdx);
-    double t2 = fabs(vv) + divc(cs, ph.dy, ph.rdy, ph.fdy);
-    t2 += divc(divc(factor_v, ph.dy, ph.rdy, ph.fdy), ph.dy, ph.rdy, ph.fdy);
-    double t3 = fabs(ww) + divc(cs, ph.dz, ph.rdz, ph.fdz);
-    t3 += divc(divc(factor_v, ph.dz, ph.rdz, ph.fdz), ph.dz, ph.rdz, ph.fdz);
-    const double s = t1 + t2 + t3;
-    if (!(s < 1.0e308)) bad = 1;  // NaN or inf
-    s_max = (s_max < s) ? s : s_max;
+    const double s = dt_cell(B, t, ph, &bad);
+    s_max = (s_max < s) ? s : s_max;  // src/Block3d.cpp:546: a NaN never wins
   }
   s_max = wave_max(s_max);
   __shared__ double red[TPB / 64];

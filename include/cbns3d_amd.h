@@ -176,6 +176,34 @@ int cbns_kernel_times(cbns_ctx* ctx, double ms_total[CBNS_KCLASSES], int64_t lau
 int cbns_face_flux_eval(int precision, int dir, double gamma, int nfaces,
                         const double* cells, const double* prim, double* flux);
 
+/* ---- batched form ------------------------------------------------------
+ * Many independent solves (each with its own grid, physics, precision, step
+ * count and block split) advanced together: each RK stage is one persistent
+ * launch per kernel over a packed table of every live solve's blocks, and one
+ * RK3 step (dt reduction, dt/t bookkeeping on the device, three stages) is
+ * replayed as one hipGraph.  Solves are admitted while the device memory they
+ * need fits the budget.  A solve's result is bitwise the same alone or in any
+ * mix.  No counterpart in the reference (one solve per MPI job there). */
+typedef struct cbns_batch cbns_batch;
+int cbns_batch_create(uint64_t hbm_budget_bytes, int capacity, cbns_batch** out);
+int cbns_batch_destroy(cbns_batch* b);
+/* Device bytes a solve of this size and split needs in a batch. */
+int cbns_batch_bytes_needed(const cbns_params* p, const int dims[3], uint64_t* bytes);
+int cbns_batch_memory(const cbns_batch* b, uint64_t* used, uint64_t* budget);
+/* Admit a solve: Taylor-Green start when Q_aos is NULL (t from p->t_cur), else
+ * the given interior AoS state at t_cur with mu := mu0; it runs `steps` steps
+ * (or until t_end).  CBNS_ERR_NOMEM when the budget or capacity is full. */
+int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int precision, const double* Q_aos,
+                   double t_cur, double mu0, int steps, int* id);
+int cbns_batch_remove(cbns_batch* b, int id);
+/* Replay nsteps RK3 steps of every live solve (asynchronous). */
+int cbns_batch_run(cbns_batch* b, int nsteps);
+int cbns_batch_sync(cbns_batch* b);
+int cbns_batch_status(cbns_batch* b, int id, int* active, int* steps_done, double* t_cur, int* nonfinite);
+/* Ids of live solves that have stopped (finished or failed); *n_live = live solves. */
+int cbns_batch_poll(cbns_batch* b, int* ids, int max_ids, int* n_done, int* n_live);
+int cbns_batch_get_state(cbns_batch* b, int id, double* Q_aos);
+
 const char* cbns_strerror(int status);
 
 #ifdef __cplusplus

@@ -1,0 +1,91 @@
+"""Batched form (persistent kernels over a packed block table, one hipGraph
+per RK3 step): every solve equals its single-solve run bitwise, whatever it
+is batched with.  GPU only."""
+import numpy as np
+import pytest
+
+from cbns3d_cart_mpi_amd import capi as C
+
+pytestmark = pytest.mark.gpu
+GB = 1 << 30
+
+
+def single(N, steps, dims=(1, 1, 1), precision=C.CBNS_FP64, **kw):
+    with C.Solver(C.make_params(N, **kw), dims, precision=precision) as s:
+        s.init_tgv()
+        s.run(steps)
+        return s.get_state(), s.t_cur
+
+
+def test_batch_of_uneven_solves_matches_reference_and_single_runs(golden):
+    g16 = golden("ref_tgv16.npz")
+    gu = golden("ref_uneven.npz")
+    gr = golden("ref_restart16.npz")
+    with C.Batch(64 * GB) as b:
+        ids = {
+            "tgv16": b.add(C.make_params(16), steps=10),
+            "uneven": b.add(C.make_params((26, 20, 16)), dims=(3, 2, 1), steps=int(gu["nsteps"])),
+            "restart": b.add(C.make_params(16), Q=gr["Q_in"], t_cur=float(gr["t_in"]), mu0=0.0,
+                             steps=int(gr["nsteps"])),
+            "fp32": b.add(C.make_params(24, Mach=0.3, Re=400.0), dims=(1, 2, 1), precision=C.CBNS_FP32_FLUX, steps=4),
+            "odd": b.add(C.make_params((20, 16, 12), Mach=0.2, Re=800.0), steps=7),
+        }
+        b.run(12)
+        done, live = b.poll()
+        assert live == 5 and sorted(done) == sorted(ids.values())
+        st = {k: b.status(v) for k, v in ids.items()}
+        assert st["tgv16"]["steps_done"] == 10 and st["odd"]["steps_done"] == 7
+        assert st["tgv16"]["t_cur"] == float(g16["t_n10"])
+        assert np.array_equal(b.get_state(ids["tgv16"]), g16["Q_n10"])
+        assert st["uneven"]["t_cur"] == float(gu["t"])
+        assert np.array_equal(b.get_state(ids["uneven"]), gu["Q"])
+        assert st["restart"]["t_cur"] == float(gr["t_out"])
+        assert np.array_equal(b.get_state(ids["restart"]), gr["Q_out"])
+        Q, t = single(24, 4, (1, 2, 1), C.CBNS_FP32_FLUX, Mach=0.3, Re=400.0)
+        assert st["fp32"]["t_cur"] == t and np.array_equal(b.get_state(ids["fp32"]), Q)
+        Q, t = single((20, 16, 12), 7, Mach=0.2, Re=800.0)
+        assert st["odd"]["t_cur"] == t and np.array_equal(b.get_state(ids["odd"]), Q)
+
+
+def test_alone_equals_in_mix():
+    p = C.make_params(32, Mach=0.4, Re=200.0)
+    with C.Batch(16 * GB) as alone:
+        a = alone.add(p, precision=C.CBNS_FP32_FLUX, steps=6)
+        alone.run(6)
+        Qa = alone.get_state(a)
+    with C.Batch(32 * GB) as mix:
+        mix.add(C.make_params(48), steps=9)
+        x = mix.add(p, precision=C.CBNS_FP32_FLUX, steps=6)
+        mix.add(C.make_params((24, 32, 16)), dims=(2, 2, 1), precision=C.CBNS_FP32_FLUX, steps=3)
+        mix.add(C.make_params(16, Mach=0.5), steps=20)
+        mix.run(4)
+        mix.run(3)  # replays of the same captured step graph
+        assert np.array_equal(mix.get_state(x), Qa)
+
+
+def test_admission_by_memory_budget_and_reuse_of_slots():
+    p = C.make_params(32)
+    need = C.batch_bytes_needed(p)
+    with C.Batch(2 * need + need // 2, capacity=8) as b:
+        i0 = b.add(p, steps=2)
+        i1 = b.add(p, steps=2)
+        with pytest.raises(C.CbnsError) as e:
+            b.add(p, steps=2)
+        assert e.value.status == -3
+        assert b.memory[0] == 2 * need
+        b.run(3)
+        Q1 = b.get_state(i1)
+        b.remove(i0)
+        i2 = b.add(p, steps=2)
+        b.run(3)
+        assert np.array_equal(b.get_state(i2), Q1)  # same inputs, same result in a reused slot
+
+
+def test_finished_solves_stay_unchanged():
+    with C.Batch(8 * GB) as b:
+        i = b.add(C.make_params(16), steps=2)
+        b.run(2)
+        Q = b.get_state(i)
+        b.run(5)
+        assert np.array_equal(b.get_state(i), Q)
+        assert b.status(i)["steps_done"] == 2 and not b.status(i)["active"]
