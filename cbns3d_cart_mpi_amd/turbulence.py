@@ -1,0 +1,77 @@
+"""Seeded compressible decaying-turbulence initial state.
+
+A random solenoidal velocity field on the periodic [-pi, pi)^3 grid with
+energy spectrum E(k) ~ k^4 exp(-2 (k/k0)^2) (k0 = 4) and uniformly random
+phases, scaled to a turbulent Mach number M_t = u_rms / c (u_rms over all
+three components), with uniform rho = 1 and p = 1/gamma.
+
+Non-dimensionalisation matches the solver's (reference src/Block3d.cpp:259-262,
+src/Block3d.cpp:381): with the input Mach number set to 1, gamma M^2 = gamma,
+so T = gamma p / rho = 1 and the sound speed is 1.  The Reynolds number is
+chosen so that the Taylor-microscale Reynolds number of the initial field,
+Re_lambda = u' lambda / nu with u' = u_rms / sqrt(3),
+lambda = sqrt(<u^2> / <(du/dx)^2>) (spectral derivatives, averaged over the
+three components) and nu = mu / (rho Re) = 1 / Re at T = 1, equals the
+requested value:  Re = Re_lambda / (u' lambda).
+
+The generator uses numpy's PCG64 stream seeded with `seed`, so the same
+seed gives the same field on every host.
+"""
+import numpy as np
+
+
+def velocity_field(N, seed, k0=4.0):
+    """(u, v, w), each [z][y][x], unscaled, zero mean, solenoidal in the
+    spectral sense."""
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(N, 1.0 / N)
+    KZ, KY, KX = np.meshgrid(k1, k1, k1, indexing="ij")
+    k2 = KX * KX + KY * KY + KZ * KZ
+    kk = np.sqrt(k2)
+    # per-mode amplitude: E(k) / (4 pi k^2) ~ k^2 exp(-2 (k/k0)^2)
+    amp = kk * np.exp(-((kk / k0) ** 2))
+    # no Nyquist modes: their real part would not stay divergence-free
+    if N % 2 == 0:
+        nyq = (np.abs(KX) == N // 2) | (np.abs(KY) == N // 2) | (np.abs(KZ) == N // 2)
+        amp[nyq] = 0.0
+    r = rng.standard_normal((3, N, N, N)) + 1j * rng.standard_normal((3, N, N, N))
+    K = np.stack([KX, KY, KZ])
+    with np.errstate(invalid="ignore", divide="ignore"):
+        kr = (K * r).sum(axis=0) / np.where(k2 > 0, k2, 1.0)
+    uh = (r - K * kr) * amp  # projection onto the plane normal to k
+    uh[:, 0, 0, 0] = 0.0
+    u = np.real(np.fft.ifftn(uh, axes=(1, 2, 3)))
+    return u
+
+
+def taylor_reynolds_factor(u, N):
+    """u' * lambda of a velocity field on [-pi, pi)^3 (spectral derivatives)."""
+    k1 = np.fft.fftfreq(N, 1.0 / N)
+    KZ, KY, KX = np.meshgrid(k1, k1, k1, indexing="ij")
+    Ks = (KX, KY, KZ)
+    num, den = 0.0, 0.0
+    for a in range(3):
+        d = np.real(np.fft.ifftn(1j * Ks[a] * np.fft.fftn(u[a])))
+        num += np.mean(u[a] ** 2)
+        den += np.mean(d ** 2)
+    u_prime = np.sqrt(num / 3.0)
+    lam = np.sqrt(num / den)
+    return u_prime * lam
+
+
+def turbulence_state(N, seed=0, mach_t=0.6, gamma=1.4, re_lambda=100.0, k0=4.0):
+    """Returns (Q, Re): Q the interior AoS state [k][j][i][eq] (N^3 x 5) and
+    the Reynolds number giving Re_lambda for it (use with Mach = 1)."""
+    u = velocity_field(N, seed, k0)
+    urms = np.sqrt(np.mean((u * u).sum(axis=0)))
+    u *= mach_t / urms  # sound speed 1 at rho = 1, p = 1/gamma
+    Re = re_lambda / taylor_reynolds_factor(u, N)
+    rho = 1.0
+    p = 1.0 / gamma
+    Q = np.empty((N, N, N, 5))
+    Q[..., 0] = rho
+    Q[..., 1] = rho * u[0]
+    Q[..., 2] = rho * u[1]
+    Q[..., 3] = rho * u[2]
+    Q[..., 4] = p / (gamma - 1.0) + 0.5 * rho * (u * u).sum(axis=0)
+    return Q, float(Re)

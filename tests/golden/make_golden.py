@@ -14,6 +14,9 @@ Fixtures
   ref_tgv64.npz      TGV 64^3 digests after 1 and 100 steps (-np 8, 2 2 2)
   ref_restart16.npz  checkpoint-injected state (is_restart = 0) under
                      MALLOC_PERTURB_=255, 5 steps, run twice
+  ref_turb16.npz     seeded decaying turbulence (M_t = 0.6, Re_lambda = 100,
+                     cbns3d_cart_mpi_amd.turbulence, seed 7) at 16^3 injected
+                     through the checkpoint (is_restart = 0), 10 steps
   ref_vtk8.npz       TGV 8^3, 2 steps: the reference's VTK files (output000,
                      output001) and checkpoint, as bytes
   ref_point.npz      the reference's compiled weno5 / rec_weno5 /
@@ -38,17 +41,17 @@ MPIEXEC = "/opt/conda/bin/mpiexec"
 NG = 4
 
 
-def input_txt(N, nsteps, is_restart=1, t_end=20.0):
+def input_txt(N, nsteps, is_restart=1, t_end=20.0, mach="0.1", re="1600.0"):
     # the 17 positional values of src/read_input.cpp:24-72, python/taylor_green.ini values
-    vals = ["checkpoint", "0.5", "0.0", repr(t_end), "0.71", "1.4", "0.1", "1600.0", "255.556", "1.0",
+    vals = ["checkpoint", "0.5", "0.0", repr(t_end), "0.71", "1.4", mach, re, "255.556", "1.0",
             str(N[0]), str(N[1]), str(N[2]), str(nsteps), str(nsteps), "1", str(is_restart)]
     return "\n".join(vals) + "\n"
 
 
-def run_ref(N, nsteps, dims=(1, 1, 1), is_restart=1, pre=None, env_extra=None):
+def run_ref(N, nsteps, dims=(1, 1, 1), is_restart=1, pre=None, env_extra=None, **kw):
     d = tempfile.mkdtemp(prefix="ref_", dir=os.environ.get("TMPDIR", "/tmp"))
     with open(os.path.join(d, "input.txt"), "w") as f:
-        f.write(input_txt(N, nsteps, is_restart))
+        f.write(input_txt(N, nsteps, is_restart, **kw))
     if pre:
         pre(d)
     env = dict(os.environ)
@@ -261,6 +264,24 @@ def gen_point():
     print("ref_point done")
 
 
+def gen_turb16():
+    from cbns3d_cart_mpi_amd.turbulence import turbulence_state
+
+    N, seed, n = 16, 7, 10
+    Q, Re = turbulence_state(N, seed=seed)
+    Qg = np.zeros((N + 2 * NG,) * 3 + (5,))
+    Qg[NG:-NG, NG:-NG, NG:-NG] = Q
+    d, _ = run_ref((N, N, N), n, is_restart=0, mach="1.0", re=repr(Re),
+                   pre=lambda dd: write_ckpt(os.path.join(dd, "checkpoint000.dat"), 0.0, Qg),
+                   env_extra={"MALLOC_PERTURB_": "255"})
+    t, Qo = read_ckpt(os.path.join(d, "checkpoint000.dat"))
+    shutil.rmtree(d)
+    np.savez_compressed(os.path.join(HERE, "ref_turb16.npz"), Q_in=Q, Re=np.float64(Re), Mach=np.float64(1.0),
+                        seed=np.int64(seed), nsteps=np.int64(n), t_out=np.float64(t),
+                        Q_out=Qo[NG:-NG, NG:-NG, NG:-NG].copy())
+    print("ref_turb16 done, Re =", Re, "t =", t)
+
+
 def gen_vtk8():
     N = (8, 8, 8)
     d, out = run_ref(N, 2)
@@ -275,6 +296,6 @@ def gen_vtk8():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["point", "tgv16", "uneven", "restart16", "vtk8", "tgv64"]
+    which = sys.argv[1:] or ["point", "tgv16", "uneven", "restart16", "turb16", "vtk8", "tgv64"]
     for w in which:
         globals()["gen_" + w]()

@@ -255,3 +255,14 @@ def test_face_occupancy_variant_changes_nothing():
         a.run(2)
         b.run(2)
         assert np.array_equal(a.get_state(), b.get_state())
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 1, 2)])
+def test_turbulence_restart_matches_reference(golden, dims):
+    g = golden("ref_turb16.npz")
+    p = C.make_params(16, Mach=float(g["Mach"]), Re=float(g["Re"]))
+    with C.Solver(p, dims) as s:
+        s.set_state(g["Q_in"], 0.0, mu0=0.0)
+        s.run(int(g["nsteps"]))
+        assert s.t_cur == float(g["t_out"])
+        assert np.array_equal(s.get_state(), g["Q_out"])

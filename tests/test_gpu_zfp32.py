@@ -53,3 +53,22 @@ def test_fp32_flux_is_deterministic_and_decomposition_invariant():
         a.run(5)
         b.run(5)
         assert np.array_equal(a.get_state(), b.get_state())
+
+
+def test_fp32_flux_turbulence_error_vs_fp64_oracle():
+    """Seeded decaying turbulence (M_t = 0.6, shocklets) at 32^3, 40 steps."""
+    from cbns3d_cart_mpi_amd.turbulence import turbulence_state
+
+    Q0, Re = turbulence_state(32, seed=11)
+    b = O.Block(O.make_params(32, Mach=1.0, Re=Re))
+    Qg = np.zeros((40, 40, 40, 5))
+    Qg[4:-4, 4:-4, 4:-4] = Q0
+    b.set_state(Qg, 0.0, mu0=0.0)
+    b.run(40)
+    with C.Solver(C.make_params(32, Mach=1.0, Re=Re), precision=C.CBNS_FP32_FLUX) as s:
+        s.set_state(Q0, 0.0, mu0=0.0)
+        s.run(40)
+        Q = s.get_state()
+    for name, (l2, li) in rel_errors(Q, b.Q_interior(0)).items():
+        print(f"FP32-flux turbulence 32^3 40 steps {name}: rel L2 {l2:.3e} rel Linf {li:.3e}")
+        assert l2 <= 1e-5, name

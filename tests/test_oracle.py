@@ -221,3 +221,30 @@ def test_diagnostics_initial_kinetic_energy():
     assert abs(ke - 0.125) < 2e-4
     # omega_z = 2 cos x cos y cos z etc.: <0.5 |omega|^2> = 0.375 for the exact field
     assert abs(ens - 0.375) / 0.375 < 0.01
+
+
+def test_oracle_turbulence_restart_bitexact_vs_reference(golden):
+    g = golden("ref_turb16.npz")
+    b = O.Block(O.make_params(16, Mach=float(g["Mach"]), Re=float(g["Re"])))
+    Qg = np.zeros((16 + 2 * NG,) * 3 + (5,))
+    Qg[NG:-NG, NG:-NG, NG:-NG] = g["Q_in"]
+    b.set_state(Qg, 0.0, mu0=0.0)
+    b.run(int(g["nsteps"]))
+    assert b.t_cur == float(g["t_out"])
+    assert np.array_equal(b.Q_interior(0), g["Q_out"])
+
+
+def test_turbulence_generator_is_seeded_solenoidal_and_scaled():
+    from cbns3d_cart_mpi_amd.turbulence import turbulence_state
+
+    Q, Re = turbulence_state(16, seed=7)
+    Q2, Re2 = turbulence_state(16, seed=7)
+    assert np.array_equal(Q, Q2) and Re == Re2
+    u = Q[..., 1:4] / Q[..., :1]
+    assert abs(np.sqrt(np.mean((u * u).sum(-1))) - 0.6) < 1e-12  # M_t with c = 1
+    N = 16
+    k1 = np.fft.fftfreq(N, 1.0 / N)
+    KZ, KY, KX = np.meshgrid(k1, k1, k1, indexing="ij")
+    div = sum(np.fft.ifftn(1j * K * np.fft.fftn(u[..., a])) for a, K in enumerate((KX, KY, KZ)))
+    assert np.abs(div).max() < 1e-12
+    assert np.all(Q[..., 0] == 1.0)
