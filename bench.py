@@ -43,6 +43,11 @@ def parse():
     ap.add_argument("--band", default="auto", help="rows per work band, or 'auto' to tune")
     ap.add_argument("--overlap", default="auto", choices=["auto", "on", "off"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--sweep-steps", type=int, default=300)
+    ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--cpu-max-steps", type=int, default=3)
     ap.add_argument("--cpu-budget-s", type=float, default=25.0)
     return ap.parse_args()
@@ -83,6 +88,74 @@ def cpu_baseline(n, max_steps, budget_s):
             "sample": f"{k} TGV {n}^3 FP64 steps after 1 warm-up step, oracle/ C restatement with OpenMP"}
 
 
+FIELDS = ("rho", "rho_u", "rho_v", "rho_w", "E")
+
+
+def fp32_leg(C, N, dims, steps_total, steps_timed, Q64):
+    """The FP32-flux form on the same workload: throughput over steps_timed
+    steps, and its per-field error against the FP64 state at the same step
+    count (relative L2 = RMS difference / RMS of the FP64 field; relative
+    L-inf = max difference / max |FP64 field|)."""
+    import numpy as np
+
+    s = C.Solver(C.make_params(N), dims, precision=C.CBNS_FP32_FLUX)
+    s.init_tgv()
+    s.run(steps_total - steps_timed)
+    s.sync()
+    t0 = time.perf_counter()
+    s.run(steps_timed)
+    el = time.perf_counter() - t0
+    Q = s.get_state()
+    s.close()
+    err = {}
+    for e, nm in enumerate(FIELDS):
+        d = Q[..., e] - Q64[..., e]
+        ref = Q64[..., e]
+        err[nm] = {"rel_l2": float(np.sqrt(np.mean(d * d) / np.mean(ref * ref))),
+                   "rel_linf": float(np.abs(d).max() / np.abs(ref).max())}
+    v = N ** 3 * steps_timed / el / 1e6
+    return {"value": round(v, 2), "unit": "Mcell-steps/s", "ms_per_step": round(el / steps_timed * 1e3, 3),
+            "error_vs_fp64_after_steps": steps_total, "error": err}
+
+
+def sweep_leg(steps):
+    """64 independent solves (grids {32,48,64,96,128}^3, Re log-U[100,3200],
+    Mach U[0.1,0.5], half Taylor-Green / half turbulence), one batch, one
+    persistent launch per kernel per stage, one hipGraph per step."""
+    from cbns3d_cart_mpi_amd import serve as S
+
+    out = {}
+    for prec in ("fp64", "fp32"):
+        reqs = S.make_requests(64, seed=2024, steps=(steps, steps), precision=prec)
+        r = S.run_sweep(reqs)
+        out[prec] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+    out["fp32_over_fp64"] = round(out["fp32"]["mcell_steps_per_s"] / out["fp64"]["mcell_steps_per_s"], 3)
+    out["config"] = f"64 solves x {steps} steps; latency = batch start to all results (one batch)"
+    return out
+
+
+def serve_leg(n):
+    """Continuous batching of a Poisson stream (reduced size): n requests, grids
+    {32..192}^3 with weights 1/N, steps U[20, 100], FP64 / FP32-flux alternating,
+    200 GB budget, offered load 85% of the measured batch capacity; against the
+    same requests solved one at a time as plain FP64."""
+    from cbns3d_cart_mpi_amd import serve as S
+
+    grids = (32, 48, 64, 96, 128, 160, 192)
+    w = [1.0 / g for g in grids]
+    probe = S.make_requests(16, 7, grids, w, (20, 20))
+    cap = S.run_sweep(probe)["mcell_steps_per_s"] * 1e6
+    import numpy as np
+    mean_work = np.mean([r.work for r in S.make_requests(n, 2024, grids, w, (20, 100))])
+    rate = 0.85 * cap / mean_work
+    reqs = S.make_requests(n, 2024, grids, w, (20, 100), rate=rate)
+    srv = S.serve(reqs, chunk=10)
+    seq = S.sequential_fp64(reqs)
+    rnd = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
+    return {"requests": n, "rate_per_s": round(rate, 3), "served": rnd(srv), "sequential_fp64": rnd(seq),
+            "config": "reduced: steps U[20,100] instead of U[200,2000]; 64 instead of 1000 requests"}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -119,7 +192,15 @@ def main():
     assert len(dts) == a.steps
     ke, en = s.diagnostics()
     mem = s.memory_bytes
-    s.close()
+    steps_total = a.warmup + (len(tuned["ms_per_step"]) * 3 if tuned else 0) + a.steps
+    fp32 = None
+    if not a.no_fp32:
+        Q64 = s.get_state()
+        s.close()
+        fp32 = fp32_leg(C, N, dims, steps_total, a.steps, Q64)
+        del Q64
+    else:
+        s.close()
 
     cells = N ** 3
     value = cells * a.steps / el / 1e6
@@ -156,6 +237,12 @@ def main():
         "autotune": tuned,
         "ke_enstrophy_end": [ke, en],
     }
+    if fp32:
+        line["fp32_flux"] = fp32
+    if not a.no_sweep:
+        line["sweep"] = sweep_leg(a.sweep_steps)
+    if not a.no_serve:
+        line["serving"] = serve_leg(a.serve_requests)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(N, a.cpu_max_steps, a.cpu_budget_s)
     print(json.dumps(line), flush=True)
