@@ -149,7 +149,7 @@ def serve_leg(n):
     mean_work = np.mean([r.work for r in S.make_requests(n, 2024, grids, w, (20, 100))])
     rate = 0.85 * cap / mean_work
     reqs = S.make_requests(n, 2024, grids, w, (20, 100), rate=rate)
-    srv = S.serve(reqs, chunk=10)
+    srv = S.serve(reqs, chunk=4)
     seq = S.sequential_fp64(reqs)
     rnd = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
     return {"requests": n, "rate_per_s": round(rate, 3), "served": rnd(srv), "sequential_fp64": rnd(seq),

@@ -82,6 +82,7 @@ _SIGS = {
     "cbns_batch_add": (_I, [_VP, ctypes.POINTER(Params), _PI, _I, _PD, _D, _D, _I, _PI]),
     "cbns_batch_remove": (_I, [_VP, _I]),
     "cbns_batch_run": (_I, [_VP, _I]),
+    "cbns_batch_set_budgets": (_I, [_VP, _PI, _PI, _I]),
     "cbns_batch_sync": (_I, [_VP]),
     "cbns_batch_status": (_I, [_VP, _I, _PI, _PI, _PD, _PI]),
     "cbns_batch_poll": (_I, [_VP, _PI, _I, _PI, _PI]),
@@ -355,6 +356,12 @@ class Batch:
 
     def sync(self):
         check(lib().cbns_batch_sync(self._h), "cbns_batch_sync")
+
+    def set_budgets(self, budgets):
+        """{id: steps} granted until the next call."""
+        ids = (_I * len(budgets))(*budgets.keys())
+        st = (_I * len(budgets))(*budgets.values())
+        check(lib().cbns_batch_set_budgets(self._h, ids, st, len(budgets)), "cbns_batch_set_budgets")
 
     def status(self, i):
         a, n, nf = _I(), _I(), _I()

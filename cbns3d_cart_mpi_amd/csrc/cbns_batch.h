@@ -18,6 +18,8 @@ struct InstDesc {
   int active;        // 1 while stepping; 0 when idle, finished or failed
   int nonfinite;     // set by the dt reduction, ends the solve
   int steps_done, steps_target;
+  int budget;        // steps it may still take before the host grants more
+  int stepping;      // whether the current step advances it (set by finalize)
   double t_cur, t_end, dt;
   unsigned long long smax;  // dt reduction target, reset after each use
 };
@@ -48,5 +50,6 @@ hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTab
 hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, hipStream_t s);
 hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, hipStream_t s);
 hipError_t batch_launch_finalize(InstDesc* insts, int ninst, hipStream_t s);
+hipError_t batch_launch_set_budget(InstDesc* insts, const int2* pairs, int n, hipStream_t s);
 
 }  // namespace cbns

@@ -28,7 +28,7 @@ __global__ __launch_bounds__(TPB, 2) void kb_face(const BlockDesc* __restrict__ 
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
-    if (!I.active) continue;
+    if (!I.stepping) continue;
     const int t = tr.tile * TPB + threadIdx.x;
     if (t >= face_count(B, D)) continue;
     if (P == 0) face_body<D>(B, t, which, jb, I.ph);
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(TPB) void kb_visc(const BlockDesc* __restrict__ blo
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
-    if (!I.active) continue;
+    if (!I.stepping) continue;
     const int t = tr.tile * TPB + threadIdx.x;
     if (t >= visc_count(B)) continue;
     visc_flux_body(B, t, I.ph, jb);
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(TPB) void kb_update(const BlockDesc* __restrict__ b
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
-    if (!I.active) continue;
+    if (!I.stepping) continue;
     const int t = tr.tile * TPB + threadIdx.x;
     if (t >= B.n[0] * B.n[1] * B.n[2]) continue;
     update_body<FT>(B, t, I.dt, stage, I.ph, jb);
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(TPB) void kb_ghost(const BlockDesc* __restrict__ bl
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
-    if (!I.active) continue;
+    if (!I.stepping) continue;
     const int t = tr.tile * TPB + threadIdx.x;
     if (t >= B.sg) continue;
     ghost_body(B, blocks, I.dom, I.block0, t, which, true);
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(TPB) void kb_dt(const BlockDesc* __restrict__ block
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const int inst = B.inst;
-    if (!insts[inst].active) continue;
+    if (!insts[inst].active || insts[inst].budget <= 0) continue;
     if (inst != cur) {
       if (cur >= 0) {
         const double m = wave_max(acc);
@@ -127,6 +127,7 @@ __global__ void kb_finalize(InstDesc* insts, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   InstDesc& I = insts[i];
+  I.stepping = 0;
   if (!I.active) return;
   const unsigned long long bits = I.smax;
   I.smax = 0ull;
@@ -135,12 +136,21 @@ __global__ void kb_finalize(InstDesc* insts, int n) {
     I.dt = 0.0;
     return;
   }
+  if (I.budget <= 0) return;  // paused until the host grants more steps
   const double s_max = __longlong_as_double((long long)bits);
   double dt = I.ph.CFL / s_max;
   if (I.t_end - I.t_cur < dt) dt = I.t_end - I.t_cur;
   I.t_cur += dt;
   I.dt = dt;
   I.steps_done += 1;
+  I.budget -= 1;
+  I.stepping = 1;
+}
+
+// Step budgets granted by the host: pairs of (solve id, steps).
+__global__ void kb_set_budget(InstDesc* insts, const int2* pairs, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) insts[pairs[k].x].budget = pairs[k].y;
 }
 
 hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int dir, int which, int jb,
@@ -184,6 +194,12 @@ hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTabl
 hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
   hipLaunchKernelGGL(kb_dt, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt);
+  return hipGetLastError();
+}
+
+hipError_t batch_launch_set_budget(InstDesc* insts, const int2* pairs, int n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_set_budget, dim3((n + 63) / 64), dim3(64), 0, s, insts, pairs, n);
   return hipGetLastError();
 }
 

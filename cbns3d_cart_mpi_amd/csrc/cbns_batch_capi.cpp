@@ -52,6 +52,8 @@ struct cbns_batch {
   int tiles_cap[T_COUNT] = {0};
   TileTable tt[T_COUNT] = {};
   double* d_part = nullptr;  // global-sum partials (FP32 reference states)
+  int2* d_pairs = nullptr;   // staging of step budgets
+  int pairs_cap = 0;
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -244,6 +246,7 @@ int cbns_batch_destroy(cbns_batch* b) {
   if (b->d_blocks) (void)hipFree(b->d_blocks);
   if (b->d_insts) (void)hipFree(b->d_insts);
   if (b->d_part) (void)hipFree(b->d_part);
+  if (b->d_pairs) (void)hipFree(b->d_pairs);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return CBNS_OK;
@@ -397,6 +400,7 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
   D.precision = precision;
   D.active = 1;
   D.steps_target = steps;
+  D.budget = 0x7fffffff;  // unbounded until the host sets budgets
   D.t_cur = t_cur;
   D.t_end = p->t_end;
   if (hipMemcpy(b->d_insts + id, &D, sizeof(D), hipMemcpyHostToDevice) != hipSuccess) {
@@ -432,6 +436,22 @@ int cbns_batch_run(cbns_batch* b, int nsteps) {
   return CBNS_OK;
 }
 
+int cbns_batch_set_budgets(cbns_batch* b, const int* ids, const int* steps, int n) {
+  if (!b || n < 0 || (n > 0 && (!ids || !steps))) return CBNS_ERR_ARG;
+  std::vector<int2> pr(n);
+  for (int k = 0; k < n; k++) {
+    if (ids[k] < 0 || ids[k] >= b->cap || !b->inst[ids[k]].live || steps[k] < 0) return CBNS_ERR_ARG;
+    pr[k] = make_int2(ids[k], steps[k]);
+  }
+  if (n == 0) return CBNS_OK;
+  CK(hipStreamSynchronize(b->stream));
+  int st = grow((void**)&b->d_pairs, &b->pairs_cap, n, sizeof(int2));
+  if (st) return st;
+  CK(hipMemcpy(b->d_pairs, pr.data(), sizeof(int2) * n, hipMemcpyHostToDevice));
+  CK(batch_launch_set_budget(b->d_insts, b->d_pairs, n, b->stream));
+  return CBNS_OK;
+}
+
 int cbns_batch_sync(cbns_batch* b) {
   if (!b) return CBNS_ERR_ARG;
   CK(hipStreamSynchronize(b->stream));
@@ -459,7 +479,7 @@ int cbns_batch_poll(cbns_batch* b, int* ids, int max_ids, int* n_done, int* n_li
   for (int id = 0; id < b->cap; id++) {
     if (!b->inst[id].live) continue;
     nl++;
-    const bool done = !D[id].active;
+    const bool done = !D[id].active || D[id].steps_done >= D[id].steps_target;
     if (done) {
       if (nd < max_ids) ids[nd] = id;
       nd++;

@@ -14,10 +14,11 @@ Requests are drawn from a seeded stream:
   * FP64 or FP32-flux (serving: alternating).
 
 Serving is continuous batching: requests arrive on a Poisson clock; each is
-admitted into the device batch as soon as its state fits the HBM budget,
-stays resident there, and advances with every other live solve in the same
-persistent launches (one hipGraph per step); completed solves are collected
-and removed between graph-replay chunks, freeing room for waiting requests.
+admitted as soon as its state fits the HBM budget, stays resident on the
+device, and advances with the other live solves of its size class in the
+same persistent launches (one hipGraph per step per class), the classes
+running concurrently; completed solves are collected and removed between
+graph-replay chunks, freeing room for waiting requests.
 Time-to-completion is measured from arrival to the host having the result.
 The baseline processes the same requests one at a time, first come first
 served, as plain FP64 single solves.
@@ -51,6 +52,9 @@ class Request:
     admitted: float = math.nan
     done: float = math.nan
     t_final: float = math.nan
+    lane: int = -1       # size class it was served in
+    bid: int = -1        # instance id in that class's batch
+    ran: int = 0         # steps issued to it so far
     payload: object = field(default=None, repr=False)
     result: object = field(default=None, repr=False)
 
@@ -121,15 +125,30 @@ def run_sweep(reqs, budget=200 * GB):
             "p50_s": percentile([r.done for r in reqs], 50), "p99_s": percentile([r.done for r in reqs], 99)}
 
 
-def serve(reqs, budget=200 * GB, chunk=10, capacity=1024, collect=True):
-    """Continuous batching over a Poisson request stream (arrival times in s)."""
+LANES = ((0, 64), (65, 128), (129, 1 << 30))  # size classes by grid edge N
+
+
+def serve(reqs, budget=200 * GB, chunk=4, capacity=1024, collect=True, lanes=LANES):
+    """Continuous batching over a Poisson request stream (arrival times in s).
+
+    Solves are grouped into size-class lanes, each a device batch with its own
+    stream and step graph, so a small solve never advances in lockstep with a
+    large one.  Between chunks the server admits waiting requests (first come,
+    first admitted, while the shared HBM budget allows), then issues every
+    lane's chunk asynchronously so the lanes run concurrently on the GPU: the
+    lane holding the largest solve replays `chunk` steps and a lane whose
+    largest solve has n cells replays about chunk * n_max / n steps.  Finished
+    solves are collected and removed after the chunk, freeing memory."""
     for r in reqs:
         r.payload = prepare(r)
     pending = sorted(reqs, key=lambda r: r.arrival)
     waiting = []
-    live = {}
     done = []
-    with C.Batch(budget, capacity=capacity) as b:
+    batches = [C.Batch(budget, capacity=capacity) for _ in lanes]
+    live = [dict() for _ in lanes]
+    used = [0]
+    host = {"admit": 0.0, "issue": 0.0, "poll": 0.0, "collect": 0.0}
+    try:
         t0 = time.perf_counter()
         k = 0
         while len(done) < len(reqs):
@@ -137,38 +156,64 @@ def serve(reqs, budget=200 * GB, chunk=10, capacity=1024, collect=True):
             while k < len(pending) and pending[k].arrival <= now:
                 waiting.append(pending[k])
                 k += 1
+            ta = time.perf_counter()
             while waiting:  # first come, first admitted, while memory allows
                 r = waiting[0]
                 p, Q = r.payload
-                try:
-                    i = b.add(p, precision=r.precision, Q=Q, t_cur=0.0, mu0=0.0, steps=r.steps)
-                except C.CbnsError as e:
-                    if e.status != -3 or not live:  # a request that cannot fit even alone
-                        raise
+                need = C.batch_bytes_needed(p)
+                nlive = sum(len(x) for x in live)
+                if used[0] + need > budget:
+                    if not nlive:
+                        raise RuntimeError(f"request {r.rid} does not fit the budget alone")
                     break
+                li = next(n for n, (lo, hi) in enumerate(lanes) if lo <= r.N <= hi)
+                i = batches[li].add(p, precision=r.precision, Q=Q, t_cur=0.0, mu0=0.0, steps=r.steps)
+                used[0] += need
                 r.admitted = time.perf_counter() - t0
-                live[i] = r
+                r.lane, r.bid, r.ran = li, i, 0
+                live[li][i] = r
                 waiting.pop(0)
-            if not live:
+            tb = time.perf_counter()
+            host["admit"] += tb - ta
+            active = [li for li in range(len(lanes)) if live[li]]
+            if not active:
                 if k < len(pending):
                     time.sleep(max(0.0, pending[k].arrival - (time.perf_counter() - t0)))
                 continue
-            b.run(chunk)
-            fin, _ = b.poll()
-            for i in fin:
-                r = live.pop(i)
-                st = b.status(i)
-                r.t_final = st["t_cur"]
-                if collect:
-                    r.result = b.get_state(i)
-                r.done = time.perf_counter() - t0
-                b.remove(i)
-                done.append(r)
+            n_all = max(r.N ** 3 for li in active for r in live[li].values())
+            for li in active:  # asynchronous: the lanes overlap on the device
+                n_lane = max(r.N ** 3 for r in live[li].values())
+                left = max(r.steps - r.ran for r in live[li].values())
+                nrep = max(1, min(left, (chunk * n_all) // n_lane))
+                batches[li].run(nrep)
+                for r in live[li].values():
+                    r.ran += nrep
+            tc = time.perf_counter()
+            host["issue"] += tc - tb
+            for li in active:
+                td = time.perf_counter()
+                fin, _ = batches[li].poll()
+                te = time.perf_counter()
+                host["poll"] += te - td
+                for i in fin:
+                    r = live[li].pop(i)
+                    r.t_final = batches[li].status(i)["t_cur"]
+                    if collect:
+                        r.result = batches[li].get_state(i)
+                    r.done = time.perf_counter() - t0
+                    batches[li].remove(i)
+                    used[0] -= C.batch_bytes_needed(r.payload[0])
+                    done.append(r)
+                host["collect"] += time.perf_counter() - te
         el = time.perf_counter() - t0
+    finally:
+        for b in batches:
+            b.close()
     lat = [r.done - r.arrival for r in reqs]
     work = sum(r.work for r in reqs)
     return {"wall_s": el, "cell_steps": work, "mcell_steps_per_s": work / el / 1e6,
-            "p50_s": percentile(lat, 50), "p99_s": percentile(lat, 99), "mean_s": float(np.mean(lat))}
+            "p50_s": percentile(lat, 50), "p99_s": percentile(lat, 99), "mean_s": float(np.mean(lat)),
+            "host_s": {k: round(v, 4) for k, v in host.items()}}
 
 
 def sequential_fp64(reqs, precision=C.CBNS_FP64):
@@ -209,7 +254,7 @@ def main(argv=None):
     ap.add_argument("--rate", type=float, default=None, help="arrivals per second (serve)")
     ap.add_argument("--load", type=float, default=0.85, help="offered load vs batch capacity when --rate is unset")
     ap.add_argument("--budget-gb", type=float, default=200.0)
-    ap.add_argument("--chunk", type=int, default=10)
+    ap.add_argument("--chunk", type=int, default=4)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-baseline", action="store_true")
     a = ap.parse_args(argv)

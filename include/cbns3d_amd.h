@@ -196,11 +196,17 @@ int cbns_batch_memory(const cbns_batch* b, uint64_t* used, uint64_t* budget);
 int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int precision, const double* Q_aos,
                    double t_cur, double mu0, int steps, int* id);
 int cbns_batch_remove(cbns_batch* b, int id);
-/* Replay nsteps RK3 steps of every live solve (asynchronous). */
+/* Replay nsteps RK3 steps of every live solve (asynchronous).  A solve with a
+ * step budget (below) takes at most that many of them, then pauses. */
 int cbns_batch_run(cbns_batch* b, int nsteps);
+/* Grant each listed solve `steps` more steps (replacing its budget).  Solves
+ * start unbounded.  Lets a scheduler give small solves many cheap steps and
+ * large ones few within one run. */
+int cbns_batch_set_budgets(cbns_batch* b, const int* ids, const int* steps, int n);
 int cbns_batch_sync(cbns_batch* b);
 int cbns_batch_status(cbns_batch* b, int id, int* active, int* steps_done, double* t_cur, int* nonfinite);
-/* Ids of live solves that have stopped (finished or failed); *n_live = live solves. */
+/* Ids of live solves that are done (took their steps, reached t_end or failed);
+ * *n_live = live solves. */
 int cbns_batch_poll(cbns_batch* b, int* ids, int max_ids, int* n_done, int* n_live);
 int cbns_batch_get_state(cbns_batch* b, int id, double* Q_aos);
 

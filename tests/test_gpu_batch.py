@@ -89,3 +89,23 @@ def test_finished_solves_stay_unchanged():
         b.run(5)
         assert np.array_equal(b.get_state(i), Q)
         assert b.status(i)["steps_done"] == 2 and not b.status(i)["active"]
+
+
+def test_step_budgets_pause_and_resume_without_changing_results():
+    p = C.make_params(16)
+    with C.Batch(8 * GB) as b:
+        i = b.add(p, steps=6)
+        j = b.add(C.make_params(24), steps=6)
+        b.set_budgets({i: 2, j: 5})
+        b.run(5)
+        assert b.status(i)["steps_done"] == 2 and b.status(j)["steps_done"] == 5
+        b.set_budgets({i: 4, j: 1})
+        b.run(4)
+        assert b.status(i)["steps_done"] == 6 and b.status(j)["steps_done"] == 6
+        done, _ = b.poll()
+        assert sorted(done) == sorted([i, j])
+        Qi, Qj = b.get_state(i), b.get_state(j)
+    Q, _ = single(16, 6)
+    assert np.array_equal(Qi, Q)
+    Q, _ = single(24, 6)
+    assert np.array_equal(Qj, Q)
