@@ -41,11 +41,17 @@ def parse():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--dims", default="1,1,1")
     ap.add_argument("--band", default="auto", help="rows per work band, or 'auto' to tune")
-    ap.add_argument("--overlap", default="auto", choices=["auto", "on", "off"])
+    ap.add_argument("--schedule", default="auto",
+                    help="stage schedule: overlap | serial, or 'auto' to tune")
+    ap.add_argument("--occ", default="auto", choices=["auto", "2", "3"],
+                    help="face-kernel waves per SIMD, or 'auto' to tune")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-turb", action="store_true")
+    ap.add_argument("--turb-n", type=int, default=128)
+    ap.add_argument("--turb-steps", type=int, default=500)
     ap.add_argument("--sweep-steps", type=int, default=300)
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--cpu-max-steps", type=int, default=3)
@@ -98,9 +104,16 @@ def fp32_leg(C, N, dims, steps_total, steps_timed, Q64):
     L-inf = max difference / max |FP64 field|)."""
     import numpy as np
 
+    from cbns3d_cart_mpi_amd import autotune
+
     s = C.Solver(C.make_params(N), dims, precision=C.CBNS_FP32_FLUX)
     s.init_tgv()
-    s.run(steps_total - steps_timed)
+    pre = steps_total - steps_timed  # so that the error is taken at the FP64 leg's step count
+    best = None
+    if pre >= 3 * 2 * len(autotune.SCHEDULES):
+        best, tms = autotune.tune(s, steps=2, bands=(16, 32), schedules=autotune.SCHEDULES, occs=(2,))
+        pre -= 3 * len(tms)
+    s.run(pre)
     s.sync()
     t0 = time.perf_counter()
     s.run(steps_timed)
@@ -115,7 +128,8 @@ def fp32_leg(C, N, dims, steps_total, steps_timed, Q64):
                    "rel_linf": float(np.abs(d).max() / np.abs(ref).max())}
     v = N ** 3 * steps_timed / el / 1e6
     return {"value": round(v, 2), "unit": "Mcell-steps/s", "ms_per_step": round(el / steps_timed * 1e3, 3),
-            "error_vs_fp64_after_steps": steps_total, "error": err}
+            "error_vs_fp64_after_steps": steps_total, "error": err,
+            "autotune": {"band": best[0], "schedule": best[1]} if best else None}
 
 
 def sweep_leg(steps):
@@ -124,13 +138,52 @@ def sweep_leg(steps):
     persistent launch per kernel per stage, one hipGraph per step."""
     from cbns3d_cart_mpi_amd import serve as S
 
+    import numpy as np
+
     out = {}
+    ref = {}
+    worst = []
     for prec in ("fp64", "fp32"):
         reqs = S.make_requests(64, seed=2024, steps=(steps, steps), precision=prec)
-        r = S.run_sweep(reqs)
+        r = S.run_sweep(reqs, collect=True)
         out[prec] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        for q in reqs:
+            if prec == "fp64":
+                ref[q.rid] = q.result
+            else:  # per-solve error of the FP32-flux form against the same solve in FP64
+                Q64 = ref.pop(q.rid)
+                worst.append(max(float(np.sqrt(np.mean((q.result[..., e] - Q64[..., e]) ** 2) /
+                                               np.mean(Q64[..., e] ** 2))) for e in range(5)))
+            q.result = None
     out["fp32_over_fp64"] = round(out["fp32"]["mcell_steps_per_s"] / out["fp64"]["mcell_steps_per_s"], 3)
+    out["fp32_rel_l2_worst_field_per_solve"] = {"max": float(np.max(worst)), "median": float(np.median(worst))}
     out["config"] = f"64 solves x {steps} steps; latency = batch start to all results (one batch)"
+    return out
+
+
+def turbulence_leg(N, steps, seed=2024):
+    """Decaying turbulence (M_t 0.6, Re_lambda 100, seeded) at N^3: the plain
+    FP64 path and the FP32-flux form over `steps` steps from the same state;
+    throughput of each, per-field error of FP32-flux against FP64 and the
+    dilatation-spectrum difference."""
+    from cbns3d_cart_mpi_amd import capi as C
+    from cbns3d_cart_mpi_amd.turbulence import precision_errors, turbulence_state
+
+    Q0, Re = turbulence_state(N, seed=seed)
+    p = C.make_params(N, Mach=1.0, Re=Re, nstep_max=steps, t_end=1.0e3)
+    out, Q = {}, {}
+    for name, prec in (("fp64", C.CBNS_FP64), ("fp32", C.CBNS_FP32_FLUX)):
+        with C.Solver(p, precision=prec) as s:
+            s.set_state(Q0, 0.0, mu0=0.0)
+            s.run(1)  # untimed first step
+            s.sync()
+            t0 = time.perf_counter()
+            s.run(steps - 1)
+            el = time.perf_counter() - t0
+            Q[name] = s.get_state()
+            out[name] = {"mcell_steps_per_s": round(N ** 3 * (steps - 1) / el / 1e6, 2), "t_end": s.t_cur}
+    out["fp32_error_vs_fp64"] = precision_errors(Q["fp32"], Q["fp64"], N)
+    out["config"] = f"{N}^3, seed {seed}, Mach 1 (sound speed 1), M_t 0.6, Re {Re:.4g}, {steps} steps"
     return out
 
 
@@ -171,17 +224,20 @@ def main():
     s.init_tgv()
     s.run(a.warmup)
     tuned = None
-    if a.band == "auto" or a.overlap == "auto":
-        from cbns3d_cart_mpi_amd import autotune
+    from cbns3d_cart_mpi_amd import autotune
+
+    if "auto" in (a.band, a.schedule, a.occ):
         bands = autotune.BANDS if a.band == "auto" else (int(a.band),)
-        ovs = (True, False) if a.overlap == "auto" else (a.overlap == "on",)
-        best, tms = autotune.tune(s, steps=2, bands=bands, overlaps=ovs)
-        tuned = {"band": best[0], "overlap": best[1], "face_waves_per_simd": best[2],
-                 "ms_per_step": {f"{b}/{'on' if o else 'off'}/{w}": round(v * 1e3, 3)
-                                 for (b, o, w), v in tms.items()}}
+        scheds = autotune.SCHEDULES if a.schedule == "auto" else (a.schedule,)
+        occs = (2, 3) if a.occ == "auto" else (int(a.occ),)
+        best, tms = autotune.tune(s, steps=2, bands=bands, schedules=scheds, occs=occs)
+        tuned = {"band": best[0], "schedule": best[1], "face_waves_per_simd": best[2],
+                 "ms_per_step (band/schedule/waves)": {
+                     f"{b}/{sc}/{w}": round(v * 1e3, 3) for (b, sc, w), v in tms.items()}}
     else:
         s.set_band(int(a.band))
-        s.set_overlap(a.overlap == "on")
+        autotune.apply_schedule(s, a.schedule)
+        s.set_face_occupancy(int(a.occ))
     s.sync()
     s.set_profiling(True)
     t0 = time.perf_counter()
@@ -192,7 +248,7 @@ def main():
     assert len(dts) == a.steps
     ke, en = s.diagnostics()
     mem = s.memory_bytes
-    steps_total = a.warmup + (len(tuned["ms_per_step"]) * 3 if tuned else 0) + a.steps
+    steps_total = a.warmup + (len(tms) * 3 if tuned else 0) + a.steps
     fp32 = None
     if not a.no_fp32:
         Q64 = s.get_state()
@@ -205,15 +261,21 @@ def main():
     cells = N ** 3
     value = cells * a.steps / el / 1e6
     ms_step = el / a.steps * 1e3
-    face_ms = sum(kt[k][0] for k in ("face_x", "face_y", "face_z"))
-    face_n = sum(kt[k][1] for k in ("face_x", "face_y", "face_z"))
+    # with the side-stream schedule the x sweep shares the GPU with the viscous
+    # kernel, so its launch time is not the sweep's own; use y and z there
+    overlap = bool(tuned and tuned["schedule"] == "overlap") or a.schedule == "overlap"
+    sweeps = ("face_y", "face_z") if overlap else ("face_x", "face_y", "face_z")
+    face_ms = sum(kt[k][0] for k in sweeps)
+    face_n = sum(kt[k][1] for k in sweeps)
     avg_face_ms = face_ms / max(face_n, 1)
     faces_per_launch = (N + 1) * N * N  # single block; per-axis launch covers every block
     achieved = FLOP_PER_FACE * faces_per_launch / (avg_face_ms * 1e-3) / 1e12
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": read_traffic(),
-            "kernel": "k_face_flux (x, y, z), FP64 VALU; the FP64 peak is the same for VALU and matrix cores",
-            "avg_launch_ms": round(avg_face_ms, 4)}
+            "kernel": f"k_face_flux ({', '.join(k[-1] for k in sweeps)}); FP64 VALU; "
+                      "the FP64 peak is the same for VALU and matrix cores",
+            "avg_launch_ms": round(avg_face_ms, 4),
+            "algorithmic_flop_per_launch": FLOP_PER_FACE * faces_per_launch}
     line = {
         "metric": "Mcell-steps/s TGV 256^3 FP64 (plain path)",
         "value": round(value, 2),
@@ -231,14 +293,16 @@ def main():
                    "grid": [N, N, N], "blocks": list(dims), "Re": 1600.0, "Mach": 0.1, "CFL": 0.5},
         "roofline": roof,
         "algorithmic_tflops_all_kernels": round(FLOP_PER_CELL_STEP * value * 1e6 / 1e12, 3),
-        "kernels_avg_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items()},
-        "kernel_share": {k: round(v[0] / (el * 1e3), 4) for k, v in kt.items()},
+        "kernels_avg_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items() if v[1]},
+        "kernel_share": {k: round(v[0] / (el * 1e3), 4) for k, v in kt.items() if v[1]},
         "device_bytes": mem,
         "autotune": tuned,
         "ke_enstrophy_end": [ke, en],
     }
     if fp32:
         line["fp32_flux"] = fp32
+    if not a.no_turb:
+        line["turbulence"] = turbulence_leg(a.turb_n, a.turb_steps)
     if not a.no_sweep:
         line["sweep"] = sweep_leg(a.sweep_steps)
     if not a.no_serve:

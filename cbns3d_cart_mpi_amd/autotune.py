@@ -5,13 +5,16 @@ grid or which stream a kernel runs on; results are bitwise identical for
 every choice (the tests check it), so the tuner may pick freely by time.
 
 Knobs: rows per band of the stencil kernels' work order (XCD/L2 locality),
-the side-stream overlap of the viscous kernels with the face fluxes, and the
-register-allocation variant (waves per SIMD) of the FP64 face kernel.
+the register-allocation variant (waves per SIMD) of the FP64 face kernels,
+and the stage schedule:
+  "overlap"   viscous-flux kernel on a side stream beside the face sweeps;
+  "serial"    viscous-flux kernel after the sweeps, on the same stream.
 """
 import itertools
 import time
 
 BANDS = (4, 8, 16, 32, 64)
+SCHEDULES = ("overlap", "serial")
 
 
 def _time_steps(solver, steps):
@@ -21,20 +24,26 @@ def _time_steps(solver, steps):
     return (time.perf_counter() - t0) / steps
 
 
-def tune(solver, steps=3, bands=BANDS, overlaps=(True, False), occs=(2, 3)):
-    """Try each (band, overlap, face occupancy) on `steps` steps of the live
+def apply_schedule(solver, sched):
+    if sched not in SCHEDULES:
+        raise ValueError(f"unknown schedule {sched!r}")
+    solver.set_overlap(sched == "overlap")
+
+
+def tune(solver, steps=3, bands=BANDS, schedules=SCHEDULES, occs=(2, 3)):
+    """Try each (band, schedule, face occupancy) on `steps` steps of the live
     state and leave the solver on the fastest.  Returns
     (best, {choice: seconds_per_step}); the state advances by
     (steps + 1) steps per choice."""
     timings = {}
-    for band, ov, occ in itertools.product(bands, overlaps, occs):
+    for band, sched, occ in itertools.product(bands, schedules, occs):
         solver.set_band(band)
-        solver.set_overlap(ov)
+        apply_schedule(solver, sched)
         solver.set_face_occupancy(occ)
         _time_steps(solver, 1)  # settle after a switch
-        timings[(band, ov, occ)] = _time_steps(solver, steps)
+        timings[(band, sched, occ)] = _time_steps(solver, steps)
     best = min(timings, key=timings.get)
     solver.set_band(best[0])
-    solver.set_overlap(best[1])
+    apply_schedule(solver, best[1])
     solver.set_face_occupancy(best[2])
     return best, timings

@@ -41,13 +41,14 @@ __device__ __forceinline__ void band_decode(int t, int ni, int nj, int nk, int j
 
 // 4th-order central derivative (src/calc_gradient.cpp:29-64): the stencil
 // sum, then the division by the spacing.
-__device__ __forceinline__ double grad4(const double* __restrict__ f, int c, int s, double h, double rh, int fh) {
+__device__ __forceinline__ double grad4(gptr<const double> __restrict__ f, int c, int s, double h, double rh,
+                                        int fh) {
   double d = (1.0 / 12.0) * f[c - 2 * s] - 2.0 / 3.0 * f[c - s] + (2.0 / 3.0) * f[c + s] - 1.0 / 12.0 * f[c + 2 * s];
   return divc(d, h, rh, fh);
 }
 
 // Divergence stencil of calc_viscous_flux_contribution (src/Block3d.cpp:894-908).
-__device__ __forceinline__ double cdiff4(const double* __restrict__ f, int c, int s) {
+__device__ __forceinline__ double cdiff4(gptr<const double> __restrict__ f, int c, int s) {
   const double c0 = 1.0 / 12.0, c1 = -2.0 / 3.0;
   return c0 * f[c - 2 * s] + c1 * f[c - s] - c1 * f[c + s] - c0 * f[c + 2 * s];
 }
@@ -67,13 +68,13 @@ __device__ __forceinline__ void face_load(const BlockDesc& B, int t, int which, 
   fi = i + ni * (j + nj * k);
   const int c0 = gidx(B, i, j, k);
   const int sD = D == 0 ? 1 : (D == 1 ? B.g[0] : B.g[0] * B.g[1]);
-  const double* __restrict__ Q = B.Q[which];
+  gptr<const double> __restrict__ Q = gmem<const double>(B.Q[which]);
   const long long sg = B.sg;
 #pragma unroll
   for (int cc = 0; cc < 6; cc++)
 #pragma unroll
     for (int e = 0; e < 5; e++) q[cc][e] = Q[e * sg + c0 + (cc - 3) * sD];
-  const double* __restrict__ P = B.prim;
+  gptr<const double> __restrict__ P = gmem<const double>(B.prim);
 #pragma unroll
   for (int f = 0; f < 5; f++) {
     pl[f] = P[f * sg + c0 - sD];
@@ -87,7 +88,7 @@ __device__ __forceinline__ void face_body(const BlockDesc& B, int t, int which, 
   int fi;
   face_load<D>(B, t, which, jb, q, pl, pr, fi);
   face_flux<D>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl);
-  double* __restrict__ F = B.F[D];
+  gptr<double> __restrict__ F = gmem(B.F[D]);
 #pragma unroll
   for (int e = 0; e < 5; e++) F[e * B.sf[D] + fi] = fl[e];
 }
@@ -100,7 +101,7 @@ __device__ __forceinline__ void face_body_f32(const BlockDesc& B, int t, int whi
   int fi;
   face_load<D>(B, t, which, jb, q, pl, pr, fi);
   face_flux_f32<D>(q, pl, pr, ph.gamma, ph.gam1, g, fl);
-  float* __restrict__ F = (float*)B.F[D];
+  gptr<float> __restrict__ F = gmem((float*)B.F[D]);
 #pragma unroll
   for (int e = 0; e < 5; e++) F[e * B.sf[D] + fi] = fl[e];
 }
@@ -125,10 +126,10 @@ __device__ __forceinline__ void visc_flux_body(const BlockDesc& B, int t, const 
   const int c = gidx(B, i, j, k);
   const int sy = B.g[0], sz = B.g[0] * B.g[1];
   const long long sg = B.sg;
-  const double* __restrict__ u = B.prim + sg;
-  const double* __restrict__ v = B.prim + 2 * sg;
-  const double* __restrict__ w = B.prim + 3 * sg;
-  const double* __restrict__ T = B.prim + 5 * sg;
+  gptr<const double> __restrict__ u = gmem<const double>(B.prim + sg);
+  gptr<const double> __restrict__ v = gmem<const double>(B.prim + 2 * sg);
+  gptr<const double> __restrict__ w = gmem<const double>(B.prim + 3 * sg);
+  gptr<const double> __restrict__ T = gmem<const double>(B.prim + 5 * sg);
   const double u_x = grad4(u, c, 1, ph.dx, ph.rdx, ph.fdx), u_y = grad4(u, c, sy, ph.dy, ph.rdy, ph.fdy),
                u_z = grad4(u, c, sz, ph.dz, ph.rdz, ph.fdz);
   const double v_x = grad4(v, c, 1, ph.dx, ph.rdx, ph.fdx), v_y = grad4(v, c, sy, ph.dy, ph.rdy, ph.fdy),
@@ -140,7 +141,7 @@ __device__ __forceinline__ void visc_flux_body(const BlockDesc& B, int t, const 
   const double T_l = T[c];
   double mu_l = (1.0 + ph.C_T_inf) / (T_l + ph.C_T_inf) * sqrt(T_l * T_l * T_l);
   if (i >= 0 && i < B.n[0] && j >= 0 && j < B.n[1] && k >= 0 && k < B.n[2])
-    B.mu[i + B.n[0] * (j + B.n[1] * k)] = mu_l;
+    gmem(B.mu)[i + B.n[0] * (j + B.n[1] * k)] = mu_l;
   mu_l *= ph.Re_inv;
   const double txx = (2.0 / 3.0) * mu_l * (2.0 * u_x - v_y - w_z);
   const double tyy = (2.0 / 3.0) * mu_l * (2.0 * v_y - u_x - w_z);
@@ -154,7 +155,7 @@ __device__ __forceinline__ void visc_flux_body(const BlockDesc& B, int t, const 
   const double ex = uu * txx + vv * txy + ww * txz - qx;
   const double ey = uu * txy + vv * tyy + ww * tyz - qy;
   const double ez = uu * txz + vv * tyz + ww * tzz - qz;
-  double* __restrict__ V = B.visc;
+  gptr<double> __restrict__ V = gmem(B.visc);
   const long long sv = B.sv;
   V[vt] = txx;
   V[sv + vt] = tyy;
@@ -168,23 +169,16 @@ __device__ __forceinline__ void visc_flux_body(const BlockDesc& B, int t, const 
 }
 
 // ---- stage tail: viscous divergence + RK3 update + primitives --------------
-// src/Block3d.cpp:879-916, 920-962 and 359-385 for one interior cell; stage
-// 1..3 only.  FT is the face-array type (double, or float for FP32-flux).
-template <typename FT>
-__device__ __forceinline__ void update_body(const BlockDesc& B, int t, double dt, int stage, const Phys& ph,
-                                            int jb) {
-  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
-  int i, j, k;
-  band_decode(t, n0, n1, n2, jb, i, j, k);
-  const int ct = i + n0 * (j + n1 * k);
-  const int c = gidx(B, i, j, k);
-  const int v0 = n0 + 2 * NGV, v1 = n1 + 2 * NGV;
+// Viscous divergence of interior cell (i, j, k) from the nine viscous fields
+// (src/Block3d.cpp:879-916).
+__device__ __forceinline__ void visc_div(const BlockDesc& B, int i, int j, int k, const Phys& ph,
+                                         double (&dfv)[4]) {
+  const int v0 = B.n[0] + 2 * NGV, v1 = B.n[1] + 2 * NGV;
   const int cv = (i + NGV) + v0 * ((j + NGV) + v1 * (k + NGV));
-  const double* __restrict__ V = B.visc;
+  gptr<const double> __restrict__ V = gmem<const double>(B.visc);
   const long long sv = B.sv;
   // Ev = (txx, txy, txz, ex), Fv = (txy, tyy, tyz, ey), Gv = (txz, tyz, tzz, ez)
   const int Ef[4] = {0, 3, 4, 6}, Ff[4] = {3, 1, 5, 7}, Gf[4] = {4, 5, 2, 8};
-  double dfv[4];
 #pragma unroll
   for (int e = 0; e < 4; e++) {
     double d = divc(cdiff4(V + Ef[e] * sv, cv, 1), ph.dx, ph.rdx, ph.fdx);
@@ -192,35 +186,37 @@ __device__ __forceinline__ void update_body(const BlockDesc& B, int t, double dt
     d += divc(cdiff4(V + Gf[e] * sv, cv, v0 * v1), ph.dz, ph.rdz, ph.fdz);
     dfv[e] = d;
   }
-  const int iE = i + (n0 + 1) * (j + n1 * k);
-  const int iF = i + n0 * (j + (n1 + 1) * k);
-  const int iG = ct;
-  double* __restrict__ Q = B.Q[0];
-  double* __restrict__ Qp = B.Q[1];
-  const long long sg = B.sg;
+}
+
+// Viscous divergence of the interior cell at work index t (band order) into
+// B.dfv (src/Block3d.cpp:879-916).  t < n0 n1 n2.
+__device__ __forceinline__ void visc_div_body(const BlockDesc& B, int t, const Phys& ph, int jb) {
+  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  int i, j, k;
+  band_decode(t, n0, n1, n2, jb, i, j, k);
+  double dfv[4];
+  visc_div(B, i, j, k, ph, dfv);
+  const int ct = i + n0 * (j + n1 * k);
+#pragma unroll
+  for (int e = 0; e < 4; e++) gmem(B.dfv)[e * B.sn + ct] = dfv[e];
+}
+
+// RK3 stage combination (src/Block3d.cpp:949-955) of cell c from the
+// right-hand side df, Q^n = Qn, the stage input Qi, into Qo; then the cell's
+// primitives into P (src/Block3d.cpp:369-381).  Qo may be Qn (stage 3).
+__device__ __forceinline__ void rk3_prim(const double (&df)[NEQ], gptr<const double> Qn, gptr<const double> Qi,
+                                         gptr<double> Qo, gptr<double> __restrict__ P, long long sg, int c,
+                                         double dt, int stage,
+                                         const Phys& ph) {
   double qn[NEQ];
 #pragma unroll
   for (int e = 0; e < NEQ; e++) {
-    const FT* __restrict__ Ep = (const FT*)B.F[0] + e * B.sf[0];
-    const FT* __restrict__ Fp = (const FT*)B.F[1] + e * B.sf[1];
-    const FT* __restrict__ Gp = (const FT*)B.F[2] + e * B.sf[2];
-    double df = divc((double)Ep[iE + 1] - (double)Ep[iE], ph.dx, ph.rdx, ph.fdx);
-    df += divc((double)Fp[iF + n0] - (double)Fp[iF], ph.dy, ph.rdy, ph.fdy);
-    df += divc((double)Gp[iG + n0 * n1] - (double)Gp[iG], ph.dz, ph.rdz, ph.fdz);
-    if (e > 0) df -= dfv[e - 1];
     const long long q = e * sg + c;
-    if (1 == stage) {
-      qn[e] = Q[q] - dt * df;
-      Qp[q] = qn[e];
-    } else if (2 == stage) {
-      qn[e] = 0.75 * Q[q] + 0.25 * (Qp[q] - dt * df);
-      Qp[q] = qn[e];
-    } else {
-      qn[e] = (Q[q] + 2.0 * (Qp[q] - dt * df)) / 3.0;
-      Q[q] = qn[e];
-    }
+    if (1 == stage) qn[e] = Qn[q] - dt * df[e];
+    else if (2 == stage) qn[e] = 0.75 * Qn[q] + 0.25 * (Qi[q] - dt * df[e]);
+    else qn[e] = (Qn[q] + 2.0 * (Qi[q] - dt * df[e])) / 3.0;
+    Qo[q] = qn[e];
   }
-  double* __restrict__ P = B.prim;
   const double rr = qn[0];
   const double uu = qn[1] / rr;
   const double vv = qn[2] / rr;
@@ -232,6 +228,38 @@ __device__ __forceinline__ void update_body(const BlockDesc& B, int t, double dt
   P[3 * sg + c] = ww;
   P[4 * sg + c] = pp;
   P[5 * sg + c] = ph.gM2 * pp / rr;
+}
+
+// src/Block3d.cpp:879-916, 920-962 and 359-385 for one interior cell; stage
+// 1..3 only, in place on Q / Q_p as the reference.  FT is the face-array type
+// (double, or float for FP32-flux).
+template <typename FT>
+__device__ __forceinline__ void update_body(const BlockDesc& B, int t, double dt, int stage, const Phys& ph,
+                                            int jb) {
+  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  int i, j, k;
+  band_decode(t, n0, n1, n2, jb, i, j, k);
+  const int ct = i + n0 * (j + n1 * k);
+  const int c = gidx(B, i, j, k);
+  double dfv[4];
+  visc_div(B, i, j, k, ph, dfv);
+  const int iE = i + (n0 + 1) * (j + n1 * k);
+  const int iF = i + n0 * (j + (n1 + 1) * k);
+  const int iG = ct;
+  double df[NEQ];
+#pragma unroll
+  for (int e = 0; e < NEQ; e++) {
+    gptr<const FT> __restrict__ Ep = gmem<const FT>((const FT*)B.F[0] + e * B.sf[0]);
+    gptr<const FT> __restrict__ Fp = gmem<const FT>((const FT*)B.F[1] + e * B.sf[1]);
+    gptr<const FT> __restrict__ Gp = gmem<const FT>((const FT*)B.F[2] + e * B.sf[2]);
+    double d = divc((double)Ep[iE + 1] - (double)Ep[iE], ph.dx, ph.rdx, ph.fdx);
+    d += divc((double)Fp[iF + n0] - (double)Fp[iF], ph.dy, ph.rdy, ph.fdy);
+    d += divc((double)Gp[iG + n0 * n1] - (double)Gp[iG], ph.dz, ph.rdz, ph.fdz);
+    if (e > 0) d -= dfv[e - 1];
+    df[e] = d;
+  }
+  rk3_prim(df, gmem<const double>(B.Q[0]), gmem<const double>(B.Q[1]), gmem(stage == 3 ? B.Q[0] : B.Q[1]),
+           gmem(B.prim), B.sg, c, dt, stage, ph);
 }
 
 // ---- ghost gather of the state and its primitives ------------------------
@@ -259,13 +287,13 @@ __device__ __forceinline__ void ghost_body(const BlockDesc& B, const BlockDesc* 
   }
   const BlockDesc& S = blocks[block0 + (bc[0] * dom.dims[1] + bc[1]) * dom.dims[2] + bc[2]];
   const int src = gidx(S, lc[0], lc[1], lc[2]);
-  const double* __restrict__ Qs = S.Q[which];
-  double* __restrict__ Qd = B.Q[which];
+  gptr<const double> __restrict__ Qs = gmem<const double>(S.Q[which]);
+  gptr<double> __restrict__ Qd = gmem(B.Q[which]);
 #pragma unroll
   for (int e = 0; e < NEQ; e++) Qd[e * B.sg + t] = Qs[e * S.sg + src];
   if (with_prim) {
 #pragma unroll
-    for (int f = 0; f < 6; f++) B.prim[f * B.sg + t] = S.prim[f * S.sg + src];
+    for (int f = 0; f < 6; f++) gmem(B.prim)[f * B.sg + t] = gmem<const double>(S.prim)[f * S.sg + src];
   }
 }
 
@@ -279,10 +307,11 @@ __device__ __forceinline__ double dt_cell(const BlockDesc& B, int t, const Phys&
   const int k = r / n1;
   const int c = gidx(B, i, j, k);
   const long long sg = B.sg;
-  const double rr = B.prim[c], uu = B.prim[sg + c], vv = B.prim[2 * sg + c], ww = B.prim[3 * sg + c];
-  const double pp = B.prim[4 * sg + c];
+  gptr<const double> __restrict__ P = gmem<const double>(B.prim);
+  const double rr = P[c], uu = P[sg + c], vv = P[2 * sg + c], ww = P[3 * sg + c];
+  const double pp = P[4 * sg + c];
   const double cs = sqrt(ph.gamma * pp / rr);
-  const double factor_v = 2.0 * B.mu[t] * ph.C_dt_v * sqrt(uu * uu + vv * vv + ww * ww) / cs / rr;
+  const double factor_v = 2.0 * gmem<const double>(B.mu)[t] * ph.C_dt_v * sqrt(uu * uu + vv * vv + ww * ww) / cs / rr;
   double t1 = fabs(uu) + divc(cs, ph.dx, ph.rdx, ph.fdx);
   t1 += divc(divc(factor_v, ph.dx, ph.rdx, ph.fdx), ph.dx, ph.rdx, ph.fdx);
   double t2 = fabs(vv) + divc(cs, ph.dy, ph.rdy, ph.fdy);

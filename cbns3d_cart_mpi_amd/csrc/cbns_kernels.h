@@ -52,6 +52,16 @@ struct Domain {
   int nb;
 };
 
+// Descriptor pointers are generic; these views tell the compiler that they
+// address global memory, so accesses compile to global_* instead of flat_*
+// instructions (no LDS aperture check, SGPR-base addressing).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> gmem(T* p) {
+  return (gptr<T>)p;
+}
+
 struct Launch {
   hipStream_t stream;
   const BlockDesc* dblocks;  // device copy

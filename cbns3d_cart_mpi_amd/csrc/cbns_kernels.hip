@@ -34,7 +34,7 @@ __device__ __forceinline__ Tile xcd_tile() {
 // Inviscid face fluxes: rec_riemann_x/y/z (src/Block3d.cpp:555-748).
 // One thread per face; face (i,j,k) along D sits between cells i-1 and i.
 // OCC = minimum workgroups per CU the register allocation must allow: 2 keeps
-// every value in registers (194 VGPRs), 3 trades a few spills for a third
+// every value in registers (182 VGPRs), 3 trades a few spills for a third
 // wave per SIMD.  Both are compiled; the launch picks one (autotuned).
 template <int D, int OCC>
 __global__ __launch_bounds__(TPB, OCC) void k_face_flux(const BlockDesc* __restrict__ blocks, int which, int jb,
@@ -95,8 +95,8 @@ __global__ __launch_bounds__(TPB) void k_primitive(const BlockDesc* __restrict__
   const long long sg = B.sg;
   const int t = tl.x * TPB + threadIdx.x;
   if (t >= sg) return;
-  const double* __restrict__ Q = B.Q[which];
-  double* __restrict__ P = B.prim;
+  gptr<const double> __restrict__ Q = gmem<const double>(B.Q[which]);
+  gptr<double> __restrict__ P = gmem(B.prim);
   double rr = Q[t];
   double uu = Q[sg + t] / rr;
   double vv = Q[2 * sg + t] / rr;
@@ -158,26 +158,9 @@ __global__ __launch_bounds__(TPB) void k_visc_flux(const BlockDesc* __restrict__
 __global__ __launch_bounds__(TPB) void k_visc_div(const BlockDesc* __restrict__ blocks, Phys ph, int jb) {
   const Tile tl = xcd_tile();
   const BlockDesc& B = blocks[tl.y];
-  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
   const int t = tl.x * TPB + threadIdx.x;
-  if (t >= n0 * n1 * n2) return;
-  int i, j, k;
-  band_decode(t, n0, n1, n2, jb, i, j, k);
-  const int ct = i + n0 * (j + n1 * k);
-  const int v0 = n0 + 2 * NGV, v1 = n1 + 2 * NGV;
-  const int c = (i + NGV) + v0 * ((j + NGV) + v1 * (k + NGV));
-  const int sy = v0, sz = v0 * v1;
-  const double* __restrict__ V = B.visc;
-  const long long sv = B.sv;
-  // Ev = (txx, txy, txz, ex), Fv = (txy, tyy, tyz, ey), Gv = (txz, tyz, tzz, ez)
-  const int Ef[4] = {0, 3, 4, 6}, Ff[4] = {3, 1, 5, 7}, Gf[4] = {4, 5, 2, 8};
-#pragma unroll
-  for (int e = 0; e < 4; e++) {
-    double df = divc(cdiff4(V + Ef[e] * sv, c, 1), ph.dx, ph.rdx, ph.fdx);
-    df += divc(cdiff4(V + Ff[e] * sv, c, sy), ph.dy, ph.rdy, ph.fdy);
-    df += divc(cdiff4(V + Gf[e] * sv, c, sz), ph.dz, ph.rdz, ph.fdz);
-    B.dfv[e * B.sn + ct] = df;
-  }
+  if (t >= B.n[0] * B.n[1] * B.n[2]) return;
+  visc_div_body(B, t, ph, jb);
 }
 
 hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int jb, hipStream_t s) {
@@ -221,18 +204,18 @@ __global__ __launch_bounds__(TPB) void k_update(const BlockDesc* __restrict__ bl
   const int iE = i + (n0 + 1) * (j + n1 * k);
   const int iF = i + n0 * (j + (n1 + 1) * k);
   const int iG = ct;
-  double* __restrict__ Q = B.Q[0];
-  double* __restrict__ Qp = B.Q[1];
+  gptr<double> __restrict__ Q = gmem(B.Q[0]);
+  gptr<double> __restrict__ Qp = gmem(B.Q[1]);
   const long long sg = B.sg;
 #pragma unroll
   for (int e = 0; e < NEQ; e++) {
-    const FT* __restrict__ Ep = (const FT*)B.F[0] + e * B.sf[0];
-    const FT* __restrict__ Fp = (const FT*)B.F[1] + e * B.sf[1];
-    const FT* __restrict__ Gp = (const FT*)B.F[2] + e * B.sf[2];
+    gptr<const FT> __restrict__ Ep = gmem<const FT>((const FT*)B.F[0] + e * B.sf[0]);
+    gptr<const FT> __restrict__ Fp = gmem<const FT>((const FT*)B.F[1] + e * B.sf[1]);
+    gptr<const FT> __restrict__ Gp = gmem<const FT>((const FT*)B.F[2] + e * B.sf[2]);
     double df = divc((double)Ep[iE + 1] - (double)Ep[iE], ph.dx, ph.rdx, ph.fdx);
     df += divc((double)Fp[iF + n0] - (double)Fp[iF], ph.dy, ph.rdy, ph.fdy);
     df += divc((double)Gp[iG + n0 * n1] - (double)Gp[iG], ph.dz, ph.rdz, ph.fdz);
-    if (e > 0) df -= B.dfv[(e - 1) * B.sn + ct];
+    if (e > 0) df -= gmem<const double>(B.dfv)[(e - 1) * B.sn + ct];
     const long long q = e * sg + c;
     if (1 == stage) {
       Qp[q] = Q[q] - dt * df;
@@ -279,10 +262,9 @@ hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb,
   long long mn = 0;
   for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
   if (stage < 1 || stage > 3) return hipSuccess;
-  if (precision == 0)
-    hipLaunchKernelGGL(k_update_fused<double>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, dt, stage, ph, jb);
-  else
-    hipLaunchKernelGGL(k_update_fused<float>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, dt, stage, ph, jb);
+  const dim3 grid(nblocks_for(mn), nb);
+  if (precision == 0) hipLaunchKernelGGL(k_update_fused<double>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb);
+  else hipLaunchKernelGGL(k_update_fused<float>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb);
   return hipGetLastError();
 }
 
@@ -349,8 +331,8 @@ hipError_t launch_dt_reduce(const BlockDesc* db, const BlockDesc* hb, int nb, co
 __global__ __launch_bounds__(TPB) void k_copy_state(const BlockDesc* __restrict__ blocks, int from, int to) {
   const BlockDesc& B = blocks[blockIdx.y];
   const long long n = NEQ * B.sg;
-  const double* __restrict__ a = B.Q[from];
-  double* __restrict__ b = B.Q[to];
+  gptr<const double> __restrict__ a = gmem<const double>(B.Q[from]);
+  gptr<double> __restrict__ b = gmem(B.Q[to]);
   for (long long t = blockIdx.x * (long long)TPB + threadIdx.x; t < n; t += (long long)gridDim.x * TPB) b[t] = a[t];
 }
 
@@ -495,15 +477,15 @@ __global__ __launch_bounds__(TPB) void k_global_sums(const BlockDesc* __restrict
     const BlockDesc& B = blocks[(bc[0] * dom.dims[1] + bc[1]) * dom.dims[2] + bc[2]];
     const int c = gidx(B, lc[0], lc[1], lc[2]);
     if (WHAT == 0) {
-      sa += B.Q[0][c];
-      sb += B.Q[0][4 * B.sg + c];
+      sa += gmem<const double>(B.Q[0])[c];
+      sb += gmem<const double>(B.Q[0])[4 * B.sg + c];
     } else {
       const long long sg = B.sg;
       const int sy = B.g[0], sz = B.g[0] * B.g[1];
-      const double* u = B.prim + sg;
-      const double* v = B.prim + 2 * sg;
-      const double* w = B.prim + 3 * sg;
-      const double rr = B.prim[c], uu = u[c], vv = v[c], ww = w[c];
+      gptr<const double> u = gmem<const double>(B.prim + sg);
+      gptr<const double> v = gmem<const double>(B.prim + 2 * sg);
+      gptr<const double> w = gmem<const double>(B.prim + 3 * sg);
+      const double rr = gmem<const double>(B.prim)[c], uu = u[c], vv = v[c], ww = w[c];
       sa += 0.5 * rr * (uu * uu + vv * vv + ww * ww);
       const double ox = grad4(w, c, sy, ph.dy, ph.rdy, ph.fdy) - grad4(v, c, sz, ph.dz, ph.rdz, ph.fdz);
       const double oy = grad4(u, c, sz, ph.dz, ph.rdz, ph.fdz) - grad4(w, c, 1, ph.dx, ph.rdx, ph.fdx);

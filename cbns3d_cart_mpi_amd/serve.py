@@ -98,8 +98,9 @@ def percentile(x, q):
     return float(np.percentile(np.asarray(x), q)) if len(x) else math.nan
 
 
-def run_sweep(reqs, budget=200 * GB):
-    """All requests admitted at time 0 into one batch and advanced together."""
+def run_sweep(reqs, budget=200 * GB, collect=False):
+    """All requests admitted at time 0 into one batch and advanced together.
+    With collect, each request's final state is kept in r.result."""
     for r in reqs:
         r.payload = prepare(r)
     t_setup = time.perf_counter()
@@ -120,6 +121,8 @@ def run_sweep(reqs, budget=200 * GB):
             assert not st["nonfinite"], r
             r.t_final = st["t_cur"]
             r.done = el
+            if collect:
+                r.result = b.get_state(ids[r.rid])
     work = sum(r.N ** 3 * (r.steps - 1) for r in reqs)  # timed steps only
     return {"wall_s": el, "setup_s": t0 - t_setup, "cell_steps": work, "mcell_steps_per_s": work / el / 1e6,
             "p50_s": percentile([r.done for r in reqs], 50), "p99_s": percentile([r.done for r in reqs], 99)}

@@ -75,3 +75,39 @@ def turbulence_state(N, seed=0, mach_t=0.6, gamma=1.4, re_lambda=100.0, k0=4.0):
     Q[..., 3] = rho * u[2]
     Q[..., 4] = p / (gamma - 1.0) + 0.5 * rho * (u * u).sum(axis=0)
     return Q, float(Re)
+
+
+def dilatation_spectrum(Q, N):
+    """Shell spectrum of the dilatation theta = div u of an interior AoS state
+    (spectral derivatives on the periodic [-pi, pi)^3 grid): E(k) for integer
+    shells k = 0 .. N/2, E(k) = sum over |kvec| in [k - 1/2, k + 1/2) of
+    |theta_hat|^2 / 2 with theta_hat normalised by N^3."""
+    k1 = np.fft.fftfreq(N, 1.0 / N)
+    KZ, KY, KX = np.meshgrid(k1, k1, k1, indexing="ij")
+    th = np.zeros((N, N, N), complex)
+    for a, K in enumerate((KX, KY, KZ)):
+        th += 1j * K * np.fft.fftn(Q[..., 1 + a] / Q[..., 0])
+    e = 0.5 * np.abs(th / N ** 3) ** 2
+    shell = np.rint(np.sqrt(KX * KX + KY * KY + KZ * KZ)).astype(int).ravel()
+    spec = np.bincount(shell, weights=e.ravel())
+    return spec[: N // 2 + 1]
+
+
+def precision_errors(Q32, Q64, N):
+    """Per-field relative L2 (RMS difference / RMS of the FP64 field) and
+    relative L-inf (max difference / max |FP64 field|) of an FP32-flux state
+    against the FP64 state, and the dilatation-spectrum difference: the
+    largest relative difference over shells 1 .. N/2 and the relative L2
+    difference of the whole spectrum."""
+    fields = ("rho", "rho_u", "rho_v", "rho_w", "E")
+    err = {}
+    for e, nm in enumerate(fields):
+        d = Q32[..., e] - Q64[..., e]
+        ref = Q64[..., e]
+        err[nm] = {"rel_l2": float(np.sqrt(np.mean(d * d) / np.mean(ref * ref))),
+                   "rel_linf": float(np.abs(d).max() / np.abs(ref).max())}
+    s32, s64 = dilatation_spectrum(Q32, N), dilatation_spectrum(Q64, N)
+    rel = np.abs(s32[1:] - s64[1:]) / s64[1:]
+    err["dilatation_spectrum"] = {"max_rel_shell": float(rel.max()),
+                                  "rel_l2": float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64))}
+    return err
