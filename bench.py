@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--turb-steps", type=int, default=500)
     ap.add_argument("--sweep-steps", type=int, default=300)
     ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-slots", type=int, default=2, help="solves resident at once (0: as memory allows)")
     ap.add_argument("--cpu-max-steps", type=int, default=3)
     ap.add_argument("--cpu-budget-s", type=float, default=25.0)
     return ap.parse_args()
@@ -187,11 +188,12 @@ def turbulence_leg(N, steps, seed=2024):
     return out
 
 
-def serve_leg(n):
+def serve_leg(n, slots):
     """Continuous batching of a Poisson stream (reduced size): n requests, grids
     {32..192}^3 with weights 1/N, steps U[20, 100], FP64 / FP32-flux alternating,
-    200 GB budget, offered load 85% of the measured batch capacity; against the
-    same requests solved one at a time as plain FP64."""
+    200 GB budget, offered load 85% of the measured batch capacity, at most
+    `slots` solves resident at once (None: as memory allows); against the same
+    requests solved one at a time as plain FP64."""
     from cbns3d_cart_mpi_amd import serve as S
 
     grids = (32, 48, 64, 96, 128, 160, 192)
@@ -202,10 +204,11 @@ def serve_leg(n):
     mean_work = np.mean([r.work for r in S.make_requests(n, 2024, grids, w, (20, 100))])
     rate = 0.85 * cap / mean_work
     reqs = S.make_requests(n, 2024, grids, w, (20, 100), rate=rate)
-    srv = S.serve(reqs, chunk=4)
+    srv = S.serve(reqs, chunk=4, max_live=slots)
     seq = S.sequential_fp64(reqs)
     rnd = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
-    return {"requests": n, "rate_per_s": round(rate, 3), "served": rnd(srv), "sequential_fp64": rnd(seq),
+    return {"requests": n, "rate_per_s": round(rate, 3), "slots": slots, "served": rnd(srv),
+            "sequential_fp64": rnd(seq),
             "config": "reduced: steps U[20,100] instead of U[200,2000]; 64 instead of 1000 requests"}
 
 
@@ -306,7 +309,7 @@ def main():
     if not a.no_sweep:
         line["sweep"] = sweep_leg(a.sweep_steps)
     if not a.no_serve:
-        line["serving"] = serve_leg(a.serve_requests)
+        line["serving"] = serve_leg(a.serve_requests, a.serve_slots or None)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(N, a.cpu_max_steps, a.cpu_budget_s)
     print(json.dumps(line), flush=True)

@@ -41,8 +41,8 @@ __device__ __forceinline__ void band_decode(int t, int ni, int nj, int nk, int j
 
 // 4th-order central derivative (src/calc_gradient.cpp:29-64): the stencil
 // sum, then the division by the spacing.
-__device__ __forceinline__ double grad4(gptr<const double> __restrict__ f, int c, int s, double h, double rh,
-                                        int fh) {
+template <class P>
+__device__ __forceinline__ double grad4(P __restrict__ f, int c, int s, double h, double rh, int fh) {
   double d = (1.0 / 12.0) * f[c - 2 * s] - 2.0 / 3.0 * f[c - s] + (2.0 / 3.0) * f[c + s] - 1.0 / 12.0 * f[c + 2 * s];
   return divc(d, h, rh, fh);
 }
@@ -115,6 +115,38 @@ __device__ __forceinline__ int visc_count(const BlockDesc& B) {
   return (B.n[0] + 2 * NGV) * (B.n[1] + 2 * NGV) * (B.n[2] + 2 * NGV);
 }
 
+// Sutherland viscosity, stress tensor, heat flux and energy fluxes of one
+// point from its gradients gr[u, v, w, T][x, y, z], temperature and velocity
+// (src/Block3d.cpp:777-804, 843-862).  out = txx tyy tzz txy txz tyz ex ey ez;
+// mu_out = Sutherland mu (before the 1/Re factor).
+__device__ __forceinline__ void visc_terms(const Phys& ph, const double (&gr)[4][3], double T_l, double uu,
+                                           double vv, double ww, double (&out)[9], double& mu_out) {
+  const double u_x = gr[0][0], u_y = gr[0][1], u_z = gr[0][2];
+  const double v_x = gr[1][0], v_y = gr[1][1], v_z = gr[1][2];
+  const double w_x = gr[2][0], w_y = gr[2][1], w_z = gr[2][2];
+  const double T_x = gr[3][0], T_y = gr[3][1], T_z = gr[3][2];
+  double mu_l = (1.0 + ph.C_T_inf) / (T_l + ph.C_T_inf) * sqrt(T_l * T_l * T_l);
+  mu_out = mu_l;
+  mu_l *= ph.Re_inv;
+  const double txx = (2.0 / 3.0) * mu_l * (2.0 * u_x - v_y - w_z);
+  const double tyy = (2.0 / 3.0) * mu_l * (2.0 * v_y - u_x - w_z);
+  const double tzz = (2.0 / 3.0) * mu_l * (2.0 * w_z - u_x - v_y);
+  const double txy = mu_l * (u_y + v_x);
+  const double txz = mu_l * (u_z + w_x);
+  const double tyz = mu_l * (v_z + w_y);
+  mu_l *= ph.kappa;
+  const double qx = mu_l * T_x, qy = mu_l * T_y, qz = mu_l * T_z;
+  out[0] = txx;
+  out[1] = tyy;
+  out[2] = tzz;
+  out[3] = txy;
+  out[4] = txz;
+  out[5] = tyz;
+  out[6] = uu * txx + vv * txy + ww * txz - qx;
+  out[7] = uu * txy + vv * tyy + ww * tyz - qy;
+  out[8] = uu * txz + vv * tyz + ww * tzz - qz;
+}
+
 __device__ __forceinline__ void visc_flux_body(const BlockDesc& B, int t, const Phys& ph, int jb) {
   const int v0 = B.n[0] + 2 * NGV, v1 = B.n[1] + 2 * NGV, v2 = B.n[2] + 2 * NGV;
   int i, j, k;
@@ -130,42 +162,22 @@ __device__ __forceinline__ void visc_flux_body(const BlockDesc& B, int t, const 
   gptr<const double> __restrict__ v = gmem<const double>(B.prim + 2 * sg);
   gptr<const double> __restrict__ w = gmem<const double>(B.prim + 3 * sg);
   gptr<const double> __restrict__ T = gmem<const double>(B.prim + 5 * sg);
-  const double u_x = grad4(u, c, 1, ph.dx, ph.rdx, ph.fdx), u_y = grad4(u, c, sy, ph.dy, ph.rdy, ph.fdy),
-               u_z = grad4(u, c, sz, ph.dz, ph.rdz, ph.fdz);
-  const double v_x = grad4(v, c, 1, ph.dx, ph.rdx, ph.fdx), v_y = grad4(v, c, sy, ph.dy, ph.rdy, ph.fdy),
-               v_z = grad4(v, c, sz, ph.dz, ph.rdz, ph.fdz);
-  const double w_x = grad4(w, c, 1, ph.dx, ph.rdx, ph.fdx), w_y = grad4(w, c, sy, ph.dy, ph.rdy, ph.fdy),
-               w_z = grad4(w, c, sz, ph.dz, ph.rdz, ph.fdz);
-  const double T_x = grad4(T, c, 1, ph.dx, ph.rdx, ph.fdx), T_y = grad4(T, c, sy, ph.dy, ph.rdy, ph.fdy),
-               T_z = grad4(T, c, sz, ph.dz, ph.rdz, ph.fdz);
-  const double T_l = T[c];
-  double mu_l = (1.0 + ph.C_T_inf) / (T_l + ph.C_T_inf) * sqrt(T_l * T_l * T_l);
+  double gr[4][3];
+  gptr<const double> f4[4] = {u, v, w, T};
+#pragma unroll
+  for (int f = 0; f < 4; f++) {
+    gr[f][0] = grad4(f4[f], c, 1, ph.dx, ph.rdx, ph.fdx);
+    gr[f][1] = grad4(f4[f], c, sy, ph.dy, ph.rdy, ph.fdy);
+    gr[f][2] = grad4(f4[f], c, sz, ph.dz, ph.rdz, ph.fdz);
+  }
+  double o[9], mu_l;
+  visc_terms(ph, gr, T[c], u[c], v[c], w[c], o, mu_l);
   if (i >= 0 && i < B.n[0] && j >= 0 && j < B.n[1] && k >= 0 && k < B.n[2])
     gmem(B.mu)[i + B.n[0] * (j + B.n[1] * k)] = mu_l;
-  mu_l *= ph.Re_inv;
-  const double txx = (2.0 / 3.0) * mu_l * (2.0 * u_x - v_y - w_z);
-  const double tyy = (2.0 / 3.0) * mu_l * (2.0 * v_y - u_x - w_z);
-  const double tzz = (2.0 / 3.0) * mu_l * (2.0 * w_z - u_x - v_y);
-  const double txy = mu_l * (u_y + v_x);
-  const double txz = mu_l * (u_z + w_x);
-  const double tyz = mu_l * (v_z + w_y);
-  mu_l *= ph.kappa;
-  const double qx = mu_l * T_x, qy = mu_l * T_y, qz = mu_l * T_z;
-  const double uu = u[c], vv = v[c], ww = w[c];
-  const double ex = uu * txx + vv * txy + ww * txz - qx;
-  const double ey = uu * txy + vv * tyy + ww * tyz - qy;
-  const double ez = uu * txz + vv * tyz + ww * tzz - qz;
   gptr<double> __restrict__ V = gmem(B.visc);
   const long long sv = B.sv;
-  V[vt] = txx;
-  V[sv + vt] = tyy;
-  V[2 * sv + vt] = tzz;
-  V[3 * sv + vt] = txy;
-  V[4 * sv + vt] = txz;
-  V[5 * sv + vt] = tyz;
-  V[6 * sv + vt] = ex;
-  V[7 * sv + vt] = ey;
-  V[8 * sv + vt] = ez;
+#pragma unroll
+  for (int q = 0; q < 9; q++) V[q * sv + vt] = o[q];
 }
 
 // ---- stage tail: viscous divergence + RK3 update + primitives --------------

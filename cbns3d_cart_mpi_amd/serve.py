@@ -131,7 +131,7 @@ def run_sweep(reqs, budget=200 * GB, collect=False):
 LANES = ((0, 64), (65, 128), (129, 1 << 30))  # size classes by grid edge N
 
 
-def serve(reqs, budget=200 * GB, chunk=4, capacity=1024, collect=True, lanes=LANES):
+def serve(reqs, budget=200 * GB, chunk=4, capacity=1024, collect=True, lanes=LANES, max_live=None):
     """Continuous batching over a Poisson request stream (arrival times in s).
 
     Solves are grouped into size-class lanes, each a device batch with its own
@@ -141,7 +141,8 @@ def serve(reqs, budget=200 * GB, chunk=4, capacity=1024, collect=True, lanes=LAN
     lane's chunk asynchronously so the lanes run concurrently on the GPU: the
     lane holding the largest solve replays `chunk` steps and a lane whose
     largest solve has n cells replays about chunk * n_max / n steps.  Finished
-    solves are collected and removed after the chunk, freeing memory."""
+    solves are collected and removed after the chunk, freeing memory.
+    max_live caps the number of solves resident at once (None: memory only)."""
     for r in reqs:
         r.payload = prepare(r)
     pending = sorted(reqs, key=lambda r: r.arrival)
@@ -165,6 +166,8 @@ def serve(reqs, budget=200 * GB, chunk=4, capacity=1024, collect=True, lanes=LAN
                 p, Q = r.payload
                 need = C.batch_bytes_needed(p)
                 nlive = sum(len(x) for x in live)
+                if max_live is not None and nlive >= max_live:
+                    break
                 if used[0] + need > budget:
                     if not nlive:
                         raise RuntimeError(f"request {r.rid} does not fit the budget alone")
@@ -258,6 +261,7 @@ def main(argv=None):
     ap.add_argument("--load", type=float, default=0.85, help="offered load vs batch capacity when --rate is unset")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--chunk", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=2, help="solves resident at once (0: as memory allows)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-baseline", action="store_true")
     a = ap.parse_args(argv)
@@ -279,7 +283,8 @@ def main(argv=None):
         mean_work = np.mean([r.work for r in make_requests(n, a.seed, grids, weights, (a.steps_min, a.steps_max))])
         rate = a.load * cap / mean_work
     reqs = make_requests(n, a.seed, grids, weights, (a.steps_min, a.steps_max), rate=rate)
-    res = {"requests": n, "rate_per_s": rate, "served": serve(reqs, budget, a.chunk)}
+    res = {"requests": n, "rate_per_s": rate,
+           "served": serve(reqs, budget, a.chunk, max_live=a.slots or None)}
     if not a.no_baseline:
         res["sequential_fp64"] = sequential_fp64(reqs)
     print(json.dumps(res))
