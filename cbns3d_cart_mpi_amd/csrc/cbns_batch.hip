@@ -45,9 +45,9 @@ __global__ __launch_bounds__(TPB) void kb_visc(const BlockDesc* __restrict__ blo
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
     if (!I.stepping) continue;
-    const int t = tr.tile * TPB + threadIdx.x;
-    if (t >= visc_count(B)) continue;
-    visc_flux_body(B, t, I.ph, jb);
+    // one (column tile, z-chunk) unit of the viscous tile march per entry
+    const int ntiles = ((B.n[0] + 2 * NGV + VX - 1) / VX) * ((B.n[1] + 2 * NGV + VY - 1) / VY);
+    visc_tile_body(B, I.ph, tr.tile % ntiles, tr.tile / ntiles);
   }
 }
 
@@ -76,9 +76,9 @@ __global__ __launch_bounds__(TPB) void kb_ghost(const BlockDesc* __restrict__ bl
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
     if (!I.stepping) continue;
-    const int t = tr.tile * TPB + threadIdx.x;
-    if (t >= B.sg) continue;
-    ghost_body(B, blocks, I.dom, I.block0, t, which, true);
+    const int g = tr.tile * TPB + threadIdx.x;
+    if (g >= B.sg - B.sn) continue;
+    ghost_body(B, blocks, I.dom, I.block0, ghost_cell(B, g), which, true);
   }
 }
 

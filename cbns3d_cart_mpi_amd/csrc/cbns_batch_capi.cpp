@@ -151,9 +151,11 @@ static int rebuild(cbns_batch* b) {
         for (int t = 0; t < nt; t++) tab[cls].push_back({bi, t});
       };
       for (int d = 0; d < 3; d++) add((I.precision ? T_FACE32 : T_FACE64) + d, B.sf[d]);
-      add(T_VISC, B.sv);
+      const int vunits = ((B.n[0] + 2 * NGV + VX - 1) / VX) * ((B.n[1] + 2 * NGV + VY - 1) / VY) *
+                         ((B.n[2] + 2 * NGV + VZ - 1) / VZ);
+      for (int t = 0; t < vunits; t++) tab[T_VISC].push_back({bi, t});  // whole-workgroup units
       add(I.precision ? T_UPD32 : T_UPD64, B.sn);
-      add(T_GHOST, B.sg);
+      add(T_GHOST, B.sg - B.sn);  // ghost cells only
       add(T_DT, B.sn);
     }
     // the device descriptor's block0 (other fields hold live state)

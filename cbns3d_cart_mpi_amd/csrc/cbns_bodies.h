@@ -180,6 +180,101 @@ __device__ __forceinline__ void visc_flux_body(const BlockDesc& B, int t, const 
   for (int q = 0; q < 9; q++) V[q * sv + vt] = o[q];
 }
 
+// Tile march of the viscous fluxes.  A workgroup owns a VX x VY column tile
+// of the extended box and marches VZ planes along z.  Each plane of u, v, w,
+// T with its 2-point x/y halo is staged in LDS (double-buffered, one barrier
+// per plane) for the x- and y-gradients; each thread keeps its own column's
+// five z-planes in registers for the z-gradient.  The next plane's loads are
+// issued before the current plane's arithmetic.  Every value is formed by the
+// same expressions as visc_flux_body's (grad4, visc_terms), so results are
+// identical.  One call = one (column tile, z-chunk) unit, made by all
+// VX * VY threads of a workgroup with the same tile and chunk (it
+// synchronises the workgroup).
+__device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& ph, int tile, int chunk) {
+  const int v0 = B.n[0] + 2 * NGV, v1 = B.n[1] + 2 * NGV, v2 = B.n[2] + 2 * NGV;
+  const int ntx = (v0 + VX - 1) / VX, nty = (v1 + VY - 1) / VY;
+  const int Z0 = chunk * VZ;
+  if (tile >= ntx * nty || Z0 >= v2) return;  // uniform per workgroup
+  const int Z1 = Z0 + VZ < v2 ? Z0 + VZ : v2;
+  const int X0 = (tile % ntx) * VX, Y0 = (tile / ntx) * VY;
+  const int tx = threadIdx.x % VX, ty = threadIdx.x / VX;
+  const int bx = X0 + tx, by = Y0 + ty;  // box point; its cell is (bx - 2, by - 2, bz - 2)
+  const bool act = bx < v0 && by < v1;
+  constexpr int LX = VX + 4, LY = VY + 4, NT = VX * VY, NP = (LX * LY + NT - 1) / NT;
+  __shared__ double L[2][4][LY][LX];
+  const long long sg = B.sg;
+  const int sz = B.g[0] * B.g[1];
+  gptr<const double> __restrict__ P = gmem<const double>(B.prim);
+  const long long foff[4] = {sg, 2 * sg, 3 * sg, 5 * sg};  // u, v, w, T
+  // LDS entries e = threadIdx.x + p NT this thread fills: cell (X0 + lx - 4, Y0 + ly - 4)
+  int fidx[NP];
+  bool fok[NP];
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    const int e = threadIdx.x + p * NT;
+    const int lx = e % LX, ly = e / LX;
+    const int cx = X0 + lx - 4, cy = Y0 + ly - 4;
+    fok[p] = e < LX * LY && cx < B.n[0] + NG && cy < B.n[1] + NG;
+    fidx[p] = fok[p] ? gidx(B, cx, cy, Z0 - 2) : 0;
+  }
+  double pf[4][NP], rn[4], r[4][5];
+  int ci = act ? gidx(B, bx - 2, by - 2, Z0 - 2) : 0;
+#pragma unroll
+  for (int f = 0; f < 4; f++) {
+#pragma unroll
+    for (int d = 0; d < 4; d++) r[f][d] = act ? P[foff[f] + ci + (d - 2) * sz] : 0.0;
+    rn[f] = act ? P[foff[f] + ci + 2 * sz] : 0.0;
+#pragma unroll
+    for (int p = 0; p < NP; p++) pf[f][p] = fok[p] ? P[foff[f] + fidx[p]] : 0.0;
+  }
+  gptr<double> __restrict__ V = gmem(B.visc);
+  const long long sv = B.sv;
+  for (int bz = Z0; bz < Z1; bz++, ci += sz) {
+    const int buf = (bz - Z0) & 1;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+#pragma unroll
+      for (int p = 0; p < NP; p++) {
+        const int e = threadIdx.x + p * NT;
+        if (e < LX * LY) (&L[buf][f][0][0])[e] = pf[f][p];
+      }
+      r[f][4] = rn[f];
+    }
+    __syncthreads();
+    if (bz + 1 < Z1) {
+#pragma unroll
+      for (int f = 0; f < 4; f++) {
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+          if (fok[p]) pf[f][p] = P[foff[f] + fidx[p] + (long long)(bz + 1 - Z0) * sz];
+        if (act) rn[f] = P[foff[f] + ci + 3 * sz];
+      }
+    }
+    if (act) {
+      double gr[4][3];
+#pragma unroll
+      for (int f = 0; f < 4; f++) {
+        gr[f][0] = grad4(&L[buf][f][ty + 2][0], tx + 2, 1, ph.dx, ph.rdx, ph.fdx);
+        gr[f][1] = grad4(&L[buf][f][0][tx + 2], (ty + 2) * LX, LX, ph.dy, ph.rdy, ph.fdy);
+        gr[f][2] = grad4(&r[f][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
+      }
+      double o[9], mu_l;
+      visc_terms(ph, gr, r[3][2], r[0][2], r[1][2], r[2][2], o, mu_l);
+      const int i = bx - 2, j = by - 2, k = bz - 2;
+      if (i >= 0 && i < B.n[0] && j >= 0 && j < B.n[1] && k >= 0 && k < B.n[2])
+        gmem(B.mu)[i + B.n[0] * (j + B.n[1] * k)] = mu_l;
+      const int vt = bx + v0 * (by + v1 * bz);
+#pragma unroll
+      for (int q = 0; q < 9; q++) V[q * sv + vt] = o[q];
+    }
+#pragma unroll
+    for (int f = 0; f < 4; f++)
+#pragma unroll
+      for (int d = 0; d < 4; d++) r[f][d] = r[f][d + 1];
+  }
+}
+
+
 // ---- stage tail: viscous divergence + RK3 update + primitives --------------
 // Viscous divergence of interior cell (i, j, k) from the nine viscous fields
 // (src/Block3d.cpp:879-916).
@@ -275,6 +370,31 @@ __device__ __forceinline__ void update_body(const BlockDesc& B, int t, double dt
 }
 
 // ---- ghost gather of the state and its primitives ------------------------
+// The g-th ghost cell (0 <= g < sg - sn) of a block as a ghosted linear index:
+// first the NG z-planes below and above the interior (whole planes), then
+// per interior plane the NG y-rows on each side (whole rows), then per
+// interior row the NG x-cells on each side.
+__device__ __forceinline__ int ghost_cell(const BlockDesc& B, int g) {
+  const int g0 = B.g[0], g1 = B.g[1], n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  const int plane = g0 * g1;
+  const int zs = 2 * NG * plane;
+  if (g < zs) {
+    const int p = g / plane;
+    return (p < NG ? p : n2 + p) * plane + (g - p * plane);
+  }
+  g -= zs;
+  const int yrow = 2 * NG * g0;
+  if (g < n2 * yrow) {
+    const int kk = g / yrow, r = g - kk * yrow;
+    const int row = r / g0;
+    return (kk + NG) * plane + (row < NG ? row : n1 + row) * g0 + (r - row * g0);
+  }
+  g -= n2 * yrow;
+  const int row = g / (2 * NG), c = g - row * (2 * NG);
+  const int kk = row / n1, jj = row - kk * n1;
+  return (kk + NG) * plane + (jj + NG) * g0 + (c < NG ? c : n0 + c);
+}
+
 // boundary_condition (src/boundary_condition.cpp:4-231): every ghost cell
 // takes its periodic image's value, gathered from the block (of the same
 // solve: blocks[block0 + ...]) that owns the image.  The primitives of a

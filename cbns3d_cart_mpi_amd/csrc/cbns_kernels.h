@@ -96,6 +96,9 @@ hipError_t launch_scatter_aos(const BlockDesc* db, const BlockDesc* hb, const Do
 hipError_t launch_gather_aos(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
                              double* aos, hipStream_t s);
 hipError_t launch_fill_mu(const BlockDesc* db, const BlockDesc* hb, int nb, double mu0, hipStream_t s);
+// Viscous tile march: VX x VY columns per workgroup (VX * VY threads), VZ
+// planes per z-chunk.
+constexpr int VX = 64, VY = 4, VZ = 16;
 constexpr int GSUM_WG = 512;  // workgroups of the fixed-order global sums
 hipError_t launch_global_sums(const BlockDesc* db, const Domain& dom, const Phys& ph, int what, double* d_part,
                               double* d_out, hipStream_t s);

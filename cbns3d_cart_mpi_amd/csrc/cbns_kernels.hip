@@ -127,15 +127,15 @@ hipError_t launch_primitive(const BlockDesc* db, const BlockDesc* hb, int nb, in
 // gathers that value directly from the block that owns it.
 __global__ __launch_bounds__(TPB) void k_ghost_fill(const BlockDesc* __restrict__ blocks, Domain dom, int which) {
   const BlockDesc& B = blocks[blockIdx.y];
-  const int t = blockIdx.x * TPB + threadIdx.x;
-  if (t >= B.sg) return;
-  ghost_body(B, blocks, dom, 0, t, which, false);
+  const int g = blockIdx.x * TPB + threadIdx.x;
+  if (g >= B.sg - B.sn) return;
+  ghost_body(B, blocks, dom, 0, ghost_cell(B, g), which, false);
 }
 
 hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
                              hipStream_t s) {
   long long mx = 0;
-  for (int b = 0; b < dom.nb; b++) mx = hb[b].sg > mx ? hb[b].sg : mx;
+  for (int b = 0; b < dom.nb; b++) mx = std::max(mx, hb[b].sg - hb[b].sn);
   hipLaunchKernelGGL(k_ghost_fill, dim3(nblocks_for(mx), dom.nb), dim3(TPB), 0, s, db, dom, which);
   return hipGetLastError();
 }
@@ -146,100 +146,10 @@ hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Dom
 // (src/Block3d.cpp:812-868) fused per point of the extended box [-2, n+2)^3,
 // keeping the nine distinct flux components.  mu is kept on the interior for
 // calc_dt (the reference stores it everywhere; only the interior is read).
-//
-// A workgroup owns a VX x VY column tile of the box and marches VZ planes
-// along z.  Each plane of u, v, w, T with its 2-point x/y halo is staged in
-// LDS (double-buffered, one barrier per plane) for the x- and y-gradients;
-// each thread keeps its own column's five z-planes in registers for the
-// z-gradient.  The next plane's loads are issued before the current plane's
-// arithmetic.  Every value is formed by the same expressions as
-// visc_flux_body's (grad4, visc_terms), so results are identical.
-constexpr int VX = 64, VY = 4, VZ = 16;
-
+// Grid: (column tiles, z-chunks, blocks); see visc_tile_body.
 __global__ __launch_bounds__(VX * VY) void k_visc_flux(const BlockDesc* __restrict__ blocks, Phys ph) {
   const Tile tl = xcd_tile();
-  const BlockDesc& B = blocks[blockIdx.z];
-  const int v0 = B.n[0] + 2 * NGV, v1 = B.n[1] + 2 * NGV, v2 = B.n[2] + 2 * NGV;
-  const int ntx = (v0 + VX - 1) / VX, nty = (v1 + VY - 1) / VY;
-  const int tile = tl.x, Z0 = tl.y * VZ;
-  if (tile >= ntx * nty || Z0 >= v2) return;  // uniform per workgroup
-  const int Z1 = Z0 + VZ < v2 ? Z0 + VZ : v2;
-  const int X0 = (tile % ntx) * VX, Y0 = (tile / ntx) * VY;
-  const int tx = threadIdx.x % VX, ty = threadIdx.x / VX;
-  const int bx = X0 + tx, by = Y0 + ty;  // box point; its cell is (bx - 2, by - 2, bz - 2)
-  const bool act = bx < v0 && by < v1;
-  constexpr int LX = VX + 4, LY = VY + 4, NT = VX * VY, NP = (LX * LY + NT - 1) / NT;
-  __shared__ double L[2][4][LY][LX];
-  const long long sg = B.sg;
-  const int sz = B.g[0] * B.g[1];
-  gptr<const double> __restrict__ P = gmem<const double>(B.prim);
-  const long long foff[4] = {sg, 2 * sg, 3 * sg, 5 * sg};  // u, v, w, T
-  // LDS entries e = threadIdx.x + p NT this thread fills: cell (X0 + lx - 4, Y0 + ly - 4)
-  int fidx[NP];
-  bool fok[NP];
-#pragma unroll
-  for (int p = 0; p < NP; p++) {
-    const int e = threadIdx.x + p * NT;
-    const int lx = e % LX, ly = e / LX;
-    const int cx = X0 + lx - 4, cy = Y0 + ly - 4;
-    fok[p] = e < LX * LY && cx < B.n[0] + NG && cy < B.n[1] + NG;
-    fidx[p] = fok[p] ? gidx(B, cx, cy, Z0 - 2) : 0;
-  }
-  double pf[4][NP], rn[4], r[4][5];
-  int ci = act ? gidx(B, bx - 2, by - 2, Z0 - 2) : 0;
-#pragma unroll
-  for (int f = 0; f < 4; f++) {
-#pragma unroll
-    for (int d = 0; d < 4; d++) r[f][d] = act ? P[foff[f] + ci + (d - 2) * sz] : 0.0;
-    rn[f] = act ? P[foff[f] + ci + 2 * sz] : 0.0;
-#pragma unroll
-    for (int p = 0; p < NP; p++) pf[f][p] = fok[p] ? P[foff[f] + fidx[p]] : 0.0;
-  }
-  gptr<double> __restrict__ V = gmem(B.visc);
-  const long long sv = B.sv;
-  for (int bz = Z0; bz < Z1; bz++, ci += sz) {
-    const int buf = (bz - Z0) & 1;
-#pragma unroll
-    for (int f = 0; f < 4; f++) {
-#pragma unroll
-      for (int p = 0; p < NP; p++) {
-        const int e = threadIdx.x + p * NT;
-        if (e < LX * LY) (&L[buf][f][0][0])[e] = pf[f][p];
-      }
-      r[f][4] = rn[f];
-    }
-    __syncthreads();
-    if (bz + 1 < Z1) {
-#pragma unroll
-      for (int f = 0; f < 4; f++) {
-#pragma unroll
-        for (int p = 0; p < NP; p++)
-          if (fok[p]) pf[f][p] = P[foff[f] + fidx[p] + (long long)(bz + 1 - Z0) * sz];
-        if (act) rn[f] = P[foff[f] + ci + 3 * sz];
-      }
-    }
-    if (act) {
-      double gr[4][3];
-#pragma unroll
-      for (int f = 0; f < 4; f++) {
-        gr[f][0] = grad4(&L[buf][f][ty + 2][0], tx + 2, 1, ph.dx, ph.rdx, ph.fdx);
-        gr[f][1] = grad4(&L[buf][f][0][tx + 2], (ty + 2) * LX, LX, ph.dy, ph.rdy, ph.fdy);
-        gr[f][2] = grad4(&r[f][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
-      }
-      double o[9], mu_l;
-      visc_terms(ph, gr, r[3][2], r[0][2], r[1][2], r[2][2], o, mu_l);
-      const int i = bx - 2, j = by - 2, k = bz - 2;
-      if (i >= 0 && i < B.n[0] && j >= 0 && j < B.n[1] && k >= 0 && k < B.n[2])
-        gmem(B.mu)[i + B.n[0] * (j + B.n[1] * k)] = mu_l;
-      const int vt = bx + v0 * (by + v1 * bz);
-#pragma unroll
-      for (int q = 0; q < 9; q++) V[q * sv + vt] = o[q];
-    }
-#pragma unroll
-    for (int f = 0; f < 4; f++)
-#pragma unroll
-      for (int d = 0; d < 4; d++) r[f][d] = r[f][d + 1];
-  }
+  visc_tile_body(blocks[blockIdx.z], ph, tl.x, tl.y);
 }
 
 static dim3 visc_grid(const BlockDesc* hb, int nb) {
@@ -363,15 +273,15 @@ hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb,
 // a ghost cell are those of its periodic image (calc_primitive is pointwise).
 __global__ __launch_bounds__(TPB) void k_ghost_fill_prim(const BlockDesc* __restrict__ blocks, Domain dom, int which) {
   const BlockDesc& B = blocks[blockIdx.y];
-  const int t = blockIdx.x * TPB + threadIdx.x;
-  if (t >= B.sg) return;
-  ghost_body(B, blocks, dom, 0, t, which, true);
+  const int g = blockIdx.x * TPB + threadIdx.x;
+  if (g >= B.sg - B.sn) return;
+  ghost_body(B, blocks, dom, 0, ghost_cell(B, g), which, true);
 }
 
 hipError_t launch_ghost_fill_prim(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
                                   hipStream_t s) {
   long long mx = 0;
-  for (int b = 0; b < dom.nb; b++) mx = hb[b].sg > mx ? hb[b].sg : mx;
+  for (int b = 0; b < dom.nb; b++) mx = std::max(mx, hb[b].sg - hb[b].sn);
   hipLaunchKernelGGL(k_ghost_fill_prim, dim3(nblocks_for(mx), dom.nb), dim3(TPB), 0, s, db, dom, which);
   return hipGetLastError();
 }
