@@ -47,6 +47,7 @@ def parse():
                     help="face-kernel waves per SIMD, or 'auto' to tune")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-fma", action="store_true", help="skip the contracted FP64 form")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-turb", action="store_true")
@@ -98,21 +99,23 @@ def cpu_baseline(n, max_steps, budget_s):
 FIELDS = ("rho", "rho_u", "rho_v", "rho_w", "E")
 
 
-def fp32_leg(C, N, dims, steps_total, steps_timed, Q64):
-    """The FP32-flux form on the same workload: throughput over steps_timed
-    steps, and its per-field error against the FP64 state at the same step
+def form_leg(C, N, dims, steps_total, steps_timed, Q64, precision):
+    """Another arithmetic form (FP32-flux, or contracted FP64) on the same
+    workload: throughput over steps_timed steps after its own short tuning,
+    and its per-field error against the plain FP64 state at the same step
     count (relative L2 = RMS difference / RMS of the FP64 field; relative
     L-inf = max difference / max |FP64 field|)."""
     import numpy as np
 
     from cbns3d_cart_mpi_amd import autotune
 
-    s = C.Solver(C.make_params(N), dims, precision=C.CBNS_FP32_FLUX)
+    s = C.Solver(C.make_params(N), dims, precision=precision)
     s.init_tgv()
     pre = steps_total - steps_timed  # so that the error is taken at the FP64 leg's step count
+    occs = (2,) if precision == C.CBNS_FP32_FLUX else (2, 3)
     best = None
-    if pre >= 3 * 2 * len(autotune.SCHEDULES):
-        best, tms = autotune.tune(s, steps=2, bands=(16, 32), schedules=autotune.SCHEDULES, occs=(2,))
+    if pre >= 3 * 2 * len(autotune.SCHEDULES) * len(occs):
+        best, tms = autotune.tune(s, steps=2, bands=(16, 32), schedules=autotune.SCHEDULES, occs=occs)
         pre -= 3 * len(tms)
     s.run(pre)
     s.sync()
@@ -130,7 +133,7 @@ def fp32_leg(C, N, dims, steps_total, steps_timed, Q64):
     v = N ** 3 * steps_timed / el / 1e6
     return {"value": round(v, 2), "unit": "Mcell-steps/s", "ms_per_step": round(el / steps_timed * 1e3, 3),
             "error_vs_fp64_after_steps": steps_total, "error": err,
-            "autotune": {"band": best[0], "schedule": best[1]} if best else None}
+            "autotune": {"band": best[0], "schedule": best[1], "face_waves_per_simd": best[2]} if best else None}
 
 
 def sweep_leg(steps):
@@ -252,11 +255,14 @@ def main():
     ke, en = s.diagnostics()
     mem = s.memory_bytes
     steps_total = a.warmup + (len(tms) * 3 if tuned else 0) + a.steps
-    fp32 = None
-    if not a.no_fp32:
+    fp32 = fma = None
+    if not (a.no_fp32 and a.no_fma):
         Q64 = s.get_state()
         s.close()
-        fp32 = fp32_leg(C, N, dims, steps_total, a.steps, Q64)
+        if not a.no_fma:
+            fma = form_leg(C, N, dims, steps_total, a.steps, Q64, C.CBNS_FP64_FMA)
+        if not a.no_fp32:
+            fp32 = form_leg(C, N, dims, steps_total, a.steps, Q64, C.CBNS_FP32_FLUX)
         del Q64
     else:
         s.close()
@@ -302,6 +308,8 @@ def main():
         "autotune": tuned,
         "ke_enstrophy_end": [ke, en],
     }
+    if fma:
+        line["fp64_fma"] = fma
     if fp32:
         line["fp32_flux"] = fp32
     if not a.no_turb:

@@ -21,7 +21,9 @@ HIP_FLAGS = [
     "-fPIC", "-Wall", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-SOURCES = ["cbns_kernels.hip", "cbns_capi.cpp", "cbns_batch.hip", "cbns_batch_capi.cpp"]
+SOURCES = ["cbns_kernels.hip", "cbns_kernels_fma.hip", "cbns_capi.cpp", "cbns_batch.hip", "cbns_batch_capi.cpp"]
+# the contracted FP64 form's face kernels are the only code built with FMA contraction
+CONTRACT = {"cbns_kernels_fma.hip": ["-ffp-contract=fast"]}
 
 
 def run(cmd):
@@ -35,7 +37,8 @@ def build_lib(verbose_resources=False):
         src = os.path.join(CSRC, s)
         obj = os.path.join(CSRC, s.rsplit(".", 1)[0] + ".o")
         extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources and s.endswith(".hip") else []
-        run([HIPCC, *HIP_FLAGS, *extra, "-x", "hip", "-c", src, "-o", obj])
+        flags = [f for f in HIP_FLAGS if not (s in CONTRACT and f.startswith("-ffp-contract"))]
+        run([HIPCC, *flags, *CONTRACT.get(s, []), *extra, "-x", "hip", "-c", src, "-o", obj])
         objs.append(obj)
     run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", LIB])
     return LIB

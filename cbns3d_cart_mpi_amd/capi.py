@@ -16,6 +16,7 @@ HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "cbns3d_amd.h")
 
 CBNS_FP64 = 0
 CBNS_FP32_FLUX = 1
+CBNS_FP64_FMA = 2
 CBNS_Q = 0
 CBNS_QP = 1
 

@@ -16,6 +16,23 @@ __device__ __forceinline__ int gidx(const BlockDesc& B, int i, int j, int k) {
   return (i + NG) + B.g[0] * ((j + NG) + B.g[1] * (k + NG));
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Stencil kernels want neighbouring rows on the same L2, so the linear
+// workgroup id is remapped so that XCD r runs one contiguous range of logical
+// tiles (a bijection on [0, total) for any total).
+struct Tile {
+  int x, y;  // logical (blockIdx.x, blockIdx.y)
+};
+__device__ __forceinline__ Tile xcd_tile() {
+  const int gx = gridDim.x;
+  const int total = gx * gridDim.y;
+  const int lin = blockIdx.y * gx + blockIdx.x;
+  const int base = total / NXCD, rem = total % NXCD;
+  const int r = lin % NXCD, q = lin / NXCD;
+  const int logical = r * base + (r < rem ? r : rem) + q;
+  return {logical % gx, logical / gx};
+}
+
 // Work index t -> (i, j, k) of an ni x nj x nk box in band order
 // (j-band of jb rows, k, j within the band, i): a contiguous run of t covers
 // a band of rows over successive planes, so a run of workgroups on one XCD
@@ -82,12 +99,12 @@ __device__ __forceinline__ void face_load(const BlockDesc& B, int t, int which, 
   }
 }
 
-template <int D>
+template <int D, bool FMA = false>
 __device__ __forceinline__ void face_body(const BlockDesc& B, int t, int which, int jb, const Phys& ph) {
   double q[6][5], pl[5], pr[5], fl[5];
   int fi;
   face_load<D>(B, t, which, jb, q, pl, pr, fi);
-  face_flux<D>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl);
+  face_flux<D, FMA>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl);
   gptr<double> __restrict__ F = gmem(B.F[D]);
 #pragma unroll
   for (int e = 0; e < 5; e++) F[e * B.sf[D] + fi] = fl[e];

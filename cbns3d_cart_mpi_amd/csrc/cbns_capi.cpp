@@ -161,7 +161,7 @@ int cbns_device_count(int* n) {
 int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx** out) {
   if (!p || !dims || !out) return CBNS_ERR_ARG;
   *out = nullptr;
-  if (precision != CBNS_FP64 && precision != CBNS_FP32_FLUX) return CBNS_ERR_ARG;
+  if (precision != CBNS_FP64 && precision != CBNS_FP32_FLUX && precision != CBNS_FP64_FMA) return CBNS_ERR_ARG;
   const int N[3] = {(int)p->IM, (int)p->JM, (int)p->KM};
   for (int a = 0; a < 3; a++) {
     if (dims[a] < 1 || N[a] < 1) return CBNS_ERR_ARG;

@@ -57,7 +57,26 @@ __device__ __forceinline__ double divc(double x, double h, double y, int fast) {
   return x / h;
 }
 
+// a / b.  FMA = false: the IEEE quotient.  FMA = true (the contracted FP64
+// form): the same reciprocal refinement and one-FMA correction as the
+// compiler's IEEE sequence without its range scaling and special-case
+// fix-up, so the quotient is still correctly rounded for operands of
+// moderate magnitude (nonzero, |a|, |b| between about 2^-300 and 2^300).
+template <bool FMA>
+__device__ __forceinline__ double qdiv(double a, double b) {
+  if constexpr (!FMA) {
+    return a / b;
+  } else {
+    double y = __builtin_amdgcn_rcp(b);
+    y = __builtin_fma(y, __builtin_fma(-b, y, 1.0), y);
+    y = __builtin_fma(y, __builtin_fma(-b, y, 1.0), y);
+    const double q = a * y;
+    return __builtin_fma(__builtin_fma(-b, q, a), y, q);
+  }
+}
+
 // src/weno.cpp:25-56
+template <bool FMA = false>
 __device__ __forceinline__ double weno5(double Um2, double Um1, double U0, double Up1, double Up2) {
   const double eps = 1.0e-6;
   double diff1 = Um2 - 2.0 * Um1 + U0;
@@ -69,22 +88,23 @@ __device__ __forceinline__ double weno5(double Um2, double Um1, double U0, doubl
   double IS1 = (13.0 / 12.0) * diff1 * diff1 + 0.25 * diff2 * diff2 + eps;
   double IS2 = (13.0 / 12.0) * diff3 * diff3 + 0.25 * diff4 * diff4 + eps;
   double IS3 = (13.0 / 12.0) * diff5 * diff5 + 0.25 * diff6 * diff6 + eps;
-  double w1 = 0.1 / (IS1 * IS1);
-  double w2 = 0.6 / (IS2 * IS2);
-  double w3 = 0.3 / (IS3 * IS3);
+  double w1 = qdiv<FMA>(0.1, IS1 * IS1);
+  double w2 = qdiv<FMA>(0.6, IS2 * IS2);
+  double w3 = qdiv<FMA>(0.3, IS3 * IS3);
   double sum = w1 + w2 + w3;
   double f1 = div6(2.0 * Um2 - 7.0 * Um1 + 11.0 * U0);
   double f2 = div6(-Um1 + 5.0 * U0 + 2.0 * Up1);
   double f3 = div6(2.0 * U0 + 5.0 * Up1 - Up2);
-  return (w1 * f1 + w2 * f2 + w3 * f3) / sum;
+  return qdiv<FMA>(w1 * f1 + w2 * f2 + w3 * f3, sum);
 }
 
 // Reconstructed conserved state -> primitive (src/weno.cpp:146-150).
+template <bool FMA = false>
 __device__ __forceinline__ void cons_to_prim(const double (&a)[5], double gam1, double (&s)[5]) {
   double rho = a[0];
-  double u = a[1] / rho;
-  double v = a[2] / rho;
-  double w = a[3] / rho;
+  double u = qdiv<FMA>(a[1], rho);
+  double v = qdiv<FMA>(a[2], rho);
+  double w = qdiv<FMA>(a[3], rho);
   s[0] = rho;
   s[1] = u;
   s[2] = v;
@@ -95,20 +115,22 @@ __device__ __forceinline__ void cons_to_prim(const double (&a)[5], double gam1, 
 // Face flux along axis D (0 = x, 1 = y, 2 = z).
 //   q[c][e]  conserved values of cells i-3 .. i+2 along D (c = 0..5)
 //   pl, pr   rho, u, v, w, p of cells i-1 and i (calc_primitive of the same Q)
-template <int D>
+// FMA = true is the contracted FP64 form: the same expressions in a
+// translation unit that lets the compiler fuse a * b + c, with qdiv<true>.
+template <int D, bool FMA = false>
 __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double (&pl)[5],
                                           const double (&pr)[5], double gamma, double gam1,
                                           double rg1, int fg1, double (&flux)[5]) {
   // ---- Roe average (src/eigenvectors_roe.cpp:87-106)
-  double c_L = sqrt(gamma * pl[4] / pl[0]);
+  double c_L = sqrt(qdiv<FMA>(gamma * pl[4], pl[0]));
   double H_L = divc(c_L * c_L, gam1, rg1, fg1) + 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
-  double c_R = sqrt(gamma * pr[4] / pr[0]);
+  double c_R = sqrt(qdiv<FMA>(gamma * pr[4], pr[0]));
   double H_R = divc(c_R * c_R, gam1, rg1, fg1) + 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
-  double ratio = sqrt(pr[0] / pl[0]);
-  double uu = (pl[1] + ratio * pr[1]) / (1.0 + ratio);
-  double vv = (pl[2] + ratio * pr[2]) / (1.0 + ratio);
-  double ww = (pl[3] + ratio * pr[3]) / (1.0 + ratio);
-  double H = (H_L + ratio * H_R) / (1.0 + ratio);
+  double ratio = sqrt(qdiv<FMA>(pr[0], pl[0]));
+  double uu = qdiv<FMA>(pl[1] + ratio * pr[1], 1.0 + ratio);
+  double vv = qdiv<FMA>(pl[2] + ratio * pr[2], 1.0 + ratio);
+  double ww = qdiv<FMA>(pl[3] + ratio * pr[3], 1.0 + ratio);
+  double H = qdiv<FMA>(H_L + ratio * H_R, 1.0 + ratio);
   double q2 = uu * uu + vv * vv + ww * ww;
   double c = sqrt(gam1 * (H - 0.5 * q2));
   // frame (n, l, m) per axis (src/eigenvectors_roe.cpp:48-85):
@@ -118,8 +140,8 @@ __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double 
   const double qn = D == 0 ? uu : (D == 1 ? vv : ww);
 
   // ---- L rows (src/eigenvectors_roe.cpp:138-169)
-  double K = gam1 / (2.0 * c * c);
-  double one_2c = 0.5 / c;
+  double K = qdiv<FMA>(gam1, 2.0 * c * c);
+  double one_2c = qdiv<FMA>(0.5, c);
   double L00 = 0.5 * K * q2 + one_2c * qn;
   double L10 = 1.0 - K * q2;
   double L20 = 0.5 * K * q2 - one_2c * qn;
@@ -164,8 +186,8 @@ __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double 
       }
     }
     // left state from cells i-3..i+1, right state from cells i+2..i-2
-    const double wl = weno5(fc[0], fc[1], fc[2], fc[3], fc[4]);
-    const double wr = weno5(fc[5], fc[4], fc[3], fc[2], fc[1]);
+    const double wl = weno5<FMA>(fc[0], fc[1], fc[2], fc[3], fc[4]);
+    const double wr = weno5<FMA>(fc[5], fc[4], fc[3], fc[2], fc[1]);
     if (m == 0) {
       aL[0] = wl;              aR[0] = wr;
       aL[1] = Rm0[0] * wl;     aR[1] = Rm0[0] * wr;
@@ -207,14 +229,14 @@ __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double 
   }
 
   double sL[5], sR[5];
-  cons_to_prim(aL, gam1, sL);
-  cons_to_prim(aR, gam1, sR);
+  cons_to_prim<FMA>(aL, gam1, sL);
+  cons_to_prim<FMA>(aR, gam1, sR);
 
   // ---- local Lax-Friedrichs flux (src/lf_flux.cpp:32-99)
-  double cR = sqrt(gamma * sR[4] / sR[0]);
+  double cR = sqrt(qdiv<FMA>(gamma * sR[4], sR[0]));
   double HR = divc(cR * cR, gam1, rg1, fg1) + 0.5 * (sR[1] * sR[1] + sR[2] * sR[2] + sR[3] * sR[3]);
   double rER = divc(sR[4], gam1, rg1, fg1) + 0.5 * sR[0] * (sR[1] * sR[1] + sR[2] * sR[2] + sR[3] * sR[3]);
-  double cL = sqrt(gamma * sL[4] / sL[0]);
+  double cL = sqrt(qdiv<FMA>(gamma * sL[4], sL[0]));
   double HL = divc(cL * cL, gam1, rg1, fg1) + 0.5 * (sL[1] * sL[1] + sL[2] * sL[2] + sL[3] * sL[3]);
   double rEL = divc(sL[4], gam1, rg1, fg1) + 0.5 * sL[0] * (sL[1] * sL[1] + sL[2] * sL[2] + sL[3] * sL[3]);
   const double qR = sR[1 + D], qL = sL[1 + D];
@@ -254,6 +276,8 @@ __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double 
 // Large backgrounds (p ~ 1/(gamma M^2), E) therefore never meet FP32
 // rounding directly.  Roe averages are taken in FP32 except at
 // near-vacuum or strong-gradient faces, where they are taken in FP64.
+// Unlike the plain path, this form lets the compiler contract a * b + c
+// into fused multiply-adds.
 struct RefState {
   double rho, E, p;  // rho_g, E_g, p_g = (gamma - 1) E_g
 };
@@ -262,6 +286,7 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 __device__ __forceinline__ float weno5f(float Um2, float Um1, float U0, float Up1, float Up2) {
+#pragma clang fp contract(fast)
   const float eps = 1.0e-6f;
   float diff1 = Um2 - 2.0f * Um1 + U0;
   float diff2 = Um2 - 4.0f * Um1 + 3.0f * U0;
@@ -285,6 +310,7 @@ template <int D>
 __device__ __forceinline__ void face_flux_f32(const double (&q)[6][5], const double (&pl)[5],
                                               const double (&pr)[5], double gamma, double gam1,
                                               const RefState& g, float (&flux)[5]) {
+#pragma clang fp contract(fast)
   const float gf = (float)gamma, g1 = (float)gam1;
   // ---- Roe average
   float uu, vv, ww, H, c;

@@ -69,7 +69,12 @@ struct Launch {
   int maxcount;              // per-kernel max work per block (set by launcher)
 };
 
-// kernel launchers (cbns_kernels.hip)
+// Arithmetic of the face fluxes (the values of cbns_precision in the C ABI).
+constexpr int PREC_FP64 = 0, PREC_FP32_FLUX = 1, PREC_FP64_FMA = 2;
+
+// kernel launchers (cbns_kernels.hip; launch_face_flux_fma in cbns_kernels_fma.hip)
+hipError_t launch_face_flux_fma(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
+                                const Phys& ph, int jb, int occ, hipStream_t s);
 struct RefState;
 hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
                             const Phys& ph, int precision, const RefState& g, int jb, int occ, hipStream_t s);

@@ -14,22 +14,6 @@ namespace cbns {
 static inline unsigned nblocks_for(long long count) { return (unsigned)((count + TPB - 1) / TPB); }
 
 
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2.  Stencil kernels want neighbouring rows on the same L2, so the linear
-// workgroup id is remapped so that XCD r runs one contiguous range of logical
-// tiles (a bijection on [0, total) for any total).
-struct Tile {
-  int x, y;  // logical (blockIdx.x, blockIdx.y)
-};
-__device__ __forceinline__ Tile xcd_tile() {
-  const int gx = gridDim.x;
-  const int total = gx * gridDim.y;
-  const int lin = blockIdx.y * gx + blockIdx.x;
-  const int base = total / NXCD, rem = total % NXCD;
-  const int r = lin % NXCD, q = lin / NXCD;
-  const int logical = r * base + (r < rem ? r : rem) + q;
-  return {logical % gx, logical / gx};
-}
 
 
 // ---------------------------------------------------------------------------
@@ -66,8 +50,9 @@ hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, in
     long long c = (long long)(hb[b].n[0] + (dir == 0)) * (hb[b].n[1] + (dir == 1)) * (hb[b].n[2] + (dir == 2));
     mx = c > mx ? c : mx;
   }
+  if (precision == PREC_FP64_FMA) return launch_face_flux_fma(db, hb, nb, dir, which, ph, jb, occ, s);
   dim3 grid(nblocks_for(mx), nb);
-  if (precision == 0) {
+  if (precision == PREC_FP64) {
 #define CBNS_FACE(DIR, OCC) \
   hipLaunchKernelGGL((k_face_flux<DIR, OCC>), grid, dim3(TPB), 0, s, db, which, jb, ph)
     if (occ >= 3) {
@@ -232,7 +217,7 @@ hipError_t launch_update(const BlockDesc* db, const BlockDesc* hb, int nb, doubl
                          const Phys& ph, int precision, int jb, hipStream_t s) {
   long long mn = 0;
   for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
-  if (precision == 0)
+  if (precision != PREC_FP32_FLUX)
     hipLaunchKernelGGL(k_update<double>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, dt, stage, ph, jb);
   else
     hipLaunchKernelGGL(k_update<float>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, dt, stage, ph, jb);
@@ -264,7 +249,7 @@ hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb,
   for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
   if (stage < 1 || stage > 3) return hipSuccess;
   const dim3 grid(nblocks_for(mn), nb);
-  if (precision == 0) hipLaunchKernelGGL(k_update_fused<double>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb);
+  if (precision != PREC_FP32_FLUX) hipLaunchKernelGGL(k_update_fused<double>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb);
   else hipLaunchKernelGGL(k_update_fused<float>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb);
   return hipGetLastError();
 }

@@ -61,10 +61,15 @@ typedef struct cbns_params {
 
 /* Arithmetic of the inviscid face flux. */
 typedef enum cbns_precision {
-  CBNS_FP64 = 0,      /* the plain path: every operation in FP64, as the reference */
-  CBNS_FP32_FLUX = 1  /* WENO, characteristic projections and LF flux in FP32 on
+  CBNS_FP64 = 0,      /* the plain path: every operation in FP64, rounded as the
+                         reference's (bit-identical results) */
+  CBNS_FP32_FLUX = 1, /* WENO, characteristic projections and LF flux in FP32 on
                          deviations from a reference state; state, Roe averages,
                          viscous terms, RK3 and reductions in FP64 */
+  CBNS_FP64_FMA = 2   /* the plain path with the face fluxes' a * b + c fused
+                         into FMAs and quotients without range scaling: FP64
+                         throughout, within a few roundings of CBNS_FP64 per
+                         face; single contexts only (not the batched form) */
 } cbns_precision;
 
 typedef struct cbns_ctx cbns_ctx;
