@@ -44,7 +44,8 @@ constexpr int BATCH_WG = 1024;  // persistent workgroups per launch
 
 hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int dir, int which, int jb,
                              int precision, hipStream_t s);
-hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, hipStream_t s);
+hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, int precision,
+                             hipStream_t s);
 hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int stage, int jb,
                                int precision, hipStream_t s);
 hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, hipStream_t s);

@@ -36,6 +36,7 @@ __global__ __launch_bounds__(TPB, 2) void kb_face(const BlockDesc* __restrict__ 
   }
 }
 
+template <typename VT>
 __global__ __launch_bounds__(TPB) void kb_visc(const BlockDesc* __restrict__ blocks,
                                                const InstDesc* __restrict__ insts, TileTable tt, int jb) {
   int first, last;
@@ -47,7 +48,7 @@ __global__ __launch_bounds__(TPB) void kb_visc(const BlockDesc* __restrict__ blo
     if (!I.stepping) continue;
     // one (column tile, z-chunk) unit of the viscous tile march per entry
     const int ntiles = ((B.n[0] + 2 * NGV + VX - 1) / VX) * ((B.n[1] + 2 * NGV + VY - 1) / VY);
-    visc_tile_body(B, I.ph, tr.tile % ntiles, tr.tile / ntiles);
+    visc_tile_body<VT>(B, I.ph, tr.tile % ntiles, tr.tile / ntiles);
   }
 }
 
@@ -169,9 +170,13 @@ hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable
   return hipGetLastError();
 }
 
-hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, hipStream_t s) {
+hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, int precision,
+                             hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_visc, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt, jb);
+  if (precision == PREC_FP32_FLUX)
+    hipLaunchKernelGGL(kb_visc<float>, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt, jb);
+  else
+    hipLaunchKernelGGL(kb_visc<double>, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt, jb);
   return hipGetLastError();
 }
 

@@ -37,7 +37,7 @@ struct Inst {
   int block0 = 0;
 };
 
-enum { T_FACE64 = 0, T_FACE32 = 3, T_VISC = 6, T_UPD64, T_UPD32, T_GHOST, T_DT, T_COUNT };
+enum { T_FACE64 = 0, T_FACE32 = 3, T_VISC64 = 6, T_VISC32, T_UPD64, T_UPD32, T_GHOST, T_DT, T_COUNT };
 
 }  // namespace
 
@@ -153,7 +153,8 @@ static int rebuild(cbns_batch* b) {
       for (int d = 0; d < 3; d++) add((I.precision ? T_FACE32 : T_FACE64) + d, B.sf[d]);
       const int vunits = ((B.n[0] + 2 * NGV + VX - 1) / VX) * ((B.n[1] + 2 * NGV + VY - 1) / VY) *
                          ((B.n[2] + 2 * NGV + VZ - 1) / VZ);
-      for (int t = 0; t < vunits; t++) tab[T_VISC].push_back({bi, t});  // whole-workgroup units
+      for (int t = 0; t < vunits; t++)  // whole-workgroup units
+        tab[I.precision ? T_VISC32 : T_VISC64].push_back({bi, t});
       add(I.precision ? T_UPD32 : T_UPD64, B.sn);
       add(T_GHOST, B.sg - B.sn);  // ghost cells only
       add(T_DT, B.sn);
@@ -189,7 +190,8 @@ static int rebuild(cbns_batch* b) {
   for (int stage = 1; stage <= 3 && e == hipSuccess; stage++) {
     const int in = stage == 1 ? CBNS_Q : CBNS_QP;
     const int out = stage == 3 ? CBNS_Q : CBNS_QP;
-    e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC], b->jb, s);
+    e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC64], b->jb, 0, s);
+    if (e == hipSuccess) e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC32], b->jb, 1, s);
     for (int d = 0; d < 3 && e == hipSuccess; d++) {
       e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE64 + d], d, in, b->jb, 0, s);
       if (e == hipSuccess) e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE32 + d], d, in, b->jb, 1, s);
@@ -326,7 +328,7 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
         ok = (B.Q[0] = alloc(8 * NEQ * B.sg)) && (B.Q[1] = alloc(8 * NEQ * B.sg)) &&
              (B.F[0] = alloc(8 * NEQ * B.sf[0])) && (B.F[1] = alloc(8 * NEQ * B.sf[1])) &&
              (B.F[2] = alloc(8 * NEQ * B.sf[2])) && (B.prim = alloc(8 * 6 * B.sg)) && (B.mu = alloc(8 * B.sn)) &&
-             (B.visc = alloc(8 * 9 * B.sv));
+             (B.visc = alloc((precision == CBNS_FP32_FLUX ? 4 : 8) * 9 * B.sv));
         I.blocks.push_back(B);
       }
   if (ok) ok = (I.dblocks = (BlockDesc*)alloc(sizeof(BlockDesc) * I.blocks.size())) != nullptr;

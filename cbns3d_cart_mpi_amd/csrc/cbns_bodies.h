@@ -65,7 +65,8 @@ __device__ __forceinline__ double grad4(P __restrict__ f, int c, int s, double h
 }
 
 // Divergence stencil of calc_viscous_flux_contribution (src/Block3d.cpp:894-908).
-__device__ __forceinline__ double cdiff4(gptr<const double> __restrict__ f, int c, int s) {
+template <typename VT>
+__device__ __forceinline__ double cdiff4(gptr<const VT> __restrict__ f, int c, int s) {
   const double c0 = 1.0 / 12.0, c1 = -2.0 / 3.0;
   return c0 * f[c - 2 * s] + c1 * f[c - s] - c1 * f[c + s] - c0 * f[c + 2 * s];
 }
@@ -126,12 +127,9 @@ __device__ __forceinline__ void face_body_f32(const BlockDesc& B, int t, int whi
 // ---- viscous fluxes on the extended box [-2, n+2)^3 -----------------------
 // calc_gradient (src/calc_gradient.cpp:17-66), calc_viscous_terms
 // (src/Block3d.cpp:750-810) and calc_viscous_flux (src/Block3d.cpp:812-868),
-// keeping the nine distinct components txx tyy tzz txy txz tyz ex ey ez;
-// mu is kept on the interior for calc_dt.
-__device__ __forceinline__ int visc_count(const BlockDesc& B) {
-  return (B.n[0] + 2 * NGV) * (B.n[1] + 2 * NGV) * (B.n[2] + 2 * NGV);
-}
-
+// keeping the nine distinct components txx tyy tzz txy txz tyz ex ey ez
+// (stored as VT: double for the FP64 forms, float for FP32-flux); mu is kept
+// on the interior for calc_dt.
 // Sutherland viscosity, stress tensor, heat flux and energy fluxes of one
 // point from its gradients gr[u, v, w, T][x, y, z], temperature and velocity
 // (src/Block3d.cpp:777-804, 843-862).  out = txx tyy tzz txy txz tyz ex ey ez;
@@ -164,49 +162,16 @@ __device__ __forceinline__ void visc_terms(const Phys& ph, const double (&gr)[4]
   out[8] = uu * txz + vv * tyz + ww * tzz - qz;
 }
 
-__device__ __forceinline__ void visc_flux_body(const BlockDesc& B, int t, const Phys& ph, int jb) {
-  const int v0 = B.n[0] + 2 * NGV, v1 = B.n[1] + 2 * NGV, v2 = B.n[2] + 2 * NGV;
-  int i, j, k;
-  band_decode(t, v0, v1, v2, jb, i, j, k);
-  const int vt = i + v0 * (j + v1 * k);
-  i -= NGV;
-  j -= NGV;
-  k -= NGV;
-  const int c = gidx(B, i, j, k);
-  const int sy = B.g[0], sz = B.g[0] * B.g[1];
-  const long long sg = B.sg;
-  gptr<const double> __restrict__ u = gmem<const double>(B.prim + sg);
-  gptr<const double> __restrict__ v = gmem<const double>(B.prim + 2 * sg);
-  gptr<const double> __restrict__ w = gmem<const double>(B.prim + 3 * sg);
-  gptr<const double> __restrict__ T = gmem<const double>(B.prim + 5 * sg);
-  double gr[4][3];
-  gptr<const double> f4[4] = {u, v, w, T};
-#pragma unroll
-  for (int f = 0; f < 4; f++) {
-    gr[f][0] = grad4(f4[f], c, 1, ph.dx, ph.rdx, ph.fdx);
-    gr[f][1] = grad4(f4[f], c, sy, ph.dy, ph.rdy, ph.fdy);
-    gr[f][2] = grad4(f4[f], c, sz, ph.dz, ph.rdz, ph.fdz);
-  }
-  double o[9], mu_l;
-  visc_terms(ph, gr, T[c], u[c], v[c], w[c], o, mu_l);
-  if (i >= 0 && i < B.n[0] && j >= 0 && j < B.n[1] && k >= 0 && k < B.n[2])
-    gmem(B.mu)[i + B.n[0] * (j + B.n[1] * k)] = mu_l;
-  gptr<double> __restrict__ V = gmem(B.visc);
-  const long long sv = B.sv;
-#pragma unroll
-  for (int q = 0; q < 9; q++) V[q * sv + vt] = o[q];
-}
-
 // Tile march of the viscous fluxes.  A workgroup owns a VX x VY column tile
 // of the extended box and marches VZ planes along z.  Each plane of u, v, w,
 // T with its 2-point x/y halo is staged in LDS (double-buffered, one barrier
 // per plane) for the x- and y-gradients; each thread keeps its own column's
 // five z-planes in registers for the z-gradient.  The next plane's loads are
 // issued before the current plane's arithmetic.  Every value is formed by the
-// same expressions as visc_flux_body's (grad4, visc_terms), so results are
-// identical.  One call = one (column tile, z-chunk) unit, made by all
+// same expressions as a one-point-per-thread evaluation (grad4, visc_terms).  One call = one (column tile, z-chunk) unit, made by all
 // VX * VY threads of a workgroup with the same tile and chunk (it
 // synchronises the workgroup).
+template <typename VT>
 __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& ph, int tile, int chunk) {
   const int v0 = B.n[0] + 2 * NGV, v1 = B.n[1] + 2 * NGV, v2 = B.n[2] + 2 * NGV;
   const int ntx = (v0 + VX - 1) / VX, nty = (v1 + VY - 1) / VY;
@@ -244,7 +209,7 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
 #pragma unroll
     for (int p = 0; p < NP; p++) pf[f][p] = fok[p] ? P[foff[f] + fidx[p]] : 0.0;
   }
-  gptr<double> __restrict__ V = gmem(B.visc);
+  gptr<VT> __restrict__ V = gmem((VT*)B.visc);
   const long long sv = B.sv;
   for (int bz = Z0; bz < Z1; bz++, ci += sz) {
     const int buf = (bz - Z0) & 1;
@@ -282,7 +247,7 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
         gmem(B.mu)[i + B.n[0] * (j + B.n[1] * k)] = mu_l;
       const int vt = bx + v0 * (by + v1 * bz);
 #pragma unroll
-      for (int q = 0; q < 9; q++) V[q * sv + vt] = o[q];
+      for (int q = 0; q < 9; q++) V[q * sv + vt] = (VT)o[q];
     }
 #pragma unroll
     for (int f = 0; f < 4; f++)
@@ -295,11 +260,12 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
 // ---- stage tail: viscous divergence + RK3 update + primitives --------------
 // Viscous divergence of interior cell (i, j, k) from the nine viscous fields
 // (src/Block3d.cpp:879-916).
+template <typename VT>
 __device__ __forceinline__ void visc_div(const BlockDesc& B, int i, int j, int k, const Phys& ph,
                                          double (&dfv)[4]) {
   const int v0 = B.n[0] + 2 * NGV, v1 = B.n[1] + 2 * NGV;
   const int cv = (i + NGV) + v0 * ((j + NGV) + v1 * (k + NGV));
-  gptr<const double> __restrict__ V = gmem<const double>(B.visc);
+  gptr<const VT> __restrict__ V = gmem<const VT>((const VT*)B.visc);
   const long long sv = B.sv;
   // Ev = (txx, txy, txz, ex), Fv = (txy, tyy, tyz, ey), Gv = (txz, tyz, tzz, ez)
   const int Ef[4] = {0, 3, 4, 6}, Ff[4] = {3, 1, 5, 7}, Gf[4] = {4, 5, 2, 8};
@@ -314,12 +280,13 @@ __device__ __forceinline__ void visc_div(const BlockDesc& B, int i, int j, int k
 
 // Viscous divergence of the interior cell at work index t (band order) into
 // B.dfv (src/Block3d.cpp:879-916).  t < n0 n1 n2.
+template <typename VT>
 __device__ __forceinline__ void visc_div_body(const BlockDesc& B, int t, const Phys& ph, int jb) {
   const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
   int i, j, k;
   band_decode(t, n0, n1, n2, jb, i, j, k);
   double dfv[4];
-  visc_div(B, i, j, k, ph, dfv);
+  visc_div<VT>(B, i, j, k, ph, dfv);
   const int ct = i + n0 * (j + n1 * k);
 #pragma unroll
   for (int e = 0; e < 4; e++) gmem(B.dfv)[e * B.sn + ct] = dfv[e];
@@ -356,7 +323,7 @@ __device__ __forceinline__ void rk3_prim(const double (&df)[NEQ], gptr<const dou
 
 // src/Block3d.cpp:879-916, 920-962 and 359-385 for one interior cell; stage
 // 1..3 only, in place on Q / Q_p as the reference.  FT is the face-array type
-// (double, or float for FP32-flux).
+// (double, or float for FP32-flux), which is also the viscous fields' type.
 template <typename FT>
 __device__ __forceinline__ void update_body(const BlockDesc& B, int t, double dt, int stage, const Phys& ph,
                                             int jb) {
@@ -366,7 +333,7 @@ __device__ __forceinline__ void update_body(const BlockDesc& B, int t, double dt
   const int ct = i + n0 * (j + n1 * k);
   const int c = gidx(B, i, j, k);
   double dfv[4];
-  visc_div(B, i, j, k, ph, dfv);
+  visc_div<FT>(B, i, j, k, ph, dfv);
   const int iE = i + (n0 + 1) * (j + n1 * k);
   const int iF = i + n0 * (j + (n1 + 1) * k);
   const int iG = ct;

@@ -242,7 +242,9 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
         B.prim = (double*)q;
         if ((st = dev_alloc(c, &q, sizeof(double) * B.sn))) break;
         B.mu = (double*)q;
-        if ((st = dev_alloc(c, &q, sizeof(double) * 9 * B.sv))) break;
+        // viscous fields: float in the FP32-flux form, double otherwise
+        const size_t vbytes = precision == CBNS_FP32_FLUX ? sizeof(float) : sizeof(double);
+        if ((st = dev_alloc(c, &q, vbytes * 9 * B.sv))) break;
         B.visc = (double*)q;
       }
   void* q = nullptr;
@@ -358,7 +360,7 @@ int cbns_rec_riemann(cbns_ctx* c, int which, int dir) {
 
 int cbns_calc_viscous_flux_contribution(cbns_ctx* c) {
   if (!c) return CBNS_ERR_ARG;
-  CK(launch_viscous(c->db, c->hb.data(), c->dom.nb, c->ph, c->jband, c->stream));
+  CK(launch_viscous(c->db, c->hb.data(), c->dom.nb, c->ph, c->precision, c->jband, c->stream));
   return CBNS_OK;
 }
 
@@ -521,7 +523,9 @@ static int run_stages(cbns_ctx* c, double dt) {
     if (c->overlap) {
       CK(hipEventRecord(c->ev_fork, c->stream));
       CK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-      CK(timed_on(c, c->side, 3, [&] { return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->jband, c->side); }));
+      CK(timed_on(c, c->side, 3, [&] {
+        return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->side);
+      }));
       CK(hipEventRecord(c->ev_join, c->side));
     }
     for (int d = 0; d < 3; d++)
@@ -530,7 +534,7 @@ static int run_stages(cbns_ctx* c, double dt) {
                                 c->stream);
       }));
     if (c->overlap) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
-    else CK(timed(c, 3, [&] { return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->jband, c->stream); }));
+    else CK(timed(c, 3, [&] { return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->stream); }));
     CK(timed(c, 4, [&] {
       return launch_update_fused(c->db, hb, c->dom.nb, dt, stage, c->ph, c->precision, c->jband, c->stream);
     }));

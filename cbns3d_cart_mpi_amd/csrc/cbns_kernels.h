@@ -82,8 +82,11 @@ hipError_t launch_primitive(const BlockDesc* db, const BlockDesc* hb, int nb, in
                             hipStream_t s);
 hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
                              hipStream_t s);
-hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int jb, hipStream_t s);
-hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int jb, hipStream_t s);
+// viscous fields stored as float for PREC_FP32_FLUX, double otherwise
+hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision, int jb,
+                          hipStream_t s);
+hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision,
+                            int jb, hipStream_t s);
 hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
                                const Phys& ph, int precision, int jb, hipStream_t s);
 hipError_t launch_ghost_fill_prim(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,

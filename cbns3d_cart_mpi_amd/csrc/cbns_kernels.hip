@@ -132,9 +132,10 @@ hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Dom
 // keeping the nine distinct flux components.  mu is kept on the interior for
 // calc_dt (the reference stores it everywhere; only the interior is read).
 // Grid: (column tiles, z-chunks, blocks); see visc_tile_body.
+template <typename VT>
 __global__ __launch_bounds__(VX * VY) void k_visc_flux(const BlockDesc* __restrict__ blocks, Phys ph) {
   const Tile tl = xcd_tile();
-  visc_tile_body(blocks[blockIdx.z], ph, tl.x, tl.y);
+  visc_tile_body<VT>(blocks[blockIdx.z], ph, tl.x, tl.y);
 }
 
 static dim3 visc_grid(const BlockDesc* hb, int nb) {
@@ -149,27 +150,35 @@ static dim3 visc_grid(const BlockDesc* hb, int nb) {
 
 // Divergence part of calc_viscous_flux_contribution (src/Block3d.cpp:879-916).
 
+template <typename VT>
 __global__ __launch_bounds__(TPB) void k_visc_div(const BlockDesc* __restrict__ blocks, Phys ph, int jb) {
   const Tile tl = xcd_tile();
   const BlockDesc& B = blocks[tl.y];
   const int t = tl.x * TPB + threadIdx.x;
   if (t >= B.n[0] * B.n[1] * B.n[2]) return;
-  visc_div_body(B, t, ph, jb);
+  visc_div_body<VT>(B, t, ph, jb);
 }
 
-hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int jb, hipStream_t s) {
+hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision,
+                            int jb, hipStream_t s) {
   (void)jb;  // the tile march fixes its own order
-  hipLaunchKernelGGL(k_visc_flux, visc_grid(hb, nb), dim3(VX * VY), 0, s, db, ph);
+  if (precision == PREC_FP32_FLUX)
+    hipLaunchKernelGGL(k_visc_flux<float>, visc_grid(hb, nb), dim3(VX * VY), 0, s, db, ph);
+  else
+    hipLaunchKernelGGL(k_visc_flux<double>, visc_grid(hb, nb), dim3(VX * VY), 0, s, db, ph);
   return hipGetLastError();
 }
 
-hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int jb, hipStream_t s) {
+hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision, int jb,
+                          hipStream_t s) {
   long long mn = 0;
   for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
-  hipLaunchKernelGGL(k_visc_flux, visc_grid(hb, nb), dim3(VX * VY), 0, s, db, ph);
-  hipError_t e = hipGetLastError();
+  hipError_t e = launch_visc_flux(db, hb, nb, ph, precision, jb, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_visc_div, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, ph, jb);
+  if (precision == PREC_FP32_FLUX)
+    hipLaunchKernelGGL(k_visc_div<float>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, ph, jb);
+  else
+    hipLaunchKernelGGL(k_visc_div<double>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, ph, jb);
   return hipGetLastError();
 }
 
