@@ -131,7 +131,7 @@ def run_sweep(reqs, budget=200 * GB, collect=False):
 LANES = ((0, 64), (65, 128), (129, 1 << 30))  # size classes by grid edge N
 
 
-def serve(reqs, budget=200 * GB, chunk=4, capacity=1024, collect=True, lanes=LANES, max_live=None):
+def serve(reqs, budget=200 * GB, chunk=4, capacity=1024, collect=True, lanes=LANES, max_live=None, log=None):
     """Continuous batching over a Poisson request stream (arrival times in s).
 
     Solves are grouped into size-class lanes, each a device batch with its own
@@ -142,7 +142,8 @@ def serve(reqs, budget=200 * GB, chunk=4, capacity=1024, collect=True, lanes=LAN
     lane holding the largest solve replays `chunk` steps and a lane whose
     largest solve has n cells replays about chunk * n_max / n steps.  Finished
     solves are collected and removed after the chunk, freeing memory.
-    max_live caps the number of solves resident at once (None: memory only)."""
+    max_live caps the number of solves resident at once (None: memory only);
+    log, if given, is called with each finished request."""
     for r in reqs:
         r.payload = prepare(r)
     pending = sorted(reqs, key=lambda r: r.arrival)
@@ -210,6 +211,8 @@ def serve(reqs, budget=200 * GB, chunk=4, capacity=1024, collect=True, lanes=LAN
                     batches[li].remove(i)
                     used[0] -= C.batch_bytes_needed(r.payload[0])
                     done.append(r)
+                    if log:
+                        log(r)
                 host["collect"] += time.perf_counter() - te
         el = time.perf_counter() - t0
     finally:
@@ -222,7 +225,7 @@ def serve(reqs, budget=200 * GB, chunk=4, capacity=1024, collect=True, lanes=LAN
             "host_s": {k: round(v, 4) for k, v in host.items()}}
 
 
-def sequential_fp64(reqs, precision=C.CBNS_FP64):
+def sequential_fp64(reqs, precision=C.CBNS_FP64, log=None):
     """The same requests solved one at a time as single solves (plain FP64
     unless told otherwise), first come first served; latencies from the
     measured service times.  Each request's result is kept in r.alone."""
@@ -238,6 +241,8 @@ def sequential_fp64(reqs, precision=C.CBNS_FP64):
             s.run(r.steps)
             r.alone = s.get_state()
         service.append(time.perf_counter() - t0)
+        if log:
+            log(r)
     clock, lat = 0.0, []
     for r, sv in sorted(zip(reqs, service), key=lambda x: x[0].arrival):
         start = max(clock, r.arrival)
@@ -283,10 +288,14 @@ def main(argv=None):
         mean_work = np.mean([r.work for r in make_requests(n, a.seed, grids, weights, (a.steps_min, a.steps_max))])
         rate = a.load * cap / mean_work
     reqs = make_requests(n, a.seed, grids, weights, (a.steps_min, a.steps_max), rate=rate)
-    res = {"requests": n, "rate_per_s": rate,
-           "served": serve(reqs, budget, a.chunk, max_live=a.slots or None)}
+    def progress(r):
+        print(f"request {r.rid} N={r.N} steps={r.steps} done", flush=True)
+
+    res = {"requests": n, "rate_per_s": rate, "slots": a.slots,
+           "served": serve(reqs, budget, a.chunk, max_live=a.slots or None, log=progress)}
+    print(json.dumps(res), flush=True)
     if not a.no_baseline:
-        res["sequential_fp64"] = sequential_fp64(reqs)
+        res["sequential_fp64"] = sequential_fp64(reqs, log=progress)
     print(json.dumps(res))
     return 0
 
