@@ -286,7 +286,7 @@ def main():
             "avg_launch_ms": round(avg_face_ms, 4),
             "algorithmic_flop_per_launch": FLOP_PER_FACE * faces_per_launch}
     line = {
-        "metric": "Mcell-steps/s TGV 256^3 FP64 (plain path)",
+        "metric": f"Mcell-steps/s TGV {N}^3 FP64 (plain path)",
         "value": round(value, 2),
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
