@@ -85,6 +85,7 @@ _SIGS = {
     "cbns_batch_run": (_I, [_VP, _I]),
     "cbns_batch_set_budgets": (_I, [_VP, _PI, _PI, _I]),
     "cbns_batch_sync": (_I, [_VP]),
+    "cbns_batch_set_workgroups": (_I, [_VP, _I]),
     "cbns_batch_status": (_I, [_VP, _I, _PI, _PI, _PD, _PI]),
     "cbns_batch_poll": (_I, [_VP, _PI, _I, _PI, _PI]),
     "cbns_batch_get_state": (_I, [_VP, _I, _PD]),
@@ -354,6 +355,10 @@ class Batch:
 
     def run(self, nsteps):
         check(lib().cbns_batch_run(self._h, int(nsteps)), "cbns_batch_run")
+
+    def set_workgroups(self, n):
+        """Persistent workgroups of the memory-bound stage kernels."""
+        check(lib().cbns_batch_set_workgroups(self._h, int(n)), "cbns_batch_set_workgroups")
 
     def sync(self):
         check(lib().cbns_batch_sync(self._h), "cbns_batch_sync")

@@ -46,10 +46,12 @@ hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable
                              int precision, hipStream_t s);
 hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, int precision,
                              hipStream_t s);
+// wgs: persistent workgroups of the memory-bound kernels (update, ghost, dt)
 hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int stage, int jb,
-                               int precision, hipStream_t s);
-hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, hipStream_t s);
-hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, hipStream_t s);
+                               int precision, int wgs, hipStream_t s);
+hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, int wgs,
+                              hipStream_t s);
+hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int wgs, hipStream_t s);
 hipError_t batch_launch_finalize(InstDesc* insts, int ninst, hipStream_t s);
 hipError_t batch_launch_set_budget(InstDesc* insts, const int2* pairs, int n, hipStream_t s);
 

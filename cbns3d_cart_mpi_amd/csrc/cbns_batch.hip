@@ -181,24 +181,25 @@ hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable
 }
 
 hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int stage, int jb,
-                               int precision, hipStream_t s) {
+                               int precision, int wgs, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
   if (precision == 0)
-    hipLaunchKernelGGL(kb_update<double>, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt, stage, jb);
+    hipLaunchKernelGGL(kb_update<double>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, stage, jb);
   else
-    hipLaunchKernelGGL(kb_update<float>, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt, stage, jb);
+    hipLaunchKernelGGL(kb_update<float>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, stage, jb);
   return hipGetLastError();
 }
 
-hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, hipStream_t s) {
+hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, int wgs,
+                              hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_ghost, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt, which);
+  hipLaunchKernelGGL(kb_ghost, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, which);
   return hipGetLastError();
 }
 
-hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, hipStream_t s) {
+hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int wgs, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_dt, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt);
+  hipLaunchKernelGGL(kb_dt, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt);
   return hipGetLastError();
 }
 

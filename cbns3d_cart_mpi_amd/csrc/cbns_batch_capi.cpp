@@ -59,6 +59,7 @@ struct cbns_batch {
   hipGraphExec_t exec = nullptr;
   bool dirty = true;
   int jb = 32;
+  int wg_mem = BATCH_WG;  // persistent workgroups of the memory-bound kernels
   int ninst_hw = 0;  // highest live id + 1
 };
 
@@ -185,7 +186,7 @@ static int rebuild(cbns_batch* b) {
   }
   hipStream_t s = b->stream;
   CK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipError_t e = batch_launch_dt(b->d_blocks, b->d_insts, b->tt[T_DT], s);
+  hipError_t e = batch_launch_dt(b->d_blocks, b->d_insts, b->tt[T_DT], b->wg_mem, s);
   if (e == hipSuccess) e = batch_launch_finalize(b->d_insts, hw, s);
   for (int stage = 1; stage <= 3 && e == hipSuccess; stage++) {
     const int in = stage == 1 ? CBNS_Q : CBNS_QP;
@@ -196,9 +197,11 @@ static int rebuild(cbns_batch* b) {
       e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE64 + d], d, in, b->jb, 0, s);
       if (e == hipSuccess) e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE32 + d], d, in, b->jb, 1, s);
     }
-    if (e == hipSuccess) e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD64], stage, b->jb, 0, s);
-    if (e == hipSuccess) e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD32], stage, b->jb, 1, s);
-    if (e == hipSuccess) e = batch_launch_ghost(b->d_blocks, b->d_insts, b->tt[T_GHOST], out, s);
+    if (e == hipSuccess)
+      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD64], stage, b->jb, 0, b->wg_mem, s);
+    if (e == hipSuccess)
+      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD32], stage, b->jb, 1, b->wg_mem, s);
+    if (e == hipSuccess) e = batch_launch_ghost(b->d_blocks, b->d_insts, b->tt[T_GHOST], out, b->wg_mem, s);
   }
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(s, &g);
@@ -453,6 +456,14 @@ int cbns_batch_set_budgets(cbns_batch* b, const int* ids, const int* steps, int 
   if (st) return st;
   CK(hipMemcpy(b->d_pairs, pr.data(), sizeof(int2) * n, hipMemcpyHostToDevice));
   CK(batch_launch_set_budget(b->d_insts, b->d_pairs, n, b->stream));
+  return CBNS_OK;
+}
+
+int cbns_batch_set_workgroups(cbns_batch* b, int wgs) {
+  if (!b || wgs < 1 || wgs > 65535) return CBNS_ERR_ARG;
+  CK(hipStreamSynchronize(b->stream));
+  b->wg_mem = wgs;
+  b->dirty = true;  // the step graph holds the grid size
   return CBNS_OK;
 }
 

@@ -98,13 +98,16 @@ def percentile(x, q):
     return float(np.percentile(np.asarray(x), q)) if len(x) else math.nan
 
 
-def run_sweep(reqs, budget=200 * GB, collect=False):
+def run_sweep(reqs, budget=200 * GB, collect=False, workgroups=None):
     """All requests admitted at time 0 into one batch and advanced together.
-    With collect, each request's final state is kept in r.result."""
+    With collect, each request's final state is kept in r.result; workgroups
+    sets the persistent grid of the memory-bound kernels."""
     for r in reqs:
         r.payload = prepare(r)
     t_setup = time.perf_counter()
     with C.Batch(budget, capacity=max(len(reqs), 1)) as b:
+        if workgroups:
+            b.set_workgroups(workgroups)
         ids = {}
         for r in reqs:
             p, Q = r.payload

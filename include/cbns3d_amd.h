@@ -208,6 +208,10 @@ int cbns_batch_run(cbns_batch* b, int nsteps);
  * start unbounded.  Lets a scheduler give small solves many cheap steps and
  * large ones few within one run. */
 int cbns_batch_set_budgets(cbns_batch* b, const int* ids, const int* steps, int n);
+/* Persistent workgroups per launch of the memory-bound stage kernels (stage
+ * tail, ghost gather, dt; default 1024).  A tuning knob only; results are
+ * identical for every value.  Takes effect at the next run. */
+int cbns_batch_set_workgroups(cbns_batch* b, int workgroups);
 int cbns_batch_sync(cbns_batch* b);
 int cbns_batch_status(cbns_batch* b, int id, int* active, int* steps_done, double* t_cur, int* nonfinite);
 /* Ids of live solves that are done (took their steps, reached t_end or failed);
