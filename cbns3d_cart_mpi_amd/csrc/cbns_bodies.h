@@ -353,6 +353,127 @@ __device__ __forceinline__ void update_body(const BlockDesc& B, int t, double dt
            gmem(B.prim), B.sg, c, dt, stage, ph);
 }
 
+// ---- stage tail over x-pairs of cells ---------------------------------------
+// The same arithmetic as update_body for the two cells (i, j, k), (i+1, j, k)
+// with i even, so that every field of the pair is one 2-wide load or store
+// (for FP32-flux, the float face and viscous arrays move 8 bytes per lane
+// instead of 4).  Requires an even n0: every field stride and row offset is
+// then even, so the pair loads are naturally aligned.
+template <typename T>
+using vec2 = T __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ __forceinline__ vec2<T> ld2(gptr<const T> p, long long o) {
+  return *(gptr<const vec2<T>>)(p + o);
+}
+
+template <typename FT>
+__device__ __forceinline__ void update_pair_body(const BlockDesc& B, int t, double dt, int stage, const Phys& ph,
+                                                 int jb) {
+  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  int ih, j, k;
+  band_decode(t, n0 / 2, n1, n2, jb, ih, j, k);
+  const int i = 2 * ih;
+  // viscous divergence (visc_div) of both cells
+  const int v0 = n0 + 2 * NGV, v1 = n1 + 2 * NGV;
+  const int cv = (i + NGV) + v0 * ((j + NGV) + v1 * (k + NGV));
+  gptr<const FT> __restrict__ V = gmem<const FT>((const FT*)B.visc);
+  const long long sv = B.sv;
+  const double c0 = 1.0 / 12.0, c1 = -2.0 / 3.0;
+  const int Ef[4] = {0, 3, 4, 6}, Ff[4] = {3, 1, 5, 7}, Gf[4] = {4, 5, 2, 8};
+  double dfv[2][4];
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    gptr<const FT> X = V + Ef[e] * sv;
+    const vec2<FT> xa = ld2(X, cv - 2), xb = ld2(X, cv), xc = ld2(X, cv + 2);  // cv-2 .. cv+3
+    double dx0 = c0 * xa.x + c1 * xa.y - c1 * xb.y - c0 * xc.x;
+    double dx1 = c0 * xa.y + c1 * xb.x - c1 * xc.x - c0 * xc.y;
+    gptr<const FT> Y = V + Ff[e] * sv;
+    const vec2<FT> ym2 = ld2(Y, cv - 2 * v0), ym1 = ld2(Y, cv - v0), yp1 = ld2(Y, cv + v0), yp2 = ld2(Y, cv + 2 * v0);
+    const double dy0 = c0 * ym2.x + c1 * ym1.x - c1 * yp1.x - c0 * yp2.x;
+    const double dy1 = c0 * ym2.y + c1 * ym1.y - c1 * yp1.y - c0 * yp2.y;
+    gptr<const FT> Z = V + Gf[e] * sv;
+    const int sz = v0 * v1;
+    const vec2<FT> zm2 = ld2(Z, cv - 2 * sz), zm1 = ld2(Z, cv - sz), zp1 = ld2(Z, cv + sz), zp2 = ld2(Z, cv + 2 * sz);
+    const double dz0 = c0 * zm2.x + c1 * zm1.x - c1 * zp1.x - c0 * zp2.x;
+    const double dz1 = c0 * zm2.y + c1 * zm1.y - c1 * zp1.y - c0 * zp2.y;
+    double d = divc(dx0, ph.dx, ph.rdx, ph.fdx);
+    d += divc(dy0, ph.dy, ph.rdy, ph.fdy);
+    d += divc(dz0, ph.dz, ph.rdz, ph.fdz);
+    dfv[0][e] = d;
+    d = divc(dx1, ph.dx, ph.rdx, ph.fdx);
+    d += divc(dy1, ph.dy, ph.rdy, ph.fdy);
+    d += divc(dz1, ph.dz, ph.rdz, ph.fdz);
+    dfv[1][e] = d;
+  }
+  // inviscid flux differences, RK3 and primitives
+  const int iE = i + (n0 + 1) * (j + n1 * k);
+  const int iF = i + n0 * (j + (n1 + 1) * k);
+  const int iG = i + n0 * (j + n1 * k);
+  const long long sg = B.sg;
+  const int c = gidx(B, i, j, k);
+  gptr<const double> Qn = gmem<const double>(B.Q[0]);
+  gptr<const double> Qi = gmem<const double>(B.Q[1]);
+  gptr<double> Qo = gmem(stage == 3 ? B.Q[0] : B.Q[1]);
+  double qn[2][NEQ];
+#pragma unroll
+  for (int e = 0; e < NEQ; e++) {
+    gptr<const FT> Ep = gmem<const FT>((const FT*)B.F[0] + e * B.sf[0]);
+    gptr<const FT> Fp = gmem<const FT>((const FT*)B.F[1] + e * B.sf[1]);
+    gptr<const FT> Gp = gmem<const FT>((const FT*)B.F[2] + e * B.sf[2]);
+    const double e0 = (double)Ep[iE], e1 = (double)Ep[iE + 1], e2 = (double)Ep[iE + 2];
+    const vec2<FT> fa = ld2(Fp, iF), fb = ld2(Fp, iF + n0);
+    const vec2<FT> ga = ld2(Gp, iG), gb = ld2(Gp, iG + (long long)n0 * n1);
+    double df0 = divc(e1 - e0, ph.dx, ph.rdx, ph.fdx);
+    df0 += divc((double)fb.x - (double)fa.x, ph.dy, ph.rdy, ph.fdy);
+    df0 += divc((double)gb.x - (double)ga.x, ph.dz, ph.rdz, ph.fdz);
+    double df1 = divc(e2 - e1, ph.dx, ph.rdx, ph.fdx);
+    df1 += divc((double)fb.y - (double)fa.y, ph.dy, ph.rdy, ph.fdy);
+    df1 += divc((double)gb.y - (double)ga.y, ph.dz, ph.rdz, ph.fdz);
+    if (e > 0) {
+      df0 -= dfv[0][e - 1];
+      df1 -= dfv[1][e - 1];
+    }
+    const long long q = e * sg + c;
+    const vec2<double> a = ld2(Qn, q);
+    vec2<double> r;
+    if (1 == stage) {
+      r.x = a.x - dt * df0;
+      r.y = a.y - dt * df1;
+    } else {
+      const vec2<double> b = ld2(Qi, q);
+      if (2 == stage) {
+        r.x = 0.75 * a.x + 0.25 * (b.x - dt * df0);
+        r.y = 0.75 * a.y + 0.25 * (b.y - dt * df1);
+      } else {
+        r.x = (a.x + 2.0 * (b.x - dt * df0)) / 3.0;
+        r.y = (a.y + 2.0 * (b.y - dt * df1)) / 3.0;
+      }
+    }
+    *(gptr<vec2<double>>)(Qo + q) = r;
+    qn[0][e] = r.x;
+    qn[1][e] = r.y;
+  }
+  gptr<double> P = gmem(B.prim);
+  vec2<double> pr[6];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const double rr = qn[h][0];
+    const double uu = qn[h][1] / rr;
+    const double vv = qn[h][2] / rr;
+    const double ww = qn[h][3] / rr;
+    const double pp = ph.gam1 * (qn[h][4] - 0.5 * rr * (uu * uu + vv * vv + ww * ww));
+    pr[0][h] = rr;
+    pr[1][h] = uu;
+    pr[2][h] = vv;
+    pr[3][h] = ww;
+    pr[4][h] = pp;
+    pr[5][h] = ph.gM2 * pp / rr;
+  }
+#pragma unroll
+  for (int f = 0; f < 6; f++) *(gptr<vec2<double>>)(P + f * sg + c) = pr[f];
+}
+
 // ---- ghost gather of the state and its primitives ------------------------
 // The g-th ghost cell (0 <= g < sg - sn) of a block as a ghosted linear index:
 // first the NG z-planes below and above the interior (whole planes), then

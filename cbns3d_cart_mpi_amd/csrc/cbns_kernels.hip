@@ -252,11 +252,32 @@ __global__ __launch_bounds__(TPB) void k_update_fused(const BlockDesc* __restric
   update_body<FT>(B, t, dt, stage, ph, jb);
 }
 
+// The same over x-pairs of cells (update_pair_body), for blocks with even n0.
+template <typename FT>
+__global__ __launch_bounds__(TPB) void k_update_pair(const BlockDesc* __restrict__ blocks, double dt, int stage,
+                                                     Phys ph, int jb) {
+  const Tile tl = xcd_tile();
+  const BlockDesc& B = blocks[tl.y];
+  const int t = tl.x * TPB + threadIdx.x;
+  if (t >= (B.n[0] / 2) * B.n[1] * B.n[2]) return;
+  update_pair_body<FT>(B, t, dt, stage, ph, jb);
+}
+
 hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
                                const Phys& ph, int precision, int jb, hipStream_t s) {
   long long mn = 0;
-  for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
+  bool even = true;
+  for (int b = 0; b < nb; b++) {
+    mn = hb[b].sn > mn ? hb[b].sn : mn;
+    even = even && hb[b].n[0] % 2 == 0;
+  }
   if (stage < 1 || stage > 3) return hipSuccess;
+  if (even) {
+    const dim3 grid(nblocks_for(mn / 2), nb);
+    if (precision != PREC_FP32_FLUX) hipLaunchKernelGGL(k_update_pair<double>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb);
+    else hipLaunchKernelGGL(k_update_pair<float>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb);
+    return hipGetLastError();
+  }
   const dim3 grid(nblocks_for(mn), nb);
   if (precision != PREC_FP32_FLUX) hipLaunchKernelGGL(k_update_fused<double>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb);
   else hipLaunchKernelGGL(k_update_fused<float>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb);
