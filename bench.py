@@ -119,9 +119,11 @@ def form_leg(C, N, dims, steps_total, steps_timed, Q64, precision):
         pre -= 3 * len(tms)
     s.run(pre)
     s.sync()
+    s.set_profiling(True)
     t0 = time.perf_counter()
     s.run(steps_timed)
     el = time.perf_counter() - t0
+    kt = s.kernel_times()
     Q = s.get_state()
     s.close()
     err = {}
@@ -133,6 +135,7 @@ def form_leg(C, N, dims, steps_total, steps_timed, Q64, precision):
     v = N ** 3 * steps_timed / el / 1e6
     return {"value": round(v, 2), "unit": "Mcell-steps/s", "ms_per_step": round(el / steps_timed * 1e3, 3),
             "error_vs_fp64_after_steps": steps_total, "error": err,
+            "kernels_avg_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items() if v[1]},
             "autotune": {"band": best[0], "schedule": best[1], "face_waves_per_simd": best[2]} if best else None}
 
 

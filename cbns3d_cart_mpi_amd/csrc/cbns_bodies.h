@@ -353,6 +353,24 @@ __device__ __forceinline__ void update_body(const BlockDesc& B, int t, double dt
            gmem(B.prim), B.sg, c, dt, stage, ph);
 }
 
+// ---- calc_dt spectral radius of one cell from its primitives and mu
+// (src/Block3d.cpp:527-546).  Returns t1 + t2 + t3; *bad is set for a NaN
+// or infinite (>= 1e308) result.
+__device__ __forceinline__ double dt_point(double rr, double uu, double vv, double ww, double pp, double mu,
+                                           const Phys& ph, int* bad) {
+  const double cs = sqrt(ph.gamma * pp / rr);
+  const double factor_v = 2.0 * mu * ph.C_dt_v * sqrt(uu * uu + vv * vv + ww * ww) / cs / rr;
+  double t1 = fabs(uu) + divc(cs, ph.dx, ph.rdx, ph.fdx);
+  t1 += divc(divc(factor_v, ph.dx, ph.rdx, ph.fdx), ph.dx, ph.rdx, ph.fdx);
+  double t2 = fabs(vv) + divc(cs, ph.dy, ph.rdy, ph.fdy);
+  t2 += divc(divc(factor_v, ph.dy, ph.rdy, ph.fdy), ph.dy, ph.rdy, ph.fdy);
+  double t3 = fabs(ww) + divc(cs, ph.dz, ph.rdz, ph.fdz);
+  t3 += divc(divc(factor_v, ph.dz, ph.rdz, ph.fdz), ph.dz, ph.rdz, ph.fdz);
+  const double s = t1 + t2 + t3;
+  if (!(s < 1.0e308)) *bad = 1;
+  return s;
+}
+
 // ---- stage tail over x-pairs of cells ---------------------------------------
 // The same arithmetic as update_body for the two cells (i, j, k), (i+1, j, k)
 // with i even, so that every field of the pair is one 2-wide load or store
@@ -367,9 +385,13 @@ __device__ __forceinline__ vec2<T> ld2(gptr<const T> p, long long o) {
   return *(gptr<const vec2<T>>)(p + o);
 }
 
+// With dts != nullptr (stage 3) it also returns in *s_out the larger calc_dt
+// spectral radius of the two updated cells (a NaN never wins), with mu from
+// this stage's viscous pass, which is what the next step's calc_dt reads.
 template <typename FT>
 __device__ __forceinline__ void update_pair_body(const BlockDesc& B, int t, double dt, int stage, const Phys& ph,
-                                                 int jb) {
+                                                 int jb, bool want_s = false, double* s_out = nullptr,
+                                                 int* bad = nullptr) {
   const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
   int ih, j, k;
   band_decode(t, n0 / 2, n1, n2, jb, ih, j, k);
@@ -469,6 +491,10 @@ __device__ __forceinline__ void update_pair_body(const BlockDesc& B, int t, doub
     pr[3][h] = ww;
     pr[4][h] = pp;
     pr[5][h] = ph.gM2 * pp / rr;
+    if (want_s) {
+      const double sc = dt_point(rr, uu, vv, ww, pp, gmem<const double>(B.mu)[iG + h], ph, bad);
+      *s_out = (*s_out < sc) ? sc : *s_out;  // src/Block3d.cpp:546: a NaN never wins
+    }
   }
 #pragma unroll
   for (int f = 0; f < 6; f++) *(gptr<vec2<double>>)(P + f * sg + c) = pr[f];
@@ -547,17 +573,7 @@ __device__ __forceinline__ double dt_cell(const BlockDesc& B, int t, const Phys&
   gptr<const double> __restrict__ P = gmem<const double>(B.prim);
   const double rr = P[c], uu = P[sg + c], vv = P[2 * sg + c], ww = P[3 * sg + c];
   const double pp = P[4 * sg + c];
-  const double cs = sqrt(ph.gamma * pp / rr);
-  const double factor_v = 2.0 * gmem<const double>(B.mu)[t] * ph.C_dt_v * sqrt(uu * uu + vv * vv + ww * ww) / cs / rr;
-  double t1 = fabs(uu) + divc(cs, ph.dx, ph.rdx, ph.fdx);
-  t1 += divc(divc(factor_v, ph.dx, ph.rdx, ph.fdx), ph.dx, ph.rdx, ph.fdx);
-  double t2 = fabs(vv) + divc(cs, ph.dy, ph.rdy, ph.fdy);
-  t2 += divc(divc(factor_v, ph.dy, ph.rdy, ph.fdy), ph.dy, ph.rdy, ph.fdy);
-  double t3 = fabs(ww) + divc(cs, ph.dz, ph.rdz, ph.fdz);
-  t3 += divc(divc(factor_v, ph.dz, ph.rdz, ph.fdz), ph.dz, ph.rdz, ph.fdz);
-  const double s = t1 + t2 + t3;
-  if (!(s < 1.0e308)) *bad = 1;
-  return s;
+  return dt_point(rr, uu, vv, ww, pp, gmem<const double>(B.mu)[t], ph, bad);
 }
 
 __device__ __forceinline__ double wave_max(double v) {

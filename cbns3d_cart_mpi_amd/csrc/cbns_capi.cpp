@@ -39,6 +39,9 @@ struct cbns_ctx {
   uint64_t bytes = 0;
   bool have_state = false;
   int nonfinite = 0;
+  // d_smax/d_flag hold the calc_dt reduction of the current state, made by the
+  // last stage's update; cleared by every call that changes the state or mu
+  bool s_ready = false;
   RefState ref{1.0, 0.0, 0.0};  // FP32-flux reference state
   // live kernel timing
   bool prof = false;
@@ -317,12 +320,14 @@ int cbns_sync(cbns_ctx* c) {
 
 int cbns_boundary_condition(cbns_ctx* c, int which) {
   if (!c || (which != CBNS_Q && which != CBNS_QP)) return CBNS_ERR_ARG;
+  c->s_ready = false;
   CK(launch_ghost_fill(c->db, c->hb.data(), c->dom, which, c->stream));
   return CBNS_OK;
 }
 
 int cbns_calc_primitive(cbns_ctx* c, int which) {
   if (!c || (which != CBNS_Q && which != CBNS_QP)) return CBNS_ERR_ARG;
+  c->s_ready = false;
   CK(launch_primitive(c->db, c->hb.data(), c->dom.nb, which, c->ph, c->stream));
   return CBNS_OK;
 }
@@ -333,12 +338,8 @@ int cbns_init_Q_p(cbns_ctx* c) {
   return CBNS_OK;
 }
 
-int cbns_calc_dt(cbns_ctx* c, double* dt) {
-  if (!c || !dt) return CBNS_ERR_ARG;
-  if (!c->have_state) return CBNS_ERR_STATE;
-  CK(hipMemsetAsync(c->d_smax, 0, sizeof(unsigned long long), c->stream));
-  CK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
-  CK(timed(c, 7, [&] { return launch_dt_reduce(c->db, c->hb.data(), c->dom.nb, c->ph, c->d_smax, c->d_flag, c->stream); }));
+// dt = CFL / s_max from the reduction in d_smax / d_flag (src/Block3d.cpp:551).
+static int read_dt(cbns_ctx* c, double* dt) {
   unsigned long long bits = 0;
   int flag = 0;
   CK(hipMemcpyAsync(&bits, c->d_smax, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
@@ -349,7 +350,17 @@ int cbns_calc_dt(cbns_ctx* c, double* dt) {
   std::memcpy(&s_max, &bits, sizeof(s_max));
   c->nonfinite = flag;
   *dt = c->par.CFL / s_max;  // src/Block3d.cpp:551
+  c->s_ready = false;
   return CBNS_OK;
+}
+
+int cbns_calc_dt(cbns_ctx* c, double* dt) {
+  if (!c || !dt) return CBNS_ERR_ARG;
+  if (!c->have_state) return CBNS_ERR_STATE;
+  CK(hipMemsetAsync(c->d_smax, 0, sizeof(unsigned long long), c->stream));
+  CK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+  CK(timed(c, 7, [&] { return launch_dt_reduce(c->db, c->hb.data(), c->dom.nb, c->ph, c->d_smax, c->d_flag, c->stream); }));
+  return read_dt(c, dt);
 }
 
 int cbns_rec_riemann(cbns_ctx* c, int which, int dir) {
@@ -360,12 +371,14 @@ int cbns_rec_riemann(cbns_ctx* c, int which, int dir) {
 
 int cbns_calc_viscous_flux_contribution(cbns_ctx* c) {
   if (!c) return CBNS_ERR_ARG;
+  c->s_ready = false;
   CK(launch_viscous(c->db, c->hb.data(), c->dom.nb, c->ph, c->precision, c->jband, c->stream));
   return CBNS_OK;
 }
 
 int cbns_update_rk3(cbns_ctx* c, double dt, int stage) {
   if (!c) return CBNS_ERR_ARG;
+  c->s_ready = false;
   if (stage < 1 || stage > 3) return CBNS_OK;  // the reference silently does nothing
   CK(launch_update(c->db, c->hb.data(), c->dom.nb, dt, stage, c->ph, c->precision, c->jband, c->stream));
   return CBNS_OK;
@@ -376,6 +389,7 @@ int cbns_update_rk3(cbns_ctx* c, double dt, int stage) {
 static int finish_setup(cbns_ctx* c) {
   // src/cbns3d.cpp:149-153
   int st;
+  c->s_ready = false;
   if ((st = cbns_boundary_condition(c, CBNS_Q))) return st;
   if ((st = cbns_calc_primitive(c, CBNS_Q))) return st;
   if ((st = cbns_init_Q_p(c))) return st;
@@ -475,6 +489,7 @@ int cbns_set_block_state(cbns_ctx* c, int b, int which, const double* in) {
   CK(hipStreamSynchronize(c->stream));
   CK(hipMemcpy(B.Q[which], soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
   c->have_state = true;
+  c->s_ready = false;
   return CBNS_OK;
 }
 
@@ -535,9 +550,16 @@ static int run_stages(cbns_ctx* c, double dt) {
       }));
     if (c->overlap) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     else CK(timed(c, 3, [&] { return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->stream); }));
+    bool s_done = false;
+    if (stage == 3) {  // the next step's calc_dt reduction rides on the last update
+      CK(hipMemsetAsync(c->d_smax, 0, sizeof(unsigned long long), c->stream));
+      CK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+    }
     CK(timed(c, 4, [&] {
-      return launch_update_fused(c->db, hb, c->dom.nb, dt, stage, c->ph, c->precision, c->jband, c->stream);
+      return launch_update_fused(c->db, hb, c->dom.nb, dt, stage, c->ph, c->precision, c->jband, c->stream,
+                                 stage == 3 ? c->d_smax : nullptr, stage == 3 ? c->d_flag : nullptr, &s_done);
     }));
+    if (stage == 3) c->s_ready = s_done;
     CK(timed(c, 5, [&] { return launch_ghost_fill_prim(c->db, hb, c->dom, out, c->stream); }));
   }
   return CBNS_OK;
@@ -553,7 +575,7 @@ int cbns_step(cbns_ctx* c, double* dt_out, int* finished) {
     return CBNS_OK;
   }
   double dt;
-  int st = cbns_calc_dt(c, &dt);
+  int st = c->s_ready ? read_dt(c, &dt) : cbns_calc_dt(c, &dt);
   if (st) return st;
   if (c->nonfinite) return CBNS_ERR_NONFINITE;
   // src/cbns3d.cpp:191-194

@@ -87,8 +87,12 @@ hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, cons
                           hipStream_t s);
 hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision,
                             int jb, hipStream_t s);
+// smax/flag (optional): also reduce the next step's calc_dt spectral radius of
+// the updated cells (only over x-pairs, for blocks of even n0; *s_done says
+// whether it was done).
 hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
-                               const Phys& ph, int precision, int jb, hipStream_t s);
+                               const Phys& ph, int precision, int jb, hipStream_t s,
+                               unsigned long long* smax = nullptr, int* flag = nullptr, bool* s_done = nullptr);
 hipError_t launch_ghost_fill_prim(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
                                   hipStream_t s);
 hipError_t launch_update(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,

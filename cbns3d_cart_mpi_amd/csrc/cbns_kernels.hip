@@ -253,18 +253,37 @@ __global__ __launch_bounds__(TPB) void k_update_fused(const BlockDesc* __restric
 }
 
 // The same over x-pairs of cells (update_pair_body), for blocks with even n0.
+// With smax != nullptr it also reduces the next step's calc_dt spectral
+// radius of the updated cells into *smax (as k_dt does: integer atomic max on
+// the bits, one per workgroup) and raises *flag for a non-finite one.
 template <typename FT>
 __global__ __launch_bounds__(TPB) void k_update_pair(const BlockDesc* __restrict__ blocks, double dt, int stage,
-                                                     Phys ph, int jb) {
+                                                     Phys ph, int jb, unsigned long long* smax, int* flag) {
   const Tile tl = xcd_tile();
   const BlockDesc& B = blocks[tl.y];
   const int t = tl.x * TPB + threadIdx.x;
-  if (t >= (B.n[0] / 2) * B.n[1] * B.n[2]) return;
-  update_pair_body<FT>(B, t, dt, stage, ph, jb);
+  const bool want = smax != nullptr;  // uniform
+  double s = 0.0;
+  int bad = 0;
+  if (t < (B.n[0] / 2) * B.n[1] * B.n[2]) update_pair_body<FT>(B, t, dt, stage, ph, jb, want, &s, &bad);
+  if (!want) return;
+  s = wave_max(s);
+  __shared__ double red[TPB / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int a = 1; a < TPB / 64; a++) m = red[a] > m ? red[a] : m;
+    atomicMax(smax, (unsigned long long)__double_as_longlong(m));
+  }
+  if (bad) atomicOr(flag, 1);
 }
 
 hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
-                               const Phys& ph, int precision, int jb, hipStream_t s) {
+                               const Phys& ph, int precision, int jb, hipStream_t s, unsigned long long* smax,
+                               int* flag, bool* s_done) {
+  if (s_done) *s_done = false;
   long long mn = 0;
   bool even = true;
   for (int b = 0; b < nb; b++) {
@@ -274,8 +293,11 @@ hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb,
   if (stage < 1 || stage > 3) return hipSuccess;
   if (even) {
     const dim3 grid(nblocks_for(mn / 2), nb);
-    if (precision != PREC_FP32_FLUX) hipLaunchKernelGGL(k_update_pair<double>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb);
-    else hipLaunchKernelGGL(k_update_pair<float>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb);
+    if (precision != PREC_FP32_FLUX)
+      hipLaunchKernelGGL(k_update_pair<double>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb, smax, flag);
+    else
+      hipLaunchKernelGGL(k_update_pair<float>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb, smax, flag);
+    if (s_done) *s_done = smax != nullptr;
     return hipGetLastError();
   }
   const dim3 grid(nblocks_for(mn), nb);
