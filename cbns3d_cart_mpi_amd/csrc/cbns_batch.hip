@@ -63,8 +63,13 @@ __global__ __launch_bounds__(TPB) void kb_update(const BlockDesc* __restrict__ b
     const InstDesc& I = insts[B.inst];
     if (!I.stepping) continue;
     const int t = tr.tile * TPB + threadIdx.x;
-    if (t >= B.n[0] * B.n[1] * B.n[2]) continue;
-    update_body<FT>(B, t, I.dt, stage, I.ph, jb);
+    if (B.n[0] % 2 == 0) {  // x-pairs of cells (the table holds pair tiles for such blocks)
+      if (t >= (B.n[0] / 2) * B.n[1] * B.n[2]) continue;
+      update_pair_body<FT>(B, t, I.dt, stage, I.ph, jb);
+    } else {
+      if (t >= B.n[0] * B.n[1] * B.n[2]) continue;
+      update_body<FT>(B, t, I.dt, stage, I.ph, jb);
+    }
   }
 }
 

@@ -156,7 +156,7 @@ static int rebuild(cbns_batch* b) {
                          ((B.n[2] + 2 * NGV + VZ - 1) / VZ);
       for (int t = 0; t < vunits; t++)  // whole-workgroup units
         tab[I.precision ? T_VISC32 : T_VISC64].push_back({bi, t});
-      add(I.precision ? T_UPD32 : T_UPD64, B.sn);
+      add(I.precision ? T_UPD32 : T_UPD64, B.n[0] % 2 == 0 ? B.sn / 2 : B.sn);  // x-pairs for even n0
       add(T_GHOST, B.sg - B.sn);  // ghost cells only
       add(T_DT, B.sn);
     }
