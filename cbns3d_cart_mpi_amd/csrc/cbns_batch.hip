@@ -84,7 +84,8 @@ __global__ __launch_bounds__(TPB) void kb_ghost(const BlockDesc* __restrict__ bl
     if (!I.stepping) continue;
     const int g = tr.tile * TPB + threadIdx.x;
     if (g >= B.sg - B.sn) continue;
-    ghost_body(B, blocks, I.dom, I.block0, ghost_cell(B, g), which, true);
+    if (I.precision == PREC_FP32_FLUX) ghost_body<float>(B, blocks, I.dom, I.block0, ghost_cell(B, g), which, true);
+    else ghost_body<double>(B, blocks, I.dom, I.block0, ghost_cell(B, g), which, true);
   }
 }
 
@@ -115,7 +116,8 @@ __global__ __launch_bounds__(TPB) void kb_dt(const BlockDesc* __restrict__ block
     }
     const int t = tr.tile * TPB + threadIdx.x;
     if (t < B.n[0] * B.n[1] * B.n[2]) {
-      const double s = dt_cell(B, t, insts[inst].ph, &bad);
+      const double s = insts[inst].precision == PREC_FP32_FLUX ? dt_cell<float>(B, t, insts[inst].ph, &bad)
+                                                               : dt_cell<double>(B, t, insts[inst].ph, &bad);
       acc = (acc < s) ? s : acc;  // src/Block3d.cpp:546: a NaN never wins
     }
   }

@@ -330,7 +330,7 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
           B.sf[d] = (long long)(B.n[0] + (d == 0)) * (B.n[1] + (d == 1)) * (B.n[2] + (d == 2));
         ok = (B.Q[0] = alloc(8 * NEQ * B.sg)) && (B.Q[1] = alloc(8 * NEQ * B.sg)) &&
              (B.F[0] = alloc(8 * NEQ * B.sf[0])) && (B.F[1] = alloc(8 * NEQ * B.sf[1])) &&
-             (B.F[2] = alloc(8 * NEQ * B.sf[2])) && (B.prim = alloc(8 * 6 * B.sg)) && (B.mu = alloc(8 * B.sn)) &&
+             (B.F[2] = alloc(8 * NEQ * B.sf[2])) && (B.prim = alloc((precision == CBNS_FP32_FLUX ? 4 : 8) * 6 * B.sg)) && (B.mu = alloc(8 * B.sn)) &&
              (B.visc = alloc((precision == CBNS_FP32_FLUX ? 4 : 8) * 9 * B.sv));
         I.blocks.push_back(B);
       }
@@ -380,7 +380,7 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
   }
   // boundary_condition, calc_primitive, init_Q_p (src/cbns3d.cpp:149-153)
   if (!st && (launch_ghost_fill(I.dblocks, I.blocks.data(), I.dom, CBNS_Q, s) != hipSuccess ||
-              launch_primitive(I.dblocks, I.blocks.data(), nb, CBNS_Q, ph, s) != hipSuccess ||
+              launch_primitive(I.dblocks, I.blocks.data(), nb, CBNS_Q, ph, precision, s) != hipSuccess ||
               launch_copy_state(I.dblocks, I.blocks.data(), nb, CBNS_Q, CBNS_QP, s) != hipSuccess))
     st = CBNS_ERR_HIP;
   InstDesc D;
@@ -389,7 +389,7 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
   D.ref = RefState{1.0, 0.0, 0.0};
   if (!st && precision == CBNS_FP32_FLUX) {
     double sums[2];
-    if (launch_global_sums(I.dblocks, I.dom, ph, 0, b->d_part, b->d_part + 2 * GSUM_WG, s) != hipSuccess ||
+    if (launch_global_sums(I.dblocks, I.dom, ph, 0, b->d_part, b->d_part + 2 * GSUM_WG, precision, s) != hipSuccess ||
         hipMemcpyAsync(sums, b->d_part + 2 * GSUM_WG, sizeof(sums), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       st = CBNS_ERR_HIP;

@@ -77,7 +77,8 @@ __device__ __forceinline__ int face_count(const BlockDesc& B, int D) {
 
 // ---- inviscid face flux: rec_riemann_x/y/z (src/Block3d.cpp:555-748) ----
 // Face (i,j,k) along D sits between cells i-1 and i; t < face_count(B, D).
-template <int D>
+// PT: storage type of the primitives (double, or float in the FP32-flux form).
+template <int D, typename PT = double>
 __device__ __forceinline__ void face_load(const BlockDesc& B, int t, int which, int jb, double (&q)[6][5],
                                           double (&pl)[5], double (&pr)[5], int& fi) {
   const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
@@ -92,11 +93,11 @@ __device__ __forceinline__ void face_load(const BlockDesc& B, int t, int which, 
   for (int cc = 0; cc < 6; cc++)
 #pragma unroll
     for (int e = 0; e < 5; e++) q[cc][e] = Q[e * sg + c0 + (cc - 3) * sD];
-  gptr<const double> __restrict__ P = gmem<const double>(B.prim);
+  gptr<const PT> __restrict__ P = gmem<const PT>((const PT*)B.prim);
 #pragma unroll
   for (int f = 0; f < 5; f++) {
-    pl[f] = P[f * sg + c0 - sD];
-    pr[f] = P[f * sg + c0];
+    pl[f] = (double)P[f * sg + c0 - sD];
+    pr[f] = (double)P[f * sg + c0];
   }
 }
 
@@ -117,7 +118,7 @@ __device__ __forceinline__ void face_body_f32(const BlockDesc& B, int t, int whi
   double q[6][5], pl[5], pr[5];
   float fl[5];
   int fi;
-  face_load<D>(B, t, which, jb, q, pl, pr, fi);
+  face_load<D, float>(B, t, which, jb, q, pl, pr, fi);
   face_flux_f32<D>(q, pl, pr, ph.gamma, ph.gam1, g, fl);
   gptr<float> __restrict__ F = gmem((float*)B.F[D]);
 #pragma unroll
@@ -183,12 +184,20 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
   const int bx = X0 + tx, by = Y0 + ty;  // box point; its cell is (bx - 2, by - 2, bz - 2)
   const bool act = bx < v0 && by < v1;
   constexpr int LX = VX + 4, LY = VY + 4, NT = VX * VY, NP = (LX * LY + NT - 1) / NT;
-  __shared__ double L[2][4][LY][LX];
+  __shared__ VT L[2][4][LY][LX];  // planes in the storage type, widened at use
   const long long sg = B.sg;
   const int sz = B.g[0] * B.g[1];
-  gptr<const double> __restrict__ P = gmem<const double>(B.prim);
+  gptr<const VT> __restrict__ P = gmem<const VT>((const VT*)B.prim);  // primitives share VT
   const long long foff[4] = {sg, 2 * sg, 3 * sg, 5 * sg};  // u, v, w, T
-  // LDS entries e = threadIdx.x + p NT this thread fills: cell (X0 + lx - 4, Y0 + ly - 4)
+  // LDS entries e = threadIdx.x + p NT this thread fills: cell (X0 + lx - 4,
+  // Y0 + ly - 4).  A position past the ghosted block (no active point's
+  // stencil reaches it) or past the tile, and an inactive point's column,
+  // is either skipped (predicated loads) or, with CLAMP, clamped to the
+  // block's last ghost cell so that every load is unconditional.  Both give
+  // the same values at every position that is used; each variant takes the
+  // form the compiler schedules better for it (with float storage the
+  // predicated form leaves waits on the prefetch inside the arithmetic).
+  constexpr bool CLAMP = sizeof(VT) == 4;
   int fidx[NP];
   bool fok[NP];
 #pragma unroll
@@ -196,18 +205,23 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
     const int e = threadIdx.x + p * NT;
     const int lx = e % LX, ly = e / LX;
     const int cx = X0 + lx - 4, cy = Y0 + ly - 4;
-    fok[p] = e < LX * LY && cx < B.n[0] + NG && cy < B.n[1] + NG;
-    fidx[p] = fok[p] ? gidx(B, cx, cy, Z0 - 2) : 0;
+    fok[p] = CLAMP || (e < LX * LY && cx < B.n[0] + NG && cy < B.n[1] + NG);
+    if (CLAMP)
+      fidx[p] = gidx(B, cx < B.n[0] + NG ? cx : B.n[0] + NG - 1, cy < B.n[1] + NG ? cy : B.n[1] + NG - 1, Z0 - 2);
+    else fidx[p] = fok[p] ? gidx(B, cx, cy, Z0 - 2) : 0;
   }
-  double pf[4][NP], rn[4], r[4][5];
-  int ci = act ? gidx(B, bx - 2, by - 2, Z0 - 2) : 0;
+  const bool ld = CLAMP || act;
+  VT pf[4][NP], rn[4];  // next-plane prefetch
+  double r[4][5];
+  int ci = CLAMP ? gidx(B, (bx < v0 ? bx : v0 - 1) - 2, (by < v1 ? by : v1 - 1) - 2, Z0 - 2)
+                 : (act ? gidx(B, bx - 2, by - 2, Z0 - 2) : 0);
 #pragma unroll
   for (int f = 0; f < 4; f++) {
 #pragma unroll
-    for (int d = 0; d < 4; d++) r[f][d] = act ? P[foff[f] + ci + (d - 2) * sz] : 0.0;
-    rn[f] = act ? P[foff[f] + ci + 2 * sz] : 0.0;
+    for (int d = 0; d < 4; d++) r[f][d] = ld ? (double)P[foff[f] + ci + (d - 2) * sz] : 0.0;
+    rn[f] = ld ? P[foff[f] + ci + 2 * sz] : (VT)0;
 #pragma unroll
-    for (int p = 0; p < NP; p++) pf[f][p] = fok[p] ? P[foff[f] + fidx[p]] : 0.0;
+    for (int p = 0; p < NP; p++) pf[f][p] = fok[p] ? P[foff[f] + fidx[p]] : (VT)0;
   }
   gptr<VT> __restrict__ V = gmem((VT*)B.visc);
   const long long sv = B.sv;
@@ -220,7 +234,7 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
         const int e = threadIdx.x + p * NT;
         if (e < LX * LY) (&L[buf][f][0][0])[e] = pf[f][p];
       }
-      r[f][4] = rn[f];
+      r[f][4] = (double)rn[f];
     }
     __syncthreads();
     if (bz + 1 < Z1) {
@@ -229,7 +243,7 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
 #pragma unroll
         for (int p = 0; p < NP; p++)
           if (fok[p]) pf[f][p] = P[foff[f] + fidx[p] + (long long)(bz + 1 - Z0) * sz];
-        if (act) rn[f] = P[foff[f] + ci + 3 * sz];
+        if (ld) rn[f] = P[foff[f] + ci + 3 * sz];
       }
     }
     if (act) {
@@ -295,8 +309,9 @@ __device__ __forceinline__ void visc_div_body(const BlockDesc& B, int t, const P
 // RK3 stage combination (src/Block3d.cpp:949-955) of cell c from the
 // right-hand side df, Q^n = Qn, the stage input Qi, into Qo; then the cell's
 // primitives into P (src/Block3d.cpp:369-381).  Qo may be Qn (stage 3).
+template <typename PT>
 __device__ __forceinline__ void rk3_prim(const double (&df)[NEQ], gptr<const double> Qn, gptr<const double> Qi,
-                                         gptr<double> Qo, gptr<double> __restrict__ P, long long sg, int c,
+                                         gptr<double> Qo, gptr<PT> __restrict__ P, long long sg, int c,
                                          double dt, int stage,
                                          const Phys& ph) {
   double qn[NEQ];
@@ -313,12 +328,12 @@ __device__ __forceinline__ void rk3_prim(const double (&df)[NEQ], gptr<const dou
   const double vv = qn[2] / rr;
   const double ww = qn[3] / rr;
   const double pp = ph.gam1 * (qn[4] - 0.5 * rr * (uu * uu + vv * vv + ww * ww));
-  P[c] = rr;
-  P[sg + c] = uu;
-  P[2 * sg + c] = vv;
-  P[3 * sg + c] = ww;
-  P[4 * sg + c] = pp;
-  P[5 * sg + c] = ph.gM2 * pp / rr;
+  P[c] = (PT)rr;
+  P[sg + c] = (PT)uu;
+  P[2 * sg + c] = (PT)vv;
+  P[3 * sg + c] = (PT)ww;
+  P[4 * sg + c] = (PT)pp;
+  P[5 * sg + c] = (PT)(ph.gM2 * pp / rr);
 }
 
 // src/Block3d.cpp:879-916, 920-962 and 359-385 for one interior cell; stage
@@ -349,8 +364,8 @@ __device__ __forceinline__ void update_body(const BlockDesc& B, int t, double dt
     if (e > 0) d -= dfv[e - 1];
     df[e] = d;
   }
-  rk3_prim(df, gmem<const double>(B.Q[0]), gmem<const double>(B.Q[1]), gmem(stage == 3 ? B.Q[0] : B.Q[1]),
-           gmem(B.prim), B.sg, c, dt, stage, ph);
+  rk3_prim<FT>(df, gmem<const double>(B.Q[0]), gmem<const double>(B.Q[1]), gmem(stage == 3 ? B.Q[0] : B.Q[1]),
+               gmem((FT*)B.prim), B.sg, c, dt, stage, ph);
 }
 
 // ---- calc_dt spectral radius of one cell from its primitives and mu
@@ -476,8 +491,8 @@ __device__ __forceinline__ void update_pair_body(const BlockDesc& B, int t, doub
     qn[0][e] = r.x;
     qn[1][e] = r.y;
   }
-  gptr<double> P = gmem(B.prim);
-  vec2<double> pr[6];
+  gptr<FT> P = gmem((FT*)B.prim);
+  vec2<FT> pr[6];
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const double rr = qn[h][0];
@@ -485,19 +500,19 @@ __device__ __forceinline__ void update_pair_body(const BlockDesc& B, int t, doub
     const double vv = qn[h][2] / rr;
     const double ww = qn[h][3] / rr;
     const double pp = ph.gam1 * (qn[h][4] - 0.5 * rr * (uu * uu + vv * vv + ww * ww));
-    pr[0][h] = rr;
-    pr[1][h] = uu;
-    pr[2][h] = vv;
-    pr[3][h] = ww;
-    pr[4][h] = pp;
-    pr[5][h] = ph.gM2 * pp / rr;
+    pr[0][h] = (FT)rr;
+    pr[1][h] = (FT)uu;
+    pr[2][h] = (FT)vv;
+    pr[3][h] = (FT)ww;
+    pr[4][h] = (FT)pp;
+    pr[5][h] = (FT)(ph.gM2 * pp / rr);
     if (want_s) {
       const double sc = dt_point(rr, uu, vv, ww, pp, gmem<const double>(B.mu)[iG + h], ph, bad);
       *s_out = (*s_out < sc) ? sc : *s_out;  // src/Block3d.cpp:546: a NaN never wins
     }
   }
 #pragma unroll
-  for (int f = 0; f < 6; f++) *(gptr<vec2<double>>)(P + f * sg + c) = pr[f];
+  for (int f = 0; f < 6; f++) *(gptr<vec2<FT>>)(P + f * sg + c) = pr[f];
 }
 
 // ---- ghost gather of the state and its primitives ------------------------
@@ -530,6 +545,7 @@ __device__ __forceinline__ int ghost_cell(const BlockDesc& B, int g) {
 // takes its periodic image's value, gathered from the block (of the same
 // solve: blocks[block0 + ...]) that owns the image.  The primitives of a
 // ghost are its image's (calc_primitive is pointwise).  t < B.sg.
+template <typename PT = double>
 __device__ __forceinline__ void ghost_body(const BlockDesc& B, const BlockDesc* __restrict__ blocks,
                                            const Domain& dom, int block0, int t, int which, bool with_prim) {
   const int i = t % B.g[0] - NG;
@@ -556,12 +572,15 @@ __device__ __forceinline__ void ghost_body(const BlockDesc& B, const BlockDesc* 
   for (int e = 0; e < NEQ; e++) Qd[e * B.sg + t] = Qs[e * S.sg + src];
   if (with_prim) {
 #pragma unroll
-    for (int f = 0; f < 6; f++) gmem(B.prim)[f * B.sg + t] = gmem<const double>(S.prim)[f * S.sg + src];
+    for (int f = 0; f < 6; f++) gmem((PT*)B.prim)[f * B.sg + t] = gmem<const PT>((const PT*)S.prim)[f * S.sg + src];
   }
 }
 
 // ---- calc_dt spectral radius of one interior cell (src/Block3d.cpp:527-546)
 // Returns t1 + t2 + t3; *bad is set for a NaN or infinite result.
+// PT = float (FP32-flux form, float primitives): the FP64 primitives are
+// formed from Q by calc_primitive's expressions instead of read.
+template <typename PT = double>
 __device__ __forceinline__ double dt_cell(const BlockDesc& B, int t, const Phys& ph, int* bad) {
   const int n0 = B.n[0], n1 = B.n[1];
   const int i = t % n0;
@@ -570,9 +589,19 @@ __device__ __forceinline__ double dt_cell(const BlockDesc& B, int t, const Phys&
   const int k = r / n1;
   const int c = gidx(B, i, j, k);
   const long long sg = B.sg;
-  gptr<const double> __restrict__ P = gmem<const double>(B.prim);
-  const double rr = P[c], uu = P[sg + c], vv = P[2 * sg + c], ww = P[3 * sg + c];
-  const double pp = P[4 * sg + c];
+  double rr, uu, vv, ww, pp;
+  if constexpr (sizeof(PT) == 8) {
+    gptr<const double> __restrict__ P = gmem<const double>(B.prim);
+    rr = P[c], uu = P[sg + c], vv = P[2 * sg + c], ww = P[3 * sg + c];
+    pp = P[4 * sg + c];
+  } else {  // src/Block3d.cpp:369-373
+    gptr<const double> __restrict__ Q = gmem<const double>(B.Q[0]);
+    rr = Q[c];
+    uu = Q[sg + c] / rr;
+    vv = Q[2 * sg + c] / rr;
+    ww = Q[3 * sg + c] / rr;
+    pp = ph.gam1 * (Q[4 * sg + c] - 0.5 * rr * (uu * uu + vv * vv + ww * ww));
+  }
   return dt_point(rr, uu, vv, ww, pp, gmem<const double>(B.mu)[t], ph, bad);
 }
 

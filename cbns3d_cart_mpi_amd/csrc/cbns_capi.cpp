@@ -241,12 +241,12 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
         if (st) break;
         if ((st = dev_alloc(c, &q, sizeof(double) * 4 * B.sn))) break;
         B.dfv = (double*)q;
-        if ((st = dev_alloc(c, &q, sizeof(double) * 6 * B.sg))) break;
+        // primitives and viscous fields: float in the FP32-flux form, double otherwise
+        const size_t vbytes = precision == CBNS_FP32_FLUX ? sizeof(float) : sizeof(double);
+        if ((st = dev_alloc(c, &q, vbytes * 6 * B.sg))) break;
         B.prim = (double*)q;
         if ((st = dev_alloc(c, &q, sizeof(double) * B.sn))) break;
         B.mu = (double*)q;
-        // viscous fields: float in the FP32-flux form, double otherwise
-        const size_t vbytes = precision == CBNS_FP32_FLUX ? sizeof(float) : sizeof(double);
         if ((st = dev_alloc(c, &q, vbytes * 9 * B.sv))) break;
         B.visc = (double*)q;
       }
@@ -328,7 +328,7 @@ int cbns_boundary_condition(cbns_ctx* c, int which) {
 int cbns_calc_primitive(cbns_ctx* c, int which) {
   if (!c || (which != CBNS_Q && which != CBNS_QP)) return CBNS_ERR_ARG;
   c->s_ready = false;
-  CK(launch_primitive(c->db, c->hb.data(), c->dom.nb, which, c->ph, c->stream));
+  CK(launch_primitive(c->db, c->hb.data(), c->dom.nb, which, c->ph, c->precision, c->stream));
   return CBNS_OK;
 }
 
@@ -359,7 +359,9 @@ int cbns_calc_dt(cbns_ctx* c, double* dt) {
   if (!c->have_state) return CBNS_ERR_STATE;
   CK(hipMemsetAsync(c->d_smax, 0, sizeof(unsigned long long), c->stream));
   CK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
-  CK(timed(c, 7, [&] { return launch_dt_reduce(c->db, c->hb.data(), c->dom.nb, c->ph, c->d_smax, c->d_flag, c->stream); }));
+  CK(timed(c, 7, [&] {
+    return launch_dt_reduce(c->db, c->hb.data(), c->dom.nb, c->ph, c->d_smax, c->d_flag, c->precision, c->stream);
+  }));
   return read_dt(c, dt);
 }
 
@@ -398,7 +400,7 @@ static int finish_setup(cbns_ctx* c) {
   if (c->precision == CBNS_FP32_FLUX) {
     // reference state of the deviation form: interior means of rho and E
     double* out = c->d_diag + 2 * c->diag_parts;
-    CK(launch_global_sums(c->db, c->dom, c->ph, 0, c->d_diag, out, c->stream));
+    CK(launch_global_sums(c->db, c->dom, c->ph, 0, c->d_diag, out, c->precision, c->stream));
     double sums[2];
     CK(hipMemcpyAsync(sums, out, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
     CK(hipStreamSynchronize(c->stream));
@@ -497,6 +499,12 @@ int cbns_get_block_primitive(cbns_ctx* c, int b, int f, double* out) {
   if (!c || !out || b < 0 || b >= c->dom.nb || f < 0 || f > 6) return CBNS_ERR_ARG;
   const BlockDesc& B = c->hb[b];
   CK(hipStreamSynchronize(c->stream));
+  if (f < 6 && c->precision == CBNS_FP32_FLUX) {  // float storage, widened
+    std::vector<float> pf(B.sg);
+    CK(hipMemcpy(pf.data(), (const float*)B.prim + f * B.sg, B.sg * sizeof(float), hipMemcpyDeviceToHost));
+    for (long long t = 0; t < B.sg; t++) out[t] = pf[t];
+    return CBNS_OK;
+  }
   if (f < 6) {
     CK(hipMemcpy(out, B.prim + f * B.sg, B.sg * sizeof(double), hipMemcpyDeviceToHost));
     return CBNS_OK;
@@ -560,7 +568,7 @@ static int run_stages(cbns_ctx* c, double dt) {
                                  stage == 3 ? c->d_smax : nullptr, stage == 3 ? c->d_flag : nullptr, &s_done);
     }));
     if (stage == 3) c->s_ready = s_done;
-    CK(timed(c, 5, [&] { return launch_ghost_fill_prim(c->db, hb, c->dom, out, c->stream); }));
+    CK(timed(c, 5, [&] { return launch_ghost_fill_prim(c->db, hb, c->dom, out, c->precision, c->stream); }));
   }
   return CBNS_OK;
 }
@@ -649,7 +657,8 @@ int cbns_kernel_times(cbns_ctx* c, double ms[CBNS_KCLASSES], int64_t n[CBNS_KCLA
 
 int cbns_diagnostics(cbns_ctx* c, double* ke, double* ens) {
   if (!c || !ke || !ens) return CBNS_ERR_ARG;
-  CK(launch_global_sums(c->db, c->dom, c->ph, 1, c->d_diag, c->d_diag + 2 * c->diag_parts, c->stream));
+  CK(launch_global_sums(c->db, c->dom, c->ph, 1, c->d_diag, c->d_diag + 2 * c->diag_parts, c->precision,
+                        c->stream));
   double out[2];
   CK(hipMemcpyAsync(out, c->d_diag + 2 * c->diag_parts, sizeof(out), hipMemcpyDeviceToHost, c->stream));
   CK(hipStreamSynchronize(c->stream));

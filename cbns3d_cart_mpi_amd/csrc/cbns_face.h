@@ -297,12 +297,21 @@ __device__ __forceinline__ float weno5f(float Um2, float Um1, float U0, float Up
   float IS1 = (13.0f / 12.0f) * diff1 * diff1 + 0.25f * diff2 * diff2 + eps;
   float IS2 = (13.0f / 12.0f) * diff3 * diff3 + 0.25f * diff4 * diff4 + eps;
   float IS3 = (13.0f / 12.0f) * diff5 * diff5 + 0.25f * diff6 * diff6 + eps;
-  float w1 = 0.1f * frcp(IS1 * IS1);
-  float w2 = 0.6f * frcp(IS2 * IS2);
-  float w3 = 0.3f * frcp(IS3 * IS3);
   float f1 = 2.0f * Um2 - 7.0f * Um1 + 11.0f * U0;
   float f2 = -Um1 + 5.0f * U0 + 2.0f * Up1;
   float f3 = 2.0f * U0 + 5.0f * Up1 - Up2;
+  // weights c_k / IS_k^2 scaled by IS1^2 IS2^2 IS3^2 (the scale cancels in
+  // the normalisation): one reciprocal instead of four.  IS >= eps keeps the
+  // products from underflowing; while den < 1e30 (every IS below ~1e7, so
+  // |f_k| below ~1e4) neither den nor the numerator can overflow.  Past that
+  // (or for a NaN) the unscaled weights are used.
+  const float a = IS1 * IS1, b = IS2 * IS2, c = IS3 * IS3;
+  const float s1 = 0.1f * (b * c), s2 = 0.6f * (a * c), s3 = 0.3f * (a * b);
+  const float den = 6.0f * (s1 + s2 + s3);
+  if (den < 1.0e30f) return (s1 * f1 + s2 * f2 + s3 * f3) * frcp(den);
+  float w1 = 0.1f * frcp(a);
+  float w2 = 0.6f * frcp(b);
+  float w3 = 0.3f * frcp(c);
   return (w1 * f1 + w2 * f2 + w3 * f3) * frcp(6.0f * (w1 + w2 + w3));
 }
 
@@ -314,9 +323,12 @@ __device__ __forceinline__ void face_flux_f32(const double (&q)[6][5], const dou
   const float gf = (float)gamma, g1 = (float)gam1;
   // ---- Roe average
   float uu, vv, ww, H, c;
-  const double rr = pr[0] / pl[0];
-  const bool wide = !(rr > 0.5 && rr < 2.0) || pl[0] < 1.0e-3 * g.rho || pr[0] < 1.0e-3 * g.rho;
+  const float rl = (float)pl[0], rrf = (float)pr[0];
+  const float irl = frcp(rl), ratf = rrf * irl;
+  const float rho_min = (float)(1.0e-3 * g.rho);
+  const bool wide = !(ratf > 0.5f && ratf < 2.0f) || rl < rho_min || rrf < rho_min;
   if (wide) {
+    const double rr = pr[0] / pl[0];
     double c_L = sqrt(gamma * pl[4] / pl[0]);
     double H_L = c_L * c_L / gam1 + 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
     double c_R = sqrt(gamma * pr[4] / pr[0]);
@@ -331,12 +343,12 @@ __device__ __forceinline__ void face_flux_f32(const double (&q)[6][5], const dou
     H = (float)H_;
     c = (float)sqrt(gam1 * (H_ - 0.5 * (u_ * u_ + v_ * v_ + w_ * w_)));
   } else {
-    const float rl = (float)pl[0], rrf = (float)pr[0];
     const float ul = (float)pl[1], vl = (float)pl[2], wl = (float)pl[3], plf = (float)pl[4];
     const float ur = (float)pr[1], vr = (float)pr[2], wr = (float)pr[3], prf = (float)pr[4];
-    const float HL = gf / g1 * plf * frcp(rl) + 0.5f * (ul * ul + vl * vl + wl * wl);
-    const float HR = gf / g1 * prf * frcp(rrf) + 0.5f * (ur * ur + vr * vr + wr * wr);
-    const float ratio = fsqrt((float)rr);
+    const float gg1 = gf * frcp(g1);
+    const float HL = gg1 * plf * irl + 0.5f * (ul * ul + vl * vl + wl * wl);
+    const float HR = gg1 * prf * frcp(rrf) + 0.5f * (ur * ur + vr * vr + wr * wr);
+    const float ratio = fsqrt(ratf);
     const float inv = frcp(1.0f + ratio);
     uu = (ul + ratio * ur) * inv;
     vv = (vl + ratio * vr) * inv;

@@ -31,7 +31,7 @@ struct BlockDesc {
   double* Q[2];   // Q, Q_p: [5][g2][g1][g0], ghosted
   double* F[3];   // face fluxes Ep, Fp, Gp: [5][n2+dz][n1+dy][n0+dx]
   double* dfv;    // viscous divergence [4][n2][n1][n0]
-  double* prim;   // rho, u, v, w, p, T: [6][g2][g1][g0]
+  double* prim;   // rho, u, v, w, p, T: [6][g2][g1][g0] (float storage in the FP32-flux form)
   double* mu;     // Sutherland viscosity on the interior [n2][n1][n0]
   double* visc;   // txx tyy tzz txy txz tyz ex ey ez on the (n+4)^3 box
   int n[3];       // interior cells
@@ -79,7 +79,7 @@ struct RefState;
 hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
                             const Phys& ph, int precision, const RefState& g, int jb, int occ, hipStream_t s);
 hipError_t launch_primitive(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
-                            hipStream_t s);
+                            int precision, hipStream_t s);
 hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
                              hipStream_t s);
 // viscous fields stored as float for PREC_FP32_FLUX, double otherwise
@@ -94,11 +94,11 @@ hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb,
                                const Phys& ph, int precision, int jb, hipStream_t s,
                                unsigned long long* smax = nullptr, int* flag = nullptr, bool* s_done = nullptr);
 hipError_t launch_ghost_fill_prim(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
-                                  hipStream_t s);
+                                  int precision, hipStream_t s);
 hipError_t launch_update(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
                          const Phys& ph, int precision, int jb, hipStream_t s);
 hipError_t launch_dt_reduce(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph,
-                            unsigned long long* d_smax, int* d_flag, hipStream_t s);
+                            unsigned long long* d_smax, int* d_flag, int precision, hipStream_t s);
 hipError_t launch_copy_state(const BlockDesc* db, const BlockDesc* hb, int nb, int from, int to, hipStream_t s);
 hipError_t launch_tgv(const BlockDesc* db, const BlockDesc* hb, int nb, const double* tab, int tabstride,
                       const Phys& ph, double p_inf, hipStream_t s);
@@ -113,7 +113,7 @@ hipError_t launch_fill_mu(const BlockDesc* db, const BlockDesc* hb, int nb, doub
 constexpr int VX = 64, VY = 4, VZ = 16;
 constexpr int GSUM_WG = 512;  // workgroups of the fixed-order global sums
 hipError_t launch_global_sums(const BlockDesc* db, const Domain& dom, const Phys& ph, int what, double* d_part,
-                              double* d_out, hipStream_t s);
+                              double* d_out, int precision, hipStream_t s);
 hipError_t launch_face_eval(int dir, const Phys& ph, int nfaces, const double* cells, const double* prim,
                             double* flux, hipStream_t s);
 // RN(1/h) and whether three-operation division by h is exact (see Phys).

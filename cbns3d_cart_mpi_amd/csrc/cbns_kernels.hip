@@ -75,6 +75,8 @@ hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, in
 
 // ---------------------------------------------------------------------------
 // calc_primitive over the whole ghosted box (src/Block3d.cpp:359-385).
+// PT: primitive storage (float in the FP32-flux form), computed in FP64.
+template <typename PT>
 __global__ __launch_bounds__(TPB) void k_primitive(const BlockDesc* __restrict__ blocks, int which, double gam1,
                                                    double gM2) {
   const Tile tl = xcd_tile();
@@ -83,25 +85,28 @@ __global__ __launch_bounds__(TPB) void k_primitive(const BlockDesc* __restrict__
   const int t = tl.x * TPB + threadIdx.x;
   if (t >= sg) return;
   gptr<const double> __restrict__ Q = gmem<const double>(B.Q[which]);
-  gptr<double> __restrict__ P = gmem(B.prim);
+  gptr<PT> __restrict__ P = gmem((PT*)B.prim);
   double rr = Q[t];
   double uu = Q[sg + t] / rr;
   double vv = Q[2 * sg + t] / rr;
   double ww = Q[3 * sg + t] / rr;
   double pp = gam1 * (Q[4 * sg + t] - 0.5 * rr * (uu * uu + vv * vv + ww * ww));
-  P[t] = rr;
-  P[sg + t] = uu;
-  P[2 * sg + t] = vv;
-  P[3 * sg + t] = ww;
-  P[4 * sg + t] = pp;
-  P[5 * sg + t] = gM2 * pp / rr;
+  P[t] = (PT)rr;
+  P[sg + t] = (PT)uu;
+  P[2 * sg + t] = (PT)vv;
+  P[3 * sg + t] = (PT)ww;
+  P[4 * sg + t] = (PT)pp;
+  P[5 * sg + t] = (PT)(gM2 * pp / rr);
 }
 
 hipError_t launch_primitive(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
-                            hipStream_t s) {
+                            int precision, hipStream_t s) {
   long long mx = 0;
   for (int b = 0; b < nb; b++) mx = hb[b].sg > mx ? hb[b].sg : mx;
-  hipLaunchKernelGGL(k_primitive, dim3(nblocks_for(mx), nb), dim3(TPB), 0, s, db, which, ph.gam1, ph.gM2);
+  const dim3 grid(nblocks_for(mx), nb);
+  if (precision == PREC_FP32_FLUX)
+    hipLaunchKernelGGL(k_primitive<float>, grid, dim3(TPB), 0, s, db, which, ph.gam1, ph.gM2);
+  else hipLaunchKernelGGL(k_primitive<double>, grid, dim3(TPB), 0, s, db, which, ph.gam1, ph.gM2);
   return hipGetLastError();
 }
 
@@ -308,18 +313,21 @@ hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb,
 
 // Ghost fill of the state and of its primitives together: the primitives of
 // a ghost cell are those of its periodic image (calc_primitive is pointwise).
+template <typename PT>
 __global__ __launch_bounds__(TPB) void k_ghost_fill_prim(const BlockDesc* __restrict__ blocks, Domain dom, int which) {
   const BlockDesc& B = blocks[blockIdx.y];
   const int g = blockIdx.x * TPB + threadIdx.x;
   if (g >= B.sg - B.sn) return;
-  ghost_body(B, blocks, dom, 0, ghost_cell(B, g), which, true);
+  ghost_body<PT>(B, blocks, dom, 0, ghost_cell(B, g), which, true);
 }
 
 hipError_t launch_ghost_fill_prim(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
-                                  hipStream_t s) {
+                                  int precision, hipStream_t s) {
   long long mx = 0;
   for (int b = 0; b < dom.nb; b++) mx = std::max(mx, hb[b].sg - hb[b].sn);
-  hipLaunchKernelGGL(k_ghost_fill_prim, dim3(nblocks_for(mx), dom.nb), dim3(TPB), 0, s, db, dom, which);
+  const dim3 grid(nblocks_for(mx), dom.nb);
+  if (precision == PREC_FP32_FLUX) hipLaunchKernelGGL(k_ghost_fill_prim<float>, grid, dim3(TPB), 0, s, db, dom, which);
+  else hipLaunchKernelGGL(k_ghost_fill_prim<double>, grid, dim3(TPB), 0, s, db, dom, which);
   return hipGetLastError();
 }
 
@@ -331,6 +339,7 @@ hipError_t launch_ghost_fill_prim(const BlockDesc* db, const BlockDesc* hb, cons
 // max is an integer atomic max.  A NaN never wins (the reference's ternary
 // skips it) but raises the non-finite flag, as does +inf.
 
+template <typename PT>
 __global__ __launch_bounds__(TPB) void k_dt(const BlockDesc* __restrict__ blocks, Phys ph,
                                             unsigned long long* smax, int* flag) {
   const BlockDesc& B = blocks[blockIdx.y];
@@ -338,7 +347,7 @@ __global__ __launch_bounds__(TPB) void k_dt(const BlockDesc* __restrict__ blocks
   double s_max = 0.0;
   int bad = 0;
   for (int t = blockIdx.x * TPB + threadIdx.x; t < total; t += gridDim.x * TPB) {
-    const double s = dt_cell(B, t, ph, &bad);
+    const double s = dt_cell<PT>(B, t, ph, &bad);
     s_max = (s_max < s) ? s : s_max;  // src/Block3d.cpp:546: a NaN never wins
   }
   s_max = wave_max(s_max);
@@ -355,12 +364,13 @@ __global__ __launch_bounds__(TPB) void k_dt(const BlockDesc* __restrict__ blocks
 }
 
 hipError_t launch_dt_reduce(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph,
-                            unsigned long long* d_smax, int* d_flag, hipStream_t s) {
+                            unsigned long long* d_smax, int* d_flag, int precision, hipStream_t s) {
   long long mn = 0;
   for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
   unsigned gx = nblocks_for(mn);
   gx = gx > 1024 ? 1024 : gx;
-  hipLaunchKernelGGL(k_dt, dim3(gx, nb), dim3(TPB), 0, s, db, ph, d_smax, d_flag);
+  if (precision == PREC_FP32_FLUX) hipLaunchKernelGGL(k_dt<float>, dim3(gx, nb), dim3(TPB), 0, s, db, ph, d_smax, d_flag);
+  else hipLaunchKernelGGL(k_dt<double>, dim3(gx, nb), dim3(TPB), 0, s, db, ph, d_smax, d_flag);
   return hipGetLastError();
 }
 
@@ -493,7 +503,7 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <int WHAT>
+template <int WHAT, typename PT = double>
 __global__ __launch_bounds__(TPB) void k_global_sums(const BlockDesc* __restrict__ blocks, Domain dom, Phys ph,
                                                      double* part) {
   const long long total = (long long)dom.N[0] * dom.N[1] * dom.N[2];
@@ -520,10 +530,11 @@ __global__ __launch_bounds__(TPB) void k_global_sums(const BlockDesc* __restrict
     } else {
       const long long sg = B.sg;
       const int sy = B.g[0], sz = B.g[0] * B.g[1];
-      gptr<const double> u = gmem<const double>(B.prim + sg);
-      gptr<const double> v = gmem<const double>(B.prim + 2 * sg);
-      gptr<const double> w = gmem<const double>(B.prim + 3 * sg);
-      const double rr = gmem<const double>(B.prim)[c], uu = u[c], vv = v[c], ww = w[c];
+      const PT* pb = (const PT*)B.prim;
+      gptr<const PT> u = gmem<const PT>(pb + sg);
+      gptr<const PT> v = gmem<const PT>(pb + 2 * sg);
+      gptr<const PT> w = gmem<const PT>(pb + 3 * sg);
+      const double rr = gmem<const PT>(pb)[c], uu = u[c], vv = v[c], ww = w[c];
       sa += 0.5 * rr * (uu * uu + vv * vv + ww * ww);
       const double ox = grad4(w, c, sy, ph.dy, ph.rdy, ph.fdy) - grad4(v, c, sz, ph.dz, ph.rdz, ph.fdz);
       const double oy = grad4(u, c, sz, ph.dz, ph.rdz, ph.fdz) - grad4(w, c, 1, ph.dx, ph.rdx, ph.fdx);
@@ -564,9 +575,11 @@ __global__ void k_sums_final(const double* part, int nparts, double* out) {
 }
 
 hipError_t launch_global_sums(const BlockDesc* db, const Domain& dom, const Phys& ph, int what, double* d_part,
-                              double* d_out, hipStream_t s) {
+                              double* d_out, int precision, hipStream_t s) {
   if (what == 0) hipLaunchKernelGGL(k_global_sums<0>, dim3(GSUM_WG), dim3(TPB), 0, s, db, dom, ph, d_part);
-  else hipLaunchKernelGGL(k_global_sums<1>, dim3(GSUM_WG), dim3(TPB), 0, s, db, dom, ph, d_part);
+  else if (precision == PREC_FP32_FLUX)
+    hipLaunchKernelGGL((k_global_sums<1, float>), dim3(GSUM_WG), dim3(TPB), 0, s, db, dom, ph, d_part);
+  else hipLaunchKernelGGL((k_global_sums<1, double>), dim3(GSUM_WG), dim3(TPB), 0, s, db, dom, ph, d_part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(64), 0, s, d_part, GSUM_WG, d_out);
