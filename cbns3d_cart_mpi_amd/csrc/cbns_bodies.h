@@ -125,6 +125,47 @@ __device__ __forceinline__ void face_body_f32(const BlockDesc& B, int t, int whi
   for (int e = 0; e < 5; e++) F[e * B.sf[D] + fi] = fl[e];
 }
 
+// One face on explicit inputs: cells [nf][6][5] (cells i-3..i+2), prim
+// [nf][2][5] (rho u v w p of cells i-1, i), flux [nf][5].  MODE 0: plain
+// FP64; 1: contracted FP64 (instantiated only in the translation unit built
+// with FMA contraction); 2: FP32-flux, with the face's own reference state
+// (rho and E means of cells i-1 and i, p = (gamma - 1) E), the primitives
+// rounded to float as they are stored, and F(Ug) (p_g in the normal
+// momentum) added back in FP64 so that the result is the flux itself.
+template <int D, int MODE>
+__device__ __forceinline__ void face_eval_body(const Phys& ph, int t, const double* __restrict__ cells,
+                                               const double* __restrict__ prim, double* __restrict__ flux) {
+  double q[6][5], pl[5], pr[5], fl[5];
+#pragma unroll
+  for (int c = 0; c < 6; c++)
+#pragma unroll
+    for (int e = 0; e < 5; e++) q[c][e] = cells[(long long)t * 30 + 5 * c + e];
+#pragma unroll
+  for (int e = 0; e < 5; e++) {
+    pl[e] = prim[(long long)t * 10 + e];
+    pr[e] = prim[(long long)t * 10 + 5 + e];
+  }
+  if constexpr (MODE == 2) {
+    RefState g;
+    g.rho = 0.5 * (q[2][0] + q[3][0]);
+    g.E = 0.5 * (q[2][4] + q[3][4]);
+    g.p = ph.gam1 * g.E;
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+      pl[e] = (double)(float)pl[e];
+      pr[e] = (double)(float)pr[e];
+    }
+    float f32[5];
+    face_flux_f32<D>(q, pl, pr, ph.gamma, ph.gam1, g, f32);
+#pragma unroll
+    for (int e = 0; e < 5; e++) fl[e] = (double)f32[e] + (e == 1 + D ? g.p : 0.0);
+  } else {
+    face_flux<D, MODE == 1>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl);
+  }
+#pragma unroll
+  for (int e = 0; e < 5; e++) flux[(long long)t * 5 + e] = fl[e];
+}
+
 // ---- viscous fluxes on the extended box [-2, n+2)^3 -----------------------
 // calc_gradient (src/calc_gradient.cpp:17-66), calc_viscous_terms
 // (src/Block3d.cpp:750-810) and calc_viscous_flux (src/Block3d.cpp:812-868),

@@ -675,7 +675,7 @@ int cbns_face_flux_eval(int precision, int dir, double gamma, int nf, const doub
   ph.gamma = gamma;
   ph.gam1 = gamma - 1.0;
   reciprocal_check(ph.gam1, &ph.rgam1, &ph.fgam1);
-  if (precision != CBNS_FP64) return CBNS_ERR_ARG;
+  if (precision != CBNS_FP64 && precision != CBNS_FP32_FLUX && precision != CBNS_FP64_FMA) return CBNS_ERR_ARG;
   if (nf == 0) return CBNS_OK;
   double *dc = nullptr, *dp = nullptr, *df = nullptr;
   int st = CBNS_OK;
@@ -685,7 +685,7 @@ int cbns_face_flux_eval(int precision, int dir, double gamma, int nf, const doub
     st = CBNS_ERR_NOMEM;
   if (!st && (hipMemcpy(dc, cells, sizeof(double) * 30 * (size_t)nf, hipMemcpyHostToDevice) != hipSuccess ||
               hipMemcpy(dp, prim, sizeof(double) * 10 * (size_t)nf, hipMemcpyHostToDevice) != hipSuccess ||
-              launch_face_eval(dir, ph, nf, dc, dp, df, nullptr) != hipSuccess ||
+              launch_face_eval(dir, ph, precision, nf, dc, dp, df, nullptr) != hipSuccess ||
               hipDeviceSynchronize() != hipSuccess ||
               hipMemcpy(flux, df, sizeof(double) * 5 * (size_t)nf, hipMemcpyDeviceToHost) != hipSuccess))
     st = CBNS_ERR_HIP;

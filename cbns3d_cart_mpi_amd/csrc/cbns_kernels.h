@@ -114,8 +114,10 @@ constexpr int VX = 64, VY = 4, VZ = 16;
 constexpr int GSUM_WG = 512;  // workgroups of the fixed-order global sums
 hipError_t launch_global_sums(const BlockDesc* db, const Domain& dom, const Phys& ph, int what, double* d_part,
                               double* d_out, int precision, hipStream_t s);
-hipError_t launch_face_eval(int dir, const Phys& ph, int nfaces, const double* cells, const double* prim,
-                            double* flux, hipStream_t s);
+hipError_t launch_face_eval(int dir, const Phys& ph, int precision, int nfaces, const double* cells,
+                            const double* prim, double* flux, hipStream_t s);
+hipError_t launch_face_eval_fma(int dir, const Phys& ph, int nfaces, const double* cells, const double* prim,
+                                double* flux, hipStream_t s);
 // RN(1/h) and whether three-operation division by h is exact (see Phys).
 void reciprocal_check(double h, double* rh, int* fast);
 

@@ -588,39 +588,29 @@ hipError_t launch_global_sums(const BlockDesc* db, const Domain& dom, const Phys
 
 // ---------------------------------------------------------------------------
 // Face flux on explicit inputs (parity tests of the per-face arithmetic).
-template <int D>
-__global__ __launch_bounds__(TPB) void k_face_eval(double gamma, double gam1, double rg1, int fg1, int nf,
-                                                   const double* __restrict__ cells,
+template <int D, bool F32>
+__global__ __launch_bounds__(TPB) void k_face_eval(Phys ph, int nf, const double* __restrict__ cells,
                                                    const double* __restrict__ prim, double* __restrict__ flux) {
   const int t = blockIdx.x * TPB + threadIdx.x;
   if (t >= nf) return;
-  double q[6][5], pl[5], pr[5], fl[5];
-#pragma unroll
-  for (int c = 0; c < 6; c++)
-#pragma unroll
-    for (int e = 0; e < 5; e++) q[c][e] = cells[(long long)t * 30 + 5 * c + e];
-#pragma unroll
-  for (int e = 0; e < 5; e++) {
-    pl[e] = prim[(long long)t * 10 + e];
-    pr[e] = prim[(long long)t * 10 + 5 + e];
-  }
-  face_flux<D>(q, pl, pr, gamma, gam1, rg1, fg1, fl);
-#pragma unroll
-  for (int e = 0; e < 5; e++) flux[(long long)t * 5 + e] = fl[e];
+  face_eval_body<D, F32 ? 2 : 0>(ph, t, cells, prim, flux);
 }
 
-hipError_t launch_face_eval(int dir, const Phys& ph, int nf, const double* cells, const double* prim, double* flux,
-                            hipStream_t s) {
+hipError_t launch_face_eval(int dir, const Phys& ph, int precision, int nf, const double* cells, const double* prim,
+                            double* flux, hipStream_t s) {
+  if (precision == PREC_FP64_FMA) return launch_face_eval_fma(dir, ph, nf, cells, prim, flux, s);
   dim3 grid(nblocks_for(nf));
-  if (dir == 0)
-    hipLaunchKernelGGL(k_face_eval<0>, grid, dim3(TPB), 0, s, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, nf, cells, prim,
-                       flux);
-  else if (dir == 1)
-    hipLaunchKernelGGL(k_face_eval<1>, grid, dim3(TPB), 0, s, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, nf, cells, prim,
-                       flux);
-  else
-    hipLaunchKernelGGL(k_face_eval<2>, grid, dim3(TPB), 0, s, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, nf, cells, prim,
-                       flux);
+#define CBNS_EVAL(DIR, F32) hipLaunchKernelGGL((k_face_eval<DIR, F32>), grid, dim3(TPB), 0, s, ph, nf, cells, prim, flux)
+  if (precision == PREC_FP32_FLUX) {
+    if (dir == 0) CBNS_EVAL(0, true);
+    else if (dir == 1) CBNS_EVAL(1, true);
+    else CBNS_EVAL(2, true);
+  } else {
+    if (dir == 0) CBNS_EVAL(0, false);
+    else if (dir == 1) CBNS_EVAL(1, false);
+    else CBNS_EVAL(2, false);
+  }
+#undef CBNS_EVAL
   return hipGetLastError();
 }
 

@@ -43,4 +43,22 @@ hipError_t launch_face_flux_fma(const BlockDesc* db, const BlockDesc* hb, int nb
   return hipGetLastError();
 }
 
+// Contracted-form face flux on explicit inputs (parity tests).
+template <int D>
+__global__ __launch_bounds__(TPB) void k_face_eval_fma(Phys ph, int nf, const double* __restrict__ cells,
+                                                       const double* __restrict__ prim, double* __restrict__ flux) {
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= nf) return;
+  face_eval_body<D, 1>(ph, t, cells, prim, flux);
+}
+
+hipError_t launch_face_eval_fma(int dir, const Phys& ph, int nf, const double* cells, const double* prim,
+                                double* flux, hipStream_t s) {
+  dim3 grid(nblocks_fma(nf));
+  if (dir == 0) hipLaunchKernelGGL(k_face_eval_fma<0>, grid, dim3(TPB), 0, s, ph, nf, cells, prim, flux);
+  else if (dir == 1) hipLaunchKernelGGL(k_face_eval_fma<1>, grid, dim3(TPB), 0, s, ph, nf, cells, prim, flux);
+  else hipLaunchKernelGGL(k_face_eval_fma<2>, grid, dim3(TPB), 0, s, ph, nf, cells, prim, flux);
+  return hipGetLastError();
+}
+
 }  // namespace cbns

@@ -176,8 +176,12 @@ int cbns_kernel_times(cbns_ctx* ctx, double ms_total[CBNS_KCLASSES], int64_t lau
 
 /* Evaluate the device face flux of one face along axis dir on host-given data:
  * cells[6][5] conserved values of cells i-3..i+2 (AoS), prim[2][5] = rho, u, v,
- * w, p of cells i-1 and i.  Same arithmetic as cbns_rec_riemann; for parity
- * tests of the per-face path (src/Block3d.cpp:586-612).  nfaces faces at once. */
+ * w, p of cells i-1 and i.  Same arithmetic as cbns_rec_riemann in the given
+ * precision; for parity tests of the per-face path (src/Block3d.cpp:586-612).
+ * nfaces faces at once.  CBNS_FP32_FLUX evaluates the deviation form about
+ * the face's own reference state (rho and E means of cells i-1 and i, at
+ * rest), with the primitives rounded to float as the solver stores them, and
+ * returns the flux itself (F(Ug) added back in FP64). */
 int cbns_face_flux_eval(int precision, int dir, double gamma, int nfaces,
                         const double* cells, const double* prim, double* flux);
 
