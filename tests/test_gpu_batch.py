@@ -28,7 +28,7 @@ def test_batch_of_uneven_solves_matches_reference_and_single_runs(golden):
             "restart": b.add(C.make_params(16), Q=gr["Q_in"], t_cur=float(gr["t_in"]), mu0=0.0,
                              steps=int(gr["nsteps"])),
             "fp32": b.add(C.make_params(24, Mach=0.3, Re=400.0), dims=(1, 2, 1), precision=C.CBNS_FP32_FLUX, steps=4),
-            "odd": b.add(C.make_params((20, 16, 12), Mach=0.2, Re=800.0), steps=7),
+            "odd": b.add(C.make_params((21, 16, 12), Mach=0.2, Re=800.0), steps=7),
         }
         b.run(12)
         done, live = b.poll()
@@ -43,7 +43,7 @@ def test_batch_of_uneven_solves_matches_reference_and_single_runs(golden):
         assert np.array_equal(b.get_state(ids["restart"]), gr["Q_out"])
         Q, t = single(24, 4, (1, 2, 1), C.CBNS_FP32_FLUX, Mach=0.3, Re=400.0)
         assert st["fp32"]["t_cur"] == t and np.array_equal(b.get_state(ids["fp32"]), Q)
-        Q, t = single((20, 16, 12), 7, Mach=0.2, Re=800.0)
+        Q, t = single((21, 16, 12), 7, Mach=0.2, Re=800.0)  # odd x-width: per-cell stage tail
         assert st["odd"]["t_cur"] == t and np.array_equal(b.get_state(ids["odd"]), Q)
 
 

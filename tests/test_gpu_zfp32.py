@@ -26,7 +26,8 @@ def rel_errors(Q, R):
     return out
 
 
-@pytest.mark.parametrize("N,steps,dims", [(32, 50, (1, 1, 1)), (48, 30, (2, 1, 1))])
+@pytest.mark.parametrize("N,steps,dims", [(32, 50, (1, 1, 1)), (48, 30, (2, 1, 1)),
+                                          ((25, 18, 14), 30, (2, 1, 1))])  # odd x-widths 12/13
 def test_fp32_flux_tgv_error_vs_fp64_oracle(N, steps, dims):
     b = O.Block(O.make_params(N))
     b.init_tgv()
