@@ -315,10 +315,21 @@ __device__ __forceinline__ float weno5f(float Um2, float Um1, float U0, float Up
   return (w1 * f1 + w2 * f2 + w3 * f3) * frcp(6.0f * (w1 + w2 + w3));
 }
 
+// Deviation of one cell's conserved state from the reference state: FP64
+// difference, then FP32.
+__device__ __forceinline__ void dev_f32(const double (&q)[5], const RefState& g, float (&d)[5]) {
+  d[0] = (float)(q[0] - g.rho);
+  d[1] = (float)q[1];
+  d[2] = (float)q[2];
+  d[3] = (float)q[3];
+  d[4] = (float)(q[4] - g.E);
+}
+
+// FP32-flux face flux from the deviations d of cells i-3 .. i+2.
 template <int D>
-__device__ __forceinline__ void face_flux_f32(const double (&q)[6][5], const double (&pl)[5],
-                                              const double (&pr)[5], double gamma, double gam1,
-                                              const RefState& g, float (&flux)[5]) {
+__device__ __forceinline__ void face_flux_f32_dev(const float (&d)[6][5], const double (&pl)[5],
+                                                  const double (&pr)[5], double gamma, double gam1,
+                                                  const RefState& g, float (&flux)[5]) {
 #pragma clang fp contract(fast)
   const float gf = (float)gamma, g1 = (float)gam1;
   // ---- Roe average
@@ -371,16 +382,6 @@ __device__ __forceinline__ void face_flux_f32(const double (&q)[6][5], const dou
   const float Rp2[3] = {D == 0 ? uu + c : uu, D == 1 ? vv + c : vv, D == 2 ? ww + c : ww};
   const float R40 = H - qn * c, R41 = 0.5f * q2, R42 = H + qn * c;
 
-  // deviations from the reference state (FP64 difference, then FP32)
-  float d[6][5];
-#pragma unroll
-  for (int cc = 0; cc < 6; cc++) {
-    d[cc][0] = (float)(q[cc][0] - g.rho);
-    d[cc][1] = (float)q[cc][1];
-    d[cc][2] = (float)q[cc][2];
-    d[cc][3] = (float)q[cc][3];
-    d[cc][4] = (float)(q[cc][4] - g.E);
-  }
   float aL[5], aR[5];
 #pragma unroll
   for (int m = 0; m < 5; m++) {
@@ -447,6 +448,16 @@ __device__ __forceinline__ void face_flux_f32(const double (&q)[6][5], const dou
   if (wsL > ws) ws = wsL;
 #pragma unroll
   for (int e = 0; e < 5; e++) flux[e] = 0.5f * (F[0][e] + F[1][e] - ws * (aR[e] - aL[e]));
+}
+
+template <int D>
+__device__ __forceinline__ void face_flux_f32(const double (&q)[6][5], const double (&pl)[5],
+                                              const double (&pr)[5], double gamma, double gam1,
+                                              const RefState& g, float (&flux)[5]) {
+  float d[6][5];
+#pragma unroll
+  for (int cc = 0; cc < 6; cc++) dev_f32(q[cc], g, d[cc]);
+  face_flux_f32_dev<D>(d, pl, pr, gamma, gam1, g, flux);
 }
 
 }  // namespace cbns

@@ -32,15 +32,21 @@ __global__ __launch_bounds__(TPB, OCC) void k_face_flux(const BlockDesc* __restr
   face_body<D>(B, t, which, jb, ph);
 }
 
-// FP32-flux form: same stencil and mapping, FP32 face arrays holding F - F(Ug).
+// FP32-flux form: same stencil, FP32 face arrays holding F - F(Ug); the y
+// and z sweeps take two adjacent faces per work item (face_pair_body_f32).
 template <int D>
 __global__ __launch_bounds__(TPB) void k_face_flux_f32(const BlockDesc* __restrict__ blocks, int which, int jb,
                                                        Phys ph, RefState g) {
   const Tile tl = xcd_tile();
   const BlockDesc& B = blocks[tl.y];
   const int t = tl.x * TPB + threadIdx.x;
-  if (t >= face_count(B, D)) return;
-  face_body_f32<D>(B, t, which, jb, ph, g);
+  if (D == 0) {
+    if (t >= face_count(B, D)) return;
+    face_body_f32<D>(B, t, which, jb, ph, g);
+  } else {
+    if (t >= face_pair_count(B, D)) return;
+    face_pair_body_f32<D == 0 ? 1 : D>(B, t, which, jb, ph, g);
+  }
 }
 
 hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
@@ -51,6 +57,14 @@ hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, in
     mx = c > mx ? c : mx;
   }
   if (precision == PREC_FP64_FMA) return launch_face_flux_fma(db, hb, nb, dir, which, ph, jb, occ, s);
+  if (precision == PREC_FP32_FLUX && dir > 0) {  // work items are pairs of faces along dir
+    mx = 0;
+    for (int b = 0; b < nb; b++) {
+      const long long ni = hb[b].n[0], nj = hb[b].n[1] + (dir == 1), nk = hb[b].n[2] + (dir == 2);
+      const long long c = dir == 1 ? ni * ((nj + 1) / 2) * nk : ni * nj * ((nk + 1) / 2);
+      mx = c > mx ? c : mx;
+    }
+  }
   dim3 grid(nblocks_for(mx), nb);
   if (precision == PREC_FP64) {
 #define CBNS_FACE(DIR, OCC) \
