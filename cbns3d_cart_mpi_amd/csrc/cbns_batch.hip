@@ -30,9 +30,10 @@ __global__ __launch_bounds__(TPB, 2) void kb_face(const BlockDesc* __restrict__ 
     const InstDesc& I = insts[B.inst];
     if (!I.stepping) continue;
     const int t = tr.tile * TPB + threadIdx.x;
-    if (P == 1 && D > 0) {  // FP32-flux y, z: pairs of faces (the table counts pairs)
+    if (D > 0) {  // y, z: pairs of faces (the table counts pairs)
       if (t >= face_pair_count(B, D)) continue;
-      face_pair_body_f32<D == 0 ? 1 : D>(B, t, which, jb, I.ph, I.ref);
+      if (P == 0) face_pair_body<D == 0 ? 1 : D>(B, t, which, jb, I.ph);
+      else face_pair_body_f32<D == 0 ? 1 : D>(B, t, which, jb, I.ph, I.ref);
       continue;
     }
     if (t >= face_count(B, D)) continue;

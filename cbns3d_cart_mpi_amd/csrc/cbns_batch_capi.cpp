@@ -151,12 +151,7 @@ static int rebuild(cbns_batch* b) {
         const int nt = (int)((count + 255) / 256);
         for (int t = 0; t < nt; t++) tab[cls].push_back({bi, t});
       };
-      for (int d = 0; d < 3; d++) {
-        // FP32-flux y and z sweeps take two adjacent faces per work item
-        const long long ni = B.n[0], nj = B.n[1] + (d == 1), nk = B.n[2] + (d == 2);
-        const long long pairs = d == 1 ? ni * ((nj + 1) / 2) * nk : ni * nj * ((nk + 1) / 2);
-        add((I.precision ? T_FACE32 : T_FACE64) + d, I.precision && d > 0 ? pairs : B.sf[d]);
-      }
+      for (int d = 0; d < 3; d++) add((I.precision ? T_FACE32 : T_FACE64) + d, face_items(B, d));
       const int vunits = ((B.n[0] + 2 * NGV + VX - 1) / VX) * ((B.n[1] + 2 * NGV + VY - 1) / VY) *
                          ((B.n[2] + 2 * NGV + VZ - 1) / VZ);
       for (int t = 0; t < vunits; t++)  // whole-workgroup units

@@ -125,15 +125,69 @@ __device__ __forceinline__ void face_body_f32(const BlockDesc& B, int t, int whi
   for (int e = 0; e < 5; e++) F[e * B.sf[D] + fi] = fl[e];
 }
 
-// FP32-flux y and z sweeps: two faces adjacent along D per work item, faces
-// m and m + 1 with m even (m = j for D = 1, k for D = 2), from the seven cells
-// m-3 .. m+3 and the primitives of m-1 .. m+1: 35 state and 15 primitive
-// loads for two faces instead of 60 and 20.  The last face of an odd count
-// (n + 1 faces) is the first of a pair whose second face does not exist;
-// its extra cell (m + 3 <= n + 3) is still inside the ghosted block.
+// y and z sweeps (every form): two faces adjacent along D per work item,
+// faces m and m + 1 with m even (m = j for D = 1, k for D = 2), from the
+// seven cells m-3 .. m+3 and the primitives of m-1 .. m+1: 35 state and 15
+// primitive loads for two faces instead of 60 and 20, and one index decode.
+// The last face of an odd count (n + 1 faces) is the first of a pair whose
+// second face does not exist; its extra cell (m + 3 <= n + 3) is still
+// inside the ghosted block.  Each face's arithmetic is face_body's.
 __device__ __forceinline__ int face_pair_count(const BlockDesc& B, int D) {
   const int ni = B.n[0], nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
   return D == 1 ? ni * ((nj + 1) / 2) * nk : ni * nj * ((nk + 1) / 2);
+}
+
+// Work index t -> (i, j, k) of the first face of a pair along D.
+template <int D>
+__device__ __forceinline__ void pair_decode(const BlockDesc& B, int t, int jb, int& i, int& j, int& k) {
+  const int ni = B.n[0], nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
+  if (D == 1) {
+    band_decode(t, ni, (nj + 1) / 2, nk, jb, i, j, k);
+    j *= 2;
+  } else {
+    band_decode(t, ni, nj, (nk + 1) / 2, jb, i, j, k);
+    k *= 2;
+  }
+}
+
+template <int D, bool FMA = false>
+__device__ __forceinline__ void face_pair_body(const BlockDesc& B, int t, int which, int jb, const Phys& ph) {
+  static_assert(D == 1 || D == 2, "pairs along y or z");
+  const int ni = B.n[0], nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
+  int i, j, k;
+  pair_decode<D>(B, t, jb, i, j, k);
+  const int c0 = gidx(B, i, j, k);
+  const int sD = D == 1 ? B.g[0] : B.g[0] * B.g[1];
+  gptr<const double> __restrict__ Q = gmem<const double>(B.Q[which]);
+  const long long sg = B.sg;
+  double q7[7][5];
+#pragma unroll
+  for (int cc = 0; cc < 7; cc++)
+#pragma unroll
+    for (int e = 0; e < 5; e++) q7[cc][e] = Q[e * sg + c0 + (cc - 3) * sD];
+  gptr<const double> __restrict__ P = gmem<const double>(B.prim);
+  double p[3][5];
+#pragma unroll
+  for (int f = 0; f < 5; f++)
+#pragma unroll
+    for (int s = 0; s < 3; s++) p[s][f] = P[f * sg + c0 + (s - 1) * sD];
+  gptr<double> __restrict__ F = gmem(B.F[D]);
+  const long long sf = B.sf[D];
+  const int sF = D == 1 ? ni : ni * nj;  // face-array stride along D
+  const int fi = i + ni * (j + nj * k);
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    if (h == 1 && (D == 1 ? j + 1 >= nj : k + 1 >= nk)) break;
+    double qq[6][5];
+#pragma unroll
+    for (int cc = 0; cc < 6; cc++)
+#pragma unroll
+      for (int e = 0; e < 5; e++) qq[cc][e] = q7[cc + h][e];
+    double fl[5];
+    face_flux<D, FMA>(qq, p[h], p[h + 1], ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl);
+#pragma unroll
+    for (int e = 0; e < 5; e++) F[e * sf + fi + h * sF] = fl[e];
+  }
 }
 
 template <int D>
@@ -142,13 +196,7 @@ __device__ __forceinline__ void face_pair_body_f32(const BlockDesc& B, int t, in
   static_assert(D == 1 || D == 2, "pairs along y or z");
   const int ni = B.n[0], nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
   int i, j, k;
-  if (D == 1) {
-    band_decode(t, ni, (nj + 1) / 2, nk, jb, i, j, k);
-    j *= 2;
-  } else {
-    band_decode(t, ni, nj, (nk + 1) / 2, jb, i, j, k);
-    k *= 2;
-  }
+  pair_decode<D>(B, t, jb, i, j, k);
   const int c0 = gidx(B, i, j, k);
   const int sD = D == 1 ? B.g[0] : B.g[0] * B.g[1];
   gptr<const double> __restrict__ Q = gmem<const double>(B.Q[which]);

@@ -72,6 +72,14 @@ struct Launch {
 // Arithmetic of the face fluxes (the values of cbns_precision in the C ABI).
 constexpr int PREC_FP64 = 0, PREC_FP32_FLUX = 1, PREC_FP64_FMA = 2;
 
+// Work items of a face sweep of block b: faces along x, and along y and z
+// pairs of faces (face_pair_body) or, with pairs = false, single faces.
+inline long long face_items(const BlockDesc& b, int dir, bool pairs = true) {
+  const long long ni = b.n[0] + (dir == 0), nj = b.n[1] + (dir == 1), nk = b.n[2] + (dir == 2);
+  if (!pairs || dir == 0) return ni * nj * nk;
+  return dir == 1 ? ni * ((nj + 1) / 2) * nk : ni * nj * ((nk + 1) / 2);
+}
+
 // kernel launchers (cbns_kernels.hip; launch_face_flux_fma in cbns_kernels_fma.hip)
 hipError_t launch_face_flux_fma(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
                                 const Phys& ph, int jb, int occ, hipStream_t s);

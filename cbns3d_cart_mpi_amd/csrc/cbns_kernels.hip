@@ -28,12 +28,17 @@ __global__ __launch_bounds__(TPB, OCC) void k_face_flux(const BlockDesc* __restr
   const Tile tl = xcd_tile();
   const BlockDesc& B = blocks[tl.y];
   const int t = tl.x * TPB + threadIdx.x;
-  if (t >= face_count(B, D)) return;
-  face_body<D>(B, t, which, jb, ph);
+  if (D == 0 || OCC >= 3) {  // the pair body does not fit 3 waves' registers
+    if (t >= face_count(B, D)) return;
+    face_body<D>(B, t, which, jb, ph);
+  } else {
+    if (t >= face_pair_count(B, D)) return;
+    face_pair_body<D == 0 ? 1 : D>(B, t, which, jb, ph);
+  }
 }
 
-// FP32-flux form: same stencil, FP32 face arrays holding F - F(Ug); the y
-// and z sweeps take two adjacent faces per work item (face_pair_body_f32).
+// FP32-flux form: same stencil and work items, FP32 face arrays holding
+// F - F(Ug).
 template <int D>
 __global__ __launch_bounds__(TPB) void k_face_flux_f32(const BlockDesc* __restrict__ blocks, int which, int jb,
                                                        Phys ph, RefState g) {
@@ -51,20 +56,12 @@ __global__ __launch_bounds__(TPB) void k_face_flux_f32(const BlockDesc* __restri
 
 hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
                             const Phys& ph, int precision, const RefState& g, int jb, int occ, hipStream_t s) {
-  long long mx = 0;
-  for (int b = 0; b < nb; b++) {
-    long long c = (long long)(hb[b].n[0] + (dir == 0)) * (hb[b].n[1] + (dir == 1)) * (hb[b].n[2] + (dir == 2));
-    mx = c > mx ? c : mx;
-  }
   if (precision == PREC_FP64_FMA) return launch_face_flux_fma(db, hb, nb, dir, which, ph, jb, occ, s);
-  if (precision == PREC_FP32_FLUX && dir > 0) {  // work items are pairs of faces along dir
-    mx = 0;
-    for (int b = 0; b < nb; b++) {
-      const long long ni = hb[b].n[0], nj = hb[b].n[1] + (dir == 1), nk = hb[b].n[2] + (dir == 2);
-      const long long c = dir == 1 ? ni * ((nj + 1) / 2) * nk : ni * nj * ((nk + 1) / 2);
-      mx = c > mx ? c : mx;
-    }
-  }
+  // work items: pairs of faces along y and z, except the FP64 sweeps at 3
+  // waves per SIMD, which take single faces
+  const bool single = precision == PREC_FP64 && occ >= 3;
+  long long mx = 0;
+  for (int b = 0; b < nb; b++) mx = std::max(mx, face_items(hb[b], dir, !single));
   dim3 grid(nblocks_for(mx), nb);
   if (precision == PREC_FP64) {
 #define CBNS_FACE(DIR, OCC) \

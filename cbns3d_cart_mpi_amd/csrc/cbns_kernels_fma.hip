@@ -16,15 +16,20 @@ __global__ __launch_bounds__(TPB, OCC) void k_face_flux_fma(const BlockDesc* __r
   const Tile tl = xcd_tile();
   const BlockDesc& B = blocks[tl.y];
   const int t = tl.x * TPB + threadIdx.x;
-  if (t >= face_count(B, D)) return;
-  face_body<D, true>(B, t, which, jb, ph);
+  if (D == 0 || OCC >= 3) {  // the pair body does not fit 3 waves' registers
+    if (t >= face_count(B, D)) return;
+    face_body<D, true>(B, t, which, jb, ph);
+  } else {
+    if (t >= face_pair_count(B, D)) return;
+    face_pair_body<D == 0 ? 1 : D, true>(B, t, which, jb, ph);
+  }
 }
 
 hipError_t launch_face_flux_fma(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
                                 const Phys& ph, int jb, int occ, hipStream_t s) {
   long long mx = 0;
   for (int b = 0; b < nb; b++) {
-    long long c = (long long)(hb[b].n[0] + (dir == 0)) * (hb[b].n[1] + (dir == 1)) * (hb[b].n[2] + (dir == 2));
+    const long long c = face_items(hb[b], dir, occ < 3);
     mx = c > mx ? c : mx;
   }
   dim3 grid(nblocks_fma(mx), nb);
