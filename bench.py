@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-turb", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=60)
+    ap.add_argument("--batched-copies", type=int, default=2)
     ap.add_argument("--turb-n", type=int, default=128)
     ap.add_argument("--turb-steps", type=int, default=500)
     ap.add_argument("--sweep-steps", type=int, default=300)
@@ -165,6 +168,30 @@ def sweep_leg(steps):
     out["fp32_over_fp64"] = round(out["fp32"]["mcell_steps_per_s"] / out["fp64"]["mcell_steps_per_s"], 3)
     out["fp32_rel_l2_worst_field_per_solve"] = {"max": float(np.max(worst)), "median": float(np.median(worst))}
     out["config"] = f"64 solves x {steps} steps; latency = batch start to all results (one batch)"
+    return out
+
+
+def batched_leg(N, steps, copies, plain_fp64):
+    """The batched form (persistent kernels over the packed block table, one
+    hipGraph per step) on `copies` Taylor-Green N^3 solves, FP64 and
+    FP32-flux: Mcell-steps/s of each against the plain FP64 path's rate."""
+    from cbns3d_cart_mpi_amd import capi as C
+
+    out = {}
+    for name, prec in (("fp64", C.CBNS_FP64), ("fp32", C.CBNS_FP32_FLUX)):
+        with C.Batch(200 << 30, capacity=copies) as b:
+            for _ in range(copies):
+                b.add(C.make_params(N), precision=prec, steps=steps + 1)
+            b.run(1)  # captures the step graph and takes the first step (untimed)
+            b.sync()
+            t0 = time.perf_counter()
+            b.run(steps)
+            b.sync()
+            el = time.perf_counter() - t0
+        v = copies * N ** 3 * steps / el / 1e6
+        out[name] = {"mcell_steps_per_s": round(v, 2), "ms_per_step": round(el / steps * 1e3, 3)}
+    out["fp32_over_plain_fp64"] = round(out["fp32"]["mcell_steps_per_s"] / plain_fp64, 3)
+    out["config"] = f"{copies} Taylor-Green {N}^3 solves in one batch, {steps} timed steps"
     return out
 
 
@@ -315,6 +342,8 @@ def main():
         line["fp64_fma"] = fma
     if fp32:
         line["fp32_flux"] = fp32
+    if not a.no_batched:
+        line["batched"] = batched_leg(N, a.batched_steps, a.batched_copies, value)
     if not a.no_turb:
         line["turbulence"] = turbulence_leg(a.turb_n, a.turb_steps)
     if not a.no_sweep:

@@ -357,7 +357,8 @@ class Batch:
         check(lib().cbns_batch_run(self._h, int(nsteps)), "cbns_batch_run")
 
     def set_workgroups(self, n):
-        """Persistent workgroups of the memory-bound stage kernels."""
+        """Persistent workgroups of the memory-bound stage kernels (0: each
+        kernel's resident workgroups on the device, the default)."""
         check(lib().cbns_batch_set_workgroups(self._h, int(n)), "cbns_batch_set_workgroups")
 
     def sync(self):

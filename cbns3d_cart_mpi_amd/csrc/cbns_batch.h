@@ -40,13 +40,18 @@ struct TileTable {
   int n;
 };
 
-constexpr int BATCH_WG = 1024;  // persistent workgroups per launch
+// Persistent grid of each kernel: its resident workgroups on the device.
+struct BatchGrids {
+  int face[2][3];  // [precision][direction]
+  int visc[2], update[2], ghost, dt;
+};
+hipError_t batch_grids(BatchGrids* g);
 
+// wgs: the launch's persistent workgroups
 hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int dir, int which, int jb,
-                             int precision, hipStream_t s);
+                             int precision, int wgs, hipStream_t s);
 hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, int precision,
-                             hipStream_t s);
-// wgs: persistent workgroups of the memory-bound kernels (update, ghost, dt)
+                             int wgs, hipStream_t s);
 hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int stage, int jb,
                                int precision, int wgs, hipStream_t s);
 hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, int wgs,

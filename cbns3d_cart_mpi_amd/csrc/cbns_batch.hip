@@ -1,30 +1,31 @@
 // Persistent batch kernels: each launch walks a packed tile table that mixes
-// the blocks of many solves.  Workgroup w takes one contiguous chunk of the
-// table, and chunks are assigned so that the workgroups of one XCD (w mod 8
-// under round-robin dispatch) own one contiguous range of it, keeping each
-// solve's stencil neighbours in that XCD's L2.  The per-item arithmetic is the
-// shared bodies', so every solve's values equal a single-solve run's.
+// the blocks of many solves.  The grid is the number of workgroups that can
+// be resident at once (batch_grids), and the table is walked in rounds of
+// one tile per workgroup: in round r the grid takes tiles [r G, (r + 1) G),
+// so the tiles in flight at any time form one contiguous window of the
+// table, as in a one-shot launch, and the stencil neighbours of a tile were
+// touched by the preceding rounds.  Within a round the workgroups of one
+// XCD (w mod 8 under round-robin dispatch) take one contiguous sub-range,
+// which keeps neighbouring tiles in that XCD's L2.  The per-item arithmetic
+// is the shared bodies', so every solve's values equal a single-solve run's.
 #include "cbns_batch.h"
 #include "cbns_bodies.h"
 
 namespace cbns {
 
-__device__ __forceinline__ void tile_chunk(int n, int& first, int& last) {
+// Position of this workgroup within each round (see above).
+__device__ __forceinline__ int round_slot() {
   const int G = gridDim.x;
   const int base = G / NXCD, rem = G % NXCD;
   const int r = blockIdx.x % NXCD, q = blockIdx.x / NXCD;
-  const int L = r * base + (r < rem ? r : rem) + q;
-  first = (int)((long long)n * L / G);
-  last = (int)((long long)n * (L + 1) / G);
+  return r * base + (r < rem ? r : rem) + q;
 }
 
 template <int D, int P>
 __global__ __launch_bounds__(TPB, 2) void kb_face(const BlockDesc* __restrict__ blocks,
                                                   const InstDesc* __restrict__ insts, TileTable tt, int which,
                                                   int jb) {
-  int first, last;
-  tile_chunk(tt.n, first, last);
-  for (int x = first; x < last; x++) {
+  for (int x = round_slot(); x < tt.n; x += gridDim.x) {
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
@@ -45,9 +46,7 @@ __global__ __launch_bounds__(TPB, 2) void kb_face(const BlockDesc* __restrict__ 
 template <typename VT>
 __global__ __launch_bounds__(TPB) void kb_visc(const BlockDesc* __restrict__ blocks,
                                                const InstDesc* __restrict__ insts, TileTable tt, int jb) {
-  int first, last;
-  tile_chunk(tt.n, first, last);
-  for (int x = first; x < last; x++) {
+  for (int x = round_slot(); x < tt.n; x += gridDim.x) {
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
@@ -55,15 +54,16 @@ __global__ __launch_bounds__(TPB) void kb_visc(const BlockDesc* __restrict__ blo
     // one (column tile, z-chunk) unit of the viscous tile march per entry
     const int ntiles = ((B.n[0] + 2 * NGV + VX - 1) / VX) * ((B.n[1] + 2 * NGV + VY - 1) / VY);
     visc_tile_body<VT>(B, I.ph, tr.tile % ntiles, tr.tile / ntiles);
+    // the next unit's first plane refills LDS buffer 0, which the last plane
+    // of this unit read when the unit has an odd number of planes
+    __syncthreads();
   }
 }
 
 template <typename FT>
 __global__ __launch_bounds__(TPB) void kb_update(const BlockDesc* __restrict__ blocks,
                                                  const InstDesc* __restrict__ insts, TileTable tt, int stage, int jb) {
-  int first, last;
-  tile_chunk(tt.n, first, last);
-  for (int x = first; x < last; x++) {
+  for (int x = round_slot(); x < tt.n; x += gridDim.x) {
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
@@ -81,9 +81,7 @@ __global__ __launch_bounds__(TPB) void kb_update(const BlockDesc* __restrict__ b
 
 __global__ __launch_bounds__(TPB) void kb_ghost(const BlockDesc* __restrict__ blocks,
                                                 const InstDesc* __restrict__ insts, TileTable tt, int which) {
-  int first, last;
-  tile_chunk(tt.n, first, last);
-  for (int x = first; x < last; x++) {
+  for (int x = round_slot(); x < tt.n; x += gridDim.x) {
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
@@ -97,15 +95,14 @@ __global__ __launch_bounds__(TPB) void kb_ghost(const BlockDesc* __restrict__ bl
 
 // Per-solve max of the spectral radius: a workgroup keeps a running max
 // for the solve it is on and flushes it (integer atomic max on the bits of a
-// non-negative double) when its chunk moves to another solve or ends.
+// non-negative double) when its next tile belongs to another solve, or at
+// the end.
 __global__ __launch_bounds__(TPB) void kb_dt(const BlockDesc* __restrict__ blocks, InstDesc* insts, TileTable tt) {
-  int first, last;
-  tile_chunk(tt.n, first, last);
   const int lane = threadIdx.x & 63;
   int cur = -1;
   double acc = 0.0;
   int bad = 0;
-  for (int x = first; x < last; x++) {
+  for (int x = round_slot(); x < tt.n; x += gridDim.x) {
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const int inst = B.inst;
@@ -168,9 +165,9 @@ __global__ void kb_set_budget(InstDesc* insts, const int2* pairs, int n) {
 }
 
 hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int dir, int which, int jb,
-                             int precision, hipStream_t s) {
+                             int precision, int wgs, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
-  const dim3 g(BATCH_WG), b(TPB);
+  const dim3 g(wgs), b(TPB);
   if (precision == 0) {
     if (dir == 0) hipLaunchKernelGGL((kb_face<0, 0>), g, b, 0, s, blocks, insts, tt, which, jb);
     else if (dir == 1) hipLaunchKernelGGL((kb_face<1, 0>), g, b, 0, s, blocks, insts, tt, which, jb);
@@ -184,12 +181,12 @@ hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable
 }
 
 hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, int precision,
-                             hipStream_t s) {
+                             int wgs, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
   if (precision == PREC_FP32_FLUX)
-    hipLaunchKernelGGL(kb_visc<float>, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt, jb);
+    hipLaunchKernelGGL(kb_visc<float>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, jb);
   else
-    hipLaunchKernelGGL(kb_visc<double>, dim3(BATCH_WG), dim3(TPB), 0, s, blocks, insts, tt, jb);
+    hipLaunchKernelGGL(kb_visc<double>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, jb);
   return hipGetLastError();
 }
 
@@ -226,6 +223,33 @@ hipError_t batch_launch_finalize(InstDesc* insts, int ninst, hipStream_t s) {
   if (ninst == 0) return hipSuccess;
   hipLaunchKernelGGL(kb_finalize, dim3((ninst + 63) / 64), dim3(64), 0, s, insts, ninst);
   return hipGetLastError();
+}
+
+// Resident workgroups of each persistent kernel on the current device
+// (workgroups per CU at the kernel's register and LDS use, times the CUs).
+hipError_t batch_grids(BatchGrids* g) {
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  auto res = [&](const void* f, int* out) {
+    int per = 0;
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, TPB, 0);
+    *out = (per > 0 ? per : 1) * cus;
+  };
+  res((const void*)kb_face<0, 0>, &g->face[0][0]);
+  res((const void*)kb_face<1, 0>, &g->face[0][1]);
+  res((const void*)kb_face<2, 0>, &g->face[0][2]);
+  res((const void*)kb_face<0, 1>, &g->face[1][0]);
+  res((const void*)kb_face<1, 1>, &g->face[1][1]);
+  res((const void*)kb_face<2, 1>, &g->face[1][2]);
+  res((const void*)kb_visc<double>, &g->visc[0]);
+  res((const void*)kb_visc<float>, &g->visc[1]);
+  res((const void*)kb_update<double>, &g->update[0]);
+  res((const void*)kb_update<float>, &g->update[1]);
+  res((const void*)kb_ghost, &g->ghost);
+  res((const void*)kb_dt, &g->dt);
+  return e;
 }
 
 }  // namespace cbns

@@ -59,7 +59,8 @@ struct cbns_batch {
   hipGraphExec_t exec = nullptr;
   bool dirty = true;
   int jb = 32;
-  int wg_mem = BATCH_WG;  // persistent workgroups of the memory-bound kernels
+  BatchGrids grid{};  // resident workgroups of each persistent kernel
+  int wg_mem = 0;     // override of the memory-bound kernels' grid (0: resident)
   int ninst_hw = 0;  // highest live id + 1
 };
 
@@ -186,22 +187,25 @@ static int rebuild(cbns_batch* b) {
   }
   hipStream_t s = b->stream;
   CK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipError_t e = batch_launch_dt(b->d_blocks, b->d_insts, b->tt[T_DT], b->wg_mem, s);
+  const BatchGrids& G = b->grid;
+  auto mem = [&](int resident) { return b->wg_mem > 0 ? b->wg_mem : resident; };
+  hipError_t e = batch_launch_dt(b->d_blocks, b->d_insts, b->tt[T_DT], mem(G.dt), s);
   if (e == hipSuccess) e = batch_launch_finalize(b->d_insts, hw, s);
   for (int stage = 1; stage <= 3 && e == hipSuccess; stage++) {
     const int in = stage == 1 ? CBNS_Q : CBNS_QP;
     const int out = stage == 3 ? CBNS_Q : CBNS_QP;
-    e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC64], b->jb, 0, s);
-    if (e == hipSuccess) e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC32], b->jb, 1, s);
+    e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC64], b->jb, 0, G.visc[0], s);
+    if (e == hipSuccess) e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC32], b->jb, 1, G.visc[1], s);
     for (int d = 0; d < 3 && e == hipSuccess; d++) {
-      e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE64 + d], d, in, b->jb, 0, s);
-      if (e == hipSuccess) e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE32 + d], d, in, b->jb, 1, s);
+      e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE64 + d], d, in, b->jb, 0, G.face[0][d], s);
+      if (e == hipSuccess)
+        e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE32 + d], d, in, b->jb, 1, G.face[1][d], s);
     }
     if (e == hipSuccess)
-      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD64], stage, b->jb, 0, b->wg_mem, s);
+      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD64], stage, b->jb, 0, mem(G.update[0]), s);
     if (e == hipSuccess)
-      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD32], stage, b->jb, 1, b->wg_mem, s);
-    if (e == hipSuccess) e = batch_launch_ghost(b->d_blocks, b->d_insts, b->tt[T_GHOST], out, b->wg_mem, s);
+      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD32], stage, b->jb, 1, mem(G.update[1]), s);
+    if (e == hipSuccess) e = batch_launch_ghost(b->d_blocks, b->d_insts, b->tt[T_GHOST], out, mem(G.ghost), s);
   }
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(s, &g);
@@ -234,6 +238,7 @@ int cbns_batch_create(uint64_t hbm_budget_bytes, int capacity, cbns_batch** out)
     st = CBNS_ERR_NOMEM;
   if (!st && hipMemset(b->d_insts, 0, sizeof(InstDesc) * capacity) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
+  if (!st && batch_grids(&b->grid) != hipSuccess) st = CBNS_ERR_HIP;
   if (st) {
     cbns_batch_destroy(b);
     return st;
@@ -460,7 +465,7 @@ int cbns_batch_set_budgets(cbns_batch* b, const int* ids, const int* steps, int 
 }
 
 int cbns_batch_set_workgroups(cbns_batch* b, int wgs) {
-  if (!b || wgs < 1 || wgs > 65535) return CBNS_ERR_ARG;
+  if (!b || wgs < 0 || wgs > 65535) return CBNS_ERR_ARG;
   CK(hipStreamSynchronize(b->stream));
   b->wg_mem = wgs;
   b->dirty = true;  // the step graph holds the grid size

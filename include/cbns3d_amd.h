@@ -213,8 +213,9 @@ int cbns_batch_run(cbns_batch* b, int nsteps);
  * large ones few within one run. */
 int cbns_batch_set_budgets(cbns_batch* b, const int* ids, const int* steps, int n);
 /* Persistent workgroups per launch of the memory-bound stage kernels (stage
- * tail, ghost gather, dt; default 1024).  A tuning knob only; results are
- * identical for every value.  Takes effect at the next run. */
+ * tail, ghost gather, dt).  0, the default, uses each kernel's resident
+ * workgroups on the device (as every other batch kernel does).  A tuning knob
+ * only; results are identical for every value.  Takes effect at the next run. */
 int cbns_batch_set_workgroups(cbns_batch* b, int workgroups);
 int cbns_batch_sync(cbns_batch* b);
 int cbns_batch_status(cbns_batch* b, int id, int* active, int* steps_done, double* t_cur, int* nonfinite);

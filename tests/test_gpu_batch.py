@@ -63,6 +63,21 @@ def test_alone_equals_in_mix():
         assert np.array_equal(mix.get_state(x), Qa)
 
 
+@pytest.mark.parametrize("precision", [C.CBNS_FP64, C.CBNS_FP32_FLUX])
+def test_odd_plane_viscous_chunks_equal_single_runs(precision):
+    """z-extents whose viscous march leaves a chunk with an odd number of
+    planes (n2 + 4 = 25: chunks of 16 and 9; 29: 16 and 13), so a persistent
+    workgroup's next unit refills the LDS buffer its last plane read."""
+    cases = [((20, 16, 21), (1, 1, 1), 5), ((18, 22, 25), (1, 2, 1), 4), ((16, 16, 16), (1, 1, 1), 6)]
+    with C.Batch(32 * GB) as b:
+        ids = [b.add(C.make_params(N, Mach=0.3, Re=500.0), dims=d, precision=precision, steps=s) for N, d, s in cases]
+        b.run(6)
+        for (N, d, s), i in zip(cases, ids):
+            Q, t = single(N, s, d, precision, Mach=0.3, Re=500.0)
+            assert b.status(i)["t_cur"] == t
+            assert np.array_equal(b.get_state(i), Q), N
+
+
 def test_admission_by_memory_budget_and_reuse_of_slots():
     p = C.make_params(32)
     need = C.batch_bytes_needed(p)
