@@ -41,22 +41,30 @@ struct TileTable {
 };
 
 // Persistent grid of each kernel: its resident workgroups on the device.
+constexpr int BATCH_CTR_PER_LAUNCH = 8;  // one tile counter per XCD
+constexpr int BATCH_CTR_STRIDE = 64;     // ints between counters: one 256-byte line each
+constexpr int BATCH_COUNTERS = 64 * BATCH_CTR_PER_LAUNCH * BATCH_CTR_STRIDE;  // of one step graph
+
 struct BatchGrids {
   int face[2][3];  // [precision][direction]
   int visc[2], update[2], ghost, dt;
 };
 hipError_t batch_grids(BatchGrids* g);
 
-// wgs: the launch's persistent workgroups
+// wgs: the launch's persistent workgroups; ctr: its NXCD tile counters,
+// zero at launch (batch_launch_reset_counters earlier on the stream)
 hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int dir, int which, int jb,
-                             int precision, int wgs, hipStream_t s);
+                             int precision, int wgs, int* ctr, hipStream_t s);
 hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, int precision,
-                             int wgs, hipStream_t s);
+                             int wgs, int* ctr, hipStream_t s);
 hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int stage, int jb,
-                               int precision, int wgs, hipStream_t s);
+                               int precision, int wgs, int* ctr, hipStream_t s);
 hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, int wgs,
-                              hipStream_t s);
-hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int wgs, hipStream_t s);
+                              int* ctr, hipStream_t s);
+hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int wgs, int* ctr,
+                           hipStream_t s);
+// zeroes n counters with a kernel (the step graph holds no memset nodes)
+hipError_t batch_launch_reset_counters(int* ctr, int n, hipStream_t s);
 hipError_t batch_launch_finalize(InstDesc* insts, int ninst, hipStream_t s);
 hipError_t batch_launch_set_budget(InstDesc* insts, const int2* pairs, int n, hipStream_t s);
 

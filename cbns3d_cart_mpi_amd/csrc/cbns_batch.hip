@@ -1,112 +1,147 @@
 // Persistent batch kernels: each launch walks a packed tile table that mixes
 // the blocks of many solves.  The grid is the number of workgroups that can
-// be resident at once (batch_grids), and the table is walked in rounds of
-// one tile per workgroup: in round r the grid takes tiles [r G, (r + 1) G),
-// so the tiles in flight at any time form one contiguous window of the
-// table, as in a one-shot launch, and the stencil neighbours of a tile were
-// touched by the preceding rounds.  Within a round the workgroups of one
-// XCD (w mod 8 under round-robin dispatch) take one contiguous sub-range,
-// which keeps neighbouring tiles in that XCD's L2.  The per-item arithmetic
-// is the shared bodies', so every solve's values equal a single-solve run's.
+// be resident at once (batch_grids).  Tiles are handed out dynamically, as
+// the hardware dispatcher hands out the workgroups of a one-shot launch:
+// workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2, so XCD r serves the r-th eighth of the table (as the one-shot kernels'
+// xcd_tile does) in table order through its own counter, and a workgroup
+// takes the next tile when it finishes one.  The tiles in flight on an XCD
+// thus form one tight window that slides through its range, and the stencil
+// neighbours of a tile (the rows and planes before it) are still in that
+// XCD's L2.  (A static assignment, tile q + m G to workgroup q, lets slow
+// workgroups fall behind and widens the window past the L2: the z sweep
+// then fetched three times its data.)  The per-item arithmetic is the shared
+// bodies', so every solve's values equal a single-solve run's.
 #include "cbns_batch.h"
 #include "cbns_bodies.h"
 
 namespace cbns {
 
-// Position of this workgroup within each round (see above).
-__device__ __forceinline__ int round_slot() {
-  const int G = gridDim.x;
-  const int base = G / NXCD, rem = G % NXCD;
-  const int r = blockIdx.x % NXCD, q = blockIdx.x / NXCD;
-  return r * base + (r < rem ? r : rem) + q;
+// Runs body(x) for the tiles x this workgroup takes from its XCD's range of
+// [0, n).  ctr: this launch's 8 counters, zero at launch (kb_reset_counters).
+// The next tile's index is fetched (device-scope atomic) while the current
+// one runs.  body must not synchronise the workgroup unless every thread
+// calls it alike.
+static_assert(BATCH_CTR_PER_LAUNCH == NXCD, "one tile counter per XCD");
+
+template <class F>
+__device__ __forceinline__ void for_tiles(int n, int* __restrict__ ctr, F&& body) {
+  const bool split = gridDim.x >= NXCD;  // every XCD has workgroups
+  const int r = split ? (int)(blockIdx.x % NXCD) : 0;
+  const int parts = split ? NXCD : 1;
+  const int lo = (int)((long long)n * r / parts), hi = (int)((long long)n * (r + 1) / parts);
+  __shared__ int s_k;
+  int* c = ctr + r * BATCH_CTR_STRIDE;  // counters sit on separate lines: no contention between XCDs
+  if (threadIdx.x == 0) s_k = atomicAdd(c, 1);
+  __syncthreads();
+  int k = s_k;
+  while (lo + k < hi) {
+    int nxt = 0;
+    if (threadIdx.x == 0) nxt = atomicAdd(c, 1);
+    body(lo + k);
+    __syncthreads();  // all done with this tile (and with s_k)
+    if (threadIdx.x == 0) s_k = nxt;
+    __syncthreads();
+    k = s_k;
+  }
+}
+
+__global__ void kb_reset_counters(int* ctr, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ctr[i] = 0;
 }
 
 template <int D, int P>
 __global__ __launch_bounds__(TPB, 2) void kb_face(const BlockDesc* __restrict__ blocks,
                                                   const InstDesc* __restrict__ insts, TileTable tt, int which,
-                                                  int jb) {
-  for (int x = round_slot(); x < tt.n; x += gridDim.x) {
+                                                  int jb, int* ctr) {
+  for_tiles(tt.n, ctr, [&](int x) {
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
-    if (!I.stepping) continue;
+    if (!I.stepping) return;
     const int t = tr.tile * TPB + threadIdx.x;
     if (D > 0) {  // y, z: pairs of faces (the table counts pairs)
-      if (t >= face_pair_count(B, D)) continue;
+      if (t >= face_pair_count(B, D)) return;
       if (P == 0) face_pair_body<D == 0 ? 1 : D>(B, t, which, jb, I.ph);
       else face_pair_body_f32<D == 0 ? 1 : D>(B, t, which, jb, I.ph, I.ref);
-      continue;
+      return;
     }
-    if (t >= face_count(B, D)) continue;
+    if (t >= face_count(B, D)) return;
     if (P == 0) face_body<D>(B, t, which, jb, I.ph);
     else face_body_f32<D>(B, t, which, jb, I.ph, I.ref);
-  }
+  });
 }
 
 template <typename VT>
 __global__ __launch_bounds__(TPB) void kb_visc(const BlockDesc* __restrict__ blocks,
-                                               const InstDesc* __restrict__ insts, TileTable tt, int jb) {
-  for (int x = round_slot(); x < tt.n; x += gridDim.x) {
+                                               const InstDesc* __restrict__ insts, TileTable tt, int jb,
+                                               int* ctr) {
+  // visc_tile_body synchronises the workgroup: every condition before it is
+  // uniform over the workgroup (one tile, one block, one solve); for_tiles'
+  // barrier after each unit also protects the LDS planes that the next unit
+  // refills
+  for_tiles(tt.n, ctr, [&](int x) {
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
-    if (!I.stepping) continue;
+    if (!I.stepping) return;
     // one (column tile, z-chunk) unit of the viscous tile march per entry
     const int ntiles = ((B.n[0] + 2 * NGV + VX - 1) / VX) * ((B.n[1] + 2 * NGV + VY - 1) / VY);
     visc_tile_body<VT>(B, I.ph, tr.tile % ntiles, tr.tile / ntiles);
-    // the next unit's first plane refills LDS buffer 0, which the last plane
-    // of this unit read when the unit has an odd number of planes
-    __syncthreads();
-  }
+  });
 }
 
 template <typename FT>
 __global__ __launch_bounds__(TPB) void kb_update(const BlockDesc* __restrict__ blocks,
-                                                 const InstDesc* __restrict__ insts, TileTable tt, int stage, int jb) {
-  for (int x = round_slot(); x < tt.n; x += gridDim.x) {
+                                                 const InstDesc* __restrict__ insts, TileTable tt, int stage, int jb,
+                                                 int* ctr) {
+  for_tiles(tt.n, ctr, [&](int x) {
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
-    if (!I.stepping) continue;
+    if (!I.stepping) return;
     const int t = tr.tile * TPB + threadIdx.x;
     if (B.n[0] % 2 == 0) {  // x-pairs of cells (the table holds pair tiles for such blocks)
-      if (t >= (B.n[0] / 2) * B.n[1] * B.n[2]) continue;
+      if (t >= (B.n[0] / 2) * B.n[1] * B.n[2]) return;
       update_pair_body<FT>(B, t, I.dt, stage, I.ph, jb);
     } else {
-      if (t >= B.n[0] * B.n[1] * B.n[2]) continue;
+      if (t >= B.n[0] * B.n[1] * B.n[2]) return;
       update_body<FT>(B, t, I.dt, stage, I.ph, jb);
     }
-  }
+  });
 }
 
 __global__ __launch_bounds__(TPB) void kb_ghost(const BlockDesc* __restrict__ blocks,
-                                                const InstDesc* __restrict__ insts, TileTable tt, int which) {
-  for (int x = round_slot(); x < tt.n; x += gridDim.x) {
+                                                const InstDesc* __restrict__ insts, TileTable tt, int which,
+                                                int* ctr) {
+  for_tiles(tt.n, ctr, [&](int x) {
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
-    if (!I.stepping) continue;
+    if (!I.stepping) return;
     const int g = tr.tile * TPB + threadIdx.x;
-    if (g >= B.sg - B.sn) continue;
+    if (g >= B.sg - B.sn) return;
     if (I.precision == PREC_FP32_FLUX) ghost_body<float>(B, blocks, I.dom, I.block0, ghost_cell(B, g), which, true);
     else ghost_body<double>(B, blocks, I.dom, I.block0, ghost_cell(B, g), which, true);
-  }
+  });
 }
 
 // Per-solve max of the spectral radius: a workgroup keeps a running max
 // for the solve it is on and flushes it (integer atomic max on the bits of a
 // non-negative double) when its next tile belongs to another solve, or at
 // the end.
-__global__ __launch_bounds__(TPB) void kb_dt(const BlockDesc* __restrict__ blocks, InstDesc* insts, TileTable tt) {
+__global__ __launch_bounds__(TPB) void kb_dt(const BlockDesc* __restrict__ blocks, InstDesc* insts, TileTable tt,
+                                             int* ctr) {
   const int lane = threadIdx.x & 63;
   int cur = -1;
   double acc = 0.0;
   int bad = 0;
-  for (int x = round_slot(); x < tt.n; x += gridDim.x) {
+  for_tiles(tt.n, ctr, [&](int x) {
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const int inst = B.inst;
-    if (!insts[inst].active || insts[inst].budget <= 0) continue;
+    if (!insts[inst].active || insts[inst].budget <= 0) return;
     if (inst != cur) {
       if (cur >= 0) {
         const double m = wave_max(acc);
@@ -123,7 +158,7 @@ __global__ __launch_bounds__(TPB) void kb_dt(const BlockDesc* __restrict__ block
                                                                : dt_cell<double>(B, t, insts[inst].ph, &bad);
       acc = (acc < s) ? s : acc;  // src/Block3d.cpp:546: a NaN never wins
     }
-  }
+  });
   if (cur >= 0) {
     const double m = wave_max(acc);
     if (lane == 0) atomicMax(&insts[cur].smax, (unsigned long long)__double_as_longlong(m));
@@ -165,51 +200,58 @@ __global__ void kb_set_budget(InstDesc* insts, const int2* pairs, int n) {
 }
 
 hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int dir, int which, int jb,
-                             int precision, int wgs, hipStream_t s) {
+                             int precision, int wgs, int* ctr, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
   const dim3 g(wgs), b(TPB);
   if (precision == 0) {
-    if (dir == 0) hipLaunchKernelGGL((kb_face<0, 0>), g, b, 0, s, blocks, insts, tt, which, jb);
-    else if (dir == 1) hipLaunchKernelGGL((kb_face<1, 0>), g, b, 0, s, blocks, insts, tt, which, jb);
-    else hipLaunchKernelGGL((kb_face<2, 0>), g, b, 0, s, blocks, insts, tt, which, jb);
+    if (dir == 0) hipLaunchKernelGGL((kb_face<0, 0>), g, b, 0, s, blocks, insts, tt, which, jb, ctr);
+    else if (dir == 1) hipLaunchKernelGGL((kb_face<1, 0>), g, b, 0, s, blocks, insts, tt, which, jb, ctr);
+    else hipLaunchKernelGGL((kb_face<2, 0>), g, b, 0, s, blocks, insts, tt, which, jb, ctr);
   } else {
-    if (dir == 0) hipLaunchKernelGGL((kb_face<0, 1>), g, b, 0, s, blocks, insts, tt, which, jb);
-    else if (dir == 1) hipLaunchKernelGGL((kb_face<1, 1>), g, b, 0, s, blocks, insts, tt, which, jb);
-    else hipLaunchKernelGGL((kb_face<2, 1>), g, b, 0, s, blocks, insts, tt, which, jb);
+    if (dir == 0) hipLaunchKernelGGL((kb_face<0, 1>), g, b, 0, s, blocks, insts, tt, which, jb, ctr);
+    else if (dir == 1) hipLaunchKernelGGL((kb_face<1, 1>), g, b, 0, s, blocks, insts, tt, which, jb, ctr);
+    else hipLaunchKernelGGL((kb_face<2, 1>), g, b, 0, s, blocks, insts, tt, which, jb, ctr);
   }
   return hipGetLastError();
 }
 
 hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, int precision,
-                             int wgs, hipStream_t s) {
+                             int wgs, int* ctr, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
   if (precision == PREC_FP32_FLUX)
-    hipLaunchKernelGGL(kb_visc<float>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, jb);
+    hipLaunchKernelGGL(kb_visc<float>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, jb, ctr);
   else
-    hipLaunchKernelGGL(kb_visc<double>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, jb);
+    hipLaunchKernelGGL(kb_visc<double>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, jb, ctr);
   return hipGetLastError();
 }
 
 hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int stage, int jb,
-                               int precision, int wgs, hipStream_t s) {
+                               int precision, int wgs, int* ctr, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
   if (precision == 0)
-    hipLaunchKernelGGL(kb_update<double>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, stage, jb);
+    hipLaunchKernelGGL(kb_update<double>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, stage, jb, ctr);
   else
-    hipLaunchKernelGGL(kb_update<float>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, stage, jb);
+    hipLaunchKernelGGL(kb_update<float>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, stage, jb, ctr);
   return hipGetLastError();
 }
 
 hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, int wgs,
-                              hipStream_t s) {
+                              int* ctr, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_ghost, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, which);
+  hipLaunchKernelGGL(kb_ghost, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, which, ctr);
   return hipGetLastError();
 }
 
-hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int wgs, hipStream_t s) {
+hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int wgs, int* ctr,
+                           hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_dt, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt);
+  hipLaunchKernelGGL(kb_dt, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, ctr);
+  return hipGetLastError();
+}
+
+hipError_t batch_launch_reset_counters(int* ctr, int n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_reset_counters, dim3((n + 255) / 256), dim3(256), 0, s, ctr, n);
   return hipGetLastError();
 }
 

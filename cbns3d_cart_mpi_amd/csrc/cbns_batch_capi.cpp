@@ -60,6 +60,7 @@ struct cbns_batch {
   bool dirty = true;
   int jb = 32;
   BatchGrids grid{};  // resident workgroups of each persistent kernel
+  int* d_ctr = nullptr;  // tile counters of the step graph's launches
   int wg_mem = 0;     // override of the memory-bound kernels' grid (0: resident)
   int ninst_hw = 0;  // highest live id + 1
 };
@@ -189,24 +190,36 @@ static int rebuild(cbns_batch* b) {
   CK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   const BatchGrids& G = b->grid;
   auto mem = [&](int resident) { return b->wg_mem > 0 ? b->wg_mem : resident; };
-  hipError_t e = batch_launch_dt(b->d_blocks, b->d_insts, b->tt[T_DT], mem(G.dt), s);
+  // every persistent launch of the step takes its own block of counters, all
+  // zeroed by the first node of the graph
+  constexpr int LAUNCHES = 1 + 3 * 11;  // dt; per stage 2 viscous, 6 face, 2 update, 1 ghost
+  constexpr int K = BATCH_CTR_PER_LAUNCH * BATCH_CTR_STRIDE;
+  static_assert(LAUNCHES * K <= BATCH_COUNTERS, "counter block");
+  int used = 0;
+  auto ctr = [&]() { return b->d_ctr + K * used++; };
+  hipError_t e = batch_launch_reset_counters(b->d_ctr, LAUNCHES * K, s);
+  if (e == hipSuccess) e = batch_launch_dt(b->d_blocks, b->d_insts, b->tt[T_DT], mem(G.dt), ctr(), s);
   if (e == hipSuccess) e = batch_launch_finalize(b->d_insts, hw, s);
   for (int stage = 1; stage <= 3 && e == hipSuccess; stage++) {
     const int in = stage == 1 ? CBNS_Q : CBNS_QP;
     const int out = stage == 3 ? CBNS_Q : CBNS_QP;
-    e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC64], b->jb, 0, G.visc[0], s);
-    if (e == hipSuccess) e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC32], b->jb, 1, G.visc[1], s);
+    e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC64], b->jb, 0, G.visc[0], ctr(), s);
+    if (e == hipSuccess)
+      e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC32], b->jb, 1, G.visc[1], ctr(), s);
     for (int d = 0; d < 3 && e == hipSuccess; d++) {
-      e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE64 + d], d, in, b->jb, 0, G.face[0][d], s);
+      e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE64 + d], d, in, b->jb, 0, G.face[0][d], ctr(), s);
       if (e == hipSuccess)
-        e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE32 + d], d, in, b->jb, 1, G.face[1][d], s);
+        e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE32 + d], d, in, b->jb, 1, G.face[1][d], ctr(),
+                              s);
     }
     if (e == hipSuccess)
-      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD64], stage, b->jb, 0, mem(G.update[0]), s);
+      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD64], stage, b->jb, 0, mem(G.update[0]), ctr(), s);
     if (e == hipSuccess)
-      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD32], stage, b->jb, 1, mem(G.update[1]), s);
-    if (e == hipSuccess) e = batch_launch_ghost(b->d_blocks, b->d_insts, b->tt[T_GHOST], out, mem(G.ghost), s);
+      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD32], stage, b->jb, 1, mem(G.update[1]), ctr(), s);
+    if (e == hipSuccess)
+      e = batch_launch_ghost(b->d_blocks, b->d_insts, b->tt[T_GHOST], out, mem(G.ghost), ctr(), s);
   }
+  if (used > LAUNCHES) e = hipErrorInvalidValue;  // a launch without a zeroed counter block
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(s, &g);
   if (e != hipSuccess || e2 != hipSuccess) {
@@ -234,7 +247,8 @@ int cbns_batch_create(uint64_t hbm_budget_bytes, int capacity, cbns_batch** out)
   b->inst.resize(capacity);
   int st = CBNS_OK;
   if (hipMalloc(&b->d_insts, sizeof(InstDesc) * capacity) != hipSuccess ||
-      hipMalloc(&b->d_part, sizeof(double) * (2 * GSUM_WG + 2)) != hipSuccess)
+      hipMalloc(&b->d_part, sizeof(double) * (2 * GSUM_WG + 2)) != hipSuccess ||
+      hipMalloc(&b->d_ctr, sizeof(int) * BATCH_COUNTERS) != hipSuccess)
     st = CBNS_ERR_NOMEM;
   if (!st && hipMemset(b->d_insts, 0, sizeof(InstDesc) * capacity) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
@@ -258,6 +272,7 @@ int cbns_batch_destroy(cbns_batch* b) {
   if (b->d_blocks) (void)hipFree(b->d_blocks);
   if (b->d_insts) (void)hipFree(b->d_insts);
   if (b->d_part) (void)hipFree(b->d_part);
+  if (b->d_ctr) (void)hipFree(b->d_ctr);
   if (b->d_pairs) (void)hipFree(b->d_pairs);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
