@@ -20,6 +20,8 @@ struct InstDesc {
   int steps_done, steps_target;
   int budget;        // steps it may still take before the host grants more
   int stepping;      // whether the current step advances it (set by finalize)
+  int pairs;         // every block has an even x-width (stage tail over x-pairs)
+  int s_fused;       // smax was formed by the last stage-3 update (kb_dt skips it)
   double t_cur, t_end, dt;
   unsigned long long smax;  // dt reduction target, reset after each use
 };

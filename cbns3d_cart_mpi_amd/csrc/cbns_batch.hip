@@ -92,10 +92,26 @@ __global__ __launch_bounds__(TPB) void kb_visc(const BlockDesc* __restrict__ blo
   });
 }
 
+// Flush of a workgroup's running spectral-radius max for solve inst (an
+// integer atomic max on the bits of a non-negative double, per wave).
+__device__ __forceinline__ void flush_smax(InstDesc* insts, int inst, double acc, int bad) {
+  const double m = wave_max(acc);
+  if ((threadIdx.x & 63) == 0) atomicMax(&insts[inst].smax, (unsigned long long)__double_as_longlong(m));
+  if (bad) atomicOr(&insts[inst].nonfinite, 1);
+}
+
+// At stage 3 the pair form also reduces the next step's calc_dt spectral
+// radius of every updated cell into its solve's smax (as the single-solve
+// stage tail does), so kb_dt skips such solves on the next step.  The
+// descriptors are read through the read-only insts (so that their uniform
+// fields come through the scalar cache) and only smax and nonfinite, which
+// no kernel of the stage reads, are written through insts_rw.
 template <typename FT>
 __global__ __launch_bounds__(TPB) void kb_update(const BlockDesc* __restrict__ blocks,
-                                                 const InstDesc* __restrict__ insts, TileTable tt, int stage, int jb,
-                                                 int* ctr) {
+                                                 const InstDesc* __restrict__ insts, InstDesc* insts_rw,
+                                                 TileTable tt, int stage, int jb, int* ctr) {
+  int cur = -1, bad = 0;
+  double acc = 0.0;
   for_tiles(tt.n, ctr, [&](int x) {
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
@@ -103,13 +119,21 @@ __global__ __launch_bounds__(TPB) void kb_update(const BlockDesc* __restrict__ b
     if (!I.stepping) return;
     const int t = tr.tile * TPB + threadIdx.x;
     if (B.n[0] % 2 == 0) {  // x-pairs of cells (the table holds pair tiles for such blocks)
+      const bool want_s = stage == 3 && I.pairs;
+      if (want_s && B.inst != cur) {  // uniform: one block per tile
+        if (cur >= 0) flush_smax(insts_rw, cur, acc, bad);
+        cur = B.inst;
+        acc = 0.0;
+        bad = 0;
+      }
       if (t >= (B.n[0] / 2) * B.n[1] * B.n[2]) return;
-      update_pair_body<FT>(B, t, I.dt, stage, I.ph, jb);
+      update_pair_body<FT>(B, t, I.dt, stage, I.ph, jb, want_s, &acc, &bad);
     } else {
       if (t >= B.n[0] * B.n[1] * B.n[2]) return;
       update_body<FT>(B, t, I.dt, stage, I.ph, jb);
     }
   });
+  if (cur >= 0) flush_smax(insts_rw, cur, acc, bad);
 }
 
 __global__ __launch_bounds__(TPB) void kb_ghost(const BlockDesc* __restrict__ blocks,
@@ -131,9 +155,9 @@ __global__ __launch_bounds__(TPB) void kb_ghost(const BlockDesc* __restrict__ bl
 // for the solve it is on and flushes it (integer atomic max on the bits of a
 // non-negative double) when its next tile belongs to another solve, or at
 // the end.
-__global__ __launch_bounds__(TPB) void kb_dt(const BlockDesc* __restrict__ blocks, InstDesc* insts, TileTable tt,
+__global__ __launch_bounds__(TPB) void kb_dt(const BlockDesc* __restrict__ blocks,
+                                             const InstDesc* __restrict__ insts, InstDesc* insts_rw, TileTable tt,
                                              int* ctr) {
-  const int lane = threadIdx.x & 63;
   int cur = -1;
   double acc = 0.0;
   int bad = 0;
@@ -141,13 +165,9 @@ __global__ __launch_bounds__(TPB) void kb_dt(const BlockDesc* __restrict__ block
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
     const int inst = B.inst;
-    if (!insts[inst].active || insts[inst].budget <= 0) return;
+    if (!insts[inst].active || insts[inst].budget <= 0 || insts[inst].s_fused) return;
     if (inst != cur) {
-      if (cur >= 0) {
-        const double m = wave_max(acc);
-        if (lane == 0) atomicMax(&insts[cur].smax, (unsigned long long)__double_as_longlong(m));
-        if (bad) atomicOr(&insts[cur].nonfinite, 1);
-      }
+      if (cur >= 0) flush_smax(insts_rw, cur, acc, bad);
       cur = inst;
       acc = 0.0;
       bad = 0;
@@ -159,11 +179,7 @@ __global__ __launch_bounds__(TPB) void kb_dt(const BlockDesc* __restrict__ block
       acc = (acc < s) ? s : acc;  // src/Block3d.cpp:546: a NaN never wins
     }
   });
-  if (cur >= 0) {
-    const double m = wave_max(acc);
-    if (lane == 0) atomicMax(&insts[cur].smax, (unsigned long long)__double_as_longlong(m));
-    if (bad) atomicOr(&insts[cur].nonfinite, 1);
-  }
+  if (cur >= 0) flush_smax(insts_rw, cur, acc, bad);
 }
 
 // The driver's per-step bookkeeping on the device (src/cbns3d.cpp:178-194):
@@ -174,6 +190,7 @@ __global__ void kb_finalize(InstDesc* insts, int n) {
   if (i >= n) return;
   InstDesc& I = insts[i];
   I.stepping = 0;
+  I.s_fused = 0;
   if (!I.active) return;
   const unsigned long long bits = I.smax;
   I.smax = 0ull;
@@ -191,6 +208,7 @@ __global__ void kb_finalize(InstDesc* insts, int n) {
   I.steps_done += 1;
   I.budget -= 1;
   I.stepping = 1;
+  I.s_fused = I.pairs;  // this step's stage 3 forms the next step's smax
 }
 
 // Step budgets granted by the host: pairs of (solve id, steps).
@@ -229,9 +247,9 @@ hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTab
                                int precision, int wgs, int* ctr, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
   if (precision == 0)
-    hipLaunchKernelGGL(kb_update<double>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, stage, jb, ctr);
+    hipLaunchKernelGGL(kb_update<double>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, insts, tt, stage, jb, ctr);
   else
-    hipLaunchKernelGGL(kb_update<float>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, stage, jb, ctr);
+    hipLaunchKernelGGL(kb_update<float>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, insts, tt, stage, jb, ctr);
   return hipGetLastError();
 }
 
@@ -245,7 +263,7 @@ hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTabl
 hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int wgs, int* ctr,
                            hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_dt, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, ctr);
+  hipLaunchKernelGGL(kb_dt, dim3(wgs), dim3(TPB), 0, s, blocks, insts, insts, tt, ctr);
   return hipGetLastError();
 }
 

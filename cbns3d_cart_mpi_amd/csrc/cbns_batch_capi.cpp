@@ -425,6 +425,8 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
   D.dom = I.dom;
   D.nblk = nb;
   D.precision = precision;
+  D.pairs = 1;
+  for (const BlockDesc& B : I.blocks) D.pairs &= B.n[0] % 2 == 0;
   D.active = 1;
   D.steps_target = steps;
   D.budget = 0x7fffffff;  // unbounded until the host sets budgets
