@@ -86,6 +86,7 @@ _SIGS = {
     "cbns_batch_set_budgets": (_I, [_VP, _PI, _PI, _I]),
     "cbns_batch_sync": (_I, [_VP]),
     "cbns_batch_set_workgroups": (_I, [_VP, _I]),
+    "cbns_batch_set_overlap": (_I, [_VP, _I]),
     "cbns_batch_status": (_I, [_VP, _I, _PI, _PI, _PD, _PI]),
     "cbns_batch_poll": (_I, [_VP, _PI, _I, _PI, _PI]),
     "cbns_batch_get_state": (_I, [_VP, _I, _PD]),
@@ -355,6 +356,11 @@ class Batch:
 
     def run(self, nsteps):
         check(lib().cbns_batch_run(self._h, int(nsteps)), "cbns_batch_run")
+
+    def set_overlap(self, viscous_workgroups):
+        """0: viscous kernels in line; n > 0: beside the face sweeps on a side
+        stream with n persistent workgroups."""
+        check(lib().cbns_batch_set_overlap(self._h, int(viscous_workgroups)), "cbns_batch_set_overlap")
 
     def set_workgroups(self, n):
         """Persistent workgroups of the memory-bound stage kernels (0: each

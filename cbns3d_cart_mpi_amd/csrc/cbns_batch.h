@@ -50,6 +50,7 @@ constexpr int BATCH_COUNTERS = 64 * BATCH_CTR_PER_LAUNCH * BATCH_CTR_STRIDE;  //
 struct BatchGrids {
   int face[2][3];  // [precision][direction]
   int visc[2], update[2], ghost, dt;
+  int cus;  // compute units of the device
 };
 hipError_t batch_grids(BatchGrids* g);
 

@@ -292,6 +292,7 @@ hipError_t batch_grids(BatchGrids* g) {
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
+  g->cus = cus;
   auto res = [&](const void* f, int* out) {
     int per = 0;
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, TPB, 0);

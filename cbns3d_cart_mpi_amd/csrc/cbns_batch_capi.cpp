@@ -55,6 +55,10 @@ struct cbns_batch {
   int2* d_pairs = nullptr;   // staging of step budgets
   int pairs_cap = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // viscous kernels beside the face sweeps (overlap)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int visc_side_wgs = 0;       // 0: viscous kernels in line; > 0: on the side stream with this grid
+                               // (default: one workgroup per CU, the fastest measured)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool dirty = true;
@@ -203,15 +207,26 @@ static int rebuild(cbns_batch* b) {
   for (int stage = 1; stage <= 3 && e == hipSuccess; stage++) {
     const int in = stage == 1 ? CBNS_Q : CBNS_QP;
     const int out = stage == 3 ? CBNS_Q : CBNS_QP;
-    e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC64], b->jb, 0, G.visc[0], ctr(), s);
+    const bool ov = b->visc_side_wgs > 0;
+    hipStream_t sv = ov ? b->side : s;
+    if (ov) {  // fork: the viscous kernels run beside the face sweeps
+      e = hipEventRecord(b->ev_fork, s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(sv, b->ev_fork, 0);
+    }
     if (e == hipSuccess)
-      e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC32], b->jb, 1, G.visc[1], ctr(), s);
+      e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC64], b->jb, 0, ov ? b->visc_side_wgs : G.visc[0],
+                            ctr(), sv);
+    if (e == hipSuccess)
+      e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC32], b->jb, 1, ov ? b->visc_side_wgs : G.visc[1],
+                            ctr(), sv);
+    if (ov && e == hipSuccess) e = hipEventRecord(b->ev_join, sv);
     for (int d = 0; d < 3 && e == hipSuccess; d++) {
       e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE64 + d], d, in, b->jb, 0, G.face[0][d], ctr(), s);
       if (e == hipSuccess)
         e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE32 + d], d, in, b->jb, 1, G.face[1][d], ctr(),
                               s);
     }
+    if (ov && e == hipSuccess) e = hipStreamWaitEvent(s, b->ev_join, 0);  // join before the stage tail
     if (e == hipSuccess)
       e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD64], stage, b->jb, 0, mem(G.update[0]), ctr(), s);
     if (e == hipSuccess)
@@ -253,6 +268,11 @@ int cbns_batch_create(uint64_t hbm_budget_bytes, int capacity, cbns_batch** out)
   if (!st && hipMemset(b->d_insts, 0, sizeof(InstDesc) * capacity) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && batch_grids(&b->grid) != hipSuccess) st = CBNS_ERR_HIP;
+  if (!st) b->visc_side_wgs = b->grid.cus;
+  if (!st && (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+              hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess))
+    st = CBNS_ERR_HIP;
   if (st) {
     cbns_batch_destroy(b);
     return st;
@@ -273,6 +293,9 @@ int cbns_batch_destroy(cbns_batch* b) {
   if (b->d_insts) (void)hipFree(b->d_insts);
   if (b->d_part) (void)hipFree(b->d_part);
   if (b->d_ctr) (void)hipFree(b->d_ctr);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+  if (b->side) (void)hipStreamDestroy(b->side);
   if (b->d_pairs) (void)hipFree(b->d_pairs);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -486,6 +509,14 @@ int cbns_batch_set_workgroups(cbns_batch* b, int wgs) {
   CK(hipStreamSynchronize(b->stream));
   b->wg_mem = wgs;
   b->dirty = true;  // the step graph holds the grid size
+  return CBNS_OK;
+}
+
+int cbns_batch_set_overlap(cbns_batch* b, int viscous_workgroups) {
+  if (!b || viscous_workgroups < 0 || viscous_workgroups > 65535) return CBNS_ERR_ARG;
+  CK(hipStreamSynchronize(b->stream));
+  b->visc_side_wgs = viscous_workgroups;
+  b->dirty = true;  // the step graph holds the schedule
   return CBNS_OK;
 }
 

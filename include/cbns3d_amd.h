@@ -217,6 +217,13 @@ int cbns_batch_set_budgets(cbns_batch* b, const int* ids, const int* steps, int 
  * workgroups on the device (as every other batch kernel does).  A tuning knob
  * only; results are identical for every value.  Takes effect at the next run. */
 int cbns_batch_set_workgroups(cbns_batch* b, int workgroups);
+/* Stage schedule of the step graph: 0 runs the viscous-flux kernels before
+ * the face sweeps; n > 0 runs them beside the sweeps on a second stream with
+ * n persistent workgroups (joined before the stage tail).  The default is one
+ * workgroup per compute unit.  A tuning knob only; results are identical for
+ * every value.  Takes effect at the next run.  No counterpart in the
+ * reference. */
+int cbns_batch_set_overlap(cbns_batch* b, int viscous_workgroups);
 int cbns_batch_sync(cbns_batch* b);
 int cbns_batch_status(cbns_batch* b, int id, int* active, int* steps_done, double* t_cur, int* nonfinite);
 /* Ids of live solves that are done (took their steps, reached t_end or failed);

@@ -78,6 +78,26 @@ def test_odd_plane_viscous_chunks_equal_single_runs(precision):
             assert np.array_equal(b.get_state(i), Q), N
 
 
+def test_viscous_overlap_and_grids_change_nothing():
+    """The step graph's schedule (viscous kernels in line or beside the face
+    sweeps, with any grid) and the memory-bound kernels' grid leave every
+    solve's values unchanged."""
+    cases = [((24, 20, 16), (1, 1, 1), C.CBNS_FP64), ((20, 16, 21), (2, 1, 1), C.CBNS_FP32_FLUX),
+             ((32, 32, 32), (1, 1, 1), C.CBNS_FP32_FLUX)]
+    out = []
+    for ov, wg in ((0, 0), (64, 0), (None, 96), (300, 5)):
+        with C.Batch(32 * GB) as b:
+            if ov is not None:
+                b.set_overlap(ov)
+            b.set_workgroups(wg)
+            ids = [b.add(C.make_params(N, Mach=0.3, Re=700.0), dims=d, precision=p, steps=5) for N, d, p in cases]
+            b.run(5)
+            out.append([(b.status(i)["t_cur"], b.get_state(i)) for i in ids])
+    for other in out[1:]:
+        for (t0, q0), (t1, q1) in zip(out[0], other):
+            assert t0 == t1 and np.array_equal(q0, q1)
+
+
 def test_admission_by_memory_budget_and_reuse_of_slots():
     p = C.make_params(32)
     need = C.batch_bytes_needed(p)
