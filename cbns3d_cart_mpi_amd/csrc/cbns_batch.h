@@ -31,22 +31,18 @@ struct TileRef {
   int block, tile;
 };
 
-// Work classes of a step, each with its own packed tile table.
-enum BatchWork {
-  BW_FACE0 = 0, BW_FACE1, BW_FACE2,  // per precision: see BatchTables
-  BW_VISC, BW_UPDATE, BW_GHOST, BW_DT, BW_COUNT
-};
-
 struct TileTable {
   const TileRef* tiles;
   int n;
 };
 
-// Persistent grid of each kernel: its resident workgroups on the device.
+// Dynamic tile queues: each persistent launch of a step graph owns one
+// counter per XCD, each on its own line.
 constexpr int BATCH_CTR_PER_LAUNCH = 8;  // one tile counter per XCD
 constexpr int BATCH_CTR_STRIDE = 64;     // ints between counters: one 256-byte line each
 constexpr int BATCH_COUNTERS = 64 * BATCH_CTR_PER_LAUNCH * BATCH_CTR_STRIDE;  // of one step graph
 
+// Persistent grid of each kernel: its resident workgroups on the device.
 struct BatchGrids {
   int face[2][3];  // [precision][direction]
   int visc[2], update[2], ghost, dt;
