@@ -173,23 +173,30 @@ def sweep_leg(steps):
 
 def batched_leg(N, steps, copies, plain_fp64):
     """The batched form (persistent kernels over the packed block table, one
-    hipGraph per step) on `copies` Taylor-Green N^3 solves, FP64 and
-    FP32-flux: Mcell-steps/s of each against the plain FP64 path's rate."""
+    hipGraph per step, band and viscous schedule from the batch autotuner) on
+    `copies` Taylor-Green N^3 solves, FP64 and FP32-flux: Mcell-steps/s of
+    each against the plain FP64 path's rate."""
+    from cbns3d_cart_mpi_amd import autotune
     from cbns3d_cart_mpi_amd import capi as C
 
+    tune_steps = 2
+    tuning = (tune_steps + 1) * len(autotune.BATCH_BANDS) * len(autotune.BATCH_OVERLAPS)
     out = {}
     for name, prec in (("fp64", C.CBNS_FP64), ("fp32", C.CBNS_FP32_FLUX)):
         with C.Batch(200 << 30, capacity=copies) as b:
             for _ in range(copies):
-                b.add(C.make_params(N), precision=prec, steps=steps + 1)
+                b.add(C.make_params(N), precision=prec, steps=2 + tuning + steps)
             b.run(1)  # captures the step graph and takes the first step (untimed)
+            best, _ = autotune.tune_batch(b, steps=tune_steps)  # ordinary steps of the solves, untimed
+            b.run(1)  # re-capture on the chosen schedule (untimed)
             b.sync()
             t0 = time.perf_counter()
             b.run(steps)
             b.sync()
             el = time.perf_counter() - t0
         v = copies * N ** 3 * steps / el / 1e6
-        out[name] = {"mcell_steps_per_s": round(v, 2), "ms_per_step": round(el / steps * 1e3, 3)}
+        out[name] = {"mcell_steps_per_s": round(v, 2), "ms_per_step": round(el / steps * 1e3, 3),
+                     "autotune": {"band": best[0], "viscous_overlap_workgroups": best[1]}}
     out["fp32_over_plain_fp64"] = round(out["fp32"]["mcell_steps_per_s"] / plain_fp64, 3)
     out["config"] = f"{copies} Taylor-Green {N}^3 solves in one batch, {steps} timed steps"
     return out

@@ -1,14 +1,17 @@
-"""Launch-shape autotuner for a solver context.
+"""Launch-shape autotuner for a solver context and for a batch.
 
 Every knob tuned here changes only the order in which workgroups visit the
-grid or which stream a kernel runs on; results are bitwise identical for
-every choice (the tests check it), so the tuner may pick freely by time.
+grid, how many persistent workgroups a launch has, or which stream a kernel
+runs on; results are bitwise identical for every choice (the tests check
+it), so the tuner may pick freely by time, on the live state.
 
-Knobs: rows per band of the stencil kernels' work order (XCD/L2 locality),
-the register-allocation variant (waves per SIMD) of the FP64 face kernels,
-and the stage schedule:
+Single context (tune): rows per band of the stencil kernels' work order
+(XCD/L2 locality), the register-allocation variant (waves per SIMD) of the
+FP64 face kernels, and the stage schedule:
   "overlap"   viscous-flux kernel on a side stream beside the face sweeps;
   "serial"    viscous-flux kernel after the sweeps, on the same stream.
+Batch (tune_batch): rows per band, and the viscous kernels' schedule in the
+step graph (0: in line; n: beside the sweeps with n persistent workgroups).
 """
 import itertools
 import time
@@ -46,4 +49,34 @@ def tune(solver, steps=3, bands=BANDS, schedules=SCHEDULES, occs=(2, 3)):
     solver.set_band(best[0])
     apply_schedule(solver, best[1])
     solver.set_face_occupancy(best[2])
+    return best, timings
+
+
+BATCH_BANDS = (8, 16, 32, 64)
+BATCH_OVERLAPS = (0, 256, 512)
+
+
+def _time_batch(batch, steps):
+    batch.sync()
+    t0 = time.perf_counter()
+    batch.run(steps)
+    batch.sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def tune_batch(batch, steps=2, bands=BATCH_BANDS, overlaps=BATCH_OVERLAPS):
+    """Try each (band, viscous overlap) on `steps` steps of the live batch and
+    leave it on the fastest.  Returns (best, {choice: seconds_per_step}).
+    Every solve advances by (steps + 1) steps per choice (a choice's first
+    step re-captures the step graph), so the solves need that many steps
+    left; the steps taken are ordinary steps of the solves."""
+    timings = {}
+    for band, ov in itertools.product(bands, overlaps):
+        batch.set_band(band)
+        batch.set_overlap(ov)
+        batch.run(1)  # re-captures the step graph
+        timings[(band, ov)] = _time_batch(batch, steps)
+    best = min(timings, key=timings.get)
+    batch.set_band(best[0])
+    batch.set_overlap(best[1])
     return best, timings

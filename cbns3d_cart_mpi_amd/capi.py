@@ -87,6 +87,7 @@ _SIGS = {
     "cbns_batch_sync": (_I, [_VP]),
     "cbns_batch_set_workgroups": (_I, [_VP, _I]),
     "cbns_batch_set_overlap": (_I, [_VP, _I]),
+    "cbns_batch_set_band": (_I, [_VP, _I]),
     "cbns_batch_status": (_I, [_VP, _I, _PI, _PI, _PD, _PI]),
     "cbns_batch_poll": (_I, [_VP, _PI, _I, _PI, _PI]),
     "cbns_batch_get_state": (_I, [_VP, _I, _PD]),
@@ -361,6 +362,10 @@ class Batch:
         """0: viscous kernels in line; n > 0: beside the face sweeps on a side
         stream with n persistent workgroups."""
         check(lib().cbns_batch_set_overlap(self._h, int(viscous_workgroups)), "cbns_batch_set_overlap")
+
+    def set_band(self, rows):
+        """Rows per band in the work order of the sweeps and the stage tail."""
+        check(lib().cbns_batch_set_band(self._h, int(rows)), "cbns_batch_set_band")
 
     def set_workgroups(self, n):
         """Persistent workgroups of the memory-bound stage kernels (0: each

@@ -512,6 +512,14 @@ int cbns_batch_set_workgroups(cbns_batch* b, int wgs) {
   return CBNS_OK;
 }
 
+int cbns_batch_set_band(cbns_batch* b, int rows) {
+  if (!b || rows < 1 || rows > 4096) return CBNS_ERR_ARG;
+  CK(hipStreamSynchronize(b->stream));
+  b->jb = rows;
+  b->dirty = true;  // the step graph holds the band
+  return CBNS_OK;
+}
+
 int cbns_batch_set_overlap(cbns_batch* b, int viscous_workgroups) {
   if (!b || viscous_workgroups < 0 || viscous_workgroups > 65535) return CBNS_ERR_ARG;
   CK(hipStreamSynchronize(b->stream));

@@ -224,6 +224,11 @@ int cbns_batch_set_workgroups(cbns_batch* b, int workgroups);
  * every value.  Takes effect at the next run.  No counterpart in the
  * reference. */
 int cbns_batch_set_overlap(cbns_batch* b, int viscous_workgroups);
+/* Rows per band in the work order of the face sweeps and the stage tail
+ * (default 32), as cbns_set_band for a single context.  A tuning knob only;
+ * results are identical for every value.  Takes effect at the next run.  No
+ * counterpart in the reference. */
+int cbns_batch_set_band(cbns_batch* b, int rows);
 int cbns_batch_sync(cbns_batch* b);
 int cbns_batch_status(cbns_batch* b, int id, int* active, int* steps_done, double* t_cur, int* nonfinite);
 /* Ids of live solves that are done (took their steps, reached t_end or failed);

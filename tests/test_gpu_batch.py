@@ -80,15 +80,17 @@ def test_odd_plane_viscous_chunks_equal_single_runs(precision):
 
 def test_viscous_overlap_and_grids_change_nothing():
     """The step graph's schedule (viscous kernels in line or beside the face
-    sweeps, with any grid) and the memory-bound kernels' grid leave every
-    solve's values unchanged."""
+    sweeps, with any grid), the memory-bound kernels' grid and the band of the
+    work order leave every solve's values unchanged."""
     cases = [((24, 20, 16), (1, 1, 1), C.CBNS_FP64), ((20, 16, 21), (2, 1, 1), C.CBNS_FP32_FLUX),
              ((32, 32, 32), (1, 1, 1), C.CBNS_FP32_FLUX)]
     out = []
-    for ov, wg in ((0, 0), (64, 0), (None, 96), (300, 5)):
+    for ov, wg, band in ((0, 0, None), (64, 0, 7), (None, 96, 64), (300, 5, 1)):
         with C.Batch(32 * GB) as b:
             if ov is not None:
                 b.set_overlap(ov)
+            if band is not None:
+                b.set_band(band)
             b.set_workgroups(wg)
             ids = [b.add(C.make_params(N, Mach=0.3, Re=700.0), dims=d, precision=p, steps=5) for N, d, p in cases]
             b.run(5)
