@@ -38,6 +38,18 @@ def test_null_arguments_are_rejected():
     assert L.cbns_destroy(None) == -1
     assert L.cbns_step(None, None, None) == -1
     assert L.cbns_face_flux_eval(0, 5, 1.4, 1, None, None, None) == -1
+    for f in ("cbns_batch_set_band", "cbns_batch_set_overlap", "cbns_batch_set_workgroups"):
+        assert getattr(L, f)(None, 32) == -1, f
+    assert L.cbns_batch_sync(None) == -1
+    assert L.cbns_batch_destroy(None) == -1
+
+
+def test_batch_create_without_gpu_reports_no_device():
+    if C.device_count() > 0:
+        pytest.skip("a gfx950 device is present")
+    h = ctypes.c_void_p()
+    assert C.lib().cbns_batch_create(1 << 30, 4, ctypes.byref(h)) == -4
+    assert C.lib().cbns_batch_create(1 << 30, 0, ctypes.byref(h)) == -1
 
 
 def test_create_without_gpu_reports_no_device():
