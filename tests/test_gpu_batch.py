@@ -68,7 +68,9 @@ def test_odd_plane_viscous_chunks_equal_single_runs(precision):
     """z-extents whose viscous march leaves a chunk with an odd number of
     planes (n2 + 4 = 25: chunks of 16 and 9; 29: 16 and 13), so a persistent
     workgroup's next unit refills the LDS buffer its last plane read."""
-    cases = [((20, 16, 21), (1, 1, 1), 5), ((18, 22, 25), (1, 2, 1), 4), ((16, 16, 16), (1, 1, 1), 6)]
+    # the last case has an odd x-width: per-cell stage tail, dt from kb_dt every step
+    cases = [((20, 16, 21), (1, 1, 1), 5), ((18, 22, 25), (1, 2, 1), 4), ((16, 16, 16), (1, 1, 1), 6),
+             ((21, 18, 16), (1, 1, 1), 4)]
     with C.Batch(32 * GB) as b:
         ids = [b.add(C.make_params(N, Mach=0.3, Re=500.0), dims=d, precision=precision, steps=s) for N, d, s in cases]
         b.run(6)
