@@ -106,7 +106,7 @@ __device__ __forceinline__ void face_body(const BlockDesc& B, int t, int which, 
   double q[6][5], pl[5], pr[5], fl[5];
   int fi;
   face_load<D>(B, t, which, jb, q, pl, pr, fi);
-  face_flux<D, FMA>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl);
+  face_flux<D, FMA, !FMA>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl);
   gptr<double> __restrict__ F = gmem(B.F[D]);
 #pragma unroll
   for (int e = 0; e < 5; e++) F[e * B.sf[D] + fi] = fl[e];
@@ -269,7 +269,7 @@ __device__ __forceinline__ void face_eval_body(const Phys& ph, int t, const doub
 #pragma unroll
     for (int e = 0; e < 5; e++) fl[e] = (double)f32[e] + (e == 1 + D ? g.p : 0.0);
   } else {
-    face_flux<D, MODE == 1>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl);
+    face_flux<D, MODE == 1, MODE == 0>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl);
   }
 #pragma unroll
   for (int e = 0; e < 5; e++) flux[(long long)t * 5 + e] = fl[e];

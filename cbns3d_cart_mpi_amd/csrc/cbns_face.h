@@ -76,7 +76,18 @@ __device__ __forceinline__ double qdiv(double a, double b) {
 }
 
 // src/weno.cpp:25-56
-template <bool FMA = false>
+// GUARD (plain path): when every active lane's smoothness indicators are below
+// 2^200, the three weight quotients 0.1 / IS1^2 .. (divisors in [1e-12,
+// 2^400]) take qdiv<true>, and then so does the final quotient when every
+// lane's numerator lies in [2^-500, 2^300] in magnitude (the divisor, the sum
+// of the weights, lies in [2^-404, 2^41]).  For such operands the IEEE
+// sequence's range scaling (v_div_scale) is the identity and its special-case
+// fix-up (v_div_fixup) passes the quotient through, so qdiv<true> returns the
+// IEEE quotient bit for bit, except that a numerator of -0 gives +0.  Other
+// waves take the IEEE division.  The wave-uniform branches cut the compiler's
+// scheduling regions, which costs the two-face pair bodies more than the
+// three saved instructions per quotient, so only the single-face body uses it.
+template <bool FMA = false, bool GUARD = false>
 __device__ __forceinline__ double weno5(double Um2, double Um1, double U0, double Up1, double Up2) {
   const double eps = 1.0e-6;
   double diff1 = Um2 - 2.0 * Um1 + U0;
@@ -88,14 +99,24 @@ __device__ __forceinline__ double weno5(double Um2, double Um1, double U0, doubl
   double IS1 = (13.0 / 12.0) * diff1 * diff1 + 0.25 * diff2 * diff2 + eps;
   double IS2 = (13.0 / 12.0) * diff3 * diff3 + 0.25 * diff4 * diff4 + eps;
   double IS3 = (13.0 / 12.0) * diff5 * diff5 + 0.25 * diff6 * diff6 + eps;
-  double w1 = qdiv<FMA>(0.1, IS1 * IS1);
-  double w2 = qdiv<FMA>(0.6, IS2 * IS2);
-  double w3 = qdiv<FMA>(0.3, IS3 * IS3);
+  const bool gw = FMA || (GUARD && __all(IS1 < 0x1p200 && IS2 < 0x1p200 && IS3 < 0x1p200));
+  double w1, w2, w3;
+  if (gw) {
+    w1 = qdiv<true>(0.1, IS1 * IS1);
+    w2 = qdiv<true>(0.6, IS2 * IS2);
+    w3 = qdiv<true>(0.3, IS3 * IS3);
+  } else {
+    w1 = 0.1 / (IS1 * IS1);
+    w2 = 0.6 / (IS2 * IS2);
+    w3 = 0.3 / (IS3 * IS3);
+  }
   double sum = w1 + w2 + w3;
   double f1 = div6(2.0 * Um2 - 7.0 * Um1 + 11.0 * U0);
   double f2 = div6(-Um1 + 5.0 * U0 + 2.0 * Up1);
   double f3 = div6(2.0 * U0 + 5.0 * Up1 - Up2);
-  return qdiv<FMA>(w1 * f1 + w2 * f2 + w3 * f3, sum);
+  const double num = w1 * f1 + w2 * f2 + w3 * f3;
+  if (FMA || (gw && __all(fabs(num) >= 0x1p-500 && fabs(num) <= 0x1p300))) return qdiv<true>(num, sum);
+  return num / sum;
 }
 
 // Reconstructed conserved state -> primitive (src/weno.cpp:146-150).
@@ -117,7 +138,7 @@ __device__ __forceinline__ void cons_to_prim(const double (&a)[5], double gam1, 
 //   pl, pr   rho, u, v, w, p of cells i-1 and i (calc_primitive of the same Q)
 // FMA = true is the contracted FP64 form: the same expressions in a
 // translation unit that lets the compiler fuse a * b + c, with qdiv<true>.
-template <int D, bool FMA = false>
+template <int D, bool FMA = false, bool GUARD = false>
 __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double (&pl)[5],
                                           const double (&pr)[5], double gamma, double gam1,
                                           double rg1, int fg1, double (&flux)[5]) {
@@ -186,8 +207,8 @@ __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double 
       }
     }
     // left state from cells i-3..i+1, right state from cells i+2..i-2
-    const double wl = weno5<FMA>(fc[0], fc[1], fc[2], fc[3], fc[4]);
-    const double wr = weno5<FMA>(fc[5], fc[4], fc[3], fc[2], fc[1]);
+    const double wl = weno5<FMA, GUARD>(fc[0], fc[1], fc[2], fc[3], fc[4]);
+    const double wr = weno5<FMA, GUARD>(fc[5], fc[4], fc[3], fc[2], fc[1]);
     if (m == 0) {
       aL[0] = wl;              aR[0] = wr;
       aL[1] = Rm0[0] * wl;     aR[1] = Rm0[0] * wr;

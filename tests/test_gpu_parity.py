@@ -67,6 +67,34 @@ def test_face_flux_matches_oracle_exactly(d):
     assert not bad.any(), "%d faces differ, e.g. %s" % (bad.any(axis=1).sum(), np.argwhere(bad)[:3].tolist())
 
 
+@pytest.mark.parametrize("d", [0, 2])
+def test_face_flux_exact_at_range_extremes(d):
+    """Stencils scaled by 1e35 and 1e60 (smoothness indicators past 2^200),
+    by 1e-200 (reconstruction numerators below 2^-500) and with a zero
+    velocity component (exactly zero characteristic fields), interleaved with
+    ordinary ones so that waves mix lanes of both kinds: the WENO quotients
+    leave their unscaled form wherever an operand nears the exponent range's
+    ends, and every face still equals the reference arithmetic."""
+    rng = np.random.default_rng(700 + d)
+    cells, prim = _random_faces(rng, 3000)
+    kind = rng.integers(0, 5, cells.shape[0])  # 0 ordinary, 1..3 scaled, 4 zero w
+    for k, sc in ((1, 1e35), (2, 1e60), (3, 1e-200)):
+        cells[kind == k] *= sc
+        prim[kind == k, :, 0] *= sc
+        prim[kind == k, :, 4] *= sc
+    z = kind == 4
+    cells[z, :, 4] -= 0.5 * cells[z, :, 3] ** 2 / cells[z, :, 0]
+    cells[z, :, 3] = 0.0
+    prim[z, :, 3] = 0.0
+    prim[z, :, 4] = 0.4 * (cells[z][:, [2, 3], 4] - 0.5 * prim[z, :, 0] * (prim[z, :, 1] ** 2 + prim[z, :, 2] ** 2))
+    dev = C.face_flux_eval(d, 1.4, cells, prim)
+    ref = np.array([_oracle_face(d, 1.4, cells[f], prim[f]) for f in range(cells.shape[0])])
+    nan_ref = np.isnan(ref)
+    bad = ~((dev == ref) | (nan_ref & np.isnan(dev)))
+    assert not bad.any(), "%d faces differ, e.g. %s" % (bad.any(axis=1).sum(), np.argwhere(bad)[:3].tolist())
+    assert np.isfinite(ref[kind == 2]).all(axis=1).mean() > 0.5
+
+
 # ---------------------------------------------------------- whole solver
 @pytest.mark.parametrize("n", [1, 2, 10])
 def test_tgv16_matches_reference(golden, n):
