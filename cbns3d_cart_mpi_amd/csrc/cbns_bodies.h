@@ -101,12 +101,47 @@ __device__ __forceinline__ void face_load(const BlockDesc& B, int t, int which, 
   }
 }
 
+// Plain path: the face flux of the face whose cells i-3 .. i+2 along D start
+// at ghosted index c0 - 3 sD, recomputed from memory with the IEEE division
+// and square root (face_flux<D, false, false>) for a face whose speculative
+// form's operand checks failed.
+template <int D>
+__device__ __forceinline__ void face_flux_ieee(const BlockDesc& B, int which, int c0, int sD, const Phys& ph,
+                                            double (&fl)[5]) {
+  gptr<const double> __restrict__ Q = gmem<const double>(B.Q[which]);
+  gptr<const double> __restrict__ P = gmem<const double>(B.prim);
+  const long long sg = B.sg;
+  double q[6][5], pl[5], pr[5];
+#pragma unroll
+  for (int cc = 0; cc < 6; cc++)
+#pragma unroll
+    for (int e = 0; e < 5; e++) q[cc][e] = Q[e * sg + c0 + (cc - 3) * sD];
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    pl[f] = P[f * sg + c0 - sD];
+    pr[f] = P[f * sg + c0];
+  }
+  bool ok = true;
+  face_flux<D, false, false>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl, ok);
+}
+
+// The plain path runs the speculative form (face_flux SPEC) and recomputes
+// a face whose checks fail with the IEEE forms.
 template <int D, bool FMA = false>
 __device__ __forceinline__ void face_body(const BlockDesc& B, int t, int which, int jb, const Phys& ph) {
   double q[6][5], pl[5], pr[5], fl[5];
   int fi;
   face_load<D>(B, t, which, jb, q, pl, pr, fi);
-  face_flux<D, FMA, !FMA>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl);
+  bool ok = true;
+  face_flux<D, FMA, !FMA>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl, ok);
+  if constexpr (!FMA) {
+    if (!ok) {
+      const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
+      int i, j, k;
+      band_decode(t, ni, nj, nk, jb, i, j, k);
+      face_flux_ieee<D>(B, which, gidx(B, i, j, k), D == 0 ? 1 : (D == 1 ? B.g[0] : B.g[0] * B.g[1]), ph, fl);
+    }
+  }
   gptr<double> __restrict__ F = gmem(B.F[D]);
 #pragma unroll
   for (int e = 0; e < 5; e++) F[e * B.sf[D] + fi] = fl[e];
@@ -184,7 +219,11 @@ __device__ __forceinline__ void face_pair_body(const BlockDesc& B, int t, int wh
 #pragma unroll
       for (int e = 0; e < 5; e++) qq[cc][e] = q7[cc + h][e];
     double fl[5];
-    face_flux<D, FMA>(qq, p[h], p[h + 1], ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl);
+    bool ok = true;
+    face_flux<D, FMA, !FMA>(qq, p[h], p[h + 1], ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl, ok);
+    if constexpr (!FMA) {
+      if (!ok) face_flux_ieee<D>(B, which, c0 + h * sD, sD, ph, fl);
+    }
 #pragma unroll
     for (int e = 0; e < 5; e++) F[e * sf + fi + h * sF] = fl[e];
   }
@@ -269,7 +308,9 @@ __device__ __forceinline__ void face_eval_body(const Phys& ph, int t, const doub
 #pragma unroll
     for (int e = 0; e < 5; e++) fl[e] = (double)f32[e] + (e == 1 + D ? g.p : 0.0);
   } else {
-    face_flux<D, MODE == 1, MODE == 0>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl);
+    bool ok = true;
+    face_flux<D, MODE == 1, MODE == 0>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl, ok);
+    if (MODE == 0 && !ok) face_flux<D, false, false>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl, ok);
   }
 #pragma unroll
   for (int e = 0; e < 5; e++) flux[(long long)t * 5 + e] = fl[e];

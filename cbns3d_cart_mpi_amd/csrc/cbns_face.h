@@ -24,7 +24,12 @@
 //     m into both reconstructed states, which adds R[li][m] * w_m in the same
 //     order m = 0..4 as the reference's back-projection loop;
 //   * the LF wave speed |ws . n| reduces to the normal component, the only
-//     non-zero term.
+//     non-zero term;
+//   * the face kernels run the speculative form (SPEC, below): quotients and
+//     square roots whose operand ranges each face checks take the IEEE
+//     sequences without their range scaling and special-case fix-ups, which
+//     cannot act on such operands; a face failing a check is recomputed with
+//     the IEEE forms.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -75,20 +80,44 @@ __device__ __forceinline__ double qdiv(double a, double b) {
   }
 }
 
+// sqrt(x) as the compiler's IEEE sequence computes it (v_rsq_f64, then two
+// Newton-Raphson corrections of g ~ sqrt(x) and h ~ 1/(2 sqrt(x))), without
+// its input scaling for x < 2^-767 and its select of x itself for zero and
+// infinity: the same value for 2^-767 <= x < infinity.
+__device__ __forceinline__ double sqrt_noscale(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double h = y * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, h, g);
+}
+
+// Speculative form of the plain path (SPEC).  The compiler's IEEE sequence
+// for a / b is v_div_scale x2, v_rcp_f64, two Newton steps, q = a y,
+// fma(fma(-b, q, a), y, q) in v_div_fmas, then v_div_fixup.  The scaling is
+// the identity and the fix-up passes the quotient through unless an operand
+// or the quotient is zero, infinite, NaN or near the ends of the exponent
+// range, so qdiv<true> gives the IEEE quotient bit for bit when the operands
+// are moderate; likewise sqrt_noscale.  SPEC takes those forms for the
+// quotients and roots whose operands it checks, and clears ok where a check
+// fails; the caller recomputes such a face with the IEEE forms.  Ranges used:
+//   positive operands in [2^-300, 2^300]: quotients of two lie in
+//     [2^-600, 2^600], exponent gap < 768, numerator far from 2^-970;
+//   WENO weights c / IS^2 with IS in [1e-6, 2^200): divisor in
+//     [1e-12, 2^400], quotient in [2^-404, 2^40];
+//   WENO final quotient: divisor (the weight sum) in [2^-404, 2^41],
+//     numerator 0 or of magnitude in [2^-500, 2^300].  A numerator of -0
+//     gives +0 where IEEE division gives -0 (the only difference).
+__device__ __forceinline__ bool in_range300(double x) { return x >= 0x1p-300 && x <= 0x1p300; }
+
 // src/weno.cpp:25-56
-// GUARD (plain path): when every active lane's smoothness indicators are below
-// 2^200, the three weight quotients 0.1 / IS1^2 .. (divisors in [1e-12,
-// 2^400]) take qdiv<true>, and then so does the final quotient when every
-// lane's numerator lies in [2^-500, 2^300] in magnitude (the divisor, the sum
-// of the weights, lies in [2^-404, 2^41]).  For such operands the IEEE
-// sequence's range scaling (v_div_scale) is the identity and its special-case
-// fix-up (v_div_fixup) passes the quotient through, so qdiv<true> returns the
-// IEEE quotient bit for bit, except that a numerator of -0 gives +0.  Other
-// waves take the IEEE division.  The wave-uniform branches cut the compiler's
-// scheduling regions, which costs the two-face pair bodies more than the
-// three saved instructions per quotient, so only the single-face body uses it.
-template <bool FMA = false, bool GUARD = false>
-__device__ __forceinline__ double weno5(double Um2, double Um1, double U0, double Up1, double Up2) {
+template <bool FMA = false, bool SPEC = false>
+__device__ __forceinline__ double weno5(double Um2, double Um1, double U0, double Up1, double Up2, bool& ok) {
   const double eps = 1.0e-6;
   double diff1 = Um2 - 2.0 * Um1 + U0;
   double diff2 = Um2 - 4.0 * Um1 + 3.0 * U0;
@@ -99,24 +128,21 @@ __device__ __forceinline__ double weno5(double Um2, double Um1, double U0, doubl
   double IS1 = (13.0 / 12.0) * diff1 * diff1 + 0.25 * diff2 * diff2 + eps;
   double IS2 = (13.0 / 12.0) * diff3 * diff3 + 0.25 * diff4 * diff4 + eps;
   double IS3 = (13.0 / 12.0) * diff5 * diff5 + 0.25 * diff6 * diff6 + eps;
-  const bool gw = FMA || (GUARD && __all(IS1 < 0x1p200 && IS2 < 0x1p200 && IS3 < 0x1p200));
-  double w1, w2, w3;
-  if (gw) {
-    w1 = qdiv<true>(0.1, IS1 * IS1);
-    w2 = qdiv<true>(0.6, IS2 * IS2);
-    w3 = qdiv<true>(0.3, IS3 * IS3);
-  } else {
-    w1 = 0.1 / (IS1 * IS1);
-    w2 = 0.6 / (IS2 * IS2);
-    w3 = 0.3 / (IS3 * IS3);
-  }
+  constexpr bool F = FMA || SPEC;
+  double w1 = qdiv<F>(0.1, IS1 * IS1);
+  double w2 = qdiv<F>(0.6, IS2 * IS2);
+  double w3 = qdiv<F>(0.3, IS3 * IS3);
   double sum = w1 + w2 + w3;
   double f1 = div6(2.0 * Um2 - 7.0 * Um1 + 11.0 * U0);
   double f2 = div6(-Um1 + 5.0 * U0 + 2.0 * Up1);
   double f3 = div6(2.0 * U0 + 5.0 * Up1 - Up2);
   const double num = w1 * f1 + w2 * f2 + w3 * f3;
-  if (FMA || (gw && __all(fabs(num) >= 0x1p-500 && fabs(num) <= 0x1p300))) return qdiv<true>(num, sum);
-  return num / sum;
+  if constexpr (SPEC) {
+    const double an = fabs(num);
+    ok = ok && IS1 < 0x1p200 && IS2 < 0x1p200 && IS3 < 0x1p200 && an <= 0x1p300 &&
+         (an >= 0x1p-500 || an == 0.0);
+  }
+  return qdiv<F>(num, sum);
 }
 
 // Reconstructed conserved state -> primitive (src/weno.cpp:146-150).
@@ -138,22 +164,29 @@ __device__ __forceinline__ void cons_to_prim(const double (&a)[5], double gam1, 
 //   pl, pr   rho, u, v, w, p of cells i-1 and i (calc_primitive of the same Q)
 // FMA = true is the contracted FP64 form: the same expressions in a
 // translation unit that lets the compiler fuse a * b + c, with qdiv<true>.
-template <int D, bool FMA = false, bool GUARD = false>
+template <int D, bool FMA = false, bool SPEC = false>
 __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double (&pl)[5],
                                           const double (&pr)[5], double gamma, double gam1,
-                                          double rg1, int fg1, double (&flux)[5]) {
+                                          double rg1, int fg1, double (&flux)[5], bool& ok) {
+  constexpr bool F = FMA || SPEC;
+  const auto root = [](double x) { return SPEC ? sqrt_noscale(x) : sqrt(x); };
+  if constexpr (SPEC)
+    ok = in_range300(pl[0]) && in_range300(pl[4]) && in_range300(pr[0]) && in_range300(pr[4]) &&
+         gamma >= 0x1p-100 && gamma <= 0x1p100 && gam1 >= 0x1p-100 && gam1 <= 0x1p100;
   // ---- Roe average (src/eigenvectors_roe.cpp:87-106)
-  double c_L = sqrt(qdiv<FMA>(gamma * pl[4], pl[0]));
+  double c_L = root(qdiv<F>(gamma * pl[4], pl[0]));
   double H_L = divc(c_L * c_L, gam1, rg1, fg1) + 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
-  double c_R = sqrt(qdiv<FMA>(gamma * pr[4], pr[0]));
+  double c_R = root(qdiv<F>(gamma * pr[4], pr[0]));
   double H_R = divc(c_R * c_R, gam1, rg1, fg1) + 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
-  double ratio = sqrt(qdiv<FMA>(pr[0], pl[0]));
+  double ratio = root(qdiv<F>(pr[0], pl[0]));
   double uu = qdiv<FMA>(pl[1] + ratio * pr[1], 1.0 + ratio);
   double vv = qdiv<FMA>(pl[2] + ratio * pr[2], 1.0 + ratio);
   double ww = qdiv<FMA>(pl[3] + ratio * pr[3], 1.0 + ratio);
   double H = qdiv<FMA>(H_L + ratio * H_R, 1.0 + ratio);
   double q2 = uu * uu + vv * vv + ww * ww;
-  double c = sqrt(gam1 * (H - 0.5 * q2));
+  const double c2 = gam1 * (H - 0.5 * q2);
+  if constexpr (SPEC) ok = ok && c2 >= 0x1p-600 && c2 <= 0x1p600;  // c in [2^-300, 2^300]
+  double c = root(c2);
   // frame (n, l, m) per axis (src/eigenvectors_roe.cpp:48-85):
   //   x: n=(1,0,0) l=(-0,0,1) m=(0,-1,0)   qn=uu ql=ww  qm=-vv
   //   y: n=(0,1,0) l=(0,0,-1) m=(-1,0,0)   qn=vv ql=-ww qm=-uu
@@ -161,8 +194,8 @@ __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double 
   const double qn = D == 0 ? uu : (D == 1 ? vv : ww);
 
   // ---- L rows (src/eigenvectors_roe.cpp:138-169)
-  double K = qdiv<FMA>(gam1, 2.0 * c * c);
-  double one_2c = qdiv<FMA>(0.5, c);
+  double K = qdiv<F>(gam1, 2.0 * c * c);
+  double one_2c = qdiv<F>(0.5, c);
   double L00 = 0.5 * K * q2 + one_2c * qn;
   double L10 = 1.0 - K * q2;
   double L20 = 0.5 * K * q2 - one_2c * qn;
@@ -207,8 +240,8 @@ __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double 
       }
     }
     // left state from cells i-3..i+1, right state from cells i+2..i-2
-    const double wl = weno5<FMA, GUARD>(fc[0], fc[1], fc[2], fc[3], fc[4]);
-    const double wr = weno5<FMA, GUARD>(fc[5], fc[4], fc[3], fc[2], fc[1]);
+    const double wl = weno5<FMA, SPEC>(fc[0], fc[1], fc[2], fc[3], fc[4], ok);
+    const double wr = weno5<FMA, SPEC>(fc[5], fc[4], fc[3], fc[2], fc[1], ok);
     if (m == 0) {
       aL[0] = wl;              aR[0] = wr;
       aL[1] = Rm0[0] * wl;     aR[1] = Rm0[0] * wr;
@@ -254,10 +287,11 @@ __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double 
   cons_to_prim<FMA>(aR, gam1, sR);
 
   // ---- local Lax-Friedrichs flux (src/lf_flux.cpp:32-99)
-  double cR = sqrt(qdiv<FMA>(gamma * sR[4], sR[0]));
+  if constexpr (SPEC) ok = ok && in_range300(sR[0]) && in_range300(sR[4]) && in_range300(sL[0]) && in_range300(sL[4]);
+  double cR = root(qdiv<F>(gamma * sR[4], sR[0]));
   double HR = divc(cR * cR, gam1, rg1, fg1) + 0.5 * (sR[1] * sR[1] + sR[2] * sR[2] + sR[3] * sR[3]);
   double rER = divc(sR[4], gam1, rg1, fg1) + 0.5 * sR[0] * (sR[1] * sR[1] + sR[2] * sR[2] + sR[3] * sR[3]);
-  double cL = sqrt(qdiv<FMA>(gamma * sL[4], sL[0]));
+  double cL = root(qdiv<F>(gamma * sL[4], sL[0]));
   double HL = divc(cL * cL, gam1, rg1, fg1) + 0.5 * (sL[1] * sL[1] + sL[2] * sL[2] + sL[3] * sL[3]);
   double rEL = divc(sL[4], gam1, rg1, fg1) + 0.5 * sL[0] * (sL[1] * sL[1] + sL[2] * sL[2] + sL[3] * sL[3]);
   const double qR = sR[1 + D], qL = sL[1 + D];

@@ -72,9 +72,9 @@ def test_face_flux_exact_at_range_extremes(d):
     """Stencils scaled by 1e35 and 1e60 (smoothness indicators past 2^200),
     by 1e-200 (reconstruction numerators below 2^-500) and with a zero
     velocity component (exactly zero characteristic fields), interleaved with
-    ordinary ones so that waves mix lanes of both kinds: the WENO quotients
-    leave their unscaled form wherever an operand nears the exponent range's
-    ends, and every face still equals the reference arithmetic."""
+    ordinary ones so that waves mix lanes of both kinds: faces whose
+    quotient or root operands leave the checked ranges are recomputed with
+    the IEEE forms, and every face still equals the reference arithmetic."""
     rng = np.random.default_rng(700 + d)
     cells, prim = _random_faces(rng, 3000)
     kind = rng.integers(0, 5, cells.shape[0])  # 0 ordinary, 1..3 scaled, 4 zero w
