@@ -60,7 +60,7 @@ hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTab
                                int precision, int wgs, int* ctr, hipStream_t s);
 hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, int wgs,
                               int* ctr, hipStream_t s);
-hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int wgs, int* ctr,
+hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int wgs, int* ctr, int ninst,
                            hipStream_t s);
 // zeroes n counters with a kernel (the step graph holds no memset nodes)
 hipError_t batch_launch_reset_counters(int* ctr, int n, hipStream_t s);

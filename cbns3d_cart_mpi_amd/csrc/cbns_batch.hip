@@ -155,9 +155,15 @@ __global__ __launch_bounds__(TPB) void kb_ghost(const BlockDesc* __restrict__ bl
 // for the solve it is on and flushes it (integer atomic max on the bits of a
 // non-negative double) when its next tile belongs to another solve, or at
 // the end.
+// When no live solve needs this pass (every stepping solve took its smax
+// from the last stage-3 update), each workgroup leaves before touching the
+// tile queue.
 __global__ __launch_bounds__(TPB) void kb_dt(const BlockDesc* __restrict__ blocks,
                                              const InstDesc* __restrict__ insts, InstDesc* insts_rw, TileTable tt,
-                                             int* ctr) {
+                                             int* ctr, int ninst) {
+  bool any = false;
+  for (int k = 0; k < ninst; k++) any = any || (insts[k].active && insts[k].budget > 0 && !insts[k].s_fused);
+  if (!any) return;  // uniform: every thread reads the same flags
   int cur = -1;
   double acc = 0.0;
   int bad = 0;
@@ -260,10 +266,10 @@ hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTabl
   return hipGetLastError();
 }
 
-hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int wgs, int* ctr,
+hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int wgs, int* ctr, int ninst,
                            hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(kb_dt, dim3(wgs), dim3(TPB), 0, s, blocks, insts, insts, tt, ctr);
+  hipLaunchKernelGGL(kb_dt, dim3(wgs), dim3(TPB), 0, s, blocks, insts, insts, tt, ctr, ninst);
   return hipGetLastError();
 }
 

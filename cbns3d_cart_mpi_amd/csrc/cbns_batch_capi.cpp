@@ -202,7 +202,7 @@ static int rebuild(cbns_batch* b) {
   int used = 0;
   auto ctr = [&]() { return b->d_ctr + K * used++; };
   hipError_t e = batch_launch_reset_counters(b->d_ctr, LAUNCHES * K, s);
-  if (e == hipSuccess) e = batch_launch_dt(b->d_blocks, b->d_insts, b->tt[T_DT], mem(G.dt), ctr(), s);
+  if (e == hipSuccess) e = batch_launch_dt(b->d_blocks, b->d_insts, b->tt[T_DT], mem(G.dt), ctr(), hw, s);
   if (e == hipSuccess) e = batch_launch_finalize(b->d_insts, hw, s);
   for (int stage = 1; stage <= 3 && e == hipSuccess; stage++) {
     const int in = stage == 1 ? CBNS_Q : CBNS_QP;

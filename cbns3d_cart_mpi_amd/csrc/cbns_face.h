@@ -63,10 +63,12 @@ __device__ __forceinline__ double divc(double x, double h, double y, int fast) {
 }
 
 // a / b.  FMA = false: the IEEE quotient.  FMA = true (the contracted FP64
-// form): the same reciprocal refinement and one-FMA correction as the
-// compiler's IEEE sequence without its range scaling and special-case
-// fix-up, so the quotient is still correctly rounded for operands of
-// moderate magnitude (nonzero, |a|, |b| between about 2^-300 and 2^300).
+// form, and the checked operands of the plain path's speculative form): the
+// same reciprocal refinement and one-FMA correction as the compiler's IEEE
+// sequence without its range scaling and special-case fix-up, so the
+// quotient is still correctly rounded for operands of moderate magnitude
+// (nonzero, |a|, |b| between about 2^-300 and 2^300).  The contracted form
+// takes sqrt_noscale for its square roots likewise.
 template <bool FMA>
 __device__ __forceinline__ double qdiv(double a, double b) {
   if constexpr (!FMA) {
@@ -169,7 +171,7 @@ __device__ __forceinline__ void face_flux(const double (&q)[6][5], const double 
                                           const double (&pr)[5], double gamma, double gam1,
                                           double rg1, int fg1, double (&flux)[5], bool& ok) {
   constexpr bool F = FMA || SPEC;
-  const auto root = [](double x) { return SPEC ? sqrt_noscale(x) : sqrt(x); };
+  const auto root = [](double x) { return F ? sqrt_noscale(x) : sqrt(x); };
   if constexpr (SPEC)
     ok = in_range300(pl[0]) && in_range300(pl[4]) && in_range300(pr[0]) && in_range300(pr[4]) &&
          gamma >= 0x1p-100 && gamma <= 0x1p100 && gam1 >= 0x1p-100 && gam1 <= 0x1p100;

@@ -67,7 +67,8 @@ typedef enum cbns_precision {
                          deviations from a reference state; state, Roe averages,
                          viscous terms, RK3 and reductions in FP64 */
   CBNS_FP64_FMA = 2   /* the plain path with the face fluxes' a * b + c fused
-                         into FMAs and quotients without range scaling: FP64
+                         into FMAs, quotients and square roots without range
+                         scaling: FP64
                          throughout, within a few roundings of CBNS_FP64 per
                          face; single contexts only (not the batched form) */
 } cbns_precision;
