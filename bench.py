@@ -73,6 +73,16 @@ def read_traffic():
     return float(m.group(1)) if m else None
 
 
+def read_valu_active():
+    """Mean VALU-active fraction of the FP64 y and z face sweeps, from the
+    committed rocprofv3 PMC summary (SQ_ACTIVE_INST_VALU / (CUs x cycles))."""
+    fn = os.path.join(ROOT, "profiles", "valu_tgv256_pmc.md")
+    if not os.path.exists(fn):
+        return None
+    m = re.search(r"valu_active_face_yz:\s*([0-9.]+)", open(fn).read())
+    return float(m.group(1)) if m else None
+
+
 def cpu_baseline(n, max_steps, budget_s):
     """The oracle (CPU restatement) on this host's cores, a bounded sample."""
     from oracle import oracle as O
@@ -321,7 +331,10 @@ def main():
             "kernel": f"k_face_flux ({', '.join(k[-1] for k in sweeps)}); FP64 VALU; "
                       "the FP64 peak is the same for VALU and matrix cores",
             "avg_launch_ms": round(avg_face_ms, 4),
-            "algorithmic_flop_per_launch": FLOP_PER_FACE * faces_per_launch}
+            "algorithmic_flop_per_launch": FLOP_PER_FACE * faces_per_launch,
+            # issue-side view of the same bound: fraction of cycles in which
+            # each CU issues a VALU instruction (profiles/valu_tgv256_pmc.md)
+            "valu_active_frac": read_valu_active()}
     line = {
         "metric": f"Mcell-steps/s TGV {N}^3 FP64 (plain path)",
         "value": round(value, 2),

@@ -52,8 +52,8 @@ def tune(solver, steps=3, bands=BANDS, schedules=SCHEDULES, occs=(2, 3)):
     return best, timings
 
 
-BATCH_BANDS = (8, 16, 32, 64)
-BATCH_OVERLAPS = (0, 256, 512)
+BATCH_BANDS = (16, 32, 64, 128)
+BATCH_OVERLAPS = (0, 128, 256, 384, 512)
 
 
 def _time_batch(batch, steps):
