@@ -249,11 +249,11 @@ __device__ __forceinline__ void face_pair_body_f32(const BlockDesc& B, int t, in
     dev_f32(q, g, d[cc]);
   }
   gptr<const float> __restrict__ P = gmem<const float>((const float*)B.prim);
-  double p[3][5];
+  float p[3][5];
 #pragma unroll
   for (int f = 0; f < 5; f++)
 #pragma unroll
-    for (int s = 0; s < 3; s++) p[s][f] = (double)P[f * sg + c0 + (s - 1) * sD];
+    for (int s = 0; s < 3; s++) p[s][f] = P[f * sg + c0 + (s - 1) * sD];
   gptr<float> __restrict__ F = gmem((float*)B.F[D]);
   const long long sf = B.sf[D];
   const int sF = D == 1 ? ni : ni * nj;  // face-array stride along D

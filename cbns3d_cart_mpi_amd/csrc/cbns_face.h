@@ -382,20 +382,27 @@ __device__ __forceinline__ void dev_f32(const double (&q)[5], const RefState& g,
   d[4] = (float)(q[4] - g.E);
 }
 
-// FP32-flux face flux from the deviations d of cells i-3 .. i+2.
+// FP32-flux face flux from the deviations d of cells i-3 .. i+2 and the
+// stored (float) primitives pf, qf = rho, u, v, w, p of cells i-1 and i.
 template <int D>
-__device__ __forceinline__ void face_flux_f32_dev(const float (&d)[6][5], const double (&pl)[5],
-                                                  const double (&pr)[5], double gamma, double gam1,
+__device__ __forceinline__ void face_flux_f32_dev(const float (&d)[6][5], const float (&pf)[5],
+                                                  const float (&qf)[5], double gamma, double gam1,
                                                   const RefState& g, float (&flux)[5]) {
 #pragma clang fp contract(fast)
   const float gf = (float)gamma, g1 = (float)gam1;
   // ---- Roe average
   float uu, vv, ww, H, c;
-  const float rl = (float)pl[0], rrf = (float)pr[0];
+  const float rl = pf[0], rrf = qf[0];
   const float irl = frcp(rl), ratf = rrf * irl;
   const float rho_min = (float)(1.0e-3 * g.rho);
   const bool wide = !(ratf > 0.5f && ratf < 2.0f) || rl < rho_min || rrf < rho_min;
   if (wide) {
+    double pl[5], pr[5];
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+      pl[e] = (double)pf[e];
+      pr[e] = (double)qf[e];
+    }
     const double rr = pr[0] / pl[0];
     double c_L = sqrt(gamma * pl[4] / pl[0]);
     double H_L = c_L * c_L / gam1 + 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
@@ -411,8 +418,8 @@ __device__ __forceinline__ void face_flux_f32_dev(const float (&d)[6][5], const 
     H = (float)H_;
     c = (float)sqrt(gam1 * (H_ - 0.5 * (u_ * u_ + v_ * v_ + w_ * w_)));
   } else {
-    const float ul = (float)pl[1], vl = (float)pl[2], wl = (float)pl[3], plf = (float)pl[4];
-    const float ur = (float)pr[1], vr = (float)pr[2], wr = (float)pr[3], prf = (float)pr[4];
+    const float ul = pf[1], vl = pf[2], wl = pf[3], plf = pf[4];
+    const float ur = qf[1], vr = qf[2], wr = qf[3], prf = qf[4];
     const float gg1 = gf * frcp(g1);
     const float HL = gg1 * plf * irl + 0.5f * (ul * ul + vl * vl + wl * wl);
     const float HR = gg1 * prf * frcp(rrf) + 0.5f * (ur * ur + vr * vr + wr * wr);
@@ -511,10 +518,15 @@ template <int D>
 __device__ __forceinline__ void face_flux_f32(const double (&q)[6][5], const double (&pl)[5],
                                               const double (&pr)[5], double gamma, double gam1,
                                               const RefState& g, float (&flux)[5]) {
-  float d[6][5];
+  float d[6][5], pf[5], qf[5];
 #pragma unroll
   for (int cc = 0; cc < 6; cc++) dev_f32(q[cc], g, d[cc]);
-  face_flux_f32_dev<D>(d, pl, pr, gamma, gam1, g, flux);
+#pragma unroll
+  for (int e = 0; e < 5; e++) {
+    pf[e] = (float)pl[e];
+    qf[e] = (float)pr[e];
+  }
+  face_flux_f32_dev<D>(d, pf, qf, gamma, gam1, g, flux);
 }
 
 }  // namespace cbns
