@@ -96,6 +96,32 @@ def test_face_flux_exact_at_range_extremes(d):
 
 
 # ---------------------------------------------------------- whole solver
+@pytest.mark.parametrize("scale,dims", [(1e31, (1, 1, 1)), (1e40, (1, 1, 1)), (1e31, (2, 1, 2))])
+def test_scaled_states_take_the_ieee_faces_and_match_reference(scale, dims):
+    """The Taylor-Green state with density, momentum and energy scaled by
+    1e31 (the smoothness indicators of 30-90% of the faces, by direction,
+    pass 2^200) or 1e40 (all of them): the face kernels recompute those
+    faces from memory with the IEEE division and square root, beside faces
+    that keep the unscaled sequences, in the same waves and face pairs.
+    Two steps equal the reference."""
+    N = (20, 16, 12)
+    ob = O.Block(O.make_params(N))
+    ob.init_tgv()
+    Q0 = ob.Q_interior(0) * scale
+    o = O.Block(O.make_params(N))
+    Qg = np.zeros((N[2] + 2 * NG, N[1] + 2 * NG, N[0] + 2 * NG, 5))
+    Qg[NG:-NG, NG:-NG, NG:-NG] = Q0
+    o.set_state(Qg, 0.0, mu0=0.0)
+    o.run(2)
+    ref = o.Q_interior(0)
+    assert np.isfinite(ref).all()
+    with solver(N, dims) as s:
+        s.set_state(Q0, 0.0, mu0=0.0)
+        s.run(2)
+        assert s.t_cur == o.t_cur
+        assert np.array_equal(s.get_state(), ref)
+
+
 @pytest.mark.parametrize("n", [1, 2, 10])
 def test_tgv16_matches_reference(golden, n):
     g = golden("ref_tgv16.npz")
