@@ -148,3 +148,23 @@ def test_step_budgets_pause_and_resume_without_changing_results():
     assert np.array_equal(Qi, Q)
     Q, _ = single(24, 6)
     assert np.array_equal(Qj, Q)
+
+
+def test_scaled_solve_recomputes_faces_as_single_run():
+    """A Taylor-Green state scaled by 1e31 (many faces leave the range the
+    face kernels' unscaled quotients are checked for and are recomputed with
+    the IEEE forms) beside an ordinary solve: both equal their single runs."""
+    N = (20, 16, 12)
+    with C.Solver(C.make_params(N)) as s:
+        s.init_tgv()
+        Q0 = s.get_state() * 1e31
+        s.set_state(Q0, 0.0, mu0=0.0)
+        s.run(3)
+        Qs, ts = s.get_state(), s.t_cur
+    with C.Batch(16 * GB) as b:
+        a = b.add(C.make_params(N), Q=Q0, t_cur=0.0, mu0=0.0, steps=3)
+        o = b.add(C.make_params(24), steps=3)
+        b.run(3)
+        assert b.status(a)["t_cur"] == ts and np.array_equal(b.get_state(a), Qs)
+        Q, t = single(24, 3)
+        assert b.status(o)["t_cur"] == t and np.array_equal(b.get_state(o), Q)
