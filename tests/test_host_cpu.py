@@ -1,0 +1,111 @@
+"""Host logic that runs without a GPU: the launch-shape autotuners (on
+stand-in solver and batch objects with a scripted clock) and bench.py's
+contract pieces (profile readers, the one-GPU guard)."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+from cbns3d_cart_mpi_amd import autotune
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+class FakeSolver:
+    """Records the knobs and advances a clock by a cost that depends on them."""
+
+    def __init__(self, cost):
+        self.cost = cost
+        self.band, self.overlap, self.occ = 16, True, 2
+        self.steps = 0
+        self.now = 0.0
+
+    def set_band(self, b):
+        self.band = b
+
+    def set_overlap(self, on):
+        self.overlap = on
+
+    def set_face_occupancy(self, w):
+        self.occ = w
+
+    def sync(self):
+        pass
+
+    def run(self, n):
+        self.steps += n
+        self.now += n * self.cost(self.band, self.overlap, self.occ)
+
+
+@pytest.fixture
+def clock(monkeypatch):
+    """Route autotune's perf_counter to the fake object's clock."""
+    box = {}
+    monkeypatch.setattr(autotune.time, "perf_counter", lambda: box["obj"].now)
+    return box
+
+
+def test_tune_tries_every_choice_and_keeps_the_fastest(clock):
+    def cost(band, overlap, occ):
+        return 1.0 + abs(band - 32) / 100 + (0.0 if overlap else 0.05) + (0.02 if occ == 3 else 0.0)
+
+    s = FakeSolver(cost)
+    clock["obj"] = s
+    best, tms = autotune.tune(s, steps=3)
+    assert len(tms) == len(autotune.BANDS) * len(autotune.SCHEDULES) * 2
+    assert best == (32, "overlap", 2)
+    assert (s.band, s.overlap, s.occ) == (32, True, 2)
+    assert s.steps == 4 * len(tms)  # one settling step + `steps` timed steps per choice
+    assert tms[(32, "overlap", 2)] == pytest.approx(1.0)
+
+
+def test_apply_schedule_rejects_unknown_names():
+    with pytest.raises(ValueError):
+        autotune.apply_schedule(FakeSolver(lambda *a: 1.0), "sideways")
+
+
+class FakeBatch:
+    def __init__(self):
+        self.band, self.ov, self.steps, self.now = 32, 0, 0, 0.0
+
+    def set_band(self, b):
+        self.band = b
+
+    def set_overlap(self, n):
+        self.ov = n
+
+    def sync(self):
+        pass
+
+    def run(self, n):
+        self.steps += n
+        self.now += n * (2.0 + abs(self.band - 64) / 1000 + abs(self.ov - 256) / 10000)
+
+
+def test_tune_batch_tries_every_choice_and_keeps_the_fastest(clock):
+    b = FakeBatch()
+    clock["obj"] = b
+    best, tms = autotune.tune_batch(b, steps=2)
+    assert len(tms) == len(autotune.BATCH_BANDS) * len(autotune.BATCH_OVERLAPS)
+    assert best == (64, 256) and (b.band, b.ov) == (64, 256)
+    assert b.steps == 3 * len(tms)  # a re-capturing step + `steps` timed steps per choice
+
+
+def test_bench_reads_the_committed_profiles():
+    sys.path.insert(0, ROOT)
+    import bench
+
+    traffic = bench.read_traffic()
+    valu = bench.read_valu_active()
+    # one FP64 face sweep at 256^3 moves ~2.1 GB algorithmically; the VALU
+    # issue fraction of a VALU-bound sweep lies well inside (0.5, 1]
+    assert traffic is not None and 1.5e9 < traffic < 4e9
+    assert valu is not None and 0.5 < valu <= 1.0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
