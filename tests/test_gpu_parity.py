@@ -207,6 +207,24 @@ def test_dt_and_primitives_match_oracle():
         assert np.array_equal(mu, ob.prim(6)[NG:-NG, NG:-NG, NG:-NG])
 
 
+@pytest.mark.parametrize("N,dims", [((12, 8, 16), (3, 2, 4)), ((13, 9, 10), (3, 2, 2))])
+def test_minimum_blocks_match_oracle(N, dims):
+    """Blocks of the minimum size, 4 cells (the halo depth) along every axis
+    (24 blocks), and blocks of 4 and 5 cells with odd widths: every ghost
+    cell of a block is an interior cell of a neighbouring block, the face
+    pairs' last pair is a single face, and each viscous tile covers a
+    whole block.  Three steps equal the single-block oracle."""
+    ob = O.Block(O.make_params(N, Mach=0.3, Re=600.0))
+    ob.init_tgv()
+    ob.run(3)
+    with solver(N, dims, Mach=0.3, Re=600.0) as s:
+        assert s.num_blocks == dims[0] * dims[1] * dims[2]
+        s.init_tgv()
+        s.run(3)
+        assert s.t_cur == ob.t_cur
+        assert np.array_equal(s.get_state(), ob.Q_interior(0))
+
+
 @pytest.mark.parametrize("dims", [(1, 1, 1), (2, 1, 1), (3, 2, 1)])
 def test_odd_width_blocks_match_oracle(dims):
     """Blocks with an odd x-width take the per-cell stage tail (the x-pair
