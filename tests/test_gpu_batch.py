@@ -168,3 +168,18 @@ def test_scaled_solve_recomputes_faces_as_single_run():
         assert b.status(a)["t_cur"] == ts and np.array_equal(b.get_state(a), Qs)
         Q, t = single(24, 3)
         assert b.status(o)["t_cur"] == t and np.array_equal(b.get_state(o), Q)
+
+
+@pytest.mark.parametrize("precision", [C.CBNS_FP64, C.CBNS_FP32_FLUX])
+def test_minimum_blocks_in_batch_equal_single_runs(precision):
+    """A solve of 24 minimum-size blocks (4 cells per axis) and one of 4- and
+    5-cell odd blocks, batched together: each equals its single run."""
+    cases = [((12, 8, 16), (3, 2, 4), 3), ((13, 9, 10), (3, 2, 2), 4)]
+    with C.Batch(16 * GB) as b:
+        ids = [b.add(C.make_params(N, Mach=0.3, Re=600.0), dims=d, precision=precision, steps=s)
+               for N, d, s in cases]
+        b.run(4)
+        for (N, d, s), i in zip(cases, ids):
+            Q, t = single(N, s, d, precision, Mach=0.3, Re=600.0)
+            assert b.status(i)["t_cur"] == t
+            assert np.array_equal(b.get_state(i), Q), N
