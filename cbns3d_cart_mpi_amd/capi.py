@@ -72,6 +72,7 @@ _SIGS = {
     "cbns_sync": (_I, [_VP]),
     "cbns_face_flux_eval": (_I, [_I, _I, _D, _I, _PD, _PD, _PD]),
     "cbns_set_profiling": (_I, [_VP, _I]),
+    "cbns_set_profiling_classes": (_I, [_VP, ctypes.c_uint]),
     "cbns_set_overlap": (_I, [_VP, _I]),
     "cbns_set_band": (_I, [_VP, _I]),
     "cbns_set_face_occupancy": (_I, [_VP, _I]),
@@ -286,6 +287,13 @@ class Solver:
 
     def set_profiling(self, on):
         check(lib().cbns_set_profiling(self._h, int(bool(on))), "cbns_set_profiling")
+
+    def set_profiling_classes(self, names):
+        """Time only the launches of the named kernel classes (see kernel_times)."""
+        mask = 0
+        for n in names:
+            mask |= 1 << self.KERNEL_CLASSES.index(n)
+        check(lib().cbns_set_profiling_classes(self._h, mask), "cbns_set_profiling_classes")
 
     def kernel_times(self):
         """{class: (total_ms, launches)} of launches timed since profiling was turned on."""

@@ -42,9 +42,19 @@ struct cbns_ctx {
   // d_smax/d_flag hold the calc_dt reduction of the current state, made by the
   // last stage's update; cleared by every call that changes the state or mu
   bool s_ready = false;
+  // The last update also copies that reduction to pinned host memory (h_dt)
+  // and records ev_dt after the copy, so that the next step can issue its
+  // first face sweeps, which do not depend on dt, before it waits for dt.
+  struct DtWords {
+    unsigned long long smax;
+    int flag;
+  };
+  DtWords* h_dt = nullptr;
+  hipEvent_t ev_dt = nullptr;
+  bool dt_inflight = false;
   RefState ref{1.0, 0.0, 0.0};  // FP32-flux reference state
   // live kernel timing
-  bool prof = false;
+  unsigned prof = 0;  // bit k: time the launches of kernel class k
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
   double k_ms[CBNS_KCLASSES] = {0};
@@ -79,7 +89,7 @@ static void harvest(cbns_ctx* c) {
 // Launch `body` on stream s bracketed by events of class k when timing is on.
 template <class F>
 static hipError_t timed_on(cbns_ctx* c, hipStream_t s, int k, F body) {
-  if (!c->prof) return body();
+  if (!(c->prof >> k & 1u)) return body();
   hipEvent_t a = take_event(c), b = take_event(c);
   if (!a || !b) return hipErrorOutOfMemory;
   hipError_t e = hipEventRecord(a, s);
@@ -264,6 +274,9 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
   if (!st && hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) st = CBNS_ERR_HIP;
+  if (!st && hipEventCreateWithFlags(&c->ev_dt, hipEventDisableTiming) != hipSuccess) st = CBNS_ERR_HIP;
+  if (!st && hipHostMalloc((void**)&c->h_dt, sizeof(cbns_ctx::DtWords), hipHostMallocDefault) != hipSuccess)
+    st = CBNS_ERR_NOMEM;
   if (!st && hipMemcpy(c->db, c->hb.data(), sizeof(BlockDesc) * nb, hipMemcpyHostToDevice) != hipSuccess)
     st = CBNS_ERR_HIP;
   if (st) {
@@ -283,6 +296,8 @@ int cbns_destroy(cbns_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_dt) (void)hipEventDestroy(c->ev_dt);
+  if (c->h_dt) (void)hipHostFree(c->h_dt);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -351,6 +366,7 @@ static int read_dt(cbns_ctx* c, double* dt) {
   c->nonfinite = flag;
   *dt = c->par.CFL / s_max;  // src/Block3d.cpp:551
   c->s_ready = false;
+  c->dt_inflight = false;
   return CBNS_OK;
 }
 
@@ -534,41 +550,67 @@ int cbns_set_time(cbns_ctx* c, double t) {
   return CBNS_OK;
 }
 
-static int run_stages(cbns_ctx* c, double dt) {
-  // src/cbns3d.cpp:203-240, as five launches per stage: the three face-flux
-  // sweeps, the viscous fluxes (beside them on the side stream when overlap
-  // is on), the fused divergence + RK3 update + primitive recovery, and the
-  // ghost gather of the state with its primitives.
-  for (int stage = 1; stage <= 3; stage++) {
-    const int in = stage == 1 ? CBNS_Q : CBNS_QP;
-    const int out = stage == 3 ? CBNS_Q : CBNS_QP;
-    const BlockDesc* hb = c->hb.data();
-    if (c->overlap) {
-      CK(hipEventRecord(c->ev_fork, c->stream));
-      CK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-      CK(timed_on(c, c->side, 3, [&] {
-        return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->side);
-      }));
-      CK(hipEventRecord(c->ev_join, c->side));
-    }
-    for (int d = 0; d < 3; d++)
-      CK(timed(c, d, [&] {
-        return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->precision, c->ref, c->jband, c->face_occ,
-                                c->stream);
-      }));
-    if (c->overlap) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
-    else CK(timed(c, 3, [&] { return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->stream); }));
-    bool s_done = false;
-    if (stage == 3) {  // the next step's calc_dt reduction rides on the last update
-      CK(hipMemsetAsync(c->d_smax, 0, sizeof(unsigned long long), c->stream));
-      CK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
-    }
-    CK(timed(c, 4, [&] {
-      return launch_update_fused(c->db, hb, c->dom.nb, dt, stage, c->ph, c->precision, c->jband, c->stream,
-                                 stage == 3 ? c->d_smax : nullptr, stage == 3 ? c->d_flag : nullptr, &s_done);
+// src/cbns3d.cpp:203-240, as five launches per stage: the three face-flux
+// sweeps, the viscous fluxes (beside them on the side stream when overlap is
+// on), the fused divergence + RK3 update + primitive recovery, and the ghost
+// gather of the state with its primitives.  launch_sweeps issues the first
+// two (which read only the stage input), finish_stage the rest.
+static int launch_sweeps(cbns_ctx* c, int stage) {
+  const int in = stage == 1 ? CBNS_Q : CBNS_QP;
+  const BlockDesc* hb = c->hb.data();
+  if (c->overlap) {
+    CK(hipEventRecord(c->ev_fork, c->stream));
+    CK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    CK(timed_on(c, c->side, 3, [&] {
+      return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->side);
     }));
-    if (stage == 3) c->s_ready = s_done;
-    CK(timed(c, 5, [&] { return launch_ghost_fill_prim(c->db, hb, c->dom, out, c->precision, c->stream); }));
+    CK(hipEventRecord(c->ev_join, c->side));
+  }
+  for (int d = 0; d < 3; d++)
+    CK(timed(c, d, [&] {
+      return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->precision, c->ref, c->jband, c->face_occ,
+                              c->stream);
+    }));
+  return CBNS_OK;
+}
+
+static int finish_stage(cbns_ctx* c, int stage, double dt) {
+  const int out = stage == 3 ? CBNS_Q : CBNS_QP;
+  const BlockDesc* hb = c->hb.data();
+  if (c->overlap) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  else CK(timed(c, 3, [&] { return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->stream); }));
+  bool s_done = false;
+  if (stage == 3) {  // the next step's calc_dt reduction rides on the last update
+    CK(hipMemsetAsync(c->d_smax, 0, sizeof(unsigned long long), c->stream));
+    CK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+  }
+  CK(timed(c, 4, [&] {
+    return launch_update_fused(c->db, hb, c->dom.nb, dt, stage, c->ph, c->precision, c->jband, c->stream,
+                               stage == 3 ? c->d_smax : nullptr, stage == 3 ? c->d_flag : nullptr, &s_done);
+  }));
+  if (stage == 3) {
+    c->s_ready = s_done;
+    c->dt_inflight = false;
+    if (s_done) {
+      CK(hipMemcpyAsync(&c->h_dt->smax, c->d_smax, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+      CK(hipMemcpyAsync(&c->h_dt->flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      CK(hipEventRecord(c->ev_dt, c->stream));
+      c->dt_inflight = true;
+    }
+  }
+  CK(timed(c, 5, [&] { return launch_ghost_fill_prim(c->db, hb, c->dom, out, c->precision, c->stream); }));
+  return CBNS_OK;
+}
+
+// first_issued: stage 1's sweeps are already on the stream.
+static int run_stages(cbns_ctx* c, double dt, bool first_issued) {
+  for (int stage = 1; stage <= 3; stage++) {
+    if (stage > 1 || !first_issued) {
+      int st = launch_sweeps(c, stage);
+      if (st) return st;
+    }
+    int st = finish_stage(c, stage, dt);
+    if (st) return st;
   }
   return CBNS_OK;
 }
@@ -583,14 +625,31 @@ int cbns_step(cbns_ctx* c, double* dt_out, int* finished) {
     return CBNS_OK;
   }
   double dt;
-  int st = c->s_ready ? read_dt(c, &dt) : cbns_calc_dt(c, &dt);
-  if (st) return st;
-  if (c->nonfinite) return CBNS_ERR_NONFINITE;
+  bool first_issued = false;
+  if (c->s_ready && c->dt_inflight) {
+    // stage 1's sweeps read only the state: issue them, then wait for the
+    // last update's dt reduction (already on its way to pinned memory)
+    // while they run, instead of leaving the GPU idle for the round trip
+    int st = launch_sweeps(c, 1);
+    if (st) return st;
+    first_issued = true;
+    CK(hipEventSynchronize(c->ev_dt));
+    double s_max;
+    std::memcpy(&s_max, &c->h_dt->smax, sizeof(s_max));
+    c->nonfinite = c->h_dt->flag;
+    dt = c->par.CFL / s_max;  // src/Block3d.cpp:551
+    c->s_ready = false;
+    c->dt_inflight = false;
+  } else {
+    int st = c->s_ready ? read_dt(c, &dt) : cbns_calc_dt(c, &dt);
+    if (st) return st;
+  }
+  if (c->nonfinite) return CBNS_ERR_NONFINITE;  // the issued sweeps change no state
   // src/cbns3d.cpp:191-194
   if (c->par.t_end - c->par.t_cur < dt) dt = c->par.t_end - c->par.t_cur;
   c->par.t_cur += dt;
   if (dt_out) *dt_out = dt;
-  return run_stages(c, dt);
+  return run_stages(c, dt, first_issued);
 }
 
 int cbns_run(cbns_ctx* c, int nsteps, double* dts, int* steps_done) {
@@ -632,16 +691,21 @@ int cbns_set_overlap(cbns_ctx* c, int on) {
   return CBNS_OK;
 }
 
-int cbns_set_profiling(cbns_ctx* c, int on) {
-  if (!c) return CBNS_ERR_ARG;
+int cbns_set_profiling_classes(cbns_ctx* c, unsigned mask) {
+  if (!c || (mask >> CBNS_KCLASSES)) return CBNS_ERR_ARG;
   CK(hipStreamSynchronize(c->stream));
+  if (c->side) CK(hipStreamSynchronize(c->side));
   harvest(c);
-  c->prof = on != 0;
+  c->prof = mask;
   for (int k = 0; k < CBNS_KCLASSES; k++) {
     c->k_ms[k] = 0.0;
     c->k_n[k] = 0;
   }
   return CBNS_OK;
+}
+
+int cbns_set_profiling(cbns_ctx* c, int on) {
+  return cbns_set_profiling_classes(c, on ? (1u << CBNS_KCLASSES) - 1 : 0u);
 }
 
 int cbns_kernel_times(cbns_ctx* c, double ms[CBNS_KCLASSES], int64_t n[CBNS_KCLASSES]) {

@@ -173,6 +173,10 @@ int cbns_set_face_occupancy(cbns_ctx* ctx, int waves);
  * reference (which times only the whole loop, src/cbns3d.cpp:174,272-286). */
 #define CBNS_KCLASSES 8 /* face x, face y, face z, viscous, update, ghost fill, primitive, dt */
 int cbns_set_profiling(cbns_ctx* ctx, int on);
+/* The same for the kernel classes whose bit is set in mask (bit k = class k
+ * above; 0 = off): each timed launch adds two events to the stream, so a
+ * caller that needs one class's live time can leave the others untimed. */
+int cbns_set_profiling_classes(cbns_ctx* ctx, unsigned mask);
 int cbns_kernel_times(cbns_ctx* ctx, double ms_total[CBNS_KCLASSES], int64_t launches[CBNS_KCLASSES]);
 
 /* Evaluate the device face flux of one face along axis dir on host-given data:
