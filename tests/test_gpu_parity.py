@@ -292,6 +292,28 @@ def test_kinetic_energy_and_enstrophy_histories_match_oracle():
         assert worst <= 1e-8
 
 
+def test_run_to_t_end_clips_the_last_dt_like_the_reference():
+    """t_end between two steps: the last dt is clipped to t_end - t_cur
+    (src/cbns3d.cpp:191-194) and the next call reports the run finished
+    (is_finished, src/Block3d.cpp:502-513); every dt, the final time and the
+    state equal the oracle's.  Steps after the first take their dt from the
+    previous step's stage-3 reduction, read while the next sweeps run."""
+    N = (20, 16, 12)
+    t_end = 0.031
+    ob = O.Block(O.make_params(N, t_end=t_end))
+    ob.init_tgv()
+    ref = [dt for dt in ob.run(12) if dt is not None]
+    with C.Solver(C.make_params(N, t_end=t_end), (2, 1, 1)) as s:
+        s.init_tgv()
+        dts = s.run(12)
+        assert len(dts) == len(ref) < 12
+        assert list(dts) == ref
+        assert ref[-1] < ref[-2]  # clipped
+        assert s.t_cur == ob.t_cur
+        assert np.array_equal(s.get_state(), ob.Q_interior(0))
+        assert len(s.run(3)) == 0  # finished
+
+
 def test_block_state_round_trip_and_checkpoint_layout():
     with solver(16, (2, 1, 1)) as s:
         s.init_tgv()
