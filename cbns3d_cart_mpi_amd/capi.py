@@ -19,6 +19,8 @@ CBNS_FP32_FLUX = 1
 CBNS_FP64_FMA = 2
 CBNS_Q = 0
 CBNS_QP = 1
+CBNS_ERR_NOMEM = -3
+CBNS_ERR_NONFINITE = -8
 
 _D = ctypes.c_double
 _PD = ctypes.POINTER(ctypes.c_double)

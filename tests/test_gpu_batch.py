@@ -183,3 +183,22 @@ def test_minimum_blocks_in_batch_equal_single_runs(precision):
             Q, t = single(N, s, d, precision, Mach=0.3, Re=600.0)
             assert b.status(i)["t_cur"] == t
             assert np.array_equal(b.get_state(i), Q), N
+
+
+def test_non_finite_solve_stops_alone():
+    """A solve that turns non-finite in its first step (a Taylor-Green state
+    scaled by 1e-200) is flagged and deactivated by the on-device dt
+    bookkeeping; the ordinary solve beside it runs on and equals its single
+    run."""
+    N = (20, 16, 12)
+    with C.Solver(C.make_params(N)) as s:
+        s.init_tgv()
+        Q0 = s.get_state() * 1e-200
+    with C.Batch(16 * GB) as b:
+        bad = b.add(C.make_params(N), Q=Q0, t_cur=0.0, mu0=0.0, steps=5)
+        good = b.add(C.make_params(24), steps=5)
+        b.run(5)
+        st = b.status(bad)
+        assert st["nonfinite"] and not st["active"] and st["steps_done"] < 5
+        Q, t = single(24, 5)
+        assert b.status(good)["t_cur"] == t and np.array_equal(b.get_state(good), Q)

@@ -314,6 +314,35 @@ def test_run_to_t_end_clips_the_last_dt_like_the_reference():
         assert len(s.run(3)) == 0  # finished
 
 
+def test_non_finite_state_stops_the_run():
+    """A Taylor-Green state scaled by 1e-200 turns non-finite in its first
+    step (as in the reference arithmetic: the oracle's state after one step
+    holds NaNs).  The stage-3 dt reduction flags it, and the next step, which
+    reads that flag while its first sweeps run, returns CBNS_ERR_NONFINITE
+    without advancing time or changing the state."""
+    N = (20, 16, 12)
+    ob = O.Block(O.make_params(N))
+    ob.init_tgv()
+    Q0 = ob.Q_interior(0) * 1e-200
+    o = O.Block(O.make_params(N))
+    Qg = np.zeros((N[2] + 2 * NG, N[1] + 2 * NG, N[0] + 2 * NG, 5))
+    Qg[NG:-NG, NG:-NG, NG:-NG] = Q0
+    o.set_state(Qg, 0.0, mu0=0.0)
+    dt1 = o.step()
+    Q1 = o.Q_interior(0)
+    assert not np.isfinite(Q1).all()
+    with solver(N) as s:
+        s.set_state(Q0, 0.0, mu0=0.0)
+        assert list(s.run(1)) == [dt1]
+        Qs = s.get_state()
+        assert np.array_equal(np.isnan(Qs), np.isnan(Q1))
+        with pytest.raises(C.CbnsError) as e:
+            s.run(1)
+        assert e.value.status == C.CBNS_ERR_NONFINITE
+        assert s.t_cur == dt1
+        assert np.array_equal(s.get_state(), Qs, equal_nan=True)
+
+
 def test_block_state_round_trip_and_checkpoint_layout():
     with solver(16, (2, 1, 1)) as s:
         s.init_tgv()
