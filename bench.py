@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--dims", default="1,1,1")
     ap.add_argument("--band", default="auto", help="rows per work band, or 'auto' to tune")
     ap.add_argument("--schedule", default="auto",
-                    help="stage schedule: overlap | serial, or 'auto' to tune")
+                    help="stage schedule: overlap | overlap:N (N viscous workgroups) | serial, or 'auto' to tune")
     ap.add_argument("--occ", default="auto", choices=["auto", "2", "3"],
                     help="face-kernel waves per SIMD, or 'auto' to tune")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -299,7 +299,7 @@ def main():
         s.set_face_occupancy(int(a.occ))
     # with the side-stream schedule the x sweep shares the GPU with the viscous
     # kernel, so its launch time is not the sweep's own; use y and z there
-    overlap = bool(tuned and tuned["schedule"] == "overlap") or a.schedule == "overlap"
+    overlap = (tuned["schedule"] if tuned else a.schedule).startswith("overlap")
     sweeps = ("face_y", "face_z") if overlap else ("face_x", "face_y", "face_z")
     s.sync()
     s.set_profiling_classes(sweeps)  # live HIP-event timing of the roofline kernel over the timed steps

@@ -8,7 +8,8 @@ it), so the tuner may pick freely by time, on the live state.
 Single context (tune): rows per band of the stencil kernels' work order
 (XCD/L2 locality), the register-allocation variant (waves per SIMD) of the
 FP64 face kernels, and the stage schedule:
-  "overlap"   viscous-flux kernel on a side stream beside the face sweeps;
+  "overlap"   viscous-flux kernel on a side stream beside the face sweeps
+              ("overlap:n": on n workgroups taking its tile units in turn);
   "serial"    viscous-flux kernel after the sweeps, on the same stream.
 Batch (tune_batch): rows per band, and the viscous kernels' schedule in the
 step graph (0: in line; n: beside the sweeps with n persistent workgroups).
@@ -17,7 +18,7 @@ import itertools
 import time
 
 BANDS = (4, 8, 16, 32, 64)
-SCHEDULES = ("overlap", "serial")
+SCHEDULES = ("overlap", "overlap:128", "overlap:256", "serial")
 
 
 def _time_steps(solver, steps):
@@ -30,7 +31,9 @@ def _time_steps(solver, steps):
 def apply_schedule(solver, sched):
     if sched not in SCHEDULES:
         raise ValueError(f"unknown schedule {sched!r}")
-    solver.set_overlap(sched == "overlap")
+    over = sched.startswith("overlap")
+    solver.set_overlap(over)
+    solver.set_viscous_workgroups(int(sched.split(":")[1]) if ":" in sched else 0)
 
 
 def tune(solver, steps=3, bands=BANDS, schedules=SCHEDULES, occs=(2, 3)):

@@ -76,6 +76,7 @@ _SIGS = {
     "cbns_set_profiling": (_I, [_VP, _I]),
     "cbns_set_profiling_classes": (_I, [_VP, ctypes.c_uint]),
     "cbns_set_overlap": (_I, [_VP, _I]),
+    "cbns_set_viscous_workgroups": (_I, [_VP, _I]),
     "cbns_set_band": (_I, [_VP, _I]),
     "cbns_set_face_occupancy": (_I, [_VP, _I]),
     "cbns_kernel_times": (_I, [_VP, _PD, ctypes.POINTER(ctypes.c_int64)]),
@@ -286,6 +287,10 @@ class Solver:
 
     def set_overlap(self, on):
         check(lib().cbns_set_overlap(self._h, int(bool(on))), "cbns_set_overlap")
+
+    def set_viscous_workgroups(self, n):
+        """With overlap: workgroups of the side-stream viscous kernel (0 = one per unit)."""
+        check(lib().cbns_set_viscous_workgroups(self._h, int(n)), "cbns_set_viscous_workgroups")
 
     def set_profiling(self, on):
         check(lib().cbns_set_profiling(self._h, int(bool(on))), "cbns_set_profiling")

@@ -29,6 +29,7 @@ struct cbns_ctx {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool overlap = true;
+  int visc_wgs = 0;  // with overlap: workgroups of the side-stream viscous kernel (0 = one per unit)
   int jband = 16;  // rows per band of the stencil kernels' work order
   int face_occ = 2;  // register-allocation variant of the FP64 face kernel
   unsigned long long* d_smax = nullptr;
@@ -562,7 +563,7 @@ static int launch_sweeps(cbns_ctx* c, int stage) {
     CK(hipEventRecord(c->ev_fork, c->stream));
     CK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     CK(timed_on(c, c->side, 3, [&] {
-      return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->side);
+      return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->side, c->visc_wgs);
     }));
     CK(hipEventRecord(c->ev_join, c->side));
   }
@@ -688,6 +689,13 @@ int cbns_set_overlap(cbns_ctx* c, int on) {
   if (!c) return CBNS_ERR_ARG;
   CK(hipStreamSynchronize(c->stream));
   c->overlap = on != 0;
+  return CBNS_OK;
+}
+
+int cbns_set_viscous_workgroups(cbns_ctx* c, int workgroups) {
+  if (!c || workgroups < 0) return CBNS_ERR_ARG;
+  CK(hipStreamSynchronize(c->stream));
+  c->visc_wgs = workgroups;
   return CBNS_OK;
 }
 

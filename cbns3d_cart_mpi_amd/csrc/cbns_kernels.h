@@ -93,8 +93,9 @@ hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Dom
 // viscous fields stored as float for PREC_FP32_FLUX, double otherwise
 hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision, int jb,
                           hipStream_t s);
+// wgs > 0: a fixed number of workgroups taking the tile units in turn
 hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision,
-                            int jb, hipStream_t s);
+                            int jb, hipStream_t s, int wgs = 0);
 // smax/flag (optional): also reduce the next step's calc_dt spectral radius of
 // the updated cells (only over x-pairs, for blocks of even n0; *s_done says
 // whether it was done).

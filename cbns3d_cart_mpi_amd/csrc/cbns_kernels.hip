@@ -175,13 +175,35 @@ __global__ __launch_bounds__(TPB) void k_visc_div(const BlockDesc* __restrict__ 
   visc_div_body<VT>(B, t, ph, jb);
 }
 
+// The same march with a fixed number of workgroups, each taking units
+// (block, column tile, z-chunk) in turn: beside the face sweeps on a second
+// stream it then occupies only that many workgroup slots.
+template <typename VT>
+__global__ __launch_bounds__(VX * VY) void k_visc_flux_units(const BlockDesc* __restrict__ blocks, Phys ph,
+                                                             int tiles, int chunks, int nb) {
+  const int per_block = tiles * chunks, total = per_block * nb;
+  for (int u = blockIdx.x; u < total; u += gridDim.x) {  // uniform per workgroup
+    const int b = u / per_block, r = u - b * per_block;
+    visc_tile_body<VT>(blocks[b], ph, r % tiles, r / tiles);
+    __syncthreads();  // the next unit refills the LDS planes this one read
+  }
+}
+
 hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision,
-                            int jb, hipStream_t s) {
+                            int jb, hipStream_t s, int wgs) {
   (void)jb;  // the tile march fixes its own order
+  const dim3 grid = visc_grid(hb, nb);
+  if (wgs > 0) {
+    if (precision == PREC_FP32_FLUX)
+      hipLaunchKernelGGL(k_visc_flux_units<float>, dim3(wgs), dim3(VX * VY), 0, s, db, ph, (int)grid.x, (int)grid.y, nb);
+    else
+      hipLaunchKernelGGL(k_visc_flux_units<double>, dim3(wgs), dim3(VX * VY), 0, s, db, ph, (int)grid.x, (int)grid.y, nb);
+    return hipGetLastError();
+  }
   if (precision == PREC_FP32_FLUX)
-    hipLaunchKernelGGL(k_visc_flux<float>, visc_grid(hb, nb), dim3(VX * VY), 0, s, db, ph);
+    hipLaunchKernelGGL(k_visc_flux<float>, grid, dim3(VX * VY), 0, s, db, ph);
   else
-    hipLaunchKernelGGL(k_visc_flux<double>, visc_grid(hb, nb), dim3(VX * VY), 0, s, db, ph);
+    hipLaunchKernelGGL(k_visc_flux<double>, grid, dim3(VX * VY), 0, s, db, ph);
   return hipGetLastError();
 }
 

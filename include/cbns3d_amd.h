@@ -159,6 +159,11 @@ int cbns_sync(cbns_ctx* ctx);
 /* Run the viscous kernels of each stage on a second stream beside the face
  * fluxes (default on).  Results are identical either way. */
 int cbns_set_overlap(cbns_ctx* ctx, int on);
+/* With overlap on: the side-stream viscous kernel runs on this many
+ * workgroups that take its tile units in turn, so that it occupies fewer of
+ * the slots the face sweeps use (0, the default: one workgroup per unit).
+ * A tuning knob only; results are identical. */
+int cbns_set_viscous_workgroups(cbns_ctx* ctx, int workgroups);
 /* Rows per band of the stencil kernels' work order (default 16): each XCD
  * walks bands of this many y-rows through all z-planes.  A tuning knob only;
  * results are identical for every value. */

@@ -414,6 +414,22 @@ def test_side_stream_overlap_changes_nothing():
             assert np.array_equal(a.get_state(w), b.get_state(w)), w
 
 
+@pytest.mark.parametrize("workgroups", [1, 7, 128])
+def test_side_stream_viscous_workgroups_change_nothing(workgroups):
+    """The side-stream viscous kernel on a few workgroups that take its tile
+    units in turn (one workgroup: every unit in sequence through the same
+    LDS planes) gives the same states as one workgroup per unit."""
+    with solver((26, 20, 36), (2, 1, 1)) as a, solver((26, 20, 36), (2, 1, 1)) as b:
+        b.set_overlap(True)
+        b.set_viscous_workgroups(workgroups)
+        a.init_tgv()
+        b.init_tgv()
+        a.run(3)
+        b.run(3)
+        for w in (C.CBNS_Q, C.CBNS_QP):
+            assert np.array_equal(a.get_state(w), b.get_state(w)), w
+
+
 @pytest.mark.parametrize("band", [1, 5, 16, 64])
 def test_band_order_changes_nothing(band):
     with solver((26, 20, 16), (2, 1, 1)) as a, solver((26, 20, 16), (2, 1, 1)) as b:

@@ -30,6 +30,9 @@ class FakeSolver:
     def set_face_occupancy(self, w):
         self.occ = w
 
+    def set_viscous_workgroups(self, n):
+        self.vw = n
+
     def sync(self):
         pass
 
