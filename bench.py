@@ -345,7 +345,11 @@ def main():
             "algorithmic_flop_per_launch": FLOP_PER_FACE * faces_per_launch,
             # issue-side view of the same bound: fraction of cycles in which
             # each CU issues a VALU instruction (profiles/valu_tgv256_pmc.md)
-            "valu_active_frac": read_valu_active()}
+            "valu_active_frac": read_valu_active(),
+            # the bit-exact path rounds every add and multiply separately (no
+            # FMA), so one VALU slot retires at most 1 flop: half the peak
+            "peak_without_fma": PEAK_FP64_TFLOPS / 2,
+            "frac_of_peak_without_fma": round(achieved / (PEAK_FP64_TFLOPS / 2), 4)}
     line = {
         "metric": f"Mcell-steps/s TGV {N}^3 FP64 (plain path)",
         "value": round(value, 2),
