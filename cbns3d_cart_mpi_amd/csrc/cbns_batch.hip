@@ -26,6 +26,14 @@ static_assert(BATCH_CTR_PER_LAUNCH == NXCD, "one tile counter per XCD");
 
 template <class F>
 __device__ __forceinline__ void for_tiles(int n, int* __restrict__ ctr, F&& body) {
+  if (ctr == nullptr) {  // direct launch: one tile per workgroup, no queue
+    // workgroups reach the XCDs in turn; XCD r takes the r-th contiguous run
+    // of tiles, as the queues do (the grid is NXCD * per workgroups)
+    const int per = (n + NXCD - 1) / NXCD;
+    const int x = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
+    if (x < n) body(x);
+    return;
+  }
   const bool split = gridDim.x >= NXCD;  // every XCD has workgroups
   const int r = split ? (int)(blockIdx.x % NXCD) : 0;
   const int parts = split ? NXCD : 1;
@@ -252,6 +260,10 @@ hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable
 hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int stage, int jb,
                                int precision, int wgs, int* ctr, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
+  if (wgs == 0) {  // direct launch
+    wgs = (tt.n + NXCD - 1) / NXCD * NXCD;
+    ctr = nullptr;
+  }
   if (precision == 0)
     hipLaunchKernelGGL(kb_update<double>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, insts, tt, stage, jb, ctr);
   else
@@ -262,6 +274,10 @@ hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTab
 hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, int wgs,
                               int* ctr, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
+  if (wgs == 0) {  // direct launch
+    wgs = (tt.n + NXCD - 1) / NXCD * NXCD;
+    ctr = nullptr;
+  }
   hipLaunchKernelGGL(kb_ghost, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, which, ctr);
   return hipGetLastError();
 }

@@ -194,6 +194,9 @@ static int rebuild(cbns_batch* b) {
   CK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   const BatchGrids& G = b->grid;
   auto mem = [&](int resident) { return b->wg_mem > 0 ? b->wg_mem : resident; };
+  // the stage tail (update, ghost fill) defaults to a direct launch, one tile
+  // per workgroup and no queue (grid 0), with the tiles of each XCD contiguous
+  auto tail = [&]() { return b->wg_mem > 0 ? b->wg_mem : 0; };
   // every persistent launch of the step takes its own block of counters, all
   // zeroed by the first node of the graph
   constexpr int LAUNCHES = 1 + 3 * 11;  // dt; per stage 2 viscous, 6 face, 2 update, 1 ghost
@@ -228,11 +231,11 @@ static int rebuild(cbns_batch* b) {
     }
     if (ov && e == hipSuccess) e = hipStreamWaitEvent(s, b->ev_join, 0);  // join before the stage tail
     if (e == hipSuccess)
-      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD64], stage, b->jb, 0, mem(G.update[0]), ctr(), s);
+      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD64], stage, b->jb, 0, tail(), ctr(), s);
     if (e == hipSuccess)
-      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD32], stage, b->jb, 1, mem(G.update[1]), ctr(), s);
+      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD32], stage, b->jb, 1, tail(), ctr(), s);
     if (e == hipSuccess)
-      e = batch_launch_ghost(b->d_blocks, b->d_insts, b->tt[T_GHOST], out, mem(G.ghost), ctr(), s);
+      e = batch_launch_ghost(b->d_blocks, b->d_insts, b->tt[T_GHOST], out, tail(), ctr(), s);
   }
   if (used > LAUNCHES) e = hipErrorInvalidValue;  // a launch without a zeroed counter block
   hipGraph_t g = nullptr;

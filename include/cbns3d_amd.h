@@ -223,9 +223,11 @@ int cbns_batch_run(cbns_batch* b, int nsteps);
  * large ones few within one run. */
 int cbns_batch_set_budgets(cbns_batch* b, const int* ids, const int* steps, int n);
 /* Persistent workgroups per launch of the memory-bound stage kernels (stage
- * tail, ghost gather, dt).  0, the default, uses each kernel's resident
- * workgroups on the device (as every other batch kernel does).  A tuning knob
- * only; results are identical for every value.  Takes effect at the next run. */
+ * tail, ghost gather, dt).  0, the default, launches the stage tail (update,
+ * ghost fill) directly, one tile per workgroup with each XCD's tiles
+ * contiguous, and gives the dt pass its resident workgroups on the device.  A
+ * tuning knob only; results are identical for every value.  Takes effect at
+ * the next run. */
 int cbns_batch_set_workgroups(cbns_batch* b, int workgroups);
 /* Stage schedule of the step graph: 0 runs the viscous-flux kernels before
  * the face sweeps; n > 0 runs them beside the sweeps on a second stream with
