@@ -383,8 +383,9 @@ class Batch:
         check(lib().cbns_batch_set_band(self._h, int(rows)), "cbns_batch_set_band")
 
     def set_workgroups(self, n):
-        """Persistent workgroups of the memory-bound stage kernels (0: each
-        kernel's resident workgroups on the device, the default)."""
+        """Persistent workgroups of the memory-bound stage kernels (0, the
+        default: a direct launch of one workgroup per tile for the stage tail
+        and ghost fill, and the resident workgroups for the dt pass)."""
         check(lib().cbns_batch_set_workgroups(self._h, int(n)), "cbns_batch_set_workgroups")
 
     def sync(self):
