@@ -10,8 +10,11 @@
 // neighbours of a tile (the rows and planes before it) are still in that
 // XCD's L2.  (A static assignment, tile q + m G to workgroup q, lets slow
 // workgroups fall behind and widens the window past the L2: the z sweep
-// then fetched three times its data.)  The per-item arithmetic is the shared
-// bodies', so every solve's values equal a single-solve run's.
+// then fetched three times its data.)  The stage tail (update, ghost fill)
+// is by default launched directly instead, one workgroup per tile, with the
+// same per-XCD contiguous ranges (for_tiles with ctr == nullptr).  The
+// per-item arithmetic is the shared bodies', so every solve's values equal a
+// single-solve run's.
 #include "cbns_batch.h"
 #include "cbns_bodies.h"
 
