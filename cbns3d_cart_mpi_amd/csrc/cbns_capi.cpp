@@ -329,6 +329,7 @@ int cbns_memory_bytes(const cbns_ctx* c, uint64_t* bytes) {
 int cbns_sync(cbns_ctx* c) {
   if (!c) return CBNS_ERR_ARG;
   CK(hipStreamSynchronize(c->stream));
+  CK(hipStreamSynchronize(c->side));
   return CBNS_OK;
 }
 
@@ -645,7 +646,13 @@ int cbns_step(cbns_ctx* c, double* dt_out, int* finished) {
     int st = c->s_ready ? read_dt(c, &dt) : cbns_calc_dt(c, &dt);
     if (st) return st;
   }
-  if (c->nonfinite) return CBNS_ERR_NONFINITE;  // the issued sweeps change no state
+  if (c->nonfinite) {
+    // the issued sweeps change no state, but with overlap on the side
+    // stream's viscous kernel of stage 1 writes mu and the viscous fields:
+    // order every later call on the context's stream after it
+    if (first_issued && c->overlap) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    return CBNS_ERR_NONFINITE;
+  }
   // src/cbns3d.cpp:191-194
   if (c->par.t_end - c->par.t_cur < dt) dt = c->par.t_end - c->par.t_cur;
   c->par.t_cur += dt;

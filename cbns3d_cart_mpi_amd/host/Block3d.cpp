@@ -65,15 +65,37 @@ void Block3d::calc_conservative(value_type*) {}
 value_type* Block3d::get_Q() { return &tag_Q; }
 value_type* Block3d::get_Q_p() { return &tag_Q_p; }
 
+// File of block b for the per-rank checkpoint name fname (see Block3d.h):
+// fname itself for a single block, else fname with its trailing rank digits
+// ("%03d" of rank, then ".dat", as src/cbns3d.cpp:132-134 forms it) replaced
+// by b's.  False when several blocks are held and fname has no such suffix.
+static bool block_file(const std::string& fname, int rank, int nb, int b, std::string* out) {
+  if (nb == 1) {
+    *out = fname;
+    return true;
+  }
+  char tail[32];
+  std::snprintf(tail, sizeof tail, "%03d.dat", rank);
+  const std::string suf(tail);
+  if (fname.size() < suf.size() || fname.compare(fname.size() - suf.size(), suf.size(), suf) != 0) return false;
+  std::snprintf(tail, sizeof tail, "%03d.dat", b);
+  *out = fname.substr(0, fname.size() - suf.size()) + tail;
+  return true;
+}
+
 // src/Block3d.cpp:318-357, one file per block.  mu is not in the file and is
 // zero afterwards, as in the reference's restart path.
-int Block3d::read_bin(const std::string base) {
+int Block3d::read_bin(const std::string fname) {
   std::vector<double> all((size_t)NEQ * IM * JM * KM);
   double t = 0.0;
-  for (int b = 0; b < num_blocks(); b++) {
-    char nm[16];
-    std::snprintf(nm, sizeof nm, "%03d", b);
-    std::ifstream fh(base + nm + ".dat", std::ios::in | std::ios::binary);
+  const int nb = num_blocks();
+  for (int b = 0; b < nb; b++) {
+    std::string fn;
+    if (!block_file(fname, proc_info->rank, nb, b, &fn)) {
+      std::cout << "checkpoint name " << fname << " lacks the rank suffix for " << nb << " blocks\n";
+      return -1;
+    }
+    std::ifstream fh(fn, std::ios::in | std::ios::binary);
     if (!fh) return -1;
     int n[3], o[3];
     cbns_block_info(ctx, b, n, o);
@@ -104,20 +126,25 @@ int Block3d::read_bin(const std::string base) {
 }
 
 // src/Block3d.cpp:964-987, one file per block (ghosts included).
-int Block3d::output_bin(const std::string base) {
-  for (int b = 0; b < num_blocks(); b++) {
+int Block3d::output_bin(const std::string fname) {
+  const int nb = num_blocks();
+  for (int b = 0; b < nb; b++) {
+    std::string fn;
+    if (!block_file(fname, proc_info->rank, nb, b, &fn)) {
+      std::cout << "checkpoint name " << fname << " lacks the rank suffix for " << nb << " blocks\n";
+      return -1;
+    }
     int n[3], o[3];
     cbns_block_info(ctx, b, n, o);
     size_type d[3] = {(size_type)(n[0] + 2 * NG), (size_type)(n[1] + 2 * NG), (size_type)(n[2] + 2 * NG)};
     std::vector<double> Qg((size_t)NEQ * d[0] * d[1] * d[2]);
     if (cbns_get_block_state(ctx, b, CBNS_Q, Qg.data()) != CBNS_OK) return -1;
-    char nm[16];
-    std::snprintf(nm, sizeof nm, "%03d", b);
-    std::ofstream fh(base + nm + ".dat", std::ios::out | std::ios::binary);
+    std::ofstream fh(fn, std::ios::out | std::ios::binary);
     if (!fh) return -1;
     fh.write((const char*)&sim_pars->t_cur, sizeof(value_type));
     fh.write((const char*)d, sizeof d);
     fh.write((const char*)Qg.data(), Qg.size() * sizeof(double));
+    if (!fh) return -1;
   }
   return 0;
 }
@@ -128,6 +155,9 @@ void boundary_condition(Block3d* blk, value_type* Q) {
 
 void Block3d::calc_primitive(const value_type* Q) { status = cbns_calc_primitive(ctx, which(Q)); }
 void Block3d::init_Q_p() { status = cbns_init_Q_p(ctx); }
+
+// src/Block3d.cpp:396-469 for this rank's block
+int Block3d::output_vtr(const std::string fname) { return output_vtr(fname, proc_info->rank); }
 
 // src/Block3d.cpp:396-469 for block b
 int Block3d::output_vtr(const std::string fname, int b) {

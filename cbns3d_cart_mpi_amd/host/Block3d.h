@@ -62,17 +62,24 @@ class Block3d {
   value_type* get_Q_p();
 
   // Checkpoint I/O in the reference's per-rank format
-  // (src/Block3d.cpp:318-357, 964-987), one file per on-device block named
-  // fname_base + "%03d" + ".dat" with the block number.
-  int read_bin(const std::string fname_base);
-  int output_bin(const std::string fname_base);
+  // (src/Block3d.cpp:318-357, 964-987).  fname is the name the reference's
+  // main hands its own rank: the configured name with the rank as %03d and
+  // ".dat" appended (src/cbns3d.cpp:132-134); here that rank is
+  // proc_info->rank.  With one block, fname is the file.  With several,
+  // block b's file is fname with those rank digits replaced by b's, i.e. the
+  // file the reference's rank b would use; a name without the rank suffix is
+  // refused (-1).
+  int read_bin(const std::string fname);
+  int output_bin(const std::string fname);
 
   friend void boundary_condition(Block3d* blk, value_type* Q);
 
   void calc_primitive(const value_type* Q);
   void init_Q_p();
 
-  // VTK XML output of block b (src/Block3d.cpp:396-500).
+  // VTK XML output (src/Block3d.cpp:396-469): of this rank's block
+  // (proc_info->rank, as the reference's per-rank call), or of block b.
+  int output_vtr(const std::string fname);
   int output_vtr(const std::string fname, int b);
   int output_vtm(const std::string fname, const std::string vtr_fname, const size_type num_blocks);
 

@@ -66,6 +66,12 @@ int main(int argc, char** argv) {
   Block3d block(&proc_info, &simParam, simParam.IM, simParam.JM, simParam.KM);
   block.gen_mesh(0, 0, 0);
 
+  // src/cbns3d.cpp:132-134: the checkpoint name of this process's rank; the
+  // facade derives every block's file from it
+  std::ostringstream oss;
+  oss << std::setw(3) << std::setfill('0') << proc_info.rank;
+  simParam.checkpoint_fname += oss.str() + ".dat";
+
   // src/cbns3d.cpp:136-144 (is_restart = 1 means a fresh Taylor-Green start)
   if (simParam.is_restart) {
     block.initial_condition();
@@ -85,7 +91,7 @@ int main(int argc, char** argv) {
   }
 
   size_type cnt_out = 0;
-  std::ostringstream oss;
+  oss.str("");
   oss << std::setw(3) << std::setfill('0') << cnt_out;
   if (0 != output_vtk_xml(&block, 0, nblocks, "output" + oss.str(), "output")) return 1;
 

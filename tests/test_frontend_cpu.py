@@ -72,3 +72,13 @@ def test_driver_usage_message(tmp_path):
         pytest.skip("driver not built")
     r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True)
     assert r.returncode == 1 and "Usage" in r.stdout
+
+
+def test_facade_compiles_the_reference_drivers_block3d_calls():
+    """The reference driver's Block3d calls, with its rank-suffixed checkpoint
+    name and output_vtr(fname), compile against the facade (compile only)."""
+    host = os.path.join(ROOT, "cbns3d_cart_mpi_amd", "host")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                        "-I" + os.path.join(ROOT, "include"), "-I" + host,
+                        os.path.join(ROOT, "tests", "facade_api_check.cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -73,3 +73,34 @@ def test_driver_restart_equals_oracle_restart(tmp_path):
     b.run(1)
     assert t2 == b.t_cur
     assert np.array_equal(Q2[NG:-NG, NG:-NG, NG:-NG], b.Q_interior(0))
+
+
+def test_driver_two_block_restart_uses_the_rank_suffixed_names(tmp_path):
+    """The driver forms the checkpoint name as the reference's main does
+    (rank suffix, src/cbns3d.cpp:132-134) and the facade maps it to one file
+    per block: checkpoint000.dat and checkpoint001.dat, which a restart with
+    the same dims reads back (equal to the oracle's restart)."""
+    from oracle import oracle as O
+
+    run_driver(tmp_path, (16, 16, 16), 1, (2, 1, 1))
+    names = sorted(p.name for p in tmp_path.iterdir() if p.name.startswith("checkpoint"))
+    assert names == ["checkpoint000.dat", "checkpoint001.dat"], names
+    t1, Q1a = read_ckpt(tmp_path / "checkpoint000.dat")
+    _, Q1b = read_ckpt(tmp_path / "checkpoint001.dat")
+    Q1 = np.zeros((24, 24, 24, 5))
+    Q1[NG:-NG, NG:-NG, NG:-NG] = np.concatenate([Q1a[NG:-NG, NG:-NG, NG:-NG], Q1b[NG:-NG, NG:-NG, NG:-NG]], axis=2)
+    vals = input_txt((16, 16, 16), 1).splitlines()
+    vals[-1] = "0"
+    (tmp_path / "input.txt").write_text("\n".join(vals) + "\n")
+    r = subprocess.run([EXE, "2", "1", "1"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    names = sorted(p.name for p in tmp_path.iterdir() if p.name.startswith("checkpoint"))
+    assert names == ["checkpoint000.dat", "checkpoint001.dat"], names
+    t2, Q2a = read_ckpt(tmp_path / "checkpoint000.dat")
+    _, Q2b = read_ckpt(tmp_path / "checkpoint001.dat")
+    b = O.Block(O.make_params((16, 16, 16)))
+    b.set_state(Q1, t1, mu0=0.0)
+    b.run(1)
+    assert t2 == b.t_cur
+    got = np.concatenate([Q2a[NG:-NG, NG:-NG, NG:-NG], Q2b[NG:-NG, NG:-NG, NG:-NG]], axis=2)
+    assert np.array_equal(got, b.Q_interior(0))

@@ -271,6 +271,28 @@ def test_diagnostics_match_oracle():
     assert abs(en - en0) <= 1e-12 * abs(en0)
 
 
+def test_diagnostics_closed_form_on_manufactured_field():
+    """cbns_diagnostics on the manufactured field of tests/manufactured.py
+    (non-uniform rho, all velocity components and both terms of each
+    vorticity component non-zero, three grid spacings), split over four
+    blocks: equal to the closed form and to the oracle within 1e-13."""
+    from manufactured import closed_form, state
+
+    N = (16, 24, 20)
+    Q = state(N)
+    Qg = np.zeros((N[2] + 2 * NG, N[1] + 2 * NG, N[0] + 2 * NG, 5))
+    Qg[NG:-NG, NG:-NG, NG:-NG] = Q
+    ob = O.Block(O.make_params(N))
+    ob.set_state(Qg, 0.0, mu0=0.0)
+    ke_o, en_o = ob.diagnostics()
+    ke0, en0 = closed_form(N)
+    with solver(N, (2, 1, 2)) as s:
+        s.set_state(Q, 0.0, mu0=0.0)
+        ke, en = s.diagnostics()
+    for got, want in ((ke, ke0), (en, en0), (ke, ke_o), (en, en_o)):
+        assert abs(got - want) <= 1e-13 * abs(want), (got, want)
+
+
 def test_kinetic_energy_and_enstrophy_histories_match_oracle():
     """The Taylor-Green KE and enstrophy histories (every 5 steps over 40
     steps at 32^3, four blocks) against the oracle's: the states are
@@ -341,6 +363,16 @@ def test_non_finite_state_stops_the_run():
         assert e.value.status == C.CBNS_ERR_NONFINITE
         assert s.t_cur == dt1
         assert np.array_equal(s.get_state(), Qs, equal_nan=True)
+        # the context stays usable: a valid state set after the failure
+        # (its mu after the stage-1 viscous kernel the failed step issued)
+        # steps exactly as the oracle
+        Qok = ob.Q_interior(0)
+        s.set_state(Qok, 0.0, mu0=0.0)
+        o2 = O.Block(O.make_params(N))
+        Qg[NG:-NG, NG:-NG, NG:-NG] = Qok
+        o2.set_state(Qg, 0.0, mu0=0.0)
+        assert list(s.run(2)) == o2.run(2)
+        assert np.array_equal(s.get_state(), o2.Q_interior(0))
 
 
 def test_block_state_round_trip_and_checkpoint_layout():
