@@ -21,6 +21,8 @@ CBNS_Q = 0
 CBNS_QP = 1
 CBNS_ERR_NOMEM = -3
 CBNS_ERR_NONFINITE = -8
+CBNS_ERR_CAPACITY = -9
+CBNS_NOALLOC_MAX_FACES = 8192
 
 _D = ctypes.c_double
 _PD = ctypes.POINTER(ctypes.c_double)
@@ -73,6 +75,7 @@ _SIGS = {
     "cbns_diagnostics": (_I, [_VP, _PD, _PD]),
     "cbns_sync": (_I, [_VP]),
     "cbns_face_flux_eval": (_I, [_I, _I, _D, _I, _PD, _PD, _PD]),
+    "cbns_face_flux_eval_noalloc": (_I, [_I, _I, _D, _I, _PD, _PD, _PD]),
     "cbns_set_profiling": (_I, [_VP, _I]),
     "cbns_set_profiling_classes": (_I, [_VP, ctypes.c_uint]),
     "cbns_set_overlap": (_I, [_VP, _I]),
@@ -318,6 +321,19 @@ def face_flux_eval(direction, gamma, cells, prim, precision=CBNS_FP64):
     out = np.empty((n, 5))
     check(lib().cbns_face_flux_eval(precision, direction, gamma, n, _p(cells), _p(prim), _p(out)),
           "cbns_face_flux_eval")
+    return out
+
+
+def face_flux_eval_noalloc(direction, gamma, cells, prim, precision=CBNS_FP64):
+    """face_flux_eval staged through the library's fixed per-GPU scratch
+    (cbns_face_flux_eval_noalloc): no device allocation, at most
+    CBNS_NOALLOC_MAX_FACES faces per call."""
+    cells = np.ascontiguousarray(cells, dtype=np.float64)
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    n = cells.shape[0]
+    out = np.empty((n, 5))
+    check(lib().cbns_face_flux_eval_noalloc(precision, direction, gamma, n, _p(cells), _p(prim), _p(out)),
+          "cbns_face_flux_eval_noalloc")
     return out
 
 

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -140,6 +141,7 @@ const char* cbns_strerror(int s) {
     case CBNS_ERR_IO: return "file I/O failed";
     case CBNS_ERR_FORMAT: return "checkpoint dimensions do not match";
     case CBNS_ERR_NONFINITE: return "non-finite value in the time-step reduction";
+    case CBNS_ERR_CAPACITY: return "request exceeds a fixed capacity (no-allocation scratch)";
     default: return "unknown status";
   }
 }
@@ -775,3 +777,63 @@ int cbns_face_flux_eval(int precision, int dir, double gamma, int nf, const doub
 }
 
 }  // extern "C"
+
+// ---- face flux evaluation without device allocation -----------------------
+// Staging of cbns_face_flux_eval_noalloc: per face 30 cell values, 10
+// primitive values and 5 flux values, for up to CBNS_NOALLOC_MAX_FACES faces.
+// One copy per GPU, alive until the process ends.  In host code `scratch`
+// names a host-side stand-in: its device address on each GPU comes from
+// hipGetSymbolAddress (scratch_on_current_device) and only that address is
+// passed to copies and kernels.
+static constexpr size_t NOALLOC_CELLS = 30 * (size_t)CBNS_NOALLOC_MAX_FACES;
+static constexpr size_t NOALLOC_PRIM = 10 * (size_t)CBNS_NOALLOC_MAX_FACES;
+static constexpr size_t NOALLOC_FLUX = 5 * (size_t)CBNS_NOALLOC_MAX_FACES;
+static __device__ __align__(256) double scratch[NOALLOC_CELLS + NOALLOC_PRIM + NOALLOC_FLUX];
+
+static constexpr int NOALLOC_MAXDEV = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[NOALLOC_MAXDEV] = {};
+
+// Device address of the current device's scratch, looked up once per device
+// (with that device current) and kept; CBNS_ERR_NODEV on a device that is
+// not a gfx950.
+static int scratch_on_current_device(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAXDEV) return CBNS_ERR_NODEV;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess || std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      return CBNS_ERR_NODEV;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CBNS_ERR_HIP;
+    g_scratch_dev[dev] = (double*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return CBNS_OK;
+}
+
+extern "C" int cbns_face_flux_eval_noalloc(int precision, int dir, double gamma, int nf, const double* cells,
+                                           const double* prim, double* flux) {
+  if (!cells || !prim || !flux || nf < 0 || dir < 0 || dir > 2) return CBNS_ERR_ARG;
+  if (precision != CBNS_FP64 && precision != CBNS_FP32_FLUX && precision != CBNS_FP64_FMA) return CBNS_ERR_ARG;
+  if (nf > CBNS_NOALLOC_MAX_FACES) return CBNS_ERR_CAPACITY;
+  if (nf == 0) return CBNS_OK;
+  Phys ph{};
+  ph.gamma = gamma;
+  ph.gam1 = gamma - 1.0;
+  reciprocal_check(ph.gam1, &ph.rgam1, &ph.fgam1);
+  double* base = nullptr;
+  int st = scratch_on_current_device(&base);
+  if (st) return st;
+  double* dc = base;
+  double* dp = base + NOALLOC_CELLS;
+  double* df = base + NOALLOC_CELLS + NOALLOC_PRIM;
+  // every staged value is overwritten before it is read: the cells and
+  // primitives by the copies, the fluxes by the kernel (one per face)
+  CK(hipMemcpy(dc, cells, sizeof(double) * 30 * (size_t)nf, hipMemcpyHostToDevice));
+  CK(hipMemcpy(dp, prim, sizeof(double) * 10 * (size_t)nf, hipMemcpyHostToDevice));
+  CK(launch_face_eval(dir, ph, precision, nf, dc, dp, df, nullptr));
+  CK(hipMemcpy(flux, df, sizeof(double) * 5 * (size_t)nf, hipMemcpyDeviceToHost));
+  return CBNS_OK;
+}

@@ -46,7 +46,8 @@ typedef enum cbns_status {
   CBNS_ERR_STATE = -5,    /* call out of order (e.g. stepping before a state is set) */
   CBNS_ERR_IO = -6,       /* file could not be opened, read or written */
   CBNS_ERR_FORMAT = -7,   /* checkpoint dimensions do not match (src/Block3d.cpp:334-338) */
-  CBNS_ERR_NONFINITE = -8 /* a non-finite value met the time-step reduction */
+  CBNS_ERR_NONFINITE = -8, /* a non-finite value met the time-step reduction */
+  CBNS_ERR_CAPACITY = -9   /* request larger than a fixed capacity (the no-allocation entry point's scratch) */
 } cbns_status;
 
 /* SimulationParams (include/simulation_params.h:15-53) as plain data, with the
@@ -194,6 +195,20 @@ int cbns_kernel_times(cbns_ctx* ctx, double ms_total[CBNS_KCLASSES], int64_t lau
  * returns the flux itself (F(Ug) added back in FP64). */
 int cbns_face_flux_eval(int precision, int dir, double gamma, int nfaces,
                         const double* cells, const double* prim, double* flux);
+
+/* cbns_face_flux_eval without any device allocation, for callers that cannot
+ * allocate (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments and
+ * results, bit for bit; the inputs and outputs are staged through one fixed
+ * device array per GPU (CBNS_NOALLOC_MAX_FACES faces, 2.95 MB, held until
+ * the process ends) instead of buffers allocated per call.  nfaces above
+ * CBNS_NOALLOC_MAX_FACES returns CBNS_ERR_CAPACITY.  Every call on a GPU
+ * uses that GPU's one array: calls that overlap on one GPU (several host
+ * threads) race and must be serialised by the caller.  Synchronous: returns
+ * when flux holds the results. */
+#define CBNS_NOALLOC_MAX_FACES 8192
+int cbns_face_flux_eval_noalloc(int precision, int dir, double gamma, int nfaces,
+                                const double* cells, const double* prim, double* flux);
 
 /* ---- batched form ------------------------------------------------------
  * Many independent solves (each with its own grid, physics, precision, step
