@@ -17,6 +17,7 @@ HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "cbns3d_amd.h")
 CBNS_FP64 = 0
 CBNS_FP32_FLUX = 1
 CBNS_FP64_FMA = 2
+CBNS_FP64_FAST = 3
 CBNS_Q = 0
 CBNS_QP = 1
 CBNS_ERR_NOMEM = -3

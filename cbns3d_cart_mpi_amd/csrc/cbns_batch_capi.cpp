@@ -93,6 +93,7 @@ static Phys make_phys(const cbns_params* p) {
   reciprocal_check(ph.dy, &ph.rdy, &ph.fdy);
   reciprocal_check(ph.dz, &ph.rdz, &ph.fdz);
   reciprocal_check(ph.gam1, &ph.rgam1, &ph.fgam1);
+  ph.gg1 = ph.gamma / ph.gam1;
   return ph;
 }
 

@@ -55,6 +55,7 @@ struct cbns_ctx {
   hipEvent_t ev_dt = nullptr;
   bool dt_inflight = false;
   RefState ref{1.0, 0.0, 0.0};  // FP32-flux reference state
+  FixList fix{};                  // faces of the tolerance-bound form left to the fixup pass
   // live kernel timing
   unsigned prof = 0;  // bit k: time the launches of kernel class k
   std::vector<hipEvent_t> ev_pool;
@@ -177,7 +178,7 @@ int cbns_device_count(int* n) {
 int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx** out) {
   if (!p || !dims || !out) return CBNS_ERR_ARG;
   *out = nullptr;
-  if (precision != CBNS_FP64 && precision != CBNS_FP32_FLUX && precision != CBNS_FP64_FMA) return CBNS_ERR_ARG;
+  if ((precision < CBNS_FP64 || precision > CBNS_FP64_FAST)) return CBNS_ERR_ARG;
   const int N[3] = {(int)p->IM, (int)p->JM, (int)p->KM};
   for (int a = 0; a < 3; a++) {
     if (dims[a] < 1 || N[a] < 1) return CBNS_ERR_ARG;
@@ -221,6 +222,7 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
   reciprocal_check(c->ph.dy, &c->ph.rdy, &c->ph.fdy);
   reciprocal_check(c->ph.dz, &c->ph.rdz, &c->ph.fdz);
   reciprocal_check(c->ph.gam1, &c->ph.rgam1, &c->ph.fgam1);
+  c->ph.gg1 = c->ph.gamma / c->ph.gam1;
 
   int st = CBNS_OK;
   c->hb.resize(nb);
@@ -273,6 +275,9 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
   c->diag_parts = GSUM_WG;
   if (!st) st = dev_alloc(c, &q, sizeof(double) * (2 * c->diag_parts + 2));
   c->d_diag = (double*)q;
+  if (!st) st = dev_alloc(c, &q, FIX_BYTES);
+  if (!st && hipMemset(q, 0, 256) != hipSuccess) st = CBNS_ERR_HIP;
+  if (!st) c->fix = fixlist_at(q);
   if (!st && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) st = CBNS_ERR_HIP;
@@ -387,7 +392,9 @@ int cbns_calc_dt(cbns_ctx* c, double* dt) {
 
 int cbns_rec_riemann(cbns_ctx* c, int which, int dir) {
   if (!c || (which != CBNS_Q && which != CBNS_QP) || dir < 0 || dir > 2) return CBNS_ERR_ARG;
-  CK(launch_face_flux(c->db, c->hb.data(), c->dom.nb, dir, which, c->ph, c->precision, c->ref, c->jband, c->face_occ, c->stream));
+  CK(launch_face_flux(c->db, c->hb.data(), c->dom.nb, dir, which, c->ph, c->precision, c->ref, c->jband, c->face_occ,
+                      c->fix, c->stream));
+  if (c->precision == CBNS_FP64_FAST) CK(launch_face_fixup(c->db, c->hb.data(), c->dom.nb, which, c->ph, c->fix, c->stream));
   return CBNS_OK;
 }
 
@@ -573,8 +580,10 @@ static int launch_sweeps(cbns_ctx* c, int stage) {
   for (int d = 0; d < 3; d++)
     CK(timed(c, d, [&] {
       return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->precision, c->ref, c->jband, c->face_occ,
-                              c->stream);
+                              c->fix, c->stream);
     }));
+  // faces of the tolerance-bound form whose range checks failed (one pass for all three sweeps)
+  if (c->precision == CBNS_FP64_FAST) CK(launch_face_fixup(c->db, hb, c->dom.nb, in, c->ph, c->fix, c->stream));
   return CBNS_OK;
 }
 
@@ -756,7 +765,8 @@ int cbns_face_flux_eval(int precision, int dir, double gamma, int nf, const doub
   ph.gamma = gamma;
   ph.gam1 = gamma - 1.0;
   reciprocal_check(ph.gam1, &ph.rgam1, &ph.fgam1);
-  if (precision != CBNS_FP64 && precision != CBNS_FP32_FLUX && precision != CBNS_FP64_FMA) return CBNS_ERR_ARG;
+  ph.gg1 = ph.gamma / ph.gam1;
+  if ((precision < CBNS_FP64 || precision > CBNS_FP64_FAST)) return CBNS_ERR_ARG;
   if (nf == 0) return CBNS_OK;
   double *dc = nullptr, *dp = nullptr, *df = nullptr;
   int st = CBNS_OK;
@@ -816,13 +826,14 @@ static int scratch_on_current_device(double** out) {
 extern "C" int cbns_face_flux_eval_noalloc(int precision, int dir, double gamma, int nf, const double* cells,
                                            const double* prim, double* flux) {
   if (!cells || !prim || !flux || nf < 0 || dir < 0 || dir > 2) return CBNS_ERR_ARG;
-  if (precision != CBNS_FP64 && precision != CBNS_FP32_FLUX && precision != CBNS_FP64_FMA) return CBNS_ERR_ARG;
+  if ((precision < CBNS_FP64 || precision > CBNS_FP64_FAST)) return CBNS_ERR_ARG;
   if (nf > CBNS_NOALLOC_MAX_FACES) return CBNS_ERR_CAPACITY;
   if (nf == 0) return CBNS_OK;
   Phys ph{};
   ph.gamma = gamma;
   ph.gam1 = gamma - 1.0;
   reciprocal_check(ph.gam1, &ph.rgam1, &ph.fgam1);
+  ph.gg1 = ph.gamma / ph.gam1;
   double* base = nullptr;
   int st = scratch_on_current_device(&base);
   if (st) return st;

@@ -24,6 +24,7 @@ struct Phys {
   // the host, cbns_create); rh = RN(1/h), and the flag says whether it holds.
   double rdx, rdy, rdz, rgam1;
   int fdx, fdy, fdz, fgam1;
+  double gg1;  // gamma / (gamma - 1), for the tolerance-bound face form
 };
 
 // One on-device block: the reference's Block3d arrays, SoA.
@@ -70,7 +71,7 @@ struct Launch {
 };
 
 // Arithmetic of the face fluxes (the values of cbns_precision in the C ABI).
-constexpr int PREC_FP64 = 0, PREC_FP32_FLUX = 1, PREC_FP64_FMA = 2;
+constexpr int PREC_FP64 = 0, PREC_FP32_FLUX = 1, PREC_FP64_FMA = 2, PREC_FP64_FAST = 3;
 
 // Work items of a face sweep of block b: faces along x, and along y and z
 // pairs of faces (face_pair_body) or, with pairs = false, single faces.
@@ -80,12 +81,38 @@ inline long long face_items(const BlockDesc& b, int dir, bool pairs = true) {
   return dir == 1 ? ni * ((nj + 1) / 2) * nk : ni * nj * ((nk + 1) / 2);
 }
 
-// kernel launchers (cbns_kernels.hip; launch_face_flux_fma in cbns_kernels_fma.hip)
+// Faces of the tolerance-bound form whose range checks fail: the sweep
+// appends them here (block, axis, face index) and launch_face_fixup
+// recomputes them with IEEE division after the sweeps, so that the sweep
+// kernels carry no fallback code (and its registers).  count and done start
+// at zero; the fixup pass resets them.  More than cap failures send the fixup
+// over every face of every sweep instead.
+struct FixList {
+  unsigned* count;
+  unsigned* done;
+  unsigned long long* ent;  // face index | axis << 40 | block << 42
+  unsigned cap;
+};
+constexpr unsigned FIX_CAP = 1u << 16;
+constexpr size_t FIX_BYTES = 256 + sizeof(unsigned long long) * FIX_CAP;  // counters on their own line, then entries
+
+// kernel launchers (cbns_kernels.hip; launch_face_flux_fma in cbns_kernels_fma.hip,
+// launch_face_flux_fast and launch_face_fixup in cbns_kernels_fast.hip)
+hipError_t launch_face_flux_fast(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
+                                 const Phys& ph, int jb, int occ, const FixList& fx, hipStream_t s);
+// Recompute the faces listed in fx (of the sweeps of state `which` since the
+// last fixup) with the reference-order form; resets fx.
+hipError_t launch_face_fixup(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
+                             const FixList& fx, hipStream_t s);
+FixList fixlist_at(void* buf);
+hipError_t launch_face_eval_fast(int dir, const Phys& ph, int nfaces, const double* cells, double* flux,
+                                 hipStream_t s);
 hipError_t launch_face_flux_fma(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
                                 const Phys& ph, int jb, int occ, hipStream_t s);
 struct RefState;
 hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
-                            const Phys& ph, int precision, const RefState& g, int jb, int occ, hipStream_t s);
+                            const Phys& ph, int precision, const RefState& g, int jb, int occ, const FixList& fx,
+                            hipStream_t s);
 hipError_t launch_primitive(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
                             int precision, hipStream_t s);
 hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,

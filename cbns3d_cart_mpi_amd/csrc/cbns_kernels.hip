@@ -55,8 +55,10 @@ __global__ __launch_bounds__(TPB) void k_face_flux_f32(const BlockDesc* __restri
 }
 
 hipError_t launch_face_flux(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
-                            const Phys& ph, int precision, const RefState& g, int jb, int occ, hipStream_t s) {
+                            const Phys& ph, int precision, const RefState& g, int jb, int occ, const FixList& fx,
+                            hipStream_t s) {
   if (precision == PREC_FP64_FMA) return launch_face_flux_fma(db, hb, nb, dir, which, ph, jb, occ, s);
+  if (precision == PREC_FP64_FAST) return launch_face_flux_fast(db, hb, nb, dir, which, ph, jb, occ, fx, s);
   // work items: pairs of faces along y and z, except the FP64 sweeps at 3
   // waves per SIMD, which take single faces
   const bool single = precision == PREC_FP64 && occ >= 3;
@@ -632,6 +634,7 @@ __global__ __launch_bounds__(TPB) void k_face_eval(Phys ph, int nf, const double
 hipError_t launch_face_eval(int dir, const Phys& ph, int precision, int nf, const double* cells, const double* prim,
                             double* flux, hipStream_t s) {
   if (precision == PREC_FP64_FMA) return launch_face_eval_fma(dir, ph, nf, cells, prim, flux, s);
+  if (precision == PREC_FP64_FAST) return launch_face_eval_fast(dir, ph, nf, cells, flux, s);
   dim3 grid(nblocks_for(nf));
 #define CBNS_EVAL(DIR, F32) hipLaunchKernelGGL((k_face_eval<DIR, F32>), grid, dim3(TPB), 0, s, ph, nf, cells, prim, flux)
   if (precision == PREC_FP32_FLUX) {

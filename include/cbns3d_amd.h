@@ -67,11 +67,18 @@ typedef enum cbns_precision {
   CBNS_FP32_FLUX = 1, /* WENO, characteristic projections and LF flux in FP32 on
                          deviations from a reference state; state, Roe averages,
                          viscous terms, RK3 and reductions in FP64 */
-  CBNS_FP64_FMA = 2   /* the plain path with the face fluxes' a * b + c fused
+  CBNS_FP64_FMA = 2,  /* the plain path with the face fluxes' a * b + c fused
                          into FMAs, quotients and square roots without range
                          scaling: FP64
                          throughout, within a few roundings of CBNS_FP64 per
                          face; single contexts only (not the batched form) */
+  CBNS_FP64_FAST = 3  /* tolerance-bound FP64: the face fluxes' arithmetic
+                         restructured (characteristic projections in factored
+                         form, one division per WENO5 reconstruction, shared
+                         reciprocals, FMA contraction) and formed from the
+                         state alone; FP64 throughout, everything else as
+                         CBNS_FP64; within 1e-12 relative of the reference
+                         after one step */
 } cbns_precision;
 
 typedef struct cbns_ctx cbns_ctx;

@@ -13,6 +13,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+# the restatement built with FMA contraction (a perturbation probe, see Makefile)
+LIB_FMA_PATH = os.path.join(HERE, "liboracle_fma.so")
 REF_POINT_PATH = os.path.join(HERE, "_ref", "libref_point.so")
 REF_BIN_PATH = os.path.join(HERE, "_ref", "cbns3d_ref")
 
@@ -33,15 +35,16 @@ def _p(a):
     return a.ctypes.data_as(_PD)
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
-        L = ctypes.CDLL(LIB_PATH)
+def lib(path=None):
+    """The restatement's library (default), or the one at `path`."""
+    path = path or LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
+        L = ctypes.CDLL(path)
         L.or_params_derive.argtypes = [ctypes.POINTER(Params)]
         L.or_weno5.restype = _D
         L.or_weno5.argtypes = [_D] * 5
@@ -78,8 +81,8 @@ def lib():
         L.or_get_interior.argtypes = [ctypes.c_void_p, _I, _PD]
         L.or_diagnostics.argtypes = [ctypes.c_void_p, _PD, _PD]
         L.or_set_threads.argtypes = [_I]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def make_params(N, Mach=0.1, Re=1600.0, Pr=0.71, gamma=1.4, T_inf=255.556, CFL=0.5,
@@ -97,32 +100,35 @@ def make_params(N, Mach=0.1, Re=1600.0, Pr=0.71, gamma=1.4, T_inf=255.556, CFL=0
 class Block:
     """One oracle block covering the whole periodic domain."""
 
-    def __init__(self, params):
+    def __init__(self, params, contracted=False):
+        """contracted: run on the FMA-contracted build of the restatement
+        (liboracle_fma.so), a one-rounding perturbation probe."""
         self.params = params
-        self._h = lib().or_create(ctypes.byref(params))
+        self._L = lib(LIB_FMA_PATH if contracted else None)
+        self._h = self._L.or_create(ctypes.byref(params))
         d = (_I * 6)()
-        lib().or_dims(self._h, d)
+        self._L.or_dims(self._h, d)
         self.IM, self.JM, self.KM, self.IM_G, self.JM_G, self.KM_G = list(d)
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().or_destroy(h)
+            self._L.or_destroy(h)
             self._h = None
 
     # setup
     def init_tgv(self):
-        lib().or_init_tgv(self._h)
+        self._L.or_init_tgv(self._h)
 
     def set_state(self, Q_ghosted, t_cur, mu0=0.0):
         a = np.ascontiguousarray(Q_ghosted, dtype=np.float64).reshape(-1)
         assert a.size == 5 * self.IM_G * self.JM_G * self.KM_G
-        lib().or_set_state(self._h, _p(a), t_cur, mu0)
+        self._L.or_set_state(self._h, _p(a), t_cur, mu0)
 
     # stepping
     def step(self):
         dt = _D(0.0)
-        fin = lib().or_step(self._h, ctypes.byref(dt))
+        fin = self._L.or_step(self._h, ctypes.byref(dt))
         return None if fin else dt.value
 
     def run(self, nsteps):
@@ -130,56 +136,71 @@ class Block:
 
     @property
     def t_cur(self):
-        return lib().or_t_cur(self._h)
+        return self._L.or_t_cur(self._h)
 
     # method-level access
     def boundary_condition(self, which=0):
-        lib().or_boundary_condition(self._h, which)
+        self._L.or_boundary_condition(self._h, which)
 
     def calc_primitive(self, which=0):
-        lib().or_calc_primitive(self._h, which)
+        self._L.or_calc_primitive(self._h, which)
 
     def calc_dt(self):
-        return lib().or_calc_dt(self._h)
+        return self._L.or_calc_dt(self._h)
 
     def rec_riemann(self, which, d):
-        lib().or_rec_riemann(self._h, which, d)
+        self._L.or_rec_riemann(self._h, which, d)
 
     def calc_viscous_flux_contribution(self):
-        lib().or_calc_viscous_flux_contribution(self._h)
+        self._L.or_calc_viscous_flux_contribution(self._h)
 
     def update_rk3(self, dt, stage):
-        lib().or_update_rk3(self._h, dt, stage)
+        self._L.or_update_rk3(self._h, dt, stage)
 
     # arrays (copies)
     def Q_ghosted(self, which=0):
         n = 5 * self.IM_G * self.JM_G * self.KM_G
-        return np.ctypeslib.as_array(lib().or_Q(self._h, which), shape=(n,)).copy().reshape(
+        return np.ctypeslib.as_array(self._L.or_Q(self._h, which), shape=(n,)).copy().reshape(
             self.KM_G, self.JM_G, self.IM_G, 5)
 
     def Q_interior(self, which=0):
         out = np.empty((self.KM, self.JM, self.IM, 5))
-        lib().or_get_interior(self._h, which, _p(out))
+        self._L.or_get_interior(self._h, which, _p(out))
         return out
 
     def prim(self, field):
         n = self.IM_G * self.JM_G * self.KM_G
-        return np.ctypeslib.as_array(lib().or_prim(self._h, field), shape=(n,)).copy().reshape(
+        return np.ctypeslib.as_array(self._L.or_prim(self._h, field), shape=(n,)).copy().reshape(
             self.KM_G, self.JM_G, self.IM_G)
 
     def face_flux(self, d):
         shp = [(self.KM, self.JM, self.IM + 1), (self.KM, self.JM + 1, self.IM), (self.KM + 1, self.JM, self.IM)][d]
         n = 5 * shp[0] * shp[1] * shp[2]
-        return np.ctypeslib.as_array(lib().or_face_flux(self._h, d), shape=(n,)).copy().reshape(*shp, 5)
+        return np.ctypeslib.as_array(self._L.or_face_flux(self._h, d), shape=(n,)).copy().reshape(*shp, 5)
 
     def dfv(self):
         n = 4 * self.IM * self.JM * self.KM
-        return np.ctypeslib.as_array(lib().or_dfv(self._h), shape=(n,)).copy().reshape(self.KM, self.JM, self.IM, 4)
+        return np.ctypeslib.as_array(self._L.or_dfv(self._h), shape=(n,)).copy().reshape(self.KM, self.JM, self.IM, 4)
 
     def diagnostics(self):
         ke, en = _D(), _D()
-        lib().or_diagnostics(self._h, ctypes.byref(ke), ctypes.byref(en))
+        self._L.or_diagnostics(self._h, ctypes.byref(ke), ctypes.byref(en))
         return ke.value, en.value
+
+
+def rounding_sensitivity(N, steps):
+    """Per-field relative L-inf (each conserved field against its own
+    maximum) between the restatement and its FMA-contracted build after
+    `steps` Taylor-Green steps at N^3: how far the reference's own
+    arithmetic moves under a one-rounding perturbation that keeps every
+    operation's order."""
+    a = Block(make_params(N))
+    b = Block(make_params(N), contracted=True)
+    for x in (a, b):
+        x.init_tgv()
+        x.run(steps)
+    A, B = a.Q_interior(0), b.Q_interior(0)
+    return [float(np.abs(A[..., e] - B[..., e]).max() / np.abs(A[..., e]).max()) for e in range(5)]
 
 
 def set_threads(n):

@@ -270,3 +270,17 @@ def test_turbulence_generator_is_seeded_solenoidal_and_scaled():
     div = sum(np.fft.ifftn(1j * K * np.fft.fftn(u[..., a])) for a, K in enumerate((KX, KY, KZ)))
     assert np.abs(div).max() < 1e-12
     assert np.all(Q[..., 0] == 1.0)
+
+
+def test_reference_arithmetic_moves_rho_w_by_3e12_of_its_own_maximum_in_one_step():
+    """The floor for a form that is not bit-exact: after one Taylor-Green step
+    at 64^3, rho w (zero at t = 0, 4e-4 at most after the step) carries the
+    rounding of fluxes of size p ~ 71, so one rounding fewer per contracted
+    a * b + c moves it by about 3e-12 of its own maximum, while rho, the
+    other momenta and E move by about 1e-15 of theirs."""
+    O.set_threads(8)
+    sens = O.rounding_sensitivity(64, 1)
+    print("reference rounding sensitivity, one step:", ["%.2e" % x for x in sens])
+    assert 1e-12 < sens[3] < 1e-11
+    assert max(sens[0], sens[1], sens[2], sens[4]) < 1e-14
+
