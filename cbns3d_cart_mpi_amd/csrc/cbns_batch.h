@@ -44,7 +44,7 @@ constexpr int BATCH_COUNTERS = 64 * BATCH_CTR_PER_LAUNCH * BATCH_CTR_STRIDE;  //
 
 // Persistent grid of each kernel: its resident workgroups on the device.
 struct BatchGrids {
-  int face[2][3];  // [precision][direction]
+  int face[3][3];  // [form: FP64, FP32-flux, tolerance-bound FP64][direction]
   int visc[2], update[2], ghost, dt;
   int cus;  // compute units of the device
 };
@@ -52,8 +52,12 @@ hipError_t batch_grids(BatchGrids* g);
 
 // wgs: the launch's persistent workgroups; ctr: its NXCD tile counters,
 // zero at launch (batch_launch_reset_counters earlier on the stream)
+// form: 0 FP64, 1 FP32-flux, 2 tolerance-bound FP64 (whose failing faces go to fx)
 hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int dir, int which, int jb,
-                             int precision, int wgs, int* ctr, hipStream_t s);
+                             int form, int wgs, int* ctr, const FixList& fx, hipStream_t s);
+// the fixup pass of the tolerance-bound solves' faces (cbns_kernels_fast.hip)
+hipError_t batch_launch_fixup(const BlockDesc* blocks, const InstDesc* insts, int nb, int which, const FixList& fx,
+                              hipStream_t s);
 hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, int precision,
                              int wgs, int* ctr, hipStream_t s);
 hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int stage, int jb,

@@ -17,6 +17,7 @@
 // single-solve run's.
 #include "cbns_batch.h"
 #include "cbns_bodies.h"
+#include "cbns_face_fast.h"
 
 namespace cbns {
 
@@ -81,6 +82,29 @@ __global__ __launch_bounds__(TPB, 2) void kb_face(const BlockDesc* __restrict__ 
     if (t >= face_count(B, D)) return;
     if (P == 0) face_body<D>(B, t, which, jb, I.ph);
     else face_body_f32<D>(B, t, which, jb, I.ph, I.ref);
+  });
+}
+
+// Tolerance-bound FP64 form: the same sweep bodies as the single-solve
+// kernels (cbns_face_fast.h); faces failing their checks go to fx.  Single
+// faces (x) fit 4 waves per SIMD, pairs (y, z) 3.
+template <int D>
+__global__ __launch_bounds__(TPB, D == 0 ? 4 : 3) void kb_face_fast(const BlockDesc* __restrict__ blocks,
+                                                                   const InstDesc* __restrict__ insts, TileTable tt,
+                                                                   int which, int jb, int* ctr, FixList fx) {
+  for_tiles(tt.n, ctr, [&](int x) {
+    const TileRef tr = tt.tiles[x];
+    const BlockDesc& B = blocks[tr.block];
+    const InstDesc& I = insts[B.inst];
+    if (!I.stepping) return;
+    const int t = tr.tile * TPB + threadIdx.x;
+    if (D > 0) {
+      if (t >= face_pair_count(B, D)) return;
+      face_pair_body_fast<D == 0 ? 1 : D>(B, tr.block, t, which, jb, I.ph, fx);
+      return;
+    }
+    if (t >= face_count(B, D)) return;
+    face_body_fast<D>(B, tr.block, t, which, jb, I.ph, fx);
   });
 }
 
@@ -235,10 +259,14 @@ __global__ void kb_set_budget(InstDesc* insts, const int2* pairs, int n) {
 }
 
 hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int dir, int which, int jb,
-                             int precision, int wgs, int* ctr, hipStream_t s) {
+                             int form, int wgs, int* ctr, const FixList& fx, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
   const dim3 g(wgs), b(TPB);
-  if (precision == 0) {
+  if (form == 2) {
+    if (dir == 0) hipLaunchKernelGGL(kb_face_fast<0>, g, b, 0, s, blocks, insts, tt, which, jb, ctr, fx);
+    else if (dir == 1) hipLaunchKernelGGL(kb_face_fast<1>, g, b, 0, s, blocks, insts, tt, which, jb, ctr, fx);
+    else hipLaunchKernelGGL(kb_face_fast<2>, g, b, 0, s, blocks, insts, tt, which, jb, ctr, fx);
+  } else if (form == 0) {
     if (dir == 0) hipLaunchKernelGGL((kb_face<0, 0>), g, b, 0, s, blocks, insts, tt, which, jb, ctr);
     else if (dir == 1) hipLaunchKernelGGL((kb_face<1, 0>), g, b, 0, s, blocks, insts, tt, which, jb, ctr);
     else hipLaunchKernelGGL((kb_face<2, 0>), g, b, 0, s, blocks, insts, tt, which, jb, ctr);
@@ -329,6 +357,9 @@ hipError_t batch_grids(BatchGrids* g) {
   res((const void*)kb_face<0, 1>, &g->face[1][0]);
   res((const void*)kb_face<1, 1>, &g->face[1][1]);
   res((const void*)kb_face<2, 1>, &g->face[1][2]);
+  res((const void*)kb_face_fast<0>, &g->face[2][0]);
+  res((const void*)kb_face_fast<1>, &g->face[2][1]);
+  res((const void*)kb_face_fast<2>, &g->face[2][2]);
   res((const void*)kb_visc<double>, &g->visc[0]);
   res((const void*)kb_visc<float>, &g->visc[1]);
   res((const void*)kb_update<double>, &g->update[0]);

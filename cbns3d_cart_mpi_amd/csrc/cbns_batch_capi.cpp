@@ -37,7 +37,11 @@ struct Inst {
   int block0 = 0;
 };
 
-enum { T_FACE64 = 0, T_FACE32 = 3, T_VISC64 = 6, T_VISC32, T_UPD64, T_UPD32, T_GHOST, T_DT, T_COUNT };
+enum { T_FACE64 = 0, T_FACE32 = 3, T_FACEF = 6, T_VISC64 = 9, T_VISC32, T_UPD64, T_UPD32, T_GHOST, T_DT, T_COUNT };
+
+// Face-sweep form of a solve: 0 FP64, 1 FP32-flux, 2 tolerance-bound FP64.
+int face_form(int precision) { return precision == CBNS_FP32_FLUX ? 1 : (precision == CBNS_FP64_FAST ? 2 : 0); }
+const int FACE_CLASS[3] = {T_FACE64, T_FACE32, T_FACEF};
 
 }  // namespace
 
@@ -65,6 +69,8 @@ struct cbns_batch {
   int jb = 32;
   BatchGrids grid{};  // resident workgroups of each persistent kernel
   int* d_ctr = nullptr;  // tile counters of the step graph's launches
+  FixList fix{};         // faces of tolerance-bound solves left to the fixup pass
+  bool has_fast = false; // a live tolerance-bound solve (the step graph has fixup passes)
   int wg_mem = 0;     // override of the memory-bound kernels' grid (0: resident)
   int ninst_hw = 0;  // highest live id + 1
 };
@@ -158,12 +164,12 @@ static int rebuild(cbns_batch* b) {
         const int nt = (int)((count + 255) / 256);
         for (int t = 0; t < nt; t++) tab[cls].push_back({bi, t});
       };
-      for (int d = 0; d < 3; d++) add((I.precision ? T_FACE32 : T_FACE64) + d, face_items(B, d));
+      for (int d = 0; d < 3; d++) add(FACE_CLASS[face_form(I.precision)] + d, face_items(B, d));
       const int vunits = ((B.n[0] + 2 * NGV + VX - 1) / VX) * ((B.n[1] + 2 * NGV + VY - 1) / VY) *
                          ((B.n[2] + 2 * NGV + VZ - 1) / VZ);
       for (int t = 0; t < vunits; t++)  // whole-workgroup units
-        tab[I.precision ? T_VISC32 : T_VISC64].push_back({bi, t});
-      add(I.precision ? T_UPD32 : T_UPD64, B.n[0] % 2 == 0 ? B.sn / 2 : B.sn);  // x-pairs for even n0
+        tab[I.precision == CBNS_FP32_FLUX ? T_VISC32 : T_VISC64].push_back({bi, t});
+      add(I.precision == CBNS_FP32_FLUX ? T_UPD32 : T_UPD64, B.n[0] % 2 == 0 ? B.sn / 2 : B.sn);  // x-pairs for even n0
       add(T_GHOST, B.sg - B.sn);  // ghost cells only
       add(T_DT, B.sn);
     }
@@ -172,6 +178,8 @@ static int rebuild(cbns_batch* b) {
                  hipMemcpyHostToDevice));
   }
   b->ninst_hw = hw;
+  b->has_fast = false;
+  for (int id = 0; id < b->cap; id++) b->has_fast = b->has_fast || (b->inst[id].live && b->inst[id].precision == CBNS_FP64_FAST);
   int st;
   if ((st = grow((void**)&b->d_blocks, &b->d_blocks_cap, (int)all.size() + 1, sizeof(BlockDesc)))) return st;
   if (!all.empty())
@@ -200,7 +208,7 @@ static int rebuild(cbns_batch* b) {
   auto tail = [&]() { return b->wg_mem > 0 ? b->wg_mem : 0; };
   // every persistent launch of the step takes its own block of counters, all
   // zeroed by the first node of the graph
-  constexpr int LAUNCHES = 1 + 3 * 11;  // dt; per stage 2 viscous, 6 face, 2 update, 1 ghost
+  constexpr int LAUNCHES = 1 + 3 * 14;  // dt; per stage 2 viscous, 9 face, 2 update, 1 ghost
   constexpr int K = BATCH_CTR_PER_LAUNCH * BATCH_CTR_STRIDE;
   static_assert(LAUNCHES * K <= BATCH_COUNTERS, "counter block");
   int used = 0;
@@ -224,12 +232,12 @@ static int rebuild(cbns_batch* b) {
       e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC32], b->jb, 1, ov ? b->visc_side_wgs : G.visc[1],
                             ctr(), sv);
     if (ov && e == hipSuccess) e = hipEventRecord(b->ev_join, sv);
-    for (int d = 0; d < 3 && e == hipSuccess; d++) {
-      e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE64 + d], d, in, b->jb, 0, G.face[0][d], ctr(), s);
-      if (e == hipSuccess)
-        e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[T_FACE32 + d], d, in, b->jb, 1, G.face[1][d], ctr(),
-                              s);
-    }
+    for (int d = 0; d < 3 && e == hipSuccess; d++)
+      for (int f = 0; f < 3 && e == hipSuccess; f++)
+        e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[FACE_CLASS[f] + d], d, in, b->jb, f, G.face[f][d], ctr(),
+                              b->fix, s);
+    // faces of tolerance-bound solves whose range checks failed
+    if (b->has_fast && e == hipSuccess) e = batch_launch_fixup(b->d_blocks, b->d_insts, (int)all.size(), in, b->fix, s);
     if (ov && e == hipSuccess) e = hipStreamWaitEvent(s, b->ev_join, 0);  // join before the stage tail
     if (e == hipSuccess)
       e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD64], stage, b->jb, 0, tail(), ctr(), s);
@@ -269,6 +277,10 @@ int cbns_batch_create(uint64_t hbm_budget_bytes, int capacity, cbns_batch** out)
       hipMalloc(&b->d_part, sizeof(double) * (2 * GSUM_WG + 2)) != hipSuccess ||
       hipMalloc(&b->d_ctr, sizeof(int) * BATCH_COUNTERS) != hipSuccess)
     st = CBNS_ERR_NOMEM;
+  void* fixbuf = nullptr;
+  if (!st && hipMalloc(&fixbuf, FIX_BYTES) != hipSuccess) st = CBNS_ERR_NOMEM;
+  if (!st && hipMemset(fixbuf, 0, 256) != hipSuccess) st = CBNS_ERR_HIP;
+  if (fixbuf) b->fix = fixlist_at(fixbuf);
   if (!st && hipMemset(b->d_insts, 0, sizeof(InstDesc) * capacity) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && batch_grids(&b->grid) != hipSuccess) st = CBNS_ERR_HIP;
@@ -297,6 +309,7 @@ int cbns_batch_destroy(cbns_batch* b) {
   if (b->d_insts) (void)hipFree(b->d_insts);
   if (b->d_part) (void)hipFree(b->d_part);
   if (b->d_ctr) (void)hipFree(b->d_ctr);
+  if (b->fix.count) (void)hipFree(b->fix.count);
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   if (b->ev_join) (void)hipEventDestroy(b->ev_join);
   if (b->side) (void)hipStreamDestroy(b->side);
@@ -322,7 +335,7 @@ int cbns_batch_memory(const cbns_batch* b, uint64_t* used, uint64_t* budget) {
 int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int precision, const double* Q_aos,
                    double t_cur, double mu0, int steps, int* id_out) {
   if (!b || !p || !dims || !id_out || steps < 0) return CBNS_ERR_ARG;
-  if (precision != CBNS_FP64 && precision != CBNS_FP32_FLUX) return CBNS_ERR_ARG;
+  if (precision != CBNS_FP64 && precision != CBNS_FP32_FLUX && precision != CBNS_FP64_FAST) return CBNS_ERR_ARG;
   const int N[3] = {(int)p->IM, (int)p->JM, (int)p->KM};
   for (int a = 0; a < 3; a++) {
     if (dims[a] < 1 || N[a] < 1) return CBNS_ERR_ARG;

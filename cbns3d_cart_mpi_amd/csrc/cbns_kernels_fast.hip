@@ -1,17 +1,19 @@
 // Face-flux sweeps of the tolerance-bound FP64 form (CBNS_FP64_FAST):
 // rec_riemann_x/y/z (src/Block3d.cpp:555-748) with the restructured face
-// arithmetic of cbns_face_fast.h.  This translation unit is compiled with
-// -ffp-contract=fast.  The sweeps read only the state (the primitives of the
-// two cells beside a face are formed from it); everything else of a step is
-// shared with the plain path.
+// arithmetic of cbns_face_fast.h (its fused multiply-adds written out, so
+// the batched kernels that inline the same bodies form the same values).
+// The sweeps read only the state (the primitives of the two cells beside a
+// face are formed from it); everything else of a step is shared with the
+// plain path.  The fixup pass recomputes a failing face with the plain
+// path's IEEE arithmetic.
 //
 // A face whose range checks fail is not recomputed in the sweep: the sweep
 // appends it to a FixList and a fixup pass after the sweeps recomputes it
 // with the reference-order form (IEEE division and square root).  Keeping
 // that form out of the sweep kernels keeps their registers to the fast
-// path's own: 114 VGPRs (4 waves per SIMD) for single faces and 162 (3
-// waves) for pairs, against 180 and 236 with the recompute inlined.
-#include "cbns_bodies.h"
+// path's own (4 waves per SIMD for single faces, 3 for pairs, against 2
+// with the recompute inlined).
+#include "cbns_batch.h"
 #include "cbns_face_fast.h"
 
 namespace cbns {
@@ -25,80 +27,6 @@ FixList fixlist_at(void* buf) {
   f.ent = (unsigned long long*)((char*)buf + 256);
   f.cap = FIX_CAP;
   return f;
-}
-
-__device__ __forceinline__ void fix_append(const FixList& fx, int b, int D, long long fi) {
-  const unsigned s = atomicAdd(fx.count, 1u);
-  if (s < fx.cap) fx.ent[s] = (unsigned long long)fi | ((unsigned long long)D << 40) | ((unsigned long long)b << 42);
-}
-
-template <int D>
-__device__ __forceinline__ void load_face_cells(const BlockDesc& B, int which, int c0, double (&q)[6][5]) {
-  const int sD = D == 0 ? 1 : (D == 1 ? B.g[0] : B.g[0] * B.g[1]);
-  gptr<const double> __restrict__ Q = gmem<const double>(B.Q[which]);
-  const long long sg = B.sg;
-#pragma unroll
-  for (int cc = 0; cc < 6; cc++)
-#pragma unroll
-    for (int e = 0; e < 5; e++) q[cc][e] = Q[e * sg + c0 + (cc - 3) * sD];
-}
-
-// One face per work item (x sweep): cells i-3 .. i+2 of face (i, j, k).
-template <int D>
-__device__ __forceinline__ void face_body_fast(const BlockDesc& B, int b, int t, int which, int jb, const Phys& ph,
-                                               const FixList& fx) {
-  const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
-  int i, j, k;
-  band_decode(t, ni, nj, nk, jb, i, j, k);
-  const int fi = i + ni * (j + nj * k);
-  double q[6][5], fl[5];
-  load_face_cells<D>(B, which, gidx(B, i, j, k), q);
-  if (face_flux_fast<D>(q, ph.gamma, ph.gam1, ph.gg1, fl)) {
-    gptr<double> __restrict__ F = gmem(B.F[D]);
-#pragma unroll
-    for (int e = 0; e < 5; e++) F[e * B.sf[D] + fi] = fl[e];
-  } else {
-    fix_append(fx, b, D, fi);
-  }
-}
-
-// Two faces adjacent along D per work item (y and z sweeps): faces m and
-// m + 1 (m even) from the seven cells m-3 .. m+3, as face_pair_body.
-template <int D>
-__device__ __forceinline__ void face_pair_body_fast(const BlockDesc& B, int b, int t, int which, int jb,
-                                                    const Phys& ph, const FixList& fx) {
-  static_assert(D == 1 || D == 2, "pairs along y or z");
-  const int ni = B.n[0], nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
-  int i, j, k;
-  pair_decode<D>(B, t, jb, i, j, k);
-  const int c0 = gidx(B, i, j, k);
-  const int sD = D == 1 ? B.g[0] : B.g[0] * B.g[1];
-  gptr<const double> __restrict__ Q = gmem<const double>(B.Q[which]);
-  const long long sg = B.sg;
-  double q7[7][5];
-#pragma unroll
-  for (int cc = 0; cc < 7; cc++)
-#pragma unroll
-    for (int e = 0; e < 5; e++) q7[cc][e] = Q[e * sg + c0 + (cc - 3) * sD];
-  gptr<double> __restrict__ F = gmem(B.F[D]);
-  const long long sf = B.sf[D];
-  const int sF = D == 1 ? ni : ni * nj;
-  const int fi = i + ni * (j + nj * k);
-#pragma unroll
-  for (int h = 0; h < 2; h++) {
-    if (h == 1 && (D == 1 ? j + 1 >= nj : k + 1 >= nk)) break;
-    double qq[6][5], fl[5];
-#pragma unroll
-    for (int cc = 0; cc < 6; cc++)
-#pragma unroll
-      for (int e = 0; e < 5; e++) qq[cc][e] = q7[cc + h][e];
-    if (face_flux_fast<D>(qq, ph.gamma, ph.gam1, ph.gg1, fl)) {
-#pragma unroll
-      for (int e = 0; e < 5; e++) F[e * sf + fi + h * sF] = fl[e];
-    } else {
-      fix_append(fx, b, D, fi + h * sF);
-    }
-  }
 }
 
 // OCC: waves per SIMD the register allocation must allow (4 for single
@@ -175,7 +103,12 @@ __device__ __forceinline__ void fixup_all(const BlockDesc& B, int which, const P
 
 constexpr int FIX_WG = 64;  // workgroups of the fixup pass
 
-__global__ __launch_bounds__(TPB) void k_face_fixup(const BlockDesc* __restrict__ blocks, int nb, int which, Phys ph,
+// BATCH: blocks of many solves, each with its own physics (insts); the
+// single-solve form passes ph.  In the pass over every face, only blocks of
+// stepping tolerance-bound solves are checked.
+template <bool BATCH>
+__global__ __launch_bounds__(TPB) void k_face_fixup(const BlockDesc* __restrict__ blocks,
+                                                    const InstDesc* __restrict__ insts, int nb, int which, Phys ph,
                                                     FixList fx) {
   const unsigned n = *fx.count;  // every workgroup reads it before the last one resets it (below)
   const long long first = blockIdx.x * (long long)TPB + threadIdx.x, stride = (long long)gridDim.x * TPB;
@@ -184,15 +117,19 @@ __global__ __launch_bounds__(TPB) void k_face_fixup(const BlockDesc* __restrict_
       const unsigned long long v = fx.ent[e];
       const long long fi = (long long)(v & ((1ull << 40) - 1));
       const int D = (int)((v >> 40) & 3), b = (int)(v >> 42);
-      if (D == 0) fixup_face<0>(blocks[b], which, fi, ph);
-      else if (D == 1) fixup_face<1>(blocks[b], which, fi, ph);
-      else fixup_face<2>(blocks[b], which, fi, ph);
+      const Phys& P = BATCH ? insts[blocks[b].inst].ph : ph;
+      if (D == 0) fixup_face<0>(blocks[b], which, fi, P);
+      else if (D == 1) fixup_face<1>(blocks[b], which, fi, P);
+      else fixup_face<2>(blocks[b], which, fi, P);
     }
   } else {
     for (int b = 0; b < nb; b++) {
-      fixup_all<0>(blocks[b], which, ph, first, stride);
-      fixup_all<1>(blocks[b], which, ph, first, stride);
-      fixup_all<2>(blocks[b], which, ph, first, stride);
+      const BlockDesc& B = blocks[b];
+      if (BATCH && (!insts[B.inst].stepping || insts[B.inst].precision != PREC_FP64_FAST)) continue;
+      const Phys& P = BATCH ? insts[B.inst].ph : ph;
+      fixup_all<0>(B, which, P, first, stride);
+      fixup_all<1>(B, which, P, first, stride);
+      fixup_all<2>(B, which, P, first, stride);
     }
   }
   // the last workgroup to finish clears the list for the next stage
@@ -209,7 +146,14 @@ __global__ __launch_bounds__(TPB) void k_face_fixup(const BlockDesc* __restrict_
 hipError_t launch_face_fixup(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
                              const FixList& fx, hipStream_t s) {
   (void)hb;
-  hipLaunchKernelGGL(k_face_fixup, dim3(FIX_WG), dim3(TPB), 0, s, db, nb, which, ph, fx);
+  hipLaunchKernelGGL(k_face_fixup<false>, dim3(FIX_WG), dim3(TPB), 0, s, db, (const InstDesc*)nullptr, nb, which, ph,
+                     fx);
+  return hipGetLastError();
+}
+
+hipError_t batch_launch_fixup(const BlockDesc* blocks, const InstDesc* insts, int nb, int which, const FixList& fx,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_face_fixup<true>, dim3(FIX_WG), dim3(TPB), 0, s, blocks, insts, nb, which, Phys{}, fx);
   return hipGetLastError();
 }
 

@@ -202,3 +202,38 @@ def test_non_finite_solve_stops_alone():
         assert st["nonfinite"] and not st["active"] and st["steps_done"] < 5
         Q, t = single(24, 5)
         assert b.status(good)["t_cur"] == t and np.array_equal(b.get_state(good), Q)
+
+
+def test_tolerance_bound_fp64_solves_in_a_mix_equal_their_single_runs():
+    """CBNS_FP64_FAST solves batched with FP64 and FP32-flux solves give their
+    single-solve results bit for bit (the same sweep bodies, contraction fixed
+    per function), including a solve scaled by 1e50 whose every face goes
+    through the batch's fixup pass, an odd x-width and a split solve."""
+    from oracle import oracle as O
+
+    ob = O.Block(O.make_params(12))
+    ob.init_tgv()
+    Qs = ob.Q_interior(0) * 1e50
+    cases = {
+        "fast": (C.make_params(32, Mach=0.3, Re=400.0), (1, 1, 1), None, 6),
+        "fast_split": (C.make_params((24, 20, 16), Mach=0.2, Re=900.0), (2, 1, 2), None, 5),
+        "fast_odd": (C.make_params((21, 16, 12), Mach=0.4, Re=300.0), (1, 1, 1), None, 4),
+        "fast_scaled": (C.make_params(12), (1, 1, 1), Qs, 3),
+    }
+    with C.Batch(32 * GB) as b:
+        b.add(C.make_params(24), steps=8)
+        b.add(C.make_params(16, Mach=0.5), precision=C.CBNS_FP32_FLUX, steps=8)
+        ids = {k: b.add(p, dims=d, precision=C.CBNS_FP64_FAST, Q=Q, steps=n) for k, (p, d, Q, n) in cases.items()}
+        b.run(8)
+        for k, (p, d, Q, n) in cases.items():
+            with C.Solver(p, d, precision=C.CBNS_FP64_FAST) as s:
+                if Q is None:
+                    s.init_tgv()
+                else:
+                    s.set_state(Q, 0.0, mu0=0.0)
+                s.run(n)
+                st = b.status(ids[k])
+                assert st["steps_done"] == n and st["t_cur"] == s.t_cur, k
+                got = b.get_state(ids[k])
+                assert np.isfinite(got).all(), k
+                assert np.array_equal(got, s.get_state()), k
