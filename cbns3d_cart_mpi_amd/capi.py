@@ -93,6 +93,9 @@ _SIGS = {
     "cbns_batch_run": (_I, [_VP, _I]),
     "cbns_batch_set_budgets": (_I, [_VP, _PI, _PI, _I]),
     "cbns_batch_sync": (_I, [_VP]),
+    "cbns_batch_query": (_I, [_VP, _PI]),
+    "cbns_batch_pooled": (_I, [_VP, ctypes.POINTER(ctypes.c_uint64)]),
+    "cbns_batch_trim": (_I, [_VP]),
     "cbns_batch_set_workgroups": (_I, [_VP, _I]),
     "cbns_batch_set_overlap": (_I, [_VP, _I]),
     "cbns_batch_set_band": (_I, [_VP, _I]),
@@ -407,6 +410,23 @@ class Batch:
 
     def sync(self):
         check(lib().cbns_batch_sync(self._h), "cbns_batch_sync")
+
+    def busy(self):
+        """Whether the last run's steps are still on the device (no wait)."""
+        v = _I()
+        check(lib().cbns_batch_query(self._h, ctypes.byref(v)), "cbns_batch_query")
+        return bool(v.value)
+
+    @property
+    def pooled_bytes(self):
+        """Device bytes kept from removed solves for reuse."""
+        v = ctypes.c_uint64()
+        check(lib().cbns_batch_pooled(self._h, ctypes.byref(v)), "cbns_batch_pooled")
+        return v.value
+
+    def trim(self):
+        """Release the pooled arrays of removed solves."""
+        check(lib().cbns_batch_trim(self._h), "cbns_batch_trim")
 
     def set_budgets(self, budgets):
         """{id: steps} granted until the next call."""

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <map>
 #include <new>
 #include <vector>
 
@@ -32,7 +33,7 @@ struct Inst {
   Domain dom{};
   std::vector<BlockDesc> blocks;  // block0-relative (instance-local) descriptors
   BlockDesc* dblocks = nullptr;   // instance-local device copy (setup, gather)
-  std::vector<void*> allocs;
+  std::vector<std::pair<void*, size_t>> allocs;
   uint64_t bytes = 0;
   int block0 = 0;
 };
@@ -70,6 +71,15 @@ struct cbns_batch {
   BatchGrids grid{};  // resident workgroups of each persistent kernel
   int* d_ctr = nullptr;  // tile counters of the step graph's launches
   FixList fix{};         // faces of tolerance-bound solves left to the fixup pass
+  // Arrays of removed solves, kept for reuse by later solves of the same
+  // shapes (by size): hipFree synchronises the whole device, which would stall
+  // every other stream of a server between its chunks.  Released when fresh
+  // memory would take used + pooled past the budget, and at destroy.
+  std::map<size_t, std::vector<void*>> pool;
+  uint64_t pooled = 0;
+  double* d_stage = nullptr;  // grow-only staging of states and tables (add, get_state)
+  size_t stage_cap = 0;
+  hipEvent_t ev_run = nullptr;  // recorded after the last run's graph launches
   bool has_fast = false; // a live tolerance-bound solve (the step graph has fixup passes)
   int wg_mem = 0;     // override of the memory-bound kernels' grid (0: resident)
   int ninst_hw = 0;  // highest live id + 1
@@ -125,12 +135,52 @@ static uint64_t solve_bytes(const cbns_params* p, const int dims[3]) {
   return total + sizeof(BlockDesc) * dims[0] * dims[1] * dims[2];
 }
 
-static void free_inst(Inst& I) {
-  for (void* p : I.allocs) (void)hipFree(p);
+// A solve's arrays go to the batch's pool (see cbns_batch::pool).
+static void free_inst(cbns_batch* b, Inst& I) {
+  for (auto& a : I.allocs) {
+    b->pool[a.second].push_back(a.first);
+    b->pooled += a.second;
+  }
   I.allocs.clear();
   I.blocks.clear();
   I.dblocks = nullptr;
   I.live = false;
+}
+
+static void drain_pool(cbns_batch* b) {
+  for (auto& kv : b->pool)
+    for (void* p : kv.second) (void)hipFree(p);
+  b->pool.clear();
+  b->pooled = 0;
+}
+
+// Device memory for a solve: a pooled array of the same size, else a fresh
+// one (after releasing the pool if it would take the batch past its budget).
+static void* batch_alloc(cbns_batch* b, size_t bytes, uint64_t need) {
+  auto it = b->pool.find(bytes);
+  if (it != b->pool.end() && !it->second.empty()) {
+    void* p = it->second.back();
+    it->second.pop_back();
+    b->pooled -= bytes;
+    return p;
+  }
+  if (b->used + b->pooled + need > b->budget) drain_pool(b);
+  void* q = nullptr;
+  if (hipMalloc(&q, bytes) != hipSuccess) return nullptr;
+  return q;
+}
+
+// Grow-only staging buffer of at least `bytes`.
+static int stage(cbns_batch* b, size_t bytes, double** out) {
+  if (bytes > b->stage_cap) {
+    if (b->d_stage) (void)hipFree(b->d_stage);
+    b->d_stage = nullptr;
+    b->stage_cap = 0;
+    if (hipMalloc(&b->d_stage, bytes) != hipSuccess) return CBNS_ERR_NOMEM;
+    b->stage_cap = bytes;
+  }
+  *out = b->d_stage;
+  return CBNS_OK;
 }
 
 static int grow(void** p, int* cap, int need, size_t elem) {
@@ -287,7 +337,8 @@ int cbns_batch_create(uint64_t hbm_budget_bytes, int capacity, cbns_batch** out)
   if (!st) b->visc_side_wgs = b->grid.cus;
   if (!st && (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
               hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
-              hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess))
+              hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&b->ev_run, hipEventDisableTiming) != hipSuccess))
     st = CBNS_ERR_HIP;
   if (st) {
     cbns_batch_destroy(b);
@@ -302,7 +353,10 @@ int cbns_batch_destroy(cbns_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  for (Inst& I : b->inst) free_inst(I);
+  for (Inst& I : b->inst) free_inst(b, I);
+  drain_pool(b);
+  if (b->d_stage) (void)hipFree(b->d_stage);
+  if (b->ev_run) (void)hipEventDestroy(b->ev_run);
   for (int c = 0; c < T_COUNT; c++)
     if (b->d_tiles[c]) (void)hipFree(b->d_tiles[c]);
   if (b->d_blocks) (void)hipFree(b->d_blocks);
@@ -364,9 +418,8 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
   }
   I.dom.nb = dims[0] * dims[1] * dims[2];
   auto alloc = [&](size_t bytes) -> double* {
-    void* q = nullptr;
-    if (hipMalloc(&q, bytes) != hipSuccess) return nullptr;
-    I.allocs.push_back(q);
+    void* q = batch_alloc(b, bytes, need);
+    if (q) I.allocs.push_back({q, bytes});
     return (double*)q;
   };
   bool ok = true;
@@ -396,7 +449,7 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
       }
   if (ok) ok = (I.dblocks = (BlockDesc*)alloc(sizeof(BlockDesc) * I.blocks.size())) != nullptr;
   if (!ok) {
-    free_inst(I);
+    free_inst(b, I);
     return CBNS_ERR_NOMEM;
   }
   I.bytes = need;
@@ -420,23 +473,21 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
         tab[(3 * a + 2) * ts + i] = std::cos(2.0 * xc);
       }
     double* dtab = nullptr;
-    if (hipMalloc(&dtab, sizeof(double) * tab.size()) != hipSuccess) st = CBNS_ERR_NOMEM;
+    st = stage(b, sizeof(double) * tab.size(), &dtab);
     if (!st && (hipMemcpy(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess ||
                 launch_tgv(I.dblocks, I.blocks.data(), nb, dtab, ts, ph, p->p_inf, s) != hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess))
       st = CBNS_ERR_HIP;
-    if (dtab) (void)hipFree(dtab);
     t_cur = p->t_cur;
   } else if (!st) {
     const size_t n = (size_t)NEQ * N[0] * N[1] * N[2];
     double* d = nullptr;
-    if (hipMalloc(&d, n * sizeof(double)) != hipSuccess) st = CBNS_ERR_NOMEM;
+    st = stage(b, n * sizeof(double), &d);
     if (!st && (hipMemcpy(d, Q_aos, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
                 launch_scatter_aos(I.dblocks, I.blocks.data(), I.dom, CBNS_Q, d, s) != hipSuccess ||
                 launch_fill_mu(I.dblocks, I.blocks.data(), nb, mu0, s) != hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess))
       st = CBNS_ERR_HIP;
-    if (d) (void)hipFree(d);
   }
   // boundary_condition, calc_primitive, init_Q_p (src/cbns3d.cpp:149-153)
   if (!st && (launch_ghost_fill(I.dblocks, I.blocks.data(), I.dom, CBNS_Q, s) != hipSuccess ||
@@ -459,7 +510,7 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
     D.ref.p = ph.gam1 * D.ref.E;
   }
   if (st) {
-    free_inst(I);
+    free_inst(b, I);
     return st;
   }
   D.dom = I.dom;
@@ -473,7 +524,7 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
   D.t_cur = t_cur;
   D.t_end = p->t_end;
   if (hipMemcpy(b->d_insts + id, &D, sizeof(D), hipMemcpyHostToDevice) != hipSuccess) {
-    free_inst(I);
+    free_inst(b, I);
     return CBNS_ERR_HIP;
   }
   I.live = true;
@@ -489,7 +540,7 @@ int cbns_batch_remove(cbns_batch* b, int id) {
   int zero = 0;
   CK(hipMemcpy((char*)(b->d_insts + id) + offsetof(InstDesc, active), &zero, sizeof(int), hipMemcpyHostToDevice));
   b->used -= b->inst[id].bytes;
-  free_inst(b->inst[id]);
+  free_inst(b, b->inst[id]);
   b->dirty = true;
   return CBNS_OK;
 }
@@ -502,6 +553,29 @@ int cbns_batch_run(cbns_batch* b, int nsteps) {
   }
   if (!b->exec) return CBNS_OK;
   for (int n = 0; n < nsteps; n++) CK(hipGraphLaunch(b->exec, b->stream));
+  CK(hipEventRecord(b->ev_run, b->stream));
+  return CBNS_OK;
+}
+
+int cbns_batch_pooled(const cbns_batch* b, uint64_t* bytes) {
+  if (!b || !bytes) return CBNS_ERR_ARG;
+  *bytes = b->pooled;
+  return CBNS_OK;
+}
+
+int cbns_batch_trim(cbns_batch* b) {
+  if (!b) return CBNS_ERR_ARG;
+  CK(hipStreamSynchronize(b->stream));
+  drain_pool(b);
+  return CBNS_OK;
+}
+
+int cbns_batch_query(cbns_batch* b, int* busy) {
+  if (!b || !busy) return CBNS_ERR_ARG;
+  const hipError_t e = hipEventQuery(b->ev_run);
+  if (e == hipSuccess) *busy = 0;
+  else if (e == hipErrorNotReady) *busy = 1;
+  else return CBNS_ERR_HIP;
   return CBNS_OK;
 }
 
@@ -566,10 +640,13 @@ int cbns_batch_status(cbns_batch* b, int id, int* active, int* steps_done, doubl
 int cbns_batch_poll(cbns_batch* b, int* ids, int max_ids, int* n_done, int* n_live) {
   if (!b || (!ids && max_ids > 0)) return CBNS_ERR_ARG;
   CK(hipStreamSynchronize(b->stream));
-  std::vector<InstDesc> D(b->cap);
-  CK(hipMemcpy(D.data(), b->d_insts, sizeof(InstDesc) * b->cap, hipMemcpyDeviceToHost));
+  int hw = 0;  // highest live id + 1 (only those descriptors are read back)
+  for (int id = 0; id < b->cap; id++)
+    if (b->inst[id].live) hw = id + 1;
+  std::vector<InstDesc> D(hw);
+  if (hw) CK(hipMemcpy(D.data(), b->d_insts, sizeof(InstDesc) * hw, hipMemcpyDeviceToHost));
   int nd = 0, nl = 0;
-  for (int id = 0; id < b->cap; id++) {
+  for (int id = 0; id < hw; id++) {
     if (!b->inst[id].live) continue;
     nl++;
     const bool done = !D[id].active || D[id].steps_done >= D[id].steps_target;
@@ -589,13 +666,12 @@ int cbns_batch_get_state(cbns_batch* b, int id, double* Q_aos) {
   const size_t n = (size_t)NEQ * I.dom.N[0] * I.dom.N[1] * I.dom.N[2];
   CK(hipStreamSynchronize(b->stream));
   double* d = nullptr;
-  CK(hipMalloc(&d, n * sizeof(double)));
-  int st = CBNS_OK;
+  int st = stage(b, n * sizeof(double), &d);
+  if (st) return st;
   if (launch_gather_aos(I.dblocks, I.blocks.data(), I.dom, CBNS_Q, d, b->stream) != hipSuccess ||
       hipStreamSynchronize(b->stream) != hipSuccess ||
       hipMemcpy(Q_aos, d, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
     st = CBNS_ERR_HIP;
-  (void)hipFree(d);
   return st;
 }
 

@@ -1,7 +1,8 @@
 """Parameter-sweep solves: a batched sweep and a continuous-batching server.
 
-    python -m cbns3d_cart_mpi_amd.serve sweep  [--n 64] [--steps 300] [--precision fp64|fp32]
+    python -m cbns3d_cart_mpi_amd.serve sweep  [--n 64] [--steps 300] [--precision fp64|fp32|alt]
     python -m cbns3d_cart_mpi_amd.serve serve  [--n 128] [--steps-min 20] [--steps-max 200] [--rate R]
+                                               [--baseline measured|model|none] [--fp64-form fast|exact]
 
 Requests are drawn from a seeded stream:
   * grid N^3 from a set of sizes (sweep: {32,48,64,96,128}, uniform; serving:
@@ -17,11 +18,14 @@ Serving is continuous batching: requests arrive on a Poisson clock; each is
 admitted as soon as its state fits the HBM budget, stays resident on the
 device, and advances with the other live solves of its size class in the
 same persistent launches (one hipGraph per step per class), the classes
-running concurrently; completed solves are collected and removed between
-graph-replay chunks, freeing room for waiting requests.
-Time-to-completion is measured from arrival to the host having the result.
-The baseline processes the same requests one at a time, first come first
-served, as plain FP64 single solves.
+running concurrently; a class is serviced (finished solves collected and
+removed, waiting requests admitted, its next chunk issued) whenever its last
+chunk has finished, while the other classes keep running.  FP64 requests
+are served in the tolerance-bound FP64 form by default (--fp64-form exact
+for the bit-exact one).  Time-to-completion is measured from arrival to the
+host having the result.  The baseline processes the same requests one at a
+time, first come first served, as plain (bit-exact) FP64 single solves,
+measured request by request or from a per-size cost model.
 """
 import argparse
 import json
@@ -55,6 +59,7 @@ class Request:
     lane: int = -1       # size class it was served in
     bid: int = -1        # instance id in that class's batch
     ran: int = 0         # steps issued to it so far
+    nonfinite: bool = False  # it met a non-finite value (the result is the failed state)
     payload: object = field(default=None, repr=False)
     result: object = field(default=None, repr=False)
 
@@ -134,19 +139,30 @@ def run_sweep(reqs, budget=200 * GB, collect=False, workgroups=None):
 LANES = ((0, 64), (65, 128), (129, 1 << 30))  # size classes by grid edge N
 
 
-def serve(reqs, budget=200 * GB, chunk=4, capacity=1024, collect=True, lanes=LANES, max_live=None, log=None):
+def _form(r, fp64_form):
+    """Arithmetic a request is solved in: FP32-flux as asked, FP64 requests
+    in fp64_form (CBNS_FP64: bit-exact with the reference; CBNS_FP64_FAST:
+    tolerance-bound, within 1e-12 of it)."""
+    return r.precision if r.precision == C.CBNS_FP32_FLUX else fp64_form
+
+
+def serve(reqs, budget=200 * GB, chunk_ms=40.0, capacity=256, collect=True, lanes=LANES, max_live=None, log=None,
+          fp64_form=C.CBNS_FP64_FAST, rate_guess=1.2e9, chunk=None):
     """Continuous batching over a Poisson request stream (arrival times in s).
 
     Solves are grouped into size-class lanes, each a device batch with its own
     stream and step graph, so a small solve never advances in lockstep with a
-    large one.  Between chunks the server admits waiting requests (first come,
-    first admitted, while the shared HBM budget allows), then issues every
-    lane's chunk asynchronously so the lanes run concurrently on the GPU: the
-    lane holding the largest solve replays `chunk` steps and a lane whose
-    largest solve has n cells replays about chunk * n_max / n steps.  Finished
-    solves are collected and removed after the chunk, freeing memory.
-    max_live caps the number of solves resident at once (None: memory only);
-    log, if given, is called with each finished request."""
+    large one, and the lanes run concurrently on the GPU.  The host services
+    a lane only when its last chunk has finished (cbns_batch_query, no wait):
+    it collects the solves that are done (their results come back to the
+    host and their memory is kept for reuse), admits waiting requests of that
+    lane (first come first admitted, while the shared HBM budget and the
+    optional cap on resident solves allow), and issues the lane's next chunk
+    of steps, sized to about chunk_ms of device time but never past the step
+    at which its next solve finishes.  The other lanes keep running
+    meanwhile.  Time-to-completion runs from arrival to the result on the
+    host.  log, if given, is called with each finished request; chunk, if
+    given, fixes the steps per chunk instead."""
     for r in reqs:
         r.payload = prepare(r)
     pending = sorted(reqs, key=lambda r: r.arrival)
@@ -155,7 +171,8 @@ def serve(reqs, budget=200 * GB, chunk=4, capacity=1024, collect=True, lanes=LAN
     batches = [C.Batch(budget, capacity=capacity) for _ in lanes]
     live = [dict() for _ in lanes]
     used = [0]
-    host = {"admit": 0.0, "issue": 0.0, "poll": 0.0, "collect": 0.0}
+    host = {"admit": 0.0, "issue": 0.0, "collect": 0.0, "idle": 0.0}
+    lane_of = lambda r: next(n for n, (lo, hi) in enumerate(lanes) if lo <= r.N <= hi)
     try:
         t0 = time.perf_counter()
         k = 0
@@ -164,59 +181,69 @@ def serve(reqs, budget=200 * GB, chunk=4, capacity=1024, collect=True, lanes=LAN
             while k < len(pending) and pending[k].arrival <= now:
                 waiting.append(pending[k])
                 k += 1
-            ta = time.perf_counter()
-            while waiting:  # first come, first admitted, while memory allows
-                r = waiting[0]
-                p, Q = r.payload
-                need = C.batch_bytes_needed(p)
-                nlive = sum(len(x) for x in live)
-                if max_live is not None and nlive >= max_live:
-                    break
-                if used[0] + need > budget:
-                    if not nlive:
-                        raise RuntimeError(f"request {r.rid} does not fit the budget alone")
-                    break
-                li = next(n for n, (lo, hi) in enumerate(lanes) if lo <= r.N <= hi)
-                i = batches[li].add(p, precision=r.precision, Q=Q, t_cur=0.0, mu0=0.0, steps=r.steps)
-                used[0] += need
-                r.admitted = time.perf_counter() - t0
-                r.lane, r.bid, r.ran = li, i, 0
-                live[li][i] = r
-                waiting.pop(0)
-            tb = time.perf_counter()
-            host["admit"] += tb - ta
-            active = [li for li in range(len(lanes)) if live[li]]
-            if not active:
-                if k < len(pending):
+            serviced = False
+            for li, b in enumerate(batches):
+                if live[li] and b.busy():
+                    continue
+                # lane idle: collect its finished solves
+                tc = time.perf_counter()
+                if live[li]:
+                    fin, _ = b.poll()
+                    for i in fin:
+                        r = live[li].pop(i)
+                        st = b.status(i)
+                        r.t_final = st["t_cur"]
+                        r.nonfinite = bool(st["nonfinite"])
+                        if collect:
+                            r.result = b.get_state(i)
+                        r.done = time.perf_counter() - t0
+                        b.remove(i)
+                        used[0] -= C.batch_bytes_needed(r.payload[0])
+                        done.append(r)
+                        if log:
+                            log(r)
+                ta = time.perf_counter()
+                host["collect"] += ta - tc
+                # admit this lane's waiting requests, first come first admitted
+                for r in [w for w in waiting if lane_of(w) == li]:
+                    p, Q = r.payload
+                    need = C.batch_bytes_needed(p)
+                    nlive = sum(len(x) for x in live)
+                    if max_live is not None and nlive >= max_live:
+                        break
+                    if used[0] + need > budget:
+                        if not nlive:
+                            raise RuntimeError(f"request {r.rid} does not fit the budget alone")
+                        break
+                    if used[0] + need + sum(x.pooled_bytes for x in batches) > budget:
+                        for x in batches:  # memory kept for reuse by other lanes goes first
+                            if x is not b and not x.busy():
+                                x.trim()
+                    i = b.add(p, precision=_form(r, fp64_form), Q=Q, t_cur=0.0, mu0=0.0, steps=r.steps)
+                    used[0] += need
+                    r.admitted = time.perf_counter() - t0
+                    r.lane, r.bid, r.ran = li, i, 0
+                    live[li][i] = r
+                    waiting.remove(r)
+                tb = time.perf_counter()
+                host["admit"] += tb - ta
+                if live[li]:  # next chunk: about chunk_ms, never past the lane's next completion
+                    cells = sum(r.N ** 3 for r in live[li].values())
+                    left = min(r.steps - r.ran for r in live[li].values())
+                    want = chunk if chunk else int(chunk_ms * 1e-3 / (cells / rate_guess + 1.5e-4))
+                    nrep = max(1, min(left, want))
+                    b.run(nrep)
+                    for r in live[li].values():
+                        r.ran += nrep
+                    serviced = True
+                host["issue"] += time.perf_counter() - tb
+            if not serviced:
+                ti = time.perf_counter()
+                if not any(live) and k < len(pending):
                     time.sleep(max(0.0, pending[k].arrival - (time.perf_counter() - t0)))
-                continue
-            n_all = max(r.N ** 3 for li in active for r in live[li].values())
-            for li in active:  # asynchronous: the lanes overlap on the device
-                n_lane = max(r.N ** 3 for r in live[li].values())
-                left = max(r.steps - r.ran for r in live[li].values())
-                nrep = max(1, min(left, (chunk * n_all) // n_lane))
-                batches[li].run(nrep)
-                for r in live[li].values():
-                    r.ran += nrep
-            tc = time.perf_counter()
-            host["issue"] += tc - tb
-            for li in active:
-                td = time.perf_counter()
-                fin, _ = batches[li].poll()
-                te = time.perf_counter()
-                host["poll"] += te - td
-                for i in fin:
-                    r = live[li].pop(i)
-                    r.t_final = batches[li].status(i)["t_cur"]
-                    if collect:
-                        r.result = batches[li].get_state(i)
-                    r.done = time.perf_counter() - t0
-                    batches[li].remove(i)
-                    used[0] -= C.batch_bytes_needed(r.payload[0])
-                    done.append(r)
-                    if log:
-                        log(r)
-                host["collect"] += time.perf_counter() - te
+                else:
+                    time.sleep(2e-4)
+                host["idle"] += time.perf_counter() - ti
         el = time.perf_counter() - t0
     finally:
         for b in batches:
@@ -225,7 +252,26 @@ def serve(reqs, budget=200 * GB, chunk=4, capacity=1024, collect=True, lanes=LAN
     work = sum(r.work for r in reqs)
     return {"wall_s": el, "cell_steps": work, "mcell_steps_per_s": work / el / 1e6,
             "p50_s": percentile(lat, 50), "p99_s": percentile(lat, 99), "mean_s": float(np.mean(lat)),
+            "failed": sum(bool(getattr(r, "nonfinite", False)) for r in reqs),
             "host_s": {k: round(v, 4) for k, v in host.items()}}
+
+
+def size_lanes(reqs):
+    """One lane per grid size present in reqs."""
+    return tuple((N, N) for N in sorted({r.N for r in reqs}))
+
+
+def run_sweep_lanes(reqs, budget=200 * GB, collect=False, fp64_form=C.CBNS_FP64, chunk_ms=40.0, lanes=None):
+    """A parameter sweep: every request admitted at time 0 into its lane (by
+    default one lane per grid size), the lanes advancing concurrently;
+    per-solve completion times from the start (small solves finish first)."""
+    for r in reqs:
+        r.arrival = 0.0
+    res = serve(reqs, budget, chunk_ms=chunk_ms, collect=collect, fp64_form=fp64_form,
+                lanes=lanes or size_lanes(reqs))
+    bad = [r.rid for r in reqs if getattr(r, "nonfinite", False)]
+    assert not bad, f"solves turned non-finite: {bad}"
+    return res
 
 
 def sequential_fp64(reqs, precision=C.CBNS_FP64, log=None):
@@ -257,6 +303,50 @@ def sequential_fp64(reqs, precision=C.CBNS_FP64, log=None):
             "p99_s": percentile(lat, 99), "mean_s": float(np.mean(lat)), "makespan_s": clock}
 
 
+def sequential_model(reqs, precision=C.CBNS_FP64, probe_steps=20, log=None):
+    """The sequential baseline without running every request: for each grid
+    size in the stream, one single solve measures its fixed cost (create,
+    initial state, result read-back, destroy) and its time per step (over
+    probe_steps steps after two untimed ones); a request's service time is
+    fixed + steps x per-step, and the same first-come-first-served queue as
+    sequential_fp64 gives the latencies.  The step cost of a size does not
+    depend on the physics or the initial field, so the model is exact up to
+    run-to-run noise (bench.py's serving leg measures both on the same
+    requests)."""
+    costs = {}
+    for N in sorted({r.N for r in reqs}):
+        p = C.make_params(N)
+        t0 = time.perf_counter()
+        with C.Solver(p, precision=precision) as s:
+            s.init_tgv()
+            s.run(2)
+            s.sync()
+            t1 = time.perf_counter()
+            s.run(probe_steps)
+            t2 = time.perf_counter()
+            s.get_state()
+        t3 = time.perf_counter()
+        per = (t2 - t1) / probe_steps
+        costs[N] = ((t1 - t0) - 2 * per + (t3 - t2), per)
+        if log:
+            log(f"sequential cost N={N}: fixed {costs[N][0]:.4f} s, per step {per * 1e3:.4f} ms")
+    clock, lat, busy = 0.0, [], 0.0
+    for r in sorted(reqs, key=lambda r: r.arrival):
+        fixed, per = costs[r.N]
+        sv = fixed + r.steps * per
+        busy += sv
+        start = max(clock, r.arrival)
+        clock = start + sv
+        lat.append(clock - r.arrival)
+    work = sum(r.work for r in reqs)
+    return {"busy_s": busy, "mcell_steps_per_s": work / busy / 1e6, "p50_s": percentile(lat, 50),
+            "p99_s": percentile(lat, 99), "mean_s": float(np.mean(lat)), "makespan_s": clock,
+            "per_size_cost_s": {str(N): [round(c[0], 5), round(c[1], 7)] for N, c in costs.items()}}
+
+
+FORMS = {"fast": C.CBNS_FP64_FAST, "exact": C.CBNS_FP64}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("mode", choices=["sweep", "serve"])
@@ -265,18 +355,23 @@ def main(argv=None):
     ap.add_argument("--steps-min", type=int, default=20)
     ap.add_argument("--steps-max", type=int, default=200)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "alt"])
+    ap.add_argument("--fp64-form", default="fast", choices=sorted(FORMS),
+                    help="arithmetic of FP64 requests: tolerance-bound (fast) or bit-exact")
     ap.add_argument("--rate", type=float, default=None, help="arrivals per second (serve)")
-    ap.add_argument("--load", type=float, default=0.85, help="offered load vs batch capacity when --rate is unset")
+    ap.add_argument("--load", type=float, default=0.85, help="offered load vs measured capacity when --rate is unset")
+    ap.add_argument("--probe", type=int, default=48, help="requests of the capacity probe")
     ap.add_argument("--budget-gb", type=float, default=200.0)
-    ap.add_argument("--chunk", type=int, default=4)
-    ap.add_argument("--slots", type=int, default=2, help="solves resident at once (0: as memory allows)")
+    ap.add_argument("--chunk-ms", type=float, default=40.0)
+    ap.add_argument("--slots", type=int, default=0, help="solves resident at once (0: as memory allows)")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--no-baseline", action="store_true")
+    ap.add_argument("--baseline", default="measured", choices=["measured", "model", "none"],
+                    help="sequential plain-FP64 baseline: run every request, or the per-size cost model")
     a = ap.parse_args(argv)
     budget = int(a.budget_gb * GB)
+    form = FORMS[a.fp64_form]
     if a.mode == "sweep":
         reqs = make_requests(a.n or 64, a.seed, steps=(a.steps, a.steps), precision=a.precision)
-        res = run_sweep(reqs, budget)
+        res = run_sweep_lanes(reqs, budget, fp64_form=form, chunk_ms=a.chunk_ms)
         res["requests"] = len(reqs)
         print(json.dumps({"sweep": res}))
         return 0
@@ -284,22 +379,35 @@ def main(argv=None):
     weights = [1.0 / g for g in grids]
     n = a.n or 128
     rate = a.rate
+    cap = None
     if rate is None:
-        # calibrate the offered load on a short batched sweep of the same mix
-        probe = make_requests(min(n, 32), a.seed + 1, grids, weights, (a.steps_min, a.steps_min))
-        cap = run_sweep(probe, budget)["mcell_steps_per_s"] * 1e6
+        # calibrate the offered load: the served mix's capacity, all admitted at once
+        probe = make_requests(a.probe, a.seed + 1, grids, weights, (a.steps_min, a.steps_min))
+        cap = run_sweep_lanes(probe, budget, fp64_form=form, chunk_ms=a.chunk_ms)["mcell_steps_per_s"] * 1e6
         mean_work = np.mean([r.work for r in make_requests(n, a.seed, grids, weights, (a.steps_min, a.steps_max))])
         rate = a.load * cap / mean_work
+        print(f"capacity probe: {cap / 1e6:.1f} Mcell-steps/s -> {rate:.4f} requests/s at load {a.load}", flush=True)
     reqs = make_requests(n, a.seed, grids, weights, (a.steps_min, a.steps_max), rate=rate)
-    def progress(r):
-        print(f"request {r.rid} N={r.N} steps={r.steps} done", flush=True)
+    t_start = time.perf_counter()
 
-    res = {"requests": n, "rate_per_s": rate, "slots": a.slots,
-           "served": serve(reqs, budget, a.chunk, max_live=a.slots or None, log=progress)}
+    def progress(r):
+        if isinstance(r, str):
+            print(r, flush=True)
+        else:
+            print(f"request {r.rid} N={r.N} steps={r.steps} {'fp32' if r.precision else 'fp64'} "
+                  f"latency {r.done - r.arrival:.3f} s (at {time.perf_counter() - t_start:.1f} s)", flush=True)
+
+    res = {"requests": n, "rate_per_s": rate, "load": a.load, "capacity_probe_mcell_steps_per_s":
+           None if cap is None else cap / 1e6, "slots": a.slots, "fp64_form": a.fp64_form,
+           "steps": [a.steps_min, a.steps_max], "budget_gb": a.budget_gb,
+           "served": serve(reqs, budget, a.chunk_ms, max_live=a.slots or None, log=progress, fp64_form=form,
+                           collect=False)}
     print(json.dumps(res), flush=True)
-    if not a.no_baseline:
-        res["sequential_fp64"] = sequential_fp64(reqs, log=progress)
-    print(json.dumps(res))
+    if a.baseline == "measured":
+        res["sequential_fp64"] = sequential_fp64(reqs)
+    elif a.baseline == "model":
+        res["sequential_fp64_model"] = sequential_model(reqs, log=progress)
+    print(json.dumps(res), flush=True)
     return 0
 
 

@@ -233,7 +233,11 @@ int cbns_batch_bytes_needed(const cbns_params* p, const int dims[3], uint64_t* b
 int cbns_batch_memory(const cbns_batch* b, uint64_t* used, uint64_t* budget);
 /* Admit a solve: Taylor-Green start when Q_aos is NULL (t from p->t_cur), else
  * the given interior AoS state at t_cur with mu := mu0; it runs `steps` steps
- * (or until t_end).  CBNS_ERR_NOMEM when the budget or capacity is full. */
+ * (or until t_end).  precision: CBNS_FP64, CBNS_FP32_FLUX or CBNS_FP64_FAST.
+ * CBNS_ERR_NOMEM when the budget or capacity is full.  The arrays of removed
+ * solves are kept for reuse by later solves of the same shapes (releasing
+ * device memory synchronises the whole device); they are released when new
+ * memory would take the batch past its budget, and by cbns_batch_destroy. */
 int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int precision, const double* Q_aos,
                    double t_cur, double mu0, int steps, int* id);
 int cbns_batch_remove(cbns_batch* b, int id);
@@ -264,6 +268,14 @@ int cbns_batch_set_overlap(cbns_batch* b, int viscous_workgroups);
  * counterpart in the reference. */
 int cbns_batch_set_band(cbns_batch* b, int rows);
 int cbns_batch_sync(cbns_batch* b);
+/* Whether the steps issued by the last cbns_batch_run are still running
+ * (*busy = 1) or done (0), without waiting: a server polls its batches with
+ * it and services only the idle ones.  No counterpart in the reference. */
+int cbns_batch_query(cbns_batch* b, int* busy);
+/* Device bytes held for reuse from removed solves, and their release (see
+ * cbns_batch_add).  No counterpart in the reference. */
+int cbns_batch_pooled(const cbns_batch* b, uint64_t* bytes);
+int cbns_batch_trim(cbns_batch* b);
 int cbns_batch_status(cbns_batch* b, int id, int* active, int* steps_done, double* t_cur, int* nonfinite);
 /* Ids of live solves that are done (took their steps, reached t_end or failed);
  * *n_live = live solves. */
