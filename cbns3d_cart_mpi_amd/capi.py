@@ -101,6 +101,7 @@ _SIGS = {
     "cbns_batch_set_band": (_I, [_VP, _I]),
     "cbns_batch_status": (_I, [_VP, _I, _PI, _PI, _PD, _PI]),
     "cbns_batch_poll": (_I, [_VP, _PI, _I, _PI, _PI]),
+    "cbns_batch_poll_status": (_I, [_VP, _PI, _PD, _PI, _PI, _I, _PI, _PI]),
     "cbns_batch_get_state": (_I, [_VP, _I, _PD]),
     "cbns_strerror": (ctypes.c_char_p, [_I]),
 }
@@ -446,6 +447,17 @@ class Batch:
         nd, nl = _I(), _I()
         check(lib().cbns_batch_poll(self._h, ids, max_ids, ctypes.byref(nd), ctypes.byref(nl)), "cbns_batch_poll")
         return [ids[k] for k in range(nd.value)], nl.value
+
+    def poll_status(self, max_ids=4096):
+        """Done solves with their status, in one read-back:
+        ([{"id", "t_cur", "steps_done", "nonfinite"}], live)."""
+        ids, ns, nf = (_I * max_ids)(), (_I * max_ids)(), (_I * max_ids)()
+        t = (_D * max_ids)()
+        nd, nl = _I(), _I()
+        check(lib().cbns_batch_poll_status(self._h, ids, t, ns, nf, max_ids, ctypes.byref(nd), ctypes.byref(nl)),
+              "cbns_batch_poll_status")
+        return [{"id": ids[k], "t_cur": t[k], "steps_done": ns[k], "nonfinite": bool(nf[k])}
+                for k in range(nd.value)], nl.value
 
     def get_state(self, i):
         N = self._N[i]

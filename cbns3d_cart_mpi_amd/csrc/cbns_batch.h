@@ -26,6 +26,14 @@ struct InstDesc {
   unsigned long long smax;  // dt reduction target, reset after each use
 };
 
+// Per-solve status published to pinned host memory after each run, so a
+// server can poll a finished lane without a device copy (which could queue
+// behind another lane's work on a shared hardware queue).
+struct InstStatus {
+  double t_cur;
+  int steps_done, steps_target, active, nonfinite;
+};
+
 // One work tile: 256 consecutive work items of one block.
 struct TileRef {
   int block, tile;
@@ -69,6 +77,8 @@ hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable t
 // zeroes n counters with a kernel (the step graph holds no memset nodes)
 hipError_t batch_launch_reset_counters(int* ctr, int n, hipStream_t s);
 hipError_t batch_launch_finalize(InstDesc* insts, int ninst, hipStream_t s);
+// copies every descriptor's status to host-mapped memory (after a run)
+hipError_t batch_launch_publish(const InstDesc* insts, InstStatus* status, int ninst, hipStream_t s);
 hipError_t batch_launch_set_budget(InstDesc* insts, const int2* pairs, int n, hipStream_t s);
 
 }  // namespace cbns

@@ -252,6 +252,21 @@ __global__ void kb_finalize(InstDesc* insts, int n) {
   I.s_fused = I.pairs;  // this step's stage 3 forms the next step's smax
 }
 
+__global__ void kb_publish(const InstDesc* insts, InstStatus* status, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const InstDesc& I = insts[i];
+    InstStatus st;
+    st.t_cur = I.t_cur;
+    st.steps_done = I.steps_done;
+    st.steps_target = I.steps_target;
+    st.active = I.active;
+    st.nonfinite = I.nonfinite;
+    status[i] = st;
+  }
+  __threadfence_system();
+}
+
 // Step budgets granted by the host: pairs of (solve id, steps).
 __global__ void kb_set_budget(InstDesc* insts, const int2* pairs, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -329,6 +344,12 @@ hipError_t batch_launch_reset_counters(int* ctr, int n, hipStream_t s) {
 hipError_t batch_launch_set_budget(InstDesc* insts, const int2* pairs, int n, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(kb_set_budget, dim3((n + 63) / 64), dim3(64), 0, s, insts, pairs, n);
+  return hipGetLastError();
+}
+
+hipError_t batch_launch_publish(const InstDesc* insts, InstStatus* status, int ninst, hipStream_t s) {
+  if (ninst == 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_publish, dim3((ninst + 63) / 64), dim3(64), 0, s, insts, status, ninst);
   return hipGetLastError();
 }
 

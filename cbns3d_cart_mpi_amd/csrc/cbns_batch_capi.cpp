@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
@@ -80,6 +81,11 @@ struct cbns_batch {
   double* d_stage = nullptr;  // grow-only staging of states and tables (add, get_state)
   size_t stage_cap = 0;
   hipEvent_t ev_run = nullptr;  // recorded after the last run's graph launches
+  // status of every descriptor after the last run (host-mapped, written by
+  // kb_publish at the end of each run); valid for reading once ev_run is done
+  InstStatus* h_status = nullptr;
+  InstStatus* d_status = nullptr;
+  bool published = false;  // h_status reflects the device descriptors
   bool has_fast = false; // a live tolerance-bound solve (the step graph has fixup passes)
   int wg_mem = 0;     // override of the memory-bound kernels' grid (0: resident)
   int ninst_hw = 0;  // highest live id + 1
@@ -136,6 +142,15 @@ static uint64_t solve_bytes(const cbns_params* p, const int dims[3]) {
 }
 
 // A solve's arrays go to the batch's pool (see cbns_batch::pool).
+// Host <-> device copies of the batch go on its own stream (then wait for
+// it): a copy on the null stream can queue behind another batch's work on a
+// shared hardware queue.
+static hipError_t copy_on(cbns_batch* b, void* dst, const void* src, size_t n, hipMemcpyKind k) {
+  hipError_t e = hipMemcpyAsync(dst, src, n, k, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  return e;
+}
+
 static void free_inst(cbns_batch* b, Inst& I) {
   for (auto& a : I.allocs) {
     b->pool[a.second].push_back(a.first);
@@ -224,7 +239,7 @@ static int rebuild(cbns_batch* b) {
       add(T_DT, B.sn);
     }
     // the device descriptor's block0 (other fields hold live state)
-    CK(hipMemcpy((char*)(b->d_insts + id) + offsetof(InstDesc, block0), &I.block0, sizeof(int),
+    CK(copy_on(b, (char*)(b->d_insts + id) + offsetof(InstDesc, block0), &I.block0, sizeof(int),
                  hipMemcpyHostToDevice));
   }
   b->ninst_hw = hw;
@@ -233,11 +248,11 @@ static int rebuild(cbns_batch* b) {
   int st;
   if ((st = grow((void**)&b->d_blocks, &b->d_blocks_cap, (int)all.size() + 1, sizeof(BlockDesc)))) return st;
   if (!all.empty())
-    CK(hipMemcpy(b->d_blocks, all.data(), all.size() * sizeof(BlockDesc), hipMemcpyHostToDevice));
+    CK(copy_on(b, b->d_blocks, all.data(), all.size() * sizeof(BlockDesc), hipMemcpyHostToDevice));
   for (int c = 0; c < T_COUNT; c++) {
     if ((st = grow((void**)&b->d_tiles[c], &b->tiles_cap[c], (int)tab[c].size() + 1, sizeof(TileRef)))) return st;
     if (!tab[c].empty())
-      CK(hipMemcpy(b->d_tiles[c], tab[c].data(), tab[c].size() * sizeof(TileRef), hipMemcpyHostToDevice));
+      CK(copy_on(b, b->d_tiles[c], tab[c].data(), tab[c].size() * sizeof(TileRef), hipMemcpyHostToDevice));
     b->tt[c] = {b->d_tiles[c], (int)tab[c].size()};
   }
   // capture one RK3 step (src/cbns3d.cpp:178-240) as a graph
@@ -331,6 +346,10 @@ int cbns_batch_create(uint64_t hbm_budget_bytes, int capacity, cbns_batch** out)
   if (!st && hipMalloc(&fixbuf, FIX_BYTES) != hipSuccess) st = CBNS_ERR_NOMEM;
   if (!st && hipMemset(fixbuf, 0, 256) != hipSuccess) st = CBNS_ERR_HIP;
   if (fixbuf) b->fix = fixlist_at(fixbuf);
+  if (!st && hipHostMalloc((void**)&b->h_status, sizeof(InstStatus) * capacity, hipHostMallocMapped) != hipSuccess)
+    st = CBNS_ERR_NOMEM;
+  if (!st && hipHostGetDevicePointer((void**)&b->d_status, b->h_status, 0) != hipSuccess) st = CBNS_ERR_HIP;
+  if (!st) std::memset(b->h_status, 0, sizeof(InstStatus) * capacity);
   if (!st && hipMemset(b->d_insts, 0, sizeof(InstDesc) * capacity) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && batch_grids(&b->grid) != hipSuccess) st = CBNS_ERR_HIP;
@@ -357,6 +376,7 @@ int cbns_batch_destroy(cbns_batch* b) {
   drain_pool(b);
   if (b->d_stage) (void)hipFree(b->d_stage);
   if (b->ev_run) (void)hipEventDestroy(b->ev_run);
+  if (b->h_status) (void)hipHostFree(b->h_status);
   for (int c = 0; c < T_COUNT; c++)
     if (b->d_tiles[c]) (void)hipFree(b->d_tiles[c]);
   if (b->d_blocks) (void)hipFree(b->d_blocks);
@@ -457,7 +477,7 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
   int st = CBNS_OK;
   hipStream_t s = b->stream;
   const int nb = (int)I.blocks.size();
-  if (hipMemcpy(I.dblocks, I.blocks.data(), sizeof(BlockDesc) * nb, hipMemcpyHostToDevice) != hipSuccess)
+  if (copy_on(b, I.dblocks, I.blocks.data(), sizeof(BlockDesc) * nb, hipMemcpyHostToDevice) != hipSuccess)
     st = CBNS_ERR_HIP;
   // initial state: Taylor-Green (Q_aos == NULL) or the given interior state
   if (!st && !Q_aos) {
@@ -474,7 +494,7 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
       }
     double* dtab = nullptr;
     st = stage(b, sizeof(double) * tab.size(), &dtab);
-    if (!st && (hipMemcpy(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess ||
+    if (!st && (copy_on(b, dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess ||
                 launch_tgv(I.dblocks, I.blocks.data(), nb, dtab, ts, ph, p->p_inf, s) != hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess))
       st = CBNS_ERR_HIP;
@@ -483,7 +503,7 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
     const size_t n = (size_t)NEQ * N[0] * N[1] * N[2];
     double* d = nullptr;
     st = stage(b, n * sizeof(double), &d);
-    if (!st && (hipMemcpy(d, Q_aos, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+    if (!st && (copy_on(b, d, Q_aos, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
                 launch_scatter_aos(I.dblocks, I.blocks.data(), I.dom, CBNS_Q, d, s) != hipSuccess ||
                 launch_fill_mu(I.dblocks, I.blocks.data(), nb, mu0, s) != hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess))
@@ -523,13 +543,14 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
   D.budget = 0x7fffffff;  // unbounded until the host sets budgets
   D.t_cur = t_cur;
   D.t_end = p->t_end;
-  if (hipMemcpy(b->d_insts + id, &D, sizeof(D), hipMemcpyHostToDevice) != hipSuccess) {
+  if (copy_on(b, b->d_insts + id, &D, sizeof(D), hipMemcpyHostToDevice) != hipSuccess) {
     free_inst(b, I);
     return CBNS_ERR_HIP;
   }
   I.live = true;
   b->used += need;
   b->dirty = true;
+  b->published = false;
   *id_out = id;
   return CBNS_OK;
 }
@@ -537,8 +558,8 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
 int cbns_batch_remove(cbns_batch* b, int id) {
   if (!b || id < 0 || id >= b->cap || !b->inst[id].live) return CBNS_ERR_ARG;
   CK(hipStreamSynchronize(b->stream));
-  int zero = 0;
-  CK(hipMemcpy((char*)(b->d_insts + id) + offsetof(InstDesc, active), &zero, sizeof(int), hipMemcpyHostToDevice));
+  // stream-ordered clear of its active flag (no synchronous copy)
+  CK(hipMemsetAsync((char*)(b->d_insts + id) + offsetof(InstDesc, active), 0, sizeof(int), b->stream));
   b->used -= b->inst[id].bytes;
   free_inst(b, b->inst[id]);
   b->dirty = true;
@@ -553,7 +574,9 @@ int cbns_batch_run(cbns_batch* b, int nsteps) {
   }
   if (!b->exec) return CBNS_OK;
   for (int n = 0; n < nsteps; n++) CK(hipGraphLaunch(b->exec, b->stream));
+  CK(batch_launch_publish(b->d_insts, b->d_status, b->ninst_hw, b->stream));
   CK(hipEventRecord(b->ev_run, b->stream));
+  b->published = true;
   return CBNS_OK;
 }
 
@@ -581,6 +604,7 @@ int cbns_batch_query(cbns_batch* b, int* busy) {
 
 int cbns_batch_set_budgets(cbns_batch* b, const int* ids, const int* steps, int n) {
   if (!b || n < 0 || (n > 0 && (!ids || !steps))) return CBNS_ERR_ARG;
+  b->published = false;
   std::vector<int2> pr(n);
   for (int k = 0; k < n; k++) {
     if (ids[k] < 0 || ids[k] >= b->cap || !b->inst[ids[k]].live || steps[k] < 0) return CBNS_ERR_ARG;
@@ -590,7 +614,7 @@ int cbns_batch_set_budgets(cbns_batch* b, const int* ids, const int* steps, int 
   CK(hipStreamSynchronize(b->stream));
   int st = grow((void**)&b->d_pairs, &b->pairs_cap, n, sizeof(int2));
   if (st) return st;
-  CK(hipMemcpy(b->d_pairs, pr.data(), sizeof(int2) * n, hipMemcpyHostToDevice));
+  CK(copy_on(b, b->d_pairs, pr.data(), sizeof(int2) * n, hipMemcpyHostToDevice));
   CK(batch_launch_set_budget(b->d_insts, b->d_pairs, n, b->stream));
   return CBNS_OK;
 }
@@ -629,7 +653,7 @@ int cbns_batch_status(cbns_batch* b, int id, int* active, int* steps_done, doubl
   if (!b || id < 0 || id >= b->cap || !b->inst[id].live) return CBNS_ERR_ARG;
   InstDesc D;
   CK(hipStreamSynchronize(b->stream));
-  CK(hipMemcpy(&D, b->d_insts + id, sizeof(D), hipMemcpyDeviceToHost));
+  CK(copy_on(b, &D, b->d_insts + id, sizeof(D), hipMemcpyDeviceToHost));
   if (active) *active = D.active;
   if (steps_done) *steps_done = D.steps_done;
   if (t_cur) *t_cur = D.t_cur;
@@ -638,20 +662,42 @@ int cbns_batch_status(cbns_batch* b, int id, int* active, int* steps_done, doubl
 }
 
 int cbns_batch_poll(cbns_batch* b, int* ids, int max_ids, int* n_done, int* n_live) {
+  return cbns_batch_poll_status(b, ids, nullptr, nullptr, nullptr, max_ids, n_done, n_live);
+}
+
+int cbns_batch_poll_status(cbns_batch* b, int* ids, double* t_cur, int* steps_done, int* nonfinite, int max_ids,
+                           int* n_done, int* n_live) {
   if (!b || (!ids && max_ids > 0)) return CBNS_ERR_ARG;
-  CK(hipStreamSynchronize(b->stream));
-  int hw = 0;  // highest live id + 1 (only those descriptors are read back)
+  int hw = 0;  // highest live id + 1 (only those descriptors are read)
   for (int id = 0; id < b->cap; id++)
     if (b->inst[id].live) hw = id + 1;
-  std::vector<InstDesc> D(hw);
-  if (hw) CK(hipMemcpy(D.data(), b->d_insts, sizeof(InstDesc) * hw, hipMemcpyDeviceToHost));
+  std::vector<InstStatus> D(hw);
+  if (b->published) {  // after the run's publish: read the host copy, no device operation
+    CK(hipEventSynchronize(b->ev_run));
+    std::atomic_thread_fence(std::memory_order_acquire);
+    const volatile InstStatus* hs = b->h_status;
+    for (int id = 0; id < hw; id++)
+      D[id] = InstStatus{hs[id].t_cur, hs[id].steps_done, hs[id].steps_target, hs[id].active, hs[id].nonfinite};
+  } else {
+    CK(hipStreamSynchronize(b->stream));
+    std::vector<InstDesc> full(hw);
+    if (hw) CK(copy_on(b, full.data(), b->d_insts, sizeof(InstDesc) * hw, hipMemcpyDeviceToHost));
+    for (int id = 0; id < hw; id++)
+      D[id] = InstStatus{full[id].t_cur, full[id].steps_done, full[id].steps_target, full[id].active,
+                         full[id].nonfinite};
+  }
   int nd = 0, nl = 0;
   for (int id = 0; id < hw; id++) {
     if (!b->inst[id].live) continue;
     nl++;
     const bool done = !D[id].active || D[id].steps_done >= D[id].steps_target;
     if (done) {
-      if (nd < max_ids) ids[nd] = id;
+      if (nd < max_ids) {
+        ids[nd] = id;
+        if (t_cur) t_cur[nd] = D[id].t_cur;
+        if (steps_done) steps_done[nd] = D[id].steps_done;
+        if (nonfinite) nonfinite[nd] = D[id].nonfinite;
+      }
       nd++;
     }
   }
@@ -670,7 +716,7 @@ int cbns_batch_get_state(cbns_batch* b, int id, double* Q_aos) {
   if (st) return st;
   if (launch_gather_aos(I.dblocks, I.blocks.data(), I.dom, CBNS_Q, d, b->stream) != hipSuccess ||
       hipStreamSynchronize(b->stream) != hipSuccess ||
-      hipMemcpy(Q_aos, d, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      copy_on(b, Q_aos, d, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
     st = CBNS_ERR_HIP;
   return st;
 }

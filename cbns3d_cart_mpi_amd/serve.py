@@ -171,7 +171,7 @@ def serve(reqs, budget=200 * GB, chunk_ms=40.0, capacity=256, collect=True, lane
     batches = [C.Batch(budget, capacity=capacity) for _ in lanes]
     live = [dict() for _ in lanes]
     used = [0]
-    host = {"admit": 0.0, "issue": 0.0, "collect": 0.0, "idle": 0.0}
+    host = {"admit": 0.0, "issue": 0.0, "poll": 0.0, "collect": 0.0, "idle": 0.0}
     lane_of = lambda r: next(n for n, (lo, hi) in enumerate(lanes) if lo <= r.N <= hi)
     try:
         t0 = time.perf_counter()
@@ -188,12 +188,13 @@ def serve(reqs, budget=200 * GB, chunk_ms=40.0, capacity=256, collect=True, lane
                 # lane idle: collect its finished solves
                 tc = time.perf_counter()
                 if live[li]:
-                    fin, _ = b.poll()
-                    for i in fin:
+                    fin, _ = b.poll_status()
+                    host["poll"] += time.perf_counter() - tc
+                    for st in fin:
+                        i = st["id"]
                         r = live[li].pop(i)
-                        st = b.status(i)
                         r.t_final = st["t_cur"]
-                        r.nonfinite = bool(st["nonfinite"])
+                        r.nonfinite = st["nonfinite"]
                         if collect:
                             r.result = b.get_state(i)
                         r.done = time.perf_counter() - t0
@@ -203,7 +204,7 @@ def serve(reqs, budget=200 * GB, chunk_ms=40.0, capacity=256, collect=True, lane
                         if log:
                             log(r)
                 ta = time.perf_counter()
-                host["collect"] += ta - tc
+                host["collect"] += ta - tc  # includes the poll
                 # admit this lane's waiting requests, first come first admitted
                 for r in [w for w in waiting if lane_of(w) == li]:
                     p, Q = r.payload
@@ -363,6 +364,8 @@ def main(argv=None):
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--chunk-ms", type=float, default=40.0)
     ap.add_argument("--slots", type=int, default=0, help="solves resident at once (0: as memory allows)")
+    ap.add_argument("--lanes", default="sizes", choices=["sizes", "classes"],
+                    help="one lane per grid size, or three size classes (N <= 64, <= 128, > 128)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--baseline", default="measured", choices=["measured", "model", "none"],
                     help="sequential plain-FP64 baseline: run every request, or the per-size cost model")
@@ -383,11 +386,13 @@ def main(argv=None):
     if rate is None:
         # calibrate the offered load: the served mix's capacity, all admitted at once
         probe = make_requests(a.probe, a.seed + 1, grids, weights, (a.steps_min, a.steps_min))
-        cap = run_sweep_lanes(probe, budget, fp64_form=form, chunk_ms=a.chunk_ms)["mcell_steps_per_s"] * 1e6
+        cap = run_sweep_lanes(probe, budget, fp64_form=form, chunk_ms=a.chunk_ms,
+                              lanes=None if a.lanes == "sizes" else LANES)["mcell_steps_per_s"] * 1e6
         mean_work = np.mean([r.work for r in make_requests(n, a.seed, grids, weights, (a.steps_min, a.steps_max))])
         rate = a.load * cap / mean_work
         print(f"capacity probe: {cap / 1e6:.1f} Mcell-steps/s -> {rate:.4f} requests/s at load {a.load}", flush=True)
     reqs = make_requests(n, a.seed, grids, weights, (a.steps_min, a.steps_max), rate=rate)
+    lanes = tuple((g, g) for g in grids) if a.lanes == "sizes" else LANES
     t_start = time.perf_counter()
 
     def progress(r):
@@ -400,8 +405,9 @@ def main(argv=None):
     res = {"requests": n, "rate_per_s": rate, "load": a.load, "capacity_probe_mcell_steps_per_s":
            None if cap is None else cap / 1e6, "slots": a.slots, "fp64_form": a.fp64_form,
            "steps": [a.steps_min, a.steps_max], "budget_gb": a.budget_gb,
+           "lanes": a.lanes, "chunk_ms": a.chunk_ms,
            "served": serve(reqs, budget, a.chunk_ms, max_live=a.slots or None, log=progress, fp64_form=form,
-                           collect=False)}
+                           collect=False, lanes=lanes)}
     print(json.dumps(res), flush=True)
     if a.baseline == "measured":
         res["sequential_fp64"] = sequential_fp64(reqs)

@@ -280,6 +280,10 @@ int cbns_batch_status(cbns_batch* b, int id, int* active, int* steps_done, doubl
 /* Ids of live solves that are done (took their steps, reached t_end or failed);
  * *n_live = live solves. */
 int cbns_batch_poll(cbns_batch* b, int* ids, int max_ids, int* n_done, int* n_live);
+/* The same, with each done solve's t_cur, steps taken and non-finite flag
+ * (any of the three arrays may be NULL), read back in one copy. */
+int cbns_batch_poll_status(cbns_batch* b, int* ids, double* t_cur, int* steps_done, int* nonfinite, int max_ids,
+                           int* n_done, int* n_live);
 int cbns_batch_get_state(cbns_batch* b, int id, double* Q_aos);
 
 const char* cbns_strerror(int status);
