@@ -101,9 +101,10 @@ def test_bench_reads_the_committed_profiles():
 
     traffic = bench.read_traffic()
     valu = bench.read_valu_active()
-    # one FP64 face sweep at 256^3 moves ~2.1 GB algorithmically; the VALU
-    # issue fraction of a VALU-bound sweep lies well inside (0.5, 1]
-    assert traffic is not None and 1.5e9 < traffic < 4e9
+    # one tolerance-bound FP64 face sweep at 256^3 moves 1.39 GB
+    # algorithmically (state read once, fluxes written); the VALU issue
+    # fraction of a VALU-bound sweep lies well inside (0.5, 1]
+    assert traffic is not None and 1.3e9 < traffic < 3e9
     assert valu is not None and 0.5 < valu <= 1.0
 
 
