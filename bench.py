@@ -119,7 +119,8 @@ def cpu_baseline(n, max_steps, budget_s):
     return {"value": n ** 3 * k / el / 1e6, "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
             "sample": f"{k} TGV {n}^3 FP64 steps ({el:.1f} s) after 1 warm-up step, oracle/ C restatement "
                       "(bit-exact with the reference) with OpenMP; on the 8-core build container it runs at "
-                      "1.03x (1 thread) and 1.24x (8 threads) the reference binary's speed (-np 1 / -np 8)"}
+                      "1.09x (1 thread) and 1.68x (8 threads) the reference binary's speed at -np 1 / -np 8 "
+                      "(oracle/calibrate_cpu.py, TGV 64^3), so it overstates the reference's CPU rate"}
 
 
 FIELDS = ("rho", "rho_u", "rho_v", "rho_w", "E")

@@ -96,9 +96,11 @@ def dilatation_spectrum(Q, N):
 def precision_errors(Q32, Q64, N):
     """Per-field relative L2 (RMS difference / RMS of the FP64 field) and
     relative L-inf (max difference / max |FP64 field|) of an FP32-flux state
-    against the FP64 state, and the dilatation-spectrum difference: the
-    largest relative difference over shells 1 .. N/2 and the relative L2
-    difference of the whole spectrum."""
+    against the FP64 state; for rho and E, whose fields sit on a large
+    uniform background, also the RMS difference relative to the RMS of the
+    field's fluctuation about its mean (rel_l2_fluct); and the
+    dilatation-spectrum difference: the largest relative difference over
+    shells 1 .. N/2 and the relative L2 difference of the whole spectrum."""
     fields = ("rho", "rho_u", "rho_v", "rho_w", "E")
     err = {}
     for e, nm in enumerate(fields):
@@ -106,6 +108,9 @@ def precision_errors(Q32, Q64, N):
         ref = Q64[..., e]
         err[nm] = {"rel_l2": float(np.sqrt(np.mean(d * d) / np.mean(ref * ref))),
                    "rel_linf": float(np.abs(d).max() / np.abs(ref).max())}
+        if nm in ("rho", "E"):
+            fl = np.mean((ref - ref.mean()) ** 2)
+            err[nm]["rel_l2_fluct"] = float(np.sqrt(np.mean(d * d) / fl)) if fl > 0 else None
     s32, s64 = dilatation_spectrum(Q32, N), dilatation_spectrum(Q64, N)
     rel = np.abs(s32[1:] - s64[1:]) / s64[1:]
     err["dilatation_spectrum"] = {"max_rel_shell": float(rel.max()),

@@ -41,3 +41,46 @@ def test_precision_errors_are_zero_for_equal_states_and_scale_linearly():
         assert abs(e1[nm]["rel_l2"] - 1e-7) < 1e-12
     # u = rho u / rho is unchanged by a common factor, so theta is too
     assert e1["dilatation_spectrum"]["rel_l2"] < 1e-12
+
+
+def _per_mode_ratio(u, N, k0, shape_fn):
+    """Per shell k (|kvec| in [k - 1/2, k + 1/2)): the mean over its modes of
+    |u_hat|^2 / shape_fn(|kvec|), u_hat normalised by N^3."""
+    k1 = np.fft.fftfreq(N, 1.0 / N)
+    KZ, KY, KX = np.meshgrid(k1, k1, k1, indexing="ij")
+    kk = np.sqrt(KX * KX + KY * KY + KZ * KZ)
+    e = sum(np.abs(np.fft.fftn(u[a]) / N ** 3) ** 2 for a in range(3))
+    shell = np.rint(kk).astype(int).ravel()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = (e / shape_fn(kk, k0)).ravel()
+    ok = np.isfinite(r)
+    return np.bincount(shell[ok], weights=r[ok]) / np.maximum(np.bincount(shell[ok]), 1)
+
+
+def test_turbulence_spectrum_has_the_stated_shape_and_pressure():
+    """E(k) ~ k^4 exp(-2 (k/4)^2), i.e. 4 pi k^2 modes per shell each with
+    mean energy ~ k^2 exp(-2 (k/4)^2): over eight seeds, every mode's
+    |u_hat|^2 divided by |k|^2 exp(-2 (|k|/4)^2) averages to the same
+    constant in every shell 1..9 within 12% (the expected scatter of the
+    per-shell means is a few percent); the same test with exp(-(k/4)^2)
+    instead, or without the k^2, is off by more than a factor of 2 across
+    that range.  p = 1/gamma and rho = 1 everywhere."""
+    N, k0 = 32, 4.0
+    good = lambda kk, k0: kk ** 2 * np.exp(-2.0 * (kk / k0) ** 2)
+    wrong = (lambda kk, k0: kk ** 2 * np.exp(-((kk / k0) ** 2)), lambda kk, k0: np.exp(-2.0 * (kk / k0) ** 2))
+    acc = [0, 0, 0]
+    for seed in range(8):
+        Q, _ = turbulence_state(N, seed=seed)
+        p = 0.4 * (Q[..., 4] - 0.5 * (Q[..., 1:4] ** 2).sum(-1) / Q[..., 0])
+        assert np.abs(p - 1 / 1.4).max() < 1e-14
+        assert np.all(Q[..., 0] == 1.0)
+        u = np.stack([Q[..., 1 + a] / Q[..., 0] for a in range(3)])
+        for n, f in enumerate((good,) + wrong):
+            acc[n] = acc[n] + _per_mode_ratio(u, N, k0, f)
+    k = np.arange(1, 10)
+    r = acc[0][k] / np.mean(acc[0][k])
+    print("per-mode energy / k^2 exp(-2 (k/4)^2), normalised:", np.round(r, 3))
+    assert np.all(np.abs(r - 1.0) < 0.12)
+    for a in acc[1:]:
+        rw = a[k] / np.mean(a[k])
+        assert rw.max() / rw.min() > 2.0
