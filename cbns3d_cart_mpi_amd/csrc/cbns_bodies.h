@@ -363,8 +363,26 @@ __device__ __forceinline__ void visc_terms(const Phys& ph, const double (&gr)[4]
 // same expressions as a one-point-per-thread evaluation (grad4, visc_terms).  One call = one (column tile, z-chunk) unit, made by all
 // VX * VY threads of a workgroup with the same tile and chunk (it
 // synchronises the workgroup).
-template <typename VT>
-__device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& ph, int tile, int chunk) {
+// FROMQ (tolerance-bound FP64 form, whose stage tail stores no primitives):
+// u, v, w, T are formed from the conserved values of Q[which] as each
+// plane is staged (calc_primitive's expressions, src/Block3d.cpp:369-381,
+// with one refined reciprocal of rho); the loads of the next plane are
+// still issued before the current plane's arithmetic.
+__device__ __forceinline__ void prim_uvwT(const double (&q)[5], const Phys& ph, double (&o)[4]) {
+#pragma clang fp contract(off)
+  const double ir = rcp_nr(q[0]);
+  o[0] = q[1] * ir;
+  o[1] = q[2] * ir;
+  o[2] = q[3] * ir;
+  const double k = __builtin_fma(o[2], o[2], __builtin_fma(o[1], o[1], o[0] * o[0]));
+  const double p = ph.gam1 * __builtin_fma(-0.5 * q[0], k, q[4]);
+  o[3] = (ph.gM2 * p) * ir;
+}
+
+template <typename VT, bool FROMQ = false>
+__device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& ph, int tile, int chunk,
+                                               int which = 0) {
+  static_assert(!FROMQ || sizeof(VT) == 8, "the state is FP64");
   const int v0 = B.n[0] + 2 * NGV, v1 = B.n[1] + 2 * NGV, v2 = B.n[2] + 2 * NGV;
   const int ntx = (v0 + VX - 1) / VX, nty = (v1 + VY - 1) / VY;
   const int Z0 = chunk * VZ;
@@ -378,8 +396,22 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
   __shared__ VT L[2][4][LY][LX];  // planes in the storage type, widened at use
   const long long sg = B.sg;
   const int sz = B.g[0] * B.g[1];
-  gptr<const VT> __restrict__ P = gmem<const VT>((const VT*)B.prim);  // primitives share VT
-  const long long foff[4] = {sg, 2 * sg, 3 * sg, 5 * sg};  // u, v, w, T
+  // source fields: u, v, w, T of the primitives (sharing VT), or with FROMQ
+  // rho, rho u, rho v, rho w, E of the state
+  constexpr int NF = FROMQ ? 5 : 4;
+  gptr<const VT> __restrict__ P = gmem<const VT>(FROMQ ? (const VT*)B.Q[which] : (const VT*)B.prim);
+  const long long foff[5] = {FROMQ ? 0 : sg, FROMQ ? sg : 2 * sg, FROMQ ? 2 * sg : 3 * sg, FROMQ ? 3 * sg : 5 * sg,
+                             4 * sg};
+  // the values of one point as u, v, w, T
+  const auto uvwT = [&](const VT (&x)[NF], double (&o)[4]) {
+    if constexpr (FROMQ) {
+      const double q[5] = {(double)x[0], (double)x[1], (double)x[2], (double)x[3], (double)x[NF - 1]};
+      prim_uvwT(q, ph, o);
+    } else {
+#pragma unroll
+      for (int f = 0; f < 4; f++) o[f] = (double)x[f];
+    }
+  };
   // LDS entries e = threadIdx.x + p NT this thread fills: cell (X0 + lx - 4,
   // Y0 + ly - 4).  A position past the ghosted block (no active point's
   // stencil reaches it) or past the tile, and an inactive point's column,
@@ -402,35 +434,63 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
     else fidx[p] = fok[p] ? gidx(B, cx, cy, Z0 - 2) : 0;
   }
   const bool ld = CLAMP || act;
-  VT pf[4][NP], rn[4];  // next-plane prefetch
+  VT pf[NF][NP], rn[NF];  // next-plane prefetch (raw source values)
   double r[4][5];
   int ci = CLAMP ? gidx(B, (bx < v0 ? bx : v0 - 1) - 2, (by < v1 ? by : v1 - 1) - 2, Z0 - 2)
                  : (act ? gidx(B, bx - 2, by - 2, Z0 - 2) : 0);
 #pragma unroll
-  for (int f = 0; f < 4; f++) {
+  for (int d = 0; d < 4; d++) {
+    VT x[NF];
+    double o[4];
 #pragma unroll
-    for (int d = 0; d < 4; d++) r[f][d] = ld ? (double)P[foff[f] + ci + (d - 2) * sz] : 0.0;
-    rn[f] = ld ? P[foff[f] + ci + 2 * sz] : (VT)0;
+    for (int f = 0; f < NF; f++) x[f] = ld ? P[foff[f] + ci + (d - 2) * sz] : (VT)(FROMQ && f == 0 ? 1 : 0);
+    uvwT(x, o);
 #pragma unroll
-    for (int p = 0; p < NP; p++) pf[f][p] = fok[p] ? P[foff[f] + fidx[p]] : (VT)0;
+    for (int f = 0; f < 4; f++) r[f][d] = o[f];
+  }
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    rn[f] = ld ? P[foff[f] + ci + 2 * sz] : (VT)(FROMQ && f == 0 ? 1 : 0);
+#pragma unroll
+    for (int p = 0; p < NP; p++) pf[f][p] = fok[p] ? P[foff[f] + fidx[p]] : (VT)(FROMQ && f == 0 ? 1 : 0);
   }
   gptr<VT> __restrict__ V = gmem((VT*)B.visc);
   const long long sv = B.sv;
   for (int bz = Z0; bz < Z1; bz++, ci += sz) {
     const int buf = (bz - Z0) & 1;
-#pragma unroll
-    for (int f = 0; f < 4; f++) {
+    if constexpr (FROMQ) {
 #pragma unroll
       for (int p = 0; p < NP; p++) {
         const int e = threadIdx.x + p * NT;
-        if (e < LX * LY) (&L[buf][f][0][0])[e] = pf[f][p];
+        VT x[NF];
+        double o[4];
+#pragma unroll
+        for (int f = 0; f < NF; f++) x[f] = pf[f][p];
+        uvwT(x, o);
+        if (e < LX * LY) {
+#pragma unroll
+          for (int f = 0; f < 4; f++) (&L[buf][f][0][0])[e] = (VT)o[f];
+        }
       }
-      r[f][4] = (double)rn[f];
+      double o[4];
+      uvwT(rn, o);
+#pragma unroll
+      for (int f = 0; f < 4; f++) r[f][4] = o[f];
+    } else {
+#pragma unroll
+      for (int f = 0; f < 4; f++) {
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+          const int e = threadIdx.x + p * NT;
+          if (e < LX * LY) (&L[buf][f][0][0])[e] = pf[f][p];
+        }
+        r[f][4] = (double)rn[f];
+      }
     }
     __syncthreads();
     if (bz + 1 < Z1) {
 #pragma unroll
-      for (int f = 0; f < 4; f++) {
+      for (int f = 0; f < NF; f++) {
 #pragma unroll
         for (int p = 0; p < NP; p++)
           if (fok[p]) pf[f][p] = P[foff[f] + fidx[p] + (long long)(bz + 1 - Z0) * sz];
@@ -594,7 +654,9 @@ __device__ __forceinline__ vec2<T> ld2(gptr<const T> p, long long o) {
 // With dts != nullptr (stage 3) it also returns in *s_out the larger calc_dt
 // spectral radius of the two updated cells (a NaN never wins), with mu from
 // this stage's viscous pass, which is what the next step's calc_dt reads.
-template <typename FT>
+// PRIMS = false (tolerance-bound FP64 form, whose kernels form primitives
+// from the state): the primitives are not stored.
+template <typename FT, bool PRIMS = true>
 __device__ __forceinline__ void update_pair_body(const BlockDesc& B, int t, double dt, int stage, const Phys& ph,
                                                  int jb, bool want_s = false, double* s_out = nullptr,
                                                  int* bad = nullptr) {
@@ -702,8 +764,10 @@ __device__ __forceinline__ void update_pair_body(const BlockDesc& B, int t, doub
       *s_out = (*s_out < sc) ? sc : *s_out;  // src/Block3d.cpp:546: a NaN never wins
     }
   }
+  if constexpr (PRIMS) {
 #pragma unroll
-  for (int f = 0; f < 6; f++) *(gptr<vec2<FT>>)(P + f * sg + c) = pr[f];
+    for (int f = 0; f < 6; f++) *(gptr<vec2<FT>>)(P + f * sg + c) = pr[f];
+  }
 }
 
 // ---- ghost gather of the state and its primitives ------------------------

@@ -56,6 +56,10 @@ struct cbns_ctx {
   bool dt_inflight = false;
   RefState ref{1.0, 0.0, 0.0};  // FP32-flux reference state
   FixList fix{};                  // faces of the tolerance-bound form left to the fixup pass
+  // the tolerance-bound form's fused steps store no primitives (its kernels
+  // form them from the state): set after such a step, cleared by
+  // refresh_prims before any call that reads the stored primitives
+  bool prim_stale = false;
   // live kernel timing
   unsigned prof = 0;  // bit k: time the launches of kernel class k
   std::vector<hipEvent_t> ev_pool;
@@ -126,6 +130,16 @@ static int dev_alloc(cbns_ctx* c, void** p, size_t bytes) {
   if (hipMalloc(p, bytes) != hipSuccess) return CBNS_ERR_NOMEM;
   c->allocs.push_back(*p);
   c->bytes += bytes;
+  return CBNS_OK;
+}
+
+// Recompute the stored primitives of Q where the fused steps of the
+// tolerance-bound form left them stale (calc_primitive over the ghosted
+// blocks, src/Block3d.cpp:359-385).
+static int refresh_prims(cbns_ctx* c) {
+  if (!c->prim_stale) return CBNS_OK;
+  CK(launch_primitive(c->db, c->hb.data(), c->dom.nb, CBNS_Q, c->ph, c->precision, c->stream));
+  c->prim_stale = false;
   return CBNS_OK;
 }
 
@@ -352,6 +366,7 @@ int cbns_boundary_condition(cbns_ctx* c, int which) {
 int cbns_calc_primitive(cbns_ctx* c, int which) {
   if (!c || (which != CBNS_Q && which != CBNS_QP)) return CBNS_ERR_ARG;
   c->s_ready = false;
+  c->prim_stale = false;
   CK(launch_primitive(c->db, c->hb.data(), c->dom.nb, which, c->ph, c->precision, c->stream));
   return CBNS_OK;
 }
@@ -401,6 +416,8 @@ int cbns_rec_riemann(cbns_ctx* c, int which, int dir) {
 int cbns_calc_viscous_flux_contribution(cbns_ctx* c) {
   if (!c) return CBNS_ERR_ARG;
   c->s_ready = false;
+  int st = refresh_prims(c);
+  if (st) return st;
   CK(launch_viscous(c->db, c->hb.data(), c->dom.nb, c->ph, c->precision, c->jband, c->stream));
   return CBNS_OK;
 }
@@ -524,6 +541,8 @@ int cbns_set_block_state(cbns_ctx* c, int b, int which, const double* in) {
 
 int cbns_get_block_primitive(cbns_ctx* c, int b, int f, double* out) {
   if (!c || !out || b < 0 || b >= c->dom.nb || f < 0 || f > 6) return CBNS_ERR_ARG;
+  int st = refresh_prims(c);
+  if (st) return st;
   const BlockDesc& B = c->hb[b];
   CK(hipStreamSynchronize(c->stream));
   if (f < 6 && c->precision == CBNS_FP32_FLUX) {  // float storage, widened
@@ -573,7 +592,8 @@ static int launch_sweeps(cbns_ctx* c, int stage) {
     CK(hipEventRecord(c->ev_fork, c->stream));
     CK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     CK(timed_on(c, c->side, 3, [&] {
-      return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->side, c->visc_wgs);
+      return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->side, c->visc_wgs,
+                              c->precision == CBNS_FP64_FAST ? in : -1);
     }));
     CK(hipEventRecord(c->ev_join, c->side));
   }
@@ -588,10 +608,18 @@ static int launch_sweeps(cbns_ctx* c, int stage) {
 }
 
 static int finish_stage(cbns_ctx* c, int stage, double dt) {
+  const int in = stage == 1 ? CBNS_Q : CBNS_QP;
   const int out = stage == 3 ? CBNS_Q : CBNS_QP;
   const BlockDesc* hb = c->hb.data();
+  // the tolerance-bound form stores no primitives in its fused steps: its
+  // viscous march forms them from the stage input, its stage tail skips
+  // them and its ghost fill copies the state alone
+  const bool lean = c->precision == CBNS_FP64_FAST;
   if (c->overlap) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
-  else CK(timed(c, 3, [&] { return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->stream); }));
+  else
+    CK(timed(c, 3, [&] {
+      return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->stream, 0, lean ? in : -1);
+    }));
   bool s_done = false;
   if (stage == 3) {  // the next step's calc_dt reduction rides on the last update
     CK(hipMemsetAsync(c->d_smax, 0, sizeof(unsigned long long), c->stream));
@@ -599,7 +627,7 @@ static int finish_stage(cbns_ctx* c, int stage, double dt) {
   }
   CK(timed(c, 4, [&] {
     return launch_update_fused(c->db, hb, c->dom.nb, dt, stage, c->ph, c->precision, c->jband, c->stream,
-                               stage == 3 ? c->d_smax : nullptr, stage == 3 ? c->d_flag : nullptr, &s_done);
+                               stage == 3 ? c->d_smax : nullptr, stage == 3 ? c->d_flag : nullptr, &s_done, !lean);
   }));
   if (stage == 3) {
     c->s_ready = s_done;
@@ -611,7 +639,11 @@ static int finish_stage(cbns_ctx* c, int stage, double dt) {
       c->dt_inflight = true;
     }
   }
-  CK(timed(c, 5, [&] { return launch_ghost_fill_prim(c->db, hb, c->dom, out, c->precision, c->stream); }));
+  CK(timed(c, 5, [&] {
+    return lean ? launch_ghost_fill(c->db, hb, c->dom, out, c->stream)
+                : launch_ghost_fill_prim(c->db, hb, c->dom, out, c->precision, c->stream);
+  }));
+  if (lean) c->prim_stale = true;
   return CBNS_OK;
 }
 
@@ -747,6 +779,8 @@ int cbns_kernel_times(cbns_ctx* c, double ms[CBNS_KCLASSES], int64_t n[CBNS_KCLA
 
 int cbns_diagnostics(cbns_ctx* c, double* ke, double* ens) {
   if (!c || !ke || !ens) return CBNS_ERR_ARG;
+  int st = refresh_prims(c);
+  if (st) return st;
   CK(launch_global_sums(c->db, c->dom, c->ph, 1, c->d_diag, c->d_diag + 2 * c->diag_parts, c->precision,
                         c->stream));
   double out[2];

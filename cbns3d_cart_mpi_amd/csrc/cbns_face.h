@@ -82,6 +82,15 @@ __device__ __forceinline__ double qdiv(double a, double b) {
   }
 }
 
+// 1/b within about one ulp: hardware reciprocal and two Newton steps (the
+// tolerance-bound FP64 form; not correctly rounded).
+__device__ __forceinline__ double rcp_nr(double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  y = __builtin_fma(y, __builtin_fma(-b, y, 1.0), y);
+  y = __builtin_fma(y, __builtin_fma(-b, y, 1.0), y);
+  return y;
+}
+
 // sqrt(x) as the compiler's IEEE sequence computes it (v_rsq_f64, then two
 // Newton-Raphson corrections of g ~ sqrt(x) and h ~ 1/(2 sqrt(x))), without
 // its input scaling for x < 2^-767 and its select of x itself for zero and

@@ -46,13 +46,6 @@
 
 namespace cbns {
 
-// 1/b within about one ulp: hardware reciprocal and two Newton steps.
-__device__ __forceinline__ double rcp_nr(double b) {
-  double y = __builtin_amdgcn_rcp(b);
-  y = __builtin_fma(y, __builtin_fma(-b, y, 1.0), y);
-  y = __builtin_fma(y, __builtin_fma(-b, y, 1.0), y);
-  return y;
-}
 
 // Left and right WENO5 reconstructions at one face of one characteristic
 // field f[0..5] (cells i-3 .. i+2): left from f0..f4, right from f5..f1

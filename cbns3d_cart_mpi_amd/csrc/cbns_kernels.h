@@ -120,15 +120,20 @@ hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Dom
 // viscous fields stored as float for PREC_FP32_FLUX, double otherwise
 hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision, int jb,
                           hipStream_t s);
-// wgs > 0: a fixed number of workgroups taking the tile units in turn
+// wgs > 0: a fixed number of workgroups taking the tile units in turn;
+// from_state >= 0 (FP64 only): u, v, w, T formed from Q[from_state] instead
+// of read from the stored primitives
 hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision,
-                            int jb, hipStream_t s, int wgs = 0);
+                            int jb, hipStream_t s, int wgs = 0, int from_state = -1);
 // smax/flag (optional): also reduce the next step's calc_dt spectral radius of
 // the updated cells (only over x-pairs, for blocks of even n0; *s_done says
 // whether it was done).
+// prims = false: the primitives of the updated cells are not stored (FP64
+// forms over x-pairs only; odd widths store them)
 hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
                                const Phys& ph, int precision, int jb, hipStream_t s,
-                               unsigned long long* smax = nullptr, int* flag = nullptr, bool* s_done = nullptr);
+                               unsigned long long* smax = nullptr, int* flag = nullptr, bool* s_done = nullptr,
+                               bool prims = true);
 hipError_t launch_ghost_fill_prim(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
                                   int precision, hipStream_t s);
 hipError_t launch_update(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,

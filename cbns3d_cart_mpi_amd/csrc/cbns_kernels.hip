@@ -150,10 +150,10 @@ hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Dom
 // keeping the nine distinct flux components.  mu is kept on the interior for
 // calc_dt (the reference stores it everywhere; only the interior is read).
 // Grid: (column tiles, z-chunks, blocks); see visc_tile_body.
-template <typename VT>
-__global__ __launch_bounds__(VX * VY) void k_visc_flux(const BlockDesc* __restrict__ blocks, Phys ph) {
+template <typename VT, bool FROMQ = false>
+__global__ __launch_bounds__(VX * VY) void k_visc_flux(const BlockDesc* __restrict__ blocks, Phys ph, int which) {
   const Tile tl = xcd_tile();
-  visc_tile_body<VT>(blocks[blockIdx.z], ph, tl.x, tl.y);
+  visc_tile_body<VT, FROMQ>(blocks[blockIdx.z], ph, tl.x, tl.y, which);
 }
 
 static dim3 visc_grid(const BlockDesc* hb, int nb) {
@@ -180,32 +180,39 @@ __global__ __launch_bounds__(TPB) void k_visc_div(const BlockDesc* __restrict__ 
 // The same march with a fixed number of workgroups, each taking units
 // (block, column tile, z-chunk) in turn: beside the face sweeps on a second
 // stream it then occupies only that many workgroup slots.
-template <typename VT>
+template <typename VT, bool FROMQ = false>
 __global__ __launch_bounds__(VX * VY) void k_visc_flux_units(const BlockDesc* __restrict__ blocks, Phys ph,
-                                                             int tiles, int chunks, int nb) {
+                                                             int tiles, int chunks, int nb, int which) {
   const int per_block = tiles * chunks, total = per_block * nb;
   for (int u = blockIdx.x; u < total; u += gridDim.x) {  // uniform per workgroup
     const int b = u / per_block, r = u - b * per_block;
-    visc_tile_body<VT>(blocks[b], ph, r % tiles, r / tiles);
+    visc_tile_body<VT, FROMQ>(blocks[b], ph, r % tiles, r / tiles, which);
     __syncthreads();  // the next unit refills the LDS planes this one read
   }
 }
 
 hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision,
-                            int jb, hipStream_t s, int wgs) {
+                            int jb, hipStream_t s, int wgs, int from_state) {
   (void)jb;  // the tile march fixes its own order
   const dim3 grid = visc_grid(hb, nb);
+  const bool fromq = from_state >= 0;  // u, v, w, T formed from Q[from_state]
+  const int w = fromq ? from_state : 0;
   if (wgs > 0) {
     if (precision == PREC_FP32_FLUX)
-      hipLaunchKernelGGL(k_visc_flux_units<float>, dim3(wgs), dim3(VX * VY), 0, s, db, ph, (int)grid.x, (int)grid.y, nb);
+      hipLaunchKernelGGL(k_visc_flux_units<float>, dim3(wgs), dim3(VX * VY), 0, s, db, ph, (int)grid.x, (int)grid.y, nb, w);
+    else if (fromq)
+      hipLaunchKernelGGL((k_visc_flux_units<double, true>), dim3(wgs), dim3(VX * VY), 0, s, db, ph, (int)grid.x,
+                         (int)grid.y, nb, w);
     else
-      hipLaunchKernelGGL(k_visc_flux_units<double>, dim3(wgs), dim3(VX * VY), 0, s, db, ph, (int)grid.x, (int)grid.y, nb);
+      hipLaunchKernelGGL(k_visc_flux_units<double>, dim3(wgs), dim3(VX * VY), 0, s, db, ph, (int)grid.x, (int)grid.y, nb, w);
     return hipGetLastError();
   }
   if (precision == PREC_FP32_FLUX)
-    hipLaunchKernelGGL(k_visc_flux<float>, grid, dim3(VX * VY), 0, s, db, ph);
+    hipLaunchKernelGGL(k_visc_flux<float>, grid, dim3(VX * VY), 0, s, db, ph, w);
+  else if (fromq)
+    hipLaunchKernelGGL((k_visc_flux<double, true>), grid, dim3(VX * VY), 0, s, db, ph, w);
   else
-    hipLaunchKernelGGL(k_visc_flux<double>, grid, dim3(VX * VY), 0, s, db, ph);
+    hipLaunchKernelGGL(k_visc_flux<double>, grid, dim3(VX * VY), 0, s, db, ph, w);
   return hipGetLastError();
 }
 
@@ -296,7 +303,7 @@ __global__ __launch_bounds__(TPB) void k_update_fused(const BlockDesc* __restric
 // With smax != nullptr it also reduces the next step's calc_dt spectral
 // radius of the updated cells into *smax (as k_dt does: integer atomic max on
 // the bits, one per workgroup) and raises *flag for a non-finite one.
-template <typename FT>
+template <typename FT, bool PRIMS = true>
 __global__ __launch_bounds__(TPB) void k_update_pair(const BlockDesc* __restrict__ blocks, double dt, int stage,
                                                      Phys ph, int jb, unsigned long long* smax, int* flag) {
   const Tile tl = xcd_tile();
@@ -305,7 +312,7 @@ __global__ __launch_bounds__(TPB) void k_update_pair(const BlockDesc* __restrict
   const bool want = smax != nullptr;  // uniform
   double s = 0.0;
   int bad = 0;
-  if (t < (B.n[0] / 2) * B.n[1] * B.n[2]) update_pair_body<FT>(B, t, dt, stage, ph, jb, want, &s, &bad);
+  if (t < (B.n[0] / 2) * B.n[1] * B.n[2]) update_pair_body<FT, PRIMS>(B, t, dt, stage, ph, jb, want, &s, &bad);
   if (!want) return;
   s = wave_max(s);
   __shared__ double red[TPB / 64];
@@ -322,7 +329,7 @@ __global__ __launch_bounds__(TPB) void k_update_pair(const BlockDesc* __restrict
 
 hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
                                const Phys& ph, int precision, int jb, hipStream_t s, unsigned long long* smax,
-                               int* flag, bool* s_done) {
+                               int* flag, bool* s_done, bool prims) {
   if (s_done) *s_done = false;
   long long mn = 0;
   bool even = true;
@@ -333,7 +340,9 @@ hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb,
   if (stage < 1 || stage > 3) return hipSuccess;
   if (even) {
     const dim3 grid(nblocks_for(mn / 2), nb);
-    if (precision != PREC_FP32_FLUX)
+    if (precision != PREC_FP32_FLUX && !prims)
+      hipLaunchKernelGGL((k_update_pair<double, false>), grid, dim3(TPB), 0, s, db, dt, stage, ph, jb, smax, flag);
+    else if (precision != PREC_FP32_FLUX)
       hipLaunchKernelGGL(k_update_pair<double>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb, smax, flag);
     else
       hipLaunchKernelGGL(k_update_pair<float>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb, smax, flag);
@@ -404,7 +413,10 @@ hipError_t launch_dt_reduce(const BlockDesc* db, const BlockDesc* hb, int nb, co
   for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
   unsigned gx = nblocks_for(mn);
   gx = gx > 1024 ? 1024 : gx;
-  if (precision == PREC_FP32_FLUX) hipLaunchKernelGGL(k_dt<float>, dim3(gx, nb), dim3(TPB), 0, s, db, ph, d_smax, d_flag);
+  // the FP32-flux form (float primitives) and the tolerance-bound form (no
+  // stored primitives) form the FP64 primitives from the state (dt_cell<float>)
+  if (precision == PREC_FP32_FLUX || precision == PREC_FP64_FAST)
+    hipLaunchKernelGGL(k_dt<float>, dim3(gx, nb), dim3(TPB), 0, s, db, ph, d_smax, d_flag);
   else hipLaunchKernelGGL(k_dt<double>, dim3(gx, nb), dim3(TPB), 0, s, db, ph, d_smax, d_flag);
   return hipGetLastError();
 }

@@ -179,3 +179,22 @@ def test_fast_form_scaled_state_faces_take_the_fixup_pass(N, dims):
     for nm in ("rho", "rho_u", "rho_v", "E"):
         assert err[nm] <= 1e-12, (nm, err[nm])
     assert np.abs(Q[..., 3] - R[..., 3]).max() / mom <= 1e-12
+
+
+def test_fast_form_primitives_refreshed_after_fused_steps():
+    """The tolerance-bound form's fused steps store no primitives (its
+    kernels form them from the state); a call that reads them afterwards
+    gets calc_primitive of the current state (src/Block3d.cpp:359-385),
+    ghosts included, and the method-level viscous pass that follows uses
+    them."""
+    N = (24, 20, 16)
+    with C.Solver(C.make_params(N), (2, 1, 1), precision=C.CBNS_FP64_FAST) as s:
+        s.init_tgv()
+        s.run(3)
+        for b in range(2):
+            Qg = s.get_block_state(b)
+            rr = Qg[..., 0]
+            uu, vv, ww = Qg[..., 1] / rr, Qg[..., 2] / rr, Qg[..., 3] / rr
+            pp = 0.4 * (Qg[..., 4] - 0.5 * rr * (uu * uu + vv * vv + ww * ww))
+            for f, want in enumerate((rr, uu, vv, ww, pp)):
+                assert np.array_equal(s.get_block_primitive(b, f), want), (b, f)

@@ -411,10 +411,11 @@ def test_tgv256_properties():
     assert np.all(np.abs(s1 - s0) <= 1e-13 * scale)
     w = Qa[..., 3]
     assert np.allclose(w, -w[::-1], atol=1e-13)
-    with solver(256, (2, 2, 2)) as b:
-        b.init_tgv()
-        b.run(2)
-        assert np.array_equal(b.get_state(), Qa)
+    for dims in ((2, 2, 2), (4, 2, 2)):
+        with solver(256, dims) as b:
+            b.init_tgv()
+            b.run(2)
+            assert np.array_equal(b.get_state(), Qa), dims
 
 
 @pytest.mark.slow
