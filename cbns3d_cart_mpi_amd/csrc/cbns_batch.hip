@@ -108,10 +108,12 @@ __global__ __launch_bounds__(TPB, D == 0 ? 4 : 3) void kb_face_fast(const BlockD
   });
 }
 
-template <typename VT>
+// FROMQ: the tolerance-bound solves' march, which forms u, v, w, T from
+// the stage input state `which` (they store no primitives)
+template <typename VT, bool FROMQ = false>
 __global__ __launch_bounds__(TPB) void kb_visc(const BlockDesc* __restrict__ blocks,
                                                const InstDesc* __restrict__ insts, TileTable tt, int jb,
-                                               int* ctr) {
+                                               int which, int* ctr) {
   // visc_tile_body synchronises the workgroup: every condition before it is
   // uniform over the workgroup (one tile, one block, one solve); for_tiles'
   // barrier after each unit also protects the LDS planes that the next unit
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(TPB) void kb_visc(const BlockDesc* __restrict__ blo
     if (!I.stepping) return;
     // one (column tile, z-chunk) unit of the viscous tile march per entry
     const int ntiles = ((B.n[0] + 2 * NGV + VX - 1) / VX) * ((B.n[1] + 2 * NGV + VY - 1) / VY);
-    visc_tile_body<VT>(B, I.ph, tr.tile % ntiles, tr.tile / ntiles);
+    visc_tile_body<VT, FROMQ>(B, I.ph, tr.tile % ntiles, tr.tile / ntiles, which);
   });
 }
 
@@ -141,7 +143,9 @@ __device__ __forceinline__ void flush_smax(InstDesc* insts, int inst, double acc
 // descriptors are read through the read-only insts (so that their uniform
 // fields come through the scalar cache) and only smax and nonfinite, which
 // no kernel of the stage reads, are written through insts_rw.
-template <typename FT>
+// PRIMS: whether the pair form stores the primitives (not for the
+// tolerance-bound solves, whose kernels form them from the state)
+template <typename FT, bool PRIMS = true>
 __global__ __launch_bounds__(TPB) void kb_update(const BlockDesc* __restrict__ blocks,
                                                  const InstDesc* __restrict__ insts, InstDesc* insts_rw,
                                                  TileTable tt, int stage, int jb, int* ctr) {
@@ -162,7 +166,7 @@ __global__ __launch_bounds__(TPB) void kb_update(const BlockDesc* __restrict__ b
         bad = 0;
       }
       if (t >= (B.n[0] / 2) * B.n[1] * B.n[2]) return;
-      update_pair_body<FT>(B, t, I.dt, stage, I.ph, jb, want_s, &acc, &bad);
+      update_pair_body<FT, PRIMS>(B, t, I.dt, stage, I.ph, jb, want_s, &acc, &bad);
     } else {
       if (t >= B.n[0] * B.n[1] * B.n[2]) return;
       update_body<FT>(B, t, I.dt, stage, I.ph, jb);
@@ -181,8 +185,9 @@ __global__ __launch_bounds__(TPB) void kb_ghost(const BlockDesc* __restrict__ bl
     if (!I.stepping) return;
     const int g = tr.tile * TPB + threadIdx.x;
     if (g >= B.sg - B.sn) return;
+    // the tolerance-bound solves' ghosts take the state alone
     if (I.precision == PREC_FP32_FLUX) ghost_body<float>(B, blocks, I.dom, I.block0, ghost_cell(B, g), which, true);
-    else ghost_body<double>(B, blocks, I.dom, I.block0, ghost_cell(B, g), which, true);
+    else ghost_body<double>(B, blocks, I.dom, I.block0, ghost_cell(B, g), which, I.precision != PREC_FP64_FAST);
   });
 }
 
@@ -215,8 +220,10 @@ __global__ __launch_bounds__(TPB) void kb_dt(const BlockDesc* __restrict__ block
     }
     const int t = tr.tile * TPB + threadIdx.x;
     if (t < B.n[0] * B.n[1] * B.n[2]) {
-      const double s = insts[inst].precision == PREC_FP32_FLUX ? dt_cell<float>(B, t, insts[inst].ph, &bad)
-                                                               : dt_cell<double>(B, t, insts[inst].ph, &bad);
+      // FP32-flux and tolerance-bound solves: calc_primitive's values formed from the state
+      const int pr = insts[inst].precision;
+      const double s = pr == PREC_FP32_FLUX || pr == PREC_FP64_FAST ? dt_cell<float>(B, t, insts[inst].ph, &bad)
+                                                                    : dt_cell<double>(B, t, insts[inst].ph, &bad);
       acc = (acc < s) ? s : acc;  // src/Block3d.cpp:546: a NaN never wins
     }
   });
@@ -294,12 +301,14 @@ hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable
 }
 
 hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, int precision,
-                             int wgs, int* ctr, hipStream_t s) {
+                             int which, int wgs, int* ctr, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
   if (precision == PREC_FP32_FLUX)
-    hipLaunchKernelGGL(kb_visc<float>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, jb, ctr);
+    hipLaunchKernelGGL(kb_visc<float>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, jb, which, ctr);
+  else if (precision == PREC_FP64_FAST)
+    hipLaunchKernelGGL((kb_visc<double, true>), dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, jb, which, ctr);
   else
-    hipLaunchKernelGGL(kb_visc<double>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, jb, ctr);
+    hipLaunchKernelGGL(kb_visc<double>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, jb, which, ctr);
   return hipGetLastError();
 }
 
@@ -310,8 +319,11 @@ hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTab
     wgs = (tt.n + NXCD - 1) / NXCD * NXCD;
     ctr = nullptr;
   }
-  if (precision == 0)
+  if (precision == PREC_FP64)
     hipLaunchKernelGGL(kb_update<double>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, insts, tt, stage, jb, ctr);
+  else if (precision == PREC_FP64_FAST)
+    hipLaunchKernelGGL((kb_update<double, false>), dim3(wgs), dim3(TPB), 0, s, blocks, insts, insts, tt, stage, jb,
+                       ctr);
   else
     hipLaunchKernelGGL(kb_update<float>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, insts, tt, stage, jb, ctr);
   return hipGetLastError();
@@ -383,8 +395,10 @@ hipError_t batch_grids(BatchGrids* g) {
   res((const void*)kb_face_fast<2>, &g->face[2][2]);
   res((const void*)kb_visc<double>, &g->visc[0]);
   res((const void*)kb_visc<float>, &g->visc[1]);
+  res((const void*)kb_visc<double, true>, &g->visc[2]);
   res((const void*)kb_update<double>, &g->update[0]);
   res((const void*)kb_update<float>, &g->update[1]);
+  res((const void*)kb_update<double, false>, &g->update[2]);
   res((const void*)kb_ghost, &g->ghost);
   res((const void*)kb_dt, &g->dt);
   return e;

@@ -39,11 +39,19 @@ struct Inst {
   int block0 = 0;
 };
 
-enum { T_FACE64 = 0, T_FACE32 = 3, T_FACEF = 6, T_VISC64 = 9, T_VISC32, T_UPD64, T_UPD32, T_GHOST, T_DT, T_COUNT };
+enum {
+  T_FACE64 = 0, T_FACE32 = 3, T_FACEF = 6,
+  T_VISC64 = 9, T_VISC32, T_VISCF,
+  T_UPD64, T_UPD32, T_UPDF,
+  T_GHOST, T_DT, T_COUNT
+};
 
-// Face-sweep form of a solve: 0 FP64, 1 FP32-flux, 2 tolerance-bound FP64.
+// Kernel form of a solve: 0 FP64, 1 FP32-flux, 2 tolerance-bound FP64.
 int face_form(int precision) { return precision == CBNS_FP32_FLUX ? 1 : (precision == CBNS_FP64_FAST ? 2 : 0); }
 const int FACE_CLASS[3] = {T_FACE64, T_FACE32, T_FACEF};
+const int VISC_CLASS[3] = {T_VISC64, T_VISC32, T_VISCF};
+const int UPD_CLASS[3] = {T_UPD64, T_UPD32, T_UPDF};
+const int FORM_PREC[3] = {PREC_FP64, PREC_FP32_FLUX, PREC_FP64_FAST};
 
 }  // namespace
 
@@ -233,8 +241,8 @@ static int rebuild(cbns_batch* b) {
       const int vunits = ((B.n[0] + 2 * NGV + VX - 1) / VX) * ((B.n[1] + 2 * NGV + VY - 1) / VY) *
                          ((B.n[2] + 2 * NGV + VZ - 1) / VZ);
       for (int t = 0; t < vunits; t++)  // whole-workgroup units
-        tab[I.precision == CBNS_FP32_FLUX ? T_VISC32 : T_VISC64].push_back({bi, t});
-      add(I.precision == CBNS_FP32_FLUX ? T_UPD32 : T_UPD64, B.n[0] % 2 == 0 ? B.sn / 2 : B.sn);  // x-pairs for even n0
+        tab[VISC_CLASS[face_form(I.precision)]].push_back({bi, t});
+      add(UPD_CLASS[face_form(I.precision)], B.n[0] % 2 == 0 ? B.sn / 2 : B.sn);  // x-pairs for even n0
       add(T_GHOST, B.sg - B.sn);  // ghost cells only
       add(T_DT, B.sn);
     }
@@ -273,7 +281,7 @@ static int rebuild(cbns_batch* b) {
   auto tail = [&]() { return b->wg_mem > 0 ? b->wg_mem : 0; };
   // every persistent launch of the step takes its own block of counters, all
   // zeroed by the first node of the graph
-  constexpr int LAUNCHES = 1 + 3 * 14;  // dt; per stage 2 viscous, 9 face, 2 update, 1 ghost
+  constexpr int LAUNCHES = 1 + 3 * 16;  // dt; per stage 3 viscous, 9 face, 3 update, 1 ghost
   constexpr int K = BATCH_CTR_PER_LAUNCH * BATCH_CTR_STRIDE;
   static_assert(LAUNCHES * K <= BATCH_COUNTERS, "counter block");
   int used = 0;
@@ -290,12 +298,9 @@ static int rebuild(cbns_batch* b) {
       e = hipEventRecord(b->ev_fork, s);
       if (e == hipSuccess) e = hipStreamWaitEvent(sv, b->ev_fork, 0);
     }
-    if (e == hipSuccess)
-      e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC64], b->jb, 0, ov ? b->visc_side_wgs : G.visc[0],
-                            ctr(), sv);
-    if (e == hipSuccess)
-      e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[T_VISC32], b->jb, 1, ov ? b->visc_side_wgs : G.visc[1],
-                            ctr(), sv);
+    for (int f = 0; f < 3 && e == hipSuccess; f++)
+      e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[VISC_CLASS[f]], b->jb, FORM_PREC[f], in,
+                            ov ? b->visc_side_wgs : G.visc[f], ctr(), sv);
     if (ov && e == hipSuccess) e = hipEventRecord(b->ev_join, sv);
     for (int d = 0; d < 3 && e == hipSuccess; d++)
       for (int f = 0; f < 3 && e == hipSuccess; f++)
@@ -304,10 +309,9 @@ static int rebuild(cbns_batch* b) {
     // faces of tolerance-bound solves whose range checks failed
     if (b->has_fast && e == hipSuccess) e = batch_launch_fixup(b->d_blocks, b->d_insts, (int)all.size(), in, b->fix, s);
     if (ov && e == hipSuccess) e = hipStreamWaitEvent(s, b->ev_join, 0);  // join before the stage tail
-    if (e == hipSuccess)
-      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD64], stage, b->jb, 0, tail(), ctr(), s);
-    if (e == hipSuccess)
-      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[T_UPD32], stage, b->jb, 1, tail(), ctr(), s);
+    for (int f = 0; f < 3 && e == hipSuccess; f++)
+      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[UPD_CLASS[f]], stage, b->jb, FORM_PREC[f], tail(),
+                              ctr(), s);
     if (e == hipSuccess)
       e = batch_launch_ghost(b->d_blocks, b->d_insts, b->tt[T_GHOST], out, tail(), ctr(), s);
   }

@@ -195,6 +195,6 @@ def test_fast_form_primitives_refreshed_after_fused_steps():
             Qg = s.get_block_state(b)
             rr = Qg[..., 0]
             uu, vv, ww = Qg[..., 1] / rr, Qg[..., 2] / rr, Qg[..., 3] / rr
-            pp = 0.4 * (Qg[..., 4] - 0.5 * rr * (uu * uu + vv * vv + ww * ww))
+            pp = (1.4 - 1.0) * (Qg[..., 4] - 0.5 * rr * (uu * uu + vv * vv + ww * ww))
             for f, want in enumerate((rr, uu, vv, ww, pp)):
                 assert np.array_equal(s.get_block_primitive(b, f), want), (b, f)
