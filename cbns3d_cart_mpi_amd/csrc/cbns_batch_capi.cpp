@@ -141,10 +141,9 @@ static uint64_t solve_bytes(const cbns_params* p, const int dims[3]) {
           n[a] = (bc[a] == dims[a] - 1) ? N[a] - (dims[a] - 1) * base : base;
         }
         const uint64_t g = (n[0] + 8) * (n[1] + 8) * (n[2] + 8);
-        const uint64_t v = (n[0] + 4) * (n[1] + 4) * (n[2] + 4);
         const uint64_t in = n[0] * n[1] * n[2];
         const uint64_t f = (n[0] + 1) * n[1] * n[2] + n[0] * (n[1] + 1) * n[2] + n[0] * n[1] * (n[2] + 1);
-        total += 8 * (2 * NEQ * g + NEQ * f + 6 * g + in + 9 * v);
+        total += 8 * (2 * NEQ * g + NEQ * f + 6 * g + in + 4 * in);
       }
   return total + sizeof(BlockDesc) * dims[0] * dims[1] * dims[2];
 }
@@ -238,8 +237,7 @@ static int rebuild(cbns_batch* b) {
         for (int t = 0; t < nt; t++) tab[cls].push_back({bi, t});
       };
       for (int d = 0; d < 3; d++) add(FACE_CLASS[face_form(I.precision)] + d, face_items(B, d));
-      const int vunits = ((B.n[0] + 2 * NGV + VX - 1) / VX) * ((B.n[1] + 2 * NGV + VY - 1) / VY) *
-                         ((B.n[2] + 2 * NGV + VZ - 1) / VZ);
+      const int vunits = ((B.n[0] + VX - 1) / VX) * ((B.n[1] + VY - 1) / VY) * ((B.n[2] + VZ - 1) / VZ);
       for (int t = 0; t < vunits; t++)  // whole-workgroup units
         tab[VISC_CLASS[face_form(I.precision)]].push_back({bi, t});
       add(UPD_CLASS[face_form(I.precision)], B.n[0] % 2 == 0 ? B.sn / 2 : B.sn);  // x-pairs for even n0
@@ -461,14 +459,13 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
         }
         B.inst = id;
         B.sg = (long long)B.g[0] * B.g[1] * B.g[2];
-        B.sv = (long long)(B.n[0] + 2 * NGV) * (B.n[1] + 2 * NGV) * (B.n[2] + 2 * NGV);
         B.sn = (long long)B.n[0] * B.n[1] * B.n[2];
         for (int d = 0; d < 3; d++)
           B.sf[d] = (long long)(B.n[0] + (d == 0)) * (B.n[1] + (d == 1)) * (B.n[2] + (d == 2));
         ok = (B.Q[0] = alloc(8 * NEQ * B.sg)) && (B.Q[1] = alloc(8 * NEQ * B.sg)) &&
              (B.F[0] = alloc(8 * NEQ * B.sf[0])) && (B.F[1] = alloc(8 * NEQ * B.sf[1])) &&
              (B.F[2] = alloc(8 * NEQ * B.sf[2])) && (B.prim = alloc((precision == CBNS_FP32_FLUX ? 4 : 8) * 6 * B.sg)) && (B.mu = alloc(8 * B.sn)) &&
-             (B.visc = alloc((precision == CBNS_FP32_FLUX ? 4 : 8) * 9 * B.sv));
+             (B.dfv = alloc(8 * 4 * B.sn));
         I.blocks.push_back(B);
       }
   if (ok) ok = (I.dblocks = (BlockDesc*)alloc(sizeof(BlockDesc) * I.blocks.size())) != nullptr;

@@ -316,12 +316,15 @@ __device__ __forceinline__ void face_eval_body(const Phys& ph, int t, const doub
   for (int e = 0; e < 5; e++) flux[(long long)t * 5 + e] = fl[e];
 }
 
-// ---- viscous fluxes on the extended box [-2, n+2)^3 -----------------------
+// ---- viscous fluxes and their divergence ------------------------------------
 // calc_gradient (src/calc_gradient.cpp:17-66), calc_viscous_terms
-// (src/Block3d.cpp:750-810) and calc_viscous_flux (src/Block3d.cpp:812-868),
-// keeping the nine distinct components txx tyy tzz txy txz tyz ex ey ez
-// (stored as VT: double for the FP64 forms, float for FP32-flux); mu is kept
-// on the interior for calc_dt.
+// (src/Block3d.cpp:750-810), calc_viscous_flux (src/Block3d.cpp:812-868) and
+// the divergence of calc_viscous_flux_contribution (src/Block3d.cpp:879-916)
+// in one tile march.  Only the divergence (4 fields on the interior, B.dfv)
+// and mu (on the interior, for calc_dt) are written to HBM: the nine flux
+// components txx tyy tzz txy txz tyz ex ey ez live in LDS and registers,
+// rounded to VT as stored fields would be (double for the FP64 forms, float
+// for FP32-flux).
 // Sutherland viscosity, stress tensor, heat flux and energy fluxes of one
 // point from its gradients gr[u, v, w, T][x, y, z], temperature and velocity
 // (src/Block3d.cpp:777-804, 843-862).  out = txx tyy tzz txy txz tyz ex ey ez;
@@ -354,20 +357,10 @@ __device__ __forceinline__ void visc_terms(const Phys& ph, const double (&gr)[4]
   out[8] = uu * txz + vv * tyz + ww * tzz - qz;
 }
 
-// Tile march of the viscous fluxes.  A workgroup owns a VX x VY column tile
-// of the extended box and marches VZ planes along z.  Each plane of u, v, w,
-// T with its 2-point x/y halo is staged in LDS (double-buffered, one barrier
-// per plane) for the x- and y-gradients; each thread keeps its own column's
-// five z-planes in registers for the z-gradient.  The next plane's loads are
-// issued before the current plane's arithmetic.  Every value is formed by the
-// same expressions as a one-point-per-thread evaluation (grad4, visc_terms).  One call = one (column tile, z-chunk) unit, made by all
-// VX * VY threads of a workgroup with the same tile and chunk (it
-// synchronises the workgroup).
 // FROMQ (tolerance-bound FP64 form, whose stage tail stores no primitives):
 // u, v, w, T are formed from the conserved values of Q[which] as each
-// plane is staged (calc_primitive's expressions, src/Block3d.cpp:369-381,
-// with one refined reciprocal of rho); the loads of the next plane are
-// still issued before the current plane's arithmetic.
+// point is staged (calc_primitive's expressions, src/Block3d.cpp:369-381,
+// with one refined reciprocal of rho).
 __device__ __forceinline__ void prim_uvwT(const double (&q)[5], const Phys& ph, double (&o)[4]) {
 #pragma clang fp contract(off)
   const double ir = rcp_nr(q[0]);
@@ -379,23 +372,63 @@ __device__ __forceinline__ void prim_uvwT(const double (&q)[5], const Phys& ph, 
   o[3] = (ph.gM2 * p) * ir;
 }
 
+// Tile march.  A workgroup owns a VX x VY tile of interior columns and
+// marches the planes p of one z-chunk [Z0, Z1), from Z0 - 2 to Z1 + 1.  Per
+// plane it
+//  - stages u, v, w, T of the (VX + 8) x (VY + 8) region around the tile in
+//    LDS (double-buffered) for the x- and y-gradients;
+//  - evaluates the fluxes at each thread's own column point, with the
+//    z-gradient from the thread's five-plane register queue; the z-fluxes
+//    Gv = (txz, tyz, tzz, ez) enter a five-plane queue of their own;
+//  - on the chunk's planes, also evaluates Ev = (txx, txy, txz, ex) at the
+//    two columns beside each x-side of the tile and Fv = (txy, tyy, tyz, ey)
+//    at the two rows beside each y-side (one such side point for each of the
+//    first 4 (VX + VY) threads, with a queue of the two velocities whose
+//    z-gradients it needs), and puts Ev and Fv of the tile and its sides in
+//    LDS for the x- and y-parts of the divergence;
+//  - forms the x- and y-parts of the divergence of plane p - 1 from the Ev
+//    and Fv that the previous plane left in LDS (Ev and Fv are
+//    double-buffered, so that one barrier per plane suffices), and completes
+//    the divergence of plane p - 2 with the z-part from the Gv queue, summed
+//    in the reference's order (x-part, + y-part, + z-part).
+// The next plane's loads are issued before the current plane's arithmetic.
+// Every value is formed by the same expressions as the reference's separate
+// passes (grad4, visc_terms, the divergence stencil).  One call = one
+// (column tile, z-chunk) unit, made by all VX * VY threads of a workgroup with
+// the same tile and chunk (it synchronises the workgroup).
 template <typename VT, bool FROMQ = false>
 __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& ph, int tile, int chunk,
                                                int which = 0) {
   static_assert(!FROMQ || sizeof(VT) == 8, "the state is FP64");
-  const int v0 = B.n[0] + 2 * NGV, v1 = B.n[1] + 2 * NGV, v2 = B.n[2] + 2 * NGV;
-  const int ntx = (v0 + VX - 1) / VX, nty = (v1 + VY - 1) / VY;
+  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  const int ntx = (n0 + VX - 1) / VX, nty = (n1 + VY - 1) / VY;
   const int Z0 = chunk * VZ;
-  if (tile >= ntx * nty || Z0 >= v2) return;  // uniform per workgroup
-  const int Z1 = Z0 + VZ < v2 ? Z0 + VZ : v2;
+  if (tile >= ntx * nty || Z0 >= n2) return;  // uniform per workgroup
+  const int Z1 = Z0 + VZ < n2 ? Z0 + VZ : n2;
   const int X0 = (tile % ntx) * VX, Y0 = (tile / ntx) * VY;
+  const int wx = n0 - X0 < VX ? n0 - X0 : VX, wy = n1 - Y0 < VY ? n1 - Y0 : VY;  // the tile's extent
   const int tx = threadIdx.x % VX, ty = threadIdx.x / VX;
-  const int bx = X0 + tx, by = Y0 + ty;  // box point; its cell is (bx - 2, by - 2, bz - 2)
-  const bool act = bx < v0 && by < v1;
-  constexpr int LX = VX + 4, LY = VY + 4, NT = VX * VY, NP = (LX * LY + NT - 1) / NT;
-  __shared__ VT L[2][4][LY][LX];  // planes in the storage type, widened at use
-  const long long sg = B.sg;
+  const bool act = tx < wx && ty < wy;
+  // side point of this thread (offsets from X0, Y0): kind 1 = Ev at an x-side
+  // column, 2 = Fv at a y-side row, 0 = none
+  int sk = 0, sx = 0, sy = 0;
+  if (threadIdx.x < 4 * VY) {
+    const int r = threadIdx.x >> 2, s = threadIdx.x & 3;
+    sx = s < 2 ? s - 2 : wx + s - 2;
+    sy = r;
+    sk = r < wy ? 1 : 0;
+  } else if (threadIdx.x < 4 * VY + 4 * VX) {
+    const int h = threadIdx.x - 4 * VY, c = h % VX, s = h / VX;
+    sx = c;
+    sy = s < 2 ? s - 2 : wy + s - 2;
+    sk = c < wx ? 2 : 0;
+  }
+  constexpr int SX = VX + 8, SY = VY + 8, NT = VX * VY, NP = (SX * SY + NT - 1) / NT;
+  __shared__ VT L[2][4][SY][SX];    // u, v, w, T of the staged region, in the storage type
+  __shared__ VT LE[2][4][VY][VX + 4];  // Ev of the tile and its x-sides
+  __shared__ VT LF[2][4][VY + 4][VX];  // Fv of the tile and its y-sides
   const int sz = B.g[0] * B.g[1];
+  const long long sg = B.sg;
   // source fields: u, v, w, T of the primitives (sharing VT), or with FROMQ
   // rho, rho u, rho v, rho w, E of the state
   constexpr int NF = FROMQ ? 5 : 4;
@@ -412,150 +445,191 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
       for (int f = 0; f < 4; f++) o[f] = (double)x[f];
     }
   };
-  // LDS entries e = threadIdx.x + p NT this thread fills: cell (X0 + lx - 4,
-  // Y0 + ly - 4).  A position past the ghosted block (no active point's
-  // stencil reaches it) or past the tile, and an inactive point's column,
-  // is either skipped (predicated loads) or, with CLAMP, clamped to the
-  // block's last ghost cell so that every load is unconditional.  Both give
-  // the same values at every position that is used; each variant takes the
-  // form the compiler schedules better for it (with float storage the
-  // predicated form leaves waits on the prefetch inside the arithmetic).
-  constexpr bool CLAMP = sizeof(VT) == 4;
+  // Staged entry e = threadIdx.x + q NT (e < SX SY) is the point (X0 - 4 +
+  // e % SX, Y0 - 4 + e / SX).  Positions past the block's last ghost cell
+  // are clamped to it: no used value comes from them (the farthest point a
+  // side point's stencil reaches is (X0 + wx + 3, Y0 + wy + 3)), and every
+  // load stays inside the ghosted block.
   int fidx[NP];
-  bool fok[NP];
 #pragma unroll
-  for (int p = 0; p < NP; p++) {
-    const int e = threadIdx.x + p * NT;
-    const int lx = e % LX, ly = e / LX;
-    const int cx = X0 + lx - 4, cy = Y0 + ly - 4;
-    fok[p] = CLAMP || (e < LX * LY && cx < B.n[0] + NG && cy < B.n[1] + NG);
-    if (CLAMP)
-      fidx[p] = gidx(B, cx < B.n[0] + NG ? cx : B.n[0] + NG - 1, cy < B.n[1] + NG ? cy : B.n[1] + NG - 1, Z0 - 2);
-    else fidx[p] = fok[p] ? gidx(B, cx, cy, Z0 - 2) : 0;
+  for (int q = 0; q < NP; q++) {
+    const int e = threadIdx.x + q * NT;
+    const int cx = X0 - 4 + e % SX, cy = Y0 - 4 + e / SX;
+    fidx[q] = gidx(B, cx < n0 + NG - 1 ? cx : n0 + NG - 1, cy < n1 + NG - 1 ? cy : n1 + NG - 1, 0);
   }
-  const bool ld = CLAMP || act;
-  VT pf[NF][NP], rn[NF];  // next-plane prefetch (raw source values)
-  double r[4][5];
-  int ci = CLAMP ? gidx(B, (bx < v0 ? bx : v0 - 1) - 2, (by < v1 ? by : v1 - 1) - 2, Z0 - 2)
-                 : (act ? gidx(B, bx - 2, by - 2, Z0 - 2) : 0);
+  // own column (inactive threads: the tile's last column, loaded and unused)
+  // and side column (threads without a side point: the tile's first column)
+  const int ix = X0 + (tx < wx ? tx : wx - 1), iy = Y0 + (ty < wy ? ty : wy - 1);
+  const int ci = gidx(B, ix, iy, 0);
+  const int cs = sk ? gidx(B, X0 + sx, Y0 + sy, 0) : gidx(B, X0, Y0, 0);
+  const int sa = sk == 2 ? 1 : 0;  // the side point's own velocity: u (x-side) or v (y-side); the other is w
+  // queues in the staging type (every value is exact in it: VT is the
+  // primitives' storage type, and FROMQ forms double values)
+  VT r[4][5];   // u, v, w, T of the own column, planes p - 2 .. p + 2
+  VT sq[2][5];  // the side point's two velocities, planes p - 2 .. p + 2
+  VT gq[4][5];      // Gv of the own column, planes p - 4 .. p
+  double dq[4];     // x- plus y-part of the divergence of plane p - 2
+  VT pf[NF][NP], rn[NF], hn[NF];  // next-plane prefetch (raw source values)
 #pragma unroll
   for (int d = 0; d < 4; d++) {
-    VT x[NF];
+    VT x[NF], y[NF];
     double o[4];
 #pragma unroll
-    for (int f = 0; f < NF; f++) x[f] = ld ? P[foff[f] + ci + (d - 2) * sz] : (VT)(FROMQ && f == 0 ? 1 : 0);
+    for (int f = 0; f < NF; f++) {
+      x[f] = P[foff[f] + ci + (Z0 - 4 + d) * sz];
+      y[f] = P[foff[f] + cs + (Z0 - 4 + d) * sz];
+    }
     uvwT(x, o);
 #pragma unroll
-    for (int f = 0; f < 4; f++) r[f][d] = o[f];
+    for (int f = 0; f < 4; f++) r[f][d] = (VT)o[f];
+    uvwT(y, o);
+    sq[0][d] = (VT)(sa ? o[1] : o[0]);
+    sq[1][d] = (VT)o[2];
+  }
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+#pragma unroll
+    for (int d = 0; d < 5; d++) gq[e][d] = (VT)0;
+    dq[e] = 0.0;
   }
 #pragma unroll
   for (int f = 0; f < NF; f++) {
-    rn[f] = ld ? P[foff[f] + ci + 2 * sz] : (VT)(FROMQ && f == 0 ? 1 : 0);
+    rn[f] = P[foff[f] + ci + Z0 * sz];
+    hn[f] = P[foff[f] + cs + Z0 * sz];
 #pragma unroll
-    for (int p = 0; p < NP; p++) pf[f][p] = fok[p] ? P[foff[f] + fidx[p]] : (VT)(FROMQ && f == 0 ? 1 : 0);
+    for (int q = 0; q < NP; q++) pf[f][q] = P[foff[f] + fidx[q] + (Z0 - 2) * sz];
   }
-  gptr<VT> __restrict__ V = gmem((VT*)B.visc);
-  const long long sv = B.sv;
-  for (int bz = Z0; bz < Z1; bz++, ci += sz) {
-    const int buf = (bz - Z0) & 1;
-    if constexpr (FROMQ) {
+  gptr<double> __restrict__ DV = gmem(B.dfv);
+  const long long sn = B.sn;
+  // Ev = (txx, txy, txz, ex), Fv = (txy, tyy, tyz, ey), Gv = (txz, tyz, tzz, ez)
+  const int Ef[4] = {0, 3, 4, 6}, Ff[4] = {3, 1, 5, 7}, Gf[4] = {4, 5, 2, 8};
+  const double c0 = 1.0 / 12.0, c1 = -2.0 / 3.0;
+  for (int p = Z0 - 2; p < Z1 + 2; p++) {
+    const int buf = (p - Z0) & 1;
+    // stage plane p; the column queues take plane p + 2
 #pragma unroll
-      for (int p = 0; p < NP; p++) {
-        const int e = threadIdx.x + p * NT;
-        VT x[NF];
-        double o[4];
+    for (int q = 0; q < NP; q++) {
+      const int e = threadIdx.x + q * NT;
+      VT x[NF];
+      double o[4];
 #pragma unroll
-        for (int f = 0; f < NF; f++) x[f] = pf[f][p];
-        uvwT(x, o);
-        if (e < LX * LY) {
+      for (int f = 0; f < NF; f++) x[f] = pf[f][q];
+      uvwT(x, o);
+      if (e < SX * SY) {
 #pragma unroll
-          for (int f = 0; f < 4; f++) (&L[buf][f][0][0])[e] = (VT)o[f];
-        }
+        for (int f = 0; f < 4; f++) (&L[buf][f][0][0])[e] = (VT)o[f];
       }
+    }
+    {
       double o[4];
       uvwT(rn, o);
 #pragma unroll
-      for (int f = 0; f < 4; f++) r[f][4] = o[f];
-    } else {
-#pragma unroll
-      for (int f = 0; f < 4; f++) {
-#pragma unroll
-        for (int p = 0; p < NP; p++) {
-          const int e = threadIdx.x + p * NT;
-          if (e < LX * LY) (&L[buf][f][0][0])[e] = pf[f][p];
-        }
-        r[f][4] = (double)rn[f];
-      }
+      for (int f = 0; f < 4; f++) r[f][4] = (VT)o[f];
+      uvwT(hn, o);
+      sq[0][4] = (VT)(sa ? o[1] : o[0]);
+      sq[1][4] = (VT)o[2];
     }
     __syncthreads();
-    if (bz + 1 < Z1) {
+    if (p + 1 < Z1 + 2) {  // planes p + 1 (staging) and p + 3 (queues): at most n2 + 3, the last ghost plane
 #pragma unroll
       for (int f = 0; f < NF; f++) {
 #pragma unroll
-        for (int p = 0; p < NP; p++)
-          if (fok[p]) pf[f][p] = P[foff[f] + fidx[p] + (long long)(bz + 1 - Z0) * sz];
-        if (ld) rn[f] = P[foff[f] + ci + 3 * sz];
+        for (int q = 0; q < NP; q++) pf[f][q] = P[foff[f] + fidx[q] + (p + 1) * sz];
+        rn[f] = P[foff[f] + ci + (p + 3) * sz];
+        hn[f] = P[foff[f] + cs + (p + 3) * sz];
       }
     }
+    // x- and y-parts of the divergence of plane p - 1 (its Ev, Fv are in LDS)
+    double dxy[4] = {0.0, 0.0, 0.0, 0.0};
+    if (act && p - 1 >= Z0 && p - 1 < Z1) {
+      const int b = buf ^ 1;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        double d = divc(c0 * LE[b][e][ty][tx] + c1 * LE[b][e][ty][tx + 1] - c1 * LE[b][e][ty][tx + 3] -
+                            c0 * LE[b][e][ty][tx + 4],
+                        ph.dx, ph.rdx, ph.fdx);
+        d += divc(c0 * LF[b][e][ty][tx] + c1 * LF[b][e][ty + 1][tx] - c1 * LF[b][e][ty + 3][tx] -
+                      c0 * LF[b][e][ty + 4][tx],
+                  ph.dy, ph.rdy, ph.fdy);
+        dxy[e] = d;
+      }
+    }
+    const bool out = p >= Z0 && p < Z1;  // uniform: a plane of the chunk
     if (act) {
       double gr[4][3];
 #pragma unroll
       for (int f = 0; f < 4; f++) {
-        gr[f][0] = grad4(&L[buf][f][ty + 2][0], tx + 2, 1, ph.dx, ph.rdx, ph.fdx);
-        gr[f][1] = grad4(&L[buf][f][0][tx + 2], (ty + 2) * LX, LX, ph.dy, ph.rdy, ph.fdy);
+        gr[f][0] = grad4(&L[buf][f][ty + 4][0], tx + 4, 1, ph.dx, ph.rdx, ph.fdx);
+        gr[f][1] = grad4(&L[buf][f][0][tx + 4], (ty + 4) * SX, SX, ph.dy, ph.rdy, ph.fdy);
         gr[f][2] = grad4(&r[f][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
       }
       double o[9], mu_l;
       visc_terms(ph, gr, r[3][2], r[0][2], r[1][2], r[2][2], o, mu_l);
-      const int i = bx - 2, j = by - 2, k = bz - 2;
-      if (i >= 0 && i < B.n[0] && j >= 0 && j < B.n[1] && k >= 0 && k < B.n[2])
-        gmem(B.mu)[i + B.n[0] * (j + B.n[1] * k)] = mu_l;
-      const int vt = bx + v0 * (by + v1 * bz);
 #pragma unroll
-      for (int q = 0; q < 9; q++) V[q * sv + vt] = (VT)o[q];
+      for (int e = 0; e < 4; e++) gq[e][4] = (VT)o[Gf[e]];
+      if (out) {
+        gmem(B.mu)[ix + n0 * (iy + n1 * p)] = mu_l;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          LE[buf][e][ty][tx + 2] = (VT)o[Ef[e]];
+          LF[buf][e][ty + 2][tx] = (VT)o[Ff[e]];
+        }
+      }
+    }
+    if (out && sk) {
+      const int lx = sx + 4, ly = sy + 4;
+      double gr[4][3];
+#pragma unroll
+      for (int f = 0; f < 4; f++) {
+        gr[f][0] = grad4(&L[buf][f][ly][0], lx, 1, ph.dx, ph.rdx, ph.fdx);
+        gr[f][1] = grad4(&L[buf][f][0][lx], ly * SX, SX, ph.dy, ph.rdy, ph.fdy);
+      }
+      // z-gradients: the side's own velocity and w; the others enter no
+      // component it keeps (Ev takes u_z, w_z; Fv takes v_z, w_z)
+      const double za = grad4(&sq[0][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
+      gr[0][2] = sa ? 0.0 : za;
+      gr[1][2] = sa ? za : 0.0;
+      gr[2][2] = grad4(&sq[1][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
+      gr[3][2] = 0.0;
+      double o[9], mu_s;
+      visc_terms(ph, gr, (double)L[buf][3][ly][lx], (double)L[buf][0][ly][lx], (double)L[buf][1][ly][lx],
+                 (double)L[buf][2][ly][lx], o, mu_s);
+      if (sk == 1) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) LE[buf][e][sy][sx + 2] = (VT)o[Ef[e]];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++) LF[buf][e][sy + 2][sx] = (VT)o[Ff[e]];
+      }
+    }
+    // the divergence of plane p - 2 (< Z1) is complete
+    if (act && p - 2 >= Z0) {
+      const long long o = ix + (long long)n0 * (iy + (long long)n1 * (p - 2));
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const double dz = c0 * gq[e][0] + c1 * gq[e][1] - c1 * gq[e][3] - c0 * gq[e][4];
+        DV[e * sn + o] = dq[e] + divc(dz, ph.dz, ph.rdz, ph.fdz);
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      dq[e] = dxy[e];
+#pragma unroll
+      for (int d = 0; d < 4; d++) gq[e][d] = gq[e][d + 1];
     }
 #pragma unroll
     for (int f = 0; f < 4; f++)
 #pragma unroll
       for (int d = 0; d < 4; d++) r[f][d] = r[f][d + 1];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      sq[0][d] = sq[0][d + 1];
+      sq[1][d] = sq[1][d + 1];
+    }
   }
 }
 
-
-// ---- stage tail: viscous divergence + RK3 update + primitives --------------
-// Viscous divergence of interior cell (i, j, k) from the nine viscous fields
-// (src/Block3d.cpp:879-916).
-template <typename VT>
-__device__ __forceinline__ void visc_div(const BlockDesc& B, int i, int j, int k, const Phys& ph,
-                                         double (&dfv)[4]) {
-  const int v0 = B.n[0] + 2 * NGV, v1 = B.n[1] + 2 * NGV;
-  const int cv = (i + NGV) + v0 * ((j + NGV) + v1 * (k + NGV));
-  gptr<const VT> __restrict__ V = gmem<const VT>((const VT*)B.visc);
-  const long long sv = B.sv;
-  // Ev = (txx, txy, txz, ex), Fv = (txy, tyy, tyz, ey), Gv = (txz, tyz, tzz, ez)
-  const int Ef[4] = {0, 3, 4, 6}, Ff[4] = {3, 1, 5, 7}, Gf[4] = {4, 5, 2, 8};
-#pragma unroll
-  for (int e = 0; e < 4; e++) {
-    double d = divc(cdiff4(V + Ef[e] * sv, cv, 1), ph.dx, ph.rdx, ph.fdx);
-    d += divc(cdiff4(V + Ff[e] * sv, cv, v0), ph.dy, ph.rdy, ph.fdy);
-    d += divc(cdiff4(V + Gf[e] * sv, cv, v0 * v1), ph.dz, ph.rdz, ph.fdz);
-    dfv[e] = d;
-  }
-}
-
-// Viscous divergence of the interior cell at work index t (band order) into
-// B.dfv (src/Block3d.cpp:879-916).  t < n0 n1 n2.
-template <typename VT>
-__device__ __forceinline__ void visc_div_body(const BlockDesc& B, int t, const Phys& ph, int jb) {
-  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
-  int i, j, k;
-  band_decode(t, n0, n1, n2, jb, i, j, k);
-  double dfv[4];
-  visc_div<VT>(B, i, j, k, ph, dfv);
-  const int ct = i + n0 * (j + n1 * k);
-#pragma unroll
-  for (int e = 0; e < 4; e++) gmem(B.dfv)[e * B.sn + ct] = dfv[e];
-}
+// ---- stage tail: RK3 update + primitives ---------------------------------
 
 // RK3 stage combination (src/Block3d.cpp:949-955) of cell c from the
 // right-hand side df, Q^n = Qn, the stage input Qi, into Qo; then the cell's
@@ -587,9 +661,10 @@ __device__ __forceinline__ void rk3_prim(const double (&df)[NEQ], gptr<const dou
   P[5 * sg + c] = (PT)(ph.gM2 * pp / rr);
 }
 
-// src/Block3d.cpp:879-916, 920-962 and 359-385 for one interior cell; stage
-// 1..3 only, in place on Q / Q_p as the reference.  FT is the face-array type
-// (double, or float for FP32-flux), which is also the viscous fields' type.
+// src/Block3d.cpp:920-962 and 359-385 for one interior cell, with the
+// viscous divergence of the viscous pass (B.dfv); stage 1..3 only, in place
+// on Q / Q_p as the reference.  FT is the face-array type (double, or float
+// for FP32-flux), which is also the primitives' type.
 template <typename FT>
 __device__ __forceinline__ void update_body(const BlockDesc& B, int t, double dt, int stage, const Phys& ph,
                                             int jb) {
@@ -599,7 +674,8 @@ __device__ __forceinline__ void update_body(const BlockDesc& B, int t, double dt
   const int ct = i + n0 * (j + n1 * k);
   const int c = gidx(B, i, j, k);
   double dfv[4];
-  visc_div<FT>(B, i, j, k, ph, dfv);
+#pragma unroll
+  for (int e = 0; e < 4; e++) dfv[e] = gmem<const double>(B.dfv)[e * B.sn + ct];
   const int iE = i + (n0 + 1) * (j + n1 * k);
   const int iF = i + n0 * (j + (n1 + 1) * k);
   const int iG = ct;
@@ -640,8 +716,7 @@ __device__ __forceinline__ double dt_point(double rr, double uu, double vv, doub
 // ---- stage tail over x-pairs of cells ---------------------------------------
 // The same arithmetic as update_body for the two cells (i, j, k), (i+1, j, k)
 // with i even, so that every field of the pair is one 2-wide load or store
-// (for FP32-flux, the float face and viscous arrays move 8 bytes per lane
-// instead of 4).  Requires an even n0: every field stride and row offset is
+// (for FP32-flux, the float face arrays move 8 bytes per lane instead of 4).  Requires an even n0: every field stride and row offset is
 // then even, so the pair loads are naturally aligned.
 template <typename T>
 using vec2 = T __attribute__((ext_vector_type(2)));
@@ -664,42 +739,18 @@ __device__ __forceinline__ void update_pair_body(const BlockDesc& B, int t, doub
   int ih, j, k;
   band_decode(t, n0 / 2, n1, n2, jb, ih, j, k);
   const int i = 2 * ih;
-  // viscous divergence (visc_div) of both cells
-  const int v0 = n0 + 2 * NGV, v1 = n1 + 2 * NGV;
-  const int cv = (i + NGV) + v0 * ((j + NGV) + v1 * (k + NGV));
-  gptr<const FT> __restrict__ V = gmem<const FT>((const FT*)B.visc);
-  const long long sv = B.sv;
-  const double c0 = 1.0 / 12.0, c1 = -2.0 / 3.0;
-  const int Ef[4] = {0, 3, 4, 6}, Ff[4] = {3, 1, 5, 7}, Gf[4] = {4, 5, 2, 8};
+  // viscous divergence of both cells (the viscous pass's B.dfv)
+  const int iG = i + n0 * (j + n1 * k);
   double dfv[2][4];
 #pragma unroll
   for (int e = 0; e < 4; e++) {
-    gptr<const FT> X = V + Ef[e] * sv;
-    const vec2<FT> xa = ld2(X, cv - 2), xb = ld2(X, cv), xc = ld2(X, cv + 2);  // cv-2 .. cv+3
-    double dx0 = c0 * xa.x + c1 * xa.y - c1 * xb.y - c0 * xc.x;
-    double dx1 = c0 * xa.y + c1 * xb.x - c1 * xc.x - c0 * xc.y;
-    gptr<const FT> Y = V + Ff[e] * sv;
-    const vec2<FT> ym2 = ld2(Y, cv - 2 * v0), ym1 = ld2(Y, cv - v0), yp1 = ld2(Y, cv + v0), yp2 = ld2(Y, cv + 2 * v0);
-    const double dy0 = c0 * ym2.x + c1 * ym1.x - c1 * yp1.x - c0 * yp2.x;
-    const double dy1 = c0 * ym2.y + c1 * ym1.y - c1 * yp1.y - c0 * yp2.y;
-    gptr<const FT> Z = V + Gf[e] * sv;
-    const int sz = v0 * v1;
-    const vec2<FT> zm2 = ld2(Z, cv - 2 * sz), zm1 = ld2(Z, cv - sz), zp1 = ld2(Z, cv + sz), zp2 = ld2(Z, cv + 2 * sz);
-    const double dz0 = c0 * zm2.x + c1 * zm1.x - c1 * zp1.x - c0 * zp2.x;
-    const double dz1 = c0 * zm2.y + c1 * zm1.y - c1 * zp1.y - c0 * zp2.y;
-    double d = divc(dx0, ph.dx, ph.rdx, ph.fdx);
-    d += divc(dy0, ph.dy, ph.rdy, ph.fdy);
-    d += divc(dz0, ph.dz, ph.rdz, ph.fdz);
-    dfv[0][e] = d;
-    d = divc(dx1, ph.dx, ph.rdx, ph.fdx);
-    d += divc(dy1, ph.dy, ph.rdy, ph.fdy);
-    d += divc(dz1, ph.dz, ph.rdz, ph.fdz);
-    dfv[1][e] = d;
+    const vec2<double> v = ld2(gmem<const double>(B.dfv), e * B.sn + iG);
+    dfv[0][e] = v.x;
+    dfv[1][e] = v.y;
   }
   // inviscid flux differences, RK3 and primitives
   const int iE = i + (n0 + 1) * (j + n1 * k);
   const int iF = i + n0 * (j + (n1 + 1) * k);
-  const int iG = i + n0 * (j + n1 * k);
   const long long sg = B.sg;
   const int c = gidx(B, i, j, k);
   gptr<const double> Qn = gmem<const double>(B.Q[0]);

@@ -254,7 +254,6 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
           B.o[a] = bc[a] * base;
         }
         B.sg = (long long)B.g[0] * B.g[1] * B.g[2];
-        B.sv = (long long)(B.n[0] + 2 * NGV) * (B.n[1] + 2 * NGV) * (B.n[2] + 2 * NGV);
         B.sn = (long long)B.n[0] * B.n[1] * B.n[2];
         for (int d = 0; d < 3; d++)
           B.sf[d] = (long long)(B.n[0] + (d == 0)) * (B.n[1] + (d == 1)) * (B.n[2] + (d == 2));
@@ -270,14 +269,12 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
         if (st) break;
         if ((st = dev_alloc(c, &q, sizeof(double) * 4 * B.sn))) break;
         B.dfv = (double*)q;
-        // primitives and viscous fields: float in the FP32-flux form, double otherwise
+        // primitives: float in the FP32-flux form, double otherwise
         const size_t vbytes = precision == CBNS_FP32_FLUX ? sizeof(float) : sizeof(double);
         if ((st = dev_alloc(c, &q, vbytes * 6 * B.sg))) break;
         B.prim = (double*)q;
         if ((st = dev_alloc(c, &q, sizeof(double) * B.sn))) break;
         B.mu = (double*)q;
-        if ((st = dev_alloc(c, &q, vbytes * 9 * B.sv))) break;
-        B.visc = (double*)q;
       }
   void* q = nullptr;
   if (!st) st = dev_alloc(c, &q, sizeof(BlockDesc) * nb);

@@ -11,7 +11,6 @@ namespace cbns {
 
 constexpr int NEQ = 5;
 constexpr int NG = 4;   // ghost layers
-constexpr int NGV = 2;  // viscous halo
 constexpr int MAXB = 512;
 
 // Physical constants of one solve (src/read_input.cpp:78-90 and the static
@@ -34,13 +33,11 @@ struct BlockDesc {
   double* dfv;    // viscous divergence [4][n2][n1][n0]
   double* prim;   // rho, u, v, w, p, T: [6][g2][g1][g0] (float storage in the FP32-flux form)
   double* mu;     // Sutherland viscosity on the interior [n2][n1][n0]
-  double* visc;   // txx tyy tzz txy txz tyz ex ey ez on the (n+4)^3 box
   int n[3];       // interior cells
   int g[3];       // ghosted extent n + 8
   int o[3];       // global origin
   int inst;       // owning solve (batched form); 0 for a single solve
   long long sg;   // ghosted field stride
-  long long sv;   // viscous-box field stride
   long long sf[3];// face-array field strides
   long long sn;   // interior field stride
 };
@@ -149,9 +146,9 @@ hipError_t launch_scatter_aos(const BlockDesc* db, const BlockDesc* hb, const Do
 hipError_t launch_gather_aos(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
                              double* aos, hipStream_t s);
 hipError_t launch_fill_mu(const BlockDesc* db, const BlockDesc* hb, int nb, double mu0, hipStream_t s);
-// Viscous tile march: VX x VY columns per workgroup (VX * VY threads), VZ
-// planes per z-chunk.
-constexpr int VX = 64, VY = 4, VZ = 16;
+// Viscous tile march: VX x VY interior columns per workgroup (VX * VY
+// threads), VZ planes per z-chunk.
+constexpr int VX = 16, VY = 16, VZ = 32;
 constexpr int GSUM_WG = 512;  // workgroups of the fixed-order global sums
 hipError_t launch_global_sums(const BlockDesc* db, const Domain& dom, const Phys& ph, int what, double* d_part,
                               double* d_out, int precision, hipStream_t s);

@@ -144,12 +144,14 @@ hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Dom
 }
 
 // ---------------------------------------------------------------------------
-// Viscous terms.  k_visc_flux: calc_gradient (src/calc_gradient.cpp:17-66),
-// calc_viscous_terms (src/Block3d.cpp:750-810) and calc_viscous_flux
-// (src/Block3d.cpp:812-868) fused per point of the extended box [-2, n+2)^3,
-// keeping the nine distinct flux components.  mu is kept on the interior for
-// calc_dt (the reference stores it everywhere; only the interior is read).
-// Grid: (column tiles, z-chunks, blocks); see visc_tile_body.
+// Viscous pass.  k_visc_flux: calc_gradient (src/calc_gradient.cpp:17-66),
+// calc_viscous_terms (src/Block3d.cpp:750-810), calc_viscous_flux
+// (src/Block3d.cpp:812-868) and the divergence of
+// calc_viscous_flux_contribution (src/Block3d.cpp:879-916) in one tile
+// march, writing the divergence (B.dfv) and mu, which is kept on the
+// interior for calc_dt (the reference stores it everywhere; only the
+// interior is read).  Grid: (column tiles, z-chunks, blocks); see
+// visc_tile_body.
 template <typename VT, bool FROMQ = false>
 __global__ __launch_bounds__(VX * VY) void k_visc_flux(const BlockDesc* __restrict__ blocks, Phys ph, int which) {
   const Tile tl = xcd_tile();
@@ -159,22 +161,11 @@ __global__ __launch_bounds__(VX * VY) void k_visc_flux(const BlockDesc* __restri
 static dim3 visc_grid(const BlockDesc* hb, int nb) {
   int tiles = 0, chunks = 0;
   for (int b = 0; b < nb; b++) {
-    const int v0 = hb[b].n[0] + 2 * NGV, v1 = hb[b].n[1] + 2 * NGV, v2 = hb[b].n[2] + 2 * NGV;
-    tiles = std::max(tiles, ((v0 + VX - 1) / VX) * ((v1 + VY - 1) / VY));
-    chunks = std::max(chunks, (v2 + VZ - 1) / VZ);
+    const int n0 = hb[b].n[0], n1 = hb[b].n[1], n2 = hb[b].n[2];
+    tiles = std::max(tiles, ((n0 + VX - 1) / VX) * ((n1 + VY - 1) / VY));
+    chunks = std::max(chunks, (n2 + VZ - 1) / VZ);
   }
   return dim3(tiles, chunks, nb);
-}
-
-// Divergence part of calc_viscous_flux_contribution (src/Block3d.cpp:879-916).
-
-template <typename VT>
-__global__ __launch_bounds__(TPB) void k_visc_div(const BlockDesc* __restrict__ blocks, Phys ph, int jb) {
-  const Tile tl = xcd_tile();
-  const BlockDesc& B = blocks[tl.y];
-  const int t = tl.x * TPB + threadIdx.x;
-  if (t >= B.n[0] * B.n[1] * B.n[2]) return;
-  visc_div_body<VT>(B, t, ph, jb);
 }
 
 // The same march with a fixed number of workgroups, each taking units
@@ -218,15 +209,7 @@ hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, co
 
 hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision, int jb,
                           hipStream_t s) {
-  long long mn = 0;
-  for (int b = 0; b < nb; b++) mn = hb[b].sn > mn ? hb[b].sn : mn;
-  hipError_t e = launch_visc_flux(db, hb, nb, ph, precision, jb, s);
-  if (e != hipSuccess) return e;
-  if (precision == PREC_FP32_FLUX)
-    hipLaunchKernelGGL(k_visc_div<float>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, ph, jb);
-  else
-    hipLaunchKernelGGL(k_visc_div<double>, dim3(nblocks_for(mn), nb), dim3(TPB), 0, s, db, ph, jb);
-  return hipGetLastError();
+  return launch_visc_flux(db, hb, nb, ph, precision, jb, s);
 }
 
 // ---------------------------------------------------------------------------
