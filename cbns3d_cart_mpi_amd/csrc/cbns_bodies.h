@@ -322,9 +322,9 @@ __device__ __forceinline__ void face_eval_body(const Phys& ph, int t, const doub
 // the divergence of calc_viscous_flux_contribution (src/Block3d.cpp:879-916)
 // in one tile march.  Only the divergence (4 fields on the interior, B.dfv)
 // and mu (on the interior, for calc_dt) are written to HBM: the nine flux
-// components txx tyy tzz txy txz tyz ex ey ez live in LDS and registers,
-// rounded to VT as stored fields would be (double for the FP64 forms, float
-// for FP32-flux).
+// components txx tyy tzz txy txz tyz ex ey ez live in LDS and registers.
+// The fluxes and the divergence are rounded to VT as stored fields would be
+// (double for the FP64 forms, float for FP32-flux).
 // Sutherland viscosity, stress tensor, heat flux and energy fluxes of one
 // point from its gradients gr[u, v, w, T][x, y, z], temperature and velocity
 // (src/Block3d.cpp:777-804, 843-862).  out = txx tyy tzz txy txz tyz ex ey ez;
@@ -499,7 +499,7 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
 #pragma unroll
     for (int q = 0; q < NP; q++) pf[f][q] = P[foff[f] + fidx[q] + (Z0 - 2) * sz];
   }
-  gptr<double> __restrict__ DV = gmem(B.dfv);
+  gptr<VT> __restrict__ DV = gmem((VT*)B.dfv);
   const long long sn = B.sn;
   // Ev = (txx, txy, txz, ex), Fv = (txy, tyy, tyz, ey), Gv = (txz, tyz, tzz, ez)
   const int Ef[4] = {0, 3, 4, 6}, Ff[4] = {3, 1, 5, 7}, Gf[4] = {4, 5, 2, 8};
@@ -608,7 +608,7 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
 #pragma unroll
       for (int e = 0; e < 4; e++) {
         const double dz = c0 * gq[e][0] + c1 * gq[e][1] - c1 * gq[e][3] - c0 * gq[e][4];
-        DV[e * sn + o] = dq[e] + divc(dz, ph.dz, ph.rdz, ph.fdz);
+        DV[e * sn + o] = (VT)(dq[e] + divc(dz, ph.dz, ph.rdz, ph.fdz));
       }
     }
 #pragma unroll
@@ -664,7 +664,8 @@ __device__ __forceinline__ void rk3_prim(const double (&df)[NEQ], gptr<const dou
 // src/Block3d.cpp:920-962 and 359-385 for one interior cell, with the
 // viscous divergence of the viscous pass (B.dfv); stage 1..3 only, in place
 // on Q / Q_p as the reference.  FT is the face-array type (double, or float
-// for FP32-flux), which is also the primitives' type.
+// for FP32-flux), which is also the type of the primitives and of the
+// divergence.
 template <typename FT>
 __device__ __forceinline__ void update_body(const BlockDesc& B, int t, double dt, int stage, const Phys& ph,
                                             int jb) {
@@ -675,7 +676,7 @@ __device__ __forceinline__ void update_body(const BlockDesc& B, int t, double dt
   const int c = gidx(B, i, j, k);
   double dfv[4];
 #pragma unroll
-  for (int e = 0; e < 4; e++) dfv[e] = gmem<const double>(B.dfv)[e * B.sn + ct];
+  for (int e = 0; e < 4; e++) dfv[e] = (double)gmem<const FT>((const FT*)B.dfv)[e * B.sn + ct];
   const int iE = i + (n0 + 1) * (j + n1 * k);
   const int iF = i + n0 * (j + (n1 + 1) * k);
   const int iG = ct;
@@ -744,9 +745,9 @@ __device__ __forceinline__ void update_pair_body(const BlockDesc& B, int t, doub
   double dfv[2][4];
 #pragma unroll
   for (int e = 0; e < 4; e++) {
-    const vec2<double> v = ld2(gmem<const double>(B.dfv), e * B.sn + iG);
-    dfv[0][e] = v.x;
-    dfv[1][e] = v.y;
+    const vec2<FT> v = ld2(gmem<const FT>((const FT*)B.dfv), e * B.sn + iG);
+    dfv[0][e] = (double)v.x;
+    dfv[1][e] = (double)v.y;
   }
   // inviscid flux differences, RK3 and primitives
   const int iE = i + (n0 + 1) * (j + n1 * k);

@@ -30,7 +30,7 @@ struct Phys {
 struct BlockDesc {
   double* Q[2];   // Q, Q_p: [5][g2][g1][g0], ghosted
   double* F[3];   // face fluxes Ep, Fp, Gp: [5][n2+dz][n1+dy][n0+dx]
-  double* dfv;    // viscous divergence [4][n2][n1][n0]
+  double* dfv;    // viscous divergence [4][n2][n1][n0] (float storage in the FP32-flux form)
   double* prim;   // rho, u, v, w, p, T: [6][g2][g1][g0] (float storage in the FP32-flux form)
   double* mu;     // Sutherland viscosity on the interior [n2][n1][n0]
   int n[3];       // interior cells

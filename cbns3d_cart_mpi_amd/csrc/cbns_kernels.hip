@@ -240,7 +240,7 @@ __global__ __launch_bounds__(TPB) void k_update(const BlockDesc* __restrict__ bl
     double df = divc((double)Ep[iE + 1] - (double)Ep[iE], ph.dx, ph.rdx, ph.fdx);
     df += divc((double)Fp[iF + n0] - (double)Fp[iF], ph.dy, ph.rdy, ph.fdy);
     df += divc((double)Gp[iG + n0 * n1] - (double)Gp[iG], ph.dz, ph.rdz, ph.fdz);
-    if (e > 0) df -= gmem<const double>(B.dfv)[(e - 1) * B.sn + ct];
+    if (e > 0) df -= (double)gmem<const FT>((const FT*)B.dfv)[(e - 1) * B.sn + ct];
     const long long q = e * sg + c;
     if (1 == stage) {
       Qp[q] = Q[q] - dt * df;
