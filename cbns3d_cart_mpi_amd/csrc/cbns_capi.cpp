@@ -56,6 +56,7 @@ struct cbns_ctx {
   bool dt_inflight = false;
   RefState ref{1.0, 0.0, 0.0};  // FP32-flux reference state
   FixList fix{};                  // faces of the tolerance-bound form left to the fixup pass
+  FixList cfix{};                 // cells its fused z pass left to the cell fixup pass
   // the tolerance-bound form's fused steps store no primitives (its kernels
   // form them from the state): set after such a step, cleared by
   // refresh_prims before any call that reads the stored primitives
@@ -262,6 +263,10 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
         B.Q[0] = (double*)q;
         if ((st = dev_alloc(c, &q, sizeof(double) * NEQ * B.sg))) break;
         B.Q[1] = (double*)q;
+        if (precision == CBNS_FP64_FAST) {  // the stage-1 state of its fused steps
+          if ((st = dev_alloc(c, &q, sizeof(double) * NEQ * B.sg))) break;
+          B.Q[2] = (double*)q;
+        }
         for (int d = 0; d < 3 && !st; d++) {
           if ((st = dev_alloc(c, &q, sizeof(double) * NEQ * B.sf[d]))) break;
           B.F[d] = (double*)q;
@@ -289,6 +294,9 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
   if (!st) st = dev_alloc(c, &q, FIX_BYTES);
   if (!st && hipMemset(q, 0, 256) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st) c->fix = fixlist_at(q);
+  if (!st) st = dev_alloc(c, &q, FIX_BYTES);
+  if (!st && hipMemset(q, 0, 256) != hipSuccess) st = CBNS_ERR_HIP;
+  if (!st) c->cfix = fixlist_at(q);
   if (!st && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) st = CBNS_ERR_HIP;
   if (!st && hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) st = CBNS_ERR_HIP;
@@ -582,8 +590,24 @@ int cbns_set_time(cbns_ctx* c, double t) {
 // on), the fused divergence + RK3 update + primitive recovery, and the ghost
 // gather of the state with its primitives.  launch_sweeps issues the first
 // two (which read only the stage input), finish_stage the rest.
+// Input and output state of a fused stage.  The tolerance-bound form keeps
+// its stage input unwritten (its z pass updates while it sweeps): stage 1
+// goes from Q to a third state S, stage 2 from S to Q_p, stage 3 from Q_p to
+// Q, so Q_p holds the reference's stage-2 state after each step.  The other
+// forms update Q_p and Q in place as the reference does.
+static void stage_io(const cbns_ctx* c, int stage, int* in, int* out) {
+  if (c->precision == CBNS_FP64_FAST) {
+    *in = stage == 1 ? CBNS_Q : (stage == 2 ? 2 : CBNS_QP);
+    *out = stage == 1 ? 2 : (stage == 2 ? CBNS_QP : CBNS_Q);
+  } else {
+    *in = stage == 1 ? CBNS_Q : CBNS_QP;
+    *out = stage == 3 ? CBNS_Q : CBNS_QP;
+  }
+}
+
 static int launch_sweeps(cbns_ctx* c, int stage) {
-  const int in = stage == 1 ? CBNS_Q : CBNS_QP;
+  int in, out;
+  stage_io(c, stage, &in, &out);
   const BlockDesc* hb = c->hb.data();
   if (c->overlap) {
     CK(hipEventRecord(c->ev_fork, c->stream));
@@ -594,23 +618,25 @@ static int launch_sweeps(cbns_ctx* c, int stage) {
     }));
     CK(hipEventRecord(c->ev_join, c->side));
   }
-  for (int d = 0; d < 3; d++)
+  // the tolerance-bound form's z faces are formed by its fused stage tail
+  const int nd = c->precision == CBNS_FP64_FAST ? 2 : 3;
+  for (int d = 0; d < nd; d++)
     CK(timed(c, d, [&] {
       return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->precision, c->ref, c->jband, c->face_occ,
                               c->fix, c->stream);
     }));
-  // faces of the tolerance-bound form whose range checks failed (one pass for all three sweeps)
+  // faces of the tolerance-bound form whose range checks failed (one pass for all its sweeps)
   if (c->precision == CBNS_FP64_FAST) CK(launch_face_fixup(c->db, hb, c->dom.nb, in, c->ph, c->fix, c->stream));
   return CBNS_OK;
 }
 
 static int finish_stage(cbns_ctx* c, int stage, double dt) {
-  const int in = stage == 1 ? CBNS_Q : CBNS_QP;
-  const int out = stage == 3 ? CBNS_Q : CBNS_QP;
+  int in, out;
+  stage_io(c, stage, &in, &out);
   const BlockDesc* hb = c->hb.data();
   // the tolerance-bound form stores no primitives in its fused steps: its
-  // viscous march forms them from the stage input, its stage tail skips
-  // them and its ghost fill copies the state alone
+  // viscous march forms them from the stage input, its stage tail (fused
+  // with its z faces) skips them and its ghost fill copies the state alone
   const bool lean = c->precision == CBNS_FP64_FAST;
   if (c->overlap) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   else
@@ -622,10 +648,20 @@ static int finish_stage(cbns_ctx* c, int stage, double dt) {
     CK(hipMemsetAsync(c->d_smax, 0, sizeof(unsigned long long), c->stream));
     CK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
   }
-  CK(timed(c, 4, [&] {
-    return launch_update_fused(c->db, hb, c->dom.nb, dt, stage, c->ph, c->precision, c->jband, c->stream,
-                               stage == 3 ? c->d_smax : nullptr, stage == 3 ? c->d_flag : nullptr, &s_done, !lean);
-  }));
+  if (lean) {
+    unsigned long long* sm = stage == 3 ? c->d_smax : nullptr;
+    int* fl = stage == 3 ? c->d_flag : nullptr;
+    CK(timed(c, 4, [&] {
+      return launch_zface_update_fast(c->db, hb, c->dom.nb, in, out, stage, dt, c->ph, sm, fl, c->cfix, c->stream);
+    }));
+    CK(launch_zcell_fixup(c->db, hb, c->dom.nb, in, out, stage, dt, c->ph, sm, fl, c->cfix, c->stream));
+    s_done = stage == 3;
+  } else {
+    CK(timed(c, 4, [&] {
+      return launch_update_fused(c->db, hb, c->dom.nb, dt, stage, c->ph, c->precision, c->jband, c->stream,
+                                 stage == 3 ? c->d_smax : nullptr, stage == 3 ? c->d_flag : nullptr, &s_done);
+    }));
+  }
   if (stage == 3) {
     c->s_ready = s_done;
     c->dt_inflight = false;

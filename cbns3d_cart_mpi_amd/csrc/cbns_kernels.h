@@ -28,7 +28,7 @@ struct Phys {
 
 // One on-device block: the reference's Block3d arrays, SoA.
 struct BlockDesc {
-  double* Q[2];   // Q, Q_p: [5][g2][g1][g0], ghosted
+  double* Q[3];   // Q, Q_p and (tolerance-bound single solve) the stage-1 state: [5][g2][g1][g0], ghosted
   double* F[3];   // face fluxes Ep, Fp, Gp: [5][n2+dz][n1+dy][n0+dx]
   double* dfv;    // viscous divergence [4][n2][n1][n0] (float storage in the FP32-flux form)
   double* prim;   // rho, u, v, w, p, T: [6][g2][g1][g0] (float storage in the FP32-flux form)
@@ -102,6 +102,19 @@ hipError_t launch_face_flux_fast(const BlockDesc* db, const BlockDesc* hb, int n
 hipError_t launch_face_fixup(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
                              const FixList& fx, hipStream_t s);
 FixList fixlist_at(void* buf);
+// Tolerance-bound single solve: the z-face sweep fused with the stage tail.
+// Stage `stage` reads its input state Q[in] and writes Q[out]; Q[0] is Q^n
+// (stages 2 and 3 read it pointwise).  Cells beside a z-face whose range
+// checks fail are not updated but appended to cfx (block, cell) for
+// launch_zcell_fixup, which recomputes their faces with the reference-order
+// form and updates them.  With smax (stage 3), the calc_dt spectral radius of
+// every updated cell is reduced into *smax (*flag set for a non-finite one).
+hipError_t launch_zface_update_fast(const BlockDesc* db, const BlockDesc* hb, int nb, int in, int out, int stage,
+                                    double dt, const Phys& ph, unsigned long long* smax, int* flag,
+                                    const FixList& cfx, hipStream_t s);
+hipError_t launch_zcell_fixup(const BlockDesc* db, const BlockDesc* hb, int nb, int in, int out, int stage,
+                              double dt, const Phys& ph, unsigned long long* smax, int* flag, const FixList& cfx,
+                              hipStream_t s);
 hipError_t launch_face_eval_fast(int dir, const Phys& ph, int nfaces, const double* cells, double* flux,
                                  hipStream_t s);
 hipError_t launch_face_flux_fma(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
@@ -125,12 +138,9 @@ hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, co
 // smax/flag (optional): also reduce the next step's calc_dt spectral radius of
 // the updated cells (only over x-pairs, for blocks of even n0; *s_done says
 // whether it was done).
-// prims = false: the primitives of the updated cells are not stored (FP64
-// forms over x-pairs only; odd widths store them)
 hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
                                const Phys& ph, int precision, int jb, hipStream_t s,
-                               unsigned long long* smax = nullptr, int* flag = nullptr, bool* s_done = nullptr,
-                               bool prims = true);
+                               unsigned long long* smax = nullptr, int* flag = nullptr, bool* s_done = nullptr);
 hipError_t launch_ghost_fill_prim(const BlockDesc* db, const BlockDesc* hb, const Domain& dom, int which,
                                   int precision, hipStream_t s);
 hipError_t launch_update(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,

@@ -286,7 +286,7 @@ __global__ __launch_bounds__(TPB) void k_update_fused(const BlockDesc* __restric
 // With smax != nullptr it also reduces the next step's calc_dt spectral
 // radius of the updated cells into *smax (as k_dt does: integer atomic max on
 // the bits, one per workgroup) and raises *flag for a non-finite one.
-template <typename FT, bool PRIMS = true>
+template <typename FT>
 __global__ __launch_bounds__(TPB) void k_update_pair(const BlockDesc* __restrict__ blocks, double dt, int stage,
                                                      Phys ph, int jb, unsigned long long* smax, int* flag) {
   const Tile tl = xcd_tile();
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(TPB) void k_update_pair(const BlockDesc* __restrict
   const bool want = smax != nullptr;  // uniform
   double s = 0.0;
   int bad = 0;
-  if (t < (B.n[0] / 2) * B.n[1] * B.n[2]) update_pair_body<FT, PRIMS>(B, t, dt, stage, ph, jb, want, &s, &bad);
+  if (t < (B.n[0] / 2) * B.n[1] * B.n[2]) update_pair_body<FT>(B, t, dt, stage, ph, jb, want, &s, &bad);
   if (!want) return;
   s = wave_max(s);
   __shared__ double red[TPB / 64];
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(TPB) void k_update_pair(const BlockDesc* __restrict
 
 hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb, double dt, int stage,
                                const Phys& ph, int precision, int jb, hipStream_t s, unsigned long long* smax,
-                               int* flag, bool* s_done, bool prims) {
+                               int* flag, bool* s_done) {
   if (s_done) *s_done = false;
   long long mn = 0;
   bool even = true;
@@ -323,9 +323,7 @@ hipError_t launch_update_fused(const BlockDesc* db, const BlockDesc* hb, int nb,
   if (stage < 1 || stage > 3) return hipSuccess;
   if (even) {
     const dim3 grid(nblocks_for(mn / 2), nb);
-    if (precision != PREC_FP32_FLUX && !prims)
-      hipLaunchKernelGGL((k_update_pair<double, false>), grid, dim3(TPB), 0, s, db, dt, stage, ph, jb, smax, flag);
-    else if (precision != PREC_FP32_FLUX)
+    if (precision != PREC_FP32_FLUX)
       hipLaunchKernelGGL(k_update_pair<double>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb, smax, flag);
     else
       hipLaunchKernelGGL(k_update_pair<float>, grid, dim3(TPB), 0, s, db, dt, stage, ph, jb, smax, flag);
