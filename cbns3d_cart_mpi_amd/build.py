@@ -21,8 +21,8 @@ HIP_FLAGS = [
     "-fPIC", "-Wall", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-SOURCES = ["cbns_kernels.hip", "cbns_kernels_fma.hip", "cbns_kernels_fast.hip", "cbns_capi.cpp", "cbns_batch.hip",
-           "cbns_batch_capi.cpp"]
+SOURCES = ["cbns_kernels.hip", "cbns_kernels_fma.hip", "cbns_kernels_fast.hip", "cbns_kernels_zfuse.hip",
+           "cbns_capi.cpp", "cbns_batch.hip", "cbns_batch_capi.cpp"]
 # the contracted FP64 form's face kernels are the only code built with FMA
 # contraction (the tolerance-bound form writes its fused multiply-adds out)
 CONTRACT = {"cbns_kernels_fma.hip": ["-ffp-contract=fast"]}

@@ -263,7 +263,7 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
         B.Q[0] = (double*)q;
         if ((st = dev_alloc(c, &q, sizeof(double) * NEQ * B.sg))) break;
         B.Q[1] = (double*)q;
-        if (precision == CBNS_FP64_FAST) {  // the stage-1 state of its fused steps
+        if (precision != CBNS_FP64_FMA) {  // the stage-1 state of the fused z pass's steps
           if ((st = dev_alloc(c, &q, sizeof(double) * NEQ * B.sg))) break;
           B.Q[2] = (double*)q;
         }
@@ -590,13 +590,17 @@ int cbns_set_time(cbns_ctx* c, double t) {
 // on), the fused divergence + RK3 update + primitive recovery, and the ghost
 // gather of the state with its primitives.  launch_sweeps issues the first
 // two (which read only the stage input), finish_stage the rest.
-// Input and output state of a fused stage.  The tolerance-bound form keeps
-// its stage input unwritten (its z pass updates while it sweeps): stage 1
-// goes from Q to a third state S, stage 2 from S to Q_p, stage 3 from Q_p to
-// Q, so Q_p holds the reference's stage-2 state after each step.  The other
-// forms update Q_p and Q in place as the reference does.
+// Whether a form's steps run the z-face sweep fused with the stage tail
+// (every form but the contracted FP64 one).
+static bool zfused(const cbns_ctx* c) { return c->precision != CBNS_FP64_FMA; }
+
+// Input and output state of a fused stage.  The fused z pass keeps the stage
+// input unwritten (it updates while it sweeps): stage 1 goes from Q to a
+// third state S, stage 2 from S to Q_p, stage 3 from Q_p to Q, so Q_p holds
+// the reference's stage-2 state after each step.  The contracted form
+// updates Q_p and Q in place as the reference does.
 static void stage_io(const cbns_ctx* c, int stage, int* in, int* out) {
-  if (c->precision == CBNS_FP64_FAST) {
+  if (zfused(c)) {
     *in = stage == 1 ? CBNS_Q : (stage == 2 ? 2 : CBNS_QP);
     *out = stage == 1 ? 2 : (stage == 2 ? CBNS_QP : CBNS_Q);
   } else {
@@ -618,8 +622,8 @@ static int launch_sweeps(cbns_ctx* c, int stage) {
     }));
     CK(hipEventRecord(c->ev_join, c->side));
   }
-  // the tolerance-bound form's z faces are formed by its fused stage tail
-  const int nd = c->precision == CBNS_FP64_FAST ? 2 : 3;
+  // the z faces are formed by the fused stage tail
+  const int nd = zfused(c) ? 2 : 3;
   for (int d = 0; d < nd; d++)
     CK(timed(c, d, [&] {
       return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->precision, c->ref, c->jband, c->face_occ,
@@ -648,13 +652,16 @@ static int finish_stage(cbns_ctx* c, int stage, double dt) {
     CK(hipMemsetAsync(c->d_smax, 0, sizeof(unsigned long long), c->stream));
     CK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
   }
-  if (lean) {
+  if (zfused(c)) {
     unsigned long long* sm = stage == 3 ? c->d_smax : nullptr;
     int* fl = stage == 3 ? c->d_flag : nullptr;
     CK(timed(c, 4, [&] {
-      return launch_zface_update_fast(c->db, hb, c->dom.nb, in, out, stage, dt, c->ph, sm, fl, c->cfix, c->stream);
+      return launch_zface_update(c->db, hb, c->dom.nb, c->precision, in, out, stage, dt, c->ph, c->ref, sm, fl,
+                                 c->cfix, c->stream);
     }));
-    CK(launch_zcell_fixup(c->db, hb, c->dom.nb, in, out, stage, dt, c->ph, sm, fl, c->cfix, c->stream));
+    if (c->precision != CBNS_FP32_FLUX)
+      CK(launch_zcell_fixup(c->db, hb, c->dom.nb, c->precision, in, out, stage, dt, c->ph, sm, fl, c->cfix,
+                            c->stream));
     s_done = stage == 3;
   } else {
     CK(timed(c, 4, [&] {

@@ -351,8 +351,11 @@ struct RefState {
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// FP32 arithmetic contracts a * b + c within each expression only (clang's
+// "on"), so the fused operations are fixed by the source and do not depend
+// on the kernel these functions are inlined into.
 __device__ __forceinline__ float weno5f(float Um2, float Um1, float U0, float Up1, float Up2) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   const float eps = 1.0e-6f;
   float diff1 = Um2 - 2.0f * Um1 + U0;
   float diff2 = Um2 - 4.0f * Um1 + 3.0f * U0;
@@ -397,7 +400,7 @@ template <int D>
 __device__ __forceinline__ void face_flux_f32_dev(const float (&d)[6][5], const float (&pf)[5],
                                                   const float (&qf)[5], double gamma, double gam1,
                                                   const RefState& g, float (&flux)[5]) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   const float gf = (float)gamma, g1 = (float)gam1;
   // ---- Roe average
   float uu, vv, ww, H, c;

@@ -102,19 +102,22 @@ hipError_t launch_face_flux_fast(const BlockDesc* db, const BlockDesc* hb, int n
 hipError_t launch_face_fixup(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
                              const FixList& fx, hipStream_t s);
 FixList fixlist_at(void* buf);
-// Tolerance-bound single solve: the z-face sweep fused with the stage tail.
-// Stage `stage` reads its input state Q[in] and writes Q[out]; Q[0] is Q^n
-// (stages 2 and 3 read it pointwise).  Cells beside a z-face whose range
+// Single solve (cbns_kernels_zfuse.hip): the z-face sweep fused with the
+// stage tail, for PREC_FP64, PREC_FP32_FLUX and PREC_FP64_FAST.  Stage
+// `stage` reads its input state Q[in] and writes Q[out]; Q[0] is Q^n (stages
+// 2 and 3 read it pointwise).  FP64 forms: cells beside a z-face whose range
 // checks fail are not updated but appended to cfx (block, cell) for
 // launch_zcell_fixup, which recomputes their faces with the reference-order
-// form and updates them.  With smax (stage 3), the calc_dt spectral radius of
-// every updated cell is reduced into *smax (*flag set for a non-finite one).
-hipError_t launch_zface_update_fast(const BlockDesc* db, const BlockDesc* hb, int nb, int in, int out, int stage,
-                                    double dt, const Phys& ph, unsigned long long* smax, int* flag,
-                                    const FixList& cfx, hipStream_t s);
-hipError_t launch_zcell_fixup(const BlockDesc* db, const BlockDesc* hb, int nb, int in, int out, int stage,
-                              double dt, const Phys& ph, unsigned long long* smax, int* flag, const FixList& cfx,
-                              hipStream_t s);
+// form and updates them.  With smax (stage 3), the calc_dt
+// spectral radius of every updated cell is reduced into *smax (*flag set for
+// a non-finite one).
+struct RefState;
+hipError_t launch_zface_update(const BlockDesc* db, const BlockDesc* hb, int nb, int precision, int in, int out,
+                               int stage, double dt, const Phys& ph, const RefState& ref, unsigned long long* smax,
+                               int* flag, const FixList& cfx, hipStream_t s);
+hipError_t launch_zcell_fixup(const BlockDesc* db, const BlockDesc* hb, int nb, int precision, int in, int out,
+                              int stage, double dt, const Phys& ph, unsigned long long* smax, int* flag,
+                              const FixList& cfx, hipStream_t s);
 hipError_t launch_face_eval_fast(int dir, const Phys& ph, int nfaces, const double* cells, double* flux,
                                  hipStream_t s);
 hipError_t launch_face_flux_fma(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
