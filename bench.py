@@ -1,19 +1,22 @@
-"""Benchmark of the plain FP64 RHS + TVD-RK3 time-step path on one MI355X.
+"""Benchmark of the FP64 RHS + TVD-RK3 time-step path on one MI355X.
 
 Workload: Taylor-Green vortex, 256^3 periodic cells, FP64, one on-device block
 (the reference's TGV setup: gamma 1.4, Mach 0.1, Re 1600, Pr 0.71,
 T_inf 255.556, CFL 0.5).  A step is one full TVD-RK3 step (dt reduction, three
-RHS evaluations, three ghost fills, three primitive passes) over the whole
-grid, with the state resident in HBM.
+RHS evaluations, three ghost fills) over the whole grid, with the state
+resident in HBM.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--dims 1,1,1]
 
 Prints one JSON line.  value = Mcell-steps/s = N^3 * K / wall / 1e6 over the
-K timed steps.  The dominant kernel (inviscid face flux, one launch per axis
-per stage) is timed live with HIP events on the solver's stream during the
-timed steps; its roofline is FP64 arithmetic (algorithmic 1570 flop per face:
-calc_eigenvectors 161 + 2 x rec_weno5 642 + lf_flux 125, counted literally
-from the reference source).  The CPU baseline is the repository's CPU
+K timed steps.  The dominant kernel (the inviscid face-flux sweep, launched
+for the x and y faces of each stage; the z faces are formed by the stage
+tail, which fuses them with the update) is timed live with HIP events on the
+solver's stream during the timed steps; its roofline is FP64 arithmetic
+(algorithmic 1570 flop per face: calc_eigenvectors 161 + 2 x rec_weno5 642 +
+lf_flux 125, counted literally from the reference source).  The fused z-face
+and stage-tail kernel is reported beside it against both FP64 arithmetic and
+HBM bandwidth.  The CPU baseline is the repository's CPU
 restatement of the reference (oracle/, OpenMP) on the host's cores, timed on a
 bounded sample of steps of the same 256^3 workload.
 """
@@ -294,12 +297,22 @@ def serve_leg(n):
                       "(the full configuration: profiles/serving_full.md)"}
 
 
+def zfused_bytes_per_cell(precision):
+    """Algorithmic HBM bytes per cell of one launch of the fused z-face and
+    stage-tail kernel, averaged over the three stages: the x- and y-face
+    fluxes (5 each), the viscous divergence (4), the stage input (5, read
+    once along the column's march), Q^n (5, stages 2 and 3 only) and the
+    updated state (5 written); the bit-exact form also writes 6 primitives."""
+    b = 8 * (5 + 5 + 4 + 5 + 5) + 8 * 5 * 2 / 3
+    return b + (48 if precision == 0 else 0)
+
+
 def plain_leg(C, N, dims, precision, steps, warmup, fixed=None):
     """One FP64 form on TGV N^3: autotuned launch shapes, `warmup` untimed
-    steps, then `steps` timed steps with the three face sweeps timed live
-    (HIP events on the solver's streams), then a per-kernel breakdown over
-    further steps run serially (the viscous kernel after the sweeps, so each
-    sweep's duration is its own).  Returns (line fragment, solver)."""
+    steps, then `steps` timed steps with the face sweeps timed live (HIP
+    events on the solver's streams), then a per-kernel breakdown over further
+    steps run serially (the viscous kernel after the sweeps, so each kernel's
+    duration is its own).  Returns (line fragment, solver)."""
     from cbns3d_cart_mpi_amd import autotune
 
     s = C.Solver(C.make_params(N, nstep_max=10 ** 6, t_end=20.0), dims, precision=precision)
@@ -337,11 +350,23 @@ def plain_leg(C, N, dims, precision, steps, warmup, fixed=None):
     autotune.apply_schedule(s, sched)
     faces_per_launch = (N + 1) * N * N
     flop = FLOP_PER_FACE * faces_per_launch
+    sweeps = tuple(k for k in sweeps if kt_face[k][1])  # the z faces of the fused forms are in the stage tail
     per_axis = {k: kt_face[k][0] / max(kt_face[k][1], 1) for k in sweeps}
     avg_ms = sum(kt_face[k][0] for k in sweeps) / max(sum(kt_face[k][1] for k in sweeps), 1)
     achieved = flop / (avg_ms * 1e-3) / 1e12
     serial = {k: kt[k][0] / max(kt[k][1], 1) for k in sweeps}
     value = N ** 3 * steps / el / 1e6
+    zf = None
+    if "face_z" not in sweeps and kt["update"][1]:  # the fused z-face and stage-tail kernel
+        zms = kt["update"][0] / kt["update"][1]
+        zbytes = zfused_bytes_per_cell(precision) * N ** 3
+        zf = {"avg_launch_ms_serial": round(zms, 4),
+              "valu": {"achieved_tflops": round(flop / (zms * 1e-3) / 1e12, 3),
+                       "frac": round(flop / (zms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
+                       "algorithmic_flop_per_launch": flop},
+              "hbm": {"achieved_gbs": round(zbytes / (zms * 1e-3) / 1e9, 1),
+                      "frac": round(zbytes / (zms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                      "algorithmic_bytes_per_launch": round(zbytes)}}
     frag = {
         "value": round(value, 2), "ms_per_step": round(el / steps * 1e3, 3),
         "face_roofline": {
@@ -353,6 +378,7 @@ def plain_leg(C, N, dims, precision, steps, warmup, fixed=None):
             "per_axis_frac_serial": {k: round(flop / (v * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4)
                                      for k, v in serial.items()}},
         "kernels_avg_ms_serial": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items() if v[1]},
+        "fused_z_update": zf,
         "autotune": tuned,
         "steps_before_timed": warmup + tuned_steps,
     }
@@ -405,14 +431,16 @@ def main():
     fr = head["face_roofline"]
     roof = {"bound": "valu", "achieved": fr["achieved"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": fr["frac"], "traffic": read_traffic(),
-            "kernel": "k_face_flux_fast (x, y, z sweeps; tolerance-bound FP64 form); FP64 VALU-issue bound; "
+            "kernel": "k_face_flux_fast (x and y sweeps; tolerance-bound FP64 form); FP64 VALU-issue bound; "
                       "achieved = 1570 algorithmic flop per face (literal count of the reference) x faces per "
-                      "launch / average launch duration of all three sweeps in the timed steps",
+                      "launch / average launch duration of the sweeps in the timed steps; the z faces are in "
+                      "the fused stage tail (fused_z_update)",
             "avg_launch_ms": fr["avg_launch_ms"],
             "algorithmic_flop_per_launch": FLOP_PER_FACE * (N + 1) * N * N,
             "per_axis_frac_timed_steps": fr["per_axis_frac_timed_steps"],
             "per_axis_frac_serial": fr["per_axis_frac_serial"],
-            "valu_active_frac": read_valu_active()}
+            "valu_active_frac": read_valu_active(),
+            "fused_z_update": head["fused_z_update"]}
     line = {
         "metric": "Mcell-steps/s TGV 256^3 FP64 vs FP32-flux (L2 err stated); sweep p50/p99 latency",
         "value": value,

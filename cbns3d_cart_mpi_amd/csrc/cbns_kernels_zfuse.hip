@@ -27,7 +27,7 @@
 
 namespace cbns {
 
-constexpr int ZX = 64, ZY = 4, ZM = 16;  // columns per workgroup along x, y; cells per run
+constexpr int ZX = 64, ZY = 4, ZM = 32;  // columns per workgroup along x, y; cells per run
 enum { ZF_FP64 = 0, ZF_FP32 = 1, ZF_FAST = 2 };
 
 template <int FORM>
@@ -174,18 +174,19 @@ __global__ __launch_bounds__(ZX * ZY, 2) void k_zface_update(const BlockDesc* __
     for (int cc = 0; cc < 6; cc++)
 #pragma unroll
       for (int e = 0; e < 5; e++) q[cc][e] = Qin[e * sg + c0 + (k0 - 3 + cc) * sz];
-    double pl[5];  // primitives of cell k - 1 (plain and FP32-flux forms)
+    double pl[5];  // primitives of cell k - 1 (FP32-flux form; the plain form forms them at each face)
     if constexpr (FORM != ZF_FAST) zprim<FORM>(q[2], ph, pl);
     double gp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     bool okp = true;
     for (int k = k0; k <= k1; k++) {
-      // the plain form's heavier face leaves no registers to hold the
-      // operands across it: it loads them after the face
+      // the plain form's heavier face leaves no registers to hold values
+      // across it: it loads the operands and the next cell after the face
+      // and forms the primitives of both of its cells at each face
       constexpr bool early = FORM != ZF_FP64;
       ZOperands<FT> o;
       if (early && k > k0) zload<FT>(B, i, j, k - 1, stage, o);
       double nx[5];
-      if (k < k1) {
+      if (early && k < k1) {
 #pragma unroll
         for (int e = 0; e < 5; e++) nx[e] = Qin[e * sg + c0 + (k + 3) * sz];
       }
@@ -196,6 +197,7 @@ __global__ __launch_bounds__(ZX * ZY, 2) void k_zface_update(const BlockDesc* __
       } else {
         double pr[5];  // primitives of cell k
         zprim<FORM>(q[3], ph, pr);
+        if constexpr (FORM == ZF_FP64) zprim<FORM>(q[2], ph, pl);
         if constexpr (FORM == ZF_FP32) {
           float g32[5];
           face_flux_f32<2>(q, pl, pr, ph.gamma, ph.gam1, ref, g32);
@@ -216,6 +218,10 @@ __global__ __launch_bounds__(ZX * ZY, 2) void k_zface_update(const BlockDesc* __
             cfx.ent[slot] = (unsigned long long)(i + (long long)n0 * (j + (long long)n1 * (k - 1))) |
                             ((unsigned long long)blockIdx.z << 42);
         }
+      }
+      if (!early && k < k1) {
+#pragma unroll
+        for (int e = 0; e < 5; e++) nx[e] = Qin[e * sg + c0 + (k + 3) * sz];
       }
       if (k < k1) {
 #pragma unroll
