@@ -125,7 +125,7 @@ __global__ __launch_bounds__(TPB) void kb_visc(const BlockDesc* __restrict__ blo
     if (!I.stepping) return;
     // one (column tile, z-chunk) unit of the viscous tile march per entry
     const int ntiles = ((B.n[0] + VX - 1) / VX) * ((B.n[1] + VY - 1) / VY);
-    visc_tile_body<VT, FROMQ>(B, I.ph, tr.tile % ntiles, tr.tile / ntiles, which);
+    visc_tile_body<VT, FROMQ>(B, I.ph, tr.tile % ntiles, tr.tile / ntiles, which, VZ);
   });
 }
 

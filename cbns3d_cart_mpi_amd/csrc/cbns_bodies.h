@@ -394,17 +394,17 @@ __device__ __forceinline__ void prim_uvwT(const double (&q)[5], const Phys& ph, 
 // The next plane's loads are issued before the current plane's arithmetic.
 // Every value is formed by the same expressions as the reference's separate
 // passes (grad4, visc_terms, the divergence stencil).  One call = one
-// (column tile, z-chunk) unit, made by all VX * VY threads of a workgroup with
-// the same tile and chunk (it synchronises the workgroup).
+// (column tile, z-chunk) unit of vz planes, made by all VX * VY threads of a
+// workgroup with the same tile and chunk (it synchronises the workgroup).
 template <typename VT, bool FROMQ = false>
-__device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& ph, int tile, int chunk,
-                                               int which = 0) {
+__device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& ph, int tile, int chunk, int which,
+                                               int vz) {
   static_assert(!FROMQ || sizeof(VT) == 8, "the state is FP64");
   const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
   const int ntx = (n0 + VX - 1) / VX, nty = (n1 + VY - 1) / VY;
-  const int Z0 = chunk * VZ;
+  const int Z0 = chunk * vz;
   if (tile >= ntx * nty || Z0 >= n2) return;  // uniform per workgroup
-  const int Z1 = Z0 + VZ < n2 ? Z0 + VZ : n2;
+  const int Z1 = Z0 + vz < n2 ? Z0 + vz : n2;
   const int X0 = (tile % ntx) * VX, Y0 = (tile / ntx) * VY;
   const int wx = n0 - X0 < VX ? n0 - X0 : VX, wy = n1 - Y0 < VY ? n1 - Y0 : VY;  // the tile's extent
   const int tx = threadIdx.x % VX, ty = threadIdx.x / VX;

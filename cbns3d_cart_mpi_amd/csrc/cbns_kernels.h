@@ -160,8 +160,12 @@ hipError_t launch_gather_aos(const BlockDesc* db, const BlockDesc* hb, const Dom
                              double* aos, hipStream_t s);
 hipError_t launch_fill_mu(const BlockDesc* db, const BlockDesc* hb, int nb, double mu0, hipStream_t s);
 // Viscous tile march: VX x VY interior columns per workgroup (VX * VY
-// threads), VZ planes per z-chunk.
+// threads), at most VZ planes per z-chunk (fewer where a grid has too few
+// column tiles to fill the device: visc_chunk_planes).
 constexpr int VX = 16, VY = 16, VZ = 32;
+// Planes per chunk of the single solve's viscous march: VZ, halved (down to
+// 4) until the grid has at least 1024 units.
+int visc_chunk_planes(const BlockDesc* hb, int nb);
 constexpr int GSUM_WG = 512;  // workgroups of the fixed-order global sums
 hipError_t launch_global_sums(const BlockDesc* db, const Domain& dom, const Phys& ph, int what, double* d_part,
                               double* d_out, int precision, hipStream_t s);

@@ -153,17 +153,30 @@ hipError_t launch_ghost_fill(const BlockDesc* db, const BlockDesc* hb, const Dom
 // interior is read).  Grid: (column tiles, z-chunks, blocks); see
 // visc_tile_body.
 template <typename VT, bool FROMQ = false>
-__global__ __launch_bounds__(VX * VY) void k_visc_flux(const BlockDesc* __restrict__ blocks, Phys ph, int which) {
+__global__ __launch_bounds__(VX * VY) void k_visc_flux(const BlockDesc* __restrict__ blocks, Phys ph, int which,
+                                                       int vz) {
   const Tile tl = xcd_tile();
-  visc_tile_body<VT, FROMQ>(blocks[blockIdx.z], ph, tl.x, tl.y, which);
+  visc_tile_body<VT, FROMQ>(blocks[blockIdx.z], ph, tl.x, tl.y, which, vz);
 }
 
-static dim3 visc_grid(const BlockDesc* hb, int nb) {
+int visc_chunk_planes(const BlockDesc* hb, int nb) {
+  long long tiles = 0;
+  int n2 = 0;
+  for (int b = 0; b < nb; b++) {
+    tiles = std::max(tiles, (long long)((hb[b].n[0] + VX - 1) / VX) * ((hb[b].n[1] + VY - 1) / VY));
+    n2 = std::max(n2, hb[b].n[2]);
+  }
+  int vz = VZ;
+  while (vz > 4 && tiles * nb * ((n2 + vz - 1) / vz) < 1024) vz /= 2;
+  return vz;
+}
+
+static dim3 visc_grid(const BlockDesc* hb, int nb, int vz) {
   int tiles = 0, chunks = 0;
   for (int b = 0; b < nb; b++) {
     const int n0 = hb[b].n[0], n1 = hb[b].n[1], n2 = hb[b].n[2];
     tiles = std::max(tiles, ((n0 + VX - 1) / VX) * ((n1 + VY - 1) / VY));
-    chunks = std::max(chunks, (n2 + VZ - 1) / VZ);
+    chunks = std::max(chunks, (n2 + vz - 1) / vz);
   }
   return dim3(tiles, chunks, nb);
 }
@@ -173,11 +186,11 @@ static dim3 visc_grid(const BlockDesc* hb, int nb) {
 // stream it then occupies only that many workgroup slots.
 template <typename VT, bool FROMQ = false>
 __global__ __launch_bounds__(VX * VY) void k_visc_flux_units(const BlockDesc* __restrict__ blocks, Phys ph,
-                                                             int tiles, int chunks, int nb, int which) {
+                                                             int tiles, int chunks, int nb, int which, int vz) {
   const int per_block = tiles * chunks, total = per_block * nb;
   for (int u = blockIdx.x; u < total; u += gridDim.x) {  // uniform per workgroup
     const int b = u / per_block, r = u - b * per_block;
-    visc_tile_body<VT, FROMQ>(blocks[b], ph, r % tiles, r / tiles, which);
+    visc_tile_body<VT, FROMQ>(blocks[b], ph, r % tiles, r / tiles, which, vz);
     __syncthreads();  // the next unit refills the LDS planes this one read
   }
 }
@@ -185,25 +198,28 @@ __global__ __launch_bounds__(VX * VY) void k_visc_flux_units(const BlockDesc* __
 hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision,
                             int jb, hipStream_t s, int wgs, int from_state) {
   (void)jb;  // the tile march fixes its own order
-  const dim3 grid = visc_grid(hb, nb);
+  const int vz = visc_chunk_planes(hb, nb);
+  const dim3 grid = visc_grid(hb, nb, vz);
   const bool fromq = from_state >= 0;  // u, v, w, T formed from Q[from_state]
   const int w = fromq ? from_state : 0;
   if (wgs > 0) {
     if (precision == PREC_FP32_FLUX)
-      hipLaunchKernelGGL(k_visc_flux_units<float>, dim3(wgs), dim3(VX * VY), 0, s, db, ph, (int)grid.x, (int)grid.y, nb, w);
+      hipLaunchKernelGGL(k_visc_flux_units<float>, dim3(wgs), dim3(VX * VY), 0, s, db, ph, (int)grid.x, (int)grid.y,
+                         nb, w, vz);
     else if (fromq)
       hipLaunchKernelGGL((k_visc_flux_units<double, true>), dim3(wgs), dim3(VX * VY), 0, s, db, ph, (int)grid.x,
-                         (int)grid.y, nb, w);
+                         (int)grid.y, nb, w, vz);
     else
-      hipLaunchKernelGGL(k_visc_flux_units<double>, dim3(wgs), dim3(VX * VY), 0, s, db, ph, (int)grid.x, (int)grid.y, nb, w);
+      hipLaunchKernelGGL(k_visc_flux_units<double>, dim3(wgs), dim3(VX * VY), 0, s, db, ph, (int)grid.x, (int)grid.y,
+                         nb, w, vz);
     return hipGetLastError();
   }
   if (precision == PREC_FP32_FLUX)
-    hipLaunchKernelGGL(k_visc_flux<float>, grid, dim3(VX * VY), 0, s, db, ph, w);
+    hipLaunchKernelGGL(k_visc_flux<float>, grid, dim3(VX * VY), 0, s, db, ph, w, vz);
   else if (fromq)
-    hipLaunchKernelGGL((k_visc_flux<double, true>), grid, dim3(VX * VY), 0, s, db, ph, w);
+    hipLaunchKernelGGL((k_visc_flux<double, true>), grid, dim3(VX * VY), 0, s, db, ph, w, vz);
   else
-    hipLaunchKernelGGL(k_visc_flux<double>, grid, dim3(VX * VY), 0, s, db, ph, w);
+    hipLaunchKernelGGL(k_visc_flux<double>, grid, dim3(VX * VY), 0, s, db, ph, w, vz);
   return hipGetLastError();
 }
 

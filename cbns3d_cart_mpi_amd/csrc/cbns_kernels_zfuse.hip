@@ -27,7 +27,10 @@
 
 namespace cbns {
 
-constexpr int ZX = 64, ZY = 4, ZM = 32;  // columns per workgroup along x, y; cells per run
+// columns per workgroup along x, y; at most ZM cells per run (fewer where a
+// grid has too few columns to fill the device: halved, down to 2, until the
+// grid has at least 1024 workgroups)
+constexpr int ZX = 64, ZY = 4, ZM = 32;
 enum { ZF_FP64 = 0, ZF_FP32 = 1, ZF_FAST = 2 };
 
 template <int FORM>
@@ -151,7 +154,8 @@ __device__ __forceinline__ void reduce_smax(double s, int bad, unsigned long lon
 template <int FORM>
 __global__ __launch_bounds__(ZX * ZY, 2) void k_zface_update(const BlockDesc* __restrict__ blocks, int in, int out,
                                                           int stage, double dt, Phys ph, RefState ref,
-                                                          unsigned long long* smax, int* flag, FixList cfx) {
+                                                          unsigned long long* smax, int* flag, FixList cfx,
+                                                          int zm) {
   using FT = zft<FORM>;
   const Tile tl = xcd_tile();
   const BlockDesc& B = blocks[blockIdx.z];
@@ -159,12 +163,12 @@ __global__ __launch_bounds__(ZX * ZY, 2) void k_zface_update(const BlockDesc* __
   const int ntx = (n0 + ZX - 1) / ZX;
   const int i = (tl.x % ntx) * ZX + (int)threadIdx.x % ZX;
   const int j = (tl.x / ntx) * ZY + (int)threadIdx.x / ZX;
-  const int k0 = tl.y * ZM;
+  const int k0 = tl.y * zm;
   const bool want = smax != nullptr;  // uniform
   double s = 0.0;
   int bad = 0;
   if (i < n0 && j < n1 && k0 < n2) {
-    const int k1 = k0 + ZM < n2 ? k0 + ZM : n2;  // cells [k0, k1), faces k0 .. k1
+    const int k1 = k0 + zm < n2 ? k0 + zm : n2;  // cells [k0, k1), faces k0 .. k1
     const int sz = B.g[0] * B.g[1];
     const long long sg = B.sg;
     gptr<const double> __restrict__ Qin = gmem<const double>(B.Q[in]);
@@ -242,18 +246,23 @@ __global__ __launch_bounds__(ZX * ZY, 2) void k_zface_update(const BlockDesc* __
 hipError_t launch_zface_update(const BlockDesc* db, const BlockDesc* hb, int nb, int precision, int in, int out,
                                int stage, double dt, const Phys& ph, const RefState& ref, unsigned long long* smax,
                                int* flag, const FixList& cfx, hipStream_t s) {
-  int tiles = 0, runs = 0;
+  int tiles = 0, n2 = 0;
   for (int b = 0; b < nb; b++) {
     tiles = std::max(tiles, ((hb[b].n[0] + ZX - 1) / ZX) * ((hb[b].n[1] + ZY - 1) / ZY));
-    runs = std::max(runs, (hb[b].n[2] + ZM - 1) / ZM);
+    n2 = std::max(n2, hb[b].n[2]);
   }
-  const dim3 grid(tiles, runs, nb), blk(ZX * ZY);
+  int zm = ZM;
+  while (zm > 2 && (long long)tiles * nb * ((n2 + zm - 1) / zm) < 1024) zm /= 2;
+  const dim3 grid(tiles, (n2 + zm - 1) / zm, nb), blk(ZX * ZY);
   if (precision == PREC_FP64_FAST)
-    hipLaunchKernelGGL(k_zface_update<ZF_FAST>, grid, blk, 0, s, db, in, out, stage, dt, ph, ref, smax, flag, cfx);
+    hipLaunchKernelGGL(k_zface_update<ZF_FAST>, grid, blk, 0, s, db, in, out, stage, dt, ph, ref, smax, flag, cfx,
+                       zm);
   else if (precision == PREC_FP32_FLUX)
-    hipLaunchKernelGGL(k_zface_update<ZF_FP32>, grid, blk, 0, s, db, in, out, stage, dt, ph, ref, smax, flag, cfx);
+    hipLaunchKernelGGL(k_zface_update<ZF_FP32>, grid, blk, 0, s, db, in, out, stage, dt, ph, ref, smax, flag, cfx,
+                       zm);
   else if (precision == PREC_FP64)
-    hipLaunchKernelGGL(k_zface_update<ZF_FP64>, grid, blk, 0, s, db, in, out, stage, dt, ph, ref, smax, flag, cfx);
+    hipLaunchKernelGGL(k_zface_update<ZF_FP64>, grid, blk, 0, s, db, in, out, stage, dt, ph, ref, smax, flag, cfx,
+                       zm);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
