@@ -20,7 +20,7 @@ struct InstDesc {
   int steps_done, steps_target;
   int budget;        // steps it may still take before the host grants more
   int stepping;      // whether the current step advances it (set by finalize)
-  int pairs;         // every block has an even x-width (stage tail over x-pairs)
+  int pairs;         // the stage tail reduces the next step's smax (every solve: the fused z pass)
   int s_fused;       // smax was formed by the last stage-3 update (kb_dt skips it)
   double t_cur, t_end, dt;
   unsigned long long smax;  // dt reduction target, reset after each use
@@ -70,9 +70,14 @@ hipError_t batch_launch_fixup(const BlockDesc* blocks, const InstDesc* insts, in
 // u, v, w, T from the stage input state `which`)
 hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, int precision,
                              int which, int wgs, int* ctr, hipStream_t s);
-// precision as above; the tolerance-bound solves' pair form stores no primitives
-hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int stage, int jb,
-                               int precision, int wgs, int* ctr, hipStream_t s);
+// Stage tail of the solves of one form (PREC_FP64, PREC_FP32_FLUX or
+// PREC_FP64_FAST): the z-face sweep fused with the update, from Q[in] into
+// Q[out]; FP64 solves' cells beside failing z-faces go to cfx.
+hipError_t batch_launch_zupdate(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int precision, int in,
+                                int out, int stage, int wgs, int* ctr, const FixList& cfx, hipStream_t s);
+// those cells (their faces in the reference-order form, then their update); resets cfx
+hipError_t batch_launch_zfixup(const BlockDesc* blocks, InstDesc* insts, int nb, int in, int out, int stage,
+                               const FixList& cfx, hipStream_t s);
 hipError_t batch_launch_ghost(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, int wgs,
                               int* ctr, hipStream_t s);
 hipError_t batch_launch_dt(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int wgs, int* ctr, int ninst,

@@ -18,6 +18,7 @@
 #include "cbns_batch.h"
 #include "cbns_bodies.h"
 #include "cbns_face_fast.h"
+#include "cbns_zfuse.h"
 
 namespace cbns {
 
@@ -125,7 +126,8 @@ __global__ __launch_bounds__(TPB) void kb_visc(const BlockDesc* __restrict__ blo
     if (!I.stepping) return;
     // one (column tile, z-chunk) unit of the viscous tile march per entry
     const int ntiles = ((B.n[0] + VX - 1) / VX) * ((B.n[1] + VY - 1) / VY);
-    visc_tile_body<VT, FROMQ>(B, I.ph, tr.tile % ntiles, tr.tile / ntiles, which, VZ);
+    const int vz = chunk_len(ntiles, B.n[2], VZ, VISC_LOW, BATCH_UNITS);
+    visc_tile_body<VT, FROMQ>(B, I.ph, tr.tile % ntiles, tr.tile / ntiles, which, vz);
   });
 }
 
@@ -137,18 +139,21 @@ __device__ __forceinline__ void flush_smax(InstDesc* insts, int inst, double acc
   if (bad) atomicOr(&insts[inst].nonfinite, 1);
 }
 
-// At stage 3 the pair form also reduces the next step's calc_dt spectral
-// radius of every updated cell into its solve's smax (as the single-solve
-// stage tail does), so kb_dt skips such solves on the next step.  The
-// descriptors are read through the read-only insts (so that their uniform
-// fields come through the scalar cache) and only smax and nonfinite, which
-// no kernel of the stage reads, are written through insts_rw.
-// PRIMS: whether the pair form stores the primitives (not for the
-// tolerance-bound solves, whose kernels form them from the state)
-template <typename FT, bool PRIMS = true>
-__global__ __launch_bounds__(TPB) void kb_update(const BlockDesc* __restrict__ blocks,
-                                                 const InstDesc* __restrict__ insts, InstDesc* insts_rw,
-                                                 TileTable tt, int stage, int jb, int* ctr) {
+// Stage tail: the z-face sweep fused with the update (zrun, cbns_zfuse.h),
+// one (column tile, z-run) unit of one block per table entry, the run length
+// chosen from the block's size as the table was built.  At stage 3 it also
+// reduces the next step's calc_dt spectral radius of every updated cell
+// into its solve's smax (as the single-solve stage tail does), so kb_dt
+// skips such solves on the next step.  The descriptors are read through the
+// read-only insts (so that their uniform fields come through the scalar
+// cache) and only smax and nonfinite, which no kernel of the stage reads,
+// are written through insts_rw.
+template <int FORM>
+__global__ __launch_bounds__(TPB, 2) void kb_zface_update(const BlockDesc* __restrict__ blocks,
+                                                          const InstDesc* __restrict__ insts, InstDesc* insts_rw,
+                                                          TileTable tt, int in, int out, int stage, int* ctr,
+                                                          FixList cfx) {
+  static_assert(TPB == ZX * ZY, "one column per thread");
   int cur = -1, bad = 0;
   double acc = 0.0;
   for_tiles(tt.n, ctr, [&](int x) {
@@ -156,23 +161,77 @@ __global__ __launch_bounds__(TPB) void kb_update(const BlockDesc* __restrict__ b
     const BlockDesc& B = blocks[tr.block];
     const InstDesc& I = insts[B.inst];
     if (!I.stepping) return;
-    const int t = tr.tile * TPB + threadIdx.x;
-    if (B.n[0] % 2 == 0) {  // x-pairs of cells (the table holds pair tiles for such blocks)
-      const bool want_s = stage == 3 && I.pairs;
-      if (want_s && B.inst != cur) {  // uniform: one block per tile
-        if (cur >= 0) flush_smax(insts_rw, cur, acc, bad);
-        cur = B.inst;
-        acc = 0.0;
-        bad = 0;
-      }
-      if (t >= (B.n[0] / 2) * B.n[1] * B.n[2]) return;
-      update_pair_body<FT, PRIMS>(B, t, I.dt, stage, I.ph, jb, want_s, &acc, &bad);
-    } else {
-      if (t >= B.n[0] * B.n[1] * B.n[2]) return;
-      update_body<FT>(B, t, I.dt, stage, I.ph, jb);
+    if (stage == 3 && B.inst != cur) {  // uniform: one block per tile
+      if (cur >= 0) flush_smax(insts_rw, cur, acc, bad);
+      cur = B.inst;
+      acc = 0.0;
+      bad = 0;
     }
+    const int ntx = (B.n[0] + ZX - 1) / ZX, nt = ntx * ((B.n[1] + ZY - 1) / ZY);
+    const int zm = chunk_len(nt, B.n[2], ZM, ZM_LOW, BATCH_UNITS);
+    const int tile = tr.tile % nt, run = tr.tile / nt;
+    const int i = (tile % ntx) * ZX + (int)threadIdx.x % ZX, j = (tile / ntx) * ZY + (int)threadIdx.x / ZX;
+    const int k0 = run * zm;
+    if (i < B.n[0] && j < B.n[1] && k0 < B.n[2])
+      zrun<FORM>(B, tr.block, i, j, k0, zm, in, out, stage, I.dt, I.ph, I.ref, stage == 3, &acc, &bad, cfx);
   });
   if (cur >= 0) flush_smax(insts_rw, cur, acc, bad);
+}
+
+// Cells of FP64 and tolerance-bound solves that the fused pass left
+// (zcell_fix), with each solve's own physics; at stage 3 each such cell's
+// spectral radius goes into its solve's smax.  More failures than the list
+// holds: every cell of every stepping FP64 solve is checked.  The last
+// workgroup resets the list.
+__device__ __forceinline__ void zfix_one(const BlockDesc& B, const InstDesc& I, InstDesc* insts_rw, int i, int j,
+                                         int k, int in, int out, int stage, bool all) {
+  double s = 0.0;
+  int bad = 0;
+  if (I.precision == PREC_FP64_FAST)
+    zcell_fix<ZF_FAST>(B, i, j, k, in, out, stage, I.dt, I.ph, all, stage == 3, &s, &bad);
+  else
+    zcell_fix<ZF_FP64>(B, i, j, k, in, out, stage, I.dt, I.ph, all, stage == 3, &s, &bad);
+  if (stage == 3) {
+    atomicMax(&insts_rw[B.inst].smax, (unsigned long long)__double_as_longlong(s));
+    if (bad) atomicOr(&insts_rw[B.inst].nonfinite, 1);
+  }
+}
+
+constexpr int ZFIX_WG = 64;  // workgroups of the cell fixup pass
+
+__global__ __launch_bounds__(TPB) void kb_zcell_fixup(const BlockDesc* __restrict__ blocks,
+                                                      const InstDesc* __restrict__ insts, InstDesc* insts_rw, int nb,
+                                                      int in, int out, int stage, FixList cfx) {
+  const unsigned n = *cfx.count;  // every workgroup reads it before the last one resets it (below)
+  const long long first = blockIdx.x * (long long)TPB + threadIdx.x, stride = (long long)gridDim.x * TPB;
+  if (n <= cfx.cap) {
+    for (long long e = first; e < n; e += stride) {
+      const unsigned long long v = cfx.ent[e];
+      const long long ct = (long long)(v & ((1ull << 40) - 1));
+      const BlockDesc& B = blocks[(int)(v >> 42)];
+      const long long r = ct / B.n[0];
+      zfix_one(B, insts[B.inst], insts_rw, (int)(ct % B.n[0]), (int)(r % B.n[1]), (int)(r / B.n[1]), in, out, stage,
+               false);
+    }
+  } else {
+    for (int b = 0; b < nb; b++) {
+      const BlockDesc& B = blocks[b];
+      const InstDesc& I = insts[B.inst];
+      if (!I.stepping || I.precision == PREC_FP32_FLUX) continue;
+      for (long long ct = first; ct < B.sn; ct += stride) {
+        const long long r = ct / B.n[0];
+        zfix_one(B, I, insts_rw, (int)(ct % B.n[0]), (int)(r % B.n[1]), (int)(r / B.n[1]), in, out, stage, true);
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(cfx.done, 1u) == gridDim.x - 1) {
+      atomicExch(cfx.count, 0u);
+      atomicExch(cfx.done, 0u);
+    }
+  }
 }
 
 __global__ __launch_bounds__(TPB) void kb_ghost(const BlockDesc* __restrict__ blocks,
@@ -312,20 +371,26 @@ hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable
   return hipGetLastError();
 }
 
-hipError_t batch_launch_update(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int stage, int jb,
-                               int precision, int wgs, int* ctr, hipStream_t s) {
+hipError_t batch_launch_zupdate(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int precision, int in,
+                                int out, int stage, int wgs, int* ctr, const FixList& cfx, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
   if (wgs == 0) {  // direct launch
     wgs = (tt.n + NXCD - 1) / NXCD * NXCD;
     ctr = nullptr;
   }
+  const dim3 g(wgs), b(TPB);
   if (precision == PREC_FP64)
-    hipLaunchKernelGGL(kb_update<double>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, insts, tt, stage, jb, ctr);
+    hipLaunchKernelGGL(kb_zface_update<ZF_FP64>, g, b, 0, s, blocks, insts, insts, tt, in, out, stage, ctr, cfx);
   else if (precision == PREC_FP64_FAST)
-    hipLaunchKernelGGL((kb_update<double, false>), dim3(wgs), dim3(TPB), 0, s, blocks, insts, insts, tt, stage, jb,
-                       ctr);
+    hipLaunchKernelGGL(kb_zface_update<ZF_FAST>, g, b, 0, s, blocks, insts, insts, tt, in, out, stage, ctr, cfx);
   else
-    hipLaunchKernelGGL(kb_update<float>, dim3(wgs), dim3(TPB), 0, s, blocks, insts, insts, tt, stage, jb, ctr);
+    hipLaunchKernelGGL(kb_zface_update<ZF_FP32>, g, b, 0, s, blocks, insts, insts, tt, in, out, stage, ctr, cfx);
+  return hipGetLastError();
+}
+
+hipError_t batch_launch_zfixup(const BlockDesc* blocks, InstDesc* insts, int nb, int in, int out, int stage,
+                               const FixList& cfx, hipStream_t s) {
+  hipLaunchKernelGGL(kb_zcell_fixup, dim3(ZFIX_WG), dim3(TPB), 0, s, blocks, insts, insts, nb, in, out, stage, cfx);
   return hipGetLastError();
 }
 
@@ -396,9 +461,9 @@ hipError_t batch_grids(BatchGrids* g) {
   res((const void*)kb_visc<double>, &g->visc[0]);
   res((const void*)kb_visc<float>, &g->visc[1]);
   res((const void*)kb_visc<double, true>, &g->visc[2]);
-  res((const void*)kb_update<double>, &g->update[0]);
-  res((const void*)kb_update<float>, &g->update[1]);
-  res((const void*)kb_update<double, false>, &g->update[2]);
+  res((const void*)kb_zface_update<ZF_FP64>, &g->update[0]);
+  res((const void*)kb_zface_update<ZF_FP32>, &g->update[1]);
+  res((const void*)kb_zface_update<ZF_FAST>, &g->update[2]);
   res((const void*)kb_ghost, &g->ghost);
   res((const void*)kb_dt, &g->dt);
   return e;

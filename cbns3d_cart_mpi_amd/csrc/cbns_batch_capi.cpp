@@ -80,6 +80,7 @@ struct cbns_batch {
   BatchGrids grid{};  // resident workgroups of each persistent kernel
   int* d_ctr = nullptr;  // tile counters of the step graph's launches
   FixList fix{};         // faces of tolerance-bound solves left to the fixup pass
+  FixList cfix{};        // cells of FP64 solves that the fused z pass left to the cell fixup
   // Arrays of removed solves, kept for reuse by later solves of the same
   // shapes (by size): hipFree synchronises the whole device, which would stall
   // every other stream of a server between its chunks.  Released when fresh
@@ -94,7 +95,8 @@ struct cbns_batch {
   InstStatus* h_status = nullptr;
   InstStatus* d_status = nullptr;
   bool published = false;  // h_status reflects the device descriptors
-  bool has_fast = false; // a live tolerance-bound solve (the step graph has fixup passes)
+  bool has_fast = false; // a live tolerance-bound solve (the step graph has face fixup passes)
+  bool has_f64 = false;  // a live FP64 solve of either form (the step graph has cell fixup passes)
   int wg_mem = 0;     // override of the memory-bound kernels' grid (0: resident)
   int ninst_hw = 0;  // highest live id + 1
 };
@@ -143,7 +145,7 @@ static uint64_t solve_bytes(const cbns_params* p, const int dims[3]) {
         const uint64_t g = (n[0] + 8) * (n[1] + 8) * (n[2] + 8);
         const uint64_t in = n[0] * n[1] * n[2];
         const uint64_t f = (n[0] + 1) * n[1] * n[2] + n[0] * (n[1] + 1) * n[2] + n[0] * n[1] * (n[2] + 1);
-        total += 8 * (2 * NEQ * g + NEQ * f + 6 * g + in + 4 * in);
+        total += 8 * (3 * NEQ * g + NEQ * f + 6 * g + in + 4 * in);
       }
   return total + sizeof(BlockDesc) * dims[0] * dims[1] * dims[2];
 }
@@ -236,11 +238,16 @@ static int rebuild(cbns_batch* b) {
         const int nt = (int)((count + 255) / 256);
         for (int t = 0; t < nt; t++) tab[cls].push_back({bi, t});
       };
-      for (int d = 0; d < 3; d++) add(FACE_CLASS[face_form(I.precision)] + d, face_items(B, d));
-      const int vunits = ((B.n[0] + VX - 1) / VX) * ((B.n[1] + VY - 1) / VY) * ((B.n[2] + VZ - 1) / VZ);
-      for (int t = 0; t < vunits; t++)  // whole-workgroup units
-        tab[VISC_CLASS[face_form(I.precision)]].push_back({bi, t});
-      add(UPD_CLASS[face_form(I.precision)], B.n[0] % 2 == 0 ? B.sn / 2 : B.sn);  // x-pairs for even n0
+      // x and y sweeps (the z faces are formed by the fused stage tail)
+      for (int d = 0; d < 2; d++) add(FACE_CLASS[face_form(I.precision)] + d, face_items(B, d));
+      // whole-workgroup units of the viscous march and of the fused stage
+      // tail: (column tile, z-chunk), the chunk lengths as the kernels form them
+      const int vt = ((B.n[0] + VX - 1) / VX) * ((B.n[1] + VY - 1) / VY);
+      const int vz = chunk_len(vt, B.n[2], VZ, VISC_LOW, BATCH_UNITS);
+      for (int t = 0; t < vt * ((B.n[2] + vz - 1) / vz); t++) tab[VISC_CLASS[face_form(I.precision)]].push_back({bi, t});
+      const int zt = ((B.n[0] + ZX - 1) / ZX) * ((B.n[1] + ZY - 1) / ZY);
+      const int zm = chunk_len(zt, B.n[2], ZM, ZM_LOW, BATCH_UNITS);
+      for (int t = 0; t < zt * ((B.n[2] + zm - 1) / zm); t++) tab[UPD_CLASS[face_form(I.precision)]].push_back({bi, t});
       add(T_GHOST, B.sg - B.sn);  // ghost cells only
       add(T_DT, B.sn);
     }
@@ -249,8 +256,12 @@ static int rebuild(cbns_batch* b) {
                  hipMemcpyHostToDevice));
   }
   b->ninst_hw = hw;
-  b->has_fast = false;
-  for (int id = 0; id < b->cap; id++) b->has_fast = b->has_fast || (b->inst[id].live && b->inst[id].precision == CBNS_FP64_FAST);
+  b->has_fast = b->has_f64 = false;
+  for (int id = 0; id < b->cap; id++) {
+    if (!b->inst[id].live) continue;
+    b->has_fast = b->has_fast || b->inst[id].precision == CBNS_FP64_FAST;
+    b->has_f64 = b->has_f64 || b->inst[id].precision != CBNS_FP32_FLUX;
+  }
   int st;
   if ((st = grow((void**)&b->d_blocks, &b->d_blocks_cap, (int)all.size() + 1, sizeof(BlockDesc)))) return st;
   if (!all.empty())
@@ -279,7 +290,7 @@ static int rebuild(cbns_batch* b) {
   auto tail = [&]() { return b->wg_mem > 0 ? b->wg_mem : 0; };
   // every persistent launch of the step takes its own block of counters, all
   // zeroed by the first node of the graph
-  constexpr int LAUNCHES = 1 + 3 * 16;  // dt; per stage 3 viscous, 9 face, 3 update, 1 ghost
+  constexpr int LAUNCHES = 1 + 3 * 13;  // dt; per stage 3 viscous, 6 face, 3 fused z + update, 1 ghost
   constexpr int K = BATCH_CTR_PER_LAUNCH * BATCH_CTR_STRIDE;
   static_assert(LAUNCHES * K <= BATCH_COUNTERS, "counter block");
   int used = 0;
@@ -288,8 +299,10 @@ static int rebuild(cbns_batch* b) {
   if (e == hipSuccess) e = batch_launch_dt(b->d_blocks, b->d_insts, b->tt[T_DT], mem(G.dt), ctr(), hw, s);
   if (e == hipSuccess) e = batch_launch_finalize(b->d_insts, hw, s);
   for (int stage = 1; stage <= 3 && e == hipSuccess; stage++) {
-    const int in = stage == 1 ? CBNS_Q : CBNS_QP;
-    const int out = stage == 3 ? CBNS_Q : CBNS_QP;
+    // the fused z pass keeps its stage input unwritten: Q -> S -> Q_p -> Q
+    // (S: the third state), so Q_p holds the stage-2 state after each step
+    const int in = stage == 1 ? CBNS_Q : (stage == 2 ? 2 : CBNS_QP);
+    const int out = stage == 1 ? 2 : (stage == 2 ? CBNS_QP : CBNS_Q);
     const bool ov = b->visc_side_wgs > 0;
     hipStream_t sv = ov ? b->side : s;
     if (ov) {  // fork: the viscous kernels run beside the face sweeps
@@ -300,7 +313,7 @@ static int rebuild(cbns_batch* b) {
       e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[VISC_CLASS[f]], b->jb, FORM_PREC[f], in,
                             ov ? b->visc_side_wgs : G.visc[f], ctr(), sv);
     if (ov && e == hipSuccess) e = hipEventRecord(b->ev_join, sv);
-    for (int d = 0; d < 3 && e == hipSuccess; d++)
+    for (int d = 0; d < 2 && e == hipSuccess; d++)
       for (int f = 0; f < 3 && e == hipSuccess; f++)
         e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[FACE_CLASS[f] + d], d, in, b->jb, f, G.face[f][d], ctr(),
                               b->fix, s);
@@ -308,8 +321,10 @@ static int rebuild(cbns_batch* b) {
     if (b->has_fast && e == hipSuccess) e = batch_launch_fixup(b->d_blocks, b->d_insts, (int)all.size(), in, b->fix, s);
     if (ov && e == hipSuccess) e = hipStreamWaitEvent(s, b->ev_join, 0);  // join before the stage tail
     for (int f = 0; f < 3 && e == hipSuccess; f++)
-      e = batch_launch_update(b->d_blocks, b->d_insts, b->tt[UPD_CLASS[f]], stage, b->jb, FORM_PREC[f], tail(),
-                              ctr(), s);
+      e = batch_launch_zupdate(b->d_blocks, b->d_insts, b->tt[UPD_CLASS[f]], FORM_PREC[f], in, out, stage, tail(),
+                               ctr(), b->cfix, s);
+    if (b->has_f64 && e == hipSuccess)
+      e = batch_launch_zfixup(b->d_blocks, b->d_insts, (int)all.size(), in, out, stage, b->cfix, s);
     if (e == hipSuccess)
       e = batch_launch_ghost(b->d_blocks, b->d_insts, b->tt[T_GHOST], out, tail(), ctr(), s);
   }
@@ -348,6 +363,10 @@ int cbns_batch_create(uint64_t hbm_budget_bytes, int capacity, cbns_batch** out)
   if (!st && hipMalloc(&fixbuf, FIX_BYTES) != hipSuccess) st = CBNS_ERR_NOMEM;
   if (!st && hipMemset(fixbuf, 0, 256) != hipSuccess) st = CBNS_ERR_HIP;
   if (fixbuf) b->fix = fixlist_at(fixbuf);
+  void* cfixbuf = nullptr;
+  if (!st && hipMalloc(&cfixbuf, FIX_BYTES) != hipSuccess) st = CBNS_ERR_NOMEM;
+  if (!st && hipMemset(cfixbuf, 0, 256) != hipSuccess) st = CBNS_ERR_HIP;
+  if (cfixbuf) b->cfix = fixlist_at(cfixbuf);
   if (!st && hipHostMalloc((void**)&b->h_status, sizeof(InstStatus) * capacity, hipHostMallocMapped) != hipSuccess)
     st = CBNS_ERR_NOMEM;
   if (!st && hipHostGetDevicePointer((void**)&b->d_status, b->h_status, 0) != hipSuccess) st = CBNS_ERR_HIP;
@@ -386,6 +405,7 @@ int cbns_batch_destroy(cbns_batch* b) {
   if (b->d_part) (void)hipFree(b->d_part);
   if (b->d_ctr) (void)hipFree(b->d_ctr);
   if (b->fix.count) (void)hipFree(b->fix.count);
+  if (b->cfix.count) (void)hipFree(b->cfix.count);
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   if (b->ev_join) (void)hipEventDestroy(b->ev_join);
   if (b->side) (void)hipStreamDestroy(b->side);
@@ -462,7 +482,7 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
         B.sn = (long long)B.n[0] * B.n[1] * B.n[2];
         for (int d = 0; d < 3; d++)
           B.sf[d] = (long long)(B.n[0] + (d == 0)) * (B.n[1] + (d == 1)) * (B.n[2] + (d == 2));
-        ok = (B.Q[0] = alloc(8 * NEQ * B.sg)) && (B.Q[1] = alloc(8 * NEQ * B.sg)) &&
+        ok = (B.Q[0] = alloc(8 * NEQ * B.sg)) && (B.Q[1] = alloc(8 * NEQ * B.sg)) && (B.Q[2] = alloc(8 * NEQ * B.sg)) &&
              (B.F[0] = alloc(8 * NEQ * B.sf[0])) && (B.F[1] = alloc(8 * NEQ * B.sf[1])) &&
              (B.F[2] = alloc(8 * NEQ * B.sf[2])) && (B.prim = alloc((precision == CBNS_FP32_FLUX ? 4 : 8) * 6 * B.sg)) && (B.mu = alloc(8 * B.sn)) &&
              (B.dfv = alloc(8 * 4 * B.sn));
@@ -537,8 +557,7 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
   D.dom = I.dom;
   D.nblk = nb;
   D.precision = precision;
-  D.pairs = 1;
-  for (const BlockDesc& B : I.blocks) D.pairs &= B.n[0] % 2 == 0;
+  D.pairs = 1;  // every solve's stage tail reduces the next step's smax
   D.active = 1;
   D.steps_target = steps;
   D.budget = 0x7fffffff;  // unbounded until the host sets budgets

@@ -161,11 +161,24 @@ hipError_t launch_gather_aos(const BlockDesc* db, const BlockDesc* hb, const Dom
 hipError_t launch_fill_mu(const BlockDesc* db, const BlockDesc* hb, int nb, double mu0, hipStream_t s);
 // Viscous tile march: VX x VY interior columns per workgroup (VX * VY
 // threads), at most VZ planes per z-chunk (fewer where a grid has too few
-// column tiles to fill the device: visc_chunk_planes).
+// column tiles to fill the device: chunk_len).
 constexpr int VX = 16, VY = 16, VZ = 32;
-// Planes per chunk of the single solve's viscous march: VZ, halved (down to
-// 4) until the grid has at least 1024 units.
-int visc_chunk_planes(const BlockDesc* hb, int nb);
+// Planes per chunk of a march over `tiles` column tiles (of all blocks it
+// covers) and n2 planes: top, halved (down to low) until there are at least
+// `want` (tile, chunk) units.  The single solve asks 1024 units of its
+// whole grid; the batched form 256 of each block (the other solves fill the
+// device), from the same block sizes on the host and the device.
+__host__ __device__ inline int chunk_len(long long tiles, int n2, int top, int low, long long want) {
+  int v = top;
+  while (v > low && tiles * ((n2 + v - 1) / v) < want) v /= 2;
+  return v;
+}
+constexpr int VISC_LOW = 4;    // fewest planes per viscous chunk
+// Fused z-face and stage-tail pass (cbns_zfuse.h): ZX x ZY columns per
+// workgroup, one per thread, each with a run of at most ZM and at least
+// ZM_LOW cells along z (chunk_len).
+constexpr int ZX = 64, ZY = 4, ZM = 32, ZM_LOW = 2;
+constexpr int BATCH_UNITS = 256;  // units per block the batched marches ask for
 constexpr int GSUM_WG = 512;  // workgroups of the fixed-order global sums
 hipError_t launch_global_sums(const BlockDesc* db, const Domain& dom, const Phys& ph, int what, double* d_part,
                               double* d_out, int precision, hipStream_t s);

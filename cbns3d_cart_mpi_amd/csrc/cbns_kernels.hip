@@ -159,16 +159,14 @@ __global__ __launch_bounds__(VX * VY) void k_visc_flux(const BlockDesc* __restri
   visc_tile_body<VT, FROMQ>(blocks[blockIdx.z], ph, tl.x, tl.y, which, vz);
 }
 
-int visc_chunk_planes(const BlockDesc* hb, int nb) {
+static int visc_chunk_planes(const BlockDesc* hb, int nb) {
   long long tiles = 0;
   int n2 = 0;
   for (int b = 0; b < nb; b++) {
     tiles = std::max(tiles, (long long)((hb[b].n[0] + VX - 1) / VX) * ((hb[b].n[1] + VY - 1) / VY));
     n2 = std::max(n2, hb[b].n[2]);
   }
-  int vz = VZ;
-  while (vz > 4 && tiles * nb * ((n2 + vz - 1) / vz) < 1024) vz /= 2;
-  return vz;
+  return chunk_len(tiles * nb, n2, VZ, VISC_LOW, 1024);
 }
 
 static dim3 visc_grid(const BlockDesc* hb, int nb, int vz) {

@@ -1,132 +1,8 @@
-// The z-face sweep fused with the stage tail (single solve): rec_riemann_z
-// (src/Block3d.cpp:690-748) and update_rk3 with the viscous contribution and
-// calc_primitive of the updated cell (src/Block3d.cpp:870-962, 359-385) in
-// one pass, for the bit-exact FP64, the FP32-flux and the tolerance-bound
-// FP64 forms.
-//
-// A thread owns one column (i, j) and a run of ZM cells along z.  It marches
-// the faces of its run with a six-cell register queue of the stage input, so
-// each cell of Q[in] is loaded once per run (plus the five more its first
-// face needs) and no z-flux reaches HBM; after face k it updates cell k - 1
-// from faces k - 1 and k.  The operands of a cell's update (x- and y-face
-// fluxes, viscous divergence, Q^n) are loaded before its face's arithmetic,
-// so that the tail's HBM traffic runs under the face arithmetic, which
-// bounds the z sweep on its own.  The stage input is never written (stage 1:
-// Q -> S, stage 2: S -> Q_p, stage 3: Q_p -> Q, with S a third state), so
-// the runs of a column need no ordering.
-//
-// The face of each form is its sweep kernels' arithmetic on the same
-// operands: the primitives of the two cells beside a face (which the plain
-// and FP32-flux sweeps read from the stored primitives) are formed here by
-// calc_primitive's expressions from the queued state, which gives the
-// stored values (rounded to float for FP32-flux as stored there); the
-// FP32-flux face fluxes are rounded to float as its face arrays store them.
-// Every value therefore equals the separate sweep and tail kernels' (the
-// batched form runs those).
-#include "cbns_face_fast.h"
+// Single-solve kernels of the z-face sweep fused with the stage tail (the
+// arithmetic and its layout: cbns_zfuse.h).
+#include "cbns_zfuse.h"
 
 namespace cbns {
-
-// columns per workgroup along x, y; at most ZM cells per run (fewer where a
-// grid has too few columns to fill the device: halved, down to 2, until the
-// grid has at least 1024 workgroups)
-constexpr int ZX = 64, ZY = 4, ZM = 32;
-enum { ZF_FP64 = 0, ZF_FP32 = 1, ZF_FAST = 2 };
-
-template <int FORM>
-using zft = typename std::conditional<FORM == ZF_FP32, float, double>::type;
-
-// Operands of the update of one cell other than its z-fluxes and its stage
-// input, in their storage types.
-template <typename FT>
-struct ZOperands {
-  FT e0[5], e1[5], f0[5], f1[5], dv[4];
-  double qn[5];
-};
-
-template <typename FT>
-__device__ __forceinline__ void zload(const BlockDesc& B, int i, int j, int k, int stage, ZOperands<FT>& o) {
-  const int n0 = B.n[0], n1 = B.n[1];
-  const int iE = i + (n0 + 1) * (j + n1 * k), iF = i + n0 * (j + (n1 + 1) * k), ct = i + n0 * (j + n1 * k);
-  gptr<const FT> __restrict__ E = gmem<const FT>((const FT*)B.F[0]);
-  gptr<const FT> __restrict__ F = gmem<const FT>((const FT*)B.F[1]);
-  gptr<const FT> __restrict__ DV = gmem<const FT>((const FT*)B.dfv);
-  gptr<const double> __restrict__ Qn = gmem<const double>(B.Q[0]);
-  const int c = gidx(B, i, j, k);
-#pragma unroll
-  for (int e = 0; e < 5; e++) {
-    o.e0[e] = E[e * B.sf[0] + iE];
-    o.e1[e] = E[e * B.sf[0] + iE + 1];
-    o.f0[e] = F[e * B.sf[1] + iF];
-    o.f1[e] = F[e * B.sf[1] + iF + n0];
-    if (stage > 1) o.qn[e] = Qn[e * B.sg + c];
-  }
-#pragma unroll
-  for (int e = 0; e < 4; e++) o.dv[e] = DV[e * B.sn + ct];
-}
-
-// The update of interior cell (i, j, k) with the expressions and order of
-// update_pair_body: x-, y- and z-flux differences less the viscous
-// divergence, then the RK3 combination of Q^n (for stage 1 the stage input
-// qi itself) and qi, into Q[out]; the plain and FP32-flux forms also store
-// the cell's primitives.  With want_s, the calc_dt spectral radius of the
-// updated cell (src/Block3d.cpp:527-546, mu of this stage's viscous pass)
-// goes into *s (a NaN never wins; *bad flags a non-finite one).
-template <int FORM>
-__device__ __forceinline__ void zapply(const BlockDesc& B, int i, int j, int k, const ZOperands<zft<FORM>>& o,
-                                       const double (&gk)[5], const double (&gk1)[5], const double (&qi)[5], int out,
-                                       int stage, double dt, const Phys& ph, bool want_s, double* s, int* bad) {
-  using FT = zft<FORM>;
-  gptr<double> __restrict__ Qo = gmem(B.Q[out]);
-  const long long sg = B.sg;
-  const int c = gidx(B, i, j, k);
-  double r[5];
-#pragma unroll
-  for (int e = 0; e < 5; e++) {
-    double df = divc((double)o.e1[e] - (double)o.e0[e], ph.dx, ph.rdx, ph.fdx);
-    df += divc((double)o.f1[e] - (double)o.f0[e], ph.dy, ph.rdy, ph.fdy);
-    df += divc(gk1[e] - gk[e], ph.dz, ph.rdz, ph.fdz);
-    if (e > 0) df -= (double)o.dv[e - 1];
-    if (1 == stage) r[e] = qi[e] - dt * df;
-    else if (2 == stage) r[e] = 0.75 * o.qn[e] + 0.25 * (qi[e] - dt * df);
-    else r[e] = (o.qn[e] + 2.0 * (qi[e] - dt * df)) / 3.0;
-    Qo[e * sg + c] = r[e];
-  }
-  if (FORM == ZF_FAST && !want_s) return;
-  const double rr = r[0];
-  const double uu = r[1] / rr;
-  const double vv = r[2] / rr;
-  const double ww = r[3] / rr;
-  const double pp = ph.gam1 * (r[4] - 0.5 * rr * (uu * uu + vv * vv + ww * ww));
-  if constexpr (FORM != ZF_FAST) {
-    gptr<FT> __restrict__ P = gmem((FT*)B.prim);
-    P[c] = (FT)rr;
-    P[sg + c] = (FT)uu;
-    P[2 * sg + c] = (FT)vv;
-    P[3 * sg + c] = (FT)ww;
-    P[4 * sg + c] = (FT)pp;
-    P[5 * sg + c] = (FT)(ph.gM2 * pp / rr);
-  }
-  if (want_s) {
-    const double sc = dt_point(rr, uu, vv, ww, pp, gmem<const double>(B.mu)[i + B.n[0] * (j + B.n[1] * k)], ph, bad);
-    *s = (*s < sc) ? sc : *s;
-  }
-}
-
-// rho, u, v, w, p of one cell by calc_primitive's expressions
-// (src/Block3d.cpp:369-373), as the plain and FP32-flux sweeps read them
-// from the stored primitives (rounded to float for FP32-flux).
-template <int FORM>
-__device__ __forceinline__ void zprim(const double (&q)[5], const Phys& ph, double (&p)[5]) {
-  const double rr = q[0];
-  const double uu = q[1] / rr;
-  const double vv = q[2] / rr;
-  const double ww = q[3] / rr;
-  const double pp = ph.gam1 * (q[4] - 0.5 * rr * (uu * uu + vv * vv + ww * ww));
-  const double v[5] = {rr, uu, vv, ww, pp};
-#pragma unroll
-  for (int f = 0; f < 5; f++) p[f] = FORM == ZF_FP32 ? (double)(float)v[f] : v[f];
-}
 
 // Block-wide max of s into *smax (integer atomic max on the bits of a
 // non-negative double; a NaN never wins) and the non-finite flag.  Made by
@@ -145,101 +21,22 @@ __device__ __forceinline__ void reduce_smax(double s, int bad, unsigned long lon
   if (bad) atomicOr(flag, 1);
 }
 
-// One thread's run.  The tolerance-bound and plain forms leave the cells
-// beside a face whose range checks fail to cfx: k_zcell_fixup recomputes
-// such a face with the reference-order form (the plain form's sweeps
-// recompute it in place; the result is the same) and updates the cells.
-// Keeping that form out of this kernel keeps its registers to the checked
-// form's.
 template <int FORM>
 __global__ __launch_bounds__(ZX * ZY, 2) void k_zface_update(const BlockDesc* __restrict__ blocks, int in, int out,
                                                           int stage, double dt, Phys ph, RefState ref,
                                                           unsigned long long* smax, int* flag, FixList cfx,
                                                           int zm) {
-  using FT = zft<FORM>;
   const Tile tl = xcd_tile();
   const BlockDesc& B = blocks[blockIdx.z];
-  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
-  const int ntx = (n0 + ZX - 1) / ZX;
+  const int ntx = (B.n[0] + ZX - 1) / ZX;
   const int i = (tl.x % ntx) * ZX + (int)threadIdx.x % ZX;
   const int j = (tl.x / ntx) * ZY + (int)threadIdx.x / ZX;
   const int k0 = tl.y * zm;
   const bool want = smax != nullptr;  // uniform
   double s = 0.0;
   int bad = 0;
-  if (i < n0 && j < n1 && k0 < n2) {
-    const int k1 = k0 + zm < n2 ? k0 + zm : n2;  // cells [k0, k1), faces k0 .. k1
-    const int sz = B.g[0] * B.g[1];
-    const long long sg = B.sg;
-    gptr<const double> __restrict__ Qin = gmem<const double>(B.Q[in]);
-    const int c0 = gidx(B, i, j, 0);
-    double q[6][5];  // cells k - 3 .. k + 2 of face k (planes -3 .. n2 + 2: inside the ghosts)
-#pragma unroll
-    for (int cc = 0; cc < 6; cc++)
-#pragma unroll
-      for (int e = 0; e < 5; e++) q[cc][e] = Qin[e * sg + c0 + (k0 - 3 + cc) * sz];
-    double pl[5];  // primitives of cell k - 1 (FP32-flux form; the plain form forms them at each face)
-    if constexpr (FORM != ZF_FAST) zprim<FORM>(q[2], ph, pl);
-    double gp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    bool okp = true;
-    for (int k = k0; k <= k1; k++) {
-      // the plain form's heavier face leaves no registers to hold values
-      // across it: it loads the operands and the next cell after the face
-      // and forms the primitives of both of its cells at each face
-      constexpr bool early = FORM != ZF_FP64;
-      ZOperands<FT> o;
-      if (early && k > k0) zload<FT>(B, i, j, k - 1, stage, o);
-      double nx[5];
-      if (early && k < k1) {
-#pragma unroll
-        for (int e = 0; e < 5; e++) nx[e] = Qin[e * sg + c0 + (k + 3) * sz];
-      }
-      double g[5];
-      bool ok = true;
-      if constexpr (FORM == ZF_FAST) {
-        ok = face_flux_fast<2>(q, ph.gamma, ph.gam1, ph.gg1, g);
-      } else {
-        double pr[5];  // primitives of cell k
-        zprim<FORM>(q[3], ph, pr);
-        if constexpr (FORM == ZF_FP64) zprim<FORM>(q[2], ph, pl);
-        if constexpr (FORM == ZF_FP32) {
-          float g32[5];
-          face_flux_f32<2>(q, pl, pr, ph.gamma, ph.gam1, ref, g32);
-#pragma unroll
-          for (int e = 0; e < 5; e++) g[e] = (double)g32[e];
-        } else {
-          face_flux<2, false, true>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, g, ok);
-        }
-#pragma unroll
-        for (int f = 0; f < 5; f++) pl[f] = pr[f];
-      }
-      if (k > k0) {  // cell k - 1 is q[2]
-        if (!early && ok && okp) zload<FT>(B, i, j, k - 1, stage, o);
-        if (ok && okp) zapply<FORM>(B, i, j, k - 1, o, gp, g, q[2], out, stage, dt, ph, want, &s, &bad);
-        else {
-          const unsigned slot = atomicAdd(cfx.count, 1u);
-          if (slot < cfx.cap)
-            cfx.ent[slot] = (unsigned long long)(i + (long long)n0 * (j + (long long)n1 * (k - 1))) |
-                            ((unsigned long long)blockIdx.z << 42);
-        }
-      }
-      if (!early && k < k1) {
-#pragma unroll
-        for (int e = 0; e < 5; e++) nx[e] = Qin[e * sg + c0 + (k + 3) * sz];
-      }
-      if (k < k1) {
-#pragma unroll
-        for (int cc = 0; cc < 5; cc++)
-#pragma unroll
-          for (int e = 0; e < 5; e++) q[cc][e] = q[cc + 1][e];
-#pragma unroll
-        for (int e = 0; e < 5; e++) q[5][e] = nx[e];
-      }
-#pragma unroll
-      for (int e = 0; e < 5; e++) gp[e] = g[e];
-      okp = ok;
-    }
-  }
+  if (i < B.n[0] && j < B.n[1] && k0 < B.n[2])
+    zrun<FORM>(B, blockIdx.z, i, j, k0, zm, in, out, stage, dt, ph, ref, want, &s, &bad, cfx);
   if (want) reduce_smax(s, bad, smax, flag);
 }
 
@@ -251,8 +48,7 @@ hipError_t launch_zface_update(const BlockDesc* db, const BlockDesc* hb, int nb,
     tiles = std::max(tiles, ((hb[b].n[0] + ZX - 1) / ZX) * ((hb[b].n[1] + ZY - 1) / ZY));
     n2 = std::max(n2, hb[b].n[2]);
   }
-  int zm = ZM;
-  while (zm > 2 && (long long)tiles * nb * ((n2 + zm - 1) / zm) < 1024) zm /= 2;
+  const int zm = chunk_len((long long)tiles * nb, n2, ZM, ZM_LOW, 1024);
   const dim3 grid(tiles, (n2 + zm - 1) / zm, nb), blk(ZX * ZY);
   if (precision == PREC_FP64_FAST)
     hipLaunchKernelGGL(k_zface_update<ZF_FAST>, grid, blk, 0, s, db, in, out, stage, dt, ph, ref, smax, flag, cfx,
@@ -266,61 +62,6 @@ hipError_t launch_zface_update(const BlockDesc* db, const BlockDesc* hb, int nb,
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
-}
-
-// One face of a listed cell: the checked form, and where its checks fail
-// the reference-order form with IEEE division and square root.  Returns
-// whether the checked form passed.
-template <int FORM>
-__device__ __forceinline__ bool zface_checked(const double (&q)[6][5], const Phys& ph, bool recompute,
-                                              double (&g)[5]) {
-  bool ok = true;
-  if constexpr (FORM == ZF_FAST) {
-    ok = face_flux_fast<2>(q, ph.gamma, ph.gam1, ph.gg1, g);
-    if (!ok && recompute) face_flux_fast_fallback<2>(q, ph, g);
-  } else {
-    double pl[5], pr[5];
-    zprim<FORM>(q[2], ph, pl);
-    zprim<FORM>(q[3], ph, pr);
-    face_flux<2, false, true>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, g, ok);
-    if (!ok && recompute) {
-      bool chk = true;
-      face_flux<2, false, false>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, g, chk);
-    }
-  }
-  return ok;
-}
-
-// A listed cell: both of its z-faces (zface_checked), then its update.  With
-// all set (more failures than the list holds), a cell is updated only if
-// one of its faces fails its checks: exactly the cells the fused pass left.
-template <int FORM>
-__device__ __forceinline__ void zcell_fix(const BlockDesc& B, int i, int j, int k, int in, int out, int stage,
-                                          double dt, const Phys& ph, bool all, bool want_s, double* s, int* bad) {
-  const int sz = B.g[0] * B.g[1];
-  const long long sg = B.sg;
-  gptr<const double> __restrict__ Qin = gmem<const double>(B.Q[in]);
-  const int c = gidx(B, i, j, k);
-  double q7[7][5];
-#pragma unroll
-  for (int cc = 0; cc < 7; cc++)
-#pragma unroll
-    for (int e = 0; e < 5; e++) q7[cc][e] = Qin[e * sg + c + (cc - 3) * sz];
-  double qa[6][5], qb[6][5];
-#pragma unroll
-  for (int cc = 0; cc < 6; cc++)
-#pragma unroll
-    for (int e = 0; e < 5; e++) {
-      qa[cc][e] = q7[cc][e];
-      qb[cc][e] = q7[cc + 1][e];
-    }
-  double g0[5], g1[5];
-  const bool ok0 = zface_checked<FORM>(qa, ph, true, g0);
-  const bool ok1 = zface_checked<FORM>(qb, ph, true, g1);
-  if (all && ok0 && ok1) return;
-  ZOperands<double> o;
-  zload<double>(B, i, j, k, stage, o);
-  zapply<FORM>(B, i, j, k, o, g0, g1, q7[3], out, stage, dt, ph, want_s, s, bad);
 }
 
 constexpr int ZFIX_WG = 64;  // workgroups of the cell fixup pass
