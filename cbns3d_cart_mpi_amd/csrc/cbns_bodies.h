@@ -730,9 +730,7 @@ __device__ __forceinline__ vec2<T> ld2(gptr<const T> p, long long o) {
 // With dts != nullptr (stage 3) it also returns in *s_out the larger calc_dt
 // spectral radius of the two updated cells (a NaN never wins), with mu from
 // this stage's viscous pass, which is what the next step's calc_dt reads.
-// PRIMS = false (tolerance-bound FP64 form, whose kernels form primitives
-// from the state): the primitives are not stored.
-template <typename FT, bool PRIMS = true>
+template <typename FT>
 __device__ __forceinline__ void update_pair_body(const BlockDesc& B, int t, double dt, int stage, const Phys& ph,
                                                  int jb, bool want_s = false, double* s_out = nullptr,
                                                  int* bad = nullptr) {
@@ -816,10 +814,8 @@ __device__ __forceinline__ void update_pair_body(const BlockDesc& B, int t, doub
       *s_out = (*s_out < sc) ? sc : *s_out;  // src/Block3d.cpp:546: a NaN never wins
     }
   }
-  if constexpr (PRIMS) {
 #pragma unroll
-    for (int f = 0; f < 6; f++) *(gptr<vec2<FT>>)(P + f * sg + c) = pr[f];
-  }
+  for (int f = 0; f < 6; f++) *(gptr<vec2<FT>>)(P + f * sg + c) = pr[f];
 }
 
 // ---- ghost gather of the state and its primitives ------------------------
