@@ -43,7 +43,7 @@ struct ZOperands {
 };
 
 template <typename FT>
-__device__ __forceinline__ void zload(const BlockDesc& B, int i, int j, int k, int stage, ZOperands<FT>& o) {
+__device__ __forceinline__ void zload(const BlockDesc& B, int i, int j, int k, ZOperands<FT>& o) {
   const int n0 = B.n[0], n1 = B.n[1];
   const int iE = i + (n0 + 1) * (j + n1 * k), iF = i + n0 * (j + (n1 + 1) * k), ct = i + n0 * (j + n1 * k);
   gptr<const FT> __restrict__ E = gmem<const FT>((const FT*)B.F[0]);
@@ -57,7 +57,7 @@ __device__ __forceinline__ void zload(const BlockDesc& B, int i, int j, int k, i
     o.e1[e] = E[e * B.sf[0] + iE + 1];
     o.f0[e] = F[e * B.sf[1] + iF];
     o.f1[e] = F[e * B.sf[1] + iF + n0];
-    if (stage > 1) o.qn[e] = Qn[e * B.sg + c];
+    o.qn[e] = Qn[e * B.sg + c];  // used by stages 2 and 3 (loaded always: no branch among the loads)
   }
 #pragma unroll
   for (int e = 0; e < 4; e++) o.dv[e] = DV[e * B.sn + ct];
@@ -154,14 +154,19 @@ __device__ __forceinline__ void zrun(const BlockDesc& B, int b, int i, int j, in
   double gp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   bool okp = true;
   for (int k = k0; k <= k1; k++) {
-    // the plain form's heavier face leaves no registers to hold values
-    // across it: it loads the operands and the next cell after the face
-    // and forms the primitives of both of its cells at each face
+    // The other forms issue the loads of this step (the operands of cell
+    // k - 1 and the next queued cell, plane k + 3 <= n2 + 3, inside the
+    // ghosts) before the face, with no branch among them, so that the
+    // compiler's wait for them falls after the face arithmetic; the first
+    // face of a run updates no cell and loads cell k0's operands unused.
+    // The plain form's heavier face leaves no registers to hold values
+    // across it: it loads them after the face and forms the primitives of
+    // both of its cells at each face.
     constexpr bool early = FORM != ZF_FP64;
     ZOperands<FT> o;
-    if (early && k > k0) zload<FT>(B, i, j, k - 1, stage, o);
+    if (early) zload<FT>(B, i, j, k > k0 ? k - 1 : k0, o);
     double nx[5];
-    if (early && k < k1) {
+    if (early) {
 #pragma unroll
       for (int e = 0; e < 5; e++) nx[e] = Qin[e * sg + c0 + (k + 3) * sz];
     }
@@ -185,7 +190,7 @@ __device__ __forceinline__ void zrun(const BlockDesc& B, int b, int i, int j, in
       for (int f = 0; f < 5; f++) pl[f] = pr[f];
     }
     if (k > k0) {  // cell k - 1 is q[2]
-      if (!early && ok && okp) zload<FT>(B, i, j, k - 1, stage, o);
+      if (!early && ok && okp) zload<FT>(B, i, j, k - 1, o);
       if (ok && okp) zapply<FORM>(B, i, j, k - 1, o, gp, g, q[2], out, stage, dt, ph, want, s, bad);
       else {
         const unsigned slot = atomicAdd(cfx.count, 1u);
@@ -198,7 +203,7 @@ __device__ __forceinline__ void zrun(const BlockDesc& B, int b, int i, int j, in
 #pragma unroll
       for (int e = 0; e < 5; e++) nx[e] = Qin[e * sg + c0 + (k + 3) * sz];
     }
-    if (k < k1) {
+    if (early || k < k1) {
 #pragma unroll
       for (int cc = 0; cc < 5; cc++)
 #pragma unroll
@@ -263,7 +268,7 @@ __device__ __forceinline__ void zcell_fix(const BlockDesc& B, int i, int j, int 
   const bool ok1 = zface_checked<FORM>(qb, ph, true, g1);
   if (all && ok0 && ok1) return;
   ZOperands<double> o;
-  zload<double>(B, i, j, k, stage, o);
+  zload<double>(B, i, j, k, o);
   zapply<FORM>(B, i, j, k, o, g0, g1, q7[3], out, stage, dt, ph, want_s, s, bad);
 }
 
