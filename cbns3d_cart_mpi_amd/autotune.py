@@ -36,9 +36,15 @@ def apply_schedule(solver, sched):
     solver.set_viscous_workgroups(int(sched.split(":")[1]) if ":" in sched else 0)
 
 
+SERIAL_MARGIN = 0.015  # an overlapped schedule must beat the best serial one by this fraction
+
+
 def tune(solver, steps=3, bands=BANDS, schedules=SCHEDULES, occs=(2, 3)):
     """Try each (band, schedule, face occupancy) on `steps` steps of the live
-    state and leave the solver on the fastest.  Returns
+    state and leave the solver on the fastest, preferring the serial
+    schedule unless an overlapped one is faster by more than SERIAL_MARGIN
+    (within timing noise the kernels then run alone, so their own durations
+    are what a profile of the step shows).  Returns
     (best, {choice: seconds_per_step}); the state advances by
     (steps + 1) steps per choice."""
     timings = {}
@@ -49,6 +55,11 @@ def tune(solver, steps=3, bands=BANDS, schedules=SCHEDULES, occs=(2, 3)):
         _time_steps(solver, 1)  # settle after a switch
         timings[(band, sched, occ)] = _time_steps(solver, steps)
     best = min(timings, key=timings.get)
+    serial = [k for k in timings if k[1] == "serial"]
+    if serial and best[1] != "serial":
+        best_serial = min(serial, key=timings.get)
+        if timings[best_serial] <= timings[best] * (1.0 + SERIAL_MARGIN):
+            best = best_serial
     solver.set_band(best[0])
     apply_schedule(solver, best[1])
     solver.set_face_occupancy(best[2])

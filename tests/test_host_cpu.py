@@ -63,6 +63,20 @@ def test_tune_tries_every_choice_and_keeps_the_fastest(clock):
     assert tms[(32, "overlap", 2)] == pytest.approx(1.0)
 
 
+def test_tune_prefers_serial_within_the_margin(clock):
+    """An overlapped schedule less than SERIAL_MARGIN faster than the best
+    serial one does not win; one more than that faster does."""
+    for gain, want in ((0.5 * autotune.SERIAL_MARGIN, "serial"), (2.0 * autotune.SERIAL_MARGIN, "overlap")):
+        def cost(band, overlap, occ, gain=gain):
+            return (1.0 + abs(band - 32) / 100) * ((1.0 - gain) if overlap else 1.0)
+
+        s = FakeSolver(cost)
+        clock["obj"] = s
+        best, _ = autotune.tune(s, steps=3, occs=(2,))
+        assert best[0] == 32 and best[1] == want, (gain, best)
+        assert s.overlap == (want != "serial")
+
+
 def test_apply_schedule_rejects_unknown_names():
     with pytest.raises(ValueError):
         autotune.apply_schedule(FakeSolver(lambda *a: 1.0), "sideways")
