@@ -111,8 +111,9 @@ __global__ __launch_bounds__(TPB, D == 0 ? 4 : 3) void kb_face_fast(const BlockD
 
 // FROMQ: the tolerance-bound solves' march, which forms u, v, w, T from
 // the stage input state `which` (they store no primitives)
+// float storage (FP32-flux) fits 3 waves per SIMD, as the single-solve march
 template <typename VT, bool FROMQ = false>
-__global__ __launch_bounds__(TPB) void kb_visc(const BlockDesc* __restrict__ blocks,
+__global__ __launch_bounds__(TPB, sizeof(VT) == 4 ? 3 : 2) void kb_visc(const BlockDesc* __restrict__ blocks,
                                                const InstDesc* __restrict__ insts, TileTable tt, int jb,
                                                int which, int* ctr) {
   // visc_tile_body synchronises the workgroup: every condition before it is
