@@ -274,7 +274,9 @@ def serve_leg(n):
     offered load 85% of the capacity measured on a probe of the same mix,
     three size-class lanes; against the same requests solved one at a time as
     bit-exact plain FP64, measured request by request and from the per-size
-    cost model that the full 1000-request run uses."""
+    cost model that the full 1000-request run uses.  The probe is also served
+    in each arithmetic form alone (FP32-flux only, and FP64 only in each
+    form): the server's capacity per form."""
     from cbns3d_cart_mpi_amd import serve as S
 
     import numpy as np
@@ -283,6 +285,12 @@ def serve_leg(n):
     w = [1.0 / g for g in grids]
     probe = S.make_requests(32, 7, grids, w, (50, 50))
     cap = S.run_sweep_lanes(probe, fp64_form=S.C.CBNS_FP64_FAST, lanes=S.LANES)["mcell_steps_per_s"] * 1e6
+    # the same probe served in one arithmetic form at a time
+    cap_form = {}
+    for nm, prec, form in (("fp32_flux", "fp32", S.C.CBNS_FP64_FAST), ("fp64_tolerance_bound", "fp64", S.C.CBNS_FP64_FAST),
+                           ("fp64_bit_exact", "fp64", S.C.CBNS_FP64)):
+        pr = S.make_requests(32, 7, grids, w, (50, 50), precision=prec)
+        cap_form[nm] = round(S.run_sweep_lanes(pr, fp64_form=form, lanes=S.LANES)["mcell_steps_per_s"], 1)
     mean_work = np.mean([r.work for r in S.make_requests(n, 2024, grids, w, (50, 200))])
     rate = 0.85 * cap / mean_work
     reqs = S.make_requests(n, 2024, grids, w, (50, 200), rate=rate)
@@ -291,6 +299,8 @@ def serve_leg(n):
     model = S.sequential_model(reqs)
     rnd = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
     return {"requests": n, "rate_per_s": round(rate, 3), "capacity_probe_mcell_steps_per_s": round(cap / 1e6, 1),
+            "capacity_probe_by_form_mcell_steps_per_s": cap_form,
+            "fp32_flux_capacity_over_sequential": round(cap_form["fp32_flux"] / seq["mcell_steps_per_s"], 3),
             "served": rnd(srv), "sequential_fp64": rnd(seq), "sequential_fp64_model": rnd(model),
             "served_over_sequential": round(srv["mcell_steps_per_s"] / seq["mcell_steps_per_s"], 3),
             "config": "reduced: steps U[50,200] instead of U[200,2000]; 80 instead of 1000 requests "

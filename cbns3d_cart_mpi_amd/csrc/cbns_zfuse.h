@@ -43,7 +43,7 @@ struct ZOperands {
 };
 
 template <typename FT>
-__device__ __forceinline__ void zload(const BlockDesc& B, int i, int j, int k, ZOperands<FT>& o) {
+__device__ __forceinline__ void zload(const BlockDesc& B, int i, int j, int k, ZOperands<FT>& o, int stage = 2) {
   const int n0 = B.n[0], n1 = B.n[1];
   const int iE = i + (n0 + 1) * (j + n1 * k), iF = i + n0 * (j + (n1 + 1) * k), ct = i + n0 * (j + n1 * k);
   gptr<const FT> __restrict__ E = gmem<const FT>((const FT*)B.F[0]);
@@ -51,13 +51,16 @@ __device__ __forceinline__ void zload(const BlockDesc& B, int i, int j, int k, Z
   gptr<const FT> __restrict__ DV = gmem<const FT>((const FT*)B.dfv);
   gptr<const double> __restrict__ Qn = gmem<const double>(B.Q[0]);
   const int c = gidx(B, i, j, k);
+  // stage 1 takes Q^n from the stage input: its loads point all lanes at one line
+  const long long qs = stage > 1 ? B.sg : 0;
+  const int qc = stage > 1 ? c : 0;
 #pragma unroll
   for (int e = 0; e < 5; e++) {
     o.e0[e] = E[e * B.sf[0] + iE];
     o.e1[e] = E[e * B.sf[0] + iE + 1];
     o.f0[e] = F[e * B.sf[1] + iF];
     o.f1[e] = F[e * B.sf[1] + iF + n0];
-    o.qn[e] = Qn[e * B.sg + c];  // used by stages 2 and 3 (loaded always: no branch among the loads)
+    o.qn[e] = Qn[e * qs + qc];  // used by stages 2 and 3 (loaded always: no branch among the loads)
   }
 #pragma unroll
   for (int e = 0; e < 4; e++) o.dv[e] = DV[e * B.sn + ct];
@@ -164,7 +167,7 @@ __device__ __forceinline__ void zrun(const BlockDesc& B, int b, int i, int j, in
     // both of its cells at each face.
     constexpr bool early = FORM != ZF_FP64;
     ZOperands<FT> o;
-    if (early) zload<FT>(B, i, j, k > k0 ? k - 1 : k0, o);
+    if (early) zload<FT>(B, i, j, k > k0 ? k - 1 : k0, o, stage);
     double nx[5];
     if (early) {
 #pragma unroll
@@ -190,7 +193,7 @@ __device__ __forceinline__ void zrun(const BlockDesc& B, int b, int i, int j, in
       for (int f = 0; f < 5; f++) pl[f] = pr[f];
     }
     if (k > k0) {  // cell k - 1 is q[2]
-      if (!early && ok && okp) zload<FT>(B, i, j, k - 1, o);
+      if (!early && ok && okp) zload<FT>(B, i, j, k - 1, o, stage);
       if (ok && okp) zapply<FORM>(B, i, j, k - 1, o, gp, g, q[2], out, stage, dt, ph, want, s, bad);
       else {
         const unsigned slot = atomicAdd(cfx.count, 1u);
@@ -268,7 +271,7 @@ __device__ __forceinline__ void zcell_fix(const BlockDesc& B, int i, int j, int 
   const bool ok1 = zface_checked<FORM>(qb, ph, true, g1);
   if (all && ok0 && ok1) return;
   ZOperands<double> o;
-  zload<double>(B, i, j, k, o);
+  zload<double>(B, i, j, k, o, stage);
   zapply<FORM>(B, i, j, k, o, g0, g1, q7[3], out, stage, dt, ph, want_s, s, bad);
 }
 
