@@ -433,11 +433,14 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
   if (!b || !p || !dims || !id_out || steps < 0) return CBNS_ERR_ARG;
   if (precision != CBNS_FP64 && precision != CBNS_FP32_FLUX && precision != CBNS_FP64_FAST) return CBNS_ERR_ARG;
   const int N[3] = {(int)p->IM, (int)p->JM, (int)p->KM};
+  int largest[3];
   for (int a = 0; a < 3; a++) {
     if (dims[a] < 1 || N[a] < 1) return CBNS_ERR_ARG;
     const int base = N[a] / dims[a];
     if (base < NG || N[a] - (dims[a] - 1) * base < NG) return CBNS_ERR_ARG;
+    largest[a] = N[a] - (dims[a] - 1) * base;
   }
+  if (!block_fits(largest[0], largest[1], largest[2])) return CBNS_ERR_ARG;  // split into more blocks
   const uint64_t need = solve_bytes(p, dims);
   if (b->used + need > b->budget) return CBNS_ERR_NOMEM;
   int id = -1;

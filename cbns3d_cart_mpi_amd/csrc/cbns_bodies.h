@@ -372,20 +372,38 @@ __device__ __forceinline__ void prim_uvwT(const double (&q)[5], const Phys& ph, 
   o[3] = (ph.gM2 * p) * ir;
 }
 
-// Tile march.  A workgroup owns a VX x VY tile of interior columns and
-// marches the planes p of one z-chunk [Z0, Z1), from Z0 - 2 to Z1 + 1.  Per
-// plane it
+// Loads and stores at 32-bit byte offsets from a per-field base: the base
+// is uniform (scalar registers) and the offset one VGPR, so an access needs
+// no 64-bit address arithmetic.  Every field of a block is below 4 GB
+// (MAX_BLOCK_CELLS).
+template <typename T>
+__device__ __forceinline__ T gld(gptr<const T> base, unsigned idx) {
+  return *(gptr<const T>)((gptr<const char>)base + idx * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void gst(gptr<T> base, unsigned idx, T v) {
+  *(gptr<T>)((gptr<char>)base + idx * (unsigned)sizeof(T)) = v;
+}
+
+// Tile march.  A workgroup owns a VX x VY (16 x 16) tile of interior columns
+// and marches the planes p of one z-chunk [Z0, Z1), from Z0 - 2 to Z1 + 1.
+// Each thread keeps a five-plane register queue (p - 2 .. p + 2) of u, v, w,
+// T of its own column.  Per plane the workgroup
 //  - stages u, v, w, T of the (VX + 8) x (VY + 8) region around the tile in
-//    LDS (double-buffered) for the x- and y-gradients;
+//    LDS (double-buffered) for the x- and y-gradients: each thread writes
+//    its own column's plane-p value from its queue, and only the 320-point
+//    ring around the tile is loaded (one ring point per thread, a second
+//    for half of waves 2 and 3, the waves without side points);
 //  - evaluates the fluxes at each thread's own column point, with the
-//    z-gradient from the thread's five-plane register queue; the z-fluxes
-//    Gv = (txz, tyz, tzz, ez) enter a five-plane queue of their own;
-//  - on the chunk's planes, also evaluates Ev = (txx, txy, txz, ex) at the
-//    two columns beside each x-side of the tile and Fv = (txy, tyy, tyz, ey)
-//    at the two rows beside each y-side (one such side point for each of the
-//    first 4 (VX + VY) threads, with a queue of the two velocities whose
-//    z-gradients it needs), and puts Ev and Fv of the tile and its sides in
-//    LDS for the x- and y-parts of the divergence;
+//    z-gradient from its queue; the z-fluxes Gv = (txz, tyz, tzz, ez) enter a
+//    five-plane queue of their own;
+//  - on the chunk's planes, wave 0 also evaluates Ev = (txx, txy, txz, ex) at
+//    the two columns beside each x-side of the tile (16 rows x 4 columns)
+//    and wave 1 Fv = (txy, tyy, tyz, ey) at the two rows beside each y-side
+//    (4 rows x 16 columns): only the six x/y-gradients and the flux
+//    components that side's divergence takes, with a queue of the two
+//    velocities whose z-gradients it needs (loaded by these two waves alone);
+//    Ev and Fv of the tile and its sides go to LDS;
 //  - forms the x- and y-parts of the divergence of plane p - 1 from the Ev
 //    and Fv that the previous plane left in LDS (Ev and Fv are
 //    double-buffered, so that one barrier per plane suffices), and completes
@@ -399,6 +417,7 @@ __device__ __forceinline__ void prim_uvwT(const double (&q)[5], const Phys& ph, 
 template <typename VT, bool FROMQ = false>
 __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& ph, int tile, int chunk, int which,
                                                int vz) {
+  static_assert(VX == 16 && VY == 16, "one wave of side points per side kind");
   static_assert(!FROMQ || sizeof(VT) == 8, "the state is FP64");
   const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
   const int ntx = (n0 + VX - 1) / VX, nty = (n1 + VY - 1) / VY;
@@ -407,24 +426,27 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
   const int Z1 = Z0 + vz < n2 ? Z0 + vz : n2;
   const int X0 = (tile % ntx) * VX, Y0 = (tile / ntx) * VY;
   const int wx = n0 - X0 < VX ? n0 - X0 : VX, wy = n1 - Y0 < VY ? n1 - Y0 : VY;  // the tile's extent
-  const int tx = threadIdx.x % VX, ty = threadIdx.x / VX;
+  const int tid = threadIdx.x, tx = tid % VX, ty = tid / VX;
   const bool act = tx < wx && ty < wy;
-  // side point of this thread (offsets from X0, Y0): kind 1 = Ev at an x-side
-  // column, 2 = Fv at a y-side row, 0 = none
-  int sk = 0, sx = 0, sy = 0;
-  if (threadIdx.x < 4 * VY) {
-    const int r = threadIdx.x >> 2, s = threadIdx.x & 3;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  // side point of this thread (offsets from X0, Y0): wave 0 Ev at an x-side
+  // column, wave 1 Fv at a y-side row; sk: the point exists
+  int sx = 0, sy = 0;
+  bool sk = false;
+  if (wave == 0) {
+    const int r = lane >> 2, s = lane & 3;
     sx = s < 2 ? s - 2 : wx + s - 2;
     sy = r;
-    sk = r < wy ? 1 : 0;
-  } else if (threadIdx.x < 4 * VY + 4 * VX) {
-    const int h = threadIdx.x - 4 * VY, c = h % VX, s = h / VX;
+    sk = r < wy;
+  } else if (wave == 1) {
+    const int c = lane & 15, s = lane >> 4;
     sx = c;
     sy = s < 2 ? s - 2 : wy + s - 2;
-    sk = c < wx ? 2 : 0;
+    sk = c < wx;
   }
-  constexpr int SX = VX + 8, SY = VY + 8, NT = VX * VY, NP = (SX * SY + NT - 1) / NT;
-  __shared__ VT L[2][4][SY][SX];    // u, v, w, T of the staged region, in the storage type
+  const bool sidew = wave < 2;  // uniform
+  constexpr int SX = VX + 8, SY = VY + 8, NRING = SX * SY - VX * VY;
+  __shared__ VT L[2][4][SY][SX];       // u, v, w, T of the staged region, in the storage type
   __shared__ VT LE[2][4][VY][VX + 4];  // Ev of the tile and its x-sides
   __shared__ VT LF[2][4][VY + 4][VX];  // Fv of the tile and its y-sides
   const int sz = B.g[0] * B.g[1];
@@ -432,9 +454,10 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
   // source fields: u, v, w, T of the primitives (sharing VT), or with FROMQ
   // rho, rho u, rho v, rho w, E of the state
   constexpr int NF = FROMQ ? 5 : 4;
-  gptr<const VT> __restrict__ P = gmem<const VT>(FROMQ ? (const VT*)B.Q[which] : (const VT*)B.prim);
-  const long long foff[5] = {FROMQ ? 0 : sg, FROMQ ? sg : 2 * sg, FROMQ ? 2 * sg : 3 * sg, FROMQ ? 3 * sg : 5 * sg,
-                             4 * sg};
+  const VT* src = FROMQ ? (const VT*)B.Q[which] : (const VT*)B.prim;
+  gptr<const VT> Pf[NF];
+#pragma unroll
+  for (int f = 0; f < NF; f++) Pf[f] = gmem<const VT>(src + (FROMQ ? f * sg : (f < 3 ? f + 1 : 5) * sg));
   // the values of one point as u, v, w, T
   const auto uvwT = [&](const VT (&x)[NF], double (&o)[4]) {
     if constexpr (FROMQ) {
@@ -445,46 +468,88 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
       for (int f = 0; f < 4; f++) o[f] = (double)x[f];
     }
   };
-  // Staged entry e = threadIdx.x + q NT (e < SX SY) is the point (X0 - 4 +
-  // e % SX, Y0 - 4 + e / SX).  Positions past the block's last ghost cell
-  // are clamped to it: no used value comes from them (the farthest point a
-  // side point's stencil reaches is (X0 + wx + 3, Y0 + wy + 3)), and every
-  // load stays inside the ghosted block.
-  int fidx[NP];
+  // Positions past the block's last ghost cell are clamped to it: no used
+  // value comes from them (the farthest point a side point's stencil reaches
+  // is (X0 + wx + 3, Y0 + wy + 3)), and every load stays inside the ghosted
+  // block.
+  const auto clampx = [&](int c) { return c < n0 + NG - 1 ? c : n0 + NG - 1; };
+  const auto clampy = [&](int c) { return c < n1 + NG - 1 ? c : n1 + NG - 1; };
+  // ring entries of this thread: LDS offset in a field plane (-1: none) and
+  // column index in the ghosted block.  Entry e < 320: rows 0-3, rows
+  // VY+4 .. VY+7, then columns 0-3 and VX+4 .. VX+7 of rows 4 .. VY+3.
+  int rl[2];
+  unsigned rg[2];
 #pragma unroll
-  for (int q = 0; q < NP; q++) {
-    const int e = threadIdx.x + q * NT;
-    const int cx = X0 - 4 + e % SX, cy = Y0 - 4 + e / SX;
-    fidx[q] = gidx(B, cx < n0 + NG - 1 ? cx : n0 + NG - 1, cy < n1 + NG - 1 ? cy : n1 + NG - 1, 0);
+  for (int q = 0; q < 2; q++) {
+    int e = q == 0 ? tid : (wave >= 2 && lane < 32 ? VX * VY + (wave - 2) * 32 + lane : -1);
+    if (e >= NRING) e = -1;
+    int lx = 0, ly = 0;
+    if (e >= 0) {
+      if (e < 4 * SX) {
+        ly = e / SX;
+        lx = e % SX;
+      } else if (e < 8 * SX) {
+        const int f = e - 4 * SX;
+        ly = VY + 4 + f / SX;
+        lx = f % SX;
+      } else {
+        const int f = e - 8 * SX, side = f / (4 * VY), g = f - side * 4 * VY;
+        ly = 4 + g / 4;
+        lx = (g & 3) + side * (VX + 4);
+      }
+    }
+    rl[q] = e >= 0 ? ly * SX + lx : -1;
+    rg[q] = (unsigned)gidx(B, clampx(X0 - 4 + lx), clampy(Y0 - 4 + ly), 0);
   }
-  // own column (inactive threads: the tile's last column, loaded and unused)
-  // and side column (threads without a side point: the tile's first column)
-  const int ix = X0 + (tx < wx ? tx : wx - 1), iy = Y0 + (ty < wy ? ty : wy - 1);
-  const int ci = gidx(B, ix, iy, 0);
-  const int cs = sk ? gidx(B, X0 + sx, Y0 + sy, 0) : gidx(B, X0, Y0, 0);
-  const int sa = sk == 2 ? 1 : 0;  // the side point's own velocity: u (x-side) or v (y-side); the other is w
+  // own column (clamped like the staging; the tile's own column for act threads)
+  const unsigned ci = (unsigned)gidx(B, clampx(X0 + tx), clampy(Y0 + ty), 0);
+  const int own_l = (ty + 4) * SX + tx + 4;
+  // side column and the fields its z-gradients take: with FROMQ rho, rho a,
+  // rho w, else a, w (a = u beside an x-side, v beside a y-side)
+  const unsigned cs = sk ? (unsigned)gidx(B, X0 + sx, Y0 + sy, 0) : ci;
+  constexpr int NH = FROMQ ? 3 : 2;
+  gptr<const VT> Ph[NH];
+  if constexpr (FROMQ) {
+    Ph[0] = Pf[0];
+    Ph[1] = Pf[wave == 1 ? 2 : 1];
+    Ph[2] = Pf[3];
+  } else {
+    Ph[0] = Pf[wave == 1 ? 1 : 0];
+    Ph[1] = Pf[2];
+  }
+  // the side point's two velocities (prim_uvwT's values with FROMQ)
+  const auto side_vel = [&](const VT (&h)[NH], VT& va, VT& vw) {
+    if constexpr (FROMQ) {
+      const double ir = rcp_nr((double)h[0]);
+      va = (VT)((double)h[1] * ir);
+      vw = (VT)((double)h[2] * ir);
+    } else {
+      va = h[0];
+      vw = h[1];
+    }
+  };
   // queues in the staging type (every value is exact in it: VT is the
   // primitives' storage type, and FROMQ forms double values)
   VT r[4][5];   // u, v, w, T of the own column, planes p - 2 .. p + 2
   VT sq[2][5];  // the side point's two velocities, planes p - 2 .. p + 2
-  VT gq[4][5];      // Gv of the own column, planes p - 4 .. p
-  double dq[4];     // x- plus y-part of the divergence of plane p - 2
-  VT pf[NF][NP], rn[NF], hn[NF];  // next-plane prefetch (raw source values)
+  VT gq[4][5];  // Gv of the own column, planes p - 4 .. p
+  double dq[4]; // x- plus y-part of the divergence of plane p - 2
+  VT pf[2][NF], rn[NF], hn[NH];  // next-plane prefetch (raw source values)
 #pragma unroll
   for (int d = 0; d < 4; d++) {
-    VT x[NF], y[NF];
+    VT x[NF];
     double o[4];
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
-      x[f] = P[foff[f] + ci + (Z0 - 4 + d) * sz];
-      y[f] = P[foff[f] + cs + (Z0 - 4 + d) * sz];
-    }
+    for (int f = 0; f < NF; f++) x[f] = gld(Pf[f], ci + (unsigned)((Z0 - 4 + d) * sz));
     uvwT(x, o);
 #pragma unroll
     for (int f = 0; f < 4; f++) r[f][d] = (VT)o[f];
-    uvwT(y, o);
-    sq[0][d] = (VT)(sa ? o[1] : o[0]);
-    sq[1][d] = (VT)o[2];
+    if (sidew) {
+      VT h[NH];
+#pragma unroll
+      for (int f = 0; f < NH; f++) h[f] = gld(Ph[f], cs + (unsigned)((Z0 - 4 + d) * sz));
+      side_vel(h, sq[0][d], sq[1][d]);
+    }
   }
 #pragma unroll
   for (int e = 0; e < 4; e++) {
@@ -494,30 +559,35 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
   }
 #pragma unroll
   for (int f = 0; f < NF; f++) {
-    rn[f] = P[foff[f] + ci + Z0 * sz];
-    hn[f] = P[foff[f] + cs + Z0 * sz];
+    rn[f] = gld(Pf[f], ci + (unsigned)(Z0 * sz));
 #pragma unroll
-    for (int q = 0; q < NP; q++) pf[f][q] = P[foff[f] + fidx[q] + (Z0 - 2) * sz];
+    for (int q = 0; q < 2; q++)
+      if (rl[q] >= 0) pf[q][f] = gld(Pf[f], rg[q] + (unsigned)((Z0 - 2) * sz));
   }
-  gptr<VT> __restrict__ DV = gmem((VT*)B.dfv);
-  const long long sn = B.sn;
+  if (sidew) {
+#pragma unroll
+    for (int f = 0; f < NH; f++) hn[f] = gld(Ph[f], cs + (unsigned)(Z0 * sz));
+  }
+  gptr<VT> DV[4];
+#pragma unroll
+  for (int e = 0; e < 4; e++) DV[e] = gmem((VT*)B.dfv + e * B.sn);
+  const gptr<double> MU = gmem(B.mu);
+  const unsigned oi = (unsigned)(X0 + tx + n0 * (Y0 + ty));  // interior offset of the own column (act)
+  const unsigned pn = (unsigned)(n0 * n1);
   // Ev = (txx, txy, txz, ex), Fv = (txy, tyy, tyz, ey), Gv = (txz, tyz, tzz, ez)
   const int Ef[4] = {0, 3, 4, 6}, Ff[4] = {3, 1, 5, 7}, Gf[4] = {4, 5, 2, 8};
   const double c0 = 1.0 / 12.0, c1 = -2.0 / 3.0;
   for (int p = Z0 - 2; p < Z1 + 2; p++) {
     const int buf = (p - Z0) & 1;
-    // stage plane p; the column queues take plane p + 2
+    // stage plane p: ring entries from the prefetch, the tile from the own queue;
+    // the queues take plane p + 2
 #pragma unroll
-    for (int q = 0; q < NP; q++) {
-      const int e = threadIdx.x + q * NT;
-      VT x[NF];
-      double o[4];
+    for (int q = 0; q < 2; q++) {
+      if (rl[q] >= 0) {
+        double o[4];
+        uvwT(pf[q], o);
 #pragma unroll
-      for (int f = 0; f < NF; f++) x[f] = pf[f][q];
-      uvwT(x, o);
-      if (e < SX * SY) {
-#pragma unroll
-        for (int f = 0; f < 4; f++) (&L[buf][f][0][0])[e] = (VT)o[f];
+        for (int f = 0; f < 4; f++) (&L[buf][f][0][0])[rl[q]] = (VT)o[f];
       }
     }
     {
@@ -525,18 +595,22 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
       uvwT(rn, o);
 #pragma unroll
       for (int f = 0; f < 4; f++) r[f][4] = (VT)o[f];
-      uvwT(hn, o);
-      sq[0][4] = (VT)(sa ? o[1] : o[0]);
-      sq[1][4] = (VT)o[2];
     }
+#pragma unroll
+    for (int f = 0; f < 4; f++) (&L[buf][f][0][0])[own_l] = r[f][2];
+    if (sidew) side_vel(hn, sq[0][4], sq[1][4]);
     __syncthreads();
     if (p + 1 < Z1 + 2) {  // planes p + 1 (staging) and p + 3 (queues): at most n2 + 3, the last ghost plane
 #pragma unroll
       for (int f = 0; f < NF; f++) {
 #pragma unroll
-        for (int q = 0; q < NP; q++) pf[f][q] = P[foff[f] + fidx[q] + (p + 1) * sz];
-        rn[f] = P[foff[f] + ci + (p + 3) * sz];
-        hn[f] = P[foff[f] + cs + (p + 3) * sz];
+        for (int q = 0; q < 2; q++)
+          if (rl[q] >= 0) pf[q][f] = gld(Pf[f], rg[q] + (unsigned)((p + 1) * sz));
+        rn[f] = gld(Pf[f], ci + (unsigned)((p + 3) * sz));
+      }
+      if (sidew) {
+#pragma unroll
+        for (int f = 0; f < NH; f++) hn[f] = gld(Ph[f], cs + (unsigned)((p + 3) * sz));
       }
     }
     // x- and y-parts of the divergence of plane p - 1 (its Ev, Fv are in LDS)
@@ -568,7 +642,7 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
 #pragma unroll
       for (int e = 0; e < 4; e++) gq[e][4] = (VT)o[Gf[e]];
       if (out) {
-        gmem(B.mu)[ix + n0 * (iy + n1 * p)] = mu_l;
+        gst(MU, oi + pn * (unsigned)p, mu_l);
 #pragma unroll
         for (int e = 0; e < 4; e++) {
           LE[buf][e][ty][tx + 2] = (VT)o[Ef[e]];
@@ -576,39 +650,52 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
         }
       }
     }
-    if (out && sk) {
+    if (out && sidew && sk) {
+      // visc_terms' expressions for the components this side keeps; the
+      // z-gradients are a_z (u_z beside an x-side, v_z beside a y-side) and w_z
       const int lx = sx + 4, ly = sy + 4;
-      double gr[4][3];
-#pragma unroll
-      for (int f = 0; f < 4; f++) {
-        gr[f][0] = grad4(&L[buf][f][ly][0], lx, 1, ph.dx, ph.rdx, ph.fdx);
-        gr[f][1] = grad4(&L[buf][f][0][lx], ly * SX, SX, ph.dy, ph.rdy, ph.fdy);
-      }
-      // z-gradients: the side's own velocity and w; the others enter no
-      // component it keeps (Ev takes u_z, w_z; Fv takes v_z, w_z)
-      const double za = grad4(&sq[0][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
-      gr[0][2] = sa ? 0.0 : za;
-      gr[1][2] = sa ? za : 0.0;
-      gr[2][2] = grad4(&sq[1][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
-      gr[3][2] = 0.0;
-      double o[9], mu_s;
-      visc_terms(ph, gr, (double)L[buf][3][ly][lx], (double)L[buf][0][ly][lx], (double)L[buf][1][ly][lx],
-                 (double)L[buf][2][ly][lx], o, mu_s);
-      if (sk == 1) {
-#pragma unroll
-        for (int e = 0; e < 4; e++) LE[buf][e][sy][sx + 2] = (VT)o[Ef[e]];
-      } else {
-#pragma unroll
-        for (int e = 0; e < 4; e++) LF[buf][e][sy + 2][sx] = (VT)o[Ff[e]];
+      const double T_l = (double)L[buf][3][ly][lx], uu = (double)L[buf][0][ly][lx],
+                   vv = (double)L[buf][1][ly][lx], ww = (double)L[buf][2][ly][lx];
+      const double u_x = grad4(&L[buf][0][ly][0], lx, 1, ph.dx, ph.rdx, ph.fdx);
+      const double u_y = grad4(&L[buf][0][0][lx], ly * SX, SX, ph.dy, ph.rdy, ph.fdy);
+      const double v_x = grad4(&L[buf][1][ly][0], lx, 1, ph.dx, ph.rdx, ph.fdx);
+      const double v_y = grad4(&L[buf][1][0][lx], ly * SX, SX, ph.dy, ph.rdy, ph.fdy);
+      const double a_z = grad4(&sq[0][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
+      const double w_z = grad4(&sq[1][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
+      double mu_l = (1.0 + ph.C_T_inf) / (T_l + ph.C_T_inf) * sqrt(T_l * T_l * T_l);
+      mu_l *= ph.Re_inv;
+      const double txy = mu_l * (u_y + v_x);
+      if (wave == 0) {  // Ev = (txx, txy, txz, ex)
+        const double w_x = grad4(&L[buf][2][ly][0], lx, 1, ph.dx, ph.rdx, ph.fdx);
+        const double T_x = grad4(&L[buf][3][ly][0], lx, 1, ph.dx, ph.rdx, ph.fdx);
+        const double txx = (2.0 / 3.0) * mu_l * (2.0 * u_x - v_y - w_z);
+        const double txz = mu_l * (a_z + w_x);
+        mu_l *= ph.kappa;
+        const double qx = mu_l * T_x;
+        LE[buf][0][sy][sx + 2] = (VT)txx;
+        LE[buf][1][sy][sx + 2] = (VT)txy;
+        LE[buf][2][sy][sx + 2] = (VT)txz;
+        LE[buf][3][sy][sx + 2] = (VT)(uu * txx + vv * txy + ww * txz - qx);
+      } else {  // Fv = (txy, tyy, tyz, ey)
+        const double w_y = grad4(&L[buf][2][0][lx], ly * SX, SX, ph.dy, ph.rdy, ph.fdy);
+        const double T_y = grad4(&L[buf][3][0][lx], ly * SX, SX, ph.dy, ph.rdy, ph.fdy);
+        const double tyy = (2.0 / 3.0) * mu_l * (2.0 * v_y - u_x - w_z);
+        const double tyz = mu_l * (a_z + w_y);
+        mu_l *= ph.kappa;
+        const double qy = mu_l * T_y;
+        LF[buf][0][sy + 2][sx] = (VT)txy;
+        LF[buf][1][sy + 2][sx] = (VT)tyy;
+        LF[buf][2][sy + 2][sx] = (VT)tyz;
+        LF[buf][3][sy + 2][sx] = (VT)(uu * txy + vv * tyy + ww * tyz - qy);
       }
     }
     // the divergence of plane p - 2 (< Z1) is complete
     if (act && p - 2 >= Z0) {
-      const long long o = ix + (long long)n0 * (iy + (long long)n1 * (p - 2));
+      const unsigned o = oi + pn * (unsigned)(p - 2);
 #pragma unroll
       for (int e = 0; e < 4; e++) {
         const double dz = c0 * gq[e][0] + c1 * gq[e][1] - c1 * gq[e][3] - c0 * gq[e][4];
-        DV[e * sn + o] = (VT)(dq[e] + divc(dz, ph.dz, ph.rdz, ph.fdz));
+        gst(DV[e], o, (VT)(dq[e] + divc(dz, ph.dz, ph.rdz, ph.fdz)));
       }
     }
 #pragma unroll

@@ -195,12 +195,15 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
   *out = nullptr;
   if ((precision < CBNS_FP64 || precision > CBNS_FP64_FAST)) return CBNS_ERR_ARG;
   const int N[3] = {(int)p->IM, (int)p->JM, (int)p->KM};
+  int largest[3];
   for (int a = 0; a < 3; a++) {
     if (dims[a] < 1 || N[a] < 1) return CBNS_ERR_ARG;
     const int base = N[a] / dims[a];
     const int last = N[a] - (dims[a] - 1) * base;
     if (base < NG || last < NG) return CBNS_ERR_ARG;  // halo must fit in one neighbour
+    largest[a] = last;
   }
+  if (!block_fits(largest[0], largest[1], largest[2])) return CBNS_ERR_ARG;  // split into more blocks
   const int nb = dims[0] * dims[1] * dims[2];
   if (nb > MAXB) return CBNS_ERR_ARG;
   int dev = 0;

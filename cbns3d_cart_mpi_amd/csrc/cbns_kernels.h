@@ -12,6 +12,15 @@ namespace cbns {
 constexpr int NEQ = 5;
 constexpr int NG = 4;   // ghost layers
 constexpr int MAXB = 512;
+// Most ghosted cells of one block: every field of a block then stays below
+// 4 GB, so kernels address it at 32-bit byte offsets from the field's base.
+// Larger grids run as several on-device blocks (the decomposition is free
+// and bit-exact).
+constexpr long long MAX_BLOCK_CELLS = 1LL << 29;
+// whether a block of n interior cells per axis fits that bound
+inline bool block_fits(int n0, int n1, int n2) {
+  return (long long)(n0 + 2 * NG) * (n1 + 2 * NG) * (long long)(n2 + 2 * NG) <= MAX_BLOCK_CELLS;
+}
 
 // Physical constants of one solve (src/read_input.cpp:78-90 and the static
 // caches of src/Block3d.cpp:563-564, 764-767).

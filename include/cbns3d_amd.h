@@ -99,7 +99,10 @@ int cbns_device_count(int* n);
  * splits it over MPI ranks (src/cbns3d.cpp:102-126: N/dims cells per block, the
  * last block takes the remainder).  Blocks are numbered like the reference's
  * Cartesian ranks (row-major, z fastest).  Every block needs >= 4 cells per
- * axis (the reference's halo depth).  No state is set yet. */
+ * axis (the reference's halo depth) and at most 2^29 cells with its ghost
+ * layers, (n0 + 8)(n1 + 8)(n2 + 8) <= 2^29 (about 800^3: each field of a
+ * block stays below 4 GB); a larger grid takes more blocks.  CBNS_ERR_ARG
+ * otherwise.  No state is set yet. */
 int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx** out);
 int cbns_destroy(cbns_ctx* ctx);
 
@@ -234,6 +237,7 @@ int cbns_batch_memory(const cbns_batch* b, uint64_t* used, uint64_t* budget);
 /* Admit a solve: Taylor-Green start when Q_aos is NULL (t from p->t_cur), else
  * the given interior AoS state at t_cur with mu := mu0; it runs `steps` steps
  * (or until t_end).  precision: CBNS_FP64, CBNS_FP32_FLUX or CBNS_FP64_FAST.
+ * Block sizes are bounded as in cbns_create (CBNS_ERR_ARG).
  * CBNS_ERR_NOMEM when the budget or capacity is full.  The arrays of removed
  * solves are kept for reuse by later solves of the same shapes (releasing
  * device memory synchronises the whole device); they are released when new

@@ -66,3 +66,18 @@ def test_create_rejects_blocks_thinner_than_the_halo():
     h = ctypes.c_void_p()
     d = (ctypes.c_int * 3)(5, 1, 1)  # 3-cell blocks
     assert C.lib().cbns_create(ctypes.byref(p), d, 0, ctypes.byref(h)) == -1
+
+
+def test_create_rejects_blocks_past_the_32_bit_field_bound():
+    """A block with more than 2^29 ghosted cells is refused before any device
+    work (its fields would pass 4 GB); the same grid over more blocks is not
+    refused for its size (without a GPU it then reports the missing device)."""
+    p = C.make_params((1024, 1024, 1024))
+    h = ctypes.c_void_p()
+    one = (ctypes.c_int * 3)(1, 1, 1)
+    assert C.lib().cbns_create(ctypes.byref(p), one, 0, ctypes.byref(h)) == -1
+    q = C.make_params((800, 800, 800))  # 808^3 < 2^29
+    if C.device_count() == 0:
+        assert C.lib().cbns_create(ctypes.byref(q), one, 0, ctypes.byref(h)) == -4
+        eight = (ctypes.c_int * 3)(2, 2, 2)
+        assert C.lib().cbns_create(ctypes.byref(p), eight, 0, ctypes.byref(h)) == -4
