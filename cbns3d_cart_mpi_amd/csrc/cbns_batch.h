@@ -54,6 +54,7 @@ constexpr int BATCH_COUNTERS = 64 * BATCH_CTR_PER_LAUNCH * BATCH_CTR_STRIDE;  //
 struct BatchGrids {
   int face[3][3];  // [form: FP64, FP32-flux, tolerance-bound FP64][direction]
   int visc[3], update[3], ghost, dt;  // [FP64, FP32-flux, tolerance-bound FP64]
+  int ymarch;  // the tolerance-bound solves' y-face march
   int cus;  // compute units of the device
 };
 hipError_t batch_grids(BatchGrids* g);
@@ -66,6 +67,11 @@ hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable
 // the fixup pass of the tolerance-bound solves' faces (cbns_kernels_fast.hip)
 hipError_t batch_launch_fixup(const BlockDesc* blocks, const InstDesc* insts, int nb, int which, const FixList& fx,
                               hipStream_t s);
+// The tolerance-bound solves' y faces: the march along y (yrun_fast) that
+// stores each cell's x- plus y-flux differences, one (column tile, y-run)
+// unit per table entry (after their x faces and the face fixup).
+hipError_t batch_launch_ymarch(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, int wgs, int* ctr,
+                               hipStream_t s);
 // precision: PREC_FP64, PREC_FP32_FLUX or PREC_FP64_FAST (whose march forms
 // u, v, w, T from the stage input state `which`)
 hipError_t batch_launch_visc(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int jb, int precision,

@@ -109,6 +109,28 @@ __global__ __launch_bounds__(TPB, D == 0 ? 4 : 3) void kb_face_fast(const BlockD
   });
 }
 
+// Tolerance-bound solves' y faces: the march along y combined with their x
+// faces (yrun_fast), one (column tile (i, k), y-run) unit of one block per
+// table entry, the run length chosen from the block's size as the table was
+// built.
+__global__ __launch_bounds__(TPB, 2) void kb_yface_xy(const BlockDesc* __restrict__ blocks,
+                                                      const InstDesc* __restrict__ insts, TileTable tt, int which,
+                                                      int* ctr) {
+  static_assert(TPB == ZX * ZY, "one column per thread");
+  for_tiles(tt.n, ctr, [&](int x) {
+    const TileRef tr = tt.tiles[x];
+    const BlockDesc& B = blocks[tr.block];
+    const InstDesc& I = insts[B.inst];
+    if (!I.stepping) return;
+    const int ntx = (B.n[0] + ZX - 1) / ZX, nt = ntx * ((B.n[2] + ZY - 1) / ZY);
+    const int ym = chunk_len(nt, B.n[1], ZM, ZM_LOW, BATCH_UNITS);
+    const int tile = tr.tile % nt, run = tr.tile / nt;
+    const int i = (tile % ntx) * ZX + (int)threadIdx.x % ZX, k = (tile / ntx) * ZY + (int)threadIdx.x / ZX;
+    const int j0 = run * ym;
+    if (i < B.n[0] && k < B.n[2] && j0 < B.n[1]) yrun_fast(B, i, k, j0, ym, which, I.ph);
+  });
+}
+
 // FROMQ: the tolerance-bound solves' march, which forms u, v, w, T from
 // the stage input state `which` (they store no primitives)
 // float storage (FP32-flux) fits 3 waves per SIMD, as the single-solve march
@@ -389,6 +411,17 @@ hipError_t batch_launch_zupdate(const BlockDesc* blocks, InstDesc* insts, TileTa
   return hipGetLastError();
 }
 
+hipError_t batch_launch_ymarch(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, int wgs, int* ctr,
+                               hipStream_t s) {
+  if (tt.n == 0) return hipSuccess;
+  if (wgs == 0) {  // direct launch
+    wgs = (tt.n + NXCD - 1) / NXCD * NXCD;
+    ctr = nullptr;
+  }
+  hipLaunchKernelGGL(kb_yface_xy, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, which, ctr);
+  return hipGetLastError();
+}
+
 hipError_t batch_launch_zfixup(const BlockDesc* blocks, InstDesc* insts, int nb, int in, int out, int stage,
                                const FixList& cfx, hipStream_t s) {
   hipLaunchKernelGGL(kb_zcell_fixup, dim3(ZFIX_WG), dim3(TPB), 0, s, blocks, insts, insts, nb, in, out, stage, cfx);
@@ -465,6 +498,7 @@ hipError_t batch_grids(BatchGrids* g) {
   res((const void*)kb_zface_update<ZF_FP64>, &g->update[0]);
   res((const void*)kb_zface_update<ZF_FP32>, &g->update[1]);
   res((const void*)kb_zface_update<ZF_FAST>, &g->update[2]);
+  res((const void*)kb_yface_xy, &g->ymarch);
   res((const void*)kb_ghost, &g->ghost);
   res((const void*)kb_dt, &g->dt);
   return e;

@@ -43,6 +43,7 @@ enum {
   T_FACE64 = 0, T_FACE32 = 3, T_FACEF = 6,
   T_VISC64 = 9, T_VISC32, T_VISCF,
   T_UPD64, T_UPD32, T_UPDF,
+  T_YMF,  // the tolerance-bound solves' y-face march
   T_GHOST, T_DT, T_COUNT
 };
 
@@ -238,8 +239,16 @@ static int rebuild(cbns_batch* b) {
         const int nt = (int)((count + 255) / 256);
         for (int t = 0; t < nt; t++) tab[cls].push_back({bi, t});
       };
-      // x and y sweeps (the z faces are formed by the fused stage tail)
-      for (int d = 0; d < 2; d++) add(FACE_CLASS[face_form(I.precision)] + d, face_items(B, d));
+      // x and y sweeps (the z faces are formed by the fused stage tail); the
+      // tolerance-bound solves march their y faces in (column tile, y-run)
+      // units, the run lengths as the kernel forms them
+      const bool fast = I.precision == CBNS_FP64_FAST;
+      for (int d = 0; d < (fast ? 1 : 2); d++) add(FACE_CLASS[face_form(I.precision)] + d, face_items(B, d));
+      if (fast) {
+        const int yt = ((B.n[0] + ZX - 1) / ZX) * ((B.n[2] + ZY - 1) / ZY);
+        const int ym = chunk_len(yt, B.n[1], ZM, ZM_LOW, BATCH_UNITS);
+        for (int t = 0; t < yt * ((B.n[1] + ym - 1) / ym); t++) tab[T_YMF].push_back({bi, t});
+      }
       // whole-workgroup units of the viscous march and of the fused stage
       // tail: (column tile, z-chunk), the chunk lengths as the kernels form them
       const int vt = ((B.n[0] + VX - 1) / VX) * ((B.n[1] + VY - 1) / VY);
@@ -290,7 +299,7 @@ static int rebuild(cbns_batch* b) {
   auto tail = [&]() { return b->wg_mem > 0 ? b->wg_mem : 0; };
   // every persistent launch of the step takes its own block of counters, all
   // zeroed by the first node of the graph
-  constexpr int LAUNCHES = 1 + 3 * 13;  // dt; per stage 3 viscous, 6 face, 3 fused z + update, 1 ghost
+  constexpr int LAUNCHES = 1 + 3 * 14;  // dt; per stage 3 viscous, 6 face, 1 y march, 3 fused z + update, 1 ghost
   constexpr int K = BATCH_CTR_PER_LAUNCH * BATCH_CTR_STRIDE;
   static_assert(LAUNCHES * K <= BATCH_COUNTERS, "counter block");
   int used = 0;
@@ -313,12 +322,18 @@ static int rebuild(cbns_batch* b) {
       e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[VISC_CLASS[f]], b->jb, FORM_PREC[f], in,
                             ov ? b->visc_side_wgs : G.visc[f], ctr(), sv);
     if (ov && e == hipSuccess) e = hipEventRecord(b->ev_join, sv);
-    for (int d = 0; d < 2 && e == hipSuccess; d++)
+    for (int d = 0; d < 2 && e == hipSuccess; d++) {
       for (int f = 0; f < 3 && e == hipSuccess; f++)
         e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[FACE_CLASS[f] + d], d, in, b->jb, f, G.face[f][d], ctr(),
                               b->fix, s);
-    // faces of tolerance-bound solves whose range checks failed
-    if (b->has_fast && e == hipSuccess) e = batch_launch_fixup(b->d_blocks, b->d_insts, (int)all.size(), in, b->fix, s);
+      if (d == 0 && b->has_fast && e == hipSuccess) {
+        // x faces of tolerance-bound solves whose range checks failed, then
+        // their y-face march (which reads the x faces)
+        e = batch_launch_fixup(b->d_blocks, b->d_insts, (int)all.size(), in, b->fix, s);
+        if (e == hipSuccess)
+          e = batch_launch_ymarch(b->d_blocks, b->d_insts, b->tt[T_YMF], in, tail(), ctr(), s);
+      }
+    }
     if (ov && e == hipSuccess) e = hipStreamWaitEvent(s, b->ev_join, 0);  // join before the stage tail
     for (int f = 0; f < 3 && e == hipSuccess; f++)
       e = batch_launch_zupdate(b->d_blocks, b->d_insts, b->tt[UPD_CLASS[f]], FORM_PREC[f], in, out, stage, tail(),

@@ -417,7 +417,8 @@ int cbns_rec_riemann(cbns_ctx* c, int which, int dir) {
   if (!c || (which != CBNS_Q && which != CBNS_QP) || dir < 0 || dir > 2) return CBNS_ERR_ARG;
   CK(launch_face_flux(c->db, c->hb.data(), c->dom.nb, dir, which, c->ph, c->precision, c->ref, c->jband, c->face_occ,
                       c->fix, c->stream));
-  if (c->precision == CBNS_FP64_FAST) CK(launch_face_fixup(c->db, c->hb.data(), c->dom.nb, which, c->ph, c->fix, c->stream));
+  if (c->precision == CBNS_FP64_FAST)
+    CK(launch_face_fixup(c->db, c->hb.data(), c->dom.nb, which, c->ph, c->fix, c->stream, 1 << dir));
   return CBNS_OK;
 }
 
@@ -626,14 +627,23 @@ static int launch_sweeps(cbns_ctx* c, int stage) {
     CK(hipEventRecord(c->ev_join, c->side));
   }
   // the z faces are formed by the fused stage tail
+  if (c->precision == CBNS_FP64_FAST) {
+    // x faces, the fixup of those whose range checks failed, then the y-face
+    // march that stores each cell's x- plus y-flux differences
+    CK(timed(c, 0, [&] {
+      return launch_face_flux(c->db, hb, c->dom.nb, 0, in, c->ph, c->precision, c->ref, c->jband, c->face_occ,
+                              c->fix, c->stream);
+    }));
+    CK(launch_face_fixup(c->db, hb, c->dom.nb, in, c->ph, c->fix, c->stream, 1));
+    CK(timed(c, 1, [&] { return launch_yface_xy(c->db, hb, c->dom.nb, in, c->ph, c->face_occ, c->stream); }));
+    return CBNS_OK;
+  }
   const int nd = zfused(c) ? 2 : 3;
   for (int d = 0; d < nd; d++)
     CK(timed(c, d, [&] {
       return launch_face_flux(c->db, hb, c->dom.nb, d, in, c->ph, c->precision, c->ref, c->jband, c->face_occ,
                               c->fix, c->stream);
     }));
-  // faces of the tolerance-bound form whose range checks failed (one pass for all its sweeps)
-  if (c->precision == CBNS_FP64_FAST) CK(launch_face_fixup(c->db, hb, c->dom.nb, in, c->ph, c->fix, c->stream));
   return CBNS_OK;
 }
 

@@ -107,9 +107,11 @@ constexpr size_t FIX_BYTES = 256 + sizeof(unsigned long long) * FIX_CAP;  // cou
 hipError_t launch_face_flux_fast(const BlockDesc* db, const BlockDesc* hb, int nb, int dir, int which,
                                  const Phys& ph, int jb, int occ, const FixList& fx, hipStream_t s);
 // Recompute the faces listed in fx (of the sweeps of state `which` since the
-// last fixup) with the reference-order form; resets fx.
+// last fixup) with the reference-order form; resets fx.  dirs: bit mask of
+// the face arrays the pass over every face (more failures than fx holds)
+// checks.
 hipError_t launch_face_fixup(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
-                             const FixList& fx, hipStream_t s);
+                             const FixList& fx, hipStream_t s, int dirs = 7);
 FixList fixlist_at(void* buf);
 // Single solve (cbns_kernels_zfuse.hip): the z-face sweep fused with the
 // stage tail, for PREC_FP64, PREC_FP32_FLUX and PREC_FP64_FAST.  Stage
@@ -124,6 +126,12 @@ struct RefState;
 hipError_t launch_zface_update(const BlockDesc* db, const BlockDesc* hb, int nb, int precision, int in, int out,
                                int stage, double dt, const Phys& ph, const RefState& ref, unsigned long long* smax,
                                int* flag, const FixList& cfx, hipStream_t s);
+// Tolerance-bound form: its y faces swept by a march along y that stores
+// each interior cell's x- plus y-flux differences into F[1] (after the x
+// sweep and its face fixup); cells beside a failing y face are marked for
+// the fused z pass to leave to the cell fixup.
+hipError_t launch_yface_xy(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph, int occ,
+                           hipStream_t s);
 hipError_t launch_zcell_fixup(const BlockDesc* db, const BlockDesc* hb, int nb, int precision, int in, int out,
                               int stage, double dt, const Phys& ph, unsigned long long* smax, int* flag,
                               const FixList& cfx, hipStream_t s);

@@ -105,11 +105,13 @@ constexpr int FIX_WG = 64;  // workgroups of the fixup pass
 
 // BATCH: blocks of many solves, each with its own physics (insts); the
 // single-solve form passes ph.  In the pass over every face, only blocks of
-// stepping tolerance-bound solves are checked.
+// stepping tolerance-bound solves are checked, and only the face arrays in
+// the mask dirs (the fused steps sweep only x faces into the list: their y
+// faces are marched by yrun_fast, whose F[1] holds cell differences).
 template <bool BATCH>
 __global__ __launch_bounds__(TPB) void k_face_fixup(const BlockDesc* __restrict__ blocks,
                                                     const InstDesc* __restrict__ insts, int nb, int which, Phys ph,
-                                                    FixList fx) {
+                                                    FixList fx, int dirs) {
   const unsigned n = *fx.count;  // every workgroup reads it before the last one resets it (below)
   const long long first = blockIdx.x * (long long)TPB + threadIdx.x, stride = (long long)gridDim.x * TPB;
   if (n <= fx.cap) {
@@ -127,9 +129,9 @@ __global__ __launch_bounds__(TPB) void k_face_fixup(const BlockDesc* __restrict_
       const BlockDesc& B = blocks[b];
       if (BATCH && (!insts[B.inst].stepping || insts[B.inst].precision != PREC_FP64_FAST)) continue;
       const Phys& P = BATCH ? insts[B.inst].ph : ph;
-      fixup_all<0>(B, which, P, first, stride);
-      fixup_all<1>(B, which, P, first, stride);
-      fixup_all<2>(B, which, P, first, stride);
+      if (dirs & 1) fixup_all<0>(B, which, P, first, stride);
+      if (dirs & 2) fixup_all<1>(B, which, P, first, stride);
+      if (dirs & 4) fixup_all<2>(B, which, P, first, stride);
     }
   }
   // the last workgroup to finish clears the list for the next stage
@@ -144,16 +146,17 @@ __global__ __launch_bounds__(TPB) void k_face_fixup(const BlockDesc* __restrict_
 }
 
 hipError_t launch_face_fixup(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
-                             const FixList& fx, hipStream_t s) {
+                             const FixList& fx, hipStream_t s, int dirs) {
   (void)hb;
   hipLaunchKernelGGL(k_face_fixup<false>, dim3(FIX_WG), dim3(TPB), 0, s, db, (const InstDesc*)nullptr, nb, which, ph,
-                     fx);
+                     fx, dirs);
   return hipGetLastError();
 }
 
+// the batched step's x faces (its tolerance-bound solves march their y faces)
 hipError_t batch_launch_fixup(const BlockDesc* blocks, const InstDesc* insts, int nb, int which, const FixList& fx,
                               hipStream_t s) {
-  hipLaunchKernelGGL(k_face_fixup<true>, dim3(FIX_WG), dim3(TPB), 0, s, blocks, insts, nb, which, Phys{}, fx);
+  hipLaunchKernelGGL(k_face_fixup<true>, dim3(FIX_WG), dim3(TPB), 0, s, blocks, insts, nb, which, Phys{}, fx, 1);
   return hipGetLastError();
 }
 

@@ -64,6 +64,37 @@ hipError_t launch_zface_update(const BlockDesc* db, const BlockDesc* hb, int nb,
   return hipGetLastError();
 }
 
+// Tolerance-bound form: the y-face march combined with the x-face fluxes
+// (yrun_fast), one column (i, k) per thread, ZX x ZY columns per workgroup,
+// runs of ym cells along y.  OCC: waves per SIMD the register allocation
+// allows (3 spills a few registers, 2 none; the autotuner's face occupancy
+// picks).
+template <int OCC>
+__global__ __launch_bounds__(ZX * ZY, OCC) void k_yface_xy(const BlockDesc* __restrict__ blocks, int which, Phys ph,
+                                                           int ym) {
+  const Tile tl = xcd_tile();
+  const BlockDesc& B = blocks[blockIdx.z];
+  const int ntx = (B.n[0] + ZX - 1) / ZX;
+  const int i = (tl.x % ntx) * ZX + (int)threadIdx.x % ZX;
+  const int k = (tl.x / ntx) * ZY + (int)threadIdx.x / ZX;
+  const int j0 = tl.y * ym;
+  if (i < B.n[0] && k < B.n[2] && j0 < B.n[1]) yrun_fast(B, i, k, j0, ym, which, ph);
+}
+
+hipError_t launch_yface_xy(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph, int occ,
+                           hipStream_t s) {
+  int tiles = 0, n1 = 0;
+  for (int b = 0; b < nb; b++) {
+    tiles = std::max(tiles, ((hb[b].n[0] + ZX - 1) / ZX) * ((hb[b].n[2] + ZY - 1) / ZY));
+    n1 = std::max(n1, hb[b].n[1]);
+  }
+  const int ym = chunk_len((long long)tiles * nb, n1, ZM, ZM_LOW, 1024);
+  const dim3 grid(tiles, (n1 + ym - 1) / ym, nb), blk(ZX * ZY);
+  if (occ >= 3) hipLaunchKernelGGL(k_yface_xy<3>, grid, blk, 0, s, db, which, ph, ym);
+  else hipLaunchKernelGGL(k_yface_xy<2>, grid, blk, 0, s, db, which, ph, ym);
+  return hipGetLastError();
+}
+
 constexpr int ZFIX_WG = 64;  // workgroups of the cell fixup pass
 
 template <int FORM>

@@ -22,6 +22,11 @@
 // stored values (rounded to float for FP32-flux as stored there); the
 // FP32-flux face fluxes are rounded to float as its face arrays store them.
 // Every value therefore equals the separate sweep and tail kernels'.
+// The tolerance-bound form's y faces are swept by a march of the same shape
+// along y (yrun_fast), which combines them at once with the x-face fluxes:
+// it stores, per cell, the x- plus y-flux differences in the reference's
+// order and rounding (src/Block3d.cpp:932-938), so its fused z pass reads
+// five values per cell instead of the x- and y-face fluxes of both sides.
 // Shared by the single-solve kernels (cbns_kernels_zfuse.hip) and the
 // batched ones (cbns_batch.hip).
 #pragma once
@@ -35,14 +40,16 @@ template <int FORM>
 using zft = typename std::conditional<FORM == ZF_FP32, float, double>::type;
 
 // Operands of the update of one cell other than its z-fluxes and its stage
-// input, in their storage types.
+// input, in their storage types.  XY (tolerance-bound form): xy holds the
+// cell's x- plus y-flux differences (yrun_fast) and e0 .. f1 are unused.
 template <typename FT>
 struct ZOperands {
   FT e0[5], e1[5], f0[5], f1[5], dv[4];
+  double xy[5];
   double qn[5];
 };
 
-template <typename FT>
+template <typename FT, bool XY = false>
 __device__ __forceinline__ void zload(const BlockDesc& B, int i, int j, int k, ZOperands<FT>& o, int stage = 2) {
   const int n0 = B.n[0], n1 = B.n[1];
   const int iE = i + (n0 + 1) * (j + n1 * k), iF = i + n0 * (j + (n1 + 1) * k), ct = i + n0 * (j + n1 * k);
@@ -56,14 +63,26 @@ __device__ __forceinline__ void zload(const BlockDesc& B, int i, int j, int k, Z
   const int qc = stage > 1 ? c : 0;
 #pragma unroll
   for (int e = 0; e < 5; e++) {
-    o.e0[e] = E[e * B.sf[0] + iE];
-    o.e1[e] = E[e * B.sf[0] + iE + 1];
-    o.f0[e] = F[e * B.sf[1] + iF];
-    o.f1[e] = F[e * B.sf[1] + iF + n0];
+    if constexpr (XY) {
+      o.xy[e] = gmem<const double>(B.F[1])[e * B.sf[1] + ct];
+    } else {
+      o.e0[e] = E[e * B.sf[0] + iE];
+      o.e1[e] = E[e * B.sf[0] + iE + 1];
+      o.f0[e] = F[e * B.sf[1] + iF];
+      o.f1[e] = F[e * B.sf[1] + iF + n0];
+    }
     o.qn[e] = Qn[e * qs + qc];  // used by stages 2 and 3 (loaded always: no branch among the loads)
   }
 #pragma unroll
   for (int e = 0; e < 4; e++) o.dv[e] = DV[e * B.sn + ct];
+}
+
+// x- plus y-flux differences of one cell from its x-face fluxes e0, e1 and
+// y-face fluxes f0, f1, as update_rk3 sums them (src/Block3d.cpp:932-938)
+__device__ __forceinline__ double xy_diff(double e0, double e1, double f0, double f1, const Phys& ph) {
+  double df = divc(e1 - e0, ph.dx, ph.rdx, ph.fdx);
+  df += divc(f1 - f0, ph.dy, ph.rdy, ph.fdy);
+  return df;
 }
 
 // The update of interior cell (i, j, k) with the expressions and order of
@@ -84,8 +103,8 @@ __device__ __forceinline__ void zapply(const BlockDesc& B, int i, int j, int k, 
   double r[5];
 #pragma unroll
   for (int e = 0; e < 5; e++) {
-    double df = divc((double)o.e1[e] - (double)o.e0[e], ph.dx, ph.rdx, ph.fdx);
-    df += divc((double)o.f1[e] - (double)o.f0[e], ph.dy, ph.rdy, ph.fdy);
+    double df = FORM == ZF_FAST ? o.xy[e]
+                                : xy_diff((double)o.e0[e], (double)o.e1[e], (double)o.f0[e], (double)o.f1[e], ph);
     df += divc(gk1[e] - gk[e], ph.dz, ph.rdz, ph.fdz);
     if (e > 0) df -= (double)o.dv[e - 1];
     if (1 == stage) r[e] = qi[e] - dt * df;
@@ -167,7 +186,7 @@ __device__ __forceinline__ void zrun(const BlockDesc& B, int b, int i, int j, in
     // both of its cells at each face.
     constexpr bool early = FORM != ZF_FP64;
     ZOperands<FT> o;
-    if (early) zload<FT>(B, i, j, k > k0 ? k - 1 : k0, o, stage);
+    if (early) zload<FT, FORM == ZF_FAST>(B, i, j, k > k0 ? k - 1 : k0, o, stage);
     double nx[5];
     if (early) {
 #pragma unroll
@@ -194,7 +213,9 @@ __device__ __forceinline__ void zrun(const BlockDesc& B, int b, int i, int j, in
     }
     if (k > k0) {  // cell k - 1 is q[2]
       if (!early && ok && okp) zload<FT>(B, i, j, k - 1, o, stage);
-      if (ok && okp) zapply<FORM>(B, i, j, k - 1, o, gp, g, q[2], out, stage, dt, ph, want, s, bad);
+      // a tolerance-bound cell beside a failing y face carries a NaN mark
+      const bool okxy = FORM != ZF_FAST || o.xy[0] == o.xy[0];
+      if (ok && okp && okxy) zapply<FORM>(B, i, j, k - 1, o, gp, g, q[2], out, stage, dt, ph, want, s, bad);
       else {
         const unsigned slot = atomicAdd(cfx.count, 1u);
         if (slot < cfx.cap)
@@ -246,6 +267,9 @@ __device__ __forceinline__ bool zface_checked(const double (&q)[6][5], const Phy
 // A listed cell: both of its z-faces (zface_checked), then its update.  With
 // all set (more failures than the list holds), a cell is updated only if
 // one of its faces fails its checks: exactly the cells the fused pass left.
+// The tolerance-bound form's cells also take their y faces here (the march
+// stored no y-face fluxes), and a cell is left alone in the all pass only if
+// its two y faces pass as well.
 template <int FORM>
 __device__ __forceinline__ void zcell_fix(const BlockDesc& B, int i, int j, int k, int in, int out, int stage,
                                           double dt, const Phys& ph, bool all, bool want_s, double* s, int* bad) {
@@ -253,26 +277,109 @@ __device__ __forceinline__ void zcell_fix(const BlockDesc& B, int i, int j, int 
   const long long sg = B.sg;
   gptr<const double> __restrict__ Qin = gmem<const double>(B.Q[in]);
   const int c = gidx(B, i, j, k);
-  double q7[7][5];
+  // the two faces along axis D of the cell from its seven cells along D
+  const auto faces = [&](auto dc, double (&g0)[5], double (&g1)[5], bool& ok0, bool& ok1) {
+    constexpr int D = decltype(dc)::value;
+    const int sD = D == 1 ? B.g[0] : sz;
+    double q7[7][5];
 #pragma unroll
-  for (int cc = 0; cc < 7; cc++)
+    for (int cc = 0; cc < 7; cc++)
 #pragma unroll
-    for (int e = 0; e < 5; e++) q7[cc][e] = Qin[e * sg + c + (cc - 3) * sz];
-  double qa[6][5], qb[6][5];
+      for (int e = 0; e < 5; e++) q7[cc][e] = Qin[e * sg + c + (cc - 3) * sD];
+    double qa[6][5], qb[6][5];
+#pragma unroll
+    for (int cc = 0; cc < 6; cc++)
+#pragma unroll
+      for (int e = 0; e < 5; e++) {
+        qa[cc][e] = q7[cc][e];
+        qb[cc][e] = q7[cc + 1][e];
+      }
+    if constexpr (D == 2) {
+      ok0 = zface_checked<FORM>(qa, ph, true, g0);
+      ok1 = zface_checked<FORM>(qb, ph, true, g1);
+    } else {
+      ok0 = face_flux_fast<1>(qa, ph.gamma, ph.gam1, ph.gg1, g0);
+      if (!ok0) face_flux_fast_fallback<1>(qa, ph, g0);
+      ok1 = face_flux_fast<1>(qb, ph.gamma, ph.gam1, ph.gg1, g1);
+      if (!ok1) face_flux_fast_fallback<1>(qb, ph, g1);
+    }
+  };
+  double g0[5], g1[5], f0[5], f1[5];
+  bool ok0, ok1, oky0 = true, oky1 = true;
+  faces(std::integral_constant<int, 2>(), g0, g1, ok0, ok1);
+  if constexpr (FORM == ZF_FAST) faces(std::integral_constant<int, 1>(), f0, f1, oky0, oky1);
+  if (all && ok0 && ok1 && oky0 && oky1) return;
+  ZOperands<double> o;
+  zload<double>(B, i, j, k, o, stage);
+  if constexpr (FORM == ZF_FAST) {
+#pragma unroll
+    for (int e = 0; e < 5; e++) o.xy[e] = xy_diff(o.e0[e], o.e1[e], f0[e], f1[e], ph);
+  }
+  double qc[5];
+#pragma unroll
+  for (int e = 0; e < 5; e++) qc[e] = Qin[e * sg + c];
+  zapply<FORM>(B, i, j, k, o, g0, g1, qc, out, stage, dt, ph, want_s, s, bad);
+}
+
+// Tolerance-bound form: one thread's march along y over cells [j0, j0 + ym)
+// of column (i, k) of block B, i < n0, k < n2, j0 < n1.  It sweeps the y
+// faces j0 .. j1 with a six-cell register queue of the stage input (as zrun
+// sweeps z) and, after face j, stores the x- plus y-flux differences of cell
+// j - 1 (xy_diff, from the x-face fluxes of the x sweep and its fixup) into
+// B.F[1] at the cell's interior index.  A cell beside a y face whose range
+// checks fail gets a NaN in place of its differences: the fused z pass then
+// leaves it to the cell fixup, which forms its faces with the reference-order
+// form.
+__device__ __forceinline__ void yrun_fast(const BlockDesc& B, int i, int k, int j0, int ym, int which,
+                                          const Phys& ph) {
+  const int n0 = B.n[0], n1 = B.n[1];
+  const int j1 = j0 + ym < n1 ? j0 + ym : n1;  // cells [j0, j1), faces j0 .. j1
+  const int sy = B.g[0];
+  const long long sg = B.sg;
+  gptr<const double> __restrict__ Qin = gmem<const double>(B.Q[which]);
+  gptr<const double> __restrict__ E = gmem<const double>(B.F[0]);
+  gptr<double> __restrict__ XY = gmem(B.F[1]);
+  const long long sfe = B.sf[0], sfx = B.sf[1];
+  const int c0 = gidx(B, i, 0, k);
+  double q[6][5];  // cells j - 3 .. j + 2 of face j (rows -3 .. n1 + 2: inside the ghosts)
 #pragma unroll
   for (int cc = 0; cc < 6; cc++)
 #pragma unroll
+    for (int e = 0; e < 5; e++) q[cc][e] = Qin[e * sg + c0 + (j0 - 3 + cc) * sy];
+  double gp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  bool okp = true;
+  for (int j = j0; j <= j1; j++) {
+    // this step's loads (the x faces of cell j - 1, or of cell j0 unused on
+    // the first face, and the next queued cell, row j + 3 <= n1 + 3) go out
+    // before the face, with no branch among them
+    const int jc = j > j0 ? j - 1 : j0;
+    const int iE = i + (n0 + 1) * (jc + n1 * k);
+    double e0[5], e1[5], nx[5];
+#pragma unroll
     for (int e = 0; e < 5; e++) {
-      qa[cc][e] = q7[cc][e];
-      qb[cc][e] = q7[cc + 1][e];
+      e0[e] = E[e * sfe + iE];
+      e1[e] = E[e * sfe + iE + 1];
+      nx[e] = Qin[e * sg + c0 + (j + 3) * sy];
     }
-  double g0[5], g1[5];
-  const bool ok0 = zface_checked<FORM>(qa, ph, true, g0);
-  const bool ok1 = zface_checked<FORM>(qb, ph, true, g1);
-  if (all && ok0 && ok1) return;
-  ZOperands<double> o;
-  zload<double>(B, i, j, k, o, stage);
-  zapply<FORM>(B, i, j, k, o, g0, g1, q7[3], out, stage, dt, ph, want_s, s, bad);
+    double g[5];
+    const bool ok = face_flux_fast<1>(q, ph.gamma, ph.gam1, ph.gg1, g);
+    if (j > j0) {
+      const int ct = i + n0 * (jc + n1 * k);
+      const bool good = ok && okp;
+#pragma unroll
+      for (int e = 0; e < 5; e++) XY[e * sfx + ct] = good ? xy_diff(e0[e], e1[e], gp[e], g[e], ph) : __builtin_nan("");
+    }
+#pragma unroll
+    for (int cc = 0; cc < 5; cc++)
+#pragma unroll
+      for (int e = 0; e < 5; e++) q[cc][e] = q[cc + 1][e];
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+      q[5][e] = nx[e];
+      gp[e] = g[e];
+    }
+    okp = ok;
+  }
 }
 
 }  // namespace cbns
