@@ -56,12 +56,21 @@ __device__ __forceinline__ void band_decode(int t, int ni, int nj, int nk, int j
   }
 }
 
+// x / h as divc, or with R (the tolerance-bound form's viscous terms and
+// flux differences) x times RN(1/h): within about one ulp of the quotient,
+// in one operation instead of three.
+template <bool R>
+__device__ __forceinline__ double divr(double x, double h, double rh, int fh) {
+  if constexpr (R) return x * rh;
+  else return divc(x, h, rh, fh);
+}
+
 // 4th-order central derivative (src/calc_gradient.cpp:29-64): the stencil
-// sum, then the division by the spacing.
-template <class P>
+// sum, then the division by the spacing (divr).
+template <bool R = false, class P>
 __device__ __forceinline__ double grad4(P __restrict__ f, int c, int s, double h, double rh, int fh) {
   double d = (1.0 / 12.0) * f[c - 2 * s] - 2.0 / 3.0 * f[c - s] + (2.0 / 3.0) * f[c + s] - 1.0 / 12.0 * f[c + 2 * s];
-  return divc(d, h, rh, fh);
+  return divr<R>(d, h, rh, fh);
 }
 
 // Divergence stencil of calc_viscous_flux_contribution (src/Block3d.cpp:894-908).
@@ -411,14 +420,19 @@ __device__ __forceinline__ void gst(gptr<T> base, unsigned idx, T v) {
 //    in the reference's order (x-part, + y-part, + z-part).
 // The next plane's loads are issued before the current plane's arithmetic.
 // Every value is formed by the same expressions as the reference's separate
-// passes (grad4, visc_terms, the divergence stencil).  One call = one
-// (column tile, z-chunk) unit of vz planes, made by all VX * VY threads of a
-// workgroup with the same tile and chunk (it synchronises the workgroup).
+// passes (grad4, visc_terms, the divergence stencil); in the tolerance-bound
+// (FROMQ) and FP32-flux (float VT) forms the divisions by the spacings are
+// multiplications by their reciprocals (divr).  One call = one (column tile,
+// z-chunk) unit of vz planes, made by all VX * VY threads of a workgroup with
+// the same tile and chunk (it synchronises the workgroup).
 template <typename VT, bool FROMQ = false>
 __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& ph, int tile, int chunk, int which,
                                                int vz) {
   static_assert(VX == 16 && VY == 16, "one wave of side points per side kind");
   static_assert(!FROMQ || sizeof(VT) == 8, "the state is FP64");
+  // divisions by the spacings as multiplications by their reciprocals: the
+  // tolerance-bound form (FROMQ) and the FP32-flux form (float storage)
+  constexpr bool RX = FROMQ || sizeof(VT) == 4;
   const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
   const int ntx = (n0 + VX - 1) / VX, nty = (n1 + VY - 1) / VY;
   const int Z0 = chunk * vz;
@@ -619,10 +633,10 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
       const int b = buf ^ 1;
 #pragma unroll
       for (int e = 0; e < 4; e++) {
-        double d = divc(c0 * LE[b][e][ty][tx] + c1 * LE[b][e][ty][tx + 1] - c1 * LE[b][e][ty][tx + 3] -
+        double d = divr<RX>(c0 * LE[b][e][ty][tx] + c1 * LE[b][e][ty][tx + 1] - c1 * LE[b][e][ty][tx + 3] -
                             c0 * LE[b][e][ty][tx + 4],
                         ph.dx, ph.rdx, ph.fdx);
-        d += divc(c0 * LF[b][e][ty][tx] + c1 * LF[b][e][ty + 1][tx] - c1 * LF[b][e][ty + 3][tx] -
+        d += divr<RX>(c0 * LF[b][e][ty][tx] + c1 * LF[b][e][ty + 1][tx] - c1 * LF[b][e][ty + 3][tx] -
                       c0 * LF[b][e][ty + 4][tx],
                   ph.dy, ph.rdy, ph.fdy);
         dxy[e] = d;
@@ -633,9 +647,9 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
       double gr[4][3];
 #pragma unroll
       for (int f = 0; f < 4; f++) {
-        gr[f][0] = grad4(&L[buf][f][ty + 4][0], tx + 4, 1, ph.dx, ph.rdx, ph.fdx);
-        gr[f][1] = grad4(&L[buf][f][0][tx + 4], (ty + 4) * SX, SX, ph.dy, ph.rdy, ph.fdy);
-        gr[f][2] = grad4(&r[f][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
+        gr[f][0] = grad4<RX>(&L[buf][f][ty + 4][0], tx + 4, 1, ph.dx, ph.rdx, ph.fdx);
+        gr[f][1] = grad4<RX>(&L[buf][f][0][tx + 4], (ty + 4) * SX, SX, ph.dy, ph.rdy, ph.fdy);
+        gr[f][2] = grad4<RX>(&r[f][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
       }
       double o[9], mu_l;
       visc_terms(ph, gr, r[3][2], r[0][2], r[1][2], r[2][2], o, mu_l);
@@ -656,18 +670,18 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
       const int lx = sx + 4, ly = sy + 4;
       const double T_l = (double)L[buf][3][ly][lx], uu = (double)L[buf][0][ly][lx],
                    vv = (double)L[buf][1][ly][lx], ww = (double)L[buf][2][ly][lx];
-      const double u_x = grad4(&L[buf][0][ly][0], lx, 1, ph.dx, ph.rdx, ph.fdx);
-      const double u_y = grad4(&L[buf][0][0][lx], ly * SX, SX, ph.dy, ph.rdy, ph.fdy);
-      const double v_x = grad4(&L[buf][1][ly][0], lx, 1, ph.dx, ph.rdx, ph.fdx);
-      const double v_y = grad4(&L[buf][1][0][lx], ly * SX, SX, ph.dy, ph.rdy, ph.fdy);
-      const double a_z = grad4(&sq[0][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
-      const double w_z = grad4(&sq[1][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
+      const double u_x = grad4<RX>(&L[buf][0][ly][0], lx, 1, ph.dx, ph.rdx, ph.fdx);
+      const double u_y = grad4<RX>(&L[buf][0][0][lx], ly * SX, SX, ph.dy, ph.rdy, ph.fdy);
+      const double v_x = grad4<RX>(&L[buf][1][ly][0], lx, 1, ph.dx, ph.rdx, ph.fdx);
+      const double v_y = grad4<RX>(&L[buf][1][0][lx], ly * SX, SX, ph.dy, ph.rdy, ph.fdy);
+      const double a_z = grad4<RX>(&sq[0][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
+      const double w_z = grad4<RX>(&sq[1][0], 2, 1, ph.dz, ph.rdz, ph.fdz);
       double mu_l = (1.0 + ph.C_T_inf) / (T_l + ph.C_T_inf) * sqrt(T_l * T_l * T_l);
       mu_l *= ph.Re_inv;
       const double txy = mu_l * (u_y + v_x);
       if (wave == 0) {  // Ev = (txx, txy, txz, ex)
-        const double w_x = grad4(&L[buf][2][ly][0], lx, 1, ph.dx, ph.rdx, ph.fdx);
-        const double T_x = grad4(&L[buf][3][ly][0], lx, 1, ph.dx, ph.rdx, ph.fdx);
+        const double w_x = grad4<RX>(&L[buf][2][ly][0], lx, 1, ph.dx, ph.rdx, ph.fdx);
+        const double T_x = grad4<RX>(&L[buf][3][ly][0], lx, 1, ph.dx, ph.rdx, ph.fdx);
         const double txx = (2.0 / 3.0) * mu_l * (2.0 * u_x - v_y - w_z);
         const double txz = mu_l * (a_z + w_x);
         mu_l *= ph.kappa;
@@ -677,8 +691,8 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
         LE[buf][2][sy][sx + 2] = (VT)txz;
         LE[buf][3][sy][sx + 2] = (VT)(uu * txx + vv * txy + ww * txz - qx);
       } else {  // Fv = (txy, tyy, tyz, ey)
-        const double w_y = grad4(&L[buf][2][0][lx], ly * SX, SX, ph.dy, ph.rdy, ph.fdy);
-        const double T_y = grad4(&L[buf][3][0][lx], ly * SX, SX, ph.dy, ph.rdy, ph.fdy);
+        const double w_y = grad4<RX>(&L[buf][2][0][lx], ly * SX, SX, ph.dy, ph.rdy, ph.fdy);
+        const double T_y = grad4<RX>(&L[buf][3][0][lx], ly * SX, SX, ph.dy, ph.rdy, ph.fdy);
         const double tyy = (2.0 / 3.0) * mu_l * (2.0 * v_y - u_x - w_z);
         const double tyz = mu_l * (a_z + w_y);
         mu_l *= ph.kappa;
@@ -695,7 +709,7 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
 #pragma unroll
       for (int e = 0; e < 4; e++) {
         const double dz = c0 * gq[e][0] + c1 * gq[e][1] - c1 * gq[e][3] - c0 * gq[e][4];
-        gst(DV[e], o, (VT)(dq[e] + divc(dz, ph.dz, ph.rdz, ph.fdz)));
+        gst(DV[e], o, (VT)(dq[e] + divr<RX>(dz, ph.dz, ph.rdz, ph.fdz)));
       }
     }
 #pragma unroll

@@ -78,10 +78,13 @@ __device__ __forceinline__ void zload(const BlockDesc& B, int i, int j, int k, Z
 }
 
 // x- plus y-flux differences of one cell from its x-face fluxes e0, e1 and
-// y-face fluxes f0, f1, as update_rk3 sums them (src/Block3d.cpp:932-938)
+// y-face fluxes f0, f1, as update_rk3 sums them (src/Block3d.cpp:932-938);
+// R (tolerance-bound form): the divisions by the spacings as divr's
+// multiplications by their reciprocals.
+template <bool R = false>
 __device__ __forceinline__ double xy_diff(double e0, double e1, double f0, double f1, const Phys& ph) {
-  double df = divc(e1 - e0, ph.dx, ph.rdx, ph.fdx);
-  df += divc(f1 - f0, ph.dy, ph.rdy, ph.fdy);
+  double df = divr<R>(e1 - e0, ph.dx, ph.rdx, ph.fdx);
+  df += divr<R>(f1 - f0, ph.dy, ph.rdy, ph.fdy);
   return df;
 }
 
@@ -105,7 +108,7 @@ __device__ __forceinline__ void zapply(const BlockDesc& B, int i, int j, int k, 
   for (int e = 0; e < 5; e++) {
     double df = FORM == ZF_FAST ? o.xy[e]
                                 : xy_diff((double)o.e0[e], (double)o.e1[e], (double)o.f0[e], (double)o.f1[e], ph);
-    df += divc(gk1[e] - gk[e], ph.dz, ph.rdz, ph.fdz);
+    df += divr<FORM == ZF_FAST>(gk1[e] - gk[e], ph.dz, ph.rdz, ph.fdz);
     if (e > 0) df -= (double)o.dv[e - 1];
     if (1 == stage) r[e] = qi[e] - dt * df;
     else if (2 == stage) r[e] = 0.75 * o.qn[e] + 0.25 * (qi[e] - dt * df);
@@ -313,7 +316,7 @@ __device__ __forceinline__ void zcell_fix(const BlockDesc& B, int i, int j, int 
   zload<double>(B, i, j, k, o, stage);
   if constexpr (FORM == ZF_FAST) {
 #pragma unroll
-    for (int e = 0; e < 5; e++) o.xy[e] = xy_diff(o.e0[e], o.e1[e], f0[e], f1[e], ph);
+    for (int e = 0; e < 5; e++) o.xy[e] = xy_diff<true>(o.e0[e], o.e1[e], f0[e], f1[e], ph);
   }
   double qc[5];
 #pragma unroll
@@ -367,7 +370,8 @@ __device__ __forceinline__ void yrun_fast(const BlockDesc& B, int i, int k, int 
       const int ct = i + n0 * (jc + n1 * k);
       const bool good = ok && okp;
 #pragma unroll
-      for (int e = 0; e < 5; e++) XY[e * sfx + ct] = good ? xy_diff(e0[e], e1[e], gp[e], g[e], ph) : __builtin_nan("");
+      for (int e = 0; e < 5; e++)
+        XY[e * sfx + ct] = good ? xy_diff<true>(e0[e], e1[e], gp[e], g[e], ph) : __builtin_nan("");
     }
 #pragma unroll
     for (int cc = 0; cc < 5; cc++)

@@ -154,10 +154,11 @@ def test_fast_form_kinetic_energy_history():
 def test_fast_form_scaled_state_faces_take_the_fixup_pass(N, dims):
     """A Taylor-Green state scaled by 1e50 (the same flow: u, T and dt are
     unchanged) drives every WENO denominator past 1e300, so every face fails
-    its checks and the fixup pass recomputes it with the reference-order
-    form: from its list at 12^3 (5,616 faces), and over every face at 32^3
-    (101,376 faces, more than the list's 65,536).  Two steps against the
-    oracle on the same scaled state."""
+    its checks: the x faces go to the face fixup's list (1,872 at 12^3,
+    33,792 at 32^3), and the cells beside the failing y and z faces to the
+    cell fixup, which forms those faces with the reference-order form.  Two
+    steps against the oracle on the same scaled state.  (Both fixups past
+    their lists: test_tolerance_bound_form_past_the_cell_list.)"""
     ob = O.Block(O.make_params(N))
     ob.init_tgv()
     Q0 = ob.Q_interior(0) * 1e50

@@ -44,10 +44,11 @@ def test_bit_exact_form_on_a_scaled_state_equals_the_oracle(N, dims):
 
 def test_tolerance_bound_form_past_the_cell_list():
     """A state scaled by 1e50 fails every face's checks in the
-    tolerance-bound form: at 48^3 the 110,592 cells of the fused z pass
-    exceed the cell list, so the fixup checks every cell; two steps within
-    the form's one-step bar of the oracle (1e-12 of each field's maximum,
-    rho w against the momentum scale)."""
+    tolerance-bound form: at 48^3 the 112,896 x faces exceed the face
+    fixup's list and the 110,592 cells of the fused z pass the cell list,
+    so both fixups check everything (the cell fixup forms every cell's y
+    and z faces); two steps within the form's one-step bar of the oracle
+    (1e-12 of each field's maximum, rho w against the momentum scale)."""
     N = (48, 48, 48)
     Q0, R, t = scaled_oracle(N, 1e50, 2)
     with C.Solver(C.make_params(N), precision=C.CBNS_FP64_FAST) as s:
