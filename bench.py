@@ -144,7 +144,7 @@ def form_leg(C, N, dims, steps_total, steps_timed, Q64, precision):
     pre = steps_total - steps_timed - BREAKDOWN_STEPS  # error taken at the FP64 leg's step count
     occs = (2,) if precision == C.CBNS_FP32_FLUX else (2, 3)
     best = None
-    if pre >= 3 * 2 * len(autotune.SCHEDULES) * len(occs):
+    if pre >= 3 * (2 * len(autotune.SCHEDULES) * len(occs) + len(autotune.SHAPES)):
         best, tms = autotune.tune(s, steps=2, bands=(16, 32), schedules=autotune.SCHEDULES, occs=occs)
         pre -= 3 * len(tms)
     s.run(pre)
@@ -168,7 +168,8 @@ def form_leg(C, N, dims, steps_total, steps_timed, Q64, precision):
     return {"value": round(v, 2), "unit": "Mcell-steps/s", "ms_per_step": round(el / steps_timed * 1e3, 3),
             "error_vs_fp64_after_steps": steps_total, "error": err,
             "kernels_avg_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items() if v[1]},
-            "autotune": {"band": best[0], "schedule": best[1], "face_waves_per_simd": best[2]} if best else None}
+            "autotune": {"band": best[0], "schedule": best[1], "face_waves_per_simd": best[2],
+                         "visc_planes_run_cells": list(best[3])} if best else None}
 
 
 def sweep_leg(steps):
@@ -334,8 +335,11 @@ def plain_leg(C, N, dims, precision, steps, warmup, fixed=None):
         best, tms = autotune.tune(s, steps=2, bands=(16, 32, 64), schedules=autotune.SCHEDULES, occs=(2, 3))
         tuned_steps = 3 * len(tms)
         tuned = {"band": best[0], "schedule": best[1], "face_waves_per_simd": best[2],
-                 "ms_per_step (band/schedule/waves)": {f"{b}/{sc}/{w}": round(v * 1e3, 3)
-                                                       for (b, sc, w), v in tms.items()}}
+                 "visc_planes_run_cells": list(best[3]),
+                 "ms_per_step (band/schedule/waves)": {f"{k[0]}/{k[1]}/{k[2]}": round(v * 1e3, 3)
+                                                       for k, v in tms.items() if len(k) == 3},
+                 "ms_per_step (visc planes/run cells)": {f"{k[3][0]}/{k[3][1]}": round(v * 1e3, 3)
+                                                         for k, v in tms.items() if len(k) == 4}}
     else:  # launch shapes given on the command line (profiling runs)
         band, sched, occ = fixed
         s.set_band(band)

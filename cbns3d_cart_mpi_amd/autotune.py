@@ -10,7 +10,10 @@ Single context (tune): rows per band of the stencil kernels' work order
 FP64 face kernels, and the stage schedule:
   "overlap"   viscous-flux kernel on a side stream beside the face sweeps
               ("overlap:n": on n workgroups taking its tile units in turn);
-  "serial"    viscous-flux kernel after the sweeps, on the same stream.
+  "serial"    viscous-flux kernel after the sweeps, on the same stream;
+then, on the best of those, the tile shapes of the z-marching kernels
+(planes per chunk of the viscous tile march, cells per run of the fused
+z-face pass and the y-face march).
 Batch (tune_batch): rows per band, and the viscous kernels' schedule in the
 step graph (0: in line; n: beside the sweeps with n persistent workgroups).
 """
@@ -37,17 +40,23 @@ def apply_schedule(solver, sched):
 
 
 SERIAL_MARGIN = 0.015  # an overlapped schedule must beat the best serial one by this fraction
+# (viscous planes per chunk, cells per run); (32, 32) is the library's default
+SHAPES = ((16, 32), (32, 16), (32, 32), (32, 64), (64, 32), (64, 64))
 
 
-def tune(solver, steps=3, bands=BANDS, schedules=SCHEDULES, occs=(2, 3)):
+def tune(solver, steps=3, bands=BANDS, schedules=SCHEDULES, occs=(2, 3), shapes=SHAPES):
     """Try each (band, schedule, face occupancy) on `steps` steps of the live
-    state and leave the solver on the fastest, preferring the serial
-    schedule unless an overlapped one is faster by more than SERIAL_MARGIN
-    (within timing noise the kernels then run alone, so their own durations
-    are what a profile of the step shows).  Returns
-    (best, {choice: seconds_per_step}); the state advances by
+    state and keep the fastest, preferring the serial schedule unless an
+    overlapped one is faster by more than SERIAL_MARGIN (within timing noise
+    the kernels then run alone, so their own durations are what a profile of
+    the step shows); then try each tile shape (viscous planes per chunk,
+    cells per run) on that choice, the default (32, 32) as well.  Leaves the
+    solver on the result.  Returns (best, {choice: seconds_per_step}) with
+    best = (band, schedule, occupancy, (planes, cells)) and the timings of
+    both passes (keys of 3 and 4 entries); the state advances by
     (steps + 1) steps per choice."""
     timings = {}
+    solver.set_tile_shape(0, 0)
     for band, sched, occ in itertools.product(bands, schedules, occs):
         solver.set_band(band)
         apply_schedule(solver, sched)
@@ -63,7 +72,15 @@ def tune(solver, steps=3, bands=BANDS, schedules=SCHEDULES, occs=(2, 3)):
     solver.set_band(best[0])
     apply_schedule(solver, best[1])
     solver.set_face_occupancy(best[2])
-    return best, timings
+    shape_t = {}
+    for shape in dict.fromkeys(tuple(shapes) + ((32, 32),)):
+        solver.set_tile_shape(*shape)
+        _time_steps(solver, 1)
+        shape_t[shape] = _time_steps(solver, steps)
+        timings[best + (shape,)] = shape_t[shape]
+    shape = min(shape_t, key=shape_t.get)
+    solver.set_tile_shape(*shape)
+    return best + (shape,), timings
 
 
 BATCH_BANDS = (16, 32, 64, 128)

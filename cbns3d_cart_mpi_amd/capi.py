@@ -83,6 +83,7 @@ _SIGS = {
     "cbns_set_viscous_workgroups": (_I, [_VP, _I]),
     "cbns_set_band": (_I, [_VP, _I]),
     "cbns_set_face_occupancy": (_I, [_VP, _I]),
+    "cbns_set_tile_shape": (_I, [_VP, _I, _I]),
     "cbns_kernel_times": (_I, [_VP, _PD, ctypes.POINTER(ctypes.c_int64)]),
     "cbns_batch_create": (_I, [ctypes.c_uint64, _I, ctypes.POINTER(_VP)]),
     "cbns_batch_destroy": (_I, [_VP]),
@@ -292,6 +293,11 @@ class Solver:
 
     def set_face_occupancy(self, waves):
         check(lib().cbns_set_face_occupancy(self._h, int(waves)), "cbns_set_face_occupancy")
+
+    def set_tile_shape(self, visc_planes, run_cells):
+        """Most planes per viscous z-chunk and cells per run of the fused z
+        pass and the y-face march (0: the default, 32)."""
+        check(lib().cbns_set_tile_shape(self._h, int(visc_planes), int(run_cells)), "cbns_set_tile_shape")
 
     def set_overlap(self, on):
         check(lib().cbns_set_overlap(self._h, int(bool(on))), "cbns_set_overlap")

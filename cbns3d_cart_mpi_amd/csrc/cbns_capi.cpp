@@ -33,6 +33,8 @@ struct cbns_ctx {
   int visc_wgs = 0;  // with overlap: workgroups of the side-stream viscous kernel (0 = one per unit)
   int jband = 16;  // rows per band of the stencil kernels' work order
   int face_occ = 2;  // register-allocation variant of the FP64 face kernel
+  int visc_top = VZ;  // most planes per viscous z-chunk
+  int run_top = ZM;   // most cells per run of the fused z pass and the y-face march
   unsigned long long* d_smax = nullptr;
   int* d_flag = nullptr;
   double* d_diag = nullptr;  // partials then the two sums
@@ -622,7 +624,7 @@ static int launch_sweeps(cbns_ctx* c, int stage) {
     CK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     CK(timed_on(c, c->side, 3, [&] {
       return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->side, c->visc_wgs,
-                              c->precision == CBNS_FP64_FAST ? in : -1);
+                              c->precision == CBNS_FP64_FAST ? in : -1, c->visc_top);
     }));
     CK(hipEventRecord(c->ev_join, c->side));
   }
@@ -635,7 +637,9 @@ static int launch_sweeps(cbns_ctx* c, int stage) {
                               c->fix, c->stream);
     }));
     CK(launch_face_fixup(c->db, hb, c->dom.nb, in, c->ph, c->fix, c->stream, 1));
-    CK(timed(c, 1, [&] { return launch_yface_xy(c->db, hb, c->dom.nb, in, c->ph, c->face_occ, c->stream); }));
+    CK(timed(c, 1, [&] {
+      return launch_yface_xy(c->db, hb, c->dom.nb, in, c->ph, c->face_occ, c->stream, c->run_top);
+    }));
     return CBNS_OK;
   }
   const int nd = zfused(c) ? 2 : 3;
@@ -658,7 +662,8 @@ static int finish_stage(cbns_ctx* c, int stage, double dt) {
   if (c->overlap) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   else
     CK(timed(c, 3, [&] {
-      return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->stream, 0, lean ? in : -1);
+      return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->stream, 0, lean ? in : -1,
+                              c->visc_top);
     }));
   bool s_done = false;
   if (stage == 3) {  // the next step's calc_dt reduction rides on the last update
@@ -670,7 +675,7 @@ static int finish_stage(cbns_ctx* c, int stage, double dt) {
     int* fl = stage == 3 ? c->d_flag : nullptr;
     CK(timed(c, 4, [&] {
       return launch_zface_update(c->db, hb, c->dom.nb, c->precision, in, out, stage, dt, c->ph, c->ref, sm, fl,
-                                 c->cfix, c->stream);
+                                 c->cfix, c->stream, c->run_top);
     }));
     if (c->precision != CBNS_FP32_FLUX)
       CK(launch_zcell_fixup(c->db, hb, c->dom.nb, c->precision, in, out, stage, dt, c->ph, sm, fl, c->cfix,
@@ -785,6 +790,15 @@ int cbns_set_band(cbns_ctx* c, int rows) {
 int cbns_set_face_occupancy(cbns_ctx* c, int waves) {
   if (!c || (waves != 2 && waves != 3)) return CBNS_ERR_ARG;
   c->face_occ = waves;
+  return CBNS_OK;
+}
+
+int cbns_set_tile_shape(cbns_ctx* c, int visc_planes, int run_cells) {
+  if (!c) return CBNS_ERR_ARG;
+  if (visc_planes != 0 && (visc_planes < VISC_LOW || visc_planes > 256)) return CBNS_ERR_ARG;
+  if (run_cells != 0 && (run_cells < ZM_LOW || run_cells > 256)) return CBNS_ERR_ARG;
+  c->visc_top = visc_planes ? visc_planes : VZ;
+  c->run_top = run_cells ? run_cells : ZM;
   return CBNS_OK;
 }
 

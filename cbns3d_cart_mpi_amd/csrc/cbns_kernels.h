@@ -125,13 +125,13 @@ FixList fixlist_at(void* buf);
 struct RefState;
 hipError_t launch_zface_update(const BlockDesc* db, const BlockDesc* hb, int nb, int precision, int in, int out,
                                int stage, double dt, const Phys& ph, const RefState& ref, unsigned long long* smax,
-                               int* flag, const FixList& cfx, hipStream_t s);
+                               int* flag, const FixList& cfx, hipStream_t s, int zm_top = 32);
 // Tolerance-bound form: its y faces swept by a march along y that stores
 // each interior cell's x- plus y-flux differences into F[1] (after the x
 // sweep and its face fixup); cells beside a failing y face are marked for
 // the fused z pass to leave to the cell fixup.
 hipError_t launch_yface_xy(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph, int occ,
-                           hipStream_t s);
+                           hipStream_t s, int ym_top = 32);
 hipError_t launch_zcell_fixup(const BlockDesc* db, const BlockDesc* hb, int nb, int precision, int in, int out,
                               int stage, double dt, const Phys& ph, unsigned long long* smax, int* flag,
                               const FixList& cfx, hipStream_t s);
@@ -152,9 +152,9 @@ hipError_t launch_viscous(const BlockDesc* db, const BlockDesc* hb, int nb, cons
                           hipStream_t s);
 // wgs > 0: a fixed number of workgroups taking the tile units in turn;
 // from_state >= 0 (FP64 only): u, v, w, T formed from Q[from_state] instead
-// of read from the stored primitives
+// of read from the stored primitives; vz_top: most planes per z-chunk
 hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision,
-                            int jb, hipStream_t s, int wgs = 0, int from_state = -1);
+                            int jb, hipStream_t s, int wgs = 0, int from_state = -1, int vz_top = 32);
 // smax/flag (optional): also reduce the next step's calc_dt spectral radius of
 // the updated cells (only over x-pairs, for blocks of even n0; *s_done says
 // whether it was done).
@@ -179,7 +179,7 @@ hipError_t launch_fill_mu(const BlockDesc* db, const BlockDesc* hb, int nb, doub
 // Viscous tile march: VX x VY interior columns per workgroup (VX * VY
 // threads), at most VZ planes per z-chunk (fewer where a grid has too few
 // column tiles to fill the device: chunk_len).
-constexpr int VX = 16, VY = 16, VZ = 32;
+constexpr int VX = 16, VY = 16, VZ = 32;  // VZ: the launchers' default vz_top
 // Planes per chunk of a march over `tiles` column tiles (of all blocks it
 // covers) and n2 planes: top, halved (down to low) until there are at least
 // `want` (tile, chunk) units.  The single solve asks 1024 units of its
@@ -194,7 +194,7 @@ constexpr int VISC_LOW = 4;    // fewest planes per viscous chunk
 // Fused z-face and stage-tail pass (cbns_zfuse.h): ZX x ZY columns per
 // workgroup, one per thread, each with a run of at most ZM and at least
 // ZM_LOW cells along z (chunk_len).
-constexpr int ZX = 64, ZY = 4, ZM = 32, ZM_LOW = 2;
+constexpr int ZX = 64, ZY = 4, ZM = 32, ZM_LOW = 2;  // ZM: the launchers' default run length
 constexpr int BATCH_UNITS = 256;  // units per block the batched marches ask for
 constexpr int GSUM_WG = 512;  // workgroups of the fixed-order global sums
 hipError_t launch_global_sums(const BlockDesc* db, const Domain& dom, const Phys& ph, int what, double* d_part,

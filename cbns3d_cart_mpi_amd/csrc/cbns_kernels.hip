@@ -159,14 +159,14 @@ __global__ __launch_bounds__(VX * VY) void k_visc_flux(const BlockDesc* __restri
   visc_tile_body<VT, FROMQ>(blocks[blockIdx.z], ph, tl.x, tl.y, which, vz);
 }
 
-static int visc_chunk_planes(const BlockDesc* hb, int nb) {
+static int visc_chunk_planes(const BlockDesc* hb, int nb, int top) {
   long long tiles = 0;
   int n2 = 0;
   for (int b = 0; b < nb; b++) {
     tiles = std::max(tiles, (long long)((hb[b].n[0] + VX - 1) / VX) * ((hb[b].n[1] + VY - 1) / VY));
     n2 = std::max(n2, hb[b].n[2]);
   }
-  return chunk_len(tiles * nb, n2, VZ, VISC_LOW, 1024);
+  return chunk_len(tiles * nb, n2, top, VISC_LOW, 1024);
 }
 
 static dim3 visc_grid(const BlockDesc* hb, int nb, int vz) {
@@ -194,9 +194,9 @@ __global__ __launch_bounds__(VX * VY) void k_visc_flux_units(const BlockDesc* __
 }
 
 hipError_t launch_visc_flux(const BlockDesc* db, const BlockDesc* hb, int nb, const Phys& ph, int precision,
-                            int jb, hipStream_t s, int wgs, int from_state) {
+                            int jb, hipStream_t s, int wgs, int from_state, int vz_top) {
   (void)jb;  // the tile march fixes its own order
-  const int vz = visc_chunk_planes(hb, nb);
+  const int vz = visc_chunk_planes(hb, nb, vz_top);
   const dim3 grid = visc_grid(hb, nb, vz);
   const bool fromq = from_state >= 0;  // u, v, w, T formed from Q[from_state]
   const int w = fromq ? from_state : 0;

@@ -42,13 +42,13 @@ __global__ __launch_bounds__(ZX * ZY, 2) void k_zface_update(const BlockDesc* __
 
 hipError_t launch_zface_update(const BlockDesc* db, const BlockDesc* hb, int nb, int precision, int in, int out,
                                int stage, double dt, const Phys& ph, const RefState& ref, unsigned long long* smax,
-                               int* flag, const FixList& cfx, hipStream_t s) {
+                               int* flag, const FixList& cfx, hipStream_t s, int zm_top) {
   int tiles = 0, n2 = 0;
   for (int b = 0; b < nb; b++) {
     tiles = std::max(tiles, ((hb[b].n[0] + ZX - 1) / ZX) * ((hb[b].n[1] + ZY - 1) / ZY));
     n2 = std::max(n2, hb[b].n[2]);
   }
-  const int zm = chunk_len((long long)tiles * nb, n2, ZM, ZM_LOW, 1024);
+  const int zm = chunk_len((long long)tiles * nb, n2, zm_top, ZM_LOW, 1024);
   const dim3 grid(tiles, (n2 + zm - 1) / zm, nb), blk(ZX * ZY);
   if (precision == PREC_FP64_FAST)
     hipLaunchKernelGGL(k_zface_update<ZF_FAST>, grid, blk, 0, s, db, in, out, stage, dt, ph, ref, smax, flag, cfx,
@@ -82,13 +82,13 @@ __global__ __launch_bounds__(ZX * ZY, OCC) void k_yface_xy(const BlockDesc* __re
 }
 
 hipError_t launch_yface_xy(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph, int occ,
-                           hipStream_t s) {
+                           hipStream_t s, int ym_top) {
   int tiles = 0, n1 = 0;
   for (int b = 0; b < nb; b++) {
     tiles = std::max(tiles, ((hb[b].n[0] + ZX - 1) / ZX) * ((hb[b].n[2] + ZY - 1) / ZY));
     n1 = std::max(n1, hb[b].n[1]);
   }
-  const int ym = chunk_len((long long)tiles * nb, n1, ZM, ZM_LOW, 1024);
+  const int ym = chunk_len((long long)tiles * nb, n1, ym_top, ZM_LOW, 1024);
   const dim3 grid(tiles, (n1 + ym - 1) / ym, nb), blk(ZX * ZY);
   if (occ >= 3) hipLaunchKernelGGL(k_yface_xy<3>, grid, blk, 0, s, db, which, ph, ym);
   else hipLaunchKernelGGL(k_yface_xy<2>, grid, blk, 0, s, db, which, ph, ym);

@@ -15,9 +15,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 class FakeSolver:
     """Records the knobs and advances a clock by a cost that depends on them."""
 
-    def __init__(self, cost):
+    def __init__(self, cost, shape_cost=lambda shape: 0.0):
         self.cost = cost
-        self.band, self.overlap, self.occ = 16, True, 2
+        self.shape_cost = shape_cost
+        self.band, self.overlap, self.occ, self.shape = 16, True, 2, (32, 32)
         self.steps = 0
         self.now = 0.0
 
@@ -33,12 +34,15 @@ class FakeSolver:
     def set_viscous_workgroups(self, n):
         self.vw = n
 
+    def set_tile_shape(self, planes, cells):
+        self.shape = (planes or 32, cells or 32)
+
     def sync(self):
         pass
 
     def run(self, n):
         self.steps += n
-        self.now += n * self.cost(self.band, self.overlap, self.occ)
+        self.now += n * (self.cost(self.band, self.overlap, self.occ) + self.shape_cost(self.shape))
 
 
 @pytest.fixture
@@ -53,14 +57,17 @@ def test_tune_tries_every_choice_and_keeps_the_fastest(clock):
     def cost(band, overlap, occ):
         return 1.0 + abs(band - 32) / 100 + (0.0 if overlap else 0.05) + (0.02 if occ == 3 else 0.0)
 
-    s = FakeSolver(cost)
+    s = FakeSolver(cost, lambda shape: 0.01 * (shape != (64, 32)))
     clock["obj"] = s
     best, tms = autotune.tune(s, steps=3)
-    assert len(tms) == len(autotune.BANDS) * len(autotune.SCHEDULES) * 2
-    assert best == (32, "overlap", 2)
-    assert (s.band, s.overlap, s.occ) == (32, True, 2)
+    launch = [k for k in tms if len(k) == 3]
+    assert len(launch) == len(autotune.BANDS) * len(autotune.SCHEDULES) * 2
+    assert len(tms) - len(launch) == len(set(autotune.SHAPES) | {(32, 32)})
+    assert best == (32, "overlap", 2, (64, 32))
+    assert (s.band, s.overlap, s.occ, s.shape) == (32, True, 2, (64, 32))
     assert s.steps == 4 * len(tms)  # one settling step + `steps` timed steps per choice
-    assert tms[(32, "overlap", 2)] == pytest.approx(1.0)
+    assert tms[(32, "overlap", 2)] == pytest.approx(1.01)  # the first pass runs the default shape
+    assert tms[(32, "overlap", 2, (64, 32))] == pytest.approx(1.0)
 
 
 def test_tune_prefers_serial_within_the_margin(clock):
