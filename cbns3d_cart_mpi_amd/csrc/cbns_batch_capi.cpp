@@ -500,6 +500,7 @@ int cbns_batch_add(cbns_batch* b, const cbns_params* p, const int dims[3], int p
         B.sn = (long long)B.n[0] * B.n[1] * B.n[2];
         for (int d = 0; d < 3; d++)
           B.sf[d] = (long long)(B.n[0] + (d == 0)) * (B.n[1] + (d == 1)) * (B.n[2] + (d == 2));
+        set_face_divisors(B);
         ok = (B.Q[0] = alloc(8 * NEQ * B.sg)) && (B.Q[1] = alloc(8 * NEQ * B.sg)) && (B.Q[2] = alloc(8 * NEQ * B.sg)) &&
              (B.F[0] = alloc(8 * NEQ * B.sf[0])) && (B.F[1] = alloc(8 * NEQ * B.sf[1])) &&
              (B.F[2] = alloc(8 * NEQ * B.sf[2])) && (B.prim = alloc((precision == CBNS_FP32_FLUX ? 4 : 8) * 6 * B.sg)) && (B.mu = alloc(8 * B.sn)) &&

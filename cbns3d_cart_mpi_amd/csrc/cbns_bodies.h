@@ -56,6 +56,37 @@ __device__ __forceinline__ void band_decode(int t, int ni, int nj, int nk, int j
   }
 }
 
+__device__ __forceinline__ int fdiv(int n, FastDiv f) {
+  return (int)((__umulhi((unsigned)n, f.m) + (unsigned)n) >> f.l);
+}
+
+// band_decode of the face sweeps: the divisions by ni and nk through the
+// block's FastDiv (fni, fnk), and with jb a power of two those by jb and
+// nk jb as a shift and fnk; other bands take band_decode.
+__device__ __forceinline__ void face_decode(int t, int ni, FastDiv fni, int nj, int nk, FastDiv fnk, int jb, int& i,
+                                            int& j, int& k) {
+  if ((jb & (jb - 1)) != 0) {  // uniform: a launch argument
+    band_decode(t, ni, nj, nk, jb, i, j, k);
+    return;
+  }
+  const int jl = __builtin_ctz(jb);
+  const int R = fdiv(t, fni);
+  i = t - R * ni;
+  const int full = nj >> jl;
+  if (R < full * nk * jb) {
+    // R = (band nk + k) jb + row within the band
+    const int Rb = R >> jl;
+    const int b = fdiv(Rb, fnk);
+    k = Rb - b * nk;
+    j = (b << jl) + (R & (jb - 1));
+  } else {
+    const int last = nj - full * jb;
+    const int rem = R - full * nk * jb;
+    k = rem / last;
+    j = full * jb + (rem - k * last);
+  }
+}
+
 // x / h as divc, or with R (the tolerance-bound form's viscous terms and
 // flux differences) x times RN(1/h): within about one ulp of the quotient,
 // in one operation instead of three.
@@ -84,6 +115,14 @@ __device__ __forceinline__ int face_count(const BlockDesc& B, int D) {
   return (B.n[0] + (D == 0)) * (B.n[1] + (D == 1)) * (B.n[2] + (D == 2));
 }
 
+// Work index t -> (i, j, k) of single face t along D (band order).
+template <int D>
+__device__ __forceinline__ void single_face_decode(const BlockDesc& B, int t, int jb, int& i, int& j, int& k) {
+  const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
+  if constexpr (D == 2) band_decode(t, ni, nj, nk, jb, i, j, k);  // single z faces: the contracted form's sweep
+  else face_decode(t, ni, B.dv_row[D == 0 ? 0 : 1], nj, nk, B.dv_plane[0], jb, i, j, k);
+}
+
 // ---- inviscid face flux: rec_riemann_x/y/z (src/Block3d.cpp:555-748) ----
 // Face (i,j,k) along D sits between cells i-1 and i; t < face_count(B, D).
 // PT: storage type of the primitives (double, or float in the FP32-flux form).
@@ -92,7 +131,7 @@ __device__ __forceinline__ void face_load(const BlockDesc& B, int t, int which, 
                                           double (&pl)[5], double (&pr)[5], int& fi) {
   const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
   int i, j, k;
-  band_decode(t, ni, nj, nk, jb, i, j, k);
+  single_face_decode<D>(B, t, jb, i, j, k);
   fi = i + ni * (j + nj * k);
   const int c0 = gidx(B, i, j, k);
   const int sD = D == 0 ? 1 : (D == 1 ? B.g[0] : B.g[0] * B.g[1]);
@@ -145,9 +184,8 @@ __device__ __forceinline__ void face_body(const BlockDesc& B, int t, int which, 
   face_flux<D, FMA, !FMA>(q, pl, pr, ph.gamma, ph.gam1, ph.rgam1, ph.fgam1, fl, ok);
   if constexpr (!FMA) {
     if (!ok) {
-      const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
       int i, j, k;
-      band_decode(t, ni, nj, nk, jb, i, j, k);
+      single_face_decode<D>(B, t, jb, i, j, k);
       face_flux_ieee<D>(B, which, gidx(B, i, j, k), D == 0 ? 1 : (D == 1 ? B.g[0] : B.g[0] * B.g[1]), ph, fl);
     }
   }
@@ -186,10 +224,10 @@ template <int D>
 __device__ __forceinline__ void pair_decode(const BlockDesc& B, int t, int jb, int& i, int& j, int& k) {
   const int ni = B.n[0], nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
   if (D == 1) {
-    band_decode(t, ni, (nj + 1) / 2, nk, jb, i, j, k);
+    face_decode(t, ni, B.dv_row[1], (nj + 1) / 2, nk, B.dv_plane[0], jb, i, j, k);
     j *= 2;
   } else {
-    band_decode(t, ni, nj, (nk + 1) / 2, jb, i, j, k);
+    face_decode(t, ni, B.dv_row[1], nj, (nk + 1) / 2, B.dv_plane[1], jb, i, j, k);
     k *= 2;
   }
 }

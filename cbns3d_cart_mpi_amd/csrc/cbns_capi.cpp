@@ -263,6 +263,7 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
         B.sn = (long long)B.n[0] * B.n[1] * B.n[2];
         for (int d = 0; d < 3; d++)
           B.sf[d] = (long long)(B.n[0] + (d == 0)) * (B.n[1] + (d == 1)) * (B.n[2] + (d == 2));
+        set_face_divisors(B);
         void* q;
         if ((st = dev_alloc(c, &q, sizeof(double) * NEQ * B.sg))) break;
         B.Q[0] = (double*)q;

@@ -290,9 +290,9 @@ __device__ __forceinline__ void load_face_cells(const BlockDesc& B, int which, i
 template <int D>
 __device__ __forceinline__ void face_body_fast(const BlockDesc& B, int b, int t, int which, int jb, const Phys& ph,
                                                const FixList& fx) {
-  const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
+  const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1);
   int i, j, k;
-  band_decode(t, ni, nj, nk, jb, i, j, k);
+  single_face_decode<D>(B, t, jb, i, j, k);
   const int fi = i + ni * (j + nj * k);
   double q[6][5], fl[5];
   load_face_cells<D>(B, which, gidx(B, i, j, k), q);

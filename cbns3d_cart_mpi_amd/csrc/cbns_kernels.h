@@ -35,6 +35,23 @@ struct Phys {
   double gg1;  // gamma / (gamma - 1), for the tolerance-bound face form
 };
 
+// Division of an n < 2^31 by a run-time constant d >= 1 in three
+// instructions: q = (mulhi(n, m) + n) >> l with l = ceil(log2 d) and
+// m = floor(2^32 (2^l - d) / d) + 1 (the round-up method of Granlund and
+// Montgomery, "Division by invariant integers using multiplication", PLDI
+// 1994; the add does not overflow for n < 2^31).
+struct FastDiv {
+  unsigned m, l;
+};
+inline FastDiv make_fastdiv(unsigned d) {
+  unsigned l = 0;
+  while ((1ull << l) < d) l++;
+  FastDiv f;
+  f.m = (unsigned)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+  f.l = l;
+  return f;
+}
+
 // One on-device block: the reference's Block3d arrays, SoA.
 struct BlockDesc {
   double* Q[3];   // Q, Q_p and (tolerance-bound single solve) the stage-1 state: [5][g2][g1][g0], ghosted
@@ -49,7 +66,18 @@ struct BlockDesc {
   long long sg;   // ghosted field stride
   long long sf[3];// face-array field strides
   long long sn;   // interior field stride
+  // the face sweeps' work-index divisors: row lengths n0 + 1 (x faces) and
+  // n0 (y and z face pairs), plane counts n2 (x faces, y pairs) and
+  // (n2 + 2) / 2 (z pairs); set by set_face_divisors
+  FastDiv dv_row[2], dv_plane[2];
 };
+
+inline void set_face_divisors(BlockDesc& B) {
+  B.dv_row[0] = make_fastdiv(B.n[0] + 1);
+  B.dv_row[1] = make_fastdiv(B.n[0]);
+  B.dv_plane[0] = make_fastdiv(B.n[2]);
+  B.dv_plane[1] = make_fastdiv((B.n[2] + 2) / 2);
+}
 
 // Partition of the periodic domain (src/cbns3d.cpp:102-126).
 struct Domain {

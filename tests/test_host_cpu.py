@@ -134,3 +134,59 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_fastdiv_and_the_face_decode_restated():
+    """The face sweeps divide work indices by the block's row lengths and
+    plane counts with the round-up multiply-high method (FastDiv in
+    csrc/cbns_kernels.h) and take bands of a power-of-two height by shifts
+    (face_decode); restated here, both equal the integer divisions and the
+    generic band order for every numerator below 2^31 tried."""
+    import random
+
+    def make(d):
+        l = 0
+        while (1 << l) < d:
+            l += 1
+        m = ((1 << 32) * ((1 << l) - d)) // d + 1
+        assert m < (1 << 32)
+        return m, l
+
+    def fdiv(n, f):
+        return (((n * f[0]) >> 32) + n) >> f[1]
+
+    rnd = random.Random(7)
+    for d in list(range(1, 600)) + [rnd.randrange(1, 1 << 20) for _ in range(500)]:
+        f = make(d)
+        for n in [0, 1, d - 1, d, d + 1, (1 << 31) - 1] + [rnd.randrange(0, 1 << 31) for _ in range(100)]:
+            assert fdiv(n, f) == n // d, (n, d)
+
+    def band(t, ni, nj, nk, jb):  # band_decode
+        i, R = t % ni, t // ni
+        full = nj // jb
+        if R < full * nk * jb:
+            b = R // (nk * jb)
+            rem = R - b * nk * jb
+            k = rem // jb
+            return i, b * jb + rem - k * jb, k
+        last, rem = nj - full * jb, R - full * nk * jb
+        k = rem // last
+        return i, full * jb + rem - k * last, k
+
+    def fast(t, ni, nj, nk, jb):  # face_decode
+        jl = jb.bit_length() - 1
+        R = fdiv(t, make(ni))
+        i = t - R * ni
+        full = nj >> jl
+        if R < full * nk * jb:
+            Rb = R >> jl
+            b = fdiv(Rb, make(nk))
+            return i, (b << jl) + (R & (jb - 1)), Rb - b * nk
+        last, rem = nj - full * jb, R - full * nk * jb
+        k = rem // last
+        return i, full * jb + rem - k * last, k
+
+    for ni, nj, nk in [(257, 256, 256), (21, 16, 12), (33, 17, 9), (5, 4, 4), (129, 65, 3)]:
+        for jb in (1, 2, 4, 8, 16, 32, 64, 128):
+            for t in range(0, ni * nj * nk, max(1, ni * nj * nk // 3000)):
+                assert band(t, ni, nj, nk, jb) == fast(t, ni, nj, nk, jb), (ni, nj, nk, jb, t)
