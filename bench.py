@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--band", default="auto", help="rows per work band, or 'auto' to tune")
     ap.add_argument("--schedule", default="auto",
                     help="stage schedule: overlap | overlap:N (N viscous workgroups) | serial, or 'auto' to tune")
+    ap.add_argument("--shape", default="32,32",
+                    help="viscous planes per chunk and cells per run, with given --band/--schedule/--occ")
     ap.add_argument("--occ", default="auto", choices=["auto", "2", "3"],
                     help="face-kernel waves per SIMD of the pair sweeps, or 'auto' to tune")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -311,10 +313,13 @@ def serve_leg(n):
 def zfused_bytes_per_cell(precision):
     """Algorithmic HBM bytes per cell of one launch of the fused z-face and
     stage-tail kernel, averaged over the three stages: the x- and y-face
-    fluxes (5 each), the viscous divergence (4), the stage input (5, read
-    once along the column's march), Q^n (5, stages 2 and 3 only) and the
-    updated state (5 written); the bit-exact form also writes 6 primitives."""
-    b = 8 * (5 + 5 + 4 + 5 + 5) + 8 * 5 * 2 / 3
+    fluxes (5 each; the tolerance-bound form reads instead the 5 x- plus
+    y-flux differences its y-face march stored), the viscous divergence (4),
+    the stage input (5, read once along the column's march), Q^n (5, stages
+    2 and 3 only) and the updated state (5 written); the bit-exact form also
+    writes 6 primitives."""
+    fluxes = 5 if precision == 3 else 10
+    b = 8 * (fluxes + 4 + 5 + 5) + 8 * 5 * 2 / 3
     return b + (48 if precision == 0 else 0)
 
 
@@ -341,11 +346,13 @@ def plain_leg(C, N, dims, precision, steps, warmup, fixed=None):
                  "ms_per_step (visc planes/run cells)": {f"{k[3][0]}/{k[3][1]}": round(v * 1e3, 3)
                                                          for k, v in tms.items() if len(k) == 4}}
     else:  # launch shapes given on the command line (profiling runs)
-        band, sched, occ = fixed
+        band, sched, occ, shape = fixed
         s.set_band(band)
         autotune.apply_schedule(s, sched)
         s.set_face_occupancy(occ)
-        tuned = {"band": band, "schedule": sched, "face_waves_per_simd": occ, "given": True}
+        s.set_tile_shape(*shape)
+        tuned = {"band": band, "schedule": sched, "face_waves_per_simd": occ, "visc_planes_run_cells": list(shape),
+                 "given": True}
     sweeps = ("face_x", "face_y", "face_z")
     s.sync()
     s.set_profiling_classes(sweeps)  # live HIP-event timing of the roofline kernel over the timed steps
@@ -415,7 +422,8 @@ def main():
     # after one step); the bit-exact plain path is measured beside it
     given = "auto" not in (a.band, a.schedule, a.occ)
     # given shapes: --occ applies to the headline form, the bit-exact form keeps 2 waves per SIMD
-    fixed = lambda occ: ((int(a.band), a.schedule, occ) if given else None)
+    shape = tuple(int(x) for x in a.shape.split(","))
+    fixed = lambda occ: ((int(a.band), a.schedule, occ, shape) if given else None)
     if not given and any(x != "auto" for x in (a.band, a.schedule, a.occ)):
         sys.exit("give all of --band, --schedule and --occ, or none")
     head, s = plain_leg(C, N, dims, C.CBNS_FP64_FAST, a.steps, a.warmup, fixed(int(a.occ) if given else 3))
@@ -445,9 +453,10 @@ def main():
     fr = head["face_roofline"]
     roof = {"bound": "valu", "achieved": fr["achieved"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": fr["frac"], "traffic": read_traffic(),
-            "kernel": "k_face_flux_fast (x and y sweeps; tolerance-bound FP64 form); FP64 VALU-issue bound; "
+            "kernel": "tolerance-bound FP64 face sweeps: k_face_flux_fast (x faces) and k_yface_xy (the y-face "
+                      "march, which also stores each cell's x+y flux differences); FP64 VALU-issue bound; "
                       "achieved = 1570 algorithmic flop per face (literal count of the reference) x faces per "
-                      "launch / average launch duration of the sweeps in the timed steps; the z faces are in "
+                      "launch / average launch duration of the two in the timed steps; the z faces are in "
                       "the fused stage tail (fused_z_update)",
             "avg_launch_ms": fr["avg_launch_ms"],
             "algorithmic_flop_per_launch": FLOP_PER_FACE * (N + 1) * N * N,
