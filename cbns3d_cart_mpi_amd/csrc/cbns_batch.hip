@@ -123,7 +123,7 @@ __global__ __launch_bounds__(TPB, 2) void kb_yface_xy(const BlockDesc* __restric
     const InstDesc& I = insts[B.inst];
     if (!I.stepping) return;
     const int ntx = (B.n[0] + ZX - 1) / ZX, nt = ntx * ((B.n[2] + ZY - 1) / ZY);
-    const int ym = chunk_len(nt, B.n[1], ZM, ZM_LOW, BATCH_UNITS);
+    const int ym = chunk_len(nt, B.n[1], BATCH_ZM, ZM_LOW, BATCH_UNITS);
     const int tile = tr.tile % nt, run = tr.tile / nt;
     const int i = (tile % ntx) * ZX + (int)threadIdx.x % ZX, k = (tile / ntx) * ZY + (int)threadIdx.x / ZX;
     const int j0 = run * ym;
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(TPB, sizeof(VT) == 4 ? 3 : 2) void kb_visc(const Bl
     if (!I.stepping) return;
     // one (column tile, z-chunk) unit of the viscous tile march per entry
     const int ntiles = ((B.n[0] + VX - 1) / VX) * ((B.n[1] + VY - 1) / VY);
-    const int vz = chunk_len(ntiles, B.n[2], VZ, VISC_LOW, BATCH_UNITS);
+    const int vz = chunk_len(ntiles, B.n[2], BATCH_VZ, VISC_LOW, BATCH_UNITS);
     visc_tile_body<VT, FROMQ>(B, I.ph, tr.tile % ntiles, tr.tile / ntiles, which, vz);
   });
 }
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(TPB, 2) void kb_zface_update(const BlockDesc* __res
       bad = 0;
     }
     const int ntx = (B.n[0] + ZX - 1) / ZX, nt = ntx * ((B.n[1] + ZY - 1) / ZY);
-    const int zm = chunk_len(nt, B.n[2], ZM, ZM_LOW, BATCH_UNITS);
+    const int zm = chunk_len(nt, B.n[2], BATCH_ZM, ZM_LOW, BATCH_UNITS);
     const int tile = tr.tile % nt, run = tr.tile / nt;
     const int i = (tile % ntx) * ZX + (int)threadIdx.x % ZX, j = (tile / ntx) * ZY + (int)threadIdx.x / ZX;
     const int k0 = run * zm;

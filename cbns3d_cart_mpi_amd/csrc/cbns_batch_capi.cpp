@@ -246,16 +246,16 @@ static int rebuild(cbns_batch* b) {
       for (int d = 0; d < (fast ? 1 : 2); d++) add(FACE_CLASS[face_form(I.precision)] + d, face_items(B, d));
       if (fast) {
         const int yt = ((B.n[0] + ZX - 1) / ZX) * ((B.n[2] + ZY - 1) / ZY);
-        const int ym = chunk_len(yt, B.n[1], ZM, ZM_LOW, BATCH_UNITS);
+        const int ym = chunk_len(yt, B.n[1], BATCH_ZM, ZM_LOW, BATCH_UNITS);
         for (int t = 0; t < yt * ((B.n[1] + ym - 1) / ym); t++) tab[T_YMF].push_back({bi, t});
       }
       // whole-workgroup units of the viscous march and of the fused stage
       // tail: (column tile, z-chunk), the chunk lengths as the kernels form them
       const int vt = ((B.n[0] + VX - 1) / VX) * ((B.n[1] + VY - 1) / VY);
-      const int vz = chunk_len(vt, B.n[2], VZ, VISC_LOW, BATCH_UNITS);
+      const int vz = chunk_len(vt, B.n[2], BATCH_VZ, VISC_LOW, BATCH_UNITS);
       for (int t = 0; t < vt * ((B.n[2] + vz - 1) / vz); t++) tab[VISC_CLASS[face_form(I.precision)]].push_back({bi, t});
       const int zt = ((B.n[0] + ZX - 1) / ZX) * ((B.n[1] + ZY - 1) / ZY);
-      const int zm = chunk_len(zt, B.n[2], ZM, ZM_LOW, BATCH_UNITS);
+      const int zm = chunk_len(zt, B.n[2], BATCH_ZM, ZM_LOW, BATCH_UNITS);
       for (int t = 0; t < zt * ((B.n[2] + zm - 1) / zm); t++) tab[UPD_CLASS[face_form(I.precision)]].push_back({bi, t});
       add(T_GHOST, B.sg - B.sn);  // ghost cells only
       add(T_DT, B.sn);

@@ -223,7 +223,9 @@ constexpr int VISC_LOW = 4;    // fewest planes per viscous chunk
 // workgroup, one per thread, each with a run of at most ZM and at least
 // ZM_LOW cells along z (chunk_len).
 constexpr int ZX = 64, ZY = 4, ZM = 32, ZM_LOW = 2;  // ZM: the launchers' default run length
-constexpr int BATCH_UNITS = 256;  // units per block the batched marches ask for
+constexpr int BATCH_UNITS = 64;  // units per block the batched marches ask for
+// most viscous planes per chunk and cells per run of the batched marches
+constexpr int BATCH_VZ = 32, BATCH_ZM = 32;
 constexpr int GSUM_WG = 512;  // workgroups of the fixed-order global sums
 hipError_t launch_global_sums(const BlockDesc* db, const Domain& dom, const Phys& ph, int what, double* d_part,
                               double* d_out, int precision, hipStream_t s);
