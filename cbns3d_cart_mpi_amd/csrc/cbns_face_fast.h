@@ -72,15 +72,22 @@ __device__ __forceinline__ void weno5_side(double a, double b, double c, double 
   dmax = fmax(dmax, den);
 }
 
+// The smoothness indicators are formed times 4: 4 IS = (4 (13/12) s) s +
+// 4 eps + e^2, every operation of which is the reference-order one scaled
+// by a power of two, hence exact (no 0.25 e product); the weights' numerator
+// and denominator then carry a factor 256 that cancels in their quotient
+// (the reciprocal of a power-of-two multiple is that power's multiple), so
+// every value equals the unscaled form's, and dmax is 256 times the
+// unscaled denominator.
 __device__ __forceinline__ void weno5_lr(const double (&f)[6], double& wl, double& wr, double& dmax) {
 #pragma clang fp contract(off)
-  const double eps = 1.0e-6, c13 = 13.0 / 12.0;
+  const double eps4 = 4.0 * 1.0e-6, c13x4 = 4.0 * (13.0 / 12.0);
   const double s1 = __builtin_fma(-2.0, f[1], f[0] + f[2]);
   const double s2 = __builtin_fma(-2.0, f[2], f[1] + f[3]);
   const double s3 = __builtin_fma(-2.0, f[3], f[2] + f[4]);
   const double s4 = __builtin_fma(-2.0, f[4], f[3] + f[5]);
-  const double t1 = (c13 * s1) * s1 + eps, t2 = (c13 * s2) * s2 + eps;
-  const double t3 = (c13 * s3) * s3 + eps, t4 = (c13 * s4) * s4 + eps;
+  const double t1 = (c13x4 * s1) * s1 + eps4, t2 = (c13x4 * s2) * s2 + eps4;
+  const double t3 = (c13x4 * s3) * s3 + eps4, t4 = (c13x4 * s4) * s4 + eps4;
   // first-difference terms (diff2, diff4, diff6 of src/weno.cpp:34-39)
   const double e1 = __builtin_fma(3.0, f[2], __builtin_fma(-4.0, f[1], f[0]));  // left:  f0 - 4 f1 + 3 f2
   const double e2 = f[1] - f[3];
@@ -88,10 +95,10 @@ __device__ __forceinline__ void weno5_lr(const double (&f)[6], double& wl, doubl
   const double r1 = __builtin_fma(3.0, f[3], __builtin_fma(-4.0, f[4], f[5]));  // right: f5 - 4 f4 + 3 f3
   const double r2 = f[4] - f[2];
   const double r3 = __builtin_fma(-4.0, f[2], __builtin_fma(3.0, f[3], f[1]));  //        3 f3 - 4 f2 + f1
-  const double ISl1 = __builtin_fma(0.25 * e1, e1, t1), ISl2 = __builtin_fma(0.25 * e2, e2, t2);
-  const double ISl3 = __builtin_fma(0.25 * e3, e3, t3);
-  const double ISr1 = __builtin_fma(0.25 * r1, r1, t4), ISr2 = __builtin_fma(0.25 * r2, r2, t3);
-  const double ISr3 = __builtin_fma(0.25 * r3, r3, t2);
+  const double ISl1 = __builtin_fma(e1, e1, t1), ISl2 = __builtin_fma(e2, e2, t2);
+  const double ISl3 = __builtin_fma(e3, e3, t3);
+  const double ISr1 = __builtin_fma(r1, r1, t4), ISr2 = __builtin_fma(r2, r2, t3);
+  const double ISr3 = __builtin_fma(r3, r3, t2);
   weno5_side(f[0], f[1], f[2], f[3], f[4], ISl1, ISl2, ISl3, wl, dmax);
   weno5_side(f[5], f[4], f[3], f[2], f[1], ISr1, ISr2, ISr3, wr, dmax);
 }
@@ -184,7 +191,7 @@ __device__ __forceinline__ bool face_flux_fast(const double (&q)[6][5], double g
   double wl[5], wr[5], dmax = 0.0;
 #pragma unroll
   for (int m = 0; m < 5; m++) weno5_lr(w[m], wl[m], wr[m], dmax);
-  ok = ok && dmax < 1.0e300;
+  ok = ok && dmax < 256.0 * 1.0e300;  // every unscaled WENO denominator below 1e300
   // ---- back-projection (R's columns) and LF flux
   const double qc = qn * c;
   double U[2][5], F[2][5], wsd[2];
