@@ -80,7 +80,8 @@ def test_tune_prefers_serial_within_the_margin(clock):
         s = FakeSolver(cost)
         clock["obj"] = s
         best, _ = autotune.tune(s, steps=3, occs=(2,))
-        assert best[0] == 32 and best[1] == want, (gain, best)
+        # the three overlapped schedules cost the same here: any of them may win
+        assert best[0] == 32 and best[1].split(":")[0] == want, (gain, best)
         assert s.overlap == (want != "serial")
 
 
