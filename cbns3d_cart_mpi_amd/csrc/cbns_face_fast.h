@@ -73,12 +73,12 @@ __device__ __forceinline__ void weno5_side(double a, double b, double c, double 
 }
 
 // The smoothness indicators are formed times 4: 4 IS = (4 (13/12) s) s +
-// 4 eps + e^2, every operation of which is the reference-order one scaled
-// by a power of two, hence exact (no 0.25 e product); the weights' numerator
-// and denominator then carry a factor 256 that cancels in their quotient
-// (the reciprocal of a power-of-two multiple is that power's multiple), so
-// every value equals the unscaled form's, and dmax is 256 times the
-// unscaled denominator.
+// 4 eps + e^2, with (4 (13/12) s) s + 4 eps in one fused multiply-add and
+// the rest the reference-order operations scaled by a power of two, hence
+// exact (no 0.25 e product); the weights' numerator and denominator then
+// carry a factor 256 that cancels in their quotient (the reciprocal of a
+// power-of-two multiple is that power's multiple), and dmax is 256 times
+// the unscaled denominator.
 __device__ __forceinline__ void weno5_lr(const double (&f)[6], double& wl, double& wr, double& dmax) {
 #pragma clang fp contract(off)
   const double eps4 = 4.0 * 1.0e-6, c13x4 = 4.0 * (13.0 / 12.0);
@@ -86,8 +86,8 @@ __device__ __forceinline__ void weno5_lr(const double (&f)[6], double& wl, doubl
   const double s2 = __builtin_fma(-2.0, f[2], f[1] + f[3]);
   const double s3 = __builtin_fma(-2.0, f[3], f[2] + f[4]);
   const double s4 = __builtin_fma(-2.0, f[4], f[3] + f[5]);
-  const double t1 = (c13x4 * s1) * s1 + eps4, t2 = (c13x4 * s2) * s2 + eps4;
-  const double t3 = (c13x4 * s3) * s3 + eps4, t4 = (c13x4 * s4) * s4 + eps4;
+  const double t1 = __builtin_fma(c13x4 * s1, s1, eps4), t2 = __builtin_fma(c13x4 * s2, s2, eps4);
+  const double t3 = __builtin_fma(c13x4 * s3, s3, eps4), t4 = __builtin_fma(c13x4 * s4, s4, eps4);
   // first-difference terms (diff2, diff4, diff6 of src/weno.cpp:34-39)
   const double e1 = __builtin_fma(3.0, f[2], __builtin_fma(-4.0, f[1], f[0]));  // left:  f0 - 4 f1 + 3 f2
   const double e2 = f[1] - f[3];
