@@ -129,7 +129,7 @@ __device__ __forceinline__ void single_face_decode(const BlockDesc& B, int t, in
 template <int D, typename PT = double>
 __device__ __forceinline__ void face_load(const BlockDesc& B, int t, int which, int jb, double (&q)[6][5],
                                           double (&pl)[5], double (&pr)[5], int& fi) {
-  const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1), nk = B.n[2] + (D == 2);
+  const int ni = B.n[0] + (D == 0), nj = B.n[1] + (D == 1);
   int i, j, k;
   single_face_decode<D>(B, t, jb, i, j, k);
   fi = i + ni * (j + nj * k);

@@ -354,34 +354,43 @@ __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(
 // FP32 arithmetic contracts a * b + c within each expression only (clang's
 // "on"), so the fused operations are fixed by the source and do not depend
 // on the kernel these functions are inlined into.
-__device__ __forceinline__ float weno5f(float Um2, float Um1, float U0, float Up1, float Up2) {
+// One side of a face's WENO5 (src/weno.cpp:25-56) in FP32 from the
+// candidate stencil a..e and its three smoothness indicators (here 4 IS).
+// Weights c_k / IS_k^2 scaled by IS1^2 IS2^2 IS3^2 (the scale cancels in
+// the normalisation): one reciprocal instead of four.  IS >= eps keeps the
+// products from underflowing; while den < 256e30 (every IS below ~1e7, so
+// |f_k| below ~1e4) neither den nor the numerator can overflow.  Past that
+// (or for a NaN) the unscaled weights are used.
+__device__ __forceinline__ float weno5f_side(float a, float b, float c, float d, float e, float IS1, float IS2,
+                                             float IS3) {
 #pragma clang fp contract(on)
-  const float eps = 1.0e-6f;
-  float diff1 = Um2 - 2.0f * Um1 + U0;
-  float diff2 = Um2 - 4.0f * Um1 + 3.0f * U0;
-  float diff3 = Um1 - 2.0f * U0 + Up1;
-  float diff4 = Um1 - Up1;
-  float diff5 = U0 - 2.0f * Up1 + Up2;
-  float diff6 = 3.0f * U0 - 4.0f * Up1 + Up2;
-  float IS1 = (13.0f / 12.0f) * diff1 * diff1 + 0.25f * diff2 * diff2 + eps;
-  float IS2 = (13.0f / 12.0f) * diff3 * diff3 + 0.25f * diff4 * diff4 + eps;
-  float IS3 = (13.0f / 12.0f) * diff5 * diff5 + 0.25f * diff6 * diff6 + eps;
-  float f1 = 2.0f * Um2 - 7.0f * Um1 + 11.0f * U0;
-  float f2 = -Um1 + 5.0f * U0 + 2.0f * Up1;
-  float f3 = 2.0f * U0 + 5.0f * Up1 - Up2;
-  // weights c_k / IS_k^2 scaled by IS1^2 IS2^2 IS3^2 (the scale cancels in
-  // the normalisation): one reciprocal instead of four.  IS >= eps keeps the
-  // products from underflowing; while den < 1e30 (every IS below ~1e7, so
-  // |f_k| below ~1e4) neither den nor the numerator can overflow.  Past that
-  // (or for a NaN) the unscaled weights are used.
-  const float a = IS1 * IS1, b = IS2 * IS2, c = IS3 * IS3;
-  const float s1 = 0.1f * (b * c), s2 = 0.6f * (a * c), s3 = 0.3f * (a * b);
+  const float f1 = 2.0f * a - 7.0f * b + 11.0f * c;
+  const float f2 = -b + 5.0f * c + 2.0f * d;
+  const float f3 = 2.0f * c + 5.0f * d - e;
+  const float A = IS1 * IS1, B = IS2 * IS2, C = IS3 * IS3;
+  const float s1 = 0.1f * (B * C), s2 = 0.6f * (A * C), s3 = 0.3f * (A * B);
   const float den = 6.0f * (s1 + s2 + s3);
-  if (den < 1.0e30f) return (s1 * f1 + s2 * f2 + s3 * f3) * frcp(den);
-  float w1 = 0.1f * frcp(a);
-  float w2 = 0.6f * frcp(b);
-  float w3 = 0.3f * frcp(c);
+  if (den < 256.0f * 1.0e30f) return (s1 * f1 + s2 * f2 + s3 * f3) * frcp(den);
+  const float w1 = 0.1f * frcp(A), w2 = 0.6f * frcp(B), w3 = 0.3f * frcp(C);
   return (w1 * f1 + w2 * f2 + w3 * f3) * frcp(6.0f * (w1 + w2 + w3));
+}
+
+// Left and right FP32 WENO5 reconstructions at one face of one
+// characteristic field f[0..5] (cells i-3 .. i+2): left from f0..f4, right
+// from f5..f1, the second differences centred on cells 1..4 shared by both
+// sides, and the smoothness indicators formed times 4 (4 IS = (4 (13/12) s)
+// s + 4 eps + e^2; the factor cancels in the weights).
+__device__ __forceinline__ void weno5f_lr(const float (&f)[6], float& wl, float& wr) {
+#pragma clang fp contract(on)
+  const float eps4 = 4.0f * 1.0e-6f, c13x4 = 4.0f * (13.0f / 12.0f);
+  const float s1 = f[0] - 2.0f * f[1] + f[2], s2 = f[1] - 2.0f * f[2] + f[3];
+  const float s3 = f[2] - 2.0f * f[3] + f[4], s4 = f[3] - 2.0f * f[4] + f[5];
+  const float t1 = c13x4 * s1 * s1 + eps4, t2 = c13x4 * s2 * s2 + eps4;
+  const float t3 = c13x4 * s3 * s3 + eps4, t4 = c13x4 * s4 * s4 + eps4;
+  const float e1 = f[0] - 4.0f * f[1] + 3.0f * f[2], e2 = f[1] - f[3], e3 = 3.0f * f[2] - 4.0f * f[3] + f[4];
+  const float r1 = f[5] - 4.0f * f[4] + 3.0f * f[3], r2 = f[4] - f[2], r3 = 3.0f * f[3] - 4.0f * f[2] + f[1];
+  wl = weno5f_side(f[0], f[1], f[2], f[3], f[4], e1 * e1 + t1, e2 * e2 + t2, e3 * e3 + t3);
+  wr = weno5f_side(f[5], f[4], f[3], f[2], f[1], r1 * r1 + t4, r2 * r2 + t3, r3 * r3 + t2);
 }
 
 // Deviation of one cell's conserved state from the reference state: FP64
@@ -446,59 +455,68 @@ __device__ __forceinline__ void face_flux_f32_dev(const float (&d)[6][5], const 
   const float q2 = uu * uu + vv * vv + ww * ww;
   const float qn = D == 0 ? uu : (D == 1 ? vv : ww);
   const float ic = frcp(c);
-  const float K = 0.5f * g1 * ic * ic;
+  const float K = 0.5f * g1 * ic * ic;  // (gamma - 1) / (2 c^2)
   const float one_2c = 0.5f * ic;
-  const float L00 = 0.5f * K * q2 + one_2c * qn, L10 = 1.0f - K * q2, L20 = 0.5f * K * q2 - one_2c * qn;
-  const float Ku = K * uu, Kv = K * vv, Kw = K * ww;
-  const float L01 = D == 0 ? -(Ku + one_2c) : -Ku, L21 = D == 0 ? -(Ku - one_2c) : -Ku;
-  const float L02 = D == 1 ? -(Kv + one_2c) : -Kv, L22 = D == 1 ? -(Kv - one_2c) : -Kv;
-  const float L03 = D == 2 ? -(Kw + one_2c) : -Kw, L23 = D == 2 ? -(Kw - one_2c) : -Kw;
-  const float L11 = 2.0f * Ku, L12 = 2.0f * Kv, L13 = 2.0f * Kw, L14 = -2.0f * K;
-  const float Rm0[3] = {D == 0 ? uu - c : uu, D == 1 ? vv - c : vv, D == 2 ? ww - c : ww};
-  const float Rp2[3] = {D == 0 ? uu + c : uu, D == 1 ? vv + c : vv, D == 2 ? ww + c : ww};
-  const float R40 = H - qn * c, R41 = 0.5f * q2, R42 = H + qn * c;
-
+  const float hq2 = 0.5f * q2;
+  // characteristic fields of the six cells' deviations with the reference's
+  // L rows (src/eigenvectors_roe.cpp:138-169) in factored form: with
+  // t = E - u.m + q^2/2 rho, a = K t and b = (qn rho - m_n)/(2c), the
+  // acoustic and entropy fields are a + b, rho - 2a, a - b, the shear fields
+  // the frame rows
+  float w[5][6];
+#pragma unroll
+  for (int cc = 0; cc < 6; cc++) {
+    const float r = d[cc][0], mx = d[cc][1], my = d[cc][2], mz = d[cc][3], e = d[cc][4];
+    const float mn = D == 0 ? mx : (D == 1 ? my : mz);
+    const float t = e + hq2 * r - uu * mx - vv * my - ww * mz;
+    const float a = K * t;
+    const float b = one_2c * (qn * r - mn);
+    w[0][cc] = a + b;
+    w[1][cc] = r - 2.0f * a;
+    w[2][cc] = a - b;
+    if (D == 0) {
+      w[3][cc] = mz - ww * r;
+      w[4][cc] = vv * r - my;
+    } else if (D == 1) {
+      w[3][cc] = ww * r - mz;
+      w[4][cc] = uu * r - mx;
+    } else {
+      w[3][cc] = my - vv * r;
+      w[4][cc] = uu * r - mx;
+    }
+  }
+  float xl[5], xr[5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) weno5f_lr(w[m], xl[m], xr[m]);
+  // back-projection with R's columns (src/eigenvectors_roe.cpp:108-136) in
+  // factored form: the deviations of rho, rho u, rho v, rho w, E
+  const float qc = qn * c;
   float aL[5], aR[5];
 #pragma unroll
-  for (int m = 0; m < 5; m++) {
-    float fc[6];
-#pragma unroll
-    for (int cc = 0; cc < 6; cc++) {
-      const float f0 = d[cc][0], f1 = d[cc][1], f2 = d[cc][2], f3 = d[cc][3], f4 = d[cc][4];
-      if (m == 0) fc[cc] = L00 * f0 + L01 * f1 + L02 * f2 + L03 * f3 + K * f4;
-      else if (m == 1) fc[cc] = L10 * f0 + L11 * f1 + L12 * f2 + L13 * f3 + L14 * f4;
-      else if (m == 2) fc[cc] = L20 * f0 + L21 * f1 + L22 * f2 + L23 * f3 + K * f4;
-      else if (m == 3) fc[cc] = D == 0 ? f3 - ww * f0 : (D == 1 ? ww * f0 - f3 : f2 - vv * f0);
-      else fc[cc] = D == 0 ? vv * f0 - f2 : uu * f0 - f1;
-    }
-    const float wl = weno5f(fc[0], fc[1], fc[2], fc[3], fc[4]);
-    const float wr = weno5f(fc[5], fc[4], fc[3], fc[2], fc[1]);
-    if (m == 0) {
-      aL[0] = wl; aR[0] = wr;
-      aL[1] = Rm0[0] * wl; aR[1] = Rm0[0] * wr;
-      aL[2] = Rm0[1] * wl; aR[2] = Rm0[1] * wr;
-      aL[3] = Rm0[2] * wl; aR[3] = Rm0[2] * wr;
-      aL[4] = R40 * wl; aR[4] = R40 * wr;
-    } else if (m == 1) {
-      aL[0] += wl; aR[0] += wr;
-      aL[1] += uu * wl; aR[1] += uu * wr;
-      aL[2] += vv * wl; aR[2] += vv * wr;
-      aL[3] += ww * wl; aR[3] += ww * wr;
-      aL[4] += R41 * wl; aR[4] += R41 * wr;
-    } else if (m == 2) {
-      aL[0] += wl; aR[0] += wr;
-      aL[1] += Rp2[0] * wl; aR[1] += Rp2[0] * wr;
-      aL[2] += Rp2[1] * wl; aR[2] += Rp2[1] * wr;
-      aL[3] += Rp2[2] * wl; aR[3] += Rp2[2] * wr;
-      aL[4] += R42 * wl; aR[4] += R42 * wr;
-    } else if (m == 3) {
-      if (D == 0) { aL[3] += wl; aR[3] += wr; aL[4] += ww * wl; aR[4] += ww * wr; }
-      else if (D == 1) { aL[3] -= wl; aR[3] -= wr; aL[4] -= ww * wl; aR[4] -= ww * wr; }
-      else { aL[2] += wl; aR[2] += wr; aL[4] += vv * wl; aR[4] += vv * wr; }
+  for (int sd = 0; sd < 2; sd++) {
+    const float* x = sd == 0 ? xl : xr;
+    float* o = sd == 0 ? aL : aR;
+    const float sum = x[0] + x[2], dif = x[2] - x[0];
+    const float rho = sum + x[1];
+    float lm;
+    if (D == 0) {
+      o[1] = uu * rho + c * dif;
+      o[2] = vv * rho - x[4];
+      o[3] = ww * rho + x[3];
+      lm = ww * x[3] - vv * x[4];
+    } else if (D == 1) {
+      o[2] = vv * rho + c * dif;
+      o[3] = ww * rho - x[3];
+      o[1] = uu * rho - x[4];
+      lm = -ww * x[3] - uu * x[4];
     } else {
-      if (D == 0) { aL[2] -= wl; aR[2] -= wr; aL[4] -= vv * wl; aR[4] -= vv * wr; }
-      else { aL[1] -= wl; aR[1] -= wr; aL[4] -= uu * wl; aR[4] -= uu * wr; }
+      o[3] = ww * rho + c * dif;
+      o[2] = vv * rho + x[3];
+      o[1] = uu * rho - x[4];
+      lm = vv * x[3] - uu * x[4];
     }
+    o[0] = rho;
+    o[4] = H * sum + qc * dif + hq2 * x[1] + lm;
   }
   // ---- LF flux minus F(Ug), from the reconstructed deviations
   const float rho_g = (float)g.rho, p_g = (float)g.p, Hg = (float)(g.E + g.p);
