@@ -67,8 +67,8 @@ hipError_t launch_zface_update(const BlockDesc* db, const BlockDesc* hb, int nb,
 // Tolerance-bound form: the y-face march combined with the x-face fluxes
 // (yrun_fast), one column (i, k) per thread, ZX x ZY columns per workgroup,
 // runs of ym cells along y.  OCC: waves per SIMD the register allocation
-// allows (3 spills a few registers, 2 none; the autotuner's face occupancy
-// picks).
+// allows (at 3 the x-face operands are loaded after the face so that
+// nothing spills; the autotuner's face occupancy picks).
 template <int OCC>
 __global__ __launch_bounds__(ZX * ZY, OCC) void k_yface_xy(const BlockDesc* __restrict__ blocks, int which, Phys ph,
                                                            int ym) {
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(ZX * ZY, OCC) void k_yface_xy(const BlockDesc* __re
   const int i = (tl.x % ntx) * ZX + (int)threadIdx.x % ZX;
   const int k = (tl.x / ntx) * ZY + (int)threadIdx.x / ZX;
   const int j0 = tl.y * ym;
-  if (i < B.n[0] && k < B.n[2] && j0 < B.n[1]) yrun_fast(B, i, k, j0, ym, which, ph);
+  if (i < B.n[0] && k < B.n[2] && j0 < B.n[1]) yrun_fast<OCC < 3>(B, i, k, j0, ym, which, ph);
 }
 
 hipError_t launch_yface_xy(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph, int occ,

@@ -332,7 +332,10 @@ __device__ __forceinline__ void zcell_fix(const BlockDesc& B, int i, int j, int 
 // B.F[1] at the cell's interior index.  A cell beside a y face whose range
 // checks fail gets a NaN in place of its differences: the fused z pass then
 // leaves it to the cell fixup, which forms its faces with the reference-order
-// form.
+// form.  EARLY: the cell's x-face operands are loaded before the face
+// (with the next queued cell), else after it (fewer registers held across
+// the face: 150 VGPRs, three waves per SIMD without spills).
+template <bool EARLY = true>
 __device__ __forceinline__ void yrun_fast(const BlockDesc& B, int i, int k, int j0, int ym, int which,
                                           const Phys& ph) {
   const int n0 = B.n[0], n1 = B.n[1];
@@ -360,12 +363,21 @@ __device__ __forceinline__ void yrun_fast(const BlockDesc& B, int i, int k, int 
     double e0[5], e1[5], nx[5];
 #pragma unroll
     for (int e = 0; e < 5; e++) {
-      e0[e] = E[e * sfe + iE];
-      e1[e] = E[e * sfe + iE + 1];
+      if (EARLY) {
+        e0[e] = E[e * sfe + iE];
+        e1[e] = E[e * sfe + iE + 1];
+      }
       nx[e] = Qin[e * sg + c0 + (j + 3) * sy];
     }
     double g[5];
     const bool ok = face_flux_fast<1>(q, ph.gamma, ph.gam1, ph.gg1, g);
+    if (!EARLY) {
+#pragma unroll
+      for (int e = 0; e < 5; e++) {
+        e0[e] = E[e * sfe + iE];
+        e1[e] = E[e * sfe + iE + 1];
+      }
+    }
     if (j > j0) {
       const int ct = i + n0 * (jc + n1 * k);
       const bool good = ok && okp;
