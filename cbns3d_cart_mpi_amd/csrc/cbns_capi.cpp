@@ -592,11 +592,15 @@ int cbns_set_time(cbns_ctx* c, double t) {
   return CBNS_OK;
 }
 
-// src/cbns3d.cpp:203-240, as five launches per stage: the three face-flux
-// sweeps, the viscous fluxes (beside them on the side stream when overlap is
-// on), the fused divergence + RK3 update + primitive recovery, and the ghost
-// gather of the state with its primitives.  launch_sweeps issues the first
-// two (which read only the stage input), finish_stage the rest.
+// src/cbns3d.cpp:203-240 per stage: the x and y face sweeps (the
+// tolerance-bound form: the x sweep, the fixup of its failing faces and the
+// y-face march that stores each cell's x+y flux differences), the viscous
+// march (beside them on the side stream when overlap is on), the z-face
+// sweep fused with the RK3 update (and, but for the tolerance-bound form,
+// the primitives) with its cell fixup, and the ghost gather of the state
+// (with its primitives).  The contracted FP64 form keeps a separate z sweep
+// and update.  launch_sweeps issues the face sweeps (which read only the
+// stage input), finish_stage the rest.
 // Whether a form's steps run the z-face sweep fused with the stage tail
 // (every form but the contracted FP64 one).
 static bool zfused(const cbns_ctx* c) { return c->precision != CBNS_FP64_FMA; }
