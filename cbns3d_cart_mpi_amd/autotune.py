@@ -13,7 +13,7 @@ FP64 face kernels, and the stage schedule:
   "serial"    viscous-flux kernel after the sweeps, on the same stream;
 then, on the best of those, the tile shapes of the z-marching kernels
 (planes per chunk of the viscous tile march, cells per run of the fused
-z-face pass and the y-face march).
+z-face pass and the tolerance-bound form's x+y march).
 Batch (tune_batch): rows per band, and the viscous kernels' schedule in the
 step graph (0: in line; n: beside the sweeps with n persistent workgroups).
 """

@@ -296,7 +296,8 @@ class Solver:
 
     def set_tile_shape(self, visc_planes, run_cells):
         """Most planes per viscous z-chunk and cells per run of the fused z
-        pass and the y-face march (0: the default, 32)."""
+        pass and the tolerance-bound x+y march (0: the default, 32; the x+y
+        march takes at most 64)."""
         check(lib().cbns_set_tile_shape(self._h, int(visc_planes), int(run_cells)), "cbns_set_tile_shape")
 
     def set_overlap(self, on):

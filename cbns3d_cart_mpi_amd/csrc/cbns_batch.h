@@ -52,24 +52,22 @@ constexpr int BATCH_COUNTERS = 64 * BATCH_CTR_PER_LAUNCH * BATCH_CTR_STRIDE;  //
 
 // Persistent grid of each kernel: its resident workgroups on the device.
 struct BatchGrids {
-  int face[3][3];  // [form: FP64, FP32-flux, tolerance-bound FP64][direction]
+  int face[2][3];  // [form: FP64, FP32-flux][direction]
   int visc[3], update[3], ghost, dt;  // [FP64, FP32-flux, tolerance-bound FP64]
-  int ymarch;  // the tolerance-bound solves' y-face march
+  int ymarch;  // the tolerance-bound solves' x+y march
   int cus;  // compute units of the device
 };
 hipError_t batch_grids(BatchGrids* g);
 
 // wgs: the launch's persistent workgroups; ctr: its NXCD tile counters,
 // zero at launch (batch_launch_reset_counters earlier on the stream)
-// form: 0 FP64, 1 FP32-flux, 2 tolerance-bound FP64 (whose failing faces go to fx)
+// form: 0 FP64, 1 FP32-flux (the tolerance-bound solves march their x and
+// y faces: batch_launch_ymarch)
 hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int dir, int which, int jb,
-                             int form, int wgs, int* ctr, const FixList& fx, hipStream_t s);
-// the fixup pass of the tolerance-bound solves' faces (cbns_kernels_fast.hip)
-hipError_t batch_launch_fixup(const BlockDesc* blocks, const InstDesc* insts, int nb, int which, const FixList& fx,
-                              hipStream_t s);
-// The tolerance-bound solves' y faces: the march along y (yrun_fast) that
-// stores each cell's x- plus y-flux differences, one (column tile, y-run)
-// unit per table entry (after their x faces and the face fixup).
+                             int form, int wgs, int* ctr, hipStream_t s);
+// The tolerance-bound solves' x and y faces: the march along y
+// (xyrun_fast) that stores each cell's x- plus y-flux differences, one
+// (column tile, y-run) unit per table entry.
 hipError_t batch_launch_ymarch(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, int wgs, int* ctr,
                                hipStream_t s);
 // precision: PREC_FP64, PREC_FP32_FLUX or PREC_FP64_FAST (whose march forms

@@ -86,37 +86,14 @@ __global__ __launch_bounds__(TPB, 2) void kb_face(const BlockDesc* __restrict__ 
   });
 }
 
-// Tolerance-bound FP64 form: the same sweep bodies as the single-solve
-// kernels (cbns_face_fast.h); faces failing their checks go to fx.  Single
-// faces (x) fit 4 waves per SIMD, pairs (y, z) 3.
-template <int D>
-__global__ __launch_bounds__(TPB, D == 0 ? 4 : 3) void kb_face_fast(const BlockDesc* __restrict__ blocks,
-                                                                   const InstDesc* __restrict__ insts, TileTable tt,
-                                                                   int which, int jb, int* ctr, FixList fx) {
-  for_tiles(tt.n, ctr, [&](int x) {
-    const TileRef tr = tt.tiles[x];
-    const BlockDesc& B = blocks[tr.block];
-    const InstDesc& I = insts[B.inst];
-    if (!I.stepping) return;
-    const int t = tr.tile * TPB + threadIdx.x;
-    if (D > 0) {
-      if (t >= face_pair_count(B, D)) return;
-      face_pair_body_fast<D == 0 ? 1 : D>(B, tr.block, t, which, jb, I.ph, fx);
-      return;
-    }
-    if (t >= face_count(B, D)) return;
-    face_body_fast<D>(B, tr.block, t, which, jb, I.ph, fx);
-  });
-}
-
-// Tolerance-bound solves' y faces: the march along y combined with their x
-// faces (yrun_fast), one (column tile (i, k), y-run) unit of one block per
-// table entry, the run length chosen from the block's size as the table was
-// built.
-__global__ __launch_bounds__(TPB, 2) void kb_yface_xy(const BlockDesc* __restrict__ blocks,
-                                                      const InstDesc* __restrict__ insts, TileTable tt, int which,
-                                                      int* ctr) {
-  static_assert(TPB == ZX * ZY, "one column per thread");
+// Tolerance-bound solves' x and y faces: one march along y (xyrun_fast),
+// one (column tile (i, k), y-run) unit of one block per table entry, the
+// run length chosen from the block's size as the table was built.
+__global__ __launch_bounds__(TPB, 2) void kb_xyface(const BlockDesc* __restrict__ blocks,
+                                                    const InstDesc* __restrict__ insts, TileTable tt, int which,
+                                                    int* ctr) {
+  static_assert(TPB == ZX * ZY && ZX == 64, "one wave per row of columns");
+  static_assert(BATCH_ZM <= XY_RUN_MAX, "run length of the x+y march");
   for_tiles(tt.n, ctr, [&](int x) {
     const TileRef tr = tt.tiles[x];
     const BlockDesc& B = blocks[tr.block];
@@ -125,9 +102,10 @@ __global__ __launch_bounds__(TPB, 2) void kb_yface_xy(const BlockDesc* __restric
     const int ntx = (B.n[0] + ZX - 1) / ZX, nt = ntx * ((B.n[2] + ZY - 1) / ZY);
     const int ym = chunk_len(nt, B.n[1], BATCH_ZM, ZM_LOW, BATCH_UNITS);
     const int tile = tr.tile % nt, run = tr.tile / nt;
-    const int i = (tile % ntx) * ZX + (int)threadIdx.x % ZX, k = (tile / ntx) * ZY + (int)threadIdx.x / ZX;
+    const int i0 = (tile % ntx) * ZX, k = (tile / ntx) * ZY + (int)threadIdx.x / ZX;
     const int j0 = run * ym;
-    if (i < B.n[0] && k < B.n[2] && j0 < B.n[1]) yrun_fast(B, i, k, j0, ym, which, I.ph);
+    // uniform over each wave: all its lanes take part in the march's lane exchanges
+    if (k < B.n[2] && j0 < B.n[1]) xyrun_fast(B, i0, k, j0, ym, which, I.ph);
   });
 }
 
@@ -363,14 +341,10 @@ __global__ void kb_set_budget(InstDesc* insts, const int2* pairs, int n) {
 }
 
 hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int dir, int which, int jb,
-                             int form, int wgs, int* ctr, const FixList& fx, hipStream_t s) {
+                             int form, int wgs, int* ctr, hipStream_t s) {
   if (tt.n == 0) return hipSuccess;
   const dim3 g(wgs), b(TPB);
-  if (form == 2) {
-    if (dir == 0) hipLaunchKernelGGL(kb_face_fast<0>, g, b, 0, s, blocks, insts, tt, which, jb, ctr, fx);
-    else if (dir == 1) hipLaunchKernelGGL(kb_face_fast<1>, g, b, 0, s, blocks, insts, tt, which, jb, ctr, fx);
-    else hipLaunchKernelGGL(kb_face_fast<2>, g, b, 0, s, blocks, insts, tt, which, jb, ctr, fx);
-  } else if (form == 0) {
+  if (form == 0) {
     if (dir == 0) hipLaunchKernelGGL((kb_face<0, 0>), g, b, 0, s, blocks, insts, tt, which, jb, ctr);
     else if (dir == 1) hipLaunchKernelGGL((kb_face<1, 0>), g, b, 0, s, blocks, insts, tt, which, jb, ctr);
     else hipLaunchKernelGGL((kb_face<2, 0>), g, b, 0, s, blocks, insts, tt, which, jb, ctr);
@@ -418,7 +392,7 @@ hipError_t batch_launch_ymarch(const BlockDesc* blocks, InstDesc* insts, TileTab
     wgs = (tt.n + NXCD - 1) / NXCD * NXCD;
     ctr = nullptr;
   }
-  hipLaunchKernelGGL(kb_yface_xy, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, which, ctr);
+  hipLaunchKernelGGL(kb_xyface, dim3(wgs), dim3(TPB), 0, s, blocks, insts, tt, which, ctr);
   return hipGetLastError();
 }
 
@@ -489,16 +463,13 @@ hipError_t batch_grids(BatchGrids* g) {
   res((const void*)kb_face<0, 1>, &g->face[1][0]);
   res((const void*)kb_face<1, 1>, &g->face[1][1]);
   res((const void*)kb_face<2, 1>, &g->face[1][2]);
-  res((const void*)kb_face_fast<0>, &g->face[2][0]);
-  res((const void*)kb_face_fast<1>, &g->face[2][1]);
-  res((const void*)kb_face_fast<2>, &g->face[2][2]);
   res((const void*)kb_visc<double>, &g->visc[0]);
   res((const void*)kb_visc<float>, &g->visc[1]);
   res((const void*)kb_visc<double, true>, &g->visc[2]);
   res((const void*)kb_zface_update<ZF_FP64>, &g->update[0]);
   res((const void*)kb_zface_update<ZF_FP32>, &g->update[1]);
   res((const void*)kb_zface_update<ZF_FAST>, &g->update[2]);
-  res((const void*)kb_yface_xy, &g->ymarch);
+  res((const void*)kb_xyface, &g->ymarch);
   res((const void*)kb_ghost, &g->ghost);
   res((const void*)kb_dt, &g->dt);
   return e;

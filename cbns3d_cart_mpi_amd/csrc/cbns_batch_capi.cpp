@@ -40,16 +40,16 @@ struct Inst {
 };
 
 enum {
-  T_FACE64 = 0, T_FACE32 = 3, T_FACEF = 6,
-  T_VISC64 = 9, T_VISC32, T_VISCF,
+  T_FACE64 = 0, T_FACE32 = 3,
+  T_VISC64 = 6, T_VISC32, T_VISCF,
   T_UPD64, T_UPD32, T_UPDF,
-  T_YMF,  // the tolerance-bound solves' y-face march
+  T_YMF,  // the tolerance-bound solves' x+y march
   T_GHOST, T_DT, T_COUNT
 };
 
 // Kernel form of a solve: 0 FP64, 1 FP32-flux, 2 tolerance-bound FP64.
 int face_form(int precision) { return precision == CBNS_FP32_FLUX ? 1 : (precision == CBNS_FP64_FAST ? 2 : 0); }
-const int FACE_CLASS[3] = {T_FACE64, T_FACE32, T_FACEF};
+const int FACE_CLASS[2] = {T_FACE64, T_FACE32};  // the tolerance-bound solves march theirs (T_YMF)
 const int VISC_CLASS[3] = {T_VISC64, T_VISC32, T_VISCF};
 const int UPD_CLASS[3] = {T_UPD64, T_UPD32, T_UPDF};
 const int FORM_PREC[3] = {PREC_FP64, PREC_FP32_FLUX, PREC_FP64_FAST};
@@ -80,7 +80,6 @@ struct cbns_batch {
   int jb = 32;
   BatchGrids grid{};  // resident workgroups of each persistent kernel
   int* d_ctr = nullptr;  // tile counters of the step graph's launches
-  FixList fix{};         // faces of tolerance-bound solves left to the fixup pass
   FixList cfix{};        // cells of FP64 solves that the fused z pass left to the cell fixup
   // Arrays of removed solves, kept for reuse by later solves of the same
   // shapes (by size): hipFree synchronises the whole device, which would stall
@@ -96,7 +95,7 @@ struct cbns_batch {
   InstStatus* h_status = nullptr;
   InstStatus* d_status = nullptr;
   bool published = false;  // h_status reflects the device descriptors
-  bool has_fast = false; // a live tolerance-bound solve (the step graph has face fixup passes)
+  bool has_fast = false; // a live tolerance-bound solve (the step graph has its x+y march)
   bool has_f64 = false;  // a live FP64 solve of either form (the step graph has cell fixup passes)
   int wg_mem = 0;     // override of the memory-bound kernels' grid (0: resident)
   int ninst_hw = 0;  // highest live id + 1
@@ -240,11 +239,12 @@ static int rebuild(cbns_batch* b) {
         for (int t = 0; t < nt; t++) tab[cls].push_back({bi, t});
       };
       // x and y sweeps (the z faces are formed by the fused stage tail); the
-      // tolerance-bound solves march their y faces in (column tile, y-run)
-      // units, the run lengths as the kernel forms them
+      // tolerance-bound solves march their x and y faces together in
+      // (column tile, y-run) units, the run lengths as the kernel forms them
       const bool fast = I.precision == CBNS_FP64_FAST;
-      for (int d = 0; d < (fast ? 1 : 2); d++) add(FACE_CLASS[face_form(I.precision)] + d, face_items(B, d));
-      if (fast) {
+      if (!fast)
+        for (int d = 0; d < 2; d++) add(FACE_CLASS[face_form(I.precision)] + d, face_items(B, d));
+      else {
         const int yt = ((B.n[0] + ZX - 1) / ZX) * ((B.n[2] + ZY - 1) / ZY);
         const int ym = chunk_len(yt, B.n[1], BATCH_ZM, ZM_LOW, BATCH_UNITS);
         for (int t = 0; t < yt * ((B.n[1] + ym - 1) / ym); t++) tab[T_YMF].push_back({bi, t});
@@ -299,7 +299,7 @@ static int rebuild(cbns_batch* b) {
   auto tail = [&]() { return b->wg_mem > 0 ? b->wg_mem : 0; };
   // every persistent launch of the step takes its own block of counters, all
   // zeroed by the first node of the graph
-  constexpr int LAUNCHES = 1 + 3 * 14;  // dt; per stage 3 viscous, 6 face, 1 y march, 3 fused z + update, 1 ghost
+  constexpr int LAUNCHES = 1 + 3 * 12;  // dt; per stage 3 viscous, 4 face, 1 x+y march, 3 fused z + update, 1 ghost
   constexpr int K = BATCH_CTR_PER_LAUNCH * BATCH_CTR_STRIDE;
   static_assert(LAUNCHES * K <= BATCH_COUNTERS, "counter block");
   int used = 0;
@@ -323,16 +323,12 @@ static int rebuild(cbns_batch* b) {
                             ov ? b->visc_side_wgs : G.visc[f], ctr(), sv);
     if (ov && e == hipSuccess) e = hipEventRecord(b->ev_join, sv);
     for (int d = 0; d < 2 && e == hipSuccess; d++) {
-      for (int f = 0; f < 3 && e == hipSuccess; f++)
+      for (int f = 0; f < 2 && e == hipSuccess; f++)
         e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[FACE_CLASS[f] + d], d, in, b->jb, f, G.face[f][d], ctr(),
-                              b->fix, s);
-      if (d == 0 && b->has_fast && e == hipSuccess) {
-        // x faces of tolerance-bound solves whose range checks failed, then
-        // their y-face march (which reads the x faces)
-        e = batch_launch_fixup(b->d_blocks, b->d_insts, (int)all.size(), in, b->fix, s);
-        if (e == hipSuccess)
-          e = batch_launch_ymarch(b->d_blocks, b->d_insts, b->tt[T_YMF], in, tail(), ctr(), s);
-      }
+                              s);
+      // the tolerance-bound solves' x+y march
+      if (d == 0 && b->has_fast && e == hipSuccess)
+        e = batch_launch_ymarch(b->d_blocks, b->d_insts, b->tt[T_YMF], in, tail(), ctr(), s);
     }
     if (ov && e == hipSuccess) e = hipStreamWaitEvent(s, b->ev_join, 0);  // join before the stage tail
     for (int f = 0; f < 3 && e == hipSuccess; f++)
@@ -374,10 +370,6 @@ int cbns_batch_create(uint64_t hbm_budget_bytes, int capacity, cbns_batch** out)
       hipMalloc(&b->d_part, sizeof(double) * (2 * GSUM_WG + 2)) != hipSuccess ||
       hipMalloc(&b->d_ctr, sizeof(int) * BATCH_COUNTERS) != hipSuccess)
     st = CBNS_ERR_NOMEM;
-  void* fixbuf = nullptr;
-  if (!st && hipMalloc(&fixbuf, FIX_BYTES) != hipSuccess) st = CBNS_ERR_NOMEM;
-  if (!st && hipMemset(fixbuf, 0, 256) != hipSuccess) st = CBNS_ERR_HIP;
-  if (fixbuf) b->fix = fixlist_at(fixbuf);
   void* cfixbuf = nullptr;
   if (!st && hipMalloc(&cfixbuf, FIX_BYTES) != hipSuccess) st = CBNS_ERR_NOMEM;
   if (!st && hipMemset(cfixbuf, 0, 256) != hipSuccess) st = CBNS_ERR_HIP;
@@ -419,7 +411,6 @@ int cbns_batch_destroy(cbns_batch* b) {
   if (b->d_insts) (void)hipFree(b->d_insts);
   if (b->d_part) (void)hipFree(b->d_part);
   if (b->d_ctr) (void)hipFree(b->d_ctr);
-  if (b->fix.count) (void)hipFree(b->fix.count);
   if (b->cfix.count) (void)hipFree(b->cfix.count);
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   if (b->ev_join) (void)hipEventDestroy(b->ev_join);

@@ -32,9 +32,9 @@ struct cbns_ctx {
   bool overlap = true;
   int visc_wgs = 0;  // with overlap: workgroups of the side-stream viscous kernel (0 = one per unit)
   int jband = 16;  // rows per band of the stencil kernels' work order
-  int face_occ = 2;  // register-allocation variant of the FP64 face kernel
+  int face_occ = 2;  // register-allocation variant of the FP64 face kernels (not of the x+y march)
   int visc_top = VZ;  // most planes per viscous z-chunk
-  int run_top = ZM;   // most cells per run of the fused z pass and the y-face march
+  int run_top = ZM;   // most cells per run of the fused z pass and the x+y march (which caps it at 64)
   unsigned long long* d_smax = nullptr;
   int* d_flag = nullptr;
   double* d_diag = nullptr;  // partials then the two sums
@@ -593,8 +593,8 @@ int cbns_set_time(cbns_ctx* c, double t) {
 }
 
 // src/cbns3d.cpp:203-240 per stage: the x and y face sweeps (the
-// tolerance-bound form: the x sweep, the fixup of its failing faces and the
-// y-face march that stores each cell's x+y flux differences), the viscous
+// tolerance-bound form: one march over its x and y faces that stores each
+// cell's x+y flux differences), the viscous
 // march (beside them on the side stream when overlap is on), the z-face
 // sweep fused with the RK3 update (and, but for the tolerance-bound form,
 // the primitives) with its cell fixup, and the ghost gather of the state
@@ -635,16 +635,9 @@ static int launch_sweeps(cbns_ctx* c, int stage) {
   }
   // the z faces are formed by the fused stage tail
   if (c->precision == CBNS_FP64_FAST) {
-    // x faces, the fixup of those whose range checks failed, then the y-face
-    // march that stores each cell's x- plus y-flux differences
-    CK(timed(c, 0, [&] {
-      return launch_face_flux(c->db, hb, c->dom.nb, 0, in, c->ph, c->precision, c->ref, c->jband, c->face_occ,
-                              c->fix, c->stream);
-    }));
-    CK(launch_face_fixup(c->db, hb, c->dom.nb, in, c->ph, c->fix, c->stream, 1));
-    CK(timed(c, 1, [&] {
-      return launch_yface_xy(c->db, hb, c->dom.nb, in, c->ph, c->face_occ, c->stream, c->run_top);
-    }));
+    // the x and y faces in one march that stores each cell's x- plus
+    // y-flux differences (timed as the y class: it carries both axes)
+    CK(timed(c, 1, [&] { return launch_xyface(c->db, hb, c->dom.nb, in, c->ph, c->stream, c->run_top); }));
     return CBNS_OK;
   }
   const int nd = zfused(c) ? 2 : 3;

@@ -154,12 +154,12 @@ struct RefState;
 hipError_t launch_zface_update(const BlockDesc* db, const BlockDesc* hb, int nb, int precision, int in, int out,
                                int stage, double dt, const Phys& ph, const RefState& ref, unsigned long long* smax,
                                int* flag, const FixList& cfx, hipStream_t s, int zm_top = 32);
-// Tolerance-bound form: its y faces swept by a march along y that stores
-// each interior cell's x- plus y-flux differences into F[1] (after the x
-// sweep and its face fixup); cells beside a failing y face are marked for
-// the fused z pass to leave to the cell fixup.
-hipError_t launch_yface_xy(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph, int occ,
-                           hipStream_t s, int ym_top = 32);
+// Tolerance-bound form: its x and y faces in one march along y that stores
+// each interior cell's x- plus y-flux differences into F[1] (no x sweep, no
+// face fixup); cells beside a failing x or y face are marked for the fused
+// z pass to leave to the cell fixup.  Runs are at most 64 cells long.
+hipError_t launch_xyface(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
+                         hipStream_t s, int ym_top = 32);
 hipError_t launch_zcell_fixup(const BlockDesc* db, const BlockDesc* hb, int nb, int precision, int in, int out,
                               int stage, double dt, const Phys& ph, unsigned long long* smax, int* flag,
                               const FixList& cfx, hipStream_t s);

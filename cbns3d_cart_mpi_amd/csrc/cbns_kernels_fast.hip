@@ -103,14 +103,11 @@ __device__ __forceinline__ void fixup_all(const BlockDesc& B, int which, const P
 
 constexpr int FIX_WG = 64;  // workgroups of the fixup pass
 
-// BATCH: blocks of many solves, each with its own physics (insts); the
-// single-solve form passes ph.  In the pass over every face, only blocks of
-// stepping tolerance-bound solves are checked, and only the face arrays in
-// the mask dirs (the fused steps sweep only x faces into the list: their y
-// faces are marched by yrun_fast, whose F[1] holds cell differences).
-template <bool BATCH>
-__global__ __launch_bounds__(TPB) void k_face_fixup(const BlockDesc* __restrict__ blocks,
-                                                    const InstDesc* __restrict__ insts, int nb, int which, Phys ph,
+// The faces that the method-level sweeps (cbns_rec_riemann) left to the
+// list; in the pass over every face, only the face arrays in the mask dirs
+// are checked.  (The fused steps list no faces: their x+y march and fused
+// z pass leave failing faces' cells to the cell fixup.)
+__global__ __launch_bounds__(TPB) void k_face_fixup(const BlockDesc* __restrict__ blocks, int nb, int which, Phys ph,
                                                     FixList fx, int dirs) {
   const unsigned n = *fx.count;  // every workgroup reads it before the last one resets it (below)
   const long long first = blockIdx.x * (long long)TPB + threadIdx.x, stride = (long long)gridDim.x * TPB;
@@ -119,19 +116,16 @@ __global__ __launch_bounds__(TPB) void k_face_fixup(const BlockDesc* __restrict_
       const unsigned long long v = fx.ent[e];
       const long long fi = (long long)(v & ((1ull << 40) - 1));
       const int D = (int)((v >> 40) & 3), b = (int)(v >> 42);
-      const Phys& P = BATCH ? insts[blocks[b].inst].ph : ph;
-      if (D == 0) fixup_face<0>(blocks[b], which, fi, P);
-      else if (D == 1) fixup_face<1>(blocks[b], which, fi, P);
-      else fixup_face<2>(blocks[b], which, fi, P);
+      if (D == 0) fixup_face<0>(blocks[b], which, fi, ph);
+      else if (D == 1) fixup_face<1>(blocks[b], which, fi, ph);
+      else fixup_face<2>(blocks[b], which, fi, ph);
     }
   } else {
     for (int b = 0; b < nb; b++) {
       const BlockDesc& B = blocks[b];
-      if (BATCH && (!insts[B.inst].stepping || insts[B.inst].precision != PREC_FP64_FAST)) continue;
-      const Phys& P = BATCH ? insts[B.inst].ph : ph;
-      if (dirs & 1) fixup_all<0>(B, which, P, first, stride);
-      if (dirs & 2) fixup_all<1>(B, which, P, first, stride);
-      if (dirs & 4) fixup_all<2>(B, which, P, first, stride);
+      if (dirs & 1) fixup_all<0>(B, which, ph, first, stride);
+      if (dirs & 2) fixup_all<1>(B, which, ph, first, stride);
+      if (dirs & 4) fixup_all<2>(B, which, ph, first, stride);
     }
   }
   // the last workgroup to finish clears the list for the next stage
@@ -148,15 +142,7 @@ __global__ __launch_bounds__(TPB) void k_face_fixup(const BlockDesc* __restrict_
 hipError_t launch_face_fixup(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
                              const FixList& fx, hipStream_t s, int dirs) {
   (void)hb;
-  hipLaunchKernelGGL(k_face_fixup<false>, dim3(FIX_WG), dim3(TPB), 0, s, db, (const InstDesc*)nullptr, nb, which, ph,
-                     fx, dirs);
-  return hipGetLastError();
-}
-
-// the batched step's x faces (its tolerance-bound solves march their y faces)
-hipError_t batch_launch_fixup(const BlockDesc* blocks, const InstDesc* insts, int nb, int which, const FixList& fx,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(k_face_fixup<true>, dim3(FIX_WG), dim3(TPB), 0, s, blocks, insts, nb, which, Phys{}, fx, 1);
+  hipLaunchKernelGGL(k_face_fixup, dim3(FIX_WG), dim3(TPB), 0, s, db, nb, which, ph, fx, dirs);
   return hipGetLastError();
 }
 

@@ -64,34 +64,33 @@ hipError_t launch_zface_update(const BlockDesc* db, const BlockDesc* hb, int nb,
   return hipGetLastError();
 }
 
-// Tolerance-bound form: the y-face march combined with the x-face fluxes
-// (yrun_fast), one column (i, k) per thread, ZX x ZY columns per workgroup,
-// runs of ym cells along y.  OCC: waves per SIMD the register allocation
-// allows (at 3 the x-face operands are loaded after the face so that
-// nothing spills; the autotuner's face occupancy picks).
-template <int OCC>
-__global__ __launch_bounds__(ZX * ZY, OCC) void k_yface_xy(const BlockDesc* __restrict__ blocks, int which, Phys ph,
-                                                           int ym) {
+// Tolerance-bound form: the x and y faces in one march (xyrun_fast), one
+// wave per 64 columns at one k, ZY waves per workgroup, runs of ym <= 64
+// cells along y.  Every condition on the way to the march is uniform over
+// a wave (k from the wave index, j0 from the workgroup), so all 64 lanes
+// take part in its lane exchanges.
+__global__ __launch_bounds__(ZX * ZY, 2) void k_xyface(const BlockDesc* __restrict__ blocks, int which, Phys ph,
+                                                       int ym) {
+  static_assert(ZX == 64, "one wave per row of columns");
   const Tile tl = xcd_tile();
   const BlockDesc& B = blocks[blockIdx.z];
   const int ntx = (B.n[0] + ZX - 1) / ZX;
-  const int i = (tl.x % ntx) * ZX + (int)threadIdx.x % ZX;
+  const int i0 = (tl.x % ntx) * ZX;
   const int k = (tl.x / ntx) * ZY + (int)threadIdx.x / ZX;
   const int j0 = tl.y * ym;
-  if (i < B.n[0] && k < B.n[2] && j0 < B.n[1]) yrun_fast<OCC < 3>(B, i, k, j0, ym, which, ph);
+  if (i0 < B.n[0] && k < B.n[2] && j0 < B.n[1]) xyrun_fast(B, i0, k, j0, ym, which, ph);
 }
 
-hipError_t launch_yface_xy(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph, int occ,
-                           hipStream_t s, int ym_top) {
+hipError_t launch_xyface(const BlockDesc* db, const BlockDesc* hb, int nb, int which, const Phys& ph,
+                         hipStream_t s, int ym_top) {
   int tiles = 0, n1 = 0;
   for (int b = 0; b < nb; b++) {
     tiles = std::max(tiles, ((hb[b].n[0] + ZX - 1) / ZX) * ((hb[b].n[2] + ZY - 1) / ZY));
     n1 = std::max(n1, hb[b].n[1]);
   }
-  const int ym = chunk_len((long long)tiles * nb, n1, ym_top, ZM_LOW, 1024);
+  const int ym = chunk_len((long long)tiles * nb, n1, std::min(ym_top, XY_RUN_MAX), ZM_LOW, 1024);
   const dim3 grid(tiles, (n1 + ym - 1) / ym, nb), blk(ZX * ZY);
-  if (occ >= 3) hipLaunchKernelGGL(k_yface_xy<3>, grid, blk, 0, s, db, which, ph, ym);
-  else hipLaunchKernelGGL(k_yface_xy<2>, grid, blk, 0, s, db, which, ph, ym);
+  hipLaunchKernelGGL(k_xyface, grid, blk, 0, s, db, which, ph, ym);
   return hipGetLastError();
 }
 

@@ -22,11 +22,12 @@
 // stored values (rounded to float for FP32-flux as stored there); the
 // FP32-flux face fluxes are rounded to float as its face arrays store them.
 // Every value therefore equals the separate sweep and tail kernels'.
-// The tolerance-bound form's y faces are swept by a march of the same shape
-// along y (yrun_fast), which combines them at once with the x-face fluxes:
-// it stores, per cell, the x- plus y-flux differences in the reference's
-// order and rounding (src/Block3d.cpp:932-938), so its fused z pass reads
-// five values per cell instead of the x- and y-face fluxes of both sides.
+// The tolerance-bound form's x and y faces are formed by a march of the same
+// shape along y (xyrun_fast), x faces across the lanes of a wave, y faces
+// along each lane's column: it stores, per cell, the x- plus y-flux
+// differences in the reference's order and rounding (src/Block3d.cpp:932-938),
+// so no x or y face flux reaches memory and its fused z pass reads five
+// values per cell instead of the x- and y-face fluxes of both sides.
 // Shared by the single-solve kernels (cbns_kernels_zfuse.hip) and the
 // batched ones (cbns_batch.hip).
 #pragma once
@@ -41,7 +42,7 @@ using zft = typename std::conditional<FORM == ZF_FP32, float, double>::type;
 
 // Operands of the update of one cell other than its z-fluxes and its stage
 // input, in their storage types.  XY (tolerance-bound form): xy holds the
-// cell's x- plus y-flux differences (yrun_fast) and e0 .. f1 are unused.
+// cell's x- plus y-flux differences (xyrun_fast) and e0 .. f1 are unused.
 template <typename FT>
 struct ZOperands {
   FT e0[5], e1[5], f0[5], f1[5], dv[4];
@@ -270,9 +271,9 @@ __device__ __forceinline__ bool zface_checked(const double (&q)[6][5], const Phy
 // A listed cell: both of its z-faces (zface_checked), then its update.  With
 // all set (more failures than the list holds), a cell is updated only if
 // one of its faces fails its checks: exactly the cells the fused pass left.
-// The tolerance-bound form's cells also take their y faces here (the march
-// stored no y-face fluxes), and a cell is left alone in the all pass only if
-// its two y faces pass as well.
+// The tolerance-bound form's cells also take their x and y faces here (its
+// x+y march stores no face fluxes), and a cell is left alone in the all
+// pass only if its x and y faces pass as well.
 template <int FORM>
 __device__ __forceinline__ void zcell_fix(const BlockDesc& B, int i, int j, int k, int in, int out, int stage,
                                           double dt, const Phys& ph, bool all, bool want_s, double* s, int* bad) {
@@ -283,7 +284,7 @@ __device__ __forceinline__ void zcell_fix(const BlockDesc& B, int i, int j, int 
   // the two faces along axis D of the cell from its seven cells along D
   const auto faces = [&](auto dc, double (&g0)[5], double (&g1)[5], bool& ok0, bool& ok1) {
     constexpr int D = decltype(dc)::value;
-    const int sD = D == 1 ? B.g[0] : sz;
+    const int sD = D == 0 ? 1 : (D == 1 ? B.g[0] : sz);
     double q7[7][5];
 #pragma unroll
     for (int cc = 0; cc < 7; cc++)
@@ -301,22 +302,25 @@ __device__ __forceinline__ void zcell_fix(const BlockDesc& B, int i, int j, int 
       ok0 = zface_checked<FORM>(qa, ph, true, g0);
       ok1 = zface_checked<FORM>(qb, ph, true, g1);
     } else {
-      ok0 = face_flux_fast<1>(qa, ph.gamma, ph.gam1, ph.gg1, g0);
-      if (!ok0) face_flux_fast_fallback<1>(qa, ph, g0);
-      ok1 = face_flux_fast<1>(qb, ph.gamma, ph.gam1, ph.gg1, g1);
-      if (!ok1) face_flux_fast_fallback<1>(qb, ph, g1);
+      ok0 = face_flux_fast<D>(qa, ph.gamma, ph.gam1, ph.gg1, g0);
+      if (!ok0) face_flux_fast_fallback<D>(qa, ph, g0);
+      ok1 = face_flux_fast<D>(qb, ph.gamma, ph.gam1, ph.gg1, g1);
+      if (!ok1) face_flux_fast_fallback<D>(qb, ph, g1);
     }
   };
-  double g0[5], g1[5], f0[5], f1[5];
-  bool ok0, ok1, oky0 = true, oky1 = true;
+  double g0[5], g1[5], e0[5], e1[5], f0[5], f1[5];
+  bool ok0, ok1, okx0 = true, okx1 = true, oky0 = true, oky1 = true;
   faces(std::integral_constant<int, 2>(), g0, g1, ok0, ok1);
-  if constexpr (FORM == ZF_FAST) faces(std::integral_constant<int, 1>(), f0, f1, oky0, oky1);
-  if (all && ok0 && ok1 && oky0 && oky1) return;
+  if constexpr (FORM == ZF_FAST) {
+    faces(std::integral_constant<int, 0>(), e0, e1, okx0, okx1);
+    faces(std::integral_constant<int, 1>(), f0, f1, oky0, oky1);
+  }
+  if (all && ok0 && ok1 && okx0 && okx1 && oky0 && oky1) return;
   ZOperands<double> o;
-  zload<double>(B, i, j, k, o, stage);
+  zload<double, FORM == ZF_FAST>(B, i, j, k, o, stage);
   if constexpr (FORM == ZF_FAST) {
 #pragma unroll
-    for (int e = 0; e < 5; e++) o.xy[e] = xy_diff<true>(o.e0[e], o.e1[e], f0[e], f1[e], ph);
+    for (int e = 0; e < 5; e++) o.xy[e] = xy_diff<true>(e0[e], e1[e], f0[e], f1[e], ph);
   }
   double qc[5];
 #pragma unroll
@@ -324,66 +328,102 @@ __device__ __forceinline__ void zcell_fix(const BlockDesc& B, int i, int j, int 
   zapply<FORM>(B, i, j, k, o, g0, g1, qc, out, stage, dt, ph, want_s, s, bad);
 }
 
-// Tolerance-bound form: one thread's march along y over cells [j0, j0 + ym)
-// of column (i, k) of block B, i < n0, k < n2, j0 < n1.  It sweeps the y
-// faces j0 .. j1 with a six-cell register queue of the stage input (as zrun
-// sweeps z) and, after face j, stores the x- plus y-flux differences of cell
-// j - 1 (xy_diff, from the x-face fluxes of the x sweep and its fixup) into
-// B.F[1] at the cell's interior index.  A cell beside a y face whose range
+// 64-bit value of lane src (uniform over the wave) to every lane.
+__device__ __forceinline__ double lane_value(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, src);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Most rows per run of the x+y march: lane l forms the right x face of row
+// j0 + l for the wave's last column.
+constexpr int XY_RUN_MAX = 64;
+
+// Tolerance-bound form: the x and y faces of a stage in one march, which
+// stores each interior cell's x- plus y-flux differences (xy_diff<true>)
+// into B.F[1] at the cell's interior index; no face flux reaches memory.
+// One wave owns the 64 columns i0 .. i0 + 63 at one k (lane l: i = i0 + l)
+// and marches rows [j0, j0 + ym), ym <= 64: the y faces j0 .. j1 with a
+// six-cell register queue of the stage input along each lane's column (as
+// zrun marches z).  At row j each lane also forms the x face on the left of
+// its cell (cells i - 3 .. i + 2 of the row, face_flux_fast<0>) and takes the
+// face on the right from lane l + 1; the last lane's right faces (x index
+// i0 + 64, one per row of the run) are formed before the march, row j0 + l
+// on lane l, when that face lies in the block.  Lanes past the block's last
+// column form copies of face n0 (its stencil lies in the ghosts) and store
+// nothing, so every lane of the wave takes part in every step and the lane
+// exchanges see defined values.  Cell j's x part divr(E(i+1) - E(i), dx) is
+// formed at row j and its y part added after y face j + 1, the order and
+// rounding of xy_diff<true>.  A cell beside any x or y face whose range
 // checks fail gets a NaN in place of its differences: the fused z pass then
-// leaves it to the cell fixup, which forms its faces with the reference-order
-// form.  EARLY: the cell's x-face operands are loaded before the face
-// (with the next queued cell), else after it (fewer registers held across
-// the face: 150 VGPRs, three waves per SIMD without spills).
-template <bool EARLY = true>
-__device__ __forceinline__ void yrun_fast(const BlockDesc& B, int i, int k, int j0, int ym, int which,
-                                          const Phys& ph) {
+// leaves it to the cell fixup, which forms its faces with the
+// reference-order form.
+// The caller guarantees k < n2, j0 < n1 and i0 < n0, uniform over the wave.
+// The x stencil of a row is loaded after the row's y face (228 VGPRs, two
+// waves per SIMD; loaded before it, the queue and the stencil together
+// spill).
+__device__ __forceinline__ void xyrun_fast(const BlockDesc& B, int i0, int k, int j0, int ym, int which,
+                                           const Phys& ph) {
+  const int lane = (int)(threadIdx.x & 63);
   const int n0 = B.n[0], n1 = B.n[1];
-  const int j1 = j0 + ym < n1 ? j0 + ym : n1;  // cells [j0, j1), faces j0 .. j1
+  const int i = i0 + lane;
+  const int ic = i < n0 ? i : n0;  // the face this lane forms: left of cell ic
+  const int j1 = j0 + ym < n1 ? j0 + ym : n1;  // cells [j0, j1), y faces j0 .. j1
   const int sy = B.g[0];
   const long long sg = B.sg;
   gptr<const double> __restrict__ Qin = gmem<const double>(B.Q[which]);
-  gptr<const double> __restrict__ E = gmem<const double>(B.F[0]);
   gptr<double> __restrict__ XY = gmem(B.F[1]);
-  const long long sfe = B.sf[0], sfx = B.sf[1];
-  const int c0 = gidx(B, i, 0, k);
-  double q[6][5];  // cells j - 3 .. j + 2 of face j (rows -3 .. n1 + 2: inside the ghosts)
+  const long long sfx = B.sf[1];
+  const int c0 = gidx(B, ic, 0, k);
+  // the last lane's right faces: face i0 + 64 of row j0 + lane (rows past
+  // the run repeat its last row)
+  double ex[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int okex = 1;
+  if (i0 + 64 <= n0) {  // uniform over the wave
+    const int jr = j0 + lane < j1 ? j0 + lane : j1 - 1;
+    double xq[6][5];
+    load_face_cells<0>(B, which, gidx(B, i0 + 64, jr, k), xq);
+    okex = face_flux_fast<0>(xq, ph.gamma, ph.gam1, ph.gg1, ex);
+  }
+  double q[6][5];  // cells j - 3 .. j + 2 of y face j (rows -3 .. n1 + 2: inside the ghosts)
 #pragma unroll
   for (int cc = 0; cc < 6; cc++)
 #pragma unroll
     for (int e = 0; e < 5; e++) q[cc][e] = Qin[e * sg + c0 + (j0 - 3 + cc) * sy];
-  double gp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  bool okp = true;
+  double gp[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, xp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  bool okp = true, okxp = true;
   for (int j = j0; j <= j1; j++) {
-    // this step's loads (the x faces of cell j - 1, or of cell j0 unused on
-    // the first face, and the next queued cell, row j + 3 <= n1 + 3) go out
-    // before the face, with no branch among them
-    const int jc = j > j0 ? j - 1 : j0;
-    const int iE = i + (n0 + 1) * (jc + n1 * k);
-    double e0[5], e1[5], nx[5];
+    const bool xrow = j < j1;  // uniform: the last y face of a run closes cell j1 - 1 only
+    double nx[5], xq[6][5];
 #pragma unroll
-    for (int e = 0; e < 5; e++) {
-      if (EARLY) {
-        e0[e] = E[e * sfe + iE];
-        e1[e] = E[e * sfe + iE + 1];
-      }
-      nx[e] = Qin[e * sg + c0 + (j + 3) * sy];
-    }
+    for (int e = 0; e < 5; e++) nx[e] = Qin[e * sg + c0 + (j + 3) * sy];  // row j + 3 <= n1 + 3
     double g[5];
     const bool ok = face_flux_fast<1>(q, ph.gamma, ph.gam1, ph.gg1, g);
-    if (!EARLY) {
+    double xd[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    bool okx = true;
+    if (xrow) {
+      load_face_cells<0>(B, which, c0 + j * sy, xq);
+      double gx[5];
+      const int okl = face_flux_fast<0>(xq, ph.gamma, ph.gam1, ph.gg1, gx);
+      const int src = j - j0;
 #pragma unroll
       for (int e = 0; e < 5; e++) {
-        e0[e] = E[e * sfe + iE];
-        e1[e] = E[e * sfe + iE + 1];
+        const double right = __shfl_down(gx[e], 1, 64), last = lane_value(ex[e], src);
+        xd[e] = divr<true>((lane == 63 ? last : right) - gx[e], ph.dx, ph.rdx, ph.fdx);
       }
+      const int okr = __shfl_down(okl, 1, 64), okl63 = __builtin_amdgcn_readlane(okex, src);
+      okx = okl && (lane == 63 ? okl63 : okr);
     }
-    if (j > j0) {
-      const int ct = i + n0 * (jc + n1 * k);
-      const bool good = ok && okp;
+    if (j > j0 && i < n0) {  // cell j - 1
+      const int ct = i + n0 * ((j - 1) + n1 * k);
+      const bool good = ok && okp && okxp;
 #pragma unroll
-      for (int e = 0; e < 5; e++)
-        XY[e * sfx + ct] = good ? xy_diff<true>(e0[e], e1[e], gp[e], g[e], ph) : __builtin_nan("");
+      for (int e = 0; e < 5; e++) {
+        double df = xp[e];
+        df += divr<true>(g[e] - gp[e], ph.dy, ph.rdy, ph.fdy);
+        XY[e * sfx + ct] = good ? df : __builtin_nan("");
+      }
     }
 #pragma unroll
     for (int cc = 0; cc < 5; cc++)
@@ -393,8 +433,10 @@ __device__ __forceinline__ void yrun_fast(const BlockDesc& B, int i, int k, int 
     for (int e = 0; e < 5; e++) {
       q[5][e] = nx[e];
       gp[e] = g[e];
+      xp[e] = xd[e];
     }
     okp = ok;
+    okxp = okx;
   }
 }
 

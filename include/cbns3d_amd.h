@@ -179,14 +179,16 @@ int cbns_set_viscous_workgroups(cbns_ctx* ctx, int workgroups);
  * walks bands of this many y-rows through all z-planes.  A tuning knob only;
  * results are identical for every value. */
 int cbns_set_band(cbns_ctx* ctx, int rows);
-/* Register-allocation variant of the FP64 face kernels (and of the
- * tolerance-bound form's y-face march): 2 or 3 waves per SIMD (default 2).
+/* Register-allocation variant of the FP64 face kernels: 2 or 3 waves per
+ * SIMD (default 2).  The tolerance-bound form's stepping marches its x and y
+ * faces in one kernel with a single variant, which this leaves unchanged.
  * A tuning knob only; results are identical. */
 int cbns_set_face_occupancy(cbns_ctx* ctx, int waves);
-/* Tile shapes of the z-marching kernels: at most visc_planes planes per
+/* Tile shapes of the marching kernels: at most visc_planes planes per
  * z-chunk of the viscous tile march (4 .. 256, default 32) and at most
- * run_cells cells per run of the fused z-face pass and of the y-face march
- * (2 .. 256, default 32); 0 keeps the default.  Each is halved on grids with
+ * run_cells cells per run of the fused z-face pass and of the
+ * tolerance-bound form's x+y march (2 .. 256, default 32; the x+y march
+ * takes at most 64); 0 keeps the default.  Each is halved on grids with
  * too few column tiles to give 1024 work units.  A tuning knob only; results
  * are identical for every value.  No counterpart in the reference. */
 int cbns_set_tile_shape(cbns_ctx* ctx, int visc_planes, int run_cells);

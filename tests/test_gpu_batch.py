@@ -207,8 +207,8 @@ def test_non_finite_solve_stops_alone():
 def test_tolerance_bound_fp64_solves_in_a_mix_equal_their_single_runs():
     """CBNS_FP64_FAST solves batched with FP64 and FP32-flux solves give their
     single-solve results bit for bit (the same sweep bodies, contraction fixed
-    per function), including a solve scaled by 1e50 whose every face goes
-    through the batch's fixup pass, an odd x-width and a split solve."""
+    per function), including a solve scaled by 1e50 whose every cell goes
+    through the batch's cell fixup, an odd x-width and a split solve."""
     from oracle import oracle as O
 
     ob = O.Block(O.make_params(12))

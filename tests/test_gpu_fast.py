@@ -133,6 +133,36 @@ def test_fast_form_decompositions_identical():
         assert np.array_equal(a.get_state(), b.get_state())
 
 
+@pytest.mark.parametrize("N", [(64, 12, 20), (65, 12, 12), (130, 16, 12)])
+def test_fast_form_x_march_widths(N):
+    """The x+y march forms each row's x faces across the 64 lanes of a
+    wave and takes a cell's right face from the next lane: at width 64 the
+    last lane's right faces are formed before the march, at 65 a second wave
+    holds one cell (its next lane forms face 65), at 130 two full waves
+    precede a partial one.  One block and an x split (widths 32 to 65) give
+    the same state bit for bit, and one step stays within the form's bar of
+    the oracle (1e-12 of each field's maximum; rho w against the momentum
+    scale)."""
+    b = O.Block(O.make_params(N))
+    b.init_tgv()
+    b.run(1)
+    R = b.Q_interior(0)
+    states = []
+    for dims in ((1, 1, 1), (2, 1, 1)):
+        with C.Solver(C.make_params(N), dims, precision=C.CBNS_FP64_FAST) as s:
+            s.init_tgv()
+            s.run(1)
+            states.append(s.get_state())
+            assert abs(s.t_cur - b.t_cur) <= 1e-12 * b.t_cur
+    assert np.array_equal(states[0], states[1])
+    err = rel_linf_own(states[0], R)
+    mom = np.abs(R[..., 1:4]).max()
+    print(f"FP64-FAST TGV {N}, one step: rel Linf (own max) {err}")
+    for nm in ("rho", "rho_u", "rho_v", "E"):
+        assert err[nm] <= 1e-12, (nm, err[nm])
+    assert np.abs(states[0][..., 3] - R[..., 3]).max() / mom <= 1e-12
+
+
 def test_fast_form_kinetic_energy_history():
     """KE and enstrophy over 40 steps at 32^3 within 1e-10 of the oracle's."""
     ob = O.Block(O.make_params(32))
@@ -150,15 +180,17 @@ def test_fast_form_kinetic_energy_history():
         assert worst <= 1e-10
 
 
-@pytest.mark.parametrize("N,dims", [((12, 12, 12), (1, 1, 1)), ((32, 32, 32), (2, 1, 2))])
+@pytest.mark.parametrize("N,dims", [((12, 12, 12), (1, 1, 1)), ((32, 32, 32), (2, 1, 2)),
+                                    ((130, 12, 12), (1, 1, 1))])
 def test_fast_form_scaled_state_faces_take_the_fixup_pass(N, dims):
     """A Taylor-Green state scaled by 1e50 (the same flow: u, T and dt are
     unchanged) drives every WENO denominator past 1e300, so every face fails
-    its checks: the x faces go to the face fixup's list (1,872 at 12^3,
-    33,792 at 32^3), and the cells beside the failing y and z faces to the
-    cell fixup, which forms those faces with the reference-order form.  Two
-    steps against the oracle on the same scaled state.  (Both fixups past
-    their lists: test_tolerance_bound_form_past_the_cell_list.)"""
+    its checks: the x+y march marks every cell and the fused z pass leaves
+    each to the cell fixup's list, which forms the cell's faces with the
+    reference-order form (at width 130 the march's last-lane faces, formed
+    before it, fail as well).  Two steps against the oracle on the same
+    scaled state.  (The cell fixup past its list:
+    test_tolerance_bound_form_past_the_cell_list.)"""
     ob = O.Block(O.make_params(N))
     ob.init_tgv()
     Q0 = ob.Q_interior(0) * 1e50

@@ -44,11 +44,11 @@ def test_bit_exact_form_on_a_scaled_state_equals_the_oracle(N, dims):
 
 def test_tolerance_bound_form_past_the_cell_list():
     """A state scaled by 1e50 fails every face's checks in the
-    tolerance-bound form: at 48^3 the 112,896 x faces exceed the face
-    fixup's list and the 110,592 cells of the fused z pass the cell list,
-    so both fixups check everything (the cell fixup forms every cell's y
-    and z faces); two steps within the form's one-step bar of the oracle
-    (1e-12 of each field's maximum, rho w against the momentum scale)."""
+    tolerance-bound form: at 48^3 the 110,592 cells that the fused z pass
+    leaves exceed the cell list, so the cell fixup checks every cell (and
+    forms every cell's x, y and z faces); two steps within the form's
+    one-step bar of the oracle (1e-12 of each field's maximum, rho w against
+    the momentum scale)."""
     N = (48, 48, 48)
     Q0, R, t = scaled_oracle(N, 1e50, 2)
     with C.Solver(C.make_params(N), precision=C.CBNS_FP64_FAST) as s:
@@ -87,7 +87,7 @@ def test_short_runs_on_small_grids_equal_long_ones(precision):
 @pytest.mark.parametrize("precision", [C.CBNS_FP64, C.CBNS_FP64_FAST, C.CBNS_FP32_FLUX])
 def test_tile_shapes_give_identical_results(precision):
     """The autotuner's tile shapes (planes per viscous z-chunk, cells per run
-    of the fused z pass and the y-face march) change only the work split:
+    of the fused z pass and the x+y march) change only the work split:
     every shape gives the default's state bit for bit, including runs that
     do not divide the grid."""
     N = (40, 36, 44)

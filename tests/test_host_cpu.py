@@ -123,9 +123,10 @@ def test_bench_reads_the_committed_profiles():
 
     traffic = bench.read_traffic()
     valu = bench.read_valu_active()
-    # one tolerance-bound FP64 face sweep at 256^3 moves 1.39 GB
-    # algorithmically (state read once, fluxes written); the VALU issue
-    # fraction of a VALU-bound sweep lies well inside (0.5, 1]
+    # one launch of the tolerance-bound FP64 face kernel (the x+y march) at
+    # 256^3 moves about 1.4 GB algorithmically (the state read once, each
+    # cell's x+y differences written); the VALU issue fraction of a
+    # VALU-bound kernel lies well inside (0.5, 1]
     assert traffic is not None and 1.3e9 < traffic < 3e9
     assert valu is not None and 0.5 < valu <= 1.0
 
