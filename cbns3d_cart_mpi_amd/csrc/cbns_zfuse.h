@@ -365,17 +365,31 @@ constexpr int XY_RUN_MAX = 64;
 // spill).
 __device__ __forceinline__ void xyrun_fast(const BlockDesc& B, int i0, int k, int j0, int ym, int which,
                                            const Phys& ph) {
-  const int lane = (int)(threadIdx.x & 63);
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
   const int n0 = B.n[0], n1 = B.n[1];
   const int i = i0 + lane;
-  const int ic = i < n0 ? i : n0;  // the face this lane forms: left of cell ic
+  // the lane's queue column: its own up to n0 + 2, the last column an x
+  // stencil of a stored cell reaches (ghost columns are marched, not stored)
+  const int cq = i < n0 + 2 ? i : n0 + 2;
   const int j1 = j0 + ym < n1 ? j0 + ym : n1;  // cells [j0, j1), y faces j0 .. j1
   const int sy = B.g[0];
   const long long sg = B.sg;
   gptr<const double> __restrict__ Qin = gmem<const double>(B.Q[which]);
   gptr<double> __restrict__ XY = gmem(B.F[1]);
   const long long sfx = B.sf[1];
-  const int c0 = gidx(B, ic, 0, k);
+  const int c0 = gidx(B, cq, 0, k);
+  // Row buffers of the x stencils, two per wave (rows alternate): position p
+  // holds cell i0 - 3 + p of the row.  Each lane writes its queued cell at
+  // lane + 3; lanes 0 .. 4 load and write the halo cells i0 - 3 .. i0 - 1 and
+  // i0 + 64, i0 + 65 (clamped into the ghosts where only lanes past the
+  // block read them).  Only the wave itself touches its buffers, and its LDS
+  // accesses complete in order, so no barrier is needed.
+  constexpr int XW = 72;
+  __shared__ double xrow[2][ZY][NEQ][XW];
+  const bool hl = lane < 5;
+  const int hcell = lane < 3 ? i0 - 3 + lane : (i0 + 61 + lane < n0 + 3 ? i0 + 61 + lane : n0 + 3);
+  const int hpos = lane < 3 ? lane : 64 + lane;
+  const int h0 = gidx(B, hl ? hcell : cq, 0, k);
   // the last lane's right faces: face i0 + 64 of row j0 + lane (rows past
   // the run repeat its last row)
   double ex[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -391,19 +405,39 @@ __device__ __forceinline__ void xyrun_fast(const BlockDesc& B, int i0, int k, in
   for (int cc = 0; cc < 6; cc++)
 #pragma unroll
     for (int e = 0; e < 5; e++) q[cc][e] = Qin[e * sg + c0 + (j0 - 3 + cc) * sy];
+  {  // row j0 into buffer 0
+    double h[5];
+#pragma unroll
+    for (int e = 0; e < 5; e++) h[e] = hl ? Qin[e * sg + h0 + j0 * sy] : 0.0;
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+      xrow[0][w][e][lane + 3] = q[3][e];
+      if (hl) xrow[0][w][e][hpos] = h[e];
+    }
+  }
   double gp[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, xp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   bool okp = true, okxp = true;
   for (int j = j0; j <= j1; j++) {
-    const bool xrow = j < j1;  // uniform: the last y face of a run closes cell j1 - 1 only
-    double nx[5], xq[6][5];
+    const bool xrow_j = j < j1;   // uniform: the last y face of a run closes cell j1 - 1 only
+    const bool next = j + 1 < j1;  // uniform: row j + 1 has x faces in this run
+    const int bj = (j - j0) & 1;
+    double nx[5];
 #pragma unroll
     for (int e = 0; e < 5; e++) nx[e] = Qin[e * sg + c0 + (j + 3) * sy];  // row j + 3 <= n1 + 3
     double g[5];
     const bool ok = face_flux_fast<1>(q, ph.gamma, ph.gam1, ph.gg1, g);
     double xd[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     bool okx = true;
-    if (xrow) {
-      load_face_cells<0>(B, which, c0 + j * sy, xq);
+    if (xrow_j) {
+      // the halo of row j + 1 goes out before this row's x face
+      double hn[5];
+#pragma unroll
+      for (int e = 0; e < 5; e++) hn[e] = (next && hl) ? Qin[e * sg + h0 + (j + 1) * sy] : 0.0;
+      double xq[6][5];
+#pragma unroll
+      for (int cc = 0; cc < 6; cc++)
+#pragma unroll
+        for (int e = 0; e < 5; e++) xq[cc][e] = xrow[bj][w][e][lane + cc];
       double gx[5];
       const int okl = face_flux_fast<0>(xq, ph.gamma, ph.gam1, ph.gg1, gx);
       const int src = j - j0;
@@ -414,6 +448,13 @@ __device__ __forceinline__ void xyrun_fast(const BlockDesc& B, int i0, int k, in
       }
       const int okr = __shfl_down(okl, 1, 64), okl63 = __builtin_amdgcn_readlane(okex, src);
       okx = okl && (lane == 63 ? okl63 : okr);
+      if (next) {  // row j + 1 (the queue's cell j + 1) into the other buffer
+#pragma unroll
+        for (int e = 0; e < 5; e++) {
+          xrow[bj ^ 1][w][e][lane + 3] = q[4][e];
+          if (hl) xrow[bj ^ 1][w][e][hpos] = hn[e];
+        }
+      }
     }
     if (j > j0 && i < n0) {  // cell j - 1
       const int ct = i + n0 * ((j - 1) + n1 * k);
