@@ -66,8 +66,9 @@ hipError_t batch_grids(BatchGrids* g);
 hipError_t batch_launch_face(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int dir, int which, int jb,
                              int form, int wgs, int* ctr, hipStream_t s);
 // The tolerance-bound solves' x and y faces: the march along y
-// (xyrun_fast) that stores each cell's x- plus y-flux differences, one
-// (column tile, y-run) unit per table entry.
+// (xyrun_fast) that stores each cell's x- plus y-flux differences less its
+// viscous divergence (after their viscous pass), one (column tile, y-run)
+// unit per table entry.
 hipError_t batch_launch_ymarch(const BlockDesc* blocks, InstDesc* insts, TileTable tt, int which, int wgs, int* ctr,
                                hipStream_t s);
 // precision: PREC_FP64, PREC_FP32_FLUX or PREC_FP64_FAST (whose march forms

@@ -322,15 +322,15 @@ static int rebuild(cbns_batch* b) {
       e = batch_launch_visc(b->d_blocks, b->d_insts, b->tt[VISC_CLASS[f]], b->jb, FORM_PREC[f], in,
                             ov ? b->visc_side_wgs : G.visc[f], ctr(), sv);
     if (ov && e == hipSuccess) e = hipEventRecord(b->ev_join, sv);
-    for (int d = 0; d < 2 && e == hipSuccess; d++) {
+    for (int d = 0; d < 2 && e == hipSuccess; d++)
       for (int f = 0; f < 2 && e == hipSuccess; f++)
         e = batch_launch_face(b->d_blocks, b->d_insts, b->tt[FACE_CLASS[f] + d], d, in, b->jb, f, G.face[f][d], ctr(),
                               s);
-      // the tolerance-bound solves' x+y march
-      if (d == 0 && b->has_fast && e == hipSuccess)
-        e = batch_launch_ymarch(b->d_blocks, b->d_insts, b->tt[T_YMF], in, tail(), ctr(), s);
-    }
-    if (ov && e == hipSuccess) e = hipStreamWaitEvent(s, b->ev_join, 0);  // join before the stage tail
+    // join before the tolerance-bound solves' x+y march (which reads their
+    // viscous divergence) and the stage tail
+    if (ov && e == hipSuccess) e = hipStreamWaitEvent(s, b->ev_join, 0);
+    if (b->has_fast && e == hipSuccess)
+      e = batch_launch_ymarch(b->d_blocks, b->d_insts, b->tt[T_YMF], in, tail(), ctr(), s);
     for (int f = 0; f < 3 && e == hipSuccess; f++)
       e = batch_launch_zupdate(b->d_blocks, b->d_insts, b->tt[UPD_CLASS[f]], FORM_PREC[f], in, out, stage, tail(),
                                ctr(), b->cfix, s);

@@ -592,15 +592,15 @@ int cbns_set_time(cbns_ctx* c, double t) {
   return CBNS_OK;
 }
 
-// src/cbns3d.cpp:203-240 per stage: the x and y face sweeps (the
-// tolerance-bound form: one march over its x and y faces that stores each
-// cell's x+y flux differences), the viscous
-// march (beside them on the side stream when overlap is on), the z-face
-// sweep fused with the RK3 update (and, but for the tolerance-bound form,
-// the primitives) with its cell fixup, and the ghost gather of the state
-// (with its primitives).  The contracted FP64 form keeps a separate z sweep
-// and update.  launch_sweeps issues the face sweeps (which read only the
-// stage input), finish_stage the rest.
+// src/cbns3d.cpp:203-240 per stage: the x and y face sweeps and the
+// viscous march (beside them on the side stream when overlap is on; the
+// tolerance-bound form runs its viscous march first, then one march over its
+// x and y faces that stores each cell's x+y flux differences less the
+// viscous divergence), the z-face sweep fused with the RK3 update (and, but
+// for the tolerance-bound form, the primitives) with its cell fixup, and the
+// ghost gather of the state (with its primitives).  The contracted FP64 form
+// keeps a separate z sweep and update.  launch_sweeps issues the kernels
+// that read only the stage input, finish_stage the rest.
 // Whether a form's steps run the z-face sweep fused with the stage tail
 // (every form but the contracted FP64 one).
 static bool zfused(const cbns_ctx* c) { return c->precision != CBNS_FP64_FMA; }
@@ -620,11 +620,16 @@ static void stage_io(const cbns_ctx* c, int stage, int* in, int* out) {
   }
 }
 
+// Whether the viscous pass of a stage runs beside its face sweeps.  The
+// tolerance-bound form's x+y march reads the divergence, so its viscous
+// pass runs first, on the context's stream.
+static bool side_viscous(const cbns_ctx* c) { return c->overlap && c->precision != CBNS_FP64_FAST; }
+
 static int launch_sweeps(cbns_ctx* c, int stage) {
   int in, out;
   stage_io(c, stage, &in, &out);
   const BlockDesc* hb = c->hb.data();
-  if (c->overlap) {
+  if (side_viscous(c)) {
     CK(hipEventRecord(c->ev_fork, c->stream));
     CK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     CK(timed_on(c, c->side, 3, [&] {
@@ -635,8 +640,12 @@ static int launch_sweeps(cbns_ctx* c, int stage) {
   }
   // the z faces are formed by the fused stage tail
   if (c->precision == CBNS_FP64_FAST) {
-    // the x and y faces in one march that stores each cell's x- plus
-    // y-flux differences (timed as the y class: it carries both axes)
+    // the viscous march from the stage input, then the x and y faces in one
+    // march that stores each cell's x- plus y-flux differences less the
+    // viscous divergence (timed as the y class: it carries both axes)
+    CK(timed(c, 3, [&] {
+      return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->stream, 0, in, c->visc_top);
+    }));
     CK(timed(c, 1, [&] { return launch_xyface(c->db, hb, c->dom.nb, in, c->ph, c->stream, c->run_top); }));
     return CBNS_OK;
   }
@@ -657,11 +666,10 @@ static int finish_stage(cbns_ctx* c, int stage, double dt) {
   // viscous march forms them from the stage input, its stage tail (fused
   // with its z faces) skips them and its ghost fill copies the state alone
   const bool lean = c->precision == CBNS_FP64_FAST;
-  if (c->overlap) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
-  else
+  if (side_viscous(c)) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  else if (!lean)  // (the tolerance-bound form's ran before its x+y march)
     CK(timed(c, 3, [&] {
-      return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->stream, 0, lean ? in : -1,
-                              c->visc_top);
+      return launch_visc_flux(c->db, hb, c->dom.nb, c->ph, c->precision, c->jband, c->stream, 0, -1, c->visc_top);
     }));
   bool s_done = false;
   if (stage == 3) {  // the next step's calc_dt reduction rides on the last update
@@ -749,7 +757,7 @@ int cbns_step(cbns_ctx* c, double* dt_out, int* finished) {
     // the issued sweeps change no state, but with overlap on the side
     // stream's viscous kernel of stage 1 writes mu and the viscous fields:
     // order every later call on the context's stream after it
-    if (first_issued && c->overlap) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    if (first_issued && side_viscous(c)) CK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     return CBNS_ERR_NONFINITE;
   }
   // src/cbns3d.cpp:191-194

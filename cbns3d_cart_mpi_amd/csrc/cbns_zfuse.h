@@ -42,7 +42,8 @@ using zft = typename std::conditional<FORM == ZF_FP32, float, double>::type;
 
 // Operands of the update of one cell other than its z-fluxes and its stage
 // input, in their storage types.  XY (tolerance-bound form): xy holds the
-// cell's x- plus y-flux differences (xyrun_fast) and e0 .. f1 are unused.
+// cell's x- plus y-flux differences less its viscous divergence
+// (xyrun_fast), and e0 .. f1 and dv are unused.
 template <typename FT>
 struct ZOperands {
   FT e0[5], e1[5], f0[5], f1[5], dv[4];
@@ -74,8 +75,10 @@ __device__ __forceinline__ void zload(const BlockDesc& B, int i, int j, int k, Z
     }
     o.qn[e] = Qn[e * qs + qc];  // used by stages 2 and 3 (loaded always: no branch among the loads)
   }
+  if constexpr (!XY) {
 #pragma unroll
-  for (int e = 0; e < 4; e++) o.dv[e] = DV[e * B.sn + ct];
+    for (int e = 0; e < 4; e++) o.dv[e] = DV[e * B.sn + ct];
+  }
 }
 
 // x- plus y-flux differences of one cell from its x-face fluxes e0, e1 and
@@ -91,9 +94,10 @@ __device__ __forceinline__ double xy_diff(double e0, double e1, double f0, doubl
 
 // The update of interior cell (i, j, k) with the expressions and order of
 // update_pair_body: x-, y- and z-flux differences less the viscous
-// divergence, then the RK3 combination of Q^n (for stage 1 the stage input
-// qi itself) and qi, into Q[out]; the plain and FP32-flux forms also store
-// the cell's primitives.  With want_s, the calc_dt spectral radius of the
+// divergence (the tolerance-bound form: its x+y differences already less
+// the divergence, then the z differences), then the RK3 combination of Q^n
+// (for stage 1 the stage input qi itself) and qi, into Q[out]; the plain
+// and FP32-flux forms also store the cell's primitives.  With want_s, the calc_dt spectral radius of the
 // updated cell (src/Block3d.cpp:527-546, mu of this stage's viscous pass)
 // goes into *s (a NaN never wins; *bad flags a non-finite one).
 template <int FORM>
@@ -110,7 +114,7 @@ __device__ __forceinline__ void zapply(const BlockDesc& B, int i, int j, int k, 
     double df = FORM == ZF_FAST ? o.xy[e]
                                 : xy_diff((double)o.e0[e], (double)o.e1[e], (double)o.f0[e], (double)o.f1[e], ph);
     df += divr<FORM == ZF_FAST>(gk1[e] - gk[e], ph.dz, ph.rdz, ph.fdz);
-    if (e > 0) df -= (double)o.dv[e - 1];
+    if (FORM != ZF_FAST && e > 0) df -= (double)o.dv[e - 1];  // the tolerance-bound xy holds it
     if (1 == stage) r[e] = qi[e] - dt * df;
     else if (2 == stage) r[e] = 0.75 * o.qn[e] + 0.25 * (qi[e] - dt * df);
     else r[e] = (o.qn[e] + 2.0 * (qi[e] - dt * df)) / 3.0;
@@ -318,9 +322,14 @@ __device__ __forceinline__ void zcell_fix(const BlockDesc& B, int i, int j, int 
   if (all && ok0 && ok1 && okx0 && okx1 && oky0 && oky1) return;
   ZOperands<double> o;
   zload<double, FORM == ZF_FAST>(B, i, j, k, o, stage);
-  if constexpr (FORM == ZF_FAST) {
+  if constexpr (FORM == ZF_FAST) {  // as xyrun_fast forms them
+    gptr<const double> __restrict__ DV = gmem<const double>(B.dfv);
+    const int ct = i + B.n[0] * (j + B.n[1] * k);
 #pragma unroll
-    for (int e = 0; e < 5; e++) o.xy[e] = xy_diff<true>(e0[e], e1[e], f0[e], f1[e], ph);
+    for (int e = 0; e < 5; e++) {
+      o.xy[e] = xy_diff<true>(e0[e], e1[e], f0[e], f1[e], ph);
+      if (e > 0) o.xy[e] -= DV[(e - 1) * B.sn + ct];
+    }
   }
   double qc[5];
 #pragma unroll
@@ -342,7 +351,13 @@ constexpr int XY_RUN_MAX = 64;
 
 // Tolerance-bound form: the x and y faces of a stage in one march, which
 // stores each interior cell's x- plus y-flux differences (xy_diff<true>)
-// into B.F[1] at the cell's interior index; no face flux reaches memory.
+// less its viscous divergence (B.dfv, from the viscous pass of the stage,
+// which runs before the march) into B.F[1] at the cell's interior index;
+// no face flux reaches memory.  The fused z pass adds the z differences:
+// the reference sums x, y, z, then subtracts the divergence
+// (src/Block3d.cpp:932-940); the form moves the subtraction before the z
+// term, a reassociation within its tolerance that takes four loads per cell
+// off the memory-heavier z pass.
 // One wave owns the 64 columns i0 .. i0 + 63 at one k (lane l: i = i0 + l)
 // and marches rows [j0, j0 + ym), ym <= 64: the y faces j0 .. j1 with a
 // six-cell register queue of the stage input along each lane's column (as
@@ -355,7 +370,7 @@ constexpr int XY_RUN_MAX = 64;
 // nothing, so every lane of the wave takes part in every step and the lane
 // exchanges see defined values.  Cell j's x part divr(E(i+1) - E(i), dx) is
 // formed at row j and its y part added after y face j + 1, the order and
-// rounding of xy_diff<true>.  A cell beside any x or y face whose range
+// rounding of xy_diff<true>, then the divergence subtracted.  A cell beside any x or y face whose range
 // checks fail gets a NaN in place of its differences: the fused z pass then
 // leaves it to the cell fixup, which forms its faces with the
 // reference-order form.
@@ -376,7 +391,8 @@ __device__ __forceinline__ void xyrun_fast(const BlockDesc& B, int i0, int k, in
   const long long sg = B.sg;
   gptr<const double> __restrict__ Qin = gmem<const double>(B.Q[which]);
   gptr<double> __restrict__ XY = gmem(B.F[1]);
-  const long long sfx = B.sf[1];
+  gptr<const double> __restrict__ DV = gmem<const double>(B.dfv);
+  const long long sfx = B.sf[1], sn = B.sn;
   const int c0 = gidx(B, cq, 0, k);
   // Row buffers of the x stencils, two per wave (rows alternate): position p
   // holds cell i0 - 3 + p of the row.  Each lane writes its queued cell at
@@ -426,6 +442,11 @@ __device__ __forceinline__ void xyrun_fast(const BlockDesc& B, int i0, int k, in
     for (int e = 0; e < 5; e++) nx[e] = Qin[e * sg + c0 + (j + 3) * sy];  // row j + 3 <= n1 + 3
     double g[5];
     const bool ok = face_flux_fast<1>(q, ph.gamma, ph.gam1, ph.gg1, g);
+    // the divergence of cell j - 1 (a lane past the block reads cell n0 - 1's)
+    double dv[4];
+    const int ct = (i < n0 ? i : n0 - 1) + n0 * ((j > j0 ? j - 1 : j0) + n1 * k);
+#pragma unroll
+    for (int e = 0; e < 4; e++) dv[e] = DV[e * sn + ct];
     double xd[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     bool okx = true;
     if (xrow_j) {
@@ -457,12 +478,12 @@ __device__ __forceinline__ void xyrun_fast(const BlockDesc& B, int i0, int k, in
       }
     }
     if (j > j0 && i < n0) {  // cell j - 1
-      const int ct = i + n0 * ((j - 1) + n1 * k);
       const bool good = ok && okp && okxp;
 #pragma unroll
       for (int e = 0; e < 5; e++) {
         double df = xp[e];
         df += divr<true>(g[e] - gp[e], ph.dy, ph.rdy, ph.fdy);
+        if (e > 0) df -= dv[e - 1];
         XY[e * sfx + ct] = good ? df : __builtin_nan("");
       }
     }

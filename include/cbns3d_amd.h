@@ -168,7 +168,9 @@ int cbns_diagnostics(cbns_ctx* ctx, double* ke, double* enstrophy);
 int cbns_sync(cbns_ctx* ctx);
 
 /* Run the viscous kernels of each stage on a second stream beside the face
- * fluxes (default on).  Results are identical either way. */
+ * fluxes (default on).  Results are identical either way.  The
+ * tolerance-bound form's face kernel reads the viscous divergence, so its
+ * stages run the two in turn whatever this says. */
 int cbns_set_overlap(cbns_ctx* ctx, int on);
 /* With overlap on: the side-stream viscous kernel runs on this many
  * workgroups that take its tile units in turn, so that it occupies fewer of
