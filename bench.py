@@ -316,13 +316,13 @@ def serve_leg(n):
 def zfused_bytes_per_cell(precision):
     """Algorithmic HBM bytes per cell of one launch of the fused z-face and
     stage-tail kernel, averaged over the three stages: the x- and y-face
-    fluxes (5 each; the tolerance-bound form reads instead the 5 x- plus
-    y-flux differences its y-face march stored), the viscous divergence (4),
-    the stage input (5, read once along the column's march), Q^n (5, stages
-    2 and 3 only) and the updated state (5 written); the bit-exact form also
-    writes 6 primitives."""
-    fluxes = 5 if precision == 3 else 10
-    b = 8 * (fluxes + 4 + 5 + 5) + 8 * 5 * 2 / 3
+    fluxes (5 each) and the viscous divergence (4) (the tolerance-bound form
+    reads instead the 5 values its x+y march stored: the x- plus y-flux
+    differences less the divergence), the stage input (5, read once along
+    the column's march), Q^n (5, stages 2 and 3 only) and the updated state
+    (5 written); the bit-exact form also writes 6 primitives."""
+    operands = 5 if precision == 3 else 10 + 4
+    b = 8 * (operands + 5 + 5) + 8 * 5 * 2 / 3
     return b + (48 if precision == 0 else 0)
 
 
@@ -340,9 +340,12 @@ def plain_leg(C, N, dims, precision, steps, warmup, fixed=None):
     tuned = None
     tuned_steps = 0
     if fixed is None:
-        # the tolerance-bound form's face kernel (the x+y march) has one register variant
-        occs = (2,) if precision == C.CBNS_FP64_FAST else (2, 3)
-        best, tms = autotune.tune(s, steps=2, bands=(16, 32, 64), schedules=autotune.SCHEDULES, occs=occs)
+        # the tolerance-bound form's face kernel (the x+y march) has one register
+        # variant and reads the viscous divergence (its stages run serially)
+        fast = precision == C.CBNS_FP64_FAST
+        best, tms = autotune.tune(s, steps=2, bands=(16, 32, 64),
+                                  schedules=("serial",) if fast else autotune.SCHEDULES,
+                                  occs=(2,) if fast else (2, 3))
         tuned_steps = 3 * len(tms)
         tuned = {"band": best[0], "schedule": best[1], "face_waves_per_simd": best[2],
                  "visc_planes_run_cells": list(best[3]),
@@ -463,7 +466,8 @@ def main():
     roof = {"bound": "valu", "achieved": fr["achieved"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": fr["frac"], "traffic": read_traffic(),
             "kernel": "tolerance-bound FP64 face kernel k_xyface: the x and y faces of a stage in one march "
-                      "along y, which stores each cell's x+y flux differences (timed as face_y); FP64 "
+                      "along y, which stores each cell's x+y flux differences less its viscous divergence "
+                      "(timed as face_y); FP64 "
                       "VALU-issue bound; achieved = 1570 algorithmic flop per face (literal count of the "
                       "reference) x (x faces + y faces) per launch / its average launch duration in the timed "
                       "steps; the z faces are in the fused stage tail (fused_z_update)",
