@@ -441,12 +441,17 @@ def main():
     head, s = plain_leg(C, N, dims, C.CBNS_FP64_FAST, a.steps, a.warmup, fixed(int(a.occ) if given else 3))
     ke, en = s.diagnostics()
     mem = s.memory_bytes
-    steps_head = head["steps_before_timed"] + a.steps + BREAKDOWN_STEPS
-    Q_fast = s.get_state()
-    s.close()
+    s_fast = s
     exact, s = plain_leg(C, N, dims, C.CBNS_FP64, a.steps, a.warmup, fixed(2))
     # both forms' states after the same number of steps from the same start
-    s.run(steps_head - (exact["steps_before_timed"] + a.steps + BREAKDOWN_STEPS))
+    # (the autotuners may have run different numbers of steps)
+    done_fast = head["steps_before_timed"] + a.steps + BREAKDOWN_STEPS
+    done_exact = exact["steps_before_timed"] + a.steps + BREAKDOWN_STEPS
+    steps_head = max(done_fast, done_exact)
+    s_fast.run(steps_head - done_fast)
+    s.run(steps_head - done_exact)
+    Q_fast = s_fast.get_state()
+    s_fast.close()
     Q64 = s.get_state()
     s.close()
     fast_err = {nm: float(np.abs(Q_fast[..., e] - Q64[..., e]).max() / np.abs(Q64[..., e]).max())
