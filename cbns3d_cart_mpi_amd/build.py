@@ -22,7 +22,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["cbns_kernels.hip", "cbns_kernels_fma.hip", "cbns_kernels_fast.hip", "cbns_kernels_zfuse.hip",
-           "cbns_capi.cpp", "cbns_batch.hip", "cbns_batch_capi.cpp"]
+           "cbns_capi.cpp", "cbns_batch.hip", "cbns_batch_capi.cpp", "cbns_spectra.hip"]
 # the contracted FP64 form's face kernels are the only code built with FMA
 # contraction (the tolerance-bound form writes its fused multiply-adds out)
 CONTRACT = {"cbns_kernels_fma.hip": ["-ffp-contract=fast"]}
@@ -42,7 +42,8 @@ def build_lib(verbose_resources=False):
         flags = [f for f in HIP_FLAGS if not (s in CONTRACT and f.startswith("-ffp-contract"))]
         run([HIPCC, *flags, *CONTRACT.get(s, []), *extra, "-x", "hip", "-c", src, "-o", obj])
         objs.append(obj)
-    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", LIB])
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", LIB, "-L/opt/rocm/lib", "-lhipfft",
+         "-Wl,-rpath,/opt/rocm/lib"])
     return LIB
 
 
@@ -58,7 +59,7 @@ def build_host():
         return None
     os.makedirs(BIN, exist_ok=True)
     exe = os.path.join(BIN, "cbns3d_amd")
-    run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + HOST, *srcs,
+    run(["g++", "-O2", "-std=c++17", "-Wall", "-pthread", "-I" + os.path.join(ROOT, "include"), "-I" + HOST, *srcs,
          "-o", exe, "-L" + HERE, "-lcbns3d_amd", "-Wl,-rpath,$ORIGIN/.."])
     return exe
 

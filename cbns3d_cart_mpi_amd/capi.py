@@ -62,6 +62,12 @@ _SIGS = {
     "cbns_get_block_state": (_I, [_VP, _I, _I, _PD]),
     "cbns_set_block_state": (_I, [_VP, _I, _I, _PD]),
     "cbns_get_block_primitive": (_I, [_VP, _I, _I, _PD]),
+    "cbns_get_block_mu": (_I, [_VP, _I, _PD]),
+    "cbns_set_block_mu": (_I, [_VP, _I, _PD]),
+    "cbns_snapshot_begin": (_I, [_VP]),
+    "cbns_snapshot_wait": (_I, [_VP, _PD]),
+    "cbns_snapshot_block_state": (_I, [_VP, _I, _PD]),
+    "cbns_snapshot_block_primitive": (_I, [_VP, _I, _I, _PD]),
     "cbns_boundary_condition": (_I, [_VP, _I]),
     "cbns_calc_primitive": (_I, [_VP, _I]),
     "cbns_init_Q_p": (_I, [_VP]),
@@ -74,6 +80,7 @@ _SIGS = {
     "cbns_get_time": (_I, [_VP, _PD]),
     "cbns_set_time": (_I, [_VP, _D]),
     "cbns_diagnostics": (_I, [_VP, _PD, _PD]),
+    "cbns_spectra": (_I, [_VP, _I, _PD, _PD, _PD]),
     "cbns_sync": (_I, [_VP]),
     "cbns_face_flux_eval": (_I, [_I, _I, _D, _I, _PD, _PD, _PD]),
     "cbns_face_flux_eval_noalloc": (_I, [_I, _I, _D, _I, _PD, _PD, _PD]),
@@ -236,6 +243,41 @@ class Solver:
         check(lib().cbns_get_block_primitive(self._h, b, f, _p(out)), "cbns_get_block_primitive")
         return out
 
+    def get_block_mu(self, b):
+        """Interior mu of block b, [k][j][i]: the viscosity the next dt reads."""
+        n, _ = self.block_info(b)
+        out = np.empty((n[2], n[1], n[0]))
+        check(lib().cbns_get_block_mu(self._h, b, _p(out)), "cbns_get_block_mu")
+        return out
+
+    def set_block_mu(self, b, mu):
+        a = np.ascontiguousarray(mu, dtype=np.float64)
+        n, _ = self.block_info(b)
+        assert a.size == n[0] * n[1] * n[2]
+        check(lib().cbns_set_block_mu(self._h, b, _p(a)), "cbns_set_block_mu")
+
+    def snapshot_begin(self):
+        """Asynchronous snapshot of Q and the primitives (read with the
+        snapshot_* methods; valid until the next snapshot_begin)."""
+        check(lib().cbns_snapshot_begin(self._h), "cbns_snapshot_begin")
+
+    def snapshot_wait(self):
+        t = _D()
+        check(lib().cbns_snapshot_wait(self._h, ctypes.byref(t)), "cbns_snapshot_wait")
+        return t.value
+
+    def snapshot_block_state(self, b):
+        n, _ = self.block_info(b)
+        out = np.empty((n[2] + 8, n[1] + 8, n[0] + 8, 5))
+        check(lib().cbns_snapshot_block_state(self._h, b, _p(out)), "cbns_snapshot_block_state")
+        return out
+
+    def snapshot_block_primitive(self, b, f):
+        n, _ = self.block_info(b)
+        out = np.empty((n[2] + 8, n[1] + 8, n[0] + 8))
+        check(lib().cbns_snapshot_block_primitive(self._h, b, f, _p(out)), "cbns_snapshot_block_primitive")
+        return out
+
     # method level
     def boundary_condition(self, which=CBNS_Q):
         check(lib().cbns_boundary_condition(self._h, which), "cbns_boundary_condition")
@@ -282,6 +324,16 @@ class Solver:
         ke, en = _D(), _D()
         check(lib().cbns_diagnostics(self._h, ctypes.byref(ke), ctypes.byref(en)), "cbns_diagnostics")
         return ke.value, en.value
+
+    def spectra(self, nshell=None):
+        """Device shell spectra of the velocity of Q: (ke, enstrophy,
+        dilatation), each of nshell shells (default: up to the largest
+        |k| of the grid)."""
+        if nshell is None:
+            nshell = int(np.ceil(np.sqrt(sum((n // 2) ** 2 for n in self.N)))) + 1
+        out = [np.zeros(nshell) for _ in range(3)]
+        check(lib().cbns_spectra(self._h, int(nshell), *(_p(o) for o in out)), "cbns_spectra")
+        return tuple(out)
 
     def sync(self):
         check(lib().cbns_sync(self._h), "cbns_sync")

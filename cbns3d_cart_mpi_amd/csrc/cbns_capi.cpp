@@ -63,6 +63,18 @@ struct cbns_ctx {
   // form them from the state): set after such a step, cleared by
   // refresh_prims before any call that reads the stored primitives
   bool prim_stale = false;
+  // snapshot for file output: Q and the primitives rho u v w p of every
+  // block, copied on the stream into snap_dev, then to pinned snap_host on
+  // the copy stream while later work runs (ev_snap_host marks the end)
+  hipStream_t copy = nullptr;
+  hipEvent_t ev_snap_dev = nullptr, ev_snap_host = nullptr;
+  char* snap_dev = nullptr;
+  char* snap_host = nullptr;
+  size_t snap_bytes = 0;
+  std::vector<size_t> snap_off;  // per block: byte offset of Q; its primitives follow
+  bool snap_pending = false;
+  double snap_t = 0.0;
+  Spectra* spec = nullptr;  // velocity spectra: FFT plan and buffers (first cbns_spectra call)
   // live kernel timing
   unsigned prof = 0;  // bit k: time the launches of kernel class k
   std::vector<hipEvent_t> ev_pool;
@@ -323,6 +335,15 @@ int cbns_create(const cbns_params* p, const int dims[3], int precision, cbns_ctx
 int cbns_destroy(cbns_ctx* c) {
   if (!c) return CBNS_ERR_ARG;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->copy) {
+    (void)hipStreamSynchronize(c->copy);
+    (void)hipStreamDestroy(c->copy);
+  }
+  if (c->ev_snap_dev) (void)hipEventDestroy(c->ev_snap_dev);
+  if (c->ev_snap_host) (void)hipEventDestroy(c->ev_snap_host);
+  if (c->snap_dev) (void)hipFree(c->snap_dev);
+  spectra_destroy(c->spec);
+  if (c->snap_host) (void)hipHostFree(c->snap_host);
   harvest(c);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->side) (void)hipStreamSynchronize(c->side);
@@ -575,6 +596,131 @@ int cbns_get_block_primitive(cbns_ctx* c, int b, int f, double* out) {
       for (int i = 0; i < B.n[0]; i++)
         out[(i + NG) + (long long)B.g[0] * ((j + NG) + (long long)B.g[1] * (k + NG))] =
             mu[i + (long long)B.n[0] * (j + (long long)B.n[1] * k)];
+  return CBNS_OK;
+}
+
+// ---- asynchronous snapshot ----------------------------------------------
+
+// bytes of one block's primitive field in the snapshot (their storage type)
+static size_t snap_prim_bytes(const cbns_ctx* c) {
+  return c->precision == CBNS_FP32_FLUX ? sizeof(float) : sizeof(double);
+}
+
+int cbns_snapshot_begin(cbns_ctx* c) {
+  if (!c) return CBNS_ERR_ARG;
+  if (!c->have_state) return CBNS_ERR_STATE;
+  const int nb = c->dom.nb;
+  const size_t pb = snap_prim_bytes(c);
+  if (!c->snap_dev) {  // first snapshot: buffers, copy stream, events
+    c->snap_off.resize(nb);
+    size_t off = 0;
+    for (int b = 0; b < nb; b++) {
+      c->snap_off[b] = off;
+      off += (size_t)c->hb[b].sg * (NEQ * sizeof(double) + 5 * pb);
+      off = (off + 255) & ~(size_t)255;
+    }
+    if (hipMalloc(&c->snap_dev, off) != hipSuccess) {
+      c->snap_dev = nullptr;
+      return CBNS_ERR_NOMEM;
+    }
+    if (hipHostMalloc((void**)&c->snap_host, off, hipHostMallocDefault) != hipSuccess) {
+      c->snap_host = nullptr;
+      (void)hipFree(c->snap_dev);
+      c->snap_dev = nullptr;
+      return CBNS_ERR_NOMEM;
+    }
+    c->snap_bytes = off;
+    CK(hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking));
+    CK(hipEventCreateWithFlags(&c->ev_snap_dev, hipEventDisableTiming));
+    CK(hipEventCreateWithFlags(&c->ev_snap_host, hipEventDisableTiming));
+  }
+  // the previous snapshot's host copy must end before its device copy is overwritten
+  if (c->snap_pending) CK(hipStreamWaitEvent(c->stream, c->ev_snap_host, 0));
+  // with overlap on, the side stream may still write mu and the viscous fields,
+  // never Q or the primitives: the stream's order is enough
+  int st = refresh_prims(c);
+  if (st) return st;
+  for (int b = 0; b < nb; b++) {
+    const BlockDesc& B = c->hb[b];
+    char* dst = c->snap_dev + c->snap_off[b];
+    CK(hipMemcpyAsync(dst, B.Q[CBNS_Q], (size_t)B.sg * NEQ * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    CK(hipMemcpyAsync(dst + (size_t)B.sg * NEQ * sizeof(double), B.prim, (size_t)B.sg * 5 * pb,
+                      hipMemcpyDeviceToDevice, c->stream));
+  }
+  CK(hipEventRecord(c->ev_snap_dev, c->stream));
+  CK(hipStreamWaitEvent(c->copy, c->ev_snap_dev, 0));
+  CK(hipMemcpyAsync(c->snap_host, c->snap_dev, c->snap_bytes, hipMemcpyDeviceToHost, c->copy));
+  CK(hipEventRecord(c->ev_snap_host, c->copy));
+  c->snap_pending = true;
+  c->snap_t = c->par.t_cur;
+  return CBNS_OK;
+}
+
+int cbns_snapshot_wait(cbns_ctx* c, double* t_cur) {
+  if (!c) return CBNS_ERR_ARG;
+  if (!c->snap_pending) return CBNS_ERR_STATE;
+  CK(hipEventSynchronize(c->ev_snap_host));
+  if (t_cur) *t_cur = c->snap_t;
+  return CBNS_OK;
+}
+
+int cbns_snapshot_block_state(cbns_ctx* c, int b, double* out) {
+  if (!c || !out || b < 0 || b >= c->dom.nb) return CBNS_ERR_ARG;
+  int st = cbns_snapshot_wait(c, nullptr);
+  if (st) return st;
+  const BlockDesc& B = c->hb[b];
+  const double* soa = (const double*)(c->snap_host + c->snap_off[b]);
+  for (long long t = 0; t < B.sg; t++)
+    for (int e = 0; e < NEQ; e++) out[NEQ * t + e] = soa[e * B.sg + t];
+  return CBNS_OK;
+}
+
+int cbns_snapshot_block_primitive(cbns_ctx* c, int b, int f, double* out) {
+  if (!c || !out || b < 0 || b >= c->dom.nb || f < 0 || f > 4) return CBNS_ERR_ARG;
+  int st = cbns_snapshot_wait(c, nullptr);
+  if (st) return st;
+  const BlockDesc& B = c->hb[b];
+  const char* p = c->snap_host + c->snap_off[b] + (size_t)B.sg * NEQ * sizeof(double);
+  if (c->precision == CBNS_FP32_FLUX) {  // float storage, widened
+    const float* pf = (const float*)p + (size_t)f * B.sg;
+    for (long long t = 0; t < B.sg; t++) out[t] = pf[t];
+  } else {
+    std::memcpy(out, (const double*)p + (size_t)f * B.sg, (size_t)B.sg * sizeof(double));
+  }
+  return CBNS_OK;
+}
+
+int cbns_spectra(cbns_ctx* c, int nshell, double* ke, double* enstrophy, double* dilatation) {
+  if (!c || nshell < 1) return CBNS_ERR_ARG;
+  if (!c->have_state) return CBNS_ERR_STATE;
+  if (!c->spec && !(c->spec = spectra_create())) return CBNS_ERR_NOMEM;
+  std::vector<double> h(3 * (size_t)nshell);
+  const int r = spectra_run(c->spec, c->db, c->hb.data(), c->dom, CBNS_Q, nshell, h.data(), c->stream);
+  if (r == 1) return CBNS_ERR_NOMEM;
+  if (r) return CBNS_ERR_HIP;
+  double* outs[3] = {ke, enstrophy, dilatation};
+  for (int a = 0; a < 3; a++)
+    if (outs[a]) std::memcpy(outs[a], h.data() + (size_t)a * nshell, sizeof(double) * nshell);
+  return CBNS_OK;
+}
+
+int cbns_get_block_mu(cbns_ctx* c, int b, double* mu) {
+  if (!c || !mu || b < 0 || b >= c->dom.nb) return CBNS_ERR_ARG;
+  const BlockDesc& B = c->hb[b];
+  CK(hipStreamSynchronize(c->stream));
+  CK(hipStreamSynchronize(c->side));
+  CK(hipMemcpy(mu, B.mu, B.sn * sizeof(double), hipMemcpyDeviceToHost));
+  return CBNS_OK;
+}
+
+int cbns_set_block_mu(cbns_ctx* c, int b, const double* mu) {
+  if (!c || !mu || b < 0 || b >= c->dom.nb) return CBNS_ERR_ARG;
+  const BlockDesc& B = c->hb[b];
+  CK(hipStreamSynchronize(c->stream));
+  CK(hipStreamSynchronize(c->side));
+  CK(hipMemcpy(B.mu, mu, B.sn * sizeof(double), hipMemcpyHostToDevice));
+  c->s_ready = false;  // the next dt reads the new mu
+  c->dt_inflight = false;
   return CBNS_OK;
 }
 

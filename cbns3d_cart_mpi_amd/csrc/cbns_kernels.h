@@ -236,4 +236,15 @@ hipError_t launch_face_eval_fma(int dir, const Phys& ph, int nfaces, const doubl
 // RN(1/h) and whether three-operation division by h is exact (see Phys).
 void reciprocal_check(double h, double* rh, int* fast);
 
+// Shell spectra of the velocity field (cbns_spectra.hip): buffers and FFT
+// plan per context, made at the first call.  spectra_run writes the kinetic
+// energy, enstrophy and dilatation spectra of shells 0 .. nshell - 1 to
+// host_out[0..3 nshell) and synchronises s; 0 ok, 1 allocation failed, 2 HIP
+// or hipFFT error.
+struct Spectra;
+Spectra* spectra_create();
+void spectra_destroy(Spectra* sp);
+int spectra_run(Spectra* sp, const BlockDesc* dblocks, const BlockDesc* hblocks, const Domain& dom, int which,
+                int nshell, double* host_out, hipStream_t s);
+
 }  // namespace cbns

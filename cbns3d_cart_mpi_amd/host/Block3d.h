@@ -10,7 +10,12 @@
 #ifndef CBNS_HOST_BLOCK3D_H
 #define CBNS_HOST_BLOCK3D_H
 
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "process_info.h"
@@ -30,6 +35,26 @@ class Block3d {
 
   cbns_ctx* ctx = nullptr;
   int status = 0;  // last C-ABI status
+  bool mu_sidecar = false;
+
+  // asynchronous output: writer pool, its job queue and the snapshot's state
+  bool async_io = false;
+  long epoch = 0;        // bumped by every call that may change Q or the primitives
+  long snap_epoch = -1;  // the epoch the context's snapshot holds
+  std::vector<std::thread> io_workers;
+  std::deque<std::function<int()>> io_jobs;
+  std::mutex io_m;
+  std::condition_variable io_cv, io_idle_cv;
+  int io_busy = 0;
+  bool io_stop = false;
+  int io_err = 0;
+  void io_start(int threads);
+  void io_stop_all();
+  void io_drain();
+  int io_take_error();
+  int io_snapshot();
+  void io_enqueue(std::function<int()> job);
+  void touch() { epoch++; }
 
   // handles standing for Q and Q_p in the reference's pointer-passing calls
   value_type tag_Q = 0.0, tag_Q_p = 0.0;
@@ -71,6 +96,27 @@ class Block3d {
   // refused (-1).
   int read_bin(const std::string fname);
   int output_bin(const std::string fname);
+
+  // mu sidecar (no counterpart in the reference): with it on, output_bin also
+  // writes each block's interior mu to "<checkpoint file>.mu" (t_cur, the
+  // interior sizes as three size_type, then KM x JM x IM doubles).  read_bin
+  // loads a block's sidecar whenever that file exists, so the resumed run's
+  // first dt uses the mu of the uninterrupted run and the whole resumed run
+  // is bit-identical to it; without a sidecar mu is zero at the first dt, as
+  // in the reference's restart path.
+  void set_mu_sidecar(bool on) { mu_sidecar = on; }
+
+  // Asynchronous file output (no counterpart in the reference, whose writers
+  // format on the calling rank's thread while the run waits).  With it on,
+  // output_bin and output_vtr take a device snapshot of the state (one per
+  // state: a checkpoint and a VTK output of the same step share it) and
+  // return at once; `threads` writer threads write the files from the
+  // snapshot's pinned host copy while the next steps run.  The files are
+  // those of the synchronous writers, byte for byte.  A writer's failure is
+  // returned (-1) by the next output call or by io_wait(), which waits for
+  // every pending write; the destructor waits too.
+  void set_async_io(bool on, int threads = 4);
+  int io_wait();
 
   friend void boundary_condition(Block3d* blk, value_type* Q);
 

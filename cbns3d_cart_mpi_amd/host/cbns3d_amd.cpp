@@ -1,6 +1,6 @@
 // cbns3d_amd: drop-in driver for the reference's main (src/cbns3d.cpp:27-308).
 //
-//   cbns3d_amd <dimX> <dimY> <dimZ> [--fused]
+//   cbns3d_amd <dimX> <dimY> <dimZ> [--fused] [--mu-sidecar] [--async-io]
 //
 // Reads ./input.txt exactly as the reference does, splits the domain into
 // dimX x dimY x dimZ blocks (held on one GPU instead of one per MPI rank),
@@ -8,7 +8,12 @@
 // line, checkpoint files checkpointNNN.dat per block, VTK output directories
 // outputNNN with one .vtr per block and a .vtm, and the wall/CPU time lines.
 // --fused runs each step as one device call instead of the reference's
-// method-by-method sequence (same arithmetic).
+// method-by-method sequence (same arithmetic).  --mu-sidecar writes each
+// checkpoint's mu beside it (Block3d::set_mu_sidecar), so that a restart
+// from it continues bit for bit as the uninterrupted run.  --async-io writes
+// the checkpoint and VTK files from a device snapshot on writer threads
+// while the next steps run (Block3d::set_async_io; the same files); the
+// wall time then includes waiting for the writes still pending at the end.
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -30,7 +35,19 @@ int main(int argc, char** argv) {
   proc_info.dims[0] = std::stoi(argv[1]);
   proc_info.dims[1] = std::stoi(argv[2]);
   proc_info.dims[2] = std::stoi(argv[3]);
-  const bool fused = argc > 4 && std::strcmp(argv[4], "--fused") == 0;
+  bool fused = false, sidecar = false, async_io = false;
+  for (int a = 4; a < argc; a++) {
+    if (std::strcmp(argv[a], "--fused") == 0)
+      fused = true;
+    else if (std::strcmp(argv[a], "--mu-sidecar") == 0)
+      sidecar = true;
+    else if (std::strcmp(argv[a], "--async-io") == 0)
+      async_io = true;
+    else {
+      std::cout << "unknown option " << argv[a] << "\n";
+      return 1;
+    }
+  }
   const int nblocks = proc_info.dims[0] * proc_info.dims[1] * proc_info.dims[2];
 
   SimulationParams simParam;
@@ -65,6 +82,8 @@ int main(int argc, char** argv) {
 
   Block3d block(&proc_info, &simParam, simParam.IM, simParam.JM, simParam.KM);
   block.gen_mesh(0, 0, 0);
+  block.set_mu_sidecar(sidecar);
+  block.set_async_io(async_io);
 
   // src/cbns3d.cpp:132-134: the checkpoint name of this process's rank; the
   // facade derives every block's file from it
@@ -148,6 +167,10 @@ int main(int argc, char** argv) {
     }
   }
   cbns_sync(block.context());
+  if (0 != block.io_wait()) {
+    std::cerr << "Unable to write an output file\n";
+    return 1;
+  }
   auto end = std::chrono::high_resolution_clock::now();
   double wall_time = std::chrono::duration_cast<std::chrono::duration<double>>(end - start).count();
   std::cout << "  Wall time: " << wall_time << " seconds\n"
@@ -157,5 +180,9 @@ int main(int argc, char** argv) {
   oss.str("");
   oss << std::setw(3) << std::setfill('0') << cnt_out;
   if (0 != output_vtk_xml(&block, 0, nblocks, "output" + oss.str(), "output")) return 1;
+  if (0 != block.io_wait()) {
+    std::cerr << "Unable to write an output file\n";
+    return 1;
+  }
   return 0;
 }

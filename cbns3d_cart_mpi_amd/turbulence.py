@@ -93,6 +93,34 @@ def dilatation_spectrum(Q, N):
     return spec[: N // 2 + 1]
 
 
+def velocity_spectra(Q, nshell=None):
+    """Shell spectra of the velocity of an interior AoS state Q [k][j][i][eq]
+    of any shape on the periodic [-pi, pi)^3 grid (numpy, the host
+    counterpart of the device's cbns_spectra): with u_hat the FFT of
+    u = (rho u) / rho (and v, w) divided by the cell count and k the integer
+    wavenumbers in fftfreq order, shell s = rint(|k|) holds
+    ke = sum 0.5 |u_hat|^2, enstrophy = sum 0.5 |k x u_hat|^2 and
+    dilatation = sum 0.5 |k . u_hat|^2.  Returns the three arrays of
+    nshell shells (default: up to the largest |k|)."""
+    N2, N1, N0 = Q.shape[:3]
+    kk = [np.fft.fftfreq(n, 1.0 / n) for n in (N2, N1, N0)]
+    KZ, KY, KX = np.meshgrid(*kk, indexing="ij")
+    K = (KX, KY, KZ)
+    uh = [np.fft.fftn(Q[..., 1 + a] / Q[..., 0]) / (N0 * N1 * N2) for a in range(3)]
+    ke = 0.5 * sum(np.abs(u) ** 2 for u in uh)
+    cross = [K[1] * uh[2] - K[2] * uh[1], K[2] * uh[0] - K[0] * uh[2], K[0] * uh[1] - K[1] * uh[0]]
+    en = 0.5 * sum(np.abs(c) ** 2 for c in cross)
+    dil = 0.5 * np.abs(sum(K[a] * uh[a] for a in range(3))) ** 2
+    shell = np.rint(np.sqrt(KX * KX + KY * KY + KZ * KZ)).astype(int).ravel()
+    if nshell is None:
+        nshell = shell.max() + 1
+    out = []
+    for e in (ke, en, dil):
+        s = np.bincount(shell, weights=e.ravel(), minlength=nshell)
+        out.append(s[:nshell])
+    return tuple(out)
+
+
 def precision_errors(Q32, Q64, N):
     """Per-field relative L2 (RMS difference / RMS of the FP64 field) and
     relative L-inf (max difference / max |FP64 field|) of an FP32-flux state

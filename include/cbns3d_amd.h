@@ -136,6 +136,40 @@ int cbns_set_block_state(cbns_ctx* ctx, int b, int which, const double* Q_aos_gh
  * on the device and ghosts read as 0. */
 int cbns_get_block_primitive(cbns_ctx* ctx, int b, int f, double* out_ghosted);
 
+/* Interior mu of block b, [k][j][i] (KM x JM x IM doubles of the block): the
+ * Sutherland viscosity the last viscous pass stored (src/Block3d.cpp:777-779),
+ * which the next calc_dt reads (src/Block3d.cpp:533-546).  The reference's
+ * checkpoint (src/Block3d.cpp:964-987) does not hold it, so its restart path
+ * (src/Block3d.cpp:318-357) starts from an unwritten mu; saving it beside a
+ * checkpoint and setting it after cbns_set_state makes the resumed run's
+ * first dt, and so the whole resumed run, bit-identical to the uninterrupted
+ * run.  No counterpart in the reference. */
+int cbns_get_block_mu(cbns_ctx* ctx, int b, double* mu_interior);
+int cbns_set_block_mu(cbns_ctx* ctx, int b, const double* mu_interior);
+
+/* Asynchronous snapshot for file output, so that checkpoint and VTK writes
+ * (src/Block3d.cpp:396-500, 964-987) overlap the following steps instead of
+ * stopping them.  No counterpart in the reference, whose writers copy and
+ * format on the calling rank's thread.
+ * cbns_snapshot_begin enqueues on the context's stream a device copy of Q and
+ * of the primitives rho u v w p of every block as they stand after the work
+ * issued so far, and a copy of it to pinned host memory on a copy stream;
+ * it returns at once and later steps run meanwhile.  cbns_snapshot_wait
+ * blocks until the host copy is complete and gives the snapshot's t_cur.
+ * cbns_snapshot_block_state and cbns_snapshot_block_primitive (f = 0 rho,
+ * 1 u, 2 v, 3 w, 4 p) give block b's arrays in the layouts of
+ * cbns_get_block_state and cbns_get_block_primitive; they wait for the host
+ * copy themselves and may be called from another host thread while the
+ * context's own thread keeps stepping.  A snapshot stays readable until the
+ * next cbns_snapshot_begin, which overwrites it: the caller finishes reading
+ * one snapshot before it begins the next.  The first call allocates the
+ * device and pinned host buffers (one state and five primitive fields per
+ * block), which the context keeps. */
+int cbns_snapshot_begin(cbns_ctx* ctx);
+int cbns_snapshot_wait(cbns_ctx* ctx, double* t_cur);
+int cbns_snapshot_block_state(cbns_ctx* ctx, int b, double* Q_aos_ghosted);
+int cbns_snapshot_block_primitive(cbns_ctx* ctx, int b, int f, double* out_ghosted);
+
 /* ---- method-level entry points (the reference's stage order) ----------- */
 int cbns_boundary_condition(cbns_ctx* ctx, int which);  /* src/boundary_condition.cpp:4-231 */
 int cbns_calc_primitive(cbns_ctx* ctx, int which);      /* src/Block3d.cpp:359-385 */
@@ -163,6 +197,18 @@ int cbns_set_time(cbns_ctx* ctx, double t_cur);
  * reference's 4th-order central stencil, src/calc_gradient.cpp:29-64) from the
  * current primitives of Q.  No counterpart in the reference. */
 int cbns_diagnostics(cbns_ctx* ctx, double* ke, double* enstrophy);
+
+/* Shell spectra of the velocity (u = (rho u) / rho, v, w) of Q on the periodic
+ * [-pi, pi)^3 grid, formed on the device: a batched hipFFT real-to-complex
+ * 3-D transform of u, v, w (amplitudes divided by the cell count), then one
+ * deterministic reduction per shell.  For shells s = 0 .. nshell - 1 (the
+ * modes with rint(|k|) = s, integer wavenumbers in numpy fftfreq order):
+ * ke[s] = sum 0.5 |u_hat|^2, enstrophy[s] = sum 0.5 |k x u_hat|^2,
+ * dilatation[s] = sum 0.5 |k . u_hat|^2.  Any output may be NULL.  The first
+ * call allocates the transform buffers (3 N^3 doubles, 3 N^2 (N/2 + 1)
+ * complex) and the plan, which the context keeps.  Diagnostics of the
+ * decaying-turbulence workload; no counterpart in the reference. */
+int cbns_spectra(cbns_ctx* ctx, int nshell, double* ke, double* enstrophy, double* dilatation);
 
 /* Synchronise the context's stream. */
 int cbns_sync(cbns_ctx* ctx);

@@ -12,20 +12,21 @@ EXE = os.path.join(ROOT, "cbns3d_cart_mpi_amd", "bin", "cbns3d_amd")
 NG = 4
 
 
-def input_txt(N, nsteps, num_output=1):
-    vals = ["checkpoint", "0.5", "0.0", "20.0", "0.71", "1.4", "0.1", "1600.0", "255.556", "1.0",
-            str(N[0]), str(N[1]), str(N[2]), str(nsteps), str(nsteps), str(num_output), "1"]
+def input_txt(N, nsteps, num_output=1, freq=None, restart=False, t_end="20.0"):
+    vals = ["checkpoint", "0.5", "0.0", t_end, "0.71", "1.4", "0.1", "1600.0", "255.556", "1.0",
+            str(N[0]), str(N[1]), str(N[2]), str(nsteps), str(freq or nsteps), str(num_output),
+            "0" if restart else "1"]
     return "\n".join(vals) + "\n"
 
 
-def run_driver(tmp, N, nsteps, dims, extra=()):
-    (tmp / "input.txt").write_text(input_txt(N, nsteps))
+def run_driver(tmp, N, nsteps, dims, extra=(), **kw):
+    (tmp / "input.txt").write_text(input_txt(N, nsteps, **kw))
     r = subprocess.run([EXE, *map(str, dims), *extra], cwd=tmp, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     return r.stdout
 
 
-@pytest.mark.parametrize("extra", [(), ("--fused",)])
+@pytest.mark.parametrize("extra", [(), ("--fused",), ("--async-io",), ("--fused", "--async-io")])
 def test_driver_files_byte_identical_to_reference(golden, tmp_path, extra):
     g = golden("ref_vtk8.npz")
     out = run_driver(tmp_path, (8, 8, 8), 2, (1, 1, 1), extra)
@@ -104,3 +105,36 @@ def test_driver_two_block_restart_uses_the_rank_suffixed_names(tmp_path):
     assert t2 == b.t_cur
     got = np.concatenate([Q2a[NG:-NG, NG:-NG, NG:-NG], Q2b[NG:-NG, NG:-NG, NG:-NG]], axis=2)
     assert np.array_equal(got, b.Q_interior(0))
+
+
+def test_driver_mu_sidecar_restart_continues_bit_for_bit(tmp_path):
+    """Two blocks: a run of 3 steps writes its checkpoints with mu sidecars;
+    a restart from them for 3 more steps ends with the files of an
+    uninterrupted 6-step run, byte for byte."""
+    a, c = tmp_path / "a", tmp_path / "c"
+    a.mkdir()
+    c.mkdir()
+    run_driver(a, (16, 16, 16), 3, (2, 1, 1), ("--mu-sidecar",))
+    assert (a / "checkpoint001.dat.mu").exists()
+    run_driver(a, (16, 16, 16), 3, (2, 1, 1), ("--mu-sidecar",), restart=True)
+    run_driver(c, (16, 16, 16), 6, (2, 1, 1))
+    for nm in ("checkpoint000.dat", "checkpoint001.dat"):
+        assert (a / nm).read_bytes() == (c / nm).read_bytes(), nm
+
+
+def test_driver_async_io_writes_the_files_of_the_synchronous_writers(tmp_path):
+    """Checkpoints every 2 steps (one file name per block, rewritten) and
+    VTK output of two blocks at every 0.0125 of time (t reaches about 0.078
+    in 12 steps: six outputs besides the first and the last): the
+    asynchronous writers leave every file as the synchronous ones do."""
+    runs = {}
+    for name, extra in (("sync", ()), ("async", ("--async-io",))):
+        d = tmp_path / name
+        d.mkdir()
+        run_driver(d, (16, 16, 16), 12, (2, 1, 1), extra, num_output=20, freq=2, t_end="0.25")
+        runs[name] = {str(p.relative_to(d)): p.read_bytes() for p in sorted(d.rglob("*")) if p.is_file()
+                      and p.name != "input.txt"}
+    assert sum(nm.endswith(".vtr") for nm in runs["sync"]) >= 12
+    assert runs["sync"].keys() == runs["async"].keys()
+    for nm in runs["sync"]:
+        assert runs["sync"][nm] == runs["async"][nm], nm

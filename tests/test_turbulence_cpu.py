@@ -1,8 +1,11 @@
 """Host-side diagnostics of the decaying-turbulence workload (CPU only):
-the dilatation spectrum against closed forms."""
+the dilatation spectrum and the velocity shell spectra against closed
+forms."""
 import numpy as np
 
-from cbns3d_cart_mpi_amd.turbulence import dilatation_spectrum, precision_errors, turbulence_state
+from cbns3d_cart_mpi_amd.turbulence import (dilatation_spectrum, precision_errors, turbulence_state,
+                                            velocity_spectra)
+from spectral_modes import check_single_mode_spectra, oblique_mode_state, single_mode_state
 
 
 def _grid(N):
@@ -84,3 +87,25 @@ def test_turbulence_spectrum_has_the_stated_shape_and_pressure():
     for a in acc[1:]:
         rw = a[k] / np.mean(a[k])
         assert rw.max() / rw.min() > 2.0
+
+
+def test_velocity_spectra_of_single_modes_on_an_uneven_grid():
+    Q, exp = single_mode_state(16, 12, 10)
+    check_single_mode_spectra(velocity_spectra(Q), exp)
+
+
+def test_velocity_spectra_sum_to_the_mean_energy_and_hold_the_dilatation_spectrum():
+    N = 24
+    Q, _ = turbulence_state(N, seed=3)
+    ke, en, dil = velocity_spectra(Q)
+    u = Q[..., 1:4] / Q[..., :1]
+    assert abs(ke.sum() - 0.5 * np.mean((u * u).sum(-1))) < 1e-14 * ke.sum()
+    assert np.allclose(dil[: N // 2 + 1], dilatation_spectrum(Q, N), rtol=1e-12, atol=1e-30)
+    assert en.sum() > 0 and dil.sum() < 1e-24 * ke.sum()
+
+
+def test_velocity_spectra_of_oblique_compressive_modes():
+    Q, exp = oblique_mode_state(16, 12, 10)
+    ke, en, dil = velocity_spectra(Q)
+    check_single_mode_spectra((ke, dil), (exp[0], exp[2]))
+    assert en.max() < 1e-28
