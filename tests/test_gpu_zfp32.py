@@ -80,3 +80,45 @@ def test_fp32_flux_turbulence_error_vs_fp64_oracle():
     for name, (l2, li) in rel_errors(Q, b.Q_interior(0)).items():
         print(f"FP32-flux turbulence 32^3 40 steps {name}: rel L2 {l2:.3e} rel Linf {li:.3e}")
         assert l2 <= 1e-5, name
+
+
+@pytest.mark.parametrize("N", [128, 256])
+def test_turbulence_500_steps_forms_against_the_bit_exact_path(N):
+    """The decaying-turbulence workload at its full sizes (M_t = 0.6,
+    Re_lambda = 100, seed 2024, 500 steps): the bit-exact FP64 path is
+    decomposition-invariant there (one block and 2 x 2 x 2, bit for bit),
+    and the FP32-flux and tolerance-bound forms stay within their bars of
+    it, the FP32-flux form also in the dilatation spectrum (device spectra,
+    shells 1 .. N/2, relative per shell).  The bit-exact path itself equals
+    the oracle on this workload at 32^3 (above) and on TGV at 64^3 x 100
+    steps (test_gpu_parity.py)."""
+    from cbns3d_cart_mpi_amd.turbulence import turbulence_state
+
+    Q0, Re = turbulence_state(N, seed=2024)
+    par = C.make_params(N, Mach=1.0, Re=Re, nstep_max=10 ** 6)
+
+    def run(prec, dims):
+        with C.Solver(par, dims, precision=prec) as s:
+            s.set_state(Q0, 0.0, mu0=0.0)
+            s.run(500)
+            return s.get_state(), s.t_cur, s.spectra(N // 2 + 1)[2]
+
+    Q64, t64, dil64 = run(C.CBNS_FP64, (1, 1, 1))
+    Qd, td, _ = run(C.CBNS_FP64, (2, 2, 2))
+    assert td == t64 and np.array_equal(Qd, Q64)
+    del Qd
+    Qf, tf, _ = run(C.CBNS_FP64_FAST, (1, 1, 1))
+    for e, name in enumerate(FIELDS):
+        err = np.abs(Qf[..., e] - Q64[..., e]).max() / np.abs(Q64[..., e]).max()
+        print(f"tolerance-bound turbulence {N}^3 500 steps {name}: rel Linf (own max) {err:.3e}")
+        assert err <= 1e-10, name
+    del Qf
+    Q32, t32, dil32 = run(C.CBNS_FP32_FLUX, (1, 1, 1))
+    assert abs(t32 - t64) <= 1e-6 * t64
+    for name, (l2, li) in rel_errors(Q32, Q64).items():
+        print(f"FP32-flux turbulence {N}^3 500 steps {name}: rel L2 {l2:.3e} rel Linf {li:.3e}")
+        assert l2 <= 1e-5, name
+        assert li <= 1e-4, name
+    shell = np.abs(dil32[1:] - dil64[1:]) / dil64[1:]
+    print(f"FP32-flux turbulence {N}^3 dilatation spectrum: max rel shell difference {shell.max():.3e}")
+    assert shell.max() <= 1e-4
