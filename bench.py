@@ -250,13 +250,20 @@ def turbulence_leg(N, steps, seed=2024):
     """Decaying turbulence (M_t 0.6, Re_lambda 100, seeded) at N^3: the plain
     FP64 path and the FP32-flux form over `steps` steps from the same state;
     throughput of each, per-field error of FP32-flux against FP64 and the
-    dilatation-spectrum difference."""
+    dilatation-spectrum difference (host numpy), and the device spectra
+    (cbns_spectra): their time per call, the FP64 dilatation spectrum against
+    the host definition and the FP32-flux spectra against FP64's."""
     from cbns3d_cart_mpi_amd import capi as C
     from cbns3d_cart_mpi_amd.turbulence import precision_errors, turbulence_state
 
+    import numpy as np
+
+    from cbns3d_cart_mpi_amd.turbulence import dilatation_spectrum
+
     Q0, Re = turbulence_state(N, seed=seed)
     p = C.make_params(N, Mach=1.0, Re=Re, nstep_max=steps, t_end=1.0e3)
-    out, Q = {}, {}
+    out, Q, spec = {}, {}, {}
+    spec_ms = []
     for name, prec in (("fp64", C.CBNS_FP64), ("fp64_fast", C.CBNS_FP64_FAST), ("fp32", C.CBNS_FP32_FLUX)):
         with C.Solver(p, precision=prec) as s:
             s.set_state(Q0, 0.0, mu0=0.0)
@@ -267,8 +274,21 @@ def turbulence_leg(N, steps, seed=2024):
             el = time.perf_counter() - t0
             Q[name] = s.get_state()
             out[name] = {"mcell_steps_per_s": round(N ** 3 * (steps - 1) / el / 1e6, 2), "t_end": s.t_cur}
+            s.spectra(N // 2 + 1)  # plans the transform
+            t0 = time.perf_counter()
+            spec[name] = s.spectra(N // 2 + 1)
+            spec_ms.append((time.perf_counter() - t0) * 1e3)
     out["fp32_error_vs_fp64"] = precision_errors(Q["fp32"], Q["fp64"], N)
     out["fp64_fast_error_vs_fp64"] = precision_errors(Q["fp64_fast"], Q["fp64"], N)
+    # the device spectra (cbns_spectra) beside the host numpy definition
+    host = dilatation_spectrum(Q["fp64"], N)
+    d64, d32 = spec["fp64"][2], spec["fp32"][2]
+    out["device_spectra"] = {
+        "ms_per_call": round(min(spec_ms), 3),
+        "fp64_dilatation_vs_host_numpy_max_rel": float(np.abs(d64 - host).max() / host.max()),
+        "fp32_vs_fp64_dilatation_max_rel_shell": float((np.abs(d32[1:] - d64[1:]) / d64[1:]).max()),
+        "fp32_vs_fp64_ke_max_rel_shell": float((np.abs(spec["fp32"][0][1:] - spec["fp64"][0][1:])
+                                                / spec["fp64"][0][1:]).max())}
     out["config"] = f"{N}^3, seed {seed}, Mach 1 (sound speed 1), M_t 0.6, Re {Re:.4g}, {steps} steps"
     return out
 

@@ -14,9 +14,11 @@
 //  3. k_shell_sums: one workgroup per shell walks the (kz, ky) lines of the
 //     half spectrum and, on each, the kx range whose |k| rounds to the shell
 //     (an exact integer interval, no search), summing
-//     w 0.5 |u_hat|^2, w 0.5 |k x u_hat|^2 and w 0.5 |k . u_hat|^2
-//     (w = 1 or 2 as above, amplitudes divided by the cell count) with a
-//     fixed per-thread order and a fixed tree: the result does not depend on
+//     0.5 |u_hat|^2, 0.5 |k x u_hat|^2 and 0.5 |k . u_hat|^2 of each held
+//     mode and of the conjugate partner it stands for (amplitudes divided by
+//     the cell count; the partner's wave vector with numpy's labels, which
+//     are not the held mode's negation at a Nyquist index) with a fixed
+//     per-thread order and a fixed tree: the result does not depend on
 //     scheduling.
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
@@ -72,21 +74,40 @@ __global__ __launch_bounds__(SPEC_TPB) void k_shell_sums(const double2* __restri
     long long m0 = lo - r2 > 0 ? isqrt_floor(lo - r2 - 1) + 1 : 0;  // smallest m with m^2 >= lo - r2
     long long m1 = isqrt_floor(hi - r2);
     if (m1 > H - 1) m1 = H - 1;
+    // the conjugate partner (-kx, -ky, -kz) of a held mode carries numpy's
+    // labels: -ky, except at a Nyquist index, which fftfreq labels -N/2 on
+    // both sides (so the partner's k . u_hat and k x u_hat differ there)
+    const double pky = (N1 % 2 == 0 && 2 * jy == N1) ? (double)ky : -(double)ky;
+    const double pkz = (N2 % 2 == 0 && 2 * jz == N2) ? (double)kz : -(double)kz;
     const long long base = (jz * (long long)N1 + jy) * H;
     for (long long m = m0; m <= m1; m++) {
       const bool nyq = (N0 % 2 == 0) && m == N0 / 2;
-      const double w = (m == 0 || nyq) ? 0.5 : 1.0;  // 0.5 x (1 or 2 modes)
+      const bool two = !(m == 0 || nyq);  // the held mode stands for its partner too
       const double kx = nyq ? -(double)(N0 / 2) : (double)m;
       const double2 u = hat[base + m], v = hat[hs + base + m], q = hat[2 * hs + base + m];
-      const double dky = (double)ky, dkz = (double)kz;
-      e[0] += w * inv2 * (u.x * u.x + u.y * u.y + v.x * v.x + v.y * v.y + q.x * q.x + q.y * q.y);
-      // k x u_hat (the factor i changes no magnitude)
-      const double cx_r = dky * q.x - dkz * v.x, cx_i = dky * q.y - dkz * v.y;
-      const double cy_r = dkz * u.x - kx * q.x, cy_i = dkz * u.y - kx * q.y;
-      const double cz_r = kx * v.x - dky * u.x, cz_i = kx * v.y - dky * u.y;
-      e[1] += w * inv2 * (cx_r * cx_r + cx_i * cx_i + cy_r * cy_r + cy_i * cy_i + cz_r * cz_r + cz_i * cz_i);
-      const double d_r = kx * u.x + dky * v.x + dkz * q.x, d_i = kx * u.y + dky * v.y + dkz * q.y;
-      e[2] += w * inv2 * (d_r * d_r + d_i * d_i);
+      const double h = 0.5 * inv2;
+      e[0] += (two ? 2.0 : 1.0) * h * (u.x * u.x + u.y * u.y + v.x * v.x + v.y * v.y + q.x * q.x + q.y * q.y);
+      // |k x u_hat|^2 and |k . u_hat|^2 of wave vector (a, b, c) on this mode's
+      // amplitudes (the partner's amplitudes are their conjugates, which
+      // change no magnitude; the factor i neither)
+      const auto cross_dot = [&](double a, double b, double c, double& cr, double& dt) {
+        const double cx_r = b * q.x - c * v.x, cx_i = b * q.y - c * v.y;
+        const double cy_r = c * u.x - a * q.x, cy_i = c * u.y - a * q.y;
+        const double cz_r = a * v.x - b * u.x, cz_i = a * v.y - b * u.y;
+        cr = cx_r * cx_r + cx_i * cx_i + cy_r * cy_r + cy_i * cy_i + cz_r * cz_r + cz_i * cz_i;
+        const double d_r = a * u.x + b * v.x + c * q.x, d_i = a * u.y + b * v.y + c * q.y;
+        dt = d_r * d_r + d_i * d_i;
+      };
+      double cr, dt;
+      cross_dot(kx, (double)ky, (double)kz, cr, dt);
+      if (two) {
+        double cr2, dt2;
+        cross_dot(-kx, pky, pkz, cr2, dt2);
+        cr += cr2;
+        dt += dt2;
+      }
+      e[1] += h * cr;
+      e[2] += h * dt;
     }
   }
   __shared__ double red[3][SPEC_TPB];

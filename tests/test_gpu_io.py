@@ -102,6 +102,25 @@ def test_spectra_of_a_running_taylor_green_flow_equal_the_host_definition():
     assert dev[2].sum() > 1e-9 * dev[1].sum()  # compressible (M = 0.1): dilatation 1.6e-7 of the enstrophy
 
 
+@pytest.mark.parametrize("shape,dims", [((16, 12, 10), (2, 1, 1)), ((15, 11, 9), (1, 1, 1))])
+def test_spectra_of_white_noise_equal_the_host_definition(shape, dims):
+    """A random compressible field, so that every mode carries energy, the
+    Nyquist ones of an even grid included (numpy labels both a Nyquist mode
+    and its conjugate partner -N/2, so the partner's k . u_hat is not the
+    mirror of the held mode's): each device spectrum equals the host
+    definition to 1e-12 of its own maximum."""
+    rng = np.random.default_rng(5)
+    N0, N1, N2 = shape
+    Q = np.empty((N2, N1, N0, 5))
+    Q[..., 0] = 1.0 + 0.1 * rng.random((N2, N1, N0))
+    Q[..., 1:4] = Q[..., :1] * 0.1 * rng.standard_normal((N2, N1, N0, 3))
+    Q[..., 4] = 2.5 + 0.5 * (Q[..., 1:4] ** 2).sum(-1) / Q[..., 0]
+    dev = spectra_of(Q, dims)
+    host = velocity_spectra(Q, nshell=len(dev[0]))
+    for d, h in zip(dev, host):
+        assert np.abs(d - h).max() <= 1e-12 * h.max(), np.abs(d - h).max() / h.max()
+
+
 @pytest.mark.parametrize("make", [single_mode_state, oblique_mode_state])
 def test_spectra_of_single_modes_equal_their_closed_forms(make):
     Q, exp = make(16, 12, 10)
