@@ -697,6 +697,7 @@ int cbns_spectra(cbns_ctx* c, int nshell, double* ke, double* enstrophy, double*
   std::vector<double> h(3 * (size_t)nshell);
   const int r = spectra_run(c->spec, c->db, c->hb.data(), c->dom, CBNS_Q, nshell, h.data(), c->stream);
   if (r == 1) return CBNS_ERR_NOMEM;
+  if (r == 3) return CBNS_ERR_ARG;
   if (r) return CBNS_ERR_HIP;
   double* outs[3] = {ke, enstrophy, dilatation};
   for (int a = 0; a < 3; a++)

@@ -240,7 +240,7 @@ void reciprocal_check(double h, double* rh, int* fast);
 // plan per context, made at the first call.  spectra_run writes the kinetic
 // energy, enstrophy and dilatation spectra of shells 0 .. nshell - 1 to
 // host_out[0..3 nshell) and synchronises s; 0 ok, 1 allocation failed, 2 HIP
-// or hipFFT error.
+// or hipFFT error, 3 a domain of more than 2^31 - 1 cells.
 struct Spectra;
 Spectra* spectra_create();
 void spectra_destroy(Spectra* sp);

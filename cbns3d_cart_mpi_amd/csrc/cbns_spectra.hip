@@ -144,15 +144,23 @@ void spectra_destroy(Spectra* sp) {
   delete sp;
 }
 
-// 0 ok, 1 allocation failed, 2 HIP or hipFFT error.
+// 0 ok, 1 allocation failed, 2 HIP or hipFFT error, 3 a transform past
+// hipFFT's 32-bit sizes (more than 2^31 - 1 cells).
 int spectra_run(Spectra* sp, const BlockDesc* dblocks, const BlockDesc* hblocks, const Domain& dom, int which,
                 int nshell, double* host_out, hipStream_t s) {
   const int N0 = dom.N[0], N1 = dom.N[1], N2 = dom.N[2];
   const long long ntot = (long long)N0 * N1 * N2;
   const long long nhat = (long long)N2 * N1 * (N0 / 2 + 1);
-  if (!sp->have_plan) {
-    if (hipMalloc(&sp->vel, sizeof(double) * 3 * ntot) != hipSuccess) return 1;
-    if (hipMalloc(&sp->hat, sizeof(double2) * 3 * nhat) != hipSuccess) return 1;
+  if (ntot > 0x7fffffffLL) return 3;
+  if (!sp->have_plan) {  // buffers kept from a failed earlier attempt are reused
+    if (!sp->vel && hipMalloc(&sp->vel, sizeof(double) * 3 * ntot) != hipSuccess) {
+      sp->vel = nullptr;
+      return 1;
+    }
+    if (!sp->hat && hipMalloc(&sp->hat, sizeof(double2) * 3 * nhat) != hipSuccess) {
+      sp->hat = nullptr;
+      return 1;
+    }
     int n[3] = {N2, N1, N0};
     if (hipfftPlanMany(&sp->plan, 3, n, nullptr, 1, (int)ntot, nullptr, 1, (int)nhat, HIPFFT_D2Z, 3) !=
         HIPFFT_SUCCESS)
@@ -163,7 +171,11 @@ int spectra_run(Spectra* sp, const BlockDesc* dblocks, const BlockDesc* hblocks,
   if (nshell > sp->out_cap) {
     if (sp->out) (void)hipFree(sp->out);
     sp->out = nullptr;
-    if (hipMalloc(&sp->out, sizeof(double) * 3 * nshell) != hipSuccess) return 1;
+    sp->out_cap = 0;
+    if (hipMalloc(&sp->out, sizeof(double) * 3 * nshell) != hipSuccess) {
+      sp->out = nullptr;
+      return 1;
+    }
     sp->out_cap = nshell;
   }
   for (int b = 0; b < dom.nb; b++) {

@@ -177,6 +177,7 @@ static std::string mu_file(const std::string& fn) { return fn + ".mu"; }
 // zero afterwards, as in the reference's restart path, unless a block's mu
 // sidecar exists: then that block's mu is set from it.
 int Block3d::read_bin(const std::string fname) {
+  if (async_io && io_wait()) return -1;  // a pending write may be this file
   touch();
   std::vector<double> all((size_t)NEQ * IM * JM * KM);
   double t = 0.0;

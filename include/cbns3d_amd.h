@@ -206,8 +206,10 @@ int cbns_diagnostics(cbns_ctx* ctx, double* ke, double* enstrophy);
  * ke[s] = sum 0.5 |u_hat|^2, enstrophy[s] = sum 0.5 |k x u_hat|^2,
  * dilatation[s] = sum 0.5 |k . u_hat|^2.  Any output may be NULL.  The first
  * call allocates the transform buffers (3 N^3 doubles, 3 N^2 (N/2 + 1)
- * complex) and the plan, which the context keeps.  Diagnostics of the
- * decaying-turbulence workload; no counterpart in the reference. */
+ * complex) and the plan, which the context keeps.  A domain of more than
+ * 2^31 - 1 cells (past hipFFT's 32-bit sizes) returns CBNS_ERR_ARG.
+ * Diagnostics of the decaying-turbulence workload; no counterpart in the
+ * reference. */
 int cbns_spectra(cbns_ctx* ctx, int nshell, double* ke, double* enstrophy, double* dilatation);
 
 /* Synchronise the context's stream. */
