@@ -65,6 +65,9 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-turb", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-io", action="store_true")
+    ap.add_argument("--io-only", action="store_true",
+                    help="run only the output/diagnostics leg (for profiling it) and print it")
     ap.add_argument("--batched-steps", type=int, default=60)
     ap.add_argument("--batched-copies", type=int, default=2)
     ap.add_argument("--turb-n", type=int, default=128)
@@ -293,6 +296,53 @@ def turbulence_leg(N, steps, seed=2024):
     return out
 
 
+def io_leg(N, steps=20):
+    """Output and diagnostics beside the step on TGV N^3 (tolerance-bound
+    form, one block): the time of `steps` steps alone and with an
+    asynchronous snapshot (cbns_snapshot_begin, after a first one that
+    allocated its buffers) issued before them, whose
+    device copy runs on the solver's stream and whose host copy runs on a
+    copy stream beside the steps; the wait for the host copy after them; the
+    synchronous copies an output takes without it (cbns_get_block_state and
+    the five cbns_get_block_primitive fields); and the device velocity
+    spectra (cbns_spectra) per call."""
+    import numpy as np
+
+    from cbns3d_cart_mpi_amd import capi as C
+
+    with C.Solver(C.make_params(N, nstep_max=10 ** 6, t_end=20.0), precision=C.CBNS_FP64_FAST) as s:
+        s.init_tgv()
+        s.run(3)
+        s.sync()
+
+        def timed(f):
+            t0 = time.perf_counter()
+            r = f()
+            s.sync()
+            return (time.perf_counter() - t0) * 1e3, r
+
+        # the first snapshot allocates its device and pinned host buffers
+        first, _ = timed(lambda: (s.snapshot_begin(), s.snapshot_wait()))
+        plain, _ = timed(lambda: s.run(steps))
+        snap_state = s.get_block_state(0)  # the state the snapshot will hold
+        t_snap = s.t_cur
+        t0 = time.perf_counter()
+        s.snapshot_begin()
+        s.run(steps)
+        s.sync()
+        with_snap = (time.perf_counter() - t0) * 1e3
+        wait, t_held = timed(s.snapshot_wait)
+        same = bool(np.array_equal(s.snapshot_block_state(0), snap_state)) and t_held == t_snap
+        sync_copy, _ = timed(lambda: (s.get_block_state(0), [s.get_block_primitive(0, f) for f in range(5)]))
+        s.spectra()  # plans the transform
+        spec = min(timed(s.spectra)[0] for _ in range(3))
+    return {"grid": N, "steps": steps,
+            "steps_ms": round(plain, 2), "steps_ms_with_snapshot": round(with_snap, 2),
+            "snapshot_wait_ms_after_steps": round(wait, 3), "snapshot_equals_state_of_its_moment": same,
+            "first_snapshot_ms_incl_allocation": round(first, 1),
+            "synchronous_output_copies_ms": round(sync_copy, 2), "device_spectra_ms_per_call": round(spec, 3)}
+
+
 def serve_leg(n):
     """Continuous batching of a Poisson stream (reduced size): n requests,
     grids {32..192}^3 with weights 1/N, steps U[50, 200], FP64 / FP32-flux
@@ -450,6 +500,9 @@ def main():
 
     dims = tuple(int(x) for x in a.dims.split(","))
     N = a.n
+    if a.io_only:
+        print(json.dumps({"io": io_leg(N)}), flush=True)
+        return
     # headline: the tolerance-bound FP64 path (within 1e-12 of the reference
     # after one step); the bit-exact plain path is measured beside it
     given = "auto" not in (a.band, a.schedule, a.occ)
@@ -535,6 +588,8 @@ def main():
         line["batched"] = batched_leg(N, a.batched_steps, a.batched_copies, exact["value"], value)
     if not a.no_turb:
         line["turbulence"] = turbulence_leg(a.turb_n, a.turb_steps)
+    if not a.no_io:
+        line["io"] = io_leg(N)
     if not a.no_sweep:
         line["sweep"] = sweep_leg(a.sweep_steps)
     if not a.no_serve:

@@ -11,15 +11,16 @@
 //  2. one batched hipFFT real-to-complex 3-D transform of the three fields
 //     (the half spectrum kx = 0 .. N0/2; every other mode is the conjugate
 //     of one held, so a held mode with 0 < kx < N0/2 stands for two);
-//  3. k_shell_sums: one workgroup per shell walks the (kz, ky) lines of the
-//     half spectrum and, on each, the kx range whose |k| rounds to the shell
+//  3. k_shell_sums: 16 workgroups per shell, each over a fixed range of the
+//     (kz, ky) lines of the half spectrum, walk on each line the kx range
+//     whose |k| rounds to the shell
 //     (an exact integer interval, no search), summing
 //     0.5 |u_hat|^2, 0.5 |k x u_hat|^2 and 0.5 |k . u_hat|^2 of each held
 //     mode and of the conjugate partner it stands for (amplitudes divided by
 //     the cell count; the partner's wave vector with numpy's labels, which
 //     are not the held mode's negation at a Nyquist index) with a fixed
-//     per-thread order and a fixed tree: the result does not depend on
-//     scheduling.
+//     per-thread order and a fixed tree; k_shell_final adds the 16 partial
+//     sums of a shell in order: the result does not depend on scheduling.
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
@@ -55,18 +56,22 @@ __device__ __forceinline__ long long isqrt_floor(long long v) {
 }
 
 constexpr int SPEC_TPB = 256;
+constexpr int SPEC_CHUNKS = 16;  // workgroups per shell, each over a fixed range of (kz, ky) lines
 
-// out[0][s], out[1][s], out[2][s]: kinetic energy, enstrophy, dilatation of
-// shell s = blockIdx.x.  hat: the three transforms, each [N2][N1][N0/2+1].
+// part[(a nshell + s) SPEC_CHUNKS + c], a = 0 kinetic energy, 1 enstrophy,
+// 2 dilatation: the sums of shell s = blockIdx.x over chunk c = blockIdx.y
+// of the (kz, ky) lines.  hat: the three transforms, each [N2][N1][N0/2+1].
 __global__ __launch_bounds__(SPEC_TPB) void k_shell_sums(const double2* __restrict__ hat, int N0, int N1, int N2,
-                                                         double inv2, int nshell, double* __restrict__ out) {
-  const int s = blockIdx.x;
+                                                         double inv2, int nshell, double* __restrict__ part) {
+  const int s = blockIdx.x, c = blockIdx.y;
   const int H = N0 / 2 + 1;
   const long long hs = (long long)N2 * N1 * H;  // one transform
+  const long long nl = (long long)N2 * N1;
+  const long long l0 = nl * c / SPEC_CHUNKS, l1 = nl * (c + 1) / SPEC_CHUNKS;
   // rint(sqrt(k2)) = s  <=>  s^2 - s + 1 <= k2 <= s^2 + s  (k2 an integer, no ties)
   const long long lo = s == 0 ? 0 : (long long)s * s - s + 1, hi = (long long)s * s + s;
   double e[3] = {0.0, 0.0, 0.0};
-  for (long long line = threadIdx.x; line < (long long)N2 * N1; line += SPEC_TPB) {
+  for (long long line = l0 + threadIdx.x; line < l1; line += SPEC_TPB) {
     const int jz = (int)(line / N1), jy = (int)(line % N1);
     const long long kz = wavenumber(jz, N2), ky = wavenumber(jy, N1);
     const long long r2 = kz * kz + ky * ky;
@@ -119,7 +124,16 @@ __global__ __launch_bounds__(SPEC_TPB) void k_shell_sums(const double2* __restri
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int a = 0; a < 3; a++) out[a * nshell + s] = red[a][0];
+    for (int a = 0; a < 3; a++) part[((long long)a * nshell + s) * SPEC_CHUNKS + c] = red[a][0];
+}
+
+// out[a nshell + s]: the chunks of shell s summed in chunk order.
+__global__ __launch_bounds__(256) void k_shell_final(const double* __restrict__ part, int n, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // i = a nshell + s
+  if (i >= n) return;
+  double x = 0.0;
+  for (int c = 0; c < SPEC_CHUNKS; c++) x += part[(long long)i * SPEC_CHUNKS + c];
+  out[i] = x;
 }
 
 // Device buffers and the FFT plan of one context's spectra.
@@ -127,7 +141,7 @@ struct Spectra {
   int N[3] = {0, 0, 0};
   double* vel = nullptr;   // 3 x N0 N1 N2
   double2* hat = nullptr;  // 3 x N2 N1 (N0/2+1)
-  double* out = nullptr;   // 3 x nshell
+  double* out = nullptr;   // 3 x nshell, then 3 x nshell x SPEC_CHUNKS partial sums
   int out_cap = 0;
   hipfftHandle plan = 0;
   bool have_plan = false;
@@ -172,7 +186,7 @@ int spectra_run(Spectra* sp, const BlockDesc* dblocks, const BlockDesc* hblocks,
     if (sp->out) (void)hipFree(sp->out);
     sp->out = nullptr;
     sp->out_cap = 0;
-    if (hipMalloc(&sp->out, sizeof(double) * 3 * nshell) != hipSuccess) {
+    if (hipMalloc(&sp->out, sizeof(double) * 3 * nshell * (1 + SPEC_CHUNKS)) != hipSuccess) {
       sp->out = nullptr;
       return 1;
     }
@@ -187,7 +201,10 @@ int spectra_run(Spectra* sp, const BlockDesc* dblocks, const BlockDesc* hblocks,
   if (hipfftSetStream(sp->plan, s) != HIPFFT_SUCCESS) return 2;
   if (hipfftExecD2Z(sp->plan, (hipfftDoubleReal*)sp->vel, (hipfftDoubleComplex*)sp->hat) != HIPFFT_SUCCESS) return 2;
   const double inv = 1.0 / (double)ntot;
-  hipLaunchKernelGGL(k_shell_sums, dim3(nshell), dim3(SPEC_TPB), 0, s, sp->hat, N0, N1, N2, inv * inv, nshell, sp->out);
+  double* part = sp->out + 3 * (size_t)nshell;
+  hipLaunchKernelGGL(k_shell_sums, dim3(nshell, SPEC_CHUNKS), dim3(SPEC_TPB), 0, s, sp->hat, N0, N1, N2, inv * inv,
+                     nshell, part);
+  hipLaunchKernelGGL(k_shell_final, dim3((3 * nshell + 255) / 256), dim3(256), 0, s, part, 3 * nshell, sp->out);
   if (hipGetLastError() != hipSuccess) return 2;
   if (hipMemcpyAsync(host_out, sp->out, sizeof(double) * 3 * nshell, hipMemcpyDeviceToHost, s) != hipSuccess) return 2;
   if (hipStreamSynchronize(s) != hipSuccess) return 2;
