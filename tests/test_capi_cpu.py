@@ -42,6 +42,14 @@ def test_null_arguments_are_rejected():
         assert getattr(L, f)(None, 32) == -1, f
     assert L.cbns_batch_sync(None) == -1
     assert L.cbns_batch_destroy(None) == -1
+    # output and diagnostics beside the step
+    assert L.cbns_snapshot_begin(None) == -1
+    assert L.cbns_snapshot_wait(None, None) == -1
+    assert L.cbns_snapshot_block_state(None, 0, None) == -1
+    assert L.cbns_snapshot_block_primitive(None, 0, 0, None) == -1
+    assert L.cbns_spectra(None, 8, None, None, None) == -1
+    assert L.cbns_get_block_mu(None, 0, None) == -1
+    assert L.cbns_set_block_mu(None, 0, None) == -1
 
 
 def test_batch_create_without_gpu_reports_no_device():
