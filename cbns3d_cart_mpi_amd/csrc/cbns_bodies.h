@@ -96,19 +96,30 @@ __device__ __forceinline__ double divr(double x, double h, double rh, int fh) {
   else return divc(x, h, rh, fh);
 }
 
+// Stencil sum (1/12) a - (2/3) b + (2/3) c - (1/12) d of the 4th-order
+// central difference of the values a, b, c, d at -2, -1, +1, +2: in the
+// reference's order and rounding (src/calc_gradient.cpp:40-60; the
+// divergence's c0 a + c1 b - c1 c - c0 d with c1 = -2/3,
+// src/Block3d.cpp:894-908, rounds identically), or with R (the
+// tolerance-bound and FP32-flux forms' viscous pass) as
+// (1/12)(a - d) + (2/3)(c - b) in one multiply and one fused multiply-add.
+template <bool R>
+__device__ __forceinline__ double st4(double a, double b, double c, double d) {
+  if constexpr (R) return __builtin_fma(1.0 / 12.0, a - d, (2.0 / 3.0) * (c - b));
+  else return (1.0 / 12.0) * a - 2.0 / 3.0 * b + (2.0 / 3.0) * c - 1.0 / 12.0 * d;
+}
+
 // 4th-order central derivative (src/calc_gradient.cpp:29-64): the stencil
 // sum, then the division by the spacing (divr).
 template <bool R = false, class P>
 __device__ __forceinline__ double grad4(P __restrict__ f, int c, int s, double h, double rh, int fh) {
-  double d = (1.0 / 12.0) * f[c - 2 * s] - 2.0 / 3.0 * f[c - s] + (2.0 / 3.0) * f[c + s] - 1.0 / 12.0 * f[c + 2 * s];
-  return divr<R>(d, h, rh, fh);
+  return divr<R>(st4<R>(f[c - 2 * s], f[c - s], f[c + s], f[c + 2 * s]), h, rh, fh);
 }
 
 // Divergence stencil of calc_viscous_flux_contribution (src/Block3d.cpp:894-908).
 template <typename VT>
 __device__ __forceinline__ double cdiff4(gptr<const VT> __restrict__ f, int c, int s) {
-  const double c0 = 1.0 / 12.0, c1 = -2.0 / 3.0;
-  return c0 * f[c - 2 * s] + c1 * f[c - s] - c1 * f[c + s] - c0 * f[c + 2 * s];
+  return st4<false>(f[c - 2 * s], f[c - s], f[c + s], f[c + 2 * s]);
 }
 
 __device__ __forceinline__ int face_count(const BlockDesc& B, int D) {
@@ -628,7 +639,6 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
   const unsigned pn = (unsigned)(n0 * n1);
   // Ev = (txx, txy, txz, ex), Fv = (txy, tyy, tyz, ey), Gv = (txz, tyz, tzz, ez)
   const int Ef[4] = {0, 3, 4, 6}, Ff[4] = {3, 1, 5, 7}, Gf[4] = {4, 5, 2, 8};
-  const double c0 = 1.0 / 12.0, c1 = -2.0 / 3.0;
   for (int p = Z0 - 2; p < Z1 + 2; p++) {
     const int buf = (p - Z0) & 1;
     // stage plane p: ring entries from the prefetch, the tile from the own queue;
@@ -671,12 +681,10 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
       const int b = buf ^ 1;
 #pragma unroll
       for (int e = 0; e < 4; e++) {
-        double d = divr<RX>(c0 * LE[b][e][ty][tx] + c1 * LE[b][e][ty][tx + 1] - c1 * LE[b][e][ty][tx + 3] -
-                            c0 * LE[b][e][ty][tx + 4],
-                        ph.dx, ph.rdx, ph.fdx);
-        d += divr<RX>(c0 * LF[b][e][ty][tx] + c1 * LF[b][e][ty + 1][tx] - c1 * LF[b][e][ty + 3][tx] -
-                      c0 * LF[b][e][ty + 4][tx],
-                  ph.dy, ph.rdy, ph.fdy);
+        double d = divr<RX>(st4<RX>(LE[b][e][ty][tx], LE[b][e][ty][tx + 1], LE[b][e][ty][tx + 3], LE[b][e][ty][tx + 4]),
+                            ph.dx, ph.rdx, ph.fdx);
+        d += divr<RX>(st4<RX>(LF[b][e][ty][tx], LF[b][e][ty + 1][tx], LF[b][e][ty + 3][tx], LF[b][e][ty + 4][tx]),
+                      ph.dy, ph.rdy, ph.fdy);
         dxy[e] = d;
       }
     }
@@ -746,7 +754,7 @@ __device__ __forceinline__ void visc_tile_body(const BlockDesc& B, const Phys& p
       const unsigned o = oi + pn * (unsigned)(p - 2);
 #pragma unroll
       for (int e = 0; e < 4; e++) {
-        const double dz = c0 * gq[e][0] + c1 * gq[e][1] - c1 * gq[e][3] - c0 * gq[e][4];
+        const double dz = st4<RX>(gq[e][0], gq[e][1], gq[e][3], gq[e][4]);
         gst(DV[e], o, (VT)(dq[e] + divr<RX>(dz, ph.dz, ph.rdz, ph.fdz)));
       }
     }
