@@ -584,12 +584,14 @@ def main():
         line["fp64_fma"] = fma
     if fp32:
         line["fp32_flux"] = fp32
+    if not a.no_io:
+        # before the legs that open many streams (batches, lanes): the snapshot's
+        # copy stream then has a hardware queue of its own
+        line["io"] = io_leg(N)
     if not a.no_batched:
         line["batched"] = batched_leg(N, a.batched_steps, a.batched_copies, exact["value"], value)
     if not a.no_turb:
         line["turbulence"] = turbulence_leg(a.turb_n, a.turb_steps)
-    if not a.no_io:
-        line["io"] = io_leg(N)
     if not a.no_sweep:
         line["sweep"] = sweep_leg(a.sweep_steps)
     if not a.no_serve:
